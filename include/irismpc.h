@@ -1,0 +1,233 @@
+/*
+ * irismpc: MI355X (gfx950) implementation of the iris-mpc GPU server's
+ * secret-shared matching path, behind a C ABI.
+ *
+ * Every entry point returns 0 on success and a negative value on failure; the
+ * failure text is available from irismpc_last_error() on the same thread.
+ * Nothing aborts across the boundary. Device buffers are plain pointers
+ * (hipStream_t / device pointers passed as void*), sizes are element counts
+ * unless stated otherwise.
+ *
+ * Party model: the three semi-honest parties run on one GPU. Each party owns
+ * its ShareDB / Circuits engines and its own stream. The NcclComm of the
+ * reference is replaced by an irismpc_exchange shared by the three parties:
+ * calls the reference makes as matched NCCL send/recv pairs (reshare, the
+ * circuit rounds, open) are collectives - every party calls with its own
+ * handle and stream (from its own thread, exactly like the reference's three
+ * actors), or a single thread drives all three through the *_trio variants.
+ *
+ * Interfaces replaced (paths relative to the worldcoin/iris-mpc root):
+ *   ShareDB            iris-mpc-gpu/src/dot/share_db.rs:448-1070
+ *   ProcessedDatabase  iris-mpc-gpu/src/dot/share_db.rs:133-184
+ *   Circuits           iris-mpc-gpu/src/threshold_ring/protocol.rs:433-2611
+ *   DistanceComparator iris-mpc-gpu/src/dot/distance_comparator.rs:28-894
+ *   NcclComm           iris-mpc-gpu/src/helpers/comm.rs:90-269
+ *   derive_seed        iris-mpc-gpu/src/server/actor.rs:2914-2925
+ */
+#ifndef IRISMPC_H
+#define IRISMPC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct irismpc_exchange irismpc_exchange;
+typedef struct irismpc_share_db irismpc_share_db;
+typedef struct irismpc_db irismpc_db;
+typedef struct irismpc_query irismpc_query;
+typedef struct irismpc_circuits irismpc_circuits;
+typedef struct irismpc_comparator irismpc_comparator;
+
+/* ------------------------------------------------------------- misc */
+/* Thread-local text of the last failure (empty string if none). */
+const char* irismpc_last_error(void);
+/* ABI version (major * 10000 + minor * 100 + patch). */
+int irismpc_version(void);
+int irismpc_device_count(int* n);
+/* Streams and device memory for FFI callers without a HIP binding. */
+int irismpc_stream_create(int device, void** stream);
+int irismpc_stream_destroy(void* stream);
+int irismpc_stream_synchronize(void* stream);
+int irismpc_malloc(int device, size_t bytes, void** ptr);
+int irismpc_free(void* ptr);
+int irismpc_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
+int irismpc_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+/* Fills with a kernel (graph-capture safe). */
+int irismpc_fill_u32(void* ptr, size_t n, uint32_t value, void* stream);
+int irismpc_fill_u64(void* ptr, size_t n, uint64_t value, void* stream);
+/* HKDF-SHA256 seed derivation of server/actor.rs:2914-2925 (KDF_SALT of
+ * actor.rs:116, ikm = little-endian seed bytes, info = big-endian nonce). */
+int irismpc_derive_seed(const uint32_t seed[8], uint64_t nonce, uint32_t out[8]);
+
+/* ------------------------------------------------------------- exchange */
+/* Replaces the per-GPU NcclComm of the three parties (helpers/comm.rs,
+ * helpers/device_manager.rs:289-344). timeout_s bounds every rendezvous. */
+int irismpc_exchange_create(int device, double timeout_s, irismpc_exchange** out);
+int irismpc_exchange_destroy(irismpc_exchange* ex);
+
+/* ------------------------------------------------------------- ShareDB */
+/* ShareDB::init (share_db.rs:467-573). seeds = (own, prev) ChaCha12 seeds as
+ * derived by derive_seed; comm == NULL builds a non-remote engine (no masking,
+ * no reshare) exactly like an empty comms vector. tag distinguishes engines
+ * sharing one exchange (codes, masks, ...). */
+int irismpc_share_db_init(int device, int peer_id, size_t max_db_length, size_t query_length,
+                          size_t code_length, const uint32_t seed_own[8],
+                          const uint32_t seed_prev[8], irismpc_exchange* comm, int tag,
+                          irismpc_share_db** out);
+int irismpc_share_db_destroy(irismpc_share_db* e);
+/* ShareDB::alloc_db (share_db.rs:575-618): the DB share stays resident in HBM
+ * (the reference page-locks host memory and streams chunks). */
+int irismpc_share_db_alloc_db(irismpc_share_db* e, size_t max_db_length, irismpc_db** out);
+int irismpc_db_destroy(irismpc_db* db);
+/* ProcessedDatabase::load_single_record_from_db for n consecutive records
+ * (share_db.rs:186-195,620-655); records: n * code_length u16 on the host.
+ * Per-record sums (preprocess, share_db.rs:265-296) are computed on load. */
+int irismpc_db_load_records(irismpc_share_db* e, irismpc_db* db, size_t index, size_t n,
+                            const uint16_t* records, void* stream);
+/* load_single_record_from_s3 (share_db.rs:197-212,657-684): limb0 = low
+ * ("odd") bytes, limb1 = high ("even") bytes, already biased by -128. */
+int irismpc_db_load_records_limbs(irismpc_share_db* e, irismpc_db* db, size_t index, size_t n,
+                                  const uint8_t* limb0, const uint8_t* limb1, void* stream);
+/* ProcessedDatabase::write_at_index (share_db.rs:214-262): copies the centre
+ * rotation of query src_index into DB row dst_index. */
+int irismpc_db_write_at_index(irismpc_share_db* e, irismpc_db* db, const irismpc_query* q,
+                              size_t src_index, size_t dst_index, void* stream);
+/* Synthetic DB content: uniformly random shares for rows [index, index + n)
+ * (index a multiple of 16) from ChaCha20(key, nonce); sums are computed.
+ * Uniform u16 shares have the distribution of real Shamir shares. */
+int irismpc_db_fill_random(irismpc_share_db* e, irismpc_db* db, size_t index, size_t n,
+                           const uint32_t key[8], uint64_t nonce, void* stream);
+/* Reads DB rows back as u16 (for tests). */
+int irismpc_db_read_records(irismpc_share_db* e, const irismpc_db* db, size_t index, size_t n,
+                            uint16_t* out, void* stream);
+/* preprocess_query + htod_transfer_query + query_sums (share_db.rs:45-59,
+ * 686-765; helpers/query_processor.rs:117-140): rows = query_length rows of
+ * code_length u16 (rotations and Lagrange preprocessing already applied, as in
+ * the reference's CompactQuery). */
+int irismpc_query_upload(irismpc_share_db* e, const uint16_t* rows, size_t n_rows, void* stream,
+                         irismpc_query** out);
+/* Raw query shares of this party (batch x code_length u16, neither rotated nor
+ * preprocessed, as received in the request): the Lagrange factor, basis B
+ * (galois_engine.rs:23-40) and the 31 rotations (galois_engine.rs:42-52,
+ * 479-488) are applied on the GPU; rows beyond batch*31 are zero padding up
+ * to query_length (actor.rs:895-901 pads to max_batch_size). */
+int irismpc_query_upload_shares(irismpc_share_db* e, const uint16_t* shares, size_t batch,
+                                void* stream, irismpc_query** out);
+int irismpc_query_destroy(irismpc_query* q);
+/* ShareDB::dot + ShareDB::dot_reduce_and_multiply (share_db.rs:800-905),
+ * fused: DB rows [offset, offset + chunk_size) against all query rows; results
+ * land in the engine's result buffer as [query_length][chunk_size] u16.
+ * multiplier = 1 for codes, 2 for trimmed masks. */
+int irismpc_share_db_dot_reduce(irismpc_share_db* e, const irismpc_query* q, const irismpc_db* db,
+                                size_t chunk_size, size_t offset, uint16_t multiplier,
+                                void* stream);
+/* Whole-range form: rows [offset, offset+total) in chunks of chunk_size rows
+ * in one launch; chunk c is written at out + c * chunk_size * query_length. */
+int irismpc_share_db_dot_reduce_range(irismpc_share_db* e, const irismpc_query* q,
+                                      const irismpc_db* db, size_t offset, size_t total,
+                                      size_t chunk_size, uint16_t multiplier, uint16_t* out,
+                                      void* stream);
+/* ShareDB::reshare_results (share_db.rs:1004-1036): collective. */
+int irismpc_share_db_reshare_results(irismpc_share_db* e, size_t chunk_size, void* stream);
+int irismpc_share_db_reshare_results_trio(irismpc_share_db* const e[3], size_t chunk_size,
+                                          void* const streams[3]);
+/* ShareDB::result_chunk_shares (share_db.rs:1050-1069): (a, b) = (results,
+ * results_peer) device pointers. */
+int irismpc_share_db_result_views(irismpc_share_db* e, const uint16_t** a, const uint16_t** b);
+/* ShareDB::fetch_results (share_db.rs:1038-1048). */
+int irismpc_share_db_fetch_results(irismpc_share_db* e, uint16_t* out, size_t n, void* stream);
+int irismpc_share_db_get_counter(const irismpc_share_db* e, uint64_t* counter);
+int irismpc_share_db_set_counter(irismpc_share_db* e, uint64_t counter);
+/* GEMM tile variant (see irismpc_gemm_variants); chosen by the autotuner. */
+int irismpc_share_db_set_variant(irismpc_share_db* e, int variant);
+int irismpc_gemm_variants(int* n);
+
+/* ------------------------------------------------------------- Circuits */
+/* Circuits::new (protocol.rs:463-508): input_size elements per call,
+ * alloc_size words (elements/64) of scratch. */
+int irismpc_circuits_new(int device, int peer_id, size_t input_size, size_t alloc_size,
+                         const uint32_t seed_own[8], const uint32_t seed_prev[8],
+                         irismpc_exchange* comm, int tag, irismpc_circuits** out);
+int irismpc_circuits_destroy(irismpc_circuits* c);
+/* Circuits::set_chunk_size (protocol.rs:510-515), in 64-element words. */
+int irismpc_circuits_set_chunk_size(irismpc_circuits* c, size_t chunk_words);
+/* Circuits::compare_threshold_masked_many (protocol.rs:2519-2546): code/mask
+ * replicated u16 shares (a, b) of len elements (multiple of 512); result bit
+ * shares in plane 0 of the result buffer. Collective. */
+int irismpc_circuits_compare_threshold_masked_many(irismpc_circuits* c, const uint16_t* code_a,
+                                                   const uint16_t* code_b,
+                                                   const uint16_t* mask_a,
+                                                   const uint16_t* mask_b, size_t len,
+                                                   uint32_t threshold_a, void* stream);
+int irismpc_circuits_compare_threshold_masked_many_trio(
+    irismpc_circuits* const c[3], const uint16_t* const code_a[3], const uint16_t* const code_b[3],
+    const uint16_t* const mask_a[3], const uint16_t* const mask_b[3], size_t len,
+    uint32_t threshold_a, void* const streams[3]);
+/* take_result_buffer (protocol.rs:520-526): device pointers of the result
+ * shares (a, b), chunk_size words each. */
+int irismpc_circuits_result_buffer(irismpc_circuits* c, const uint64_t** a, const uint64_t** b);
+/* The exchange inside actor.rs:2928-2997 `open`: own b share to next, prev's b
+ * into this party's c buffer. Collective. */
+int irismpc_circuits_open(irismpc_circuits* c, void* stream);
+int irismpc_circuits_open_trio(irismpc_circuits* const c[3], void* const streams[3]);
+int irismpc_circuits_open_views(irismpc_circuits* c, const uint64_t** a, const uint64_t** b,
+                                const uint64_t** cc);
+int irismpc_circuits_get_counter(const irismpc_circuits* c, uint64_t* counter);
+int irismpc_circuits_set_counter(irismpc_circuits* c, uint64_t counter);
+/* Debug capture of the corrected lifted mask (u32 a, b), NULL to disable. */
+int irismpc_circuits_set_lift_capture(irismpc_circuits* c, uint32_t* a, uint32_t* b);
+
+/* ------------------------------------------------------------- DistanceComparator */
+/* DistanceComparator::init (distance_comparator.rs:58-185). */
+int irismpc_comparator_init(int device, size_t query_length, size_t max_db_size,
+                            irismpc_comparator** out);
+int irismpc_comparator_destroy(irismpc_comparator* d);
+/* DistanceComparator::open_results (distance_comparator.rs:196-278) without
+ * the anon-stats capture: bit = a ^ b ^ c, OR over rotations into
+ * bitmap[total_db_len * (q / 31) + db + offset]. */
+int irismpc_comparator_open_results(irismpc_comparator* d, const uint64_t* a, const uint64_t* b,
+                                    const uint64_t* c, uint64_t* bitmap, size_t chunk_len,
+                                    size_t real_chunk_len, size_t offset, size_t total_db_len,
+                                    size_t batch_size, void* stream);
+/* join_db_matches / join_db_matches_with_bitmaps (distance_comparator.rs:
+ * 487-543,713-793): AND of the eyes, or per-pair OR where or_policy has the
+ * bit set (or_policy == NULL: AND everywhere). final_results: u32 per query,
+ * initialised to UINT32_MAX by the caller. */
+int irismpc_comparator_join_db_matches(irismpc_comparator* d, const uint64_t* left,
+                                       const uint64_t* right, uint32_t* final_results,
+                                       size_t db_size, const uint64_t* or_policy,
+                                       size_t max_db_size, void* stream);
+/* Resets match counters (start of a batch). */
+int irismpc_comparator_reset(irismpc_comparator* d, void* stream);
+/* Device pointers of match_counters, all_matches (256 per query),
+ * match_counters_left/right, partial_results_left/right. */
+int irismpc_comparator_buffers(irismpc_comparator* d, uint32_t** match_counters,
+                               uint32_t** all_matches, uint32_t** counters_left,
+                               uint32_t** counters_right, uint32_t** partial_left,
+                               uint32_t** partial_right);
+
+/* ------------------------------------------------------------- DB scan */
+/* One party's full-DB scan of a query batch: the chunk loop of
+ * ServerActor::compare_query_against_db_and_self_prefiltering (actor.rs:
+ * 2246-2589) - per 32,768-row chunk: dot + dot_reduce (codes, masks),
+ * reshare_results x2, compare_threshold_masked_many(threshold_a), open,
+ * open_results into bitmap. Collective: the three parties call concurrently. */
+int irismpc_scan_db(irismpc_share_db* codes, irismpc_share_db* masks, irismpc_circuits* circuits,
+                    irismpc_comparator* cmp, const irismpc_db* code_db, const irismpc_db* mask_db,
+                    const irismpc_query* q_code, const irismpc_query* q_mask, size_t db_size,
+                    size_t batch_size, uint32_t threshold_a, uint64_t* bitmap, void* stream);
+/* The three parties' scans from one call (one host thread per party inside). */
+int irismpc_scan_db_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
+                         irismpc_circuits* const circuits[3], irismpc_comparator* const cmp[3],
+                         const irismpc_db* const code_db[3], const irismpc_db* const mask_db[3],
+                         const irismpc_query* const q_code[3], const irismpc_query* const q_mask[3],
+                         size_t db_size, size_t batch_size, uint32_t threshold_a,
+                         uint64_t* const bitmaps[3], void* const streams[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

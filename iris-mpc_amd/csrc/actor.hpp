@@ -1,0 +1,32 @@
+// One party's full-DB scan of a query batch: the chunk loop of
+// ServerActor::compare_query_against_db_and_self_prefiltering
+// (iris-mpc-gpu/src/server/actor.rs:2246-2589), with the DB resident in HBM so
+// the prefetch stage disappears.
+#pragma once
+#include "engine.hpp"
+
+namespace irismpc {
+
+struct PartyScan {
+  ShareDB* codes;
+  ShareDB* masks;
+  Circuits* circuits;
+  DistanceComparator* comparator;
+  const DbSlice* code_db;
+  const DbSlice* mask_db;
+  const QuerySet* code_query;
+  const QuerySet* mask_query;
+};
+
+// For every chunk of kDbChunk rows: code/mask dot products with the fused
+// correction and zero-share masking, reshare, compare against threshold_a,
+// open and OR the opened bits into `bitmap` (bit total_db*(q/31) + db).
+// Collective: the three parties call it concurrently (one thread each).
+void scan_db(const PartyScan& p, size_t db_size, size_t batch_size, uint32_t threshold_a,
+             uint64_t* bitmap, hipStream_t s);
+
+// Runs the three parties' scans from one call (one host thread per party).
+void scan_db_trio(const PartyScan p[3], size_t db_size, size_t batch_size, uint32_t threshold_a,
+                  uint64_t* const bitmaps[3], hipStream_t const s[3]);
+
+}  // namespace irismpc

@@ -1,0 +1,455 @@
+// extern "C" boundary of libirismpc (declared in include/irismpc.h).
+#include <cstring>
+#include <string>
+
+#include "../../include/irismpc.h"
+#include "actor.hpp"
+#include "comparator.hpp"
+#include "engine.hpp"
+#include "host_crypto.hpp"
+
+using namespace irismpc;
+
+struct irismpc_exchange {
+  Exchange impl;
+  irismpc_exchange(int d, double t) : impl(d, t) {}
+};
+struct irismpc_share_db {
+  std::unique_ptr<ShareDB> impl;
+  int device;
+};
+struct irismpc_db {
+  std::unique_ptr<DbSlice> impl;
+};
+struct irismpc_query {
+  std::unique_ptr<QuerySet> impl;
+};
+struct irismpc_circuits {
+  std::unique_ptr<Circuits> impl;
+  int device;
+};
+struct irismpc_comparator {
+  std::unique_ptr<DistanceComparator> impl;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+  } catch (...) {
+    g_last_error = "unknown error";
+  }
+  return -1;
+}
+
+ChaChaKey key_of(const uint32_t* s) {
+  ChaChaKey k;
+  if (s)
+    std::memcpy(k.k, s, 32);
+  else
+    std::memset(k.k, 0, 32);
+  return k;
+}
+
+hipStream_t st(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+struct DevSet {
+  int prev = 0;
+  explicit DevSet(int d) {
+    IRIS_HIP_CHECK(hipGetDevice(&prev));
+    IRIS_HIP_CHECK(hipSetDevice(d));
+  }
+  ~DevSet() { (void)hipSetDevice(prev); }
+};
+}  // namespace
+
+extern "C" {
+
+const char* irismpc_last_error(void) { return g_last_error.c_str(); }
+int irismpc_version(void) { return 100; }
+
+int irismpc_device_count(int* n) {
+  return guarded([&] { IRIS_HIP_CHECK(hipGetDeviceCount(n)); });
+}
+
+int irismpc_stream_create(int device, void** stream) {
+  return guarded([&] {
+    DevSet g(device);
+    hipStream_t s;
+    IRIS_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+  });
+}
+int irismpc_stream_destroy(void* stream) {
+  return guarded([&] { IRIS_HIP_CHECK(hipStreamDestroy(st(stream))); });
+}
+int irismpc_stream_synchronize(void* stream) {
+  return guarded([&] { IRIS_HIP_CHECK(hipStreamSynchronize(st(stream))); });
+}
+int irismpc_malloc(int device, size_t bytes, void** ptr) {
+  return guarded([&] {
+    DevSet g(device);
+    IRIS_HIP_CHECK(hipMalloc(ptr, bytes ? bytes : 1));
+  });
+}
+int irismpc_free(void* ptr) {
+  return guarded([&] { IRIS_HIP_CHECK(hipFree(ptr)); });
+}
+int irismpc_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream) {
+  return guarded([&] {
+    IRIS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st(stream)));
+    IRIS_HIP_CHECK(hipStreamSynchronize(st(stream)));
+  });
+}
+int irismpc_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream) {
+  return guarded([&] {
+    IRIS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st(stream)));
+    IRIS_HIP_CHECK(hipStreamSynchronize(st(stream)));
+  });
+}
+int irismpc_fill_u32(void* ptr, size_t n, uint32_t value, void* stream) {
+  return guarded([&] { fill_u32(static_cast<uint32_t*>(ptr), n, value, st(stream)); });
+}
+int irismpc_fill_u64(void* ptr, size_t n, uint64_t value, void* stream) {
+  return guarded([&] { fill_u64(static_cast<uint64_t*>(ptr), n, value, st(stream)); });
+}
+int irismpc_derive_seed(const uint32_t seed[8], uint64_t nonce, uint32_t out[8]) {
+  return guarded([&] { derive_seed(seed, nonce, out); });
+}
+
+// ------------------------------------------------------------- exchange
+int irismpc_exchange_create(int device, double timeout_s, irismpc_exchange** out) {
+  return guarded([&] { *out = new irismpc_exchange(device, timeout_s > 0 ? timeout_s : 300.0); });
+}
+int irismpc_exchange_destroy(irismpc_exchange* ex) {
+  return guarded([&] { delete ex; });
+}
+
+// ------------------------------------------------------------- ShareDB
+int irismpc_share_db_init(int device, int peer_id, size_t max_db_length, size_t query_length,
+                          size_t code_length, const uint32_t seed_own[8],
+                          const uint32_t seed_prev[8], irismpc_exchange* comm, int tag,
+                          irismpc_share_db** out) {
+  return guarded([&] {
+    auto* h = new irismpc_share_db();
+    h->device = device;
+    try {
+      h->impl = std::make_unique<ShareDB>(device, peer_id, max_db_length, query_length,
+                                          (int)code_length, key_of(seed_own), key_of(seed_prev),
+                                          comm ? &comm->impl : nullptr, tag);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+int irismpc_share_db_destroy(irismpc_share_db* e) {
+  return guarded([&] { delete e; });
+}
+int irismpc_share_db_alloc_db(irismpc_share_db* e, size_t max_db_length, irismpc_db** out) {
+  return guarded([&] {
+    auto* h = new irismpc_db();
+    h->impl = e->impl->alloc_db(max_db_length);
+    *out = h;
+  });
+}
+int irismpc_db_destroy(irismpc_db* db) {
+  return guarded([&] { delete db; });
+}
+int irismpc_db_load_records(irismpc_share_db* e, irismpc_db* db, size_t index, size_t n,
+                            const uint16_t* records, void* stream) {
+  return guarded([&] { e->impl->load_records_u16(*db->impl, index, n, records, st(stream)); });
+}
+int irismpc_db_load_records_limbs(irismpc_share_db* e, irismpc_db* db, size_t index, size_t n,
+                                  const uint8_t* limb0, const uint8_t* limb1, void* stream) {
+  return guarded(
+      [&] { e->impl->load_records_limbs(*db->impl, index, n, limb0, limb1, st(stream)); });
+}
+int irismpc_db_write_at_index(irismpc_share_db* e, irismpc_db* db, const irismpc_query* q,
+                              size_t src_index, size_t dst_index, void* stream) {
+  return guarded(
+      [&] { e->impl->write_at_index(*db->impl, *q->impl, src_index, dst_index, st(stream)); });
+}
+int irismpc_db_read_records(irismpc_share_db* e, const irismpc_db* db, size_t index, size_t n,
+                            uint16_t* out, void* stream) {
+  return guarded([&] {
+    DevSet g(e->device);
+    IRIS_REQUIRE(index + n <= db->impl->capacity, "read beyond DB capacity");
+    const int K = e->impl->code_length();
+    DevBuf<uint16_t> tmp(n * (size_t)K, false);
+    launch_untile_rows(db->impl->limbs.ptr, index, n, K, tmp.ptr, st(stream));
+    IRIS_HIP_CHECK(hipMemcpyAsync(out, tmp.ptr, tmp.n * 2, hipMemcpyDeviceToHost, st(stream)));
+    IRIS_HIP_CHECK(hipStreamSynchronize(st(stream)));
+  });
+}
+int irismpc_db_fill_random(irismpc_share_db* e, irismpc_db* db, size_t index, size_t n,
+                           const uint32_t key[8], uint64_t nonce, void* stream) {
+  return guarded([&] {
+    DevSet g(e->device);
+    IRIS_REQUIRE(index + round_up(n, 16) <= db->impl->capacity, "fill beyond DB capacity");
+    launch_fill_random(db->impl->limbs.ptr, index, n, e->impl->code_length(), key_of(key), nonce,
+                       db->impl->sum0.ptr, db->impl->sum1.ptr, st(stream));
+  });
+}
+int irismpc_query_upload(irismpc_share_db* e, const uint16_t* rows, size_t n_rows, void* stream,
+                         irismpc_query** out) {
+  return guarded([&] {
+    auto* h = new irismpc_query();
+    h->impl = e->impl->upload_query(rows, n_rows, st(stream));
+    *out = h;
+  });
+}
+int irismpc_query_upload_shares(irismpc_share_db* e, const uint16_t* shares, size_t batch,
+                                void* stream, irismpc_query** out) {
+  return guarded([&] {
+    auto* h = new irismpc_query();
+    h->impl = e->impl->upload_query_shares(shares, batch, st(stream));
+    *out = h;
+  });
+}
+int irismpc_query_destroy(irismpc_query* q) {
+  return guarded([&] { delete q; });
+}
+int irismpc_share_db_dot_reduce(irismpc_share_db* e, const irismpc_query* q, const irismpc_db* db,
+                                size_t chunk_size, size_t offset, uint16_t multiplier,
+                                void* stream) {
+  return guarded([&] {
+    e->impl->dot_reduce(*q->impl, *db->impl, chunk_size, offset, multiplier, st(stream));
+  });
+}
+int irismpc_share_db_dot_reduce_range(irismpc_share_db* e, const irismpc_query* q,
+                                      const irismpc_db* db, size_t offset, size_t total,
+                                      size_t chunk_size, uint16_t multiplier, uint16_t* out,
+                                      void* stream) {
+  return guarded([&] {
+    e->impl->dot_reduce_range(*q->impl, *db->impl, offset, total, chunk_size, multiplier, out,
+                              st(stream));
+  });
+}
+int irismpc_share_db_reshare_results(irismpc_share_db* e, size_t chunk_size, void* stream) {
+  return guarded([&] { e->impl->reshare_results(chunk_size, st(stream)); });
+}
+int irismpc_share_db_reshare_results_trio(irismpc_share_db* const e[3], size_t chunk_size,
+                                          void* const streams[3]) {
+  return guarded([&] {
+    ShareDB* d[3] = {e[0]->impl.get(), e[1]->impl.get(), e[2]->impl.get()};
+    hipStream_t s[3] = {st(streams[0]), st(streams[1]), st(streams[2])};
+    ShareDB::reshare_trio(d, chunk_size, s);
+  });
+}
+int irismpc_share_db_result_views(irismpc_share_db* e, const uint16_t** a, const uint16_t** b) {
+  return guarded([&] {
+    *a = e->impl->results();
+    *b = e->impl->results_peer();
+  });
+}
+int irismpc_share_db_fetch_results(irismpc_share_db* e, uint16_t* out, size_t n, void* stream) {
+  return guarded([&] {
+    IRIS_REQUIRE(n <= e->impl->results_capacity(), "fetch beyond the result buffer");
+    IRIS_HIP_CHECK(hipMemcpyAsync(out, e->impl->results(), n * 2, hipMemcpyDeviceToHost, st(stream)));
+    IRIS_HIP_CHECK(hipStreamSynchronize(st(stream)));
+  });
+}
+int irismpc_share_db_get_counter(const irismpc_share_db* e, uint64_t* counter) {
+  return guarded([&] { *counter = e->impl->counter(); });
+}
+int irismpc_share_db_set_counter(irismpc_share_db* e, uint64_t counter) {
+  return guarded([&] { e->impl->set_counter(counter); });
+}
+int irismpc_share_db_set_variant(irismpc_share_db* e, int variant) {
+  return guarded([&] {
+    (void)gemm_tile(variant);
+    e->impl->variant = variant;
+  });
+}
+int irismpc_gemm_variants(int* n) {
+  return guarded([&] { *n = kGemmVariants; });
+}
+
+// ------------------------------------------------------------- Circuits
+int irismpc_circuits_new(int device, int peer_id, size_t input_size, size_t alloc_size,
+                         const uint32_t seed_own[8], const uint32_t seed_prev[8],
+                         irismpc_exchange* comm, int tag, irismpc_circuits** out) {
+  return guarded([&] {
+    IRIS_REQUIRE(comm != nullptr, "Circuits needs an exchange handle");
+    auto* h = new irismpc_circuits();
+    h->device = device;
+    try {
+      h->impl = std::make_unique<Circuits>(device, peer_id, input_size, alloc_size,
+                                           key_of(seed_own), key_of(seed_prev), &comm->impl, tag);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+int irismpc_circuits_destroy(irismpc_circuits* c) {
+  return guarded([&] { delete c; });
+}
+int irismpc_circuits_set_chunk_size(irismpc_circuits* c, size_t chunk_words) {
+  return guarded([&] { c->impl->set_chunk_size(chunk_words); });
+}
+int irismpc_circuits_compare_threshold_masked_many(irismpc_circuits* c, const uint16_t* code_a,
+                                                   const uint16_t* code_b,
+                                                   const uint16_t* mask_a,
+                                                   const uint16_t* mask_b, size_t len,
+                                                   uint32_t threshold_a, void* stream) {
+  return guarded([&] {
+    c->impl->compare_threshold_masked_many(code_a, code_b, mask_a, mask_b, len, threshold_a,
+                                           st(stream));
+  });
+}
+int irismpc_circuits_compare_threshold_masked_many_trio(
+    irismpc_circuits* const c[3], const uint16_t* const code_a[3], const uint16_t* const code_b[3],
+    const uint16_t* const mask_a[3], const uint16_t* const mask_b[3], size_t len,
+    uint32_t threshold_a, void* const streams[3]) {
+  return guarded([&] {
+    Circuits* cc[3] = {c[0]->impl.get(), c[1]->impl.get(), c[2]->impl.get()};
+    hipStream_t s[3] = {st(streams[0]), st(streams[1]), st(streams[2])};
+    Circuits::compare_trio(cc, code_a, code_b, mask_a, mask_b, len, threshold_a, s);
+  });
+}
+int irismpc_circuits_result_buffer(irismpc_circuits* c, const uint64_t** a, const uint64_t** b) {
+  return guarded([&] {
+    *a = c->impl->result_a();
+    *b = c->impl->result_b();
+  });
+}
+int irismpc_circuits_open(irismpc_circuits* c, void* stream) {
+  return guarded([&] { c->impl->open_exchange(st(stream)); });
+}
+int irismpc_circuits_open_trio(irismpc_circuits* const c[3], void* const streams[3]) {
+  return guarded([&] {
+    Circuits* cc[3] = {c[0]->impl.get(), c[1]->impl.get(), c[2]->impl.get()};
+    hipStream_t s[3] = {st(streams[0]), st(streams[1]), st(streams[2])};
+    Circuits::open_exchange_trio(cc, s);
+  });
+}
+int irismpc_circuits_open_views(irismpc_circuits* c, const uint64_t** a, const uint64_t** b,
+                                const uint64_t** cc) {
+  return guarded([&] {
+    *a = c->impl->result_a();
+    *b = c->impl->result_b();
+    *cc = c->impl->result_c();
+  });
+}
+int irismpc_circuits_get_counter(const irismpc_circuits* c, uint64_t* counter) {
+  return guarded([&] { *counter = c->impl->counter(); });
+}
+int irismpc_circuits_set_counter(irismpc_circuits* c, uint64_t counter) {
+  return guarded([&] { c->impl->set_counter(counter); });
+}
+int irismpc_circuits_set_lift_capture(irismpc_circuits* c, uint32_t* a, uint32_t* b) {
+  return guarded([&] {
+    c->impl->lifted_dbg_a = a;
+    c->impl->lifted_dbg_b = b;
+  });
+}
+
+// ------------------------------------------------------------- DistanceComparator
+int irismpc_comparator_init(int device, size_t query_length, size_t max_db_size,
+                            irismpc_comparator** out) {
+  return guarded([&] {
+    auto* h = new irismpc_comparator();
+    try {
+      h->impl = std::make_unique<DistanceComparator>(device, query_length, max_db_size);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+int irismpc_comparator_destroy(irismpc_comparator* d) {
+  return guarded([&] { delete d; });
+}
+int irismpc_comparator_open_results(irismpc_comparator* d, const uint64_t* a, const uint64_t* b,
+                                    const uint64_t* c, uint64_t* bitmap, size_t chunk_len,
+                                    size_t real_chunk_len, size_t offset, size_t total_db_len,
+                                    size_t batch_size, void* stream) {
+  return guarded([&] {
+    d->impl->open_results(a, b, c, bitmap, chunk_len, real_chunk_len, offset, total_db_len,
+                          batch_size, st(stream));
+  });
+}
+int irismpc_comparator_join_db_matches(irismpc_comparator* d, const uint64_t* left,
+                                       const uint64_t* right, uint32_t* final_results,
+                                       size_t db_size, const uint64_t* or_policy,
+                                       size_t max_db_size, void* stream) {
+  return guarded([&] {
+    d->impl->join_db_matches(left, right, final_results, db_size, or_policy, max_db_size,
+                             st(stream));
+  });
+}
+int irismpc_comparator_reset(irismpc_comparator* d, void* stream) {
+  return guarded([&] { d->impl->reset(st(stream)); });
+}
+int irismpc_comparator_buffers(irismpc_comparator* d, uint32_t** match_counters,
+                               uint32_t** all_matches, uint32_t** counters_left,
+                               uint32_t** counters_right, uint32_t** partial_left,
+                               uint32_t** partial_right) {
+  return guarded([&] {
+    *match_counters = d->impl->match_counters();
+    *all_matches = d->impl->all_matches();
+    *counters_left = d->impl->match_counters_left();
+    *counters_right = d->impl->match_counters_right();
+    *partial_left = d->impl->partial_results_left();
+    *partial_right = d->impl->partial_results_right();
+  });
+}
+
+// ------------------------------------------------------------- DB scan (actor chunk loop)
+static PartyScan party_scan(irismpc_share_db* codes, irismpc_share_db* masks,
+                            irismpc_circuits* circuits, irismpc_comparator* cmp,
+                            const irismpc_db* code_db, const irismpc_db* mask_db,
+                            const irismpc_query* q_code, const irismpc_query* q_mask) {
+  PartyScan p{};
+  p.codes = codes->impl.get();
+  p.masks = masks->impl.get();
+  p.circuits = circuits->impl.get();
+  p.comparator = cmp->impl.get();
+  p.code_db = code_db->impl.get();
+  p.mask_db = mask_db->impl.get();
+  p.code_query = q_code->impl.get();
+  p.mask_query = q_mask->impl.get();
+  return p;
+}
+
+int irismpc_scan_db(irismpc_share_db* codes, irismpc_share_db* masks, irismpc_circuits* circuits,
+                    irismpc_comparator* cmp, const irismpc_db* code_db, const irismpc_db* mask_db,
+                    const irismpc_query* q_code, const irismpc_query* q_mask, size_t db_size,
+                    size_t batch_size, uint32_t threshold_a, uint64_t* bitmap, void* stream) {
+  return guarded([&] {
+    scan_db(party_scan(codes, masks, circuits, cmp, code_db, mask_db, q_code, q_mask), db_size,
+            batch_size, threshold_a, bitmap, st(stream));
+  });
+}
+
+int irismpc_scan_db_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
+                         irismpc_circuits* const circuits[3], irismpc_comparator* const cmp[3],
+                         const irismpc_db* const code_db[3], const irismpc_db* const mask_db[3],
+                         const irismpc_query* const q_code[3], const irismpc_query* const q_mask[3],
+                         size_t db_size, size_t batch_size, uint32_t threshold_a,
+                         uint64_t* const bitmaps[3], void* const streams[3]) {
+  return guarded([&] {
+    PartyScan p[3];
+    hipStream_t s[3];
+    for (int i = 0; i < 3; i++) {
+      p[i] = party_scan(codes[i], masks[i], circuits[i], cmp[i], code_db[i], mask_db[i], q_code[i],
+                        q_mask[i]);
+      s[i] = st(streams[i]);
+    }
+    scan_db_trio(p, db_size, batch_size, threshold_a, bitmaps, s);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,421 @@
+// Bit-sliced replicated-share circuits for gfx950.
+//
+// Restates threshold_ring/protocol.rs (lift_mpc :2041-2074,
+// binary_add_3_get_two_carries :2137-2241, bit_inject :1302-1643,
+// lift_mul_sub :1831-1872, extract_msb/binary_add_3_get_msb :2243-2359) and
+// threshold_ring/cuda/kernel.cu. Design for one GPU:
+//  * the 16x64 / 32x64 bit transposes are wave ballots (lane = element);
+//  * x1/x2/x3 of `split` are never materialised: for party p they are fixed
+//    selections of the transposed (a, b) planes, so x2^x3, x1^x3 and the
+//    carry-save AND collapse to one expression per party (bit-identical);
+//  * correlated randomness is generated in registers from the block counter
+//    the reference would have used (no keystream buffers);
+//  * each launch covers all three parties (grid.y); a party's threads read
+//    only that party's state and write messages into the next party's "b"
+//    arrays / inbox. The kernel boundary is the exchange barrier.
+#include "circuits.hpp"
+#include "chacha.hpp"
+
+namespace irismpc {
+
+CircuitSchedule CircuitSchedule::make(uint64_t c, size_t W, size_t alloc) {
+  CircuitSchedule s{};
+  const uint64_t b1 = ceil_div(W, 8);
+  s.lift_csa = c;
+  c += 2 * ceil_div(16 * W, 8);  // AND draw + OTP of the 16-plane exchange
+  s.lift_first = c;
+  c += 2 * b1;
+  for (int k = 1; k <= 14; k++) {
+    s.lift_round[k] = c;
+    c += 2 * b1;
+  }
+  s.lift_final = c;
+  c += 2 * b1;
+  s.inject = c;
+  c += 4 * alloc + 4 * W;  // rand buffers cover the whole alloc; u16 OTP of 128 W
+  s.msb_csa = c;
+  c += 2 * ceil_div(31 * W, 8);
+  s.msb_first = c;
+  c += 2 * b1;
+  for (int k = 1; k <= 29; k++) {
+    s.msb_round[k] = c;
+    c += 2 * b1;
+  }
+  s.end = c;
+  return s;
+}
+
+__device__ __forceinline__ int party_of(const Trio& t) { return t.party0 + (int)blockIdx.y; }
+
+// Correlated randomness F(own) ^ F(prev) as 8 u64 words of block `blk`.
+__device__ __forceinline__ void corr_u64x8(const CircuitBufs& P, uint64_t nonce, uint64_t blk,
+                                           uint64_t r[8]) {
+  uint32_t k1[16], k2[16];
+  chacha_block<12>(P.key_own, nonce, blk, k1);
+  chacha_block<12>(P.key_prev, nonce, blk, k2);
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    r[i] = (uint64_t)(k1[2 * i] ^ k2[2 * i]) | ((uint64_t)(k1[2 * i + 1] ^ k2[2 * i + 1]) << 32);
+}
+
+// 8 consecutive words; every call site addresses a multiple of 8 words from a
+// 256-byte aligned allocation (W is a multiple of 8), so 64-byte alignment
+// holds and the accesses vectorise.
+__device__ __forceinline__ void load8(const uint64_t* p, uint64_t v[8]) {
+  const uint64_t* q = static_cast<const uint64_t*>(__builtin_assume_aligned(p, 64));
+#pragma unroll
+  for (int i = 0; i < 8; i++) v[i] = q[i];
+}
+
+__device__ __forceinline__ void store8(uint64_t* p, const uint64_t v[8]) {
+  uint64_t* q = static_cast<uint64_t*>(__builtin_assume_aligned(p, 64));
+#pragma unroll
+  for (int i = 0; i < 8; i++) q[i] = v[i];
+}
+
+// ------------------------------------------------------------ transpose
+// Plane j, word i, bit k = bit j of element 64 i + k (kernel.cu:63-84).
+__global__ __launch_bounds__(256) void transpose16_kernel(const Trio t) {
+  const int p = party_of(t);
+  const CircuitBufs& P = t.p[p];
+  const int lane = threadIdx.x & 63;
+  const size_t w = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= t.W) return;
+  const uint32_t va = P.mask_a[w * 64 + lane], vb = P.mask_b[w * 64 + lane];
+  uint64_t mine = 0;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const uint64_t ba = __ballot((va >> j) & 1u);
+    const uint64_t bb = __ballot((vb >> j) & 1u);
+    if (lane == j) mine = ba;
+    if (lane == 16 + j) mine = bb;
+  }
+  if (lane < 16)
+    P.tm_a[(size_t)lane * t.W + w] = mine;
+  else if (lane < 32)
+    P.tm_b[(size_t)(lane - 16) * t.W + w] = mine;
+}
+
+// ------------------------------------------------------------ carry-save AND
+// and_pre(x1^x3, x2^x3) for the split of party p (kernel.cu:39-42,205-235):
+//   p0: x1=(a,0) x2=0 x3=(0,b)  -> a & b
+//   p1: x1=(0,b) x2=(a,0) x3=0  -> a & b
+//   p2: x1=0 x2=(0,b) x3=(a,0)  -> a ^ (a & b)
+__device__ __forceinline__ uint64_t csa_and(int p, uint64_t a, uint64_t b) {
+  return p == 2 ? (a & ~b) : (a & b);
+}
+
+// Thread per 8 consecutive words of the plane-major packed array.
+__global__ __launch_bounds__(256) void csa_kernel(const Trio t, int msb, uint64_t ctr) {
+  const int p = party_of(t);
+  const CircuitBufs& P = t.p[p];
+  const CircuitBufs& N = t.p[(p + 1) % 3];
+  const size_t n = (msb ? 31 : 16) * t.W;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (8 * g >= n) return;
+  const uint64_t* Ta = msb ? P.tx_a : P.tm_a;
+  const uint64_t* Tb = msb ? P.tx_b : P.tm_b;
+  uint64_t* out = msb ? P.cm_a : P.cl_a;
+  uint64_t* nb = msb ? N.cm_b : N.cl_b;
+  uint64_t a[8], b[8], r[8], v[8];
+  load8(Ta + 8 * g, a);
+  load8(Tb + 8 * g, b);
+  corr_u64x8(P, t.nonce, ctr + g, r);
+#pragma unroll
+  for (int i = 0; i < 8; i++) v[i] = csa_and(p, a[i], b[i]) ^ r[i];
+  store8(out + 8 * g, v);
+  store8(nb + 8 * g, v);
+}
+
+// ------------------------------------------------------------ ripple rounds
+// a_k = s plane k = transposed plane k+1; b_k = carry plane k ^ x3 plane k
+// (x3 = (0, b) for p0, (a, 0) for p2, 0 for p1).
+__device__ __forceinline__ void load_bk(int p, const uint64_t* Ca, const uint64_t* Cb,
+                                        const uint64_t* Ta, const uint64_t* Tb, size_t off,
+                                        uint64_t ba[8], uint64_t bb[8]) {
+  load8(Ca + off, ba);
+  load8(Cb + off, bb);
+  if (p == 0) {
+    uint64_t x[8];
+    load8(Tb + off, x);
+#pragma unroll
+    for (int i = 0; i < 8; i++) bb[i] ^= x[i];
+  } else if (p == 2) {
+    uint64_t x[8];
+    load8(Ta + off, x);
+#pragma unroll
+    for (int i = 0; i < 8; i++) ba[i] ^= x[i];
+  }
+}
+
+__device__ __forceinline__ uint64_t and_pre(uint64_t xa, uint64_t xb, uint64_t ya, uint64_t yb,
+                                            uint64_t r) {
+  return (xa & ya) ^ (xb & ya) ^ (xa & yb) ^ r;
+}
+
+__global__ __launch_bounds__(256) void ripple_kernel(const Trio t, int msb, int mode, int k,
+                                                     uint64_t ctr) {
+  const int p = party_of(t);
+  const CircuitBufs& P = t.p[p];
+  const CircuitBufs& N = t.p[(p + 1) % 3];
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (8 * g >= t.W) return;
+  const size_t W = t.W, o = 8 * g;
+  const uint64_t* Ta = msb ? P.tx_a : P.tm_a;
+  const uint64_t* Tb = msb ? P.tx_b : P.tm_b;
+  const uint64_t* Ca = msb ? P.cm_a : P.cl_a;
+  const uint64_t* Cb = msb ? P.cm_b : P.cl_b;
+  uint64_t* ca = msb ? P.res_a : P.c_a;
+  uint64_t* cb = msb ? P.res_b : P.c_b;
+  uint64_t* ncb = msb ? N.res_b : N.c_b;
+  uint64_t r[8];
+
+  if (mode == (int)RippleMode::First) {
+    uint64_t aa[8], ab[8], ba[8], bb[8], v[8];
+    load8(Ta + W + o, aa);
+    load8(Tb + W + o, ab);
+    load_bk(p, Ca, Cb, Ta, Tb, o, ba, bb);
+    corr_u64x8(P, t.nonce, ctr + g, r);
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = and_pre(aa[i], ab[i], ba[i], bb[i], r[i]);
+    store8(ca + o, v);
+    store8(ncb + o, v);
+    return;
+  }
+
+  // running carry c (a, b), plus the previous round's exchanged AND output
+  uint64_t xa[8], xb[8];
+  load8(ca + o, xa);
+  load8(cb + o, xb);
+  const bool has_tmp = (mode != (int)RippleMode::Round) || k >= 2;
+  if (has_tmp) {
+    const int slot = (k - 1) & 1;
+    uint64_t ta[8], tb[8];
+    load8(P.tmp_a + slot * W + o, ta);
+    load8(P.tmp_b + slot * W + o, tb);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      xa[i] ^= ta[i];
+      xb[i] ^= tb[i];
+    }
+  }
+
+  if (mode == (int)RippleMode::Round) {
+    uint64_t aa[8], ab[8], ba[8], bb[8], v[8];
+    load8(Ta + (size_t)(k + 1) * W + o, aa);
+    load8(Tb + (size_t)(k + 1) * W + o, ab);
+    load_bk(p, Ca, Cb, Ta, Tb, (size_t)k * W + o, ba, bb);
+    corr_u64x8(P, t.nonce, ctr + g, r);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      aa[i] ^= xa[i];
+      ab[i] ^= xb[i];
+      ba[i] ^= xa[i];
+      bb[i] ^= xb[i];
+      v[i] = and_pre(aa[i], ab[i], ba[i], bb[i], r[i]);
+    }
+    if (has_tmp) {
+      store8(ca + o, xa);
+      store8(cb + o, xb);
+    }
+    const int slot = k & 1;
+    store8(P.tmp_a + slot * W + o, v);
+    store8(N.tmp_b + slot * W + o, v);
+    return;
+  }
+
+  if (mode == (int)RippleMode::LiftFinal) {
+    // c2 = b15 & c (shared AND), c1 = c ^ b15 (protocol.rs:2229-2239)
+    uint64_t ba[8], bb[8], v[8];
+    load_bk(p, Ca, Cb, Ta, Tb, 15 * W + o, ba, bb);
+    corr_u64x8(P, t.nonce, ctr + g, r);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      v[i] = and_pre(ba[i], bb[i], xa[i], xb[i], r[i]);
+      xa[i] ^= ba[i];
+      xb[i] ^= bb[i];
+    }
+    store8(ca + o, xa);
+    store8(cb + o, xb);
+    store8(ca + W + o, v);
+    store8(ncb + W + o, v);
+    return;
+  }
+
+  // MsbFinal: msb = c ^ a_30 ^ b_30 (protocol.rs:2347-2352)
+  uint64_t aa[8], ab[8], ba[8], bb[8];
+  load8(Ta + 31 * W + o, aa);
+  load8(Tb + 31 * W + o, ab);
+  load_bk(p, Ca, Cb, Ta, Tb, 30 * W + o, ba, bb);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    xa[i] ^= aa[i] ^ ba[i];
+    xb[i] ^= ab[i] ^ bb[i];
+  }
+  store8(ca + o, xa);
+  store8(cb + o, xb);
+}
+
+// ------------------------------------------------------------ bit injection
+// 128 W elements e: bits of the two lift carries (e < 64 W: bit 16,
+// e >= 64 W: bit 17). Thread per 32 elements = one ChaCha block per stream.
+// kernel.cu:526-629 with the message flow P1 -> P2 -> P0 and P0 -> P1.
+__device__ __forceinline__ uint32_t bit_at(const uint64_t* w, size_t e) {
+  return (uint32_t)((w[e >> 6] >> (e & 63)) & 1);
+}
+
+__global__ __launch_bounds__(256) void inject_kernel(const Trio t, int step, uint64_t ctr) {
+  const int p = party_of(t);
+  const CircuitBufs& P = t.p[p];
+  const CircuitBufs& N = t.p[(p + 1) % 3];
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = 128 * t.W;
+  if (32 * g >= n) return;
+  // which party acts in which step
+  const bool act = (step == 1 && p != 2) || (step == 2 && p != 0) || (step == 3 && p == 0);
+  if (!act) return;
+  uint32_t ko[16], kp[16];
+  chacha_block<12>(P.key_own, t.nonce, ctr + g, ko);   // my_rng (chacha1)
+  chacha_block<12>(P.key_prev, t.nonce, ctr + g, kp);  // their_rng (chacha2)
+  const size_t e0 = 32 * g;
+#pragma unroll 4
+  for (int i = 0; i < 32; i++) {
+    const size_t e = e0 + i;
+    // u16 view of the u32 keystream (little endian): element i = half i%2 of word i/2
+    const uint16_t own_i = (uint16_t)(ko[i >> 1] >> (16 * (i & 1)));
+    const uint16_t prv_i = (uint16_t)(kp[i >> 1] >> (16 * (i & 1)));
+    if (p == 0) {
+      // rand_01 = my rng, rand_02 = their rng; in_b = party 0's b share
+      const uint16_t r01 = own_i, r02 = prv_i;
+      const uint16_t b = (uint16_t)bit_at(P.c_b, e);
+      if (step == 1) {
+        const uint16_t y = (uint16_t)(b * r01 - r02);
+        P.msg_own[e] = y;
+        N.inbox[e] = y;
+        P.corr_a[e] = (uint16_t)(r01 - (y << 1));
+      } else {
+        const uint16_t z = P.inbox[e];
+        P.corr_b[e] = (uint16_t)(-((r02 + z) << 1) + b);
+      }
+    } else if (p == 1) {
+      // rand_01 = their rng, rand_12 = my rng
+      const uint16_t r01 = prv_i, r12 = own_i;
+      if (step == 1) {
+        const uint16_t x = (uint16_t)((bit_at(P.c_a, e) ^ bit_at(P.c_b, e)) - r01);
+        P.msg_own[e] = x;
+        N.inbox[e] = x;
+        P.corr_a[e] = (uint16_t)(x - (r12 << 1));
+      } else {
+        const uint16_t y = P.inbox[e];
+        P.corr_b[e] = (uint16_t)(r01 - (y << 1));
+      }
+    } else {
+      // rand_12 = their rng, rand_02 = my rng; in_a = party 2's a share
+      const uint16_t r12 = prv_i, r02 = own_i;
+      const uint16_t x = P.inbox[e];
+      const uint16_t b = (uint16_t)bit_at(P.c_a, e);
+      const uint16_t z = (uint16_t)(b * x - r12);
+      P.msg_own[e] = z;
+      N.inbox[e] = z;
+      P.corr_a[e] = (uint16_t)(-((r02 + z) << 1) + b);
+      P.corr_b[e] = (uint16_t)(x - (r12 << 1));
+    }
+  }
+}
+
+// ------------------------------------------------------------ lift_mul_sub + transpose32
+// m = (mask - c1<<16 - c2<<17) * A - code<<16, minus 1 on x_0 (kernel.cu:181-195,
+// 407-426), then the 32x64 bit transpose (kernel.cu:86-107) by ballots.
+__global__ __launch_bounds__(256) void lms_transpose32_kernel(const Trio t, uint32_t A) {
+  const int p = party_of(t);
+  const CircuitBufs& P = t.p[p];
+  const int lane = threadIdx.x & 63;
+  const size_t w = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= t.W) return;
+  const size_t L = 64 * t.W, e = w * 64 + lane;
+  uint32_t ma = (uint32_t)P.mask_a[e] - ((uint32_t)P.corr_a[e] << 16) - ((uint32_t)P.corr_a[e + L] << 17);
+  uint32_t mb = (uint32_t)P.mask_b[e] - ((uint32_t)P.corr_b[e] << 16) - ((uint32_t)P.corr_b[e + L] << 17);
+  if (P.lifted_a) {
+    P.lifted_a[e] = ma;
+    P.lifted_b[e] = mb;
+  }
+  ma = ma * A - ((uint32_t)P.code_a[e] << 16);
+  mb = mb * A - ((uint32_t)P.code_b[e] << 16);
+  if (p == 0) ma -= 1u;
+  if (p == 1) mb -= 1u;
+  uint64_t mine = 0;
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+    const uint64_t ba = __ballot((ma >> j) & 1u);
+    const uint64_t bb = __ballot((mb >> j) & 1u);
+    if (lane == j) mine = ba;
+    if (lane == 32 + j) mine = bb;
+  }
+  if (lane < 32)
+    P.tx_a[(size_t)lane * t.W + w] = mine;
+  else
+    P.tx_b[(size_t)(lane - 32) * t.W + w] = mine;
+}
+
+// ------------------------------------------------------------ open -> bitmap
+// Wave per result word: word i covers 64 consecutive DB rows of one query row
+// (chunk widths are multiples of 64), so the opened bits land in a 64-bit run
+// of bitmap bits starting at totalDb*(q/31) + db + offset: two atomicOr.
+__global__ __launch_bounds__(256) void open_kernel(const OpenArgs a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_words) return;
+  uint64_t v = a.a[i] ^ a.b[i] ^ a.c[i];
+  const size_t e0 = i * 64;
+  const size_t q = e0 / a.chunk_len, d0 = e0 % a.chunk_len;
+  if (q >= a.query_len || d0 >= a.real_chunk_len) return;
+  const size_t valid = a.real_chunk_len - d0;
+  if (valid < 64) v &= (1ull << valid) - 1;
+  if (!v) return;
+  const size_t bit = a.total_db_len * (q / kRotations) + d0 + a.offset;
+  const size_t wi = bit >> 6, sh = bit & 63;
+  atomicOr(reinterpret_cast<unsigned long long*>(&a.bitmap[wi]), (unsigned long long)(v << sh));
+  if (sh && (v >> (64 - sh)))
+    atomicOr(reinterpret_cast<unsigned long long*>(&a.bitmap[wi + 1]),
+             (unsigned long long)(v >> (64 - sh)));
+}
+
+// ------------------------------------------------------------ launchers
+void launch_transpose16(const Trio& t, int np, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(t.W, 4), (unsigned)np);
+  transpose16_kernel<<<grid, 256, 0, s>>>(t);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_csa(const Trio& t, int np, bool msb, uint64_t ctr, hipStream_t s) {
+  const size_t groups = (msb ? 31 : 16) * t.W / 8;
+  dim3 grid((unsigned)ceil_div(groups, 256), (unsigned)np);
+  csa_kernel<<<grid, 256, 0, s>>>(t, msb ? 1 : 0, ctr);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_ripple(const Trio& t, int np, bool msb, RippleMode mode, int k, uint64_t ctr,
+                   hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(t.W / 8, 256), (unsigned)np);
+  ripple_kernel<<<grid, 256, 0, s>>>(t, msb ? 1 : 0, (int)mode, k, ctr);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_inject(const Trio& t, int np, int step, uint64_t ctr, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(128 * t.W / 32, 256), (unsigned)np);
+  inject_kernel<<<grid, 256, 0, s>>>(t, step, ctr);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_lms_transpose32(const Trio& t, int np, uint32_t A, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(t.W, 4), (unsigned)np);
+  lms_transpose32_kernel<<<grid, 256, 0, s>>>(t, A);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_open(const OpenArgs& a, hipStream_t s) {
+  if (a.n_words == 0) return;
+  open_kernel<<<(unsigned)ceil_div(a.n_words, 256), 256, 0, s>>>(a);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace irismpc

@@ -1,0 +1,78 @@
+// Threshold-comparison circuits on replicated shares (3 parties, one GPU).
+#pragma once
+#include "common.hpp"
+
+namespace irismpc {
+
+// One party's Circuits state (device pointers). W = words per bit plane
+// (elements / 64). Messages to the next party are written straight into that
+// party's "b" arrays (or its inbox), which is the device-to-device exchange
+// that replaces the reference's NCCL send/recv.
+struct CircuitBufs {
+  const uint16_t* code_a;
+  const uint16_t* code_b;
+  const uint16_t* mask_a;
+  const uint16_t* mask_b;
+  uint64_t *tm_a, *tm_b;    // 16 W: transposed mask shares
+  uint64_t *cl_a, *cl_b;    // 16 W: lift carry-save carries
+  uint64_t *c_a, *c_b;      // 2 W: ripple carry; after the lift: (bit 16, bit 17)
+  uint64_t *tmp_a, *tmp_b;  // 2 W: ping-pong exchange slots of the ripple rounds
+  uint16_t* msg_own;        // 128 W: this party's bit-inject message (y, x or z)
+  uint16_t* inbox;          // 128 W: bit-inject message received from prev
+  uint16_t *corr_a, *corr_b;  // 128 W: injected carries (arithmetic u16 shares)
+  uint64_t *tx_a, *tx_b;    // 32 W: transposed lifted comparison values
+  uint64_t *cm_a, *cm_b;    // 31 W: MSB carry-save carries
+  uint64_t *res_a, *res_b;  // W: result bit shares (plane 0)
+  uint32_t *lifted_a, *lifted_b;  // optional (debug): corrected lifted mask
+  ChaChaKey key_own, key_prev;    // Circuits ChaCha seeds (chacha1 / chacha2)
+};
+
+struct Trio {
+  CircuitBufs p[3];
+  size_t W;
+  uint64_t nonce;
+  int party0;  // first party of this launch (grid.y covers party0 .. party0+gridDim.y-1)
+};
+
+// Block counters of every RNG draw of compare_threshold_masked_many, following
+// the reference schedule (protocol.rs:2519-2546): all six ChaCha streams of the
+// three parties advance identically, OTP draws included.
+struct CircuitSchedule {
+  uint64_t lift_csa;            // packed AND over 16 W words
+  uint64_t lift_first;          // first full adder
+  uint64_t lift_round[15];      // ripple rounds k = 1..14 (index k)
+  uint64_t lift_final;          // carry-out AND
+  uint64_t inject;              // rand_01 / rand_02 / rand_12 base
+  uint64_t msb_csa;             // packed AND over 31 W words
+  uint64_t msb_first;
+  uint64_t msb_round[30];       // k = 1..29
+  uint64_t end;                 // counter after the call
+  static CircuitSchedule make(uint64_t start, size_t W, size_t alloc_words);
+};
+
+enum class RippleMode : int { First = 0, Round = 1, LiftFinal = 2, MsbFinal = 3 };
+
+void launch_transpose16(const Trio& t, int n_parties, hipStream_t s);
+void launch_csa(const Trio& t, int n_parties, bool msb, uint64_t ctr, hipStream_t s);
+void launch_ripple(const Trio& t, int n_parties, bool msb, RippleMode mode, int k, uint64_t ctr,
+                   hipStream_t s);
+void launch_inject(const Trio& t, int n_parties, int step, uint64_t ctr, hipStream_t s);
+void launch_lms_transpose32(const Trio& t, int n_parties, uint32_t threshold_a, hipStream_t s);
+
+// Open + bitmap (actor.rs:2928-2997, dot/kernel.cu:30-116). Party views:
+// a = own result.a, b = own result.b, c = prev party's result.b (received).
+struct OpenArgs {
+  const uint64_t* a;
+  const uint64_t* b;
+  const uint64_t* c;
+  uint64_t* bitmap;
+  size_t n_words;        // W
+  size_t chunk_len;      // padded chunk width (multiple of 64)
+  size_t query_len;      // real batch * 31
+  size_t offset;         // global DB offset of the chunk
+  size_t real_chunk_len;
+  size_t total_db_len;
+};
+void launch_open(const OpenArgs& a, hipStream_t s);
+
+}  // namespace irismpc

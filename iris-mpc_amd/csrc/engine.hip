@@ -1,0 +1,548 @@
+// Host engines: Exchange, ShareDB, Circuits, DistanceComparator.
+#include "engine.hpp"
+#include "comparator.hpp"
+#include "query_prep.hpp"
+
+#include <chrono>
+#include <cstring>
+
+namespace irismpc {
+
+namespace {
+struct DeviceGuard {
+  int prev = 0;
+  explicit DeviceGuard(int d) {
+    IRIS_HIP_CHECK(hipGetDevice(&prev));
+    if (prev != d) IRIS_HIP_CHECK(hipSetDevice(d));
+  }
+  ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+uint64_t op_id(const char* name, int tag) {
+  uint64_t h = 1469598103934665603ull;
+  for (const char* p = name; *p; ++p) h = (h ^ (uint8_t)*p) * 1099511628211ull;
+  return h ^ ((uint64_t)tag << 48);
+}
+}  // namespace
+
+// ====================================================================== Exchange
+Exchange::Exchange(int device, double timeout_s) : device_(device), timeout_s_(timeout_s) {
+  DeviceGuard g(device_);
+  IRIS_HIP_CHECK(hipStreamCreateWithFlags(&protocol_, hipStreamNonBlocking));
+}
+
+Exchange::~Exchange() {
+  if (protocol_) (void)hipStreamDestroy(protocol_);
+}
+
+void Exchange::collective(int party, uint64_t op, void* arg,
+                          const std::function<void(void* const*)>& fn) {
+  IRIS_REQUIRE(party >= 0 && party < 3, "party id must be 0, 1 or 2");
+  std::unique_lock<std::mutex> lk(mu_);
+  const uint64_t gen = generation_;
+  IRIS_REQUIRE(args_[party] == nullptr || arrived_ == 0, "party joined the same collective twice");
+  args_[party] = arg;
+  ops_[party] = op;
+  if (++arrived_ == 3) {
+    std::exception_ptr err;
+    if (!(ops_[0] == ops_[1] && ops_[1] == ops_[2])) {
+      err = std::make_exception_ptr(Error("collective mismatch: parties called different operations"));
+    } else {
+      try {
+        fn(args_);
+      } catch (...) {
+        err = std::current_exception();
+      }
+    }
+    for (auto& a : args_) a = nullptr;
+    arrived_ = 0;
+    error_ = err;
+    error_gen_ = gen;
+    ++generation_;
+    cv_.notify_all();
+    if (err) std::rethrow_exception(err);
+    return;
+  }
+  const bool done = cv_.wait_for(lk, std::chrono::duration<double>(timeout_s_),
+                                 [&] { return generation_ != gen; });
+  if (!done) {
+    --arrived_;
+    args_[party] = nullptr;
+    throw Error("exchange timeout: a peer party did not join the collective");
+  }
+  if (error_gen_ == gen && error_) std::rethrow_exception(error_);
+}
+
+void Exchange::join_streams(hipStream_t const s[3]) {
+  for (int p = 0; p < 3; p++) {
+    if (s[p] == protocol_) continue;
+    IRIS_HIP_CHECK(hipEventRecord(ev_party_[p].e, s[p]));
+    IRIS_HIP_CHECK(hipStreamWaitEvent(protocol_, ev_party_[p].e, 0));
+  }
+}
+
+void Exchange::fork_streams(hipStream_t const s[3]) {
+  IRIS_HIP_CHECK(hipEventRecord(ev_proto_.e, protocol_));
+  for (int p = 0; p < 3; p++)
+    if (s[p] != protocol_) IRIS_HIP_CHECK(hipStreamWaitEvent(s[p], ev_proto_.e, 0));
+}
+
+// ====================================================================== ShareDB
+ShareDB::ShareDB(int device, int peer_id, size_t max_db_length, size_t query_length,
+                 int code_length, const ChaChaKey& seed_own, const ChaChaKey& seed_prev,
+                 Exchange* ex, int tag)
+    : device_(device),
+      peer_id_(peer_id),
+      code_length_(code_length),
+      tag_(tag),
+      query_length_(query_length),
+      max_db_length_(max_db_length),
+      seed_own_(seed_own),
+      seed_prev_(seed_prev),
+      ex_(ex) {
+  IRIS_REQUIRE(peer_id >= 0 && peer_id < 3, "peer_id must be 0..2");
+  IRIS_REQUIRE(code_length % kKTile == 0 && code_length % 800 == 0,
+               "code_length must be a multiple of 800 (rows of 200 columns x 4 coefs)");
+  IRIS_REQUIRE(query_length > 0 && max_db_length > 0, "empty engine");
+  DeviceGuard g(device_);
+  const size_t rl = round_up(max_db_length * query_length, 64);
+  results_.alloc(rl);
+  results_peer_.alloc(rl);
+}
+
+ShareDB::~ShareDB() = default;
+
+std::unique_ptr<DbSlice> ShareDB::alloc_db(size_t max_db_length) const {
+  DeviceGuard g(device_);
+  auto db = std::make_unique<DbSlice>();
+  db->code_length = code_length_;
+  // capacity padded so every GEMM tile (up to 256 rows from any 16-aligned
+  // start) stays inside the allocation
+  db->capacity = round_up(max_db_length, kMaxBD) + kMaxBD;
+  db->limbs.alloc(db->capacity * (size_t)code_length_ * 2);
+  db->sum0.alloc(db->capacity);
+  db->sum1.alloc(db->capacity);
+  return db;
+}
+
+void ShareDB::load_records_u16(DbSlice& db, size_t index, size_t n, const uint16_t* host,
+                               hipStream_t s) const {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(index + n <= db.capacity - kMaxBD, "record index beyond DB capacity");
+  const size_t K = (size_t)code_length_;
+  const size_t batch = std::max<size_t>(1, (64ull << 20) / (K * 2));
+  if (db.staging.n < batch * K) db.staging.alloc(batch * K, false);
+  for (size_t done = 0; done < n; done += batch) {
+    const size_t m = std::min(batch, n - done);
+    IRIS_HIP_CHECK(hipMemcpyAsync(db.staging.ptr, host + done * K, m * K * 2, hipMemcpyHostToDevice, s));
+    launch_tile_rows(db.staging.ptr, m, code_length_, db.limbs.ptr, index + done, db.sum0.ptr,
+                     db.sum1.ptr, s);
+    IRIS_HIP_CHECK(hipStreamSynchronize(s));  // staging reuse
+  }
+}
+
+void ShareDB::load_records_limbs(DbSlice& db, size_t index, size_t n, const uint8_t* l0,
+                                 const uint8_t* l1, hipStream_t s) const {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(index + n <= db.capacity - kMaxBD, "record index beyond DB capacity");
+  const size_t K = (size_t)code_length_;
+  const size_t batch = std::max<size_t>(1, (64ull << 20) / (K * 2));
+  if (db.staging.n < batch * K) db.staging.alloc(batch * K, false);
+  uint8_t* s0 = reinterpret_cast<uint8_t*>(db.staging.ptr);
+  uint8_t* s1 = s0 + batch * K;
+  for (size_t done = 0; done < n; done += batch) {
+    const size_t m = std::min(batch, n - done);
+    IRIS_HIP_CHECK(hipMemcpyAsync(s0, l0 + done * K, m * K, hipMemcpyHostToDevice, s));
+    IRIS_HIP_CHECK(hipMemcpyAsync(s1, l1 + done * K, m * K, hipMemcpyHostToDevice, s));
+    launch_tile_limbs(s0, s1, m, code_length_, db.limbs.ptr, index + done, db.sum0.ptr, db.sum1.ptr, s);
+    IRIS_HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
+void ShareDB::write_at_index(DbSlice& db, const QuerySet& q, size_t src_index, size_t dst_index,
+                             hipStream_t s) const {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(q.code_length == code_length_, "query/engine code length mismatch");
+  // reference copies the centre rotation (rotation 15) of query src_index
+  const size_t src_row = src_index * kRotations + 15;
+  IRIS_REQUIRE(src_row < (size_t)q.rows, "src_index beyond query batch");
+  IRIS_REQUIRE(dst_index < db.capacity - kMaxBD, "dst_index beyond DB capacity");
+  launch_copy_row(q.limbs.ptr, q.sum0.ptr, q.sum1.ptr, src_row, db.limbs.ptr, db.sum0.ptr,
+                  db.sum1.ptr, dst_index, code_length_, s);
+}
+
+std::unique_ptr<QuerySet> ShareDB::upload_query_device(const uint16_t* dev_rows, size_t n_rows,
+                                                       hipStream_t s) const {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(n_rows == query_length_, "query rows must equal the engine's query_length");
+  auto q = std::make_unique<QuerySet>();
+  q->code_length = code_length_;
+  q->rows = (int)n_rows;
+  const size_t padded = round_up(n_rows, kMaxBQ);
+  q->limbs.alloc(padded * (size_t)code_length_ * 2);
+  q->sum0.alloc(padded);
+  q->sum1.alloc(padded);
+  launch_tile_rows(dev_rows, n_rows, code_length_, q->limbs.ptr, 0, q->sum0.ptr, q->sum1.ptr, s);
+  return q;
+}
+
+std::unique_ptr<QuerySet> ShareDB::upload_query(const uint16_t* host_rows, size_t n_rows,
+                                                hipStream_t s) const {
+  DeviceGuard g(device_);
+  DevBuf<uint16_t> tmp(n_rows * (size_t)code_length_, false);
+  IRIS_HIP_CHECK(hipMemcpyAsync(tmp.ptr, host_rows, tmp.n * 2, hipMemcpyHostToDevice, s));
+  auto q = upload_query_device(tmp.ptr, n_rows, s);
+  IRIS_HIP_CHECK(hipStreamSynchronize(s));
+  return q;
+}
+
+std::unique_ptr<QuerySet> ShareDB::upload_query_shares(const uint16_t* host_shares, size_t batch,
+                                                       hipStream_t s) const {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(batch >= 1 && batch * kRotations <= query_length_, "batch exceeds query_length / 31");
+  const size_t K = (size_t)code_length_;
+  DevBuf<uint16_t> raw(batch * K, false), rows(query_length_ * K, false);
+  IRIS_HIP_CHECK(hipMemcpyAsync(raw.ptr, host_shares, raw.n * 2, hipMemcpyHostToDevice, s));
+  launch_query_prep(raw.ptr, (int)batch, code_length_, peer_id_, rows.ptr, (int)query_length_, s);
+  auto q = upload_query_device(rows.ptr, query_length_, s);
+  IRIS_HIP_CHECK(hipStreamSynchronize(s));
+  return q;
+}
+
+static GemmArgs gemm_args(const QuerySet& q, const DbSlice& db, int variant) {
+  GemmArgs a{};
+  const GemmTile t = gemm_tile(variant);
+  a.q = q.limbs.ptr;
+  a.db = db.limbs.ptr;
+  a.qsum0 = q.sum0.ptr;
+  a.qsum1 = q.sum1.ptr;
+  a.dsum0 = db.sum0.ptr;
+  a.dsum1 = db.sum1.ptr;
+  a.n_kt = db.code_length / kKTile;
+  a.qr = q.rows;
+  a.n_qtiles = (int)ceil_div((size_t)q.rows, (size_t)t.bq);
+  return a;
+}
+
+void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
+                         uint16_t multiplier, hipStream_t s) {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(q.code_length == code_length_ && db.code_length == code_length_, "code length mismatch");
+  IRIS_REQUIRE((size_t)q.rows == query_length_, "query batch length mismatch");
+  IRIS_REQUIRE(chunk > 0 && chunk % 64 == 0, "chunk size must be a positive multiple of 64");
+  IRIS_REQUIRE(chunk <= max_db_length_, "chunk larger than the engine's result buffer");
+  IRIS_REQUIRE(offset % 16 == 0, "offset must be a multiple of 16");
+  IRIS_REQUIRE(offset + chunk <= db.capacity - kMaxBD, "chunk beyond DB capacity");
+  GemmArgs a = gemm_args(q, db, variant);
+  const GemmTile t = gemm_tile(variant);
+  a.out = results_.ptr;
+  a.out_peer = nullptr;
+  a.out_chunk_stride = 0;
+  a.n_dtiles = (int)ceil_div(chunk, (size_t)t.bd);
+  a.db_begin = offset;
+  a.db_end = offset + chunk;
+  a.chunk = round_up(chunk, 256);
+  a.ctr_first = counter_;
+  a.ctr_per_chunk = 0;
+  a.nonce = nonce_;
+  a.key_own = seed_own_;
+  a.key_prev = seed_prev_;
+  a.mult = multiplier;
+  a.masked = is_remote() ? 1 : 0;
+  launch_share_gemm(a, variant, s);
+  // the reference fills the mask keystream (chunk*Qr*2 bytes) only when remote
+  if (is_remote()) counter_ += ceil_div(chunk * query_length_ * 2, 64);
+}
+
+void ShareDB::dot_reduce_range(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
+                               size_t chunk, uint16_t multiplier, uint16_t* out, hipStream_t s) {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(q.code_length == code_length_ && db.code_length == code_length_, "code length mismatch");
+  IRIS_REQUIRE(total > 0 && total % 64 == 0 && chunk % 256 == 0, "bad range geometry");
+  IRIS_REQUIRE(offset % 16 == 0 && offset + total <= db.capacity - kMaxBD, "range beyond DB");
+  GemmArgs a = gemm_args(q, db, variant);
+  const GemmTile t = gemm_tile(variant);
+  a.out = out;
+  a.out_chunk_stride = chunk * query_length_;
+  a.n_dtiles = (int)ceil_div(total, (size_t)t.bd);
+  a.db_begin = offset;
+  a.db_end = offset + total;
+  a.chunk = chunk;
+  a.ctr_first = counter_;
+  // per full chunk: mask keystream + reshare OTP keystream
+  a.ctr_per_chunk = 2 * ceil_div(chunk * query_length_ * 2, 64);
+  a.nonce = nonce_;
+  a.key_own = seed_own_;
+  a.key_prev = seed_prev_;
+  a.mult = multiplier;
+  a.masked = is_remote() ? 1 : 0;
+  launch_share_gemm(a, variant, s);
+  if (is_remote()) {
+    for (size_t c0 = 0; c0 < total; c0 += chunk)
+      counter_ += 2 * ceil_div(std::min(chunk, total - c0) * query_length_ * 2, 64);
+  }
+}
+
+namespace {
+struct ReshareArg {
+  ShareDB* self;
+  size_t chunk;
+  hipStream_t s;
+};
+}  // namespace
+
+struct ShareDBTrio {
+  static void reshare(ShareDB* const d[3], size_t chunk, hipStream_t const s[3]) {
+    const size_t bytes = chunk * d[0]->query_length_ * 2;
+    for (int p = 0; p < 3; p++) {
+      IRIS_REQUIRE(d[p]->peer_id_ == p, "trio members must be parties 0, 1, 2 in order");
+      IRIS_REQUIRE(d[p]->query_length_ == d[0]->query_length_ && chunk <= d[p]->max_db_length_,
+                   "reshare geometry mismatch");
+    }
+    Event sent[3];
+    for (int p = 0; p < 3; p++) IRIS_HIP_CHECK(hipEventRecord(sent[p].e, s[p]));
+    for (int p = 0; p < 3; p++) {
+      const int q = (p + 1) % 3;
+      IRIS_HIP_CHECK(hipStreamWaitEvent(s[q], sent[p].e, 0));
+      IRIS_HIP_CHECK(hipMemcpyAsync(d[q]->results_peer_.ptr, d[p]->results_.ptr, bytes,
+                                    hipMemcpyDeviceToDevice, s[q]));
+      IRIS_HIP_CHECK(hipEventRecord(d[p]->ev_send_done_.e, s[q]));
+    }
+    // the sender's stream must not overwrite `results` before its message left
+    for (int p = 0; p < 3; p++) IRIS_HIP_CHECK(hipStreamWaitEvent(s[p], d[p]->ev_send_done_.e, 0));
+    for (int p = 0; p < 3; p++)
+      if (d[p]->is_remote()) d[p]->counter_ += ceil_div(bytes, 64);  // OTP keystream
+  }
+};
+
+void ShareDB::reshare_results(size_t chunk, hipStream_t s) {
+  IRIS_REQUIRE(ex_ != nullptr, "reshare_results needs a remote engine (exchange handle)");
+  ReshareArg arg{this, chunk, s};
+  ex_->collective(peer_id_, op_id("reshare", tag_), &arg, [](void* const* args) {
+    ShareDB* d[3];
+    hipStream_t st[3];
+    size_t chunk = static_cast<ReshareArg*>(args[0])->chunk;
+    for (int p = 0; p < 3; p++) {
+      auto* a = static_cast<ReshareArg*>(args[p]);
+      IRIS_REQUIRE(a->chunk == chunk, "parties reshare different chunk sizes");
+      d[a->self->peer_id()] = a->self;
+      st[a->self->peer_id()] = a->s;
+    }
+    ShareDB::reshare_trio(d, chunk, st);
+  });
+}
+
+void ShareDB::reshare_trio(ShareDB* const d[3], size_t chunk, hipStream_t const s[3]) {
+  DeviceGuard g(d[0]->device_);
+  ShareDBTrio::reshare(d, chunk, s);
+}
+
+// ====================================================================== Circuits
+Circuits::Circuits(int device, int peer_id, size_t input_size, size_t alloc_size,
+                   const ChaChaKey& seed_own, const ChaChaKey& seed_prev, Exchange* ex, int tag)
+    : device_(device),
+      peer_id_(peer_id),
+      tag_(tag),
+      alloc_words_(alloc_size),
+      chunk_words_(input_size / 64),
+      seed_own_(seed_own),
+      seed_prev_(seed_prev),
+      ex_(ex) {
+  IRIS_REQUIRE(peer_id >= 0 && peer_id < 3, "peer_id must be 0..2");
+  IRIS_REQUIRE(input_size % 64 == 0, "input size must be a multiple of 64");
+  IRIS_REQUIRE(alloc_size >= chunk_words_, "alloc_size must cover the chunk");
+  IRIS_REQUIRE(ex != nullptr, "Circuits needs an exchange handle");
+  DeviceGuard g(device_);
+  const size_t A = alloc_words_ + 8;
+  tm_a_.alloc(16 * A); tm_b_.alloc(16 * A);
+  cl_a_.alloc(16 * A); cl_b_.alloc(16 * A);
+  c_a_.alloc(2 * A); c_b_.alloc(2 * A);
+  tmp_a_.alloc(2 * A); tmp_b_.alloc(2 * A);
+  tx_a_.alloc(32 * A); tx_b_.alloc(32 * A);
+  cm_a_.alloc(31 * A); cm_b_.alloc(31 * A);
+  res_a_.alloc(A); res_b_.alloc(A); open_c_.alloc(A);
+  msg_own_.alloc(128 * A); inbox_.alloc(128 * A);
+  corr_a_.alloc(128 * A); corr_b_.alloc(128 * A);
+}
+
+void Circuits::set_chunk_size(size_t words) {
+  IRIS_REQUIRE(words <= alloc_words_, "chunk size exceeds the allocation");
+  chunk_words_ = words;
+}
+
+CircuitBufs Circuits::bufs(const uint16_t* ca, const uint16_t* cb, const uint16_t* ma,
+                           const uint16_t* mb) const {
+  CircuitBufs b{};
+  b.code_a = ca; b.code_b = cb; b.mask_a = ma; b.mask_b = mb;
+  b.tm_a = tm_a_.ptr; b.tm_b = tm_b_.ptr; b.cl_a = cl_a_.ptr; b.cl_b = cl_b_.ptr;
+  b.c_a = c_a_.ptr; b.c_b = c_b_.ptr; b.tmp_a = tmp_a_.ptr; b.tmp_b = tmp_b_.ptr;
+  b.msg_own = msg_own_.ptr; b.inbox = inbox_.ptr; b.corr_a = corr_a_.ptr; b.corr_b = corr_b_.ptr;
+  b.tx_a = tx_a_.ptr; b.tx_b = tx_b_.ptr; b.cm_a = cm_a_.ptr; b.cm_b = cm_b_.ptr;
+  b.res_a = res_a_.ptr; b.res_b = res_b_.ptr;
+  b.lifted_a = lifted_dbg_a; b.lifted_b = lifted_dbg_b;
+  b.key_own = seed_own_; b.key_prev = seed_prev_;
+  return b;
+}
+
+void Circuits::compare_trio(Circuits* const c[3], const uint16_t* const code_a[3],
+                            const uint16_t* const code_b[3], const uint16_t* const mask_a[3],
+                            const uint16_t* const mask_b[3], size_t len, uint32_t A,
+                            hipStream_t const s[3]) {
+  IRIS_REQUIRE(len % 64 == 0, "input length must be a multiple of 64");
+  const size_t W = len / 64;
+  IRIS_REQUIRE(W % 8 == 0, "elements per call must be a multiple of 512 "
+                           "(query_length * chunk / 64 divisible by 8)");
+  Exchange* ex = c[0]->ex_;
+  DeviceGuard g(ex->device());
+  Trio t{};
+  for (int p = 0; p < 3; p++) {
+    IRIS_REQUIRE(c[p]->peer_id_ == p, "trio members must be parties 0, 1, 2 in order");
+    IRIS_REQUIRE(c[p]->ex_ == ex, "trio members must share one exchange");
+    IRIS_REQUIRE(W <= c[p]->alloc_words_, "input larger than the Circuits allocation");
+    IRIS_REQUIRE(c[p]->counter_ == c[0]->counter_ && c[p]->alloc_words_ == c[0]->alloc_words_,
+                 "party RNG counters or allocations diverged");
+    t.p[p] = c[p]->bufs(code_a[p], code_b[p], mask_a[p], mask_b[p]);
+  }
+  t.W = W;
+  t.nonce = c[0]->nonce_;
+  t.party0 = 0;
+  const CircuitSchedule sc = CircuitSchedule::make(c[0]->counter_, W, c[0]->alloc_words_);
+  hipStream_t ps = ex->protocol_stream();
+  ex->join_streams(s);
+  launch_transpose16(t, 3, ps);
+  launch_csa(t, 3, false, sc.lift_csa, ps);
+  launch_ripple(t, 3, false, RippleMode::First, 0, sc.lift_first, ps);
+  for (int k = 1; k <= 14; k++) launch_ripple(t, 3, false, RippleMode::Round, k, sc.lift_round[k], ps);
+  launch_ripple(t, 3, false, RippleMode::LiftFinal, 15, sc.lift_final, ps);
+  for (int step = 1; step <= 3; step++) launch_inject(t, 3, step, sc.inject, ps);
+  launch_lms_transpose32(t, 3, A, ps);
+  launch_csa(t, 3, true, sc.msb_csa, ps);
+  launch_ripple(t, 3, true, RippleMode::First, 0, sc.msb_first, ps);
+  for (int k = 1; k <= 29; k++) launch_ripple(t, 3, true, RippleMode::Round, k, sc.msb_round[k], ps);
+  launch_ripple(t, 3, true, RippleMode::MsbFinal, 30, 0, ps);
+  ex->fork_streams(s);
+  for (int p = 0; p < 3; p++) c[p]->counter_ = sc.end;
+}
+
+namespace {
+struct CompareArg {
+  Circuits* self;
+  const uint16_t *ca, *cb, *ma, *mb;
+  size_t len;
+  uint32_t A;
+  hipStream_t s;
+};
+struct OpenArg {
+  Circuits* self;
+  hipStream_t s;
+};
+}  // namespace
+
+void Circuits::compare_threshold_masked_many(const uint16_t* code_a, const uint16_t* code_b,
+                                             const uint16_t* mask_a, const uint16_t* mask_b,
+                                             size_t len, uint32_t A, hipStream_t s) {
+  CompareArg arg{this, code_a, code_b, mask_a, mask_b, len, A, s};
+  ex_->collective(peer_id_, op_id("compare_threshold", tag_), &arg, [](void* const* args) {
+    Circuits* c[3];
+    const uint16_t *ca[3], *cb[3], *ma[3], *mb[3];
+    hipStream_t st[3];
+    auto* a0 = static_cast<CompareArg*>(args[0]);
+    for (int i = 0; i < 3; i++) {
+      auto* a = static_cast<CompareArg*>(args[i]);
+      IRIS_REQUIRE(a->len == a0->len && a->A == a0->A, "parties disagree on length/threshold");
+      const int p = a->self->peer_id();
+      c[p] = a->self; ca[p] = a->ca; cb[p] = a->cb; ma[p] = a->ma; mb[p] = a->mb; st[p] = a->s;
+    }
+    compare_trio(c, ca, cb, ma, mb, a0->len, a0->A, st);
+  });
+}
+
+void Circuits::open_exchange_trio(Circuits* const c[3], hipStream_t const s[3]) {
+  Exchange* ex = c[0]->ex_;
+  DeviceGuard g(ex->device());
+  hipStream_t ps = ex->protocol_stream();
+  ex->join_streams(s);
+  for (int p = 0; p < 3; p++) {
+    const int prev = (p + 2) % 3;
+    IRIS_HIP_CHECK(hipMemcpyAsync(c[p]->open_c_.ptr, c[prev]->res_b_.ptr,
+                                  c[p]->chunk_words_ * 8, hipMemcpyDeviceToDevice, ps));
+  }
+  ex->fork_streams(s);
+}
+
+void Circuits::open_exchange(hipStream_t s) {
+  OpenArg arg{this, s};
+  ex_->collective(peer_id_, op_id("open", tag_), &arg, [](void* const* args) {
+    Circuits* c[3];
+    hipStream_t st[3];
+    for (int i = 0; i < 3; i++) {
+      auto* a = static_cast<OpenArg*>(args[i]);
+      c[a->self->peer_id()] = a->self;
+      st[a->self->peer_id()] = a->s;
+    }
+    open_exchange_trio(c, st);
+  });
+}
+
+// ====================================================================== DistanceComparator
+DistanceComparator::DistanceComparator(int device, size_t query_length, size_t max_db_size)
+    : device_(device), query_length_(query_length), max_db_size_(max_db_size) {
+  IRIS_REQUIRE(query_length % kRotations == 0, "query_length must be batch * 31");
+  DeviceGuard g(device_);
+  const size_t B = query_length / kRotations;
+  match_counters_.alloc(B);
+  all_matches_.alloc(kAllMatchesLen * B);
+  mcl_.alloc(B);
+  mcr_.alloc(B);
+  prl_.alloc(kAllMatchesLen * B);
+  prr_.alloc(kAllMatchesLen * B);
+}
+
+std::unique_ptr<DevBuf<uint64_t>> DistanceComparator::prepare_db_match_list(size_t db_size) const {
+  DeviceGuard g(device_);
+  return std::make_unique<DevBuf<uint64_t>>(
+      ceil_div(db_size * (query_length_ / kRotations), 64) + 1);
+}
+
+void DistanceComparator::open_results(const uint64_t* a, const uint64_t* b, const uint64_t* c,
+                                      uint64_t* bitmap, size_t chunk_len, size_t real_chunk_len,
+                                      size_t offset, size_t total_db_len, size_t batch_size,
+                                      hipStream_t s) const {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(chunk_len % 64 == 0, "chunk length must be a multiple of 64");
+  IRIS_REQUIRE(batch_size * kRotations <= query_length_, "batch larger than query_length");
+  OpenArgs o{};
+  o.a = a; o.b = b; o.c = c; o.bitmap = bitmap;
+  o.n_words = chunk_len * query_length_ / 64;
+  o.chunk_len = chunk_len;
+  o.query_len = batch_size * kRotations;
+  o.offset = offset;
+  o.real_chunk_len = real_chunk_len;
+  o.total_db_len = total_db_len;
+  launch_open(o, s);
+}
+
+void DistanceComparator::join_db_matches(const uint64_t* left, const uint64_t* right,
+                                         uint32_t* final_results, size_t db_size,
+                                         const uint64_t* or_policy, size_t max_db_size,
+                                         hipStream_t s) {
+  DeviceGuard g(device_);
+  MergeArgs m{};
+  m.left = left; m.right = right; m.or_policy = or_policy; m.final_results = final_results;
+  m.match_counters = match_counters_.ptr; m.all_matches = all_matches_.ptr;
+  m.mcl = mcl_.ptr; m.mcr = mcr_.ptr; m.prl = prl_.ptr; m.prr = prr_.ptr;
+  m.query_len = query_length_ / kRotations;
+  m.db_len = db_size;
+  m.max_db_len = max_db_size;
+  launch_merge(m, s);
+}
+
+void DistanceComparator::reset(hipStream_t s) {
+  DeviceGuard g(device_);
+  const size_t B = query_length_ / kRotations;
+  fill_u32(match_counters_.ptr, B, 0, s);
+  fill_u32(mcl_.ptr, B, 0, s);
+  fill_u32(mcr_.ptr, B, 0, s);
+}
+
+}  // namespace irismpc

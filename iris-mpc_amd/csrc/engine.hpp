@@ -1,0 +1,228 @@
+// Host-side engines mirroring the reference GPU server's matching path:
+// ShareDB (iris-mpc-gpu/src/dot/share_db.rs), Circuits
+// (threshold_ring/protocol.rs), DistanceComparator (dot/distance_comparator.rs)
+// and the NCCL communicator replaced by an in-process three-party Exchange.
+#pragma once
+#include <condition_variable>
+#include <exception>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "circuits.hpp"
+#include "common.hpp"
+#include "share_gemm.hpp"
+
+namespace irismpc {
+
+// RAII device allocation.
+template <typename T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count, bool zero = true) { alloc(count, zero); }
+  void alloc(size_t count, bool zero = true) {
+    release();
+    n = count;
+    if (count) {
+      IRIS_HIP_CHECK(hipMalloc(&ptr, count * sizeof(T) + 256));
+      if (zero) IRIS_HIP_CHECK(hipMemset(ptr, 0, count * sizeof(T) + 256));
+    }
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { IRIS_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
+};
+
+// The link between the three parties on one GPU (replaces NcclComm,
+// helpers/comm.rs). Party p sends to next = p+1 and receives from prev = p-1.
+// Operations that the reference performs as matched send/recv pairs are
+// collectives here: every party calls with its own arguments and stream; the
+// last arrival performs the device-to-device exchange for all three.
+class Exchange {
+ public:
+  explicit Exchange(int device, double timeout_s = 300.0);
+  ~Exchange();
+  int device() const { return device_; }
+  hipStream_t protocol_stream() const { return protocol_; }
+  // Rendezvous: blocks until all three parties called with the same op id.
+  void collective(int party, uint64_t op, void* arg, const std::function<void(void* const*)>& fn);
+  // Orders `protocol` after the three party streams and the party streams
+  // after `protocol` (the exchange barrier of a joint launch).
+  void join_streams(hipStream_t const s[3]);
+  void fork_streams(hipStream_t const s[3]);
+
+ private:
+  int device_;
+  double timeout_s_;
+  hipStream_t protocol_ = nullptr;
+  Event ev_party_[3], ev_proto_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  void* args_[3] = {nullptr, nullptr, nullptr};
+  uint64_t ops_[3] = {0, 0, 0};
+  int arrived_ = 0;
+  uint64_t generation_ = 0;
+  std::exception_ptr error_;
+  uint64_t error_gen_ = ~0ull;
+};
+
+// SlicedProcessedDatabase analogue: one matrix (code or mask) of one party,
+// resident in HBM in MFMA fragment order, plus per-row limb sums.
+class ShareDB;
+struct DbSlice {
+  int code_length = 0;
+  size_t capacity = 0;   // rows addressable (padded)
+  DevBuf<int8_t> limbs;  // tiled i8 limbs
+  DevBuf<uint32_t> sum0, sum1;
+  DevBuf<uint16_t> staging;  // host upload staging
+};
+
+// CudaVec2DSlicerU8 + sums analogue for a query batch.
+struct QuerySet {
+  int code_length = 0;
+  int rows = 0;      // query_length rows
+  DevBuf<int8_t> limbs;
+  DevBuf<uint32_t> sum0, sum1;
+};
+
+class ShareDB {
+ public:
+  ShareDB(int device, int peer_id, size_t max_db_length, size_t query_length, int code_length,
+          const ChaChaKey& seed_own, const ChaChaKey& seed_prev, Exchange* ex, int tag);
+  ~ShareDB();
+  std::unique_ptr<DbSlice> alloc_db(size_t max_db_length) const;
+  // load_single_record_from_db for n consecutive rows (host u16 -> HBM)
+  void load_records_u16(DbSlice& db, size_t index, size_t n, const uint16_t* host, hipStream_t s) const;
+  // load_single_record_from_s3 (host even/odd limbs -> HBM)
+  void load_records_limbs(DbSlice& db, size_t index, size_t n, const uint8_t* l0, const uint8_t* l1,
+                          hipStream_t s) const;
+  void write_at_index(DbSlice& db, const QuerySet& q, size_t src_index, size_t dst_index,
+                      hipStream_t s) const;
+  // preprocess_query + htod + query_sums: rows x code_length u16 (already rotated
+  // and Lagrange-preprocessed, like the reference's CompactQuery).
+  std::unique_ptr<QuerySet> upload_query(const uint16_t* host_rows, size_t n_rows, hipStream_t s) const;
+  std::unique_ptr<QuerySet> upload_query_device(const uint16_t* dev_rows, size_t n_rows, hipStream_t s) const;
+  // Raw (not preprocessed, not rotated) query shares of this party: batch x
+  // code_length u16 on the host. Lagrange factor, basis B and the 31 rotations
+  // are applied on the GPU; rows are padded to query_length.
+  std::unique_ptr<QuerySet> upload_query_shares(const uint16_t* host_shares, size_t batch,
+                                                hipStream_t s) const;
+  // dot + dot_reduce_and_multiply (fused): DB rows [offset, offset + chunk)
+  // against all query rows; chunk is the padded chunk size (multiple of 64).
+  void dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
+                  uint16_t multiplier, hipStream_t s);
+  // Multi-chunk single launch: rows [offset, offset + total) in chunks of
+  // `chunk` rows, results written chunk after chunk into `out`.
+  void dot_reduce_range(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
+                        size_t chunk, uint16_t multiplier, uint16_t* out, hipStream_t s);
+  void reshare_results(size_t chunk, hipStream_t s);  // collective
+  static void reshare_trio(ShareDB* const d[3], size_t chunk, hipStream_t const s[3]);
+  uint16_t* results() const { return results_.ptr; }
+  uint16_t* results_peer() const { return results_peer_.ptr; }
+  size_t results_capacity() const { return results_.n; }
+  int peer_id() const { return peer_id_; }
+  int code_length() const { return code_length_; }
+  size_t query_length() const { return query_length_; }
+  uint64_t counter() const { return counter_; }
+  void set_counter(uint64_t c) { counter_ = c; }
+  bool is_remote() const { return ex_ != nullptr; }
+  int variant = 0;  // GEMM tile variant
+
+ private:
+  int device_, peer_id_, code_length_, tag_;
+  size_t query_length_, max_db_length_;
+  ChaChaKey seed_own_, seed_prev_;
+  uint64_t nonce_ = 0, counter_ = 0;
+  Exchange* ex_;
+  DevBuf<uint16_t> results_, results_peer_;
+  Event ev_send_done_;
+  friend struct ShareDBTrio;
+};
+
+class Circuits {
+ public:
+  Circuits(int device, int peer_id, size_t input_size, size_t alloc_size,
+           const ChaChaKey& seed_own, const ChaChaKey& seed_prev, Exchange* ex, int tag);
+  void set_chunk_size(size_t words);
+  size_t chunk_size() const { return chunk_words_; }
+  // compare_threshold_masked_many: collective over the three parties.
+  void compare_threshold_masked_many(const uint16_t* code_a, const uint16_t* code_b,
+                                     const uint16_t* mask_a, const uint16_t* mask_b, size_t len,
+                                     uint32_t threshold_a, hipStream_t s);
+  // Runs the three parties' compare in one call (single host thread).
+  static void compare_trio(Circuits* const c[3], const uint16_t* const code_a[3],
+                           const uint16_t* const code_b[3], const uint16_t* const mask_a[3],
+                           const uint16_t* const mask_b[3], size_t len, uint32_t threshold_a,
+                           hipStream_t const s[3]);
+  // open exchange (actor.rs:2928-2997): own result b -> next, prev's b -> c.
+  void open_exchange(hipStream_t s);  // collective
+  static void open_exchange_trio(Circuits* const c[3], hipStream_t const s[3]);
+  const uint64_t* result_a() const { return res_a_.ptr; }
+  const uint64_t* result_b() const { return res_b_.ptr; }
+  const uint64_t* result_c() const { return open_c_.ptr; }
+  uint64_t counter() const { return counter_; }
+  void set_counter(uint64_t c) { counter_ = c; }
+  int peer_id() const { return peer_id_; }
+  uint32_t* lifted_dbg_a = nullptr;  // optional debug capture of the lifted mask
+  uint32_t* lifted_dbg_b = nullptr;
+
+ private:
+  CircuitBufs bufs(const uint16_t* ca, const uint16_t* cb, const uint16_t* ma,
+                   const uint16_t* mb) const;
+  int device_, peer_id_, tag_;
+  size_t alloc_words_, chunk_words_;
+  ChaChaKey seed_own_, seed_prev_;
+  uint64_t nonce_ = 0, counter_ = 0;
+  Exchange* ex_;
+  DevBuf<uint64_t> tm_a_, tm_b_, cl_a_, cl_b_, c_a_, c_b_, tmp_a_, tmp_b_, tx_a_, tx_b_, cm_a_,
+      cm_b_, res_a_, res_b_, open_c_;
+  DevBuf<uint16_t> msg_own_, inbox_, corr_a_, corr_b_;
+};
+
+// DistanceComparator analogue (dot/distance_comparator.rs).
+class DistanceComparator {
+ public:
+  DistanceComparator(int device, size_t query_length, size_t max_db_size);
+  // Bitmap of size max_db * batch bits (prepare_db_match_list).
+  std::unique_ptr<DevBuf<uint64_t>> prepare_db_match_list(size_t db_size) const;
+  void open_results(const uint64_t* a, const uint64_t* b, const uint64_t* c, uint64_t* bitmap,
+                    size_t chunk_len, size_t real_chunk_len, size_t offset, size_t total_db_len,
+                    size_t batch_size, hipStream_t s) const;
+  // join_db_matches (AND policy) and join_db_matches_with_bitmaps (OR policy).
+  void join_db_matches(const uint64_t* left, const uint64_t* right, uint32_t* final_results,
+                       size_t db_size, const uint64_t* or_policy, size_t max_db_size,
+                       hipStream_t s);
+  uint32_t* match_counters() const { return match_counters_.ptr; }
+  uint32_t* all_matches() const { return all_matches_.ptr; }
+  uint32_t* match_counters_left() const { return mcl_.ptr; }
+  uint32_t* match_counters_right() const { return mcr_.ptr; }
+  uint32_t* partial_results_left() const { return prl_.ptr; }
+  uint32_t* partial_results_right() const { return prr_.ptr; }
+  size_t query_length() const { return query_length_; }
+  void reset(hipStream_t s);
+
+ private:
+  int device_;
+  size_t query_length_, max_db_size_;
+  DevBuf<uint32_t> match_counters_, all_matches_, mcl_, mcr_, prl_, prr_;
+};
+
+}  // namespace irismpc

@@ -1,0 +1,70 @@
+// Share GEMM: exact Z_2^16 dot products of u16 shares via int8 MFMA.
+#pragma once
+#include "common.hpp"
+
+namespace irismpc {
+
+// Arguments of the fused GEMM + correction + zero-share-mask kernel.
+// Computes, for query rows q < qr and DB rows r in [db_begin, db_end):
+//   c = A0.B0 + 256 (A0.B1 + A1.B0)             (share_db.rs:800-845)
+//   v = u16(c + (s0 << 7) + ((s0 + s1) << 15))  (dot/kernel.cu:16-28)
+//   out = v * mult + F(own)[idx] - F(prev)[idx]  (masks only if masked != 0)
+// with the reference's chunk geometry, relative to db_begin: rows are grouped
+// into chunks of `chunk` rows, chunk c has width
+// w_c = min(chunk, (db_end - db_begin) - c*chunk) and the result element index
+// inside the chunk is idx = q * w_c + (r - db_begin - c*chunk).
+struct GemmArgs {
+  const int8_t* q;      // tiled query limbs (rows padded to the q tile)
+  const int8_t* db;     // tiled DB limbs
+  const uint32_t* qsum0;
+  const uint32_t* qsum1;
+  const uint32_t* dsum0;
+  const uint32_t* dsum1;
+  uint16_t* out;        // results, chunk c at out + c*out_chunk_stride
+  uint16_t* out_peer;   // optional copy written straight into the next party's inbox
+  size_t out_chunk_stride;
+  int n_kt;             // K / 64
+  int qr;               // query rows produced (query_length)
+  int n_qtiles;
+  int n_dtiles;
+  size_t db_begin;      // first DB row (multiple of 16)
+  size_t db_end;        // padded end (db_end - db_begin multiple of 64)
+  size_t chunk;         // rows per chunk (multiple of 256)
+  uint64_t ctr_first;   // ChaCha block counter of chunk 0's mask fill
+  uint64_t ctr_per_chunk;
+  uint64_t nonce;
+  ChaChaKey key_own, key_prev;
+  uint32_t mult;
+  int masked;
+};
+
+// Tiles u16 rows [n_rows][K] into the fragment layout at row offset row0 and
+// writes per-row limb sums (share_db.rs:45-59,265-296).
+void launch_tile_rows(const uint16_t* src, size_t n_rows, int K, int8_t* dst, size_t row0,
+                      uint32_t* sum0, uint32_t* sum1, hipStream_t s);
+// Same from split limbs (S3 even/odd format, share_db.rs:657-684).
+void launch_tile_limbs(const uint8_t* l0, const uint8_t* l1, size_t n_rows, int K, int8_t* dst,
+                       size_t row0, uint32_t* sum0, uint32_t* sum1, hipStream_t s);
+// Copies tiled row `src_row` of `src` to row `dst_row` of `dst` with its sums
+// (write_at_index, share_db.rs:187-240).
+void launch_copy_row(const int8_t* src, const uint32_t* src_s0, const uint32_t* src_s1,
+                     size_t src_row, int8_t* dst, uint32_t* dst_s0, uint32_t* dst_s1,
+                     size_t dst_row, int K, hipStream_t s);
+// Synthetic uniformly random shares (bench data; rows rounded up to 16).
+void launch_fill_random(int8_t* dst, size_t row0, size_t n_rows, int K, const ChaChaKey& key,
+                        uint64_t nonce, uint32_t* sum0, uint32_t* sum1, hipStream_t s);
+// Untiles rows back to u16 (tests / fetch).
+void launch_untile_rows(const int8_t* src, size_t row0, size_t n_rows, int K, uint16_t* dst,
+                        hipStream_t s);
+
+// Block tile (query rows x DB rows) of each compiled GEMM variant.
+struct GemmTile {
+  int bq, bd;
+};
+constexpr int kGemmVariants = 4;
+constexpr int kMaxBQ = 128;   // query buffers are padded to this many rows
+constexpr int kMaxBD = 256;   // DB buffers are padded to this many rows
+GemmTile gemm_tile(int variant);
+void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s);
+
+}  // namespace irismpc

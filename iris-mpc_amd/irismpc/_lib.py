@@ -1,0 +1,117 @@
+"""ctypes binding of libirismpc.so (the C ABI declared in include/irismpc.h).
+
+There is no fallback: if the native library is missing or cannot be loaded,
+every entry point raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libirismpc.so")
+
+
+class IrisMpcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+vp = ctypes.c_void_p
+sz = ctypes.c_size_t
+u16 = ctypes.c_uint16
+u32 = ctypes.c_uint32
+u64 = ctypes.c_uint64
+i32 = ctypes.c_int
+dbl = ctypes.c_double
+PP = ctypes.POINTER(ctypes.c_void_p)
+
+# name -> argtypes (all return int status)
+SIGNATURES = {
+    "irismpc_version": [],
+    "irismpc_device_count": [vp],
+    "irismpc_stream_create": [i32, vp],
+    "irismpc_stream_destroy": [vp],
+    "irismpc_stream_synchronize": [vp],
+    "irismpc_malloc": [i32, sz, vp],
+    "irismpc_free": [vp],
+    "irismpc_memcpy_htod": [vp, vp, sz, vp],
+    "irismpc_memcpy_dtoh": [vp, vp, sz, vp],
+    "irismpc_fill_u32": [vp, sz, u32, vp],
+    "irismpc_fill_u64": [vp, sz, u64, vp],
+    "irismpc_derive_seed": [vp, u64, vp],
+    "irismpc_exchange_create": [i32, dbl, vp],
+    "irismpc_exchange_destroy": [vp],
+    "irismpc_share_db_init": [i32, i32, sz, sz, sz, vp, vp, vp, i32, vp],
+    "irismpc_share_db_destroy": [vp],
+    "irismpc_share_db_alloc_db": [vp, sz, vp],
+    "irismpc_db_destroy": [vp],
+    "irismpc_db_load_records": [vp, vp, sz, sz, vp, vp],
+    "irismpc_db_load_records_limbs": [vp, vp, sz, sz, vp, vp, vp],
+    "irismpc_db_write_at_index": [vp, vp, vp, sz, sz, vp],
+    "irismpc_db_read_records": [vp, vp, sz, sz, vp, vp],
+    "irismpc_db_fill_random": [vp, vp, sz, sz, vp, u64, vp],
+    "irismpc_query_upload": [vp, vp, sz, vp, vp],
+    "irismpc_query_upload_shares": [vp, vp, sz, vp, vp],
+    "irismpc_query_destroy": [vp],
+    "irismpc_scan_db": [vp, vp, vp, vp, vp, vp, vp, vp, sz, sz, u32, vp, vp],
+    "irismpc_scan_db_trio": [vp, vp, vp, vp, vp, vp, vp, vp, sz, sz, u32, vp, vp],
+    "irismpc_share_db_dot_reduce": [vp, vp, vp, sz, sz, u16, vp],
+    "irismpc_share_db_dot_reduce_range": [vp, vp, vp, sz, sz, sz, u16, vp, vp],
+    "irismpc_share_db_reshare_results": [vp, sz, vp],
+    "irismpc_share_db_reshare_results_trio": [vp, sz, vp],
+    "irismpc_share_db_result_views": [vp, vp, vp],
+    "irismpc_share_db_fetch_results": [vp, vp, sz, vp],
+    "irismpc_share_db_get_counter": [vp, vp],
+    "irismpc_share_db_set_counter": [vp, u64],
+    "irismpc_share_db_set_variant": [vp, i32],
+    "irismpc_gemm_variants": [vp],
+    "irismpc_circuits_new": [i32, i32, sz, sz, vp, vp, vp, i32, vp],
+    "irismpc_circuits_destroy": [vp],
+    "irismpc_circuits_set_chunk_size": [vp, sz],
+    "irismpc_circuits_compare_threshold_masked_many": [vp, vp, vp, vp, vp, sz, u32, vp],
+    "irismpc_circuits_compare_threshold_masked_many_trio": [vp, vp, vp, vp, vp, sz, u32, vp],
+    "irismpc_circuits_result_buffer": [vp, vp, vp],
+    "irismpc_circuits_open": [vp, vp],
+    "irismpc_circuits_open_trio": [vp, vp],
+    "irismpc_circuits_open_views": [vp, vp, vp, vp],
+    "irismpc_circuits_get_counter": [vp, vp],
+    "irismpc_circuits_set_counter": [vp, u64],
+    "irismpc_circuits_set_lift_capture": [vp, vp, vp],
+    "irismpc_comparator_init": [i32, sz, sz, vp],
+    "irismpc_comparator_destroy": [vp],
+    "irismpc_comparator_open_results": [vp, vp, vp, vp, vp, sz, sz, sz, sz, sz, vp],
+    "irismpc_comparator_join_db_matches": [vp, vp, vp, vp, sz, vp, sz, vp],
+    "irismpc_comparator_reset": [vp, vp],
+    "irismpc_comparator_buffers": [vp, vp, vp, vp, vp, vp, vp],
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise IrisMpcError(
+                f"native library {LIB_PATH} is missing: run __graft_entry__.build() "
+                "(make -C iris-mpc_amd); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        L.irismpc_last_error.restype = ctypes.c_char_p
+        L.irismpc_last_error.argtypes = []
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = i32
+        _lib = L
+    return _lib
+
+
+def call(name, *args):
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        msg = lib().irismpc_last_error().decode(errors="replace")
+        raise IrisMpcError(f"{name}: {msg}")
+    return rc
+
+
+def ptr_array(ptrs):
+    """C array of 3 (or n) void pointers."""
+    return (ctypes.c_void_p * len(ptrs))(*[p if isinstance(p, int) else (p.value if p else None) for p in ptrs])

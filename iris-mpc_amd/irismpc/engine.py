@@ -1,0 +1,346 @@
+"""Python host-side mirror of the reference engines, over the C ABI.
+
+Names and argument meanings follow iris-mpc-gpu:
+  ShareDB            src/dot/share_db.rs:448-1070
+  Circuits           src/threshold_ring/protocol.rs:433-2611
+  DistanceComparator src/dot/distance_comparator.rs:28-894
+Device buffers are passed as integer device pointers (e.g. torch tensors'
+data_ptr()); streams as integer hipStream_t handles.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import call, ptr_array
+
+IRIS_CODE_LENGTH = 12800
+MASK_CODE_LENGTH = 6400
+ROTATIONS = 31
+DB_CHUNK_SIZE = 1 << 15
+# dot/mod.rs:10-15: A = floor((1 - 2 t) * 2^16)
+THRESHOLD_A = int((1.0 - 2.0 * 0.345) * (1 << 16))
+THRESHOLD_ANON_STATS_A = int((1.0 - 2.0 * 0.375) * (1 << 16))
+
+
+def _out():
+    return ctypes.c_void_p()
+
+
+def _u32x8(seed):
+    a = np.ascontiguousarray(np.asarray(seed, dtype=np.uint32).reshape(8))
+    return a, a.ctypes.data
+
+
+def derive_seed(seed, nonce):
+    """server/actor.rs:2914-2925."""
+    s, sp = _u32x8(seed)
+    out = np.zeros(8, np.uint32)
+    call("irismpc_derive_seed", sp, nonce, out.ctypes.data)
+    return out
+
+
+def stream_create(device=0):
+    s = _out()
+    call("irismpc_stream_create", device, ctypes.byref(s))
+    return s.value
+
+
+def stream_synchronize(stream):
+    call("irismpc_stream_synchronize", stream)
+
+
+def download(ptr, n, dtype, stream=0):
+    """Copies n elements of dtype from a device pointer to a new numpy array."""
+    out = np.zeros(n, dtype)
+    call("irismpc_memcpy_dtoh", out.ctypes.data, ptr, out.nbytes, stream)
+    return out
+
+
+def upload(ptr, arr, stream=0):
+    a = np.ascontiguousarray(arr)
+    call("irismpc_memcpy_htod", ptr, a.ctypes.data, a.nbytes, stream)
+
+
+class Exchange:
+    """Replaces the three parties' NcclComm on one GPU (helpers/comm.rs)."""
+
+    def __init__(self, device=0, timeout_s=300.0):
+        h = _out()
+        call("irismpc_exchange_create", device, timeout_s, ctypes.byref(h))
+        self.h = h.value
+        self.device = device
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            call("irismpc_exchange_destroy", self.h)
+            self.h = None
+
+
+class SlicedProcessedDatabase:
+    """HBM-resident DB share of one matrix (share_db.rs:114-184)."""
+
+    def __init__(self, engine, h):
+        self.engine = engine
+        self.h = h
+
+    def load_records(self, index, records, stream=0):
+        r = np.ascontiguousarray(records, dtype=np.uint16)
+        n = r.size // self.engine.code_length
+        assert r.size == n * self.engine.code_length
+        call("irismpc_db_load_records", self.engine.h, self.h, index, n, r.ctypes.data, stream)
+
+    def load_single_record_from_db(self, index, record, stream=0):
+        self.load_records(index, record, stream)
+
+    def load_single_record_from_s3(self, index, limb0, limb1, stream=0):
+        l0 = np.ascontiguousarray(limb0, dtype=np.uint8)
+        l1 = np.ascontiguousarray(limb1, dtype=np.uint8)
+        n = l0.size // self.engine.code_length
+        call("irismpc_db_load_records_limbs", self.engine.h, self.h, index, n, l0.ctypes.data,
+             l1.ctypes.data, stream)
+
+    def load_full_db(self, records, stream=0):
+        self.load_records(0, records, stream)
+        return records.size // self.engine.code_length
+
+    def write_at_index(self, query, src_index, dst_index, stream=0):
+        call("irismpc_db_write_at_index", self.engine.h, self.h, query.h, src_index, dst_index, stream)
+
+    def fill_random(self, index, n, key, nonce=0, stream=0):
+        k, kp = _u32x8(key)
+        call("irismpc_db_fill_random", self.engine.h, self.h, index, n, kp, nonce, stream)
+
+    def read_records(self, index, n, stream=0):
+        out = np.zeros((n, self.engine.code_length), np.uint16)
+        call("irismpc_db_read_records", self.engine.h, self.h, index, n, out.ctypes.data, stream)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            call("irismpc_db_destroy", self.h)
+            self.h = None
+
+
+class DeviceQuery:
+    """Query limbs + sums on the device (CudaVec2DSlicerU8 + query_sums)."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            call("irismpc_query_destroy", self.h)
+            self.h = None
+
+
+class ShareDB:
+    """share_db.rs:448-1070. comm=None builds a non-remote engine."""
+
+    def __init__(self, peer_id, max_db_length, query_length, code_length, chacha_seeds,
+                 comm=None, tag=0, device=0):
+        so, sop = _u32x8(chacha_seeds[0])
+        sp, spp = _u32x8(chacha_seeds[1])
+        h = _out()
+        call("irismpc_share_db_init", device, peer_id, max_db_length, query_length, code_length,
+             sop, spp, comm.h if comm else None, tag, ctypes.byref(h))
+        self.h = h.value
+        self.peer_id = peer_id
+        self.max_db_length = max_db_length
+        self.query_length = query_length
+        self.code_length = code_length
+        self.comm = comm
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            call("irismpc_share_db_destroy", self.h)
+            self.h = None
+
+    def alloc_db(self, max_db_length):
+        h = _out()
+        call("irismpc_share_db_alloc_db", self.h, max_db_length, ctypes.byref(h))
+        return SlicedProcessedDatabase(self, h.value)
+
+    def upload_query(self, rows, stream=0):
+        """preprocess_query + htod + query_sums: rows = query_length x code_length u16."""
+        r = np.ascontiguousarray(rows, dtype=np.uint16)
+        n = r.size // self.code_length
+        h = _out()
+        call("irismpc_query_upload", self.h, r.ctypes.data, n, stream, ctypes.byref(h))
+        return DeviceQuery(h.value)
+
+    def upload_query_shares(self, shares, stream=0):
+        """Raw query shares (batch x code_length u16) of this party; Lagrange
+        preprocessing and the 31 rotations run on the GPU."""
+        r = np.ascontiguousarray(shares, dtype=np.uint16).reshape(-1, self.code_length)
+        h = _out()
+        call("irismpc_query_upload_shares", self.h, r.ctypes.data, r.shape[0], stream, ctypes.byref(h))
+        return DeviceQuery(h.value)
+
+    def dot_reduce(self, query, db, chunk_size, offset, multiplier=1, stream=0):
+        """dot + dot_reduce_and_multiply (fused)."""
+        call("irismpc_share_db_dot_reduce", self.h, query.h, db.h, chunk_size, offset, multiplier, stream)
+
+    def dot_reduce_range(self, query, db, offset, total, chunk_size, multiplier, out_ptr, stream=0):
+        call("irismpc_share_db_dot_reduce_range", self.h, query.h, db.h, offset, total, chunk_size,
+             multiplier, out_ptr, stream)
+
+    def reshare_results(self, chunk_size, stream=0):
+        call("irismpc_share_db_reshare_results", self.h, chunk_size, stream)
+
+    @staticmethod
+    def reshare_results_trio(engines, chunk_size, streams):
+        call("irismpc_share_db_reshare_results_trio", ptr_array([e.h for e in engines]), chunk_size,
+             ptr_array(list(streams)))
+
+    def result_chunk_shares(self):
+        a, b = _out(), _out()
+        call("irismpc_share_db_result_views", self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def fetch_results(self, n, stream=0):
+        out = np.zeros(n, np.uint16)
+        call("irismpc_share_db_fetch_results", self.h, out.ctypes.data, n, stream)
+        return out
+
+    @property
+    def counter(self):
+        c = ctypes.c_uint64()
+        call("irismpc_share_db_get_counter", self.h, ctypes.byref(c))
+        return c.value
+
+    @counter.setter
+    def counter(self, v):
+        call("irismpc_share_db_set_counter", self.h, v)
+
+    def set_variant(self, v):
+        call("irismpc_share_db_set_variant", self.h, v)
+
+
+def gemm_variants():
+    n = ctypes.c_int()
+    call("irismpc_gemm_variants", ctypes.byref(n))
+    return n.value
+
+
+class Circuits:
+    """threshold_ring/protocol.rs:433-2611."""
+
+    def __init__(self, peer_id, input_size, alloc_size, chacha_seeds, comm, tag=0, device=0):
+        so, sop = _u32x8(chacha_seeds[0])
+        sp, spp = _u32x8(chacha_seeds[1])
+        h = _out()
+        call("irismpc_circuits_new", device, peer_id, input_size, alloc_size, sop, spp, comm.h, tag,
+             ctypes.byref(h))
+        self.h = h.value
+        self.peer_id = peer_id
+        self.comm = comm
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            call("irismpc_circuits_destroy", self.h)
+            self.h = None
+
+    def set_chunk_size(self, words):
+        call("irismpc_circuits_set_chunk_size", self.h, words)
+
+    def compare_threshold_masked_many(self, code, mask, length, threshold_a, stream=0):
+        """code/mask = (a_ptr, b_ptr) replicated u16 shares. Collective."""
+        call("irismpc_circuits_compare_threshold_masked_many", self.h, code[0], code[1], mask[0],
+             mask[1], length, threshold_a, stream)
+
+    @staticmethod
+    def compare_threshold_masked_many_trio(circuits, codes, masks, length, threshold_a, streams):
+        call("irismpc_circuits_compare_threshold_masked_many_trio",
+             ptr_array([c.h for c in circuits]), ptr_array([c[0] for c in codes]),
+             ptr_array([c[1] for c in codes]), ptr_array([m[0] for m in masks]),
+             ptr_array([m[1] for m in masks]), length, threshold_a, ptr_array(list(streams)))
+
+    def take_result_buffer(self):
+        a, b = _out(), _out()
+        call("irismpc_circuits_result_buffer", self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def open(self, stream=0):
+        call("irismpc_circuits_open", self.h, stream)
+
+    @staticmethod
+    def open_trio(circuits, streams):
+        call("irismpc_circuits_open_trio", ptr_array([c.h for c in circuits]), ptr_array(list(streams)))
+
+    def open_views(self):
+        a, b, c = _out(), _out(), _out()
+        call("irismpc_circuits_open_views", self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
+    @property
+    def counter(self):
+        c = ctypes.c_uint64()
+        call("irismpc_circuits_get_counter", self.h, ctypes.byref(c))
+        return c.value
+
+    @counter.setter
+    def counter(self, v):
+        call("irismpc_circuits_set_counter", self.h, v)
+
+    def set_lift_capture(self, a_ptr, b_ptr):
+        call("irismpc_circuits_set_lift_capture", self.h, a_ptr, b_ptr)
+
+
+class DistanceComparator:
+    """dot/distance_comparator.rs:28-894."""
+
+    def __init__(self, query_length, max_db_size, device=0):
+        h = _out()
+        call("irismpc_comparator_init", device, query_length, max_db_size, ctypes.byref(h))
+        self.h = h.value
+        self.query_length = query_length
+        self.max_db_size = max_db_size
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            call("irismpc_comparator_destroy", self.h)
+            self.h = None
+
+    def open_results(self, a, b, c, bitmap_ptr, chunk_len, real_chunk_len, offset, total_db_len,
+                     batch_size, stream=0):
+        call("irismpc_comparator_open_results", self.h, a, b, c, bitmap_ptr, chunk_len,
+             real_chunk_len, offset, total_db_len, batch_size, stream)
+
+    def join_db_matches(self, left, right, final_results, db_size, or_policy=None, max_db_size=0,
+                        stream=0):
+        call("irismpc_comparator_join_db_matches", self.h, left, right, final_results, db_size,
+             or_policy, max_db_size or self.max_db_size, stream)
+
+    def reset(self, stream=0):
+        call("irismpc_comparator_reset", self.h, stream)
+
+    def buffers(self):
+        outs = [_out() for _ in range(6)]
+        call("irismpc_comparator_buffers", self.h, *[ctypes.byref(o) for o in outs])
+        return [o.value for o in outs]
+
+
+class PartyEngines:
+    """One party's engines for a DB scan (the ServerActor's codes/masks engines,
+    phase-2 Circuits and DistanceComparator, with HBM-resident DB shares)."""
+
+    def __init__(self, codes, masks, circuits, comparator, code_db, mask_db):
+        self.codes, self.masks, self.circuits, self.comparator = codes, masks, circuits, comparator
+        self.code_db, self.mask_db = code_db, mask_db
+
+
+def scan_db(party, q_code, q_mask, db_size, batch_size, threshold_a, bitmap_ptr, stream=0):
+    """compare_query_against_db_and_self_prefiltering's chunk loop for one party
+    (server/actor.rs:2246-2589). Collective: call from the three party threads."""
+    call("irismpc_scan_db", party.codes.h, party.masks.h, party.circuits.h, party.comparator.h,
+         party.code_db.h, party.mask_db.h, q_code.h, q_mask.h, db_size, batch_size, threshold_a,
+         bitmap_ptr, stream)
+
+
+def scan_db_trio(parties, q_codes, q_masks, db_size, batch_size, threshold_a, bitmap_ptrs, streams):
+    call("irismpc_scan_db_trio", ptr_array([p.codes.h for p in parties]),
+         ptr_array([p.masks.h for p in parties]), ptr_array([p.circuits.h for p in parties]),
+         ptr_array([p.comparator.h for p in parties]), ptr_array([p.code_db.h for p in parties]),
+         ptr_array([p.mask_db.h for p in parties]), ptr_array([q.h for q in q_codes]),
+         ptr_array([q.h for q in q_masks]), db_size, batch_size, threshold_a,
+         ptr_array(list(bitmap_ptrs)), ptr_array(list(streams)))

@@ -1,0 +1,143 @@
+"""GPU parity of the threshold circuits (Circuits) against the oracle.
+
+tests/threshold.rs analogue: random replicated shares of sampled code/mask dots,
+the three parties on one GPU; the result bit shares must equal the oracle's
+share for share (same ChaCha12 schedule), and the opened bits the closed form
+msb(M*A - (c<<16) - 1).
+"""
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEEDS_OWN = np.array([[0] * 8, [1] * 8, [2] * 8], np.uint32)
+SEEDS_PREV = np.array([[2] * 8, [0] * 8, [1] * 8], np.uint32)
+
+
+@pytest.fixture(scope="module")
+def irismpc():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import irismpc as m
+
+    m.lib()
+    return m
+
+
+def _sample(rng, L):
+    x = rng.integers(0, 12801, L).astype(np.int64)
+    neg = rng.integers(0, 2, L).astype(bool)
+    code = np.where(neg, (-x) & 0xFFFF, x).astype(np.uint16)
+    mask = rng.integers(0, 12801, L).astype(np.uint16)
+    return code, mask
+
+
+def _rep(rng, vals):
+    a = rng.integers(0, 1 << 16, vals.size, dtype=np.uint16)
+    b = rng.integers(0, 1 << 16, vals.size, dtype=np.uint16)
+    c = (vals.astype(np.int64) - a - b).astype(np.uint16)
+    x = [a, b, c]
+    return [x[p] for p in range(3)], [x[(p + 2) % 3] for p in range(3)]
+
+
+def _dev(torch, arr):
+    return torch.from_numpy(arr.view(np.int16).copy()).cuda()
+
+
+def _setup(irismpc, torch, W, alloc, counter=0):
+    ex = irismpc.Exchange()
+    circ = [irismpc.Circuits(p, 64 * W, alloc, (SEEDS_OWN[p], SEEDS_PREV[p]), ex) for p in range(3)]
+    for c in circ:
+        c.counter = counter
+    streams = [irismpc.stream_create() for _ in range(3)]
+    return ex, circ, streams
+
+
+def _run_case(irismpc, orc, torch, W, alloc, A, seed, counter=0, threads=False):
+    rng = np.random.default_rng(seed)
+    L = 64 * W
+    code, mask = _sample(rng, L)
+    ca, cb = _rep(rng, code)
+    ma, mb = _rep(rng, mask)
+    ex, circ, streams = _setup(irismpc, torch, W, alloc, counter)
+    dca = [_dev(torch, x) for x in ca]
+    dcb = [_dev(torch, x) for x in cb]
+    dma = [_dev(torch, x) for x in ma]
+    dmb = [_dev(torch, x) for x in mb]
+    lifted = [(torch.zeros(L, dtype=torch.int32, device="cuda"),
+               torch.zeros(L, dtype=torch.int32, device="cuda")) for _ in range(3)]
+    for p in range(3):
+        circ[p].set_lift_capture(lifted[p][0].data_ptr(), lifted[p][1].data_ptr())
+    torch.cuda.synchronize()
+    codes = [(dca[p].data_ptr(), dcb[p].data_ptr()) for p in range(3)]
+    masks = [(dma[p].data_ptr(), dmb[p].data_ptr()) for p in range(3)]
+    if threads:
+        errs = []
+
+        def party(p):
+            try:
+                circ[p].compare_threshold_masked_many(codes[p], masks[p], L, A, streams[p])
+                circ[p].open(streams[p])
+            except Exception as e:  # noqa: BLE001
+                errs.append(e)
+
+        ts = [threading.Thread(target=party, args=(p,)) for p in range(3)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        assert not errs, errs
+    else:
+        irismpc.Circuits.compare_threshold_masked_many_trio(circ, codes, masks, L, A, streams)
+        irismpc.Circuits.open_trio(circ, streams)
+    for s in streams:
+        irismpc.stream_synchronize(s)
+    def fetch(ptr, n):
+        return irismpc.download(ptr, n, np.uint64)
+
+    views = [circ[p].open_views() for p in range(3)]
+    got_a = [fetch(v[0], W) for v in views]
+    got_b = [fetch(v[1], W) for v in views]
+    got_c = [fetch(v[2], W) for v in views]
+    ra, rb, ctr, lift_ref = orc.threshold(ca, cb, ma, mb, alloc, A, SEEDS_OWN, SEEDS_PREV,
+                                          nonce=0, counter=counter, want_lifted=True)
+    for p in range(3):
+        assert np.array_equal(lifted[p][0].cpu().numpy().view(np.uint32), lift_ref[p]), f"lift p{p}"
+        assert np.array_equal(got_a[p], ra[p]), f"party {p} share a"
+        assert np.array_equal(got_b[p], rb[p]), f"party {p} share b"
+        assert np.array_equal(got_c[p], rb[(p + 2) % 3]), f"party {p} opened c"
+        assert circ[p].counter == ctr
+    opened = got_a[0] ^ got_b[0] ^ got_c[0]
+    bits = np.unpackbits(opened.view(np.uint8), bitorder="little").astype(bool)
+    exp = np.array([orc.mpc_predicate(int(c), int(m), A) for c, m in zip(code, mask)])
+    assert np.array_equal(bits, exp)
+    for p in range(3):
+        o = views[p]
+        mine = fetch(o[0], W) ^ fetch(o[1], W) ^ fetch(o[2], W)
+        assert np.array_equal(mine, opened), "all parties open the same bits"
+
+
+@pytest.mark.parametrize("A", [20316, 16384])
+def test_threshold_share_parity(irismpc, orc, A):
+    import torch
+
+    _run_case(irismpc, orc, torch, W=512, alloc=512, A=A, seed=0xDEADBEEF)
+
+
+def test_threshold_alloc_and_counter(irismpc, orc):
+    """alloc > chunk (bit-inject draws cover the alloc) and a non-zero counter."""
+    import torch
+
+    _run_case(irismpc, orc, torch, W=256, alloc=1024, A=16384, seed=11, counter=123456)
+
+
+def test_threshold_three_party_threads(irismpc, orc):
+    """Per-party collective calls from three threads (like the reference's
+    three spawn_blocking actors in tests/threshold.rs)."""
+    import torch
+
+    _run_case(irismpc, orc, torch, W=256, alloc=256, A=20316, seed=12, threads=True)
