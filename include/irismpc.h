@@ -141,9 +141,14 @@ int irismpc_share_db_result_views(irismpc_share_db* e, const uint16_t** a, const
 int irismpc_share_db_fetch_results(irismpc_share_db* e, uint16_t* out, size_t n, void* stream);
 int irismpc_share_db_get_counter(const irismpc_share_db* e, uint64_t* counter);
 int irismpc_share_db_set_counter(irismpc_share_db* e, uint64_t counter);
-/* GEMM tile variant (see irismpc_gemm_variants); chosen by the autotuner. */
+/* GEMM kernel variant: -1 (default) = automatic: the rotation-aware kernel for
+ * query sets uploaded from raw shares (irismpc_query_upload_shares), the best
+ * generic tile otherwise; 0 .. irismpc_gemm_variants()-1 = generic tiles;
+ * irismpc_gemm_rot_variants() = rotation-aware tiles (raw-share queries only).
+ * No reference counterpart: the reference has one cuBLAS-backed GEMM path. */
 int irismpc_share_db_set_variant(irismpc_share_db* e, int variant);
 int irismpc_gemm_variants(int* n);
+int irismpc_gemm_rot_variants(int* first, int* n);
 
 /* ------------------------------------------------------------- Circuits */
 /* Circuits::new (protocol.rs:463-508): input_size elements per call,

@@ -266,12 +266,18 @@ int irismpc_share_db_set_counter(irismpc_share_db* e, uint64_t counter) {
 }
 int irismpc_share_db_set_variant(irismpc_share_db* e, int variant) {
   return guarded([&] {
-    (void)gemm_tile(variant);
+    if (variant != ShareDB::kAutoVariant) (void)gemm_tile(variant);
     e->impl->variant = variant;
   });
 }
 int irismpc_gemm_variants(int* n) {
   return guarded([&] { *n = kGemmVariants; });
+}
+int irismpc_gemm_rot_variants(int* first, int* n) {
+  return guarded([&] {
+    *first = kRotVariantBase;
+    *n = kRotVariants;
+  });
 }
 
 // ------------------------------------------------------------- Circuits

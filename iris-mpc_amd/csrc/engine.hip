@@ -205,14 +205,26 @@ std::unique_ptr<QuerySet> ShareDB::upload_query_shares(const uint16_t* host_shar
   IRIS_HIP_CHECK(hipMemcpyAsync(raw.ptr, host_shares, raw.n * 2, hipMemcpyHostToDevice, s));
   launch_query_prep(raw.ptr, (int)batch, code_length_, peer_id_, rows.ptr, (int)query_length_, s);
   auto q = upload_query_device(rows.ptr, query_length_, s);
+  if (code_length_ % 800 == 0) {
+    q->rot.alloc(rot_planes_bytes((int)query_length_, code_length_));
+    launch_rot_planes(rows.ptr, (int)query_length_, code_length_, q->rot.ptr, s);
+  }
   IRIS_HIP_CHECK(hipStreamSynchronize(s));
   return q;
+}
+
+int ShareDB::resolve_variant(const QuerySet& q) const {
+  if (variant == kAutoVariant) return q.rot.ptr ? kRotVariantBase : 7;
+  IRIS_REQUIRE(!is_rot_variant(variant) || q.rot.ptr,
+               "rotation-aware GEMM variant needs a query uploaded from raw shares");
+  return variant;
 }
 
 static GemmArgs gemm_args(const QuerySet& q, const DbSlice& db, int variant) {
   GemmArgs a{};
   const GemmTile t = gemm_tile(variant);
   a.q = q.limbs.ptr;
+  a.qrot = q.rot.ptr;
   a.db = db.limbs.ptr;
   a.qsum0 = q.sum0.ptr;
   a.qsum1 = q.sum1.ptr;
@@ -233,6 +245,7 @@ void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, siz
   IRIS_REQUIRE(chunk <= max_db_length_, "chunk larger than the engine's result buffer");
   IRIS_REQUIRE(offset % 16 == 0, "offset must be a multiple of 16");
   IRIS_REQUIRE(offset + chunk <= db.capacity - kMaxBD, "chunk beyond DB capacity");
+  const int variant = resolve_variant(q);
   GemmArgs a = gemm_args(q, db, variant);
   const GemmTile t = gemm_tile(variant);
   a.out = results_.ptr;
@@ -260,6 +273,7 @@ void ShareDB::dot_reduce_range(const QuerySet& q, const DbSlice& db, size_t offs
   IRIS_REQUIRE(q.code_length == code_length_ && db.code_length == code_length_, "code length mismatch");
   IRIS_REQUIRE(total > 0 && total % 64 == 0 && chunk % 256 == 0, "bad range geometry");
   IRIS_REQUIRE(offset % 16 == 0 && offset + total <= db.capacity - kMaxBD, "range beyond DB");
+  const int variant = resolve_variant(q);
   GemmArgs a = gemm_args(q, db, variant);
   const GemmTile t = gemm_tile(variant);
   a.out = out;

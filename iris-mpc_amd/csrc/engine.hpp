@@ -101,6 +101,9 @@ struct QuerySet {
   int rows = 0;      // query_length rows
   DevBuf<int8_t> limbs;
   DevBuf<uint32_t> sum0, sum1;
+  // padded unrotated limb planes; present when the rows are the 31 rotations
+  // of each query (built from raw shares on the device)
+  DevBuf<int8_t> rot;
 };
 
 class ShareDB {
@@ -144,7 +147,11 @@ class ShareDB {
   uint64_t counter() const { return counter_; }
   void set_counter(uint64_t c) { counter_ = c; }
   bool is_remote() const { return ex_ != nullptr; }
-  int variant = 0;  // GEMM tile variant
+  // GEMM tile variant; kAutoVariant picks the rotation-aware kernel for query
+  // sets with rotation structure and the best generic one otherwise
+  static constexpr int kAutoVariant = -1;
+  int variant = kAutoVariant;
+  int resolve_variant(const QuerySet& q) const;
 
  private:
   int device_, peer_id_, code_length_, tag_;

@@ -9,11 +9,14 @@
 // Here all three limb products and the correction/mask epilogue run in one
 // kernel; the DB stays resident in HBM in MFMA fragment order.
 #include "share_gemm.hpp"
+
+#include <type_traits>
 #include "chacha.hpp"
 
 namespace irismpc {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 
 __host__ __device__ inline size_t frag_offset(size_t row_group, int kt, int limb, int n_kt) {
   return (((row_group * (size_t)n_kt + (size_t)kt) * 2 + (size_t)limb) * kFragBytes);
@@ -203,98 +206,92 @@ void launch_untile_rows(const int8_t* src, size_t row0, size_t n_rows, int K, ui
 // 16-byte k-slice selected by l / 16 for both operands, so any consistent
 // k-assignment contracts correctly; the result tile has row 4*(l/16)+r and
 // column l % 16 in register r.
-template <int WQ, int WD>
-__global__ __launch_bounds__(256, 1) void share_gemm_kernel(const GemmArgs a) {
-  constexpr int BQ = 2 * WQ * 16, BD = 2 * WD * 16;
-  __shared__ uint16_t tile[BQ * BD];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wq = wave >> 1, wd = wave & 1;
 
-  // XCD-aware mapping: consecutive block ids land on different XCDs; give
-  // each XCD a contiguous run of tiles with the query tile varying fastest so
-  // the blocks sharing a DB tile run together and hit that XCD's L2.
+// XCD-aware mapping: consecutive block ids land on different XCDs; each XCD
+// gets a contiguous run of tiles with the query tile varying fastest so the
+// blocks sharing a DB tile run together and hit that XCD's L2.
+template <int BQ, int BD>
+__device__ __forceinline__ bool block_tile(const GemmArgs& a, int& q0, size_t& db0) {
   const int T = a.n_qtiles * a.n_dtiles;
   const int T8 = (T + 7) / 8;
   const int t = (int)(blockIdx.x & 7) * T8 + (int)(blockIdx.x >> 3);
-  if (t >= T) return;
+  if (t >= T) return false;
   const int dtile = t / a.n_qtiles, qtile = t % a.n_qtiles;
-  const size_t db0 = a.db_begin + (size_t)dtile * BD;
-  const int q0 = qtile * BQ;
+  db0 = a.db_begin + (size_t)dtile * BD;
+  q0 = qtile * BQ;
+  return true;
+}
 
-  const size_t gstride = (size_t)a.n_kt * 2 * kFragBytes;  // bytes per 16-row group
-  const int8_t* qp = a.q + (size_t)(q0 / 16 + wq * WQ) * gstride + lane * 16;
-  const int8_t* dp = a.db + (db0 / 16 + (size_t)wd * WD) * gstride + lane * 16;
-
-  v4i acc0[WQ][WD], acc1[WQ][WD];
+// One K-step of the three limb products. The acc0 pass comes first and each
+// acc1 pass covers all tiles, so dependent accumulations into one register
+// set are WQ*WD MFMAs apart.
+template <int WQ, int WD>
+__device__ __forceinline__ void mma_step(v4i (&acc0)[WQ][WD], v4i (&acc1)[WQ][WD],
+                                         const v4i (&A)[WQ][2], const v4i (&B)[WD][2]) {
 #pragma unroll
   for (int i = 0; i < WQ; i++)
 #pragma unroll
-    for (int j = 0; j < WD; j++) {
-      acc0[i][j] = v4i{0, 0, 0, 0};
-      acc1[i][j] = v4i{0, 0, 0, 0};
-    }
+    for (int j = 0; j < WD; j++)
+      acc0[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][0], B[j][0], acc0[i][j], 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < WQ; i++)
+#pragma unroll
+    for (int j = 0; j < WD; j++)
+      acc1[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][0], B[j][1], acc1[i][j], 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < WQ; i++)
+#pragma unroll
+    for (int j = 0; j < WD; j++)
+      acc1[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][1], B[j][0], acc1[i][j], 0, 0, 0);
+}
 
-  v4i A0[WQ][2], B0[WD][2], A1[WQ][2], B1[WD][2];
-  auto load = [&](v4i (&A)[WQ][2], v4i (&B)[WD][2], int kt) {
-    const size_t ko = (size_t)kt * 2 * kFragBytes;
-#pragma unroll
-    for (int i = 0; i < WQ; i++) {
-      A[i][0] = *reinterpret_cast<const v4i*>(qp + i * gstride + ko);
-      A[i][1] = *reinterpret_cast<const v4i*>(qp + i * gstride + ko + kFragBytes);
-    }
-#pragma unroll
-    for (int j = 0; j < WD; j++) {
-      B[j][0] = *reinterpret_cast<const v4i*>(dp + j * gstride + ko);
-      B[j][1] = *reinterpret_cast<const v4i*>(dp + j * gstride + ko + kFragBytes);
-    }
-  };
-  auto mma = [&](const v4i (&A)[WQ][2], const v4i (&B)[WD][2]) {
-#pragma unroll
-    for (int i = 0; i < WQ; i++)
-#pragma unroll
-      for (int j = 0; j < WD; j++) {
-        acc0[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][0], B[j][0], acc0[i][j], 0, 0, 0);
-        acc1[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][0], B[j][1], acc1[i][j], 0, 0, 0);
-        acc1[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[i][1], B[j][0], acc1[i][j], 0, 0, 0);
-      }
-  };
+// Bias correction into the u16 LDS tile, then multiplier + ChaCha12
+// zero-sharing + 64-byte stores (dot/kernel.cu:16-28).
+__device__ __forceinline__ uint16_t corrected(const GemmArgs& a, int q, size_t d, uint32_t acc0,
+                                               uint32_t acc1) {
+  const uint32_t s0 = a.dsum0[d] + a.qsum0[q];
+  const uint32_t s1 = a.dsum1[d] + a.qsum1[q];
+  return (uint16_t)(acc0 + (acc1 << 8) + (s0 << 7) + ((s0 + s1) << 15));
+}
 
-  load(A0, B0, 0);
-  int kt = 0;
-  for (; kt + 2 <= a.n_kt; kt += 2) {
-    load(A1, B1, kt + 1);
-    mma(A0, B0);
-    if (kt + 2 < a.n_kt) load(A0, B0, kt + 2);
-    mma(A1, B1);
-  }
-  if (kt < a.n_kt) mma(A0, B0);
+template <int BQ, int BD>
+__device__ __forceinline__ void epilogue_store(const GemmArgs& a, const uint16_t* tile, int q0,
+                                               size_t db0, int q_end);
 
-  // ---- epilogue 1: bias correction -> u16 tile in LDS
+template <int WQ, int WD, int NWQ = 2, int NWD = 2>
+__device__ __forceinline__ void gemm_epilogue(const GemmArgs& a, uint16_t* tile,
+                                              const v4i (&acc0)[WQ][WD],
+                                              const v4i (&acc1)[WQ][WD], int q0, size_t db0,
+                                              int wq, int wd, int lane) {
+  constexpr int BQ = NWQ * WQ * 16, BD = NWD * WD * 16;
 #pragma unroll
   for (int i = 0; i < WQ; i++) {
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int ql = (wq * WQ + i) * 16 + 4 * (lane >> 4) + r;
-      const uint32_t qs0 = a.qsum0[q0 + ql], qs1 = a.qsum1[q0 + ql];
 #pragma unroll
       for (int j = 0; j < WD; j++) {
         const int dl = (wd * WD + j) * 16 + (lane & 15);
-        const uint32_t s0 = a.dsum0[db0 + dl] + qs0;
-        const uint32_t s1 = a.dsum1[db0 + dl] + qs1;
-        const uint32_t c = (uint32_t)acc0[i][j][r] + ((uint32_t)acc1[i][j][r] << 8);
-        tile[ql * BD + dl] = (uint16_t)(c + (s0 << 7) + ((s0 + s1) << 15));
+        tile[ql * BD + dl] = corrected(a, q0 + ql, db0 + dl, (uint32_t)acc0[i][j][r],
+                                       (uint32_t)acc1[i][j][r]);
       }
     }
   }
-  __syncthreads();
+  epilogue_store<BQ, BD>(a, tile, q0, db0, a.qr);
+}
 
-  // ---- epilogue 2: multiplier, ChaCha12 zero-sharing, 64-byte stores
+// Multiplier, ChaCha12 zero-sharing and 64-byte stores of the u16 tile; query
+// rows at or beyond q_end are not stored.
+template <int BQ, int BD>
+__device__ __forceinline__ void epilogue_store(const GemmArgs& a, const uint16_t* tile, int q0,
+                                               size_t db0, int q_end) {
+  __syncthreads();
   constexpr int SEGS = BD / 32;
-  for (int s = threadIdx.x; s < BQ * SEGS; s += 256) {
+  for (int s = threadIdx.x; s < BQ * SEGS; s += blockDim.x) {
     const int ql = s / SEGS, seg = s % SEGS;
     const int q = q0 + ql;
     const size_t r = db0 + (size_t)seg * 32;
-    if (q >= a.qr || r >= a.db_end) continue;
+    if (q >= q_end || r >= a.db_end) continue;
     const size_t rl = r - a.db_begin;
     const size_t c = rl / a.chunk;
     const size_t local = rl - c * a.chunk;
@@ -332,12 +329,811 @@ __global__ __launch_bounds__(256, 1) void share_gemm_kernel(const GemmArgs a) {
   }
 }
 
+// Direct form: each wave loads its fragments straight from global memory into
+// registers (double-buffered), no LDS staging.
+template <int WQ, int WD>
+__global__ __launch_bounds__(256, 1) void share_gemm_kernel(const GemmArgs a) {
+  constexpr int BQ = 2 * WQ * 16, BD = 2 * WD * 16;
+  __shared__ uint16_t tile[BQ * BD];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wq = wave >> 1, wd = wave & 1;
+  int q0;
+  size_t db0;
+  if (!block_tile<BQ, BD>(a, q0, db0)) return;
+
+  const size_t gstride = (size_t)a.n_kt * 2 * kFragBytes;  // bytes per 16-row group
+  const int8_t* qp = a.q + (size_t)(q0 / 16 + wq * WQ) * gstride + lane * 16;
+  const int8_t* dp = a.db + (db0 / 16 + (size_t)wd * WD) * gstride + lane * 16;
+
+  v4i acc0[WQ][WD], acc1[WQ][WD];
+#pragma unroll
+  for (int i = 0; i < WQ; i++)
+#pragma unroll
+    for (int j = 0; j < WD; j++) {
+      acc0[i][j] = v4i{0, 0, 0, 0};
+      acc1[i][j] = v4i{0, 0, 0, 0};
+    }
+
+  v4i A0[WQ][2], B0[WD][2], A1[WQ][2], B1[WD][2];
+  auto load = [&](v4i (&A)[WQ][2], v4i (&B)[WD][2], int kt) {
+    const size_t ko = (size_t)kt * 2 * kFragBytes;
+#pragma unroll
+    for (int i = 0; i < WQ; i++) {
+      A[i][0] = *reinterpret_cast<const v4i*>(qp + i * gstride + ko);
+      A[i][1] = *reinterpret_cast<const v4i*>(qp + i * gstride + ko + kFragBytes);
+    }
+#pragma unroll
+    for (int j = 0; j < WD; j++) {
+      B[j][0] = *reinterpret_cast<const v4i*>(dp + j * gstride + ko);
+      B[j][1] = *reinterpret_cast<const v4i*>(dp + j * gstride + ko + kFragBytes);
+    }
+  };
+
+  load(A0, B0, 0);
+  int kt = 0;
+  for (; kt + 2 <= a.n_kt; kt += 2) {
+    load(A1, B1, kt + 1);
+    mma_step<WQ, WD>(acc0, acc1, A0, B0);
+    if (kt + 2 < a.n_kt) load(A0, B0, kt + 2);
+    mma_step<WQ, WD>(acc0, acc1, A1, B1);
+  }
+  if (kt < a.n_kt) mma_step<WQ, WD>(acc0, acc1, A0, B0);
+  gemm_epilogue<WQ, WD>(a, tile, acc0, acc1, q0, db0, wq, wd, lane);
+}
+
+// LDS-staged form: per K-step the block's query and DB fragments (each 1 KiB,
+// one coalesced dwordx4 per lane) are fetched from global memory once into a
+// double-buffered LDS stage; waves read their operands with ds_read_b128.
+// NWQ x NWD waves; with 8 waves two share each SIMD, so one wave's MFMAs
+// cover the other's LDS reads.
+template <int WQ, int WD, int NWQ, int NWD>
+struct LdsStages {
+  static constexpr int NA = NWQ * WQ * 2, NB = NWD * WD * 2, NF = NA + NB;
+  union {
+    v4i stage[2][NF * 64];
+    uint16_t tile[(NWQ * WQ * 16) * (NWD * WD * 16)];
+  };
+};
+
+template <int WQ, int WD, int NWQ, int NWD>
+__global__ __launch_bounds__(64 * NWQ * NWD, 1) void share_gemm_lds_kernel(const GemmArgs a) {
+  constexpr int NW = NWQ * NWD;
+  constexpr int BQ = NWQ * WQ * 16, BD = NWD * WD * 16;
+  using S = LdsStages<WQ, WD, NWQ, NWD>;
+  constexpr int NA = S::NA, NF = S::NF;
+  static_assert(NF % NW == 0, "fragments split evenly over the waves");
+  constexpr int PER_WAVE = NF / NW;
+  __shared__ S sm;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wq = wave / NWD, wd = wave % NWD;
+  int q0;
+  size_t db0;
+  if (!block_tile<BQ, BD>(a, q0, db0)) return;
+
+  const size_t gstride = (size_t)a.n_kt * 2 * kFragBytes;
+  const int8_t* qbase = a.q + (size_t)(q0 / 16) * gstride + lane * 16;
+  const int8_t* dbase = a.db + (db0 / 16) * gstride + lane * 16;
+  // fragment f of a K-step: f < NA -> query group f/2, limb f%2; else DB group
+  // (f-NA)/2, limb (f-NA)%2. Wave w fetches fragments w, w+NW, w+2NW, ...
+  auto src = [&](int f, int kt) -> const v4i* {
+    const size_t ko = (size_t)kt * 2 * kFragBytes;
+    if (f < NA) return reinterpret_cast<const v4i*>(qbase + (f >> 1) * gstride + ko + (f & 1) * kFragBytes);
+    const int g = f - NA;
+    return reinterpret_cast<const v4i*>(dbase + (g >> 1) * gstride + ko + (g & 1) * kFragBytes);
+  };
+
+  v4i acc0[WQ][WD], acc1[WQ][WD];
+#pragma unroll
+  for (int i = 0; i < WQ; i++)
+#pragma unroll
+    for (int j = 0; j < WD; j++) {
+      acc0[i][j] = v4i{0, 0, 0, 0};
+      acc1[i][j] = v4i{0, 0, 0, 0};
+    }
+
+  v4i stg[PER_WAVE];
+#pragma unroll
+  for (int m = 0; m < PER_WAVE; m++) stg[m] = *src(wave + NW * m, 0);
+#pragma unroll
+  for (int m = 0; m < PER_WAVE; m++) sm.stage[0][(wave + NW * m) * 64 + lane] = stg[m];
+  __syncthreads();
+
+  for (int kt = 0; kt < a.n_kt; kt++) {
+    const int st = kt & 1;
+    const bool more = kt + 1 < a.n_kt;
+    if (more) {
+#pragma unroll
+      for (int m = 0; m < PER_WAVE; m++) stg[m] = *src(wave + NW * m, kt + 1);
+    }
+    v4i A[WQ][2], B[WD][2];
+#pragma unroll
+    for (int i = 0; i < WQ; i++)
+#pragma unroll
+      for (int l = 0; l < 2; l++) A[i][l] = sm.stage[st][((wq * WQ + i) * 2 + l) * 64 + lane];
+#pragma unroll
+    for (int j = 0; j < WD; j++)
+#pragma unroll
+      for (int l = 0; l < 2; l++) B[j][l] = sm.stage[st][(NA + (wd * WD + j) * 2 + l) * 64 + lane];
+    mma_step<WQ, WD>(acc0, acc1, A, B);
+    if (more) {
+#pragma unroll
+      for (int m = 0; m < PER_WAVE; m++) sm.stage[st ^ 1][(wave + NW * m) * 64 + lane] = stg[m];
+    }
+    __syncthreads();
+  }
+  gemm_epilogue<WQ, WD, NWQ, NWD>(a, sm.tile, acc0, acc1, q0, db0, wq, wd, lane);
+}
+
+
+// 32x32x32 form (v_mfma_i32_32x32x32_i8 sustains about twice the int8 rate of
+// 16x16x64 on gfx950). The HBM/LDS fragments keep their 16-row x 64-k shape;
+// for the 32-row operand, lane l (row r = l % 32, k-half h = l / 32) of k-slice
+// s reads lane (r % 16) + 16 (2 s + h) of the fragment of row group r / 16, so
+// both operands see the same k assignment. Result tile: lane l, register v
+// hold row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.
+template <int TQ, int TD, int NWQ, int NWD>
+__global__ __launch_bounds__(64 * NWQ * NWD, 1) void share_gemm_lds32_kernel(const GemmArgs a) {
+  constexpr int NW = NWQ * NWD;
+  constexpr int BQ = NWQ * TQ * 32, BD = NWD * TD * 32;
+  using S = LdsStages<TQ * 2, TD * 2, NWQ, NWD>;
+  constexpr int NA = S::NA, NF = S::NF;
+  static_assert(NF % NW == 0, "fragments split evenly over the waves");
+  constexpr int PER_WAVE = NF / NW;
+  __shared__ S sm;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wq = wave / NWD, wd = wave % NWD;
+  int q0;
+  size_t db0;
+  if (!block_tile<BQ, BD>(a, q0, db0)) return;
+
+  const size_t gstride = (size_t)a.n_kt * 2 * kFragBytes;
+  const int8_t* qbase = a.q + (size_t)(q0 / 16) * gstride + lane * 16;
+  const int8_t* dbase = a.db + (db0 / 16) * gstride + lane * 16;
+  auto src = [&](int f, int kt) -> const v4i* {
+    const size_t ko = (size_t)kt * 2 * kFragBytes;
+    if (f < NA) return reinterpret_cast<const v4i*>(qbase + (f >> 1) * gstride + ko + (f & 1) * kFragBytes);
+    const int g = f - NA;
+    return reinterpret_cast<const v4i*>(dbase + (g >> 1) * gstride + ko + (g & 1) * kFragBytes);
+  };
+  const int r32 = lane & 31, h = lane >> 5;
+  // fragment-local lane for k-slice s, plus the row-group offset (0 or 1)
+  const int lane_s0 = (r32 & 15) + 16 * h, lane_s1 = (r32 & 15) + 16 * (2 + h);
+  const int gsub = r32 >> 4;
+
+  v16i acc0[TQ][TD], acc1[TQ][TD];
+#pragma unroll
+  for (int i = 0; i < TQ; i++)
+#pragma unroll
+    for (int j = 0; j < TD; j++) {
+      acc0[i][j] = v16i{};
+      acc1[i][j] = v16i{};
+    }
+
+  v4i stg[PER_WAVE];
+#pragma unroll
+  for (int m = 0; m < PER_WAVE; m++) stg[m] = *src(wave + NW * m, 0);
+#pragma unroll
+  for (int m = 0; m < PER_WAVE; m++) sm.stage[0][(wave + NW * m) * 64 + lane] = stg[m];
+  __syncthreads();
+
+  for (int kt = 0; kt < a.n_kt; kt++) {
+    const int st = kt & 1;
+    const bool more = kt + 1 < a.n_kt;
+    if (more) {
+#pragma unroll
+      for (int m = 0; m < PER_WAVE; m++) stg[m] = *src(wave + NW * m, kt + 1);
+    }
+    v4i A[TQ][2][2], B[TD][2][2];  // [tile][k-slice][limb]
+#pragma unroll
+    for (int i = 0; i < TQ; i++) {
+      const int g = (wq * TQ + i) * 2 + gsub;
+#pragma unroll
+      for (int l = 0; l < 2; l++) {
+        A[i][0][l] = sm.stage[st][(g * 2 + l) * 64 + lane_s0];
+        A[i][1][l] = sm.stage[st][(g * 2 + l) * 64 + lane_s1];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < TD; j++) {
+      const int g = (wd * TD + j) * 2 + gsub;
+#pragma unroll
+      for (int l = 0; l < 2; l++) {
+        B[j][0][l] = sm.stage[st][(NA + g * 2 + l) * 64 + lane_s0];
+        B[j][1][l] = sm.stage[st][(NA + g * 2 + l) * 64 + lane_s1];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int i = 0; i < TQ; i++)
+#pragma unroll
+        for (int j = 0; j < TD; j++)
+          acc0[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][0], B[j][s][0], acc0[i][j], 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int i = 0; i < TQ; i++)
+#pragma unroll
+        for (int j = 0; j < TD; j++)
+          acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][0], B[j][s][1], acc1[i][j], 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int i = 0; i < TQ; i++)
+#pragma unroll
+        for (int j = 0; j < TD; j++)
+          acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][1], B[j][s][0], acc1[i][j], 0, 0, 0);
+    if (more) {
+#pragma unroll
+      for (int m = 0; m < PER_WAVE; m++) sm.stage[st ^ 1][(wave + NW * m) * 64 + lane] = stg[m];
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < TQ; i++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      const int ql = (wq * TQ + i) * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+#pragma unroll
+      for (int j = 0; j < TD; j++) {
+        const int dl = (wd * TD + j) * 32 + r32;
+        sm.tile[ql * BD + dl] = corrected(a, q0 + ql, db0 + dl, (uint32_t)acc0[i][j][v],
+                                          (uint32_t)acc1[i][j][v]);
+      }
+    }
+  epilogue_store<BQ, BD>(a, sm.tile, q0, db0, a.qr);
+}
+
+
+// Direct-to-LDS form for gfx950: each fragment (1 KiB, lane l's 16 bytes at
+// l * 16) is one wave-wide global_load_lds_dwordx4, which lands in LDS in the
+// same lane order. With no register staging the ring can be STAGES deep, so a
+// K-step's loads are in flight for STAGES - 1 steps of MFMA work instead of one.
+__device__ __forceinline__ void dma_fragment(const int8_t* lane_src, v4i* lds_frag) {
+  __builtin_amdgcn_global_load_lds(
+      (__attribute__((address_space(1))) void*)(lane_src),
+      (__attribute__((address_space(3))) void*)(lds_frag), 16, 0, 0);
+}
+
+// s_waitcnt with only the vector-memory counter constrained (gfx9 encoding:
+// vmcnt bits [3:0] and [15:14], expcnt [6:4] and lgkmcnt [11:8] left at max).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Workgroup barrier without the release/acquire fence of __syncthreads (whose
+// vmcnt(0) would drain the whole DMA ring): the explicit vmcnt wait before it
+// orders this wave's LDS DMA writes, and the empty asm with a memory clobber
+// keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void stage_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <int TQ, int TD, int NWQ, int NWD, int STAGES>
+struct DmaStages {
+  static constexpr int NA = NWQ * TQ * 4, NB = NWD * TD * 4, NF = NA + NB;
+  union {
+    v4i stage[STAGES][NF * 64];
+    uint16_t tile[(NWQ * TQ * 32) * (NWD * TD * 32)];
+  };
+};
+
+template <int TQ, int TD, int NWQ, int NWD, int STAGES>
+__global__ __launch_bounds__(64 * NWQ * NWD, 1) void share_gemm_dma_kernel(const GemmArgs a) {
+  constexpr int NW = NWQ * NWD;
+  constexpr int BQ = NWQ * TQ * 32, BD = NWD * TD * 32;
+  using S = DmaStages<TQ, TD, NWQ, NWD, STAGES>;
+  constexpr int NA = S::NA, NF = S::NF;
+  static_assert(NF % NW == 0, "fragments split evenly over the waves");
+  constexpr int PER_WAVE = NF / NW;
+  static_assert(PER_WAVE * (STAGES - 2) < 64, "outstanding loads fit vmcnt");
+  __shared__ S sm;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wq = wave / NWD, wd = wave % NWD;
+  int q0;
+  size_t db0;
+  if (!block_tile<BQ, BD>(a, q0, db0)) return;
+
+  const size_t gstride = (size_t)a.n_kt * 2 * kFragBytes;
+  const int8_t* qbase = a.q + (size_t)(q0 / 16) * gstride + lane * 16;
+  const int8_t* dbase = a.db + (db0 / 16) * gstride + lane * 16;
+  auto issue = [&](int kt, int st) {
+    const size_t ko = (size_t)kt * 2 * kFragBytes;
+#pragma unroll
+    for (int m = 0; m < PER_WAVE; m++) {
+      const int f = wave + NW * m;
+      const int8_t* src = f < NA ? qbase + (f >> 1) * gstride + ko + (f & 1) * kFragBytes
+                                 : dbase + ((f - NA) >> 1) * gstride + ko + ((f - NA) & 1) * kFragBytes;
+      dma_fragment(src, &sm.stage[st][f * 64]);
+    }
+  };
+  const int r32 = lane & 31, h = lane >> 5;
+  const int lane_s0 = (r32 & 15) + 16 * h, lane_s1 = (r32 & 15) + 16 * (2 + h);
+  const int gsub = r32 >> 4;
+
+  v16i acc0[TQ][TD], acc1[TQ][TD];
+#pragma unroll
+  for (int i = 0; i < TQ; i++)
+#pragma unroll
+    for (int j = 0; j < TD; j++) {
+      acc0[i][j] = v16i{};
+      acc1[i][j] = v16i{};
+    }
+
+  const int n_kt = a.n_kt;
+#pragma unroll
+  for (int p = 0; p < STAGES - 1; p++)
+    if (p < n_kt) issue(p, p);
+
+  int st = 0;
+  for (int kt = 0; kt < n_kt; kt++) {
+    // own loads of step kt done (later steps may stay in flight), then the
+    // barrier makes every wave's fragments of step kt visible and retires all
+    // reads of the stage that the next issue overwrites
+    if (kt + STAGES - 2 < n_kt) wait_vmcnt<PER_WAVE * (STAGES - 2)>();
+    else wait_vmcnt<0>();
+    stage_barrier();
+    if (kt + STAGES - 1 < n_kt) {
+      int sn = st + STAGES - 1;
+      if (sn >= STAGES) sn -= STAGES;
+      issue(kt + STAGES - 1, sn);
+    }
+    const v4i* stg = sm.stage[st];
+    v4i A[TQ][2][2], B[TD][2][2];  // [tile][k-slice][limb]
+#pragma unroll
+    for (int i = 0; i < TQ; i++) {
+      const int g = (wq * TQ + i) * 2 + gsub;
+#pragma unroll
+      for (int l = 0; l < 2; l++) {
+        A[i][0][l] = stg[(g * 2 + l) * 64 + lane_s0];
+        A[i][1][l] = stg[(g * 2 + l) * 64 + lane_s1];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < TD; j++) {
+      const int g = (wd * TD + j) * 2 + gsub;
+#pragma unroll
+      for (int l = 0; l < 2; l++) {
+        B[j][0][l] = stg[(NA + g * 2 + l) * 64 + lane_s0];
+        B[j][1][l] = stg[(NA + g * 2 + l) * 64 + lane_s1];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int i = 0; i < TQ; i++)
+#pragma unroll
+        for (int j = 0; j < TD; j++)
+          acc0[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][0], B[j][s][0], acc0[i][j], 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int i = 0; i < TQ; i++)
+#pragma unroll
+        for (int j = 0; j < TD; j++)
+          acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][0], B[j][s][1], acc1[i][j], 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int i = 0; i < TQ; i++)
+#pragma unroll
+        for (int j = 0; j < TD; j++)
+          acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][1], B[j][s][0], acc1[i][j], 0, 0, 0);
+    st = st + 1 == STAGES ? 0 : st + 1;
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int i = 0; i < TQ; i++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      const int ql = (wq * TQ + i) * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+#pragma unroll
+      for (int j = 0; j < TD; j++) {
+        const int dl = (wd * TD + j) * 32 + r32;
+        sm.tile[ql * BD + dl] = corrected(a, q0 + ql, db0 + dl, (uint32_t)acc0[i][j][v],
+                                          (uint32_t)acc1[i][j][v]);
+      }
+    }
+  epilogue_store<BQ, BD>(a, sm.tile, q0, db0, a.qr);
+}
+
+
+// Hybrid form: the query tile, which every wave of the block multiplies, is
+// shared through a 3-stage LDS DMA ring; each wave's own DB rows bypass LDS
+// and load straight into VGPRs. Waves split the DB dimension only (1 x NW),
+// each owning TQ x TD 32x32 tiles (accumulators in AGPRs at one wave per
+// SIMD). Per MFMA this moves well under half the LDS bytes of the all-LDS
+// forms, whose ds_read + DMA traffic saturates LDS.
+//
+// Step k opens with one group of MFMAs on A(k), B(k) (their first use retires
+// every load of step k - 1: with LDS-DMA and register loads both pending the
+// compiler waits for vmcnt(0) there), then the barrier that publishes query
+// step k + 1, then the DMA of query step k + 2, the DB loads of step k + 1
+// and the ds_reads of A(k + 1), then the rest of step k's MFMAs, which cover
+// those loads. Loads are unconditional (indices clamped to the last step; the
+// surplus DMA lands in a stage nobody reads again).
+template <int TQ, int TD, int NW>
+struct HybStages {
+  static constexpr int NA = TQ * 4;  // query fragments per K-step (16 rows x 64 k x limb)
+  static constexpr int STAGES = 3;
+  union {
+    v4i stage[STAGES][NA * 64];
+    uint16_t tile[(TQ * 32) * (NW * TD * 32)];
+  };
+};
+
+template <int TQ, int TD, int NW>
+__global__ __launch_bounds__(64 * NW, 1) void share_gemm_hyb_kernel(const GemmArgs a) {
+  constexpr int BQ = TQ * 32, BD = NW * TD * 32;
+  using S = HybStages<TQ, TD, NW>;
+  constexpr int NA = S::NA, STAGES = S::STAGES;
+  static_assert(NA % NW == 0, "query fragments split evenly over the waves");
+  constexpr int PER_WAVE = NA / NW;  // DMA instructions per wave per K-step
+  constexpr int NMFMA = 2 * 3 * TQ * TD, G1 = TQ * TD;
+  __shared__ S sm;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int q0;
+  size_t db0;
+  if (!block_tile<BQ, BD>(a, q0, db0)) return;
+
+  const size_t gstride = (size_t)a.n_kt * 2 * kFragBytes;
+  const int r32 = lane & 31, h = lane >> 5, gsub = r32 >> 4;
+  const int lane_s0 = (r32 & 15) + 16 * h, lane_s1 = lane_s0 + 32;
+  const int8_t* qbase = a.q + (size_t)(q0 / 16) * gstride + lane * 16;
+  // lane's DB row: tile j of this wave, 16-row group 2j + gsub, fragment lane lane_s0
+  const int8_t* dbase = a.db + (db0 / 16 + (size_t)wave * TD * 2 + gsub) * gstride + lane_s0 * 16;
+  const int n_kt = a.n_kt, last = n_kt - 1;
+
+  auto issue_q = [&](int kt, int st) {
+    const size_t ko = (size_t)kt * 2 * kFragBytes;
+#pragma unroll
+    for (int m = 0; m < PER_WAVE; m++) {
+      const int f = wave + NW * m;
+      dma_fragment(qbase + (f >> 1) * gstride + ko + (f & 1) * kFragBytes, &sm.stage[st][f * 64]);
+    }
+  };
+  auto load_b = [&](v4i (&B)[TD][2][2], int kt) {
+    const size_t ko = (size_t)kt * 2 * kFragBytes;
+#pragma unroll
+    for (int j = 0; j < TD; j++)
+#pragma unroll
+      for (int sl = 0; sl < 2; sl++)
+#pragma unroll
+        for (int l = 0; l < 2; l++)
+          B[j][sl][l] = *reinterpret_cast<const v4i*>(dbase + (size_t)(2 * j) * gstride + ko +
+                                                      l * kFragBytes + sl * 512);
+  };
+  auto read_a = [&](v4i (&A)[TQ][2][2], int st, int sl) {
+    const v4i* stg = sm.stage[st];
+#pragma unroll
+    for (int i = 0; i < TQ; i++)
+#pragma unroll
+      for (int l = 0; l < 2; l++)
+        A[i][sl][l] = stg[((2 * i + gsub) * 2 + l) * 64 + (sl ? lane_s1 : lane_s0)];
+  };
+
+  v16i acc0[TQ][TD], acc1[TQ][TD];
+#pragma unroll
+  for (int i = 0; i < TQ; i++)
+#pragma unroll
+    for (int j = 0; j < TD; j++) {
+      acc0[i][j] = v16i{};
+      acc1[i][j] = v16i{};
+    }
+  // MFMAs [LO, HI) of one K-step, in the order slice, product, query tile, DB
+  // tile: consecutive MFMAs on one accumulator are TQ * TD apart
+  auto mfma = [&](const v4i (&A)[TQ][2][2], const v4i (&B)[TD][2][2], auto lo, auto hi) {
+    constexpr int LO = decltype(lo)::value, HI = decltype(hi)::value;
+#pragma unroll
+    for (int sl = 0; sl < 2; sl++)
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int i = 0; i < TQ; i++)
+#pragma unroll
+          for (int j = 0; j < TD; j++) {
+            const int idx = ((sl * 3 + p) * TQ + i) * TD + j;
+            if (idx < LO || idx >= HI) continue;
+            if (p == 0)
+              acc0[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][sl][0], B[j][sl][0], acc0[i][j], 0, 0, 0);
+            else if (p == 1)
+              acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][sl][0], B[j][sl][1], acc1[i][j], 0, 0, 0);
+            else
+              acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][sl][1], B[j][sl][0], acc1[i][j], 0, 0, 0);
+          }
+  };
+  using C0 = std::integral_constant<int, 0>;
+  using CG = std::integral_constant<int, G1>;
+  using CH = std::integral_constant<int, NMFMA / 2>;
+  using CN = std::integral_constant<int, NMFMA>;
+
+  // prologue: query steps 0 and 1 in flight with DB(0); drain; A(0)
+  issue_q(0, 0);
+  issue_q(last < 1 ? last : 1, 1);
+  v4i Ba[TD][2][2], Bb[TD][2][2], Aa[TQ][2][2], Ab[TQ][2][2];
+  load_b(Ba, 0);
+  wait_vmcnt<0>();
+  stage_barrier();
+  read_a(Aa, 0, 0);
+  read_a(Aa, 0, 1);
+
+  // the ds_reads of A(k + 1) go out in two batches of at most 15 so that the
+  // lgkmcnt that guards A(k) stays encodable (it would otherwise be 0)
+  auto step = [&](int kt, int st, const v4i (&Ac)[TQ][2][2], v4i (&An)[TQ][2][2],
+                  const v4i (&Bc)[TD][2][2], v4i (&Bn)[TD][2][2]) {
+    mfma(Ac, Bc, C0{}, CG{});
+    __builtin_amdgcn_sched_barrier(0);
+    wait_vmcnt<0>();
+    stage_barrier();
+    int s1 = st + 1, s2 = st + 2;
+    if (s1 >= STAGES) s1 -= STAGES;
+    if (s2 >= STAGES) s2 -= STAGES;
+    issue_q(kt + 2 < last ? kt + 2 : last, s2);
+    load_b(Bn, kt + 1 < last ? kt + 1 : last);
+    read_a(An, s1, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    mfma(Ac, Bc, CG{}, CH{});
+    __builtin_amdgcn_sched_barrier(0);
+    read_a(An, s1, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    mfma(Ac, Bc, CH{}, CN{});
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  int kt = 0, st = 0;
+  for (; kt + 1 < n_kt; kt += 2) {
+    step(kt, st, Aa, Ab, Ba, Bb);
+    st = st + 1 == STAGES ? 0 : st + 1;
+    step(kt + 1, st, Ab, Aa, Bb, Ba);
+    st = st + 1 == STAGES ? 0 : st + 1;
+  }
+  if (kt < n_kt) mfma(Aa, Ba, C0{}, CN{});
+  __syncthreads();
+
+#pragma unroll
+  for (int i = 0; i < TQ; i++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      const int ql = i * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+#pragma unroll
+      for (int j = 0; j < TD; j++) {
+        const int dl = (wave * TD + j) * 32 + r32;
+        sm.tile[ql * BD + dl] = corrected(a, q0 + ql, db0 + dl, (uint32_t)acc0[i][j][v],
+                                          (uint32_t)acc1[i][j][v]);
+      }
+    }
+  epilogue_store<BQ, BD>(a, sm.tile, q0, db0, a.qr);
+}
+
+
+// Rotation-aware form. The 31 query rows of one iris are rotations of one
+// preprocessed share: row (b, t) holds, in every 800-coefficient row r of the
+// share, U_b[r][(c + 4 (15 - t)) mod 800] at coefficient c. A block's query
+// tile is 4 whole queries (124 rows, 4 MFMA row tiles with 4 spare rows), and
+// their unrotated shares sit in LDS for the block's whole K loop as padded
+// limb planes P_b[limb][r][i] = limb(U_b[r][(i - 60) mod 800]), i < 920, so
+// rotation t of coefficients [c, c + 16) is the 16 bytes at i = 60 + c +
+// 4 (15 - t) (4-byte aligned). The query then costs 115 KiB of global reads
+// per block instead of 3.2 MiB, the K loop needs no barriers, and only DB rows
+// stream from global memory, straight into VGPRs two K-steps ahead.
+constexpr int kRotPad = 60, kRotRow = 800 + 2 * kRotPad, kRotQueries = 4;
+
+template <int TD, int NW>
+struct RotLds {
+  static constexpr int kMaxRows = kCodeLen / 800;  // 16 rows of 800 coefficients
+  union {
+    v4i planes[kRotQueries * 2 * kMaxRows * kRotRow / 16];
+    uint16_t tile[128 * (NW * TD * 32)];
+  };
+};
+
+template <int TD, int NW>
+__global__ __launch_bounds__(64 * NW, 1) void share_gemm_rot_kernel(const GemmArgs a) {
+  constexpr int TQ = 4;  // 128 rows >= 4 x 31
+  constexpr int BQ = kRotQueries * kRotations, BD = NW * TD * 32;
+  constexpr int G = TQ * TD, HALF = 3 * TQ * TD;  // MFMAs per pass, per k-slice
+  using S = RotLds<TD, NW>;
+  __shared__ S sm;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int q0;
+  size_t db0;
+  if (!block_tile<BQ, BD>(a, q0, db0)) return;
+
+  const int n_kt = a.n_kt, last = n_kt - 1;
+  const int R = n_kt * kKTile / 800;          // 800-coefficient rows of the share
+  const int plane = R * kRotRow;              // bytes per limb plane
+  const int qbytes = 2 * plane;               // bytes per query
+  {
+    // 1 KiB per wave-wide LDS DMA, all in flight at once (the plane buffer
+    // carries 1 KiB of slack for the last partial chunk)
+    const int8_t* src = a.qrot + (size_t)(q0 / kRotations) * qbytes + lane * 16;
+    const int n_chunks = (kRotQueries * qbytes + kFragBytes - 1) / kFragBytes;
+    for (int c = wave; c < n_chunks; c += NW) dma_fragment(src + (size_t)c * kFragBytes, &sm.planes[c * 64]);
+    wait_vmcnt<0>();
+  }
+  __syncthreads();
+
+  const size_t gstride = (size_t)n_kt * 2 * kFragBytes;
+  const int r32 = lane & 31, h = lane >> 5, gsub = r32 >> 4;
+  const int lane_s0 = (r32 & 15) + 16 * h;
+  const int8_t* dbase = a.db + (db0 / 16 + (size_t)wave * TD * 2 + gsub) * gstride + lane_s0 * 16;
+  int abase[TQ];  // byte offset of (query, rotation) row of tile i, limb 0, coefficient 0
+#pragma unroll
+  for (int i = 0; i < TQ; i++) {
+    int rho = 32 * i + r32;
+    if (rho >= BQ) rho = BQ - 1;  // spare rows: computed, never stored
+    const int b = rho / kRotations, t = rho % kRotations;
+    abase[i] = b * qbytes + kRotPad + 4 * (15 - t);
+  }
+  const uint8_t* lds = reinterpret_cast<const uint8_t*>(sm.planes);
+
+  auto read_a = [&](v4i (&A)[TQ][2], int kt, int sl) {
+    const int k0 = kt * kKTile + 32 * sl + 16 * h;
+    const int off = k0 + (kRotRow - 800) * (k0 / 800);
+#pragma unroll
+    for (int i = 0; i < TQ; i++)
+#pragma unroll
+      for (int l = 0; l < 2; l++) {
+        const uint32_t* p =
+            reinterpret_cast<const uint32_t*>(__builtin_assume_aligned(lds + abase[i] + l * plane + off, 4));
+        A[i][l] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
+      }
+  };
+  auto load_b = [&](v4i (&B)[TD][2][2], int kt) {
+    const size_t ko = (size_t)kt * 2 * kFragBytes;
+#pragma unroll
+    for (int j = 0; j < TD; j++)
+#pragma unroll
+      for (int sl = 0; sl < 2; sl++)
+#pragma unroll
+        for (int l = 0; l < 2; l++)
+          B[j][sl][l] = *reinterpret_cast<const v4i*>(dbase + (size_t)(2 * j) * gstride + ko +
+                                                      l * kFragBytes + sl * 512);
+  };
+
+  v16i acc0[TQ][TD], acc1[TQ][TD];
+#pragma unroll
+  for (int i = 0; i < TQ; i++)
+#pragma unroll
+    for (int j = 0; j < TD; j++) {
+      acc0[i][j] = v16i{};
+      acc1[i][j] = v16i{};
+    }
+  // MFMAs [LO, HI) of one k-slice in the order product, query tile, DB tile:
+  // consecutive MFMAs on one accumulator are TQ * TD apart
+  auto mfma = [&](const v4i (&A)[TQ][2], const v4i (&B)[TD][2][2], int sl, auto lo, auto hi) {
+    constexpr int LO = decltype(lo)::value, HI = decltype(hi)::value;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+      for (int i = 0; i < TQ; i++)
+#pragma unroll
+        for (int j = 0; j < TD; j++) {
+          const int idx = (p * TQ + i) * TD + j;
+          if (idx < LO || idx >= HI) continue;
+          if (p == 0)
+            acc0[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][0], B[j][sl][0], acc0[i][j], 0, 0, 0);
+          else if (p == 1)
+            acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][0], B[j][sl][1], acc1[i][j], 0, 0, 0);
+          else
+            acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][1], B[j][sl][0], acc1[i][j], 0, 0, 0);
+        }
+  };
+  using C0 = std::integral_constant<int, 0>;
+  using CG = std::integral_constant<int, G>;
+  using CH = std::integral_constant<int, HALF>;
+
+  v4i A0[TQ][2], A1[TQ][2];
+  v4i B0[TD][2][2], B1[TD][2][2], B2[TD][2][2];
+  load_b(B0, 0);
+  load_b(B1, last < 1 ? last : 1);
+  read_a(A0, 0, 0);
+
+  // One K-step on B(kt): each k-slice opens with one pass of MFMAs (whose
+  // first use of the operands is where the compiler waits for them), then
+  // issues the next slice's A reads (and in slice 0 the DB loads of step
+  // kt + 2), then runs the other two passes while those land.
+  auto step = [&](int kt, const v4i (&Bc)[TD][2][2], v4i (&Bp)[TD][2][2]) {
+    mfma(A0, Bc, 0, C0{}, CG{});
+    __builtin_amdgcn_sched_barrier(0);
+    load_b(Bp, kt + 2 < last ? kt + 2 : last);
+    read_a(A1, kt, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    mfma(A0, Bc, 0, CG{}, CH{});
+    __builtin_amdgcn_sched_barrier(0);
+    mfma(A1, Bc, 1, C0{}, CG{});
+    __builtin_amdgcn_sched_barrier(0);
+    read_a(A0, kt + 1 < last ? kt + 1 : last, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    mfma(A1, Bc, 1, CG{}, CH{});
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  int kt = 0;
+  for (; kt + 2 < n_kt; kt += 3) {
+    step(kt, B0, B2);
+    step(kt + 1, B1, B0);
+    step(kt + 2, B2, B1);
+  }
+  if (kt < n_kt) step(kt, B0, B2);
+  if (kt + 1 < n_kt) step(kt + 1, B1, B0);
+  __syncthreads();
+
+  const int q_end = a.qr < q0 + BQ ? a.qr : q0 + BQ;
+#pragma unroll
+  for (int i = 0; i < TQ; i++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      const int ql = i * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+      if (q0 + ql >= q_end) continue;
+#pragma unroll
+      for (int j = 0; j < TD; j++) {
+        const int dl = (wave * TD + j) * 32 + r32;
+        sm.tile[ql * BD + dl] = corrected(a, q0 + ql, db0 + dl, (uint32_t)acc0[i][j][v],
+                                          (uint32_t)acc1[i][j][v]);
+      }
+    }
+  epilogue_store<128, BD>(a, sm.tile, q0, db0, q_end);
+}
+
+// Padded unrotated limb planes of every query slot from the rotated u16 rows
+// (rotation 15 is the unrotated share; slots past the rows are zero shares).
+__global__ __launch_bounds__(256) void rot_planes_kernel(const uint16_t* rows, int qr, int K,
+                                                         int slots, int8_t* out) {
+  const int R = K / 800;
+  const size_t n = (size_t)slots * R * kRotRow;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % kRotRow);
+  const int r = (int)((t / kRotRow) % R);
+  const int b = (int)(t / ((size_t)kRotRow * R));
+  const int src_row = b * kRotations + 15;
+  uint16_t v = 0;
+  if (src_row < qr) v = rows[(size_t)src_row * K + r * 800 + (i - kRotPad + 800) % 800];
+  int8_t* q = out + (size_t)b * 2 * R * kRotRow + (size_t)r * kRotRow + i;
+  q[0] = (int8_t)((v & 0xff) ^ 0x80);
+  q[(size_t)R * kRotRow] = (int8_t)((v >> 8) ^ 0x80);
+}
+
+size_t rot_planes_bytes(int qr, int K) {
+  const size_t slots = ceil_div((size_t)qr, (size_t)(kRotQueries * kRotations)) * kRotQueries;
+  return slots * 2 * (size_t)(K / 800) * kRotRow + kFragBytes;
+}
+
+void launch_rot_planes(const uint16_t* rows, int qr, int K, int8_t* out, hipStream_t s) {
+  IRIS_REQUIRE(K % 800 == 0 && K <= kCodeLen, "rotation planes need K a multiple of 800");
+  const int slots = (int)(ceil_div((size_t)qr, (size_t)(kRotQueries * kRotations)) * kRotQueries);
+  const size_t n = (size_t)slots * (K / 800) * kRotRow;
+  rot_planes_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(rows, qr, K, slots, out);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
 GemmTile gemm_tile(int variant) {
   switch (variant) {
     case 0: return {128, 128};
     case 1: return {64, 256};
     case 2: return {128, 256};
     case 3: return {64, 128};
+    case 4: return {128, 256};   // LDS-staged
+    case 5: return {128, 128};   // LDS-staged
+    case 6: return {64, 256};    // LDS-staged
+    case 7: return {128, 256};   // LDS-staged, 8 waves (2 per SIMD)
+    case 8: return {256, 128};   // LDS-staged, 8 waves (2 per SIMD)
+    case 9: return {128, 256};   // 32x32x32 MFMA, LDS-staged, 8 waves
+    case 10: return {256, 128};  // 32x32x32 MFMA, LDS-staged, 8 waves
+    case 11: return {128, 256};  // 32x32x32 MFMA, LDS-staged, 4 waves
+    case 12: return {128, 256};  // 32x32x32 MFMA, 3-stage LDS DMA ring, 8 waves
+    case 13: return {256, 128};  // 32x32x32 MFMA, 3-stage LDS DMA ring, 8 waves
+    case 14: return {128, 256};  // 32x32x32 MFMA, 3-stage LDS DMA ring, 4 waves
+    case 15: return {128, 256};  // hybrid: query via LDS DMA, DB direct, 4 waves of 128x64
+    case 16: return {128, 128};  // hybrid, 2 waves of 128x64
+    case 17: return {64, 256};   // hybrid, 4 waves of 64x64
+    case kRotVariantBase: return {kRotQueries * kRotations, 256};      // rotation-aware, 4 waves
+    case kRotVariantBase + 1: return {kRotQueries * kRotations, 128};  // rotation-aware, 2 waves
     default: throw Error("unknown share GEMM variant " + std::to_string(variant));
   }
 }
@@ -348,6 +1144,8 @@ void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s) {
   IRIS_REQUIRE((a.db_end - a.db_begin) % 64 == 0 && a.db_end > a.db_begin, "bad DB range");
   IRIS_REQUIRE(a.chunk % 256 == 0, "chunk must be a multiple of 256 rows");
   IRIS_REQUIRE(a.n_qtiles == (int)ceil_div((size_t)a.qr, (size_t)t.bq), "n_qtiles mismatch");
+  IRIS_REQUIRE(!is_rot_variant(variant) || (a.qrot != nullptr && (a.n_kt * kKTile) % 800 == 0),
+               "rotation-aware variant needs a query uploaded from raw shares");
   IRIS_REQUIRE(a.n_dtiles == (int)ceil_div(a.db_end - a.db_begin, (size_t)t.bd), "n_dtiles mismatch");
   const int T = a.n_qtiles * a.n_dtiles;
   const unsigned grid = (unsigned)(8 * ((T + 7) / 8));
@@ -356,6 +1154,22 @@ void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s) {
     case 1: share_gemm_kernel<2, 8><<<grid, 256, 0, s>>>(a); break;
     case 2: share_gemm_kernel<4, 8><<<grid, 256, 0, s>>>(a); break;
     case 3: share_gemm_kernel<2, 4><<<grid, 256, 0, s>>>(a); break;
+    case 4: share_gemm_lds_kernel<4, 8, 2, 2><<<grid, 256, 0, s>>>(a); break;
+    case 5: share_gemm_lds_kernel<4, 4, 2, 2><<<grid, 256, 0, s>>>(a); break;
+    case 6: share_gemm_lds_kernel<2, 8, 2, 2><<<grid, 256, 0, s>>>(a); break;
+    case 7: share_gemm_lds_kernel<4, 4, 2, 4><<<grid, 512, 0, s>>>(a); break;
+    case 8: share_gemm_lds_kernel<4, 4, 4, 2><<<grid, 512, 0, s>>>(a); break;
+    case 9: share_gemm_lds32_kernel<2, 2, 2, 4><<<grid, 512, 0, s>>>(a); break;
+    case 10: share_gemm_lds32_kernel<2, 2, 4, 2><<<grid, 512, 0, s>>>(a); break;
+    case 11: share_gemm_lds32_kernel<2, 4, 2, 2><<<grid, 256, 0, s>>>(a); break;
+    case 12: share_gemm_dma_kernel<2, 2, 2, 4, 3><<<grid, 512, 0, s>>>(a); break;
+    case 13: share_gemm_dma_kernel<2, 2, 4, 2, 3><<<grid, 512, 0, s>>>(a); break;
+    case 14: share_gemm_dma_kernel<2, 4, 2, 2, 3><<<grid, 256, 0, s>>>(a); break;
+    case 15: share_gemm_hyb_kernel<4, 2, 4><<<grid, 256, 0, s>>>(a); break;
+    case 16: share_gemm_hyb_kernel<4, 2, 2><<<grid, 128, 0, s>>>(a); break;
+    case 17: share_gemm_hyb_kernel<2, 2, 4><<<grid, 256, 0, s>>>(a); break;
+    case kRotVariantBase: share_gemm_rot_kernel<2, 4><<<grid, 256, 0, s>>>(a); break;
+    case kRotVariantBase + 1: share_gemm_rot_kernel<2, 2><<<grid, 128, 0, s>>>(a); break;
   }
   IRIS_HIP_CHECK(hipGetLastError());
 }

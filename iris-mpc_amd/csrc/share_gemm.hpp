@@ -16,6 +16,7 @@ namespace irismpc {
 struct GemmArgs {
   const int8_t* q;      // tiled query limbs (rows padded to the q tile)
   const int8_t* db;     // tiled DB limbs
+  const int8_t* qrot;   // padded unrotated query limb planes (rotation-aware variants)
   const uint32_t* qsum0;
   const uint32_t* qsum1;
   const uint32_t* dsum0;
@@ -61,10 +62,18 @@ void launch_untile_rows(const int8_t* src, size_t row0, size_t n_rows, int K, ui
 struct GemmTile {
   int bq, bd;
 };
-constexpr int kGemmVariants = 4;
-constexpr int kMaxBQ = 128;   // query buffers are padded to this many rows
+constexpr int kGemmVariants = 18;
+// Variants from kRotVariantBase on need a query set with rotation structure
+// (uploaded from raw shares, rotations generated on the device).
+constexpr int kRotVariantBase = 100, kRotVariants = 2;
+inline bool is_rot_variant(int v) { return v >= kRotVariantBase && v < kRotVariantBase + kRotVariants; }
+constexpr int kMaxBQ = 256;   // query buffers are padded to this many rows
 constexpr int kMaxBD = 256;   // DB buffers are padded to this many rows
 GemmTile gemm_tile(int variant);
 void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s);
+// Padded unrotated limb planes for the rotation-aware variants, built from the
+// rotated u16 query rows [qr][K].
+size_t rot_planes_bytes(int qr, int K);
+void launch_rot_planes(const uint16_t* rows, int qr, int K, int8_t* out, hipStream_t s);
 
 }  // namespace irismpc

@@ -65,6 +65,7 @@ SIGNATURES = {
     "irismpc_share_db_set_counter": [vp, u64],
     "irismpc_share_db_set_variant": [vp, i32],
     "irismpc_gemm_variants": [vp],
+    "irismpc_gemm_rot_variants": [vp, vp],
     "irismpc_circuits_new": [i32, i32, sz, sz, vp, vp, vp, i32, vp],
     "irismpc_circuits_destroy": [vp],
     "irismpc_circuits_set_chunk_size": [vp, sz],

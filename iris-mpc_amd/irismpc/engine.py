@@ -222,6 +222,13 @@ def gemm_variants():
     return n.value
 
 
+def gemm_rot_variants():
+    """Ids of the rotation-aware GEMM variants (queries from raw shares only)."""
+    first, n = ctypes.c_int(), ctypes.c_int()
+    call("irismpc_gemm_rot_variants", ctypes.byref(first), ctypes.byref(n))
+    return list(range(first.value, first.value + n.value))
+
+
 class Circuits:
     """threshold_ring/protocol.rs:433-2611."""
 

@@ -102,6 +102,76 @@ def test_share_gemm_masked_chunks(irismpc, orc):
     assert np.array_equal(got1, got[:Qr * chunk])
 
 
+def _rotated_rows(orc, party, raw, qr):
+    """The reference's query preparation: Lagrange/basis B per share, then the
+    31 rotations of every query; rows past the batch are zero shares."""
+    rows = np.zeros((qr, raw.shape[1]), np.uint16)
+    for j in range(raw.shape[0]):
+        rows[j * 31:(j + 1) * 31] = orc.share_rotations(orc.preprocess_query(party, raw[j]))
+    return rows
+
+
+@pytest.mark.parametrize("K,mult,batch,qr", [(12800, 1, 6, 250), (6400, 2, 9, 279)])
+def test_share_gemm_rotation_variants(irismpc, orc, K, mult, batch, qr):
+    """Rotation-aware kernels on queries uploaded from raw shares (device
+    preprocessing + rotations) against the oracle's rotated rows, next to the
+    generic variants and the automatic choice; query_length is not a multiple
+    of the 4-query tile, so the last tile is partial."""
+    rng = np.random.default_rng(11)
+    N, chunk, party = 1000, 1024, 1
+    raw = _rand(rng, (batch, K))
+    db = _rand(rng, (N, K))
+    ref = orc.u16_dot(_rotated_rows(orc, party, raw, qr), db)
+    ref = (ref.astype(np.uint32) * mult).astype(np.uint16)
+    eng = irismpc.ShareDB(party, chunk, qr, K, ([0] * 8, [0] * 8))
+    slices = eng.alloc_db(N)
+    slices.load_full_db(db)
+    dq = eng.upload_query_shares(raw)
+    for v in [-1, 7, 15] + irismpc.gemm_rot_variants():
+        eng.set_variant(v)
+        eng.dot_reduce(dq, slices, chunk, 0, mult)
+        got = eng.fetch_results(qr * chunk).reshape(qr, chunk)
+        assert np.array_equal(got[:, :N], ref), f"variant {v}"
+
+
+def test_share_gemm_rotation_masked_range(irismpc, orc):
+    """Masked multi-chunk launch with the rotation-aware kernel: same shares as
+    the generic kernel, first chunk checked against the oracle."""
+    import torch
+
+    rng = np.random.default_rng(12)
+    K, qr, batch, N, chunk = 12800, 248, 8, 768, 256
+    raw = _rand(rng, (batch, K))
+    db = _rand(rng, (N, K))
+    seeds = [np.full(8, 3, np.uint32), np.full(8, 5, np.uint32)]
+    ex = irismpc.Exchange()
+    eng = irismpc.ShareDB(2, chunk, qr, K, seeds, comm=ex, tag=0)
+    slices = eng.alloc_db(N)
+    slices.load_full_db(db)
+    dq = eng.upload_query_shares(raw)
+    outs = []
+    for v in (irismpc.gemm_rot_variants()[0], 7):
+        eng.set_variant(v)
+        eng.counter = 40
+        out = torch.zeros(N * qr + 64, dtype=torch.int16, device="cuda")
+        eng.dot_reduce_range(dq, slices, 0, N, chunk, 1, out.data_ptr())
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy().view(np.uint16))
+    assert np.array_equal(outs[0], outs[1])
+    ref = orc.u16_dot(_rotated_rows(orc, 2, raw, qr), db[:chunk]).reshape(-1)
+    ref = orc.mask_dot_share(ref, 1, seeds[0], seeds[1], 0, 40)
+    assert np.array_equal(outs[0][:chunk * qr], ref)
+
+
+def test_rotation_variant_needs_structured_query(irismpc):
+    eng = irismpc.ShareDB(0, 256, 248, 6400, ([0] * 8, [0] * 8))
+    slices = eng.alloc_db(256)
+    dq = eng.upload_query(np.zeros((248, 6400), np.uint16))
+    eng.set_variant(irismpc.gemm_rot_variants()[0])
+    with pytest.raises(irismpc.IrisMpcError, match="raw shares"):
+        eng.dot_reduce(dq, slices, 256, 0, 1)
+
+
 def test_write_at_index(irismpc, orc):
     rng = np.random.default_rng(5)
     K, Qr = 6400, 248
