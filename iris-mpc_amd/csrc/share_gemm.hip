@@ -1038,19 +1038,33 @@ __global__ __launch_bounds__(64 * NW, 1) void share_gemm_rot_kernel(const GemmAr
   // first use of the operands is where the compiler waits for them), then
   // issues the next slice's A reads (and in slice 0 the DB loads of step
   // kt + 2), then runs the other two passes while those land.
+  // The loads are threaded one or two at a time between the MFMAs of the
+  // other two passes (a batch of them would leave this SIMD's only wave
+  // without an MFMA to issue).
+  constexpr int NBL = TD * 4, NAR = TQ * 2 * 2;  // DB loads, ds_read2 per slice
+  static_assert(NBL * 2 <= HALF - G && NAR <= 2 * (HALF - G), "interleave pattern");
   auto step = [&](int kt, const v4i (&Bc)[TD][2][2], v4i (&Bp)[TD][2][2]) {
     mfma(A0, Bc, 0, C0{}, CG{});
     __builtin_amdgcn_sched_barrier(0);
     load_b(Bp, kt + 2 < last ? kt + 2 : last);
     read_a(A1, kt, 1);
-    __builtin_amdgcn_sched_barrier(0);
     mfma(A0, Bc, 0, CG{}, CH{});
+#pragma unroll
+    for (int r = 0; r < HALF - G; r++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
+      if (r < NBL) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);     // VMEM read
+      if (r < NAR / 2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // DS read
+    }
     __builtin_amdgcn_sched_barrier(0);
     mfma(A1, Bc, 1, C0{}, CG{});
     __builtin_amdgcn_sched_barrier(0);
     read_a(A0, kt + 1 < last ? kt + 1 : last, 0);
-    __builtin_amdgcn_sched_barrier(0);
     mfma(A1, Bc, 1, CG{}, CH{});
+#pragma unroll
+    for (int r = 0; r < HALF - G; r++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
+      if (r < NAR / 2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // DS read
+    }
     __builtin_amdgcn_sched_barrier(0);
   };
   int kt = 0;
@@ -1063,18 +1077,46 @@ __global__ __launch_bounds__(64 * NW, 1) void share_gemm_rot_kernel(const GemmAr
   if (kt + 1 < n_kt) step(kt + 1, B1, B0);
   __syncthreads();
 
+  // The limb-sum correction splits into a query term and a DB term:
+  // (s0 << 7) + ((s0 + s1) << 15) with s = dsum + qsum is linear in u32.
+  // Query terms of the tile's 128 rows go to LDS past the output tile.
   const int q_end = a.qr < q0 + BQ ? a.qr : q0 + BQ;
+  uint32_t* qterm = reinterpret_cast<uint32_t*>(&sm.tile[128 * BD]);
+  static_assert(sizeof(S) >= 128 * BD * 2 + 128 * 4, "query terms fit behind the tile");
+  if (threadIdx.x < 128) {
+    const int q = q0 + (int)threadIdx.x;
+    uint32_t t = 0;
+    if (q < q_end) {
+      const uint32_t s0 = a.qsum0[q], s1 = a.qsum1[q];
+      t = (s0 << 7) + ((s0 + s1) << 15);
+    }
+    qterm[threadIdx.x] = t;
+  }
+  uint32_t dterm[TD];
+#pragma unroll
+  for (int j = 0; j < TD; j++) {
+    const size_t d = db0 + (wave * TD + j) * 32 + r32;
+    const uint32_t s0 = a.dsum0[d], s1 = a.dsum1[d];
+    dterm[j] = (s0 << 7) + ((s0 + s1) << 15);
+  }
+  __syncthreads();
 #pragma unroll
   for (int i = 0; i < TQ; i++)
 #pragma unroll
-    for (int v = 0; v < 16; v++) {
-      const int ql = i * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-      if (q0 + ql >= q_end) continue;
+    for (int v4 = 0; v4 < 4; v4++) {
+      const int qb = i * 32 + 8 * v4 + 4 * h;  // rows qb .. qb + 3 in registers 4 v4 ..
+      uint32_t qt[4];
+      __builtin_memcpy(qt, &qterm[qb], 16);
 #pragma unroll
-      for (int j = 0; j < TD; j++) {
-        const int dl = (wave * TD + j) * 32 + r32;
-        sm.tile[ql * BD + dl] = corrected(a, q0 + ql, db0 + dl, (uint32_t)acc0[i][j][v],
-                                          (uint32_t)acc1[i][j][v]);
+      for (int r = 0; r < 4; r++) {
+        const int v = 4 * v4 + r, ql = qb + r;
+        if (q0 + ql >= q_end) continue;
+#pragma unroll
+        for (int j = 0; j < TD; j++) {
+          const int dl = (wave * TD + j) * 32 + r32;
+          sm.tile[ql * BD + dl] = (uint16_t)((uint32_t)acc0[i][j][v] + ((uint32_t)acc1[i][j][v] << 8) +
+                                             qt[r] + dterm[j]);
+        }
       }
     }
   epilogue_store<128, BD>(a, sm.tile, q0, db0, q_end);
