@@ -75,25 +75,44 @@ __device__ __forceinline__ void store8(uint64_t* p, const uint64_t v[8]) {
 
 // ------------------------------------------------------------ transpose
 // Plane j, word i, bit k = bit j of element 64 i + k (kernel.cu:63-84).
+// Block per 64 consecutive words: each wave turns 16 words into bit planes by
+// ballots, the planes go through LDS and leave as 512-byte runs per plane.
 __global__ __launch_bounds__(256) void transpose16_kernel(const Trio t) {
   const int p = party_of(t);
   const CircuitBufs& P = t.p[p];
-  const int lane = threadIdx.x & 63;
-  const size_t w = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (w >= t.W) return;
-  const uint32_t va = P.mask_a[w * 64 + lane], vb = P.mask_b[w * 64 + lane];
-  uint64_t mine = 0;
+  __shared__ uint64_t planes[32][64];  // planes 0..15: a, 16..31: b
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t w0 = (size_t)blockIdx.x * 64;
+#pragma unroll 4
+  for (int k = 0; k < 16; k++) {
+    const int wl = wave * 16 + k;
+    const size_t w = w0 + wl;
+    uint32_t va = 0, vb = 0;
+    if (w < t.W) {
+      va = P.mask_a[w * 64 + lane];
+      vb = P.mask_b[w * 64 + lane];
+    }
+    uint64_t mine = 0;
 #pragma unroll
-  for (int j = 0; j < 16; j++) {
-    const uint64_t ba = __ballot((va >> j) & 1u);
-    const uint64_t bb = __ballot((vb >> j) & 1u);
-    if (lane == j) mine = ba;
-    if (lane == 16 + j) mine = bb;
+    for (int j = 0; j < 16; j++) {
+      const uint64_t ba = __ballot((va >> j) & 1u);
+      const uint64_t bb = __ballot((vb >> j) & 1u);
+      if (lane == j) mine = ba;
+      if (lane == 16 + j) mine = bb;
+    }
+    if (lane < 32) planes[lane][wl] = mine;
   }
-  if (lane < 16)
-    P.tm_a[(size_t)lane * t.W + w] = mine;
-  else if (lane < 32)
-    P.tm_b[(size_t)(lane - 16) * t.W + w] = mine;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    const int idx = (int)threadIdx.x + 256 * m, pl = idx >> 6, wl = idx & 63;
+    const size_t w = w0 + wl;
+    if (w >= t.W) continue;
+    if (pl < 16)
+      P.tm_a[(size_t)pl * t.W + w] = planes[pl][wl];
+    else
+      P.tm_b[(size_t)(pl - 16) * t.W + w] = planes[pl][wl];
+  }
 }
 
 // ------------------------------------------------------------ carry-save AND
@@ -258,67 +277,117 @@ __global__ __launch_bounds__(256) void ripple_kernel(const Trio t, int msb, int 
 
 // ------------------------------------------------------------ bit injection
 // 128 W elements e: bits of the two lift carries (e < 64 W: bit 16,
-// e >= 64 W: bit 17). Thread per 32 elements = one ChaCha block per stream.
+// e >= 64 W: bit 17); ChaCha12 block g of each stream covers elements 32 g .. 32 g + 31.
 // kernel.cu:526-629 with the message flow P1 -> P2 -> P0 and P0 -> P1.
-__device__ __forceinline__ uint32_t bit_at(const uint64_t* w, size_t e) {
-  return (uint32_t)((w[e >> 6] >> (e & 63)) & 1);
+
+// Wave per 2048 elements (64 ChaCha12 blocks, one per lane; block g covers
+// elements 32 g .. 32 g + 31 as the u16 view of its keystream). The two
+// keystreams go through LDS in element order, then each lane handles 8
+// consecutive elements per pass with 16-byte loads and stores.
+__device__ __forceinline__ void u16x8_load(const uint16_t* p, uint16_t v[8]) {
+  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int i = 0; i < 8; i++) v[i] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+}
+
+__device__ __forceinline__ void u16x8_store(uint16_t* p, const uint16_t v[8]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] = (uint32_t)v[2 * i] | ((uint32_t)v[2 * i + 1] << 16);
+  *reinterpret_cast<uint4*>(p) = uint4{w[0], w[1], w[2], w[3]};
 }
 
 __global__ __launch_bounds__(256) void inject_kernel(const Trio t, int step, uint64_t ctr) {
   const int p = party_of(t);
-  const CircuitBufs& P = t.p[p];
-  const CircuitBufs& N = t.p[(p + 1) % 3];
-  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t n = 128 * t.W;
-  if (32 * g >= n) return;
   // which party acts in which step
   const bool act = (step == 1 && p != 2) || (step == 2 && p != 0) || (step == 3 && p == 0);
   if (!act) return;
-  uint32_t ko[16], kp[16];
-  chacha_block<12>(P.key_own, t.nonce, ctr + g, ko);   // my_rng (chacha1)
-  chacha_block<12>(P.key_prev, t.nonce, ctr + g, kp);  // their_rng (chacha2)
-  const size_t e0 = 32 * g;
-#pragma unroll 4
-  for (int i = 0; i < 32; i++) {
-    const size_t e = e0 + i;
-    // u16 view of the u32 keystream (little endian): element i = half i%2 of word i/2
-    const uint16_t own_i = (uint16_t)(ko[i >> 1] >> (16 * (i & 1)));
-    const uint16_t prv_i = (uint16_t)(kp[i >> 1] >> (16 * (i & 1)));
+  const CircuitBufs& P = t.p[p];
+  const CircuitBufs& N = t.p[(p + 1) % 3];
+  __shared__ uint32_t ks[4][2][1024];  // [wave][own, prev][u16 pairs in element order]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t n = 128 * t.W;
+  const size_t group = (size_t)blockIdx.x * 4 + wave;
+  const size_t e_base = group * 2048;
+  if (e_base < n) {
+    uint32_t ko[16], kp[16];
+    chacha_block<12>(P.key_own, t.nonce, ctr + group * 64 + lane, ko);   // my_rng (chacha1)
+    chacha_block<12>(P.key_prev, t.nonce, ctr + group * 64 + lane, kp);  // their_rng (chacha2)
+#pragma unroll
+    for (int i = 0; i < 16; i += 4) {
+      *reinterpret_cast<uint4*>(&ks[wave][0][lane * 16 + i]) = uint4{ko[i], ko[i + 1], ko[i + 2], ko[i + 3]};
+      *reinterpret_cast<uint4*>(&ks[wave][1][lane * 16 + i]) = uint4{kp[i], kp[i + 1], kp[i + 2], kp[i + 3]};
+    }
+  }
+  __syncthreads();
+  if (e_base >= n) return;
+  const uint16_t* own16 = reinterpret_cast<const uint16_t*>(ks[wave][0]);
+  const uint16_t* prv16 = reinterpret_cast<const uint16_t*>(ks[wave][1]);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int el = 8 * (k * 64 + lane);
+    const size_t e = e_base + el;
+    if (e >= n) break;
+    uint16_t own[8], prv[8], in[8], o1[8], o2[8], o3[8];
+    u16x8_load(own16 + el, own);
+    u16x8_load(prv16 + el, prv);
     if (p == 0) {
       // rand_01 = my rng, rand_02 = their rng; in_b = party 0's b share
-      const uint16_t r01 = own_i, r02 = prv_i;
-      const uint16_t b = (uint16_t)bit_at(P.c_b, e);
+      const uint32_t bits = (uint32_t)(P.c_b[e >> 6] >> (e & 63)) & 0xffu;
       if (step == 1) {
-        const uint16_t y = (uint16_t)(b * r01 - r02);
-        P.msg_own[e] = y;
-        N.inbox[e] = y;
-        P.corr_a[e] = (uint16_t)(r01 - (y << 1));
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const uint16_t b = (uint16_t)((bits >> i) & 1u), r01 = own[i], r02 = prv[i];
+          const uint16_t y = (uint16_t)(b * r01 - r02);
+          o1[i] = y;
+          o2[i] = (uint16_t)(r01 - (y << 1));
+        }
+        u16x8_store(N.inbox + e, o1);
+        u16x8_store(P.corr_a + e, o2);
       } else {
-        const uint16_t z = P.inbox[e];
-        P.corr_b[e] = (uint16_t)(-((r02 + z) << 1) + b);
+        u16x8_load(P.inbox + e, in);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const uint16_t b = (uint16_t)((bits >> i) & 1u), r02 = prv[i], z = in[i];
+          o1[i] = (uint16_t)(-((r02 + z) << 1) + b);
+        }
+        u16x8_store(P.corr_b + e, o1);
       }
     } else if (p == 1) {
       // rand_01 = their rng, rand_12 = my rng
-      const uint16_t r01 = prv_i, r12 = own_i;
       if (step == 1) {
-        const uint16_t x = (uint16_t)((bit_at(P.c_a, e) ^ bit_at(P.c_b, e)) - r01);
-        P.msg_own[e] = x;
-        N.inbox[e] = x;
-        P.corr_a[e] = (uint16_t)(x - (r12 << 1));
+        const uint32_t bits = (uint32_t)((P.c_a[e >> 6] ^ P.c_b[e >> 6]) >> (e & 63)) & 0xffu;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const uint16_t r01 = prv[i], r12 = own[i];
+          const uint16_t x = (uint16_t)(((bits >> i) & 1u) - r01);
+          o1[i] = x;
+          o2[i] = (uint16_t)(x - (r12 << 1));
+        }
+        u16x8_store(N.inbox + e, o1);
+        u16x8_store(P.corr_a + e, o2);
       } else {
-        const uint16_t y = P.inbox[e];
-        P.corr_b[e] = (uint16_t)(r01 - (y << 1));
+        u16x8_load(P.inbox + e, in);
+#pragma unroll
+        for (int i = 0; i < 8; i++) o1[i] = (uint16_t)(prv[i] - (in[i] << 1));
+        u16x8_store(P.corr_b + e, o1);
       }
     } else {
       // rand_12 = their rng, rand_02 = my rng; in_a = party 2's a share
-      const uint16_t r12 = prv_i, r02 = own_i;
-      const uint16_t x = P.inbox[e];
-      const uint16_t b = (uint16_t)bit_at(P.c_a, e);
-      const uint16_t z = (uint16_t)(b * x - r12);
-      P.msg_own[e] = z;
-      N.inbox[e] = z;
-      P.corr_a[e] = (uint16_t)(-((r02 + z) << 1) + b);
-      P.corr_b[e] = (uint16_t)(x - (r12 << 1));
+      const uint32_t bits = (uint32_t)(P.c_a[e >> 6] >> (e & 63)) & 0xffu;
+      u16x8_load(P.inbox + e, in);
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const uint16_t r12 = prv[i], r02 = own[i], x = in[i], b = (uint16_t)((bits >> i) & 1u);
+        const uint16_t z = (uint16_t)(b * x - r12);
+        o1[i] = z;
+        o2[i] = (uint16_t)(-((r02 + z) << 1) + b);
+        o3[i] = (uint16_t)(x - (r12 << 1));
+      }
+      u16x8_store(N.inbox + e, o1);
+      u16x8_store(P.corr_a + e, o2);
+      u16x8_store(P.corr_b + e, o3);
     }
   }
 }
@@ -329,32 +398,49 @@ __global__ __launch_bounds__(256) void inject_kernel(const Trio t, int step, uin
 __global__ __launch_bounds__(256) void lms_transpose32_kernel(const Trio t, uint32_t A) {
   const int p = party_of(t);
   const CircuitBufs& P = t.p[p];
-  const int lane = threadIdx.x & 63;
-  const size_t w = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (w >= t.W) return;
-  const size_t L = 64 * t.W, e = w * 64 + lane;
-  uint32_t ma = (uint32_t)P.mask_a[e] - ((uint32_t)P.corr_a[e] << 16) - ((uint32_t)P.corr_a[e + L] << 17);
-  uint32_t mb = (uint32_t)P.mask_b[e] - ((uint32_t)P.corr_b[e] << 16) - ((uint32_t)P.corr_b[e + L] << 17);
-  if (P.lifted_a) {
-    P.lifted_a[e] = ma;
-    P.lifted_b[e] = mb;
-  }
-  ma = ma * A - ((uint32_t)P.code_a[e] << 16);
-  mb = mb * A - ((uint32_t)P.code_b[e] << 16);
-  if (p == 0) ma -= 1u;
-  if (p == 1) mb -= 1u;
-  uint64_t mine = 0;
+  __shared__ uint64_t planes[64][64];  // planes 0..31: a, 32..63: b
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t w0 = (size_t)blockIdx.x * 64;
+  const size_t L = 64 * t.W;
+#pragma unroll 4
+  for (int k = 0; k < 16; k++) {
+    const int wl = wave * 16 + k;
+    const size_t w = w0 + wl;
+    uint32_t ma = 0, mb = 0;
+    if (w < t.W) {
+      const size_t e = w * 64 + lane;
+      ma = (uint32_t)P.mask_a[e] - ((uint32_t)P.corr_a[e] << 16) - ((uint32_t)P.corr_a[e + L] << 17);
+      mb = (uint32_t)P.mask_b[e] - ((uint32_t)P.corr_b[e] << 16) - ((uint32_t)P.corr_b[e + L] << 17);
+      if (P.lifted_a) {
+        P.lifted_a[e] = ma;
+        P.lifted_b[e] = mb;
+      }
+      ma = ma * A - ((uint32_t)P.code_a[e] << 16);
+      mb = mb * A - ((uint32_t)P.code_b[e] << 16);
+      if (p == 0) ma -= 1u;
+      if (p == 1) mb -= 1u;
+    }
+    uint64_t mine = 0;
 #pragma unroll
-  for (int j = 0; j < 32; j++) {
-    const uint64_t ba = __ballot((ma >> j) & 1u);
-    const uint64_t bb = __ballot((mb >> j) & 1u);
-    if (lane == j) mine = ba;
-    if (lane == 32 + j) mine = bb;
+    for (int j = 0; j < 32; j++) {
+      const uint64_t ba = __ballot((ma >> j) & 1u);
+      const uint64_t bb = __ballot((mb >> j) & 1u);
+      if (lane == j) mine = ba;
+      if (lane == 32 + j) mine = bb;
+    }
+    planes[lane][wl] = mine;
   }
-  if (lane < 32)
-    P.tx_a[(size_t)lane * t.W + w] = mine;
-  else
-    P.tx_b[(size_t)(lane - 32) * t.W + w] = mine;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int idx = (int)threadIdx.x + 256 * m, pl = idx >> 6, wl = idx & 63;
+    const size_t w = w0 + wl;
+    if (w >= t.W) continue;
+    if (pl < 32)
+      P.tx_a[(size_t)pl * t.W + w] = planes[pl][wl];
+    else
+      P.tx_b[(size_t)(pl - 32) * t.W + w] = planes[pl][wl];
+  }
 }
 
 // ------------------------------------------------------------ open -> bitmap
@@ -381,7 +467,7 @@ __global__ __launch_bounds__(256) void open_kernel(const OpenArgs a) {
 
 // ------------------------------------------------------------ launchers
 void launch_transpose16(const Trio& t, int np, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(t.W, 4), (unsigned)np);
+  dim3 grid((unsigned)ceil_div(t.W, 64), (unsigned)np);
   transpose16_kernel<<<grid, 256, 0, s>>>(t);
   IRIS_HIP_CHECK(hipGetLastError());
 }
@@ -401,13 +487,13 @@ void launch_ripple(const Trio& t, int np, bool msb, RippleMode mode, int k, uint
 }
 
 void launch_inject(const Trio& t, int np, int step, uint64_t ctr, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(128 * t.W / 32, 256), (unsigned)np);
+  dim3 grid((unsigned)ceil_div(128 * t.W, 8192), (unsigned)np);
   inject_kernel<<<grid, 256, 0, s>>>(t, step, ctr);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 
 void launch_lms_transpose32(const Trio& t, int np, uint32_t A, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(t.W, 4), (unsigned)np);
+  dim3 grid((unsigned)ceil_div(t.W, 64), (unsigned)np);
   lms_transpose32_kernel<<<grid, 256, 0, s>>>(t, A);
   IRIS_HIP_CHECK(hipGetLastError());
 }

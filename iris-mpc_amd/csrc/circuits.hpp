@@ -17,7 +17,6 @@ struct CircuitBufs {
   uint64_t *cl_a, *cl_b;    // 16 W: lift carry-save carries
   uint64_t *c_a, *c_b;      // 2 W: ripple carry; after the lift: (bit 16, bit 17)
   uint64_t *tmp_a, *tmp_b;  // 2 W: ping-pong exchange slots of the ripple rounds
-  uint16_t* msg_own;        // 128 W: this party's bit-inject message (y, x or z)
   uint16_t* inbox;          // 128 W: bit-inject message received from prev
   uint16_t *corr_a, *corr_b;  // 128 W: injected carries (arithmetic u16 shares)
   uint64_t *tx_a, *tx_b;    // 32 W: transposed lifted comparison values

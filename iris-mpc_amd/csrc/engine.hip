@@ -375,7 +375,7 @@ Circuits::Circuits(int device, int peer_id, size_t input_size, size_t alloc_size
   tx_a_.alloc(32 * A); tx_b_.alloc(32 * A);
   cm_a_.alloc(31 * A); cm_b_.alloc(31 * A);
   res_a_.alloc(A); res_b_.alloc(A); open_c_.alloc(A);
-  msg_own_.alloc(128 * A); inbox_.alloc(128 * A);
+  inbox_.alloc(128 * A);
   corr_a_.alloc(128 * A); corr_b_.alloc(128 * A);
 }
 
@@ -390,7 +390,7 @@ CircuitBufs Circuits::bufs(const uint16_t* ca, const uint16_t* cb, const uint16_
   b.code_a = ca; b.code_b = cb; b.mask_a = ma; b.mask_b = mb;
   b.tm_a = tm_a_.ptr; b.tm_b = tm_b_.ptr; b.cl_a = cl_a_.ptr; b.cl_b = cl_b_.ptr;
   b.c_a = c_a_.ptr; b.c_b = c_b_.ptr; b.tmp_a = tmp_a_.ptr; b.tmp_b = tmp_b_.ptr;
-  b.msg_own = msg_own_.ptr; b.inbox = inbox_.ptr; b.corr_a = corr_a_.ptr; b.corr_b = corr_b_.ptr;
+  b.inbox = inbox_.ptr; b.corr_a = corr_a_.ptr; b.corr_b = corr_b_.ptr;
   b.tx_a = tx_a_.ptr; b.tx_b = tx_b_.ptr; b.cm_a = cm_a_.ptr; b.cm_b = cm_b_.ptr;
   b.res_a = res_a_.ptr; b.res_b = res_b_.ptr;
   b.lifted_a = lifted_dbg_a; b.lifted_b = lifted_dbg_b;

@@ -201,7 +201,7 @@ class Circuits {
   Exchange* ex_;
   DevBuf<uint64_t> tm_a_, tm_b_, cl_a_, cl_b_, c_a_, c_b_, tmp_a_, tmp_b_, tx_a_, tx_b_, cm_a_,
       cm_b_, res_a_, res_b_, open_c_;
-  DevBuf<uint16_t> msg_own_, inbox_, corr_a_, corr_b_;
+  DevBuf<uint16_t> inbox_, corr_a_, corr_b_;
 };
 
 // DistanceComparator analogue (dot/distance_comparator.rs).
@@ -224,6 +224,7 @@ class DistanceComparator {
   uint32_t* partial_results_left() const { return prl_.ptr; }
   uint32_t* partial_results_right() const { return prr_.ptr; }
   size_t query_length() const { return query_length_; }
+  size_t max_db_size() const { return max_db_size_; }
   void reset(hipStream_t s);
 
  private:
