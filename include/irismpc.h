@@ -100,6 +100,16 @@ int irismpc_db_write_at_index(irismpc_share_db* e, irismpc_db* db, const irismpc
  * Uniform u16 shares have the distribution of real Shamir shares. */
 int irismpc_db_fill_random(irismpc_share_db* e, irismpc_db* db, size_t index, size_t n,
                            const uint32_t key[8], uint64_t nonce, void* stream);
+/* u16 rows already on the device (dev_rows, n x code_length) -> DB rows
+ * [index, index + n); stream-ordered. No reference counterpart (the reference
+ * loads from host memory only); used to load device-generated workloads. */
+int irismpc_db_load_records_device(irismpc_share_db* e, irismpc_db* db, size_t index, size_t n,
+                                   const uint16_t* dev_rows, void* stream);
+/* Copies DB rows [src, src + n) to [dst, dst + n) on the device (multiples of
+ * 16, non-overlapping). No reference counterpart: tiles a set of synthetic
+ * irises across a large DB. */
+int irismpc_db_replicate_records(irismpc_share_db* e, irismpc_db* db, size_t src, size_t n,
+                                 size_t dst, void* stream);
 /* Reads DB rows back as u16 (for tests). */
 int irismpc_db_read_records(irismpc_share_db* e, const irismpc_db* db, size_t index, size_t n,
                             uint16_t* out, void* stream);
@@ -231,6 +241,27 @@ int irismpc_scan_db_trio(irismpc_share_db* const codes[3], irismpc_share_db* con
                          const irismpc_query* const q_code[3], const irismpc_query* const q_mask[3],
                          size_t db_size, size_t batch_size, uint32_t threshold_a,
                          uint64_t* const bitmaps[3], void* const streams[3]);
+
+/* ------------------------------------------------ synthetic workloads */
+/* No reference counterpart: the data generator behind the benchmarks and the
+ * full-size tests (the role of iris_db::db::IrisDB::new_random_rng and
+ * GaloisRingSharedIris::generate_shares_locally in the reference's benches).
+ * ChaCha20 streams under the all-zero key; see csrc/synth.hip for the stream
+ * definitions. codes/masks are [n][200] u64 (bit i of word i / 64). */
+/* Host: plaintext iris (dataset, index). */
+int irismpc_synth_iris(uint64_t dataset, uint64_t index, uint64_t code[200], uint64_t mask[200]);
+/* Host: flips n_flips distinct code bits in place (draws from (dataset, index)). */
+int irismpc_synth_flip(uint64_t code[200], uint64_t dataset, uint64_t index, uint32_t n_flips);
+/* Device: plaintext irises index0 .. index0 + n - 1 into device arrays. */
+int irismpc_synth_plain_device(uint64_t dataset, uint64_t index0, size_t n, uint64_t* codes,
+                               uint64_t* masks, void* stream);
+/* Device: the three parties' shares of n device-resident plaintext irises
+ * (iris i shares with randomness (rand_dataset, rand_index0 + i)): code_out[p]
+ * [n][12800] u16, mask_out[p] [n][6400] u16 (trimmed mask). */
+int irismpc_synth_shares_device(const uint64_t* codes, const uint64_t* masks, size_t n,
+                                uint64_t rand_dataset, uint64_t rand_index0,
+                                uint16_t* const code_out[3], uint16_t* const mask_out[3],
+                                void* stream);
 
 #ifdef __cplusplus
 }

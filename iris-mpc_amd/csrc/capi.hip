@@ -7,6 +7,7 @@
 #include "comparator.hpp"
 #include "engine.hpp"
 #include "host_crypto.hpp"
+#include "synth.hpp"
 
 using namespace irismpc;
 
@@ -197,6 +198,38 @@ int irismpc_db_fill_random(irismpc_share_db* e, irismpc_db* db, size_t index, si
     IRIS_REQUIRE(index + round_up(n, 16) <= db->impl->capacity, "fill beyond DB capacity");
     launch_fill_random(db->impl->limbs.ptr, index, n, e->impl->code_length(), key_of(key), nonce,
                        db->impl->sum0.ptr, db->impl->sum1.ptr, st(stream));
+  });
+}
+int irismpc_db_load_records_device(irismpc_share_db* e, irismpc_db* db, size_t index, size_t n,
+                                   const uint16_t* dev_rows, void* stream) {
+  return guarded([&] {
+    DevSet g(e->device);
+    e->impl->load_records_device(*db->impl, index, n, dev_rows, st(stream));
+  });
+}
+int irismpc_db_replicate_records(irismpc_share_db* e, irismpc_db* db, size_t src, size_t n,
+                                 size_t dst, void* stream) {
+  return guarded([&] {
+    DevSet g(e->device);
+    e->impl->replicate_records(*db->impl, src, n, dst, st(stream));
+  });
+}
+int irismpc_synth_iris(uint64_t dataset, uint64_t index, uint64_t code[200], uint64_t mask[200]) {
+  return guarded([&] { synth_iris_host(dataset, index, code, mask); });
+}
+int irismpc_synth_flip(uint64_t code[200], uint64_t dataset, uint64_t index, uint32_t n_flips) {
+  return guarded([&] { synth_flip_host(code, dataset, index, n_flips); });
+}
+int irismpc_synth_plain_device(uint64_t dataset, uint64_t index0, size_t n, uint64_t* codes,
+                               uint64_t* masks, void* stream) {
+  return guarded([&] { launch_synth_plain(dataset, index0, n, codes, masks, st(stream)); });
+}
+int irismpc_synth_shares_device(const uint64_t* codes, const uint64_t* masks, size_t n,
+                                uint64_t rand_dataset, uint64_t rand_index0,
+                                uint16_t* const code_out[3], uint16_t* const mask_out[3],
+                                void* stream) {
+  return guarded([&] {
+    launch_synth_shares(codes, masks, n, rand_dataset, rand_index0, code_out, mask_out, st(stream));
   });
 }
 int irismpc_query_upload(irismpc_share_db* e, const uint16_t* rows, size_t n_rows, void* stream,

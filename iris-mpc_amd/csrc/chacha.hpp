@@ -11,7 +11,7 @@
 
 namespace irismpc {
 
-__device__ __forceinline__ uint32_t rotl32(uint32_t x, int n) {
+__host__ __device__ __forceinline__ uint32_t rotl32(uint32_t x, int n) {
   return __builtin_rotateleft32(x, n);
 }
 
@@ -23,7 +23,7 @@ __device__ __forceinline__ uint32_t rotl32(uint32_t x, int n) {
 
 // Writes the 16-word block for (key, nonce, counter) into out[].
 template <int ROUNDS>
-__device__ __forceinline__ void chacha_block(const ChaChaKey& key, uint64_t nonce,
+__host__ __device__ __forceinline__ void chacha_block(const ChaChaKey& key, uint64_t nonce,
                                              uint64_t counter, uint32_t out[16]) {
   const uint32_t i0 = 0x61707865u, i1 = 0x3320646eu, i2 = 0x79622d32u, i3 = 0x6b206574u;
   const uint32_t i12 = (uint32_t)counter, i13 = (uint32_t)(counter >> 32);

@@ -141,6 +141,26 @@ void ShareDB::load_records_u16(DbSlice& db, size_t index, size_t n, const uint16
   }
 }
 
+void ShareDB::load_records_device(DbSlice& db, size_t index, size_t n, const uint16_t* dev_rows,
+                                  hipStream_t s) const {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(index + n <= db.capacity - kMaxBD, "record index beyond DB capacity");
+  launch_tile_rows(dev_rows, n, code_length_, db.limbs.ptr, index, db.sum0.ptr, db.sum1.ptr, s);
+}
+
+void ShareDB::replicate_records(DbSlice& db, size_t src, size_t n, size_t dst, hipStream_t s) const {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(src % 16 == 0 && dst % 16 == 0 && n % 16 == 0, "replicate needs 16-row groups");
+  IRIS_REQUIRE(src + n <= dst || dst + n <= src, "replicate regions overlap");
+  IRIS_REQUIRE(std::max(src, dst) + n <= db.capacity - kMaxBD, "replicate beyond DB capacity");
+  // a 16-row group is (K / 64) * 2 fragments of 1 KiB, contiguous per group
+  const size_t group_bytes = (size_t)code_length_ / kKTile * 2 * kFragBytes;
+  IRIS_HIP_CHECK(hipMemcpyAsync(db.limbs.ptr + dst / 16 * group_bytes, db.limbs.ptr + src / 16 * group_bytes,
+                                n / 16 * group_bytes, hipMemcpyDeviceToDevice, s));
+  IRIS_HIP_CHECK(hipMemcpyAsync(db.sum0.ptr + dst, db.sum0.ptr + src, n * 4, hipMemcpyDeviceToDevice, s));
+  IRIS_HIP_CHECK(hipMemcpyAsync(db.sum1.ptr + dst, db.sum1.ptr + src, n * 4, hipMemcpyDeviceToDevice, s));
+}
+
 void ShareDB::load_records_limbs(DbSlice& db, size_t index, size_t n, const uint8_t* l0,
                                  const uint8_t* l1, hipStream_t s) const {
   DeviceGuard g(device_);

@@ -119,6 +119,11 @@ class ShareDB {
                           hipStream_t s) const;
   void write_at_index(DbSlice& db, const QuerySet& q, size_t src_index, size_t dst_index,
                       hipStream_t s) const;
+  // n u16 rows already on the device -> HBM limbs at `index` (stream-ordered)
+  void load_records_device(DbSlice& db, size_t index, size_t n, const uint16_t* dev_rows,
+                           hipStream_t s) const;
+  // copies rows [src, src + n) to [dst, dst + n) (multiples of 16, no overlap)
+  void replicate_records(DbSlice& db, size_t src, size_t n, size_t dst, hipStream_t s) const;
   // preprocess_query + htod + query_sums: rows x code_length u16 (already rotated
   // and Lagrange-preprocessed, like the reference's CompactQuery).
   std::unique_ptr<QuerySet> upload_query(const uint16_t* host_rows, size_t n_rows, hipStream_t s) const;

@@ -50,6 +50,12 @@ SIGNATURES = {
     "irismpc_db_write_at_index": [vp, vp, vp, sz, sz, vp],
     "irismpc_db_read_records": [vp, vp, sz, sz, vp, vp],
     "irismpc_db_fill_random": [vp, vp, sz, sz, vp, u64, vp],
+    "irismpc_db_load_records_device": [vp, vp, sz, sz, vp, vp],
+    "irismpc_db_replicate_records": [vp, vp, sz, sz, sz, vp],
+    "irismpc_synth_iris": [u64, u64, vp, vp],
+    "irismpc_synth_flip": [vp, u64, u64, u32],
+    "irismpc_synth_plain_device": [u64, u64, sz, vp, vp, vp],
+    "irismpc_synth_shares_device": [vp, vp, sz, u64, u64, vp, vp, vp],
     "irismpc_query_upload": [vp, vp, sz, vp, vp],
     "irismpc_query_upload_shares": [vp, vp, sz, vp, vp],
     "irismpc_query_destroy": [vp],
@@ -87,9 +93,21 @@ SIGNATURES = {
 }
 
 
+def _init_torch_runtime():
+    """PyTorch-ROCm carries its own HIP runtime next to the one this library
+    links; it has to come up first in the process (the other order leaves the
+    library's runtime without a device), so bring it up before loading."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
+
+
 def lib():
     global _lib
     if _lib is None:
+        _init_torch_runtime()
         if not os.path.exists(LIB_PATH):
             raise IrisMpcError(
                 f"native library {LIB_PATH} is missing: run __graft_entry__.build() "

@@ -110,6 +110,14 @@ class SlicedProcessedDatabase:
         k, kp = _u32x8(key)
         call("irismpc_db_fill_random", self.engine.h, self.h, index, n, kp, nonce, stream)
 
+    def load_records_device(self, index, n, dev_rows_ptr, stream=0):
+        """n u16 rows already on the device -> DB rows [index, index + n)."""
+        call("irismpc_db_load_records_device", self.engine.h, self.h, index, n, dev_rows_ptr, stream)
+
+    def replicate_records(self, src, n, dst, stream=0):
+        """Rows [src, src + n) -> [dst, dst + n) on the device (16-row multiples)."""
+        call("irismpc_db_replicate_records", self.engine.h, self.h, src, n, dst, stream)
+
     def read_records(self, index, n, stream=0):
         out = np.zeros((n, self.engine.code_length), np.uint16)
         call("irismpc_db_read_records", self.engine.h, self.h, index, n, out.ctypes.data, stream)
