@@ -40,6 +40,7 @@ typedef struct irismpc_db irismpc_db;
 typedef struct irismpc_query irismpc_query;
 typedef struct irismpc_circuits irismpc_circuits;
 typedef struct irismpc_comparator irismpc_comparator;
+typedef struct irismpc_graph irismpc_graph;
 
 /* ------------------------------------------------------------- misc */
 /* Thread-local text of the last failure (empty string if none). */
@@ -124,6 +125,11 @@ int irismpc_query_upload(irismpc_share_db* e, const uint16_t* rows, size_t n_row
  * (galois_engine.rs:23-40) and the 31 rotations (galois_engine.rs:42-52,
  * 479-488) are applied on the GPU; rows beyond batch*31 are zero padding up
  * to query_length (actor.rs:895-901 pads to max_batch_size). */
+/* The same with exactly `rows` query rows (batch * 31 <= rows; rows may
+ * exceed the engine's query_length): a packed query set of the batching form,
+ * or a batch without padding. */
+int irismpc_query_upload_shares_rows(irismpc_share_db* e, const uint16_t* shares, size_t batch,
+                                     size_t rows, void* stream, irismpc_query** out);
 int irismpc_query_upload_shares(irismpc_share_db* e, const uint16_t* shares, size_t batch,
                                 void* stream, irismpc_query** out);
 int irismpc_query_destroy(irismpc_query* q);
@@ -135,12 +141,27 @@ int irismpc_share_db_dot_reduce(irismpc_share_db* e, const irismpc_query* q, con
                                 size_t chunk_size, size_t offset, uint16_t multiplier,
                                 void* stream);
 /* Whole-range form: rows [offset, offset+total) in chunks of chunk_size rows
- * in one launch; chunk c is written at out + c * chunk_size * query_length. */
+ * in one launch; chunk c is written at out + c * chunk_size * Qr, Qr = the
+ * query set's rows. No reference counterpart: one launch in place of the
+ * per-chunk dot/dot_reduce pair (share_db.rs:800-905), with the masks at the
+ * counters those calls would use. */
 int irismpc_share_db_dot_reduce_range(irismpc_share_db* e, const irismpc_query* q,
                                       const irismpc_db* db, size_t offset, size_t total,
                                       size_t chunk_size, uint16_t multiplier, uint16_t* out,
                                       void* stream);
 /* ShareDB::reshare_results (share_db.rs:1004-1036): collective. */
+/* Batching form. n_batches query batches packed back to back in q (rows[i]
+ * rows each, sum = the query set's rows; at most 64) in one launch: batch i's
+ * results go to outs[i] with the layout of its own
+ * irismpc_share_db_dot_reduce_range. Masks: with ctr_first == NULL the
+ * counters those calls would use in order (the engine counter advances);
+ * otherwise batch i starts at ctr_first[i] and the engine counter is left
+ * alone. No reference counterpart (the reference runs one batch per launch). */
+int irismpc_share_db_dot_reduce_packed(irismpc_share_db* e, const irismpc_query* q,
+                                       const irismpc_db* db, size_t offset, size_t total,
+                                       size_t chunk_size, uint16_t multiplier,
+                                       const size_t* rows, uint16_t* const* outs, int n_batches,
+                                       const uint64_t* ctr_first, void* stream);
 int irismpc_share_db_reshare_results(irismpc_share_db* e, size_t chunk_size, void* stream);
 int irismpc_share_db_reshare_results_trio(irismpc_share_db* const e[3], size_t chunk_size,
                                           void* const streams[3]);
@@ -241,6 +262,18 @@ int irismpc_scan_db_trio(irismpc_share_db* const codes[3], irismpc_share_db* con
                          const irismpc_query* const q_code[3], const irismpc_query* const q_mask[3],
                          size_t db_size, size_t batch_size, uint32_t threshold_a,
                          uint64_t* const bitmaps[3], void* const streams[3]);
+
+/* ------------------------------------------------ HIP graphs */
+/* Captures the library calls issued on `stream` between begin and end into a
+ * graph (hipStreamCaptureModeThreadLocal) and instantiates it; launch replays
+ * it. Captured calls must not allocate or synchronize (share GEMM launches,
+ * packed launches and circuit steps qualify; the library clears memory with
+ * fill kernels, never hipMemsetAsync). Engine counters advance once, at
+ * capture: every replay reuses the captured masks. No reference counterpart. */
+int irismpc_graph_capture_begin(void* stream);
+int irismpc_graph_capture_end(void* stream, irismpc_graph** out);
+int irismpc_graph_launch(irismpc_graph* g, void* stream);
+int irismpc_graph_destroy(irismpc_graph* g);
 
 /* ------------------------------------------------ synthetic workloads */
 /* No reference counterpart: the data generator behind the benchmarks and the

@@ -1,5 +1,6 @@
 // extern "C" boundary of libirismpc (declared in include/irismpc.h).
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "../../include/irismpc.h"
@@ -214,6 +215,34 @@ int irismpc_db_replicate_records(irismpc_share_db* e, irismpc_db* db, size_t src
     e->impl->replicate_records(*db->impl, src, n, dst, st(stream));
   });
 }
+struct irismpc_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~irismpc_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+int irismpc_graph_capture_begin(void* stream) {
+  return guarded([&] {
+    IRIS_REQUIRE(stream != nullptr, "capture needs a non-default stream");
+    IRIS_HIP_CHECK(hipStreamBeginCapture(st(stream), hipStreamCaptureModeThreadLocal));
+  });
+}
+int irismpc_graph_capture_end(void* stream, irismpc_graph** out) {
+  return guarded([&] {
+    auto g = std::make_unique<irismpc_graph>();
+    IRIS_HIP_CHECK(hipStreamEndCapture(st(stream), &g->graph));
+    IRIS_HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+  });
+}
+int irismpc_graph_launch(irismpc_graph* g, void* stream) {
+  return guarded([&] { IRIS_HIP_CHECK(hipGraphLaunch(g->exec, st(stream))); });
+}
+int irismpc_graph_destroy(irismpc_graph* g) {
+  return guarded([&] { delete g; });
+}
 int irismpc_synth_iris(uint64_t dataset, uint64_t index, uint64_t code[200], uint64_t mask[200]) {
   return guarded([&] { synth_iris_host(dataset, index, code, mask); });
 }
@@ -248,6 +277,14 @@ int irismpc_query_upload_shares(irismpc_share_db* e, const uint16_t* shares, siz
     *out = h;
   });
 }
+int irismpc_query_upload_shares_rows(irismpc_share_db* e, const uint16_t* shares, size_t batch,
+                                     size_t rows, void* stream, irismpc_query** out) {
+  return guarded([&] {
+    auto* h = new irismpc_query();
+    h->impl = e->impl->upload_query_shares_rows(shares, batch, rows, st(stream));
+    *out = h;
+  });
+}
 int irismpc_query_destroy(irismpc_query* q) {
   return guarded([&] { delete q; });
 }
@@ -265,6 +302,16 @@ int irismpc_share_db_dot_reduce_range(irismpc_share_db* e, const irismpc_query* 
   return guarded([&] {
     e->impl->dot_reduce_range(*q->impl, *db->impl, offset, total, chunk_size, multiplier, out,
                               st(stream));
+  });
+}
+int irismpc_share_db_dot_reduce_packed(irismpc_share_db* e, const irismpc_query* q,
+                                       const irismpc_db* db, size_t offset, size_t total,
+                                       size_t chunk_size, uint16_t multiplier,
+                                       const size_t* rows, uint16_t* const* outs, int n_batches,
+                                       const uint64_t* ctr_first, void* stream) {
+  return guarded([&] {
+    e->impl->dot_reduce_packed(*q->impl, *db->impl, offset, total, chunk_size, multiplier, rows,
+                               outs, n_batches, ctr_first, st(stream));
   });
 }
 int irismpc_share_db_reshare_results(irismpc_share_db* e, size_t chunk_size, void* stream) {

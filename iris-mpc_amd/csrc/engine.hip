@@ -218,16 +218,29 @@ std::unique_ptr<QuerySet> ShareDB::upload_query(const uint16_t* host_rows, size_
 
 std::unique_ptr<QuerySet> ShareDB::upload_query_shares(const uint16_t* host_shares, size_t batch,
                                                        hipStream_t s) const {
+  return upload_query_shares_rows(host_shares, batch, query_length_, s);
+}
+
+std::unique_ptr<QuerySet> ShareDB::upload_query_shares_rows(const uint16_t* host_shares, size_t batch,
+                                                            size_t n_rows, hipStream_t s) const {
   DeviceGuard g(device_);
-  IRIS_REQUIRE(batch >= 1 && batch * kRotations <= query_length_, "batch exceeds query_length / 31");
+  if (n_rows == 0) n_rows = query_length_;
+  IRIS_REQUIRE(batch >= 1 && batch * kRotations <= n_rows, "batch exceeds the query rows / 31");
   const size_t K = (size_t)code_length_;
-  DevBuf<uint16_t> raw(batch * K, false), rows(query_length_ * K, false);
+  DevBuf<uint16_t> raw(batch * K, false), rows(n_rows * K, false);
   IRIS_HIP_CHECK(hipMemcpyAsync(raw.ptr, host_shares, raw.n * 2, hipMemcpyHostToDevice, s));
-  launch_query_prep(raw.ptr, (int)batch, code_length_, peer_id_, rows.ptr, (int)query_length_, s);
-  auto q = upload_query_device(rows.ptr, query_length_, s);
+  launch_query_prep(raw.ptr, (int)batch, code_length_, peer_id_, rows.ptr, (int)n_rows, s);
+  auto q = std::make_unique<QuerySet>();
+  q->code_length = code_length_;
+  q->rows = (int)n_rows;
+  const size_t padded = round_up(n_rows, kMaxBQ);
+  q->limbs.alloc(padded * K * 2);
+  q->sum0.alloc(padded);
+  q->sum1.alloc(padded);
+  launch_tile_rows(rows.ptr, n_rows, code_length_, q->limbs.ptr, 0, q->sum0.ptr, q->sum1.ptr, s);
   if (code_length_ % 800 == 0) {
-    q->rot.alloc(rot_planes_bytes((int)query_length_, code_length_));
-    launch_rot_planes(rows.ptr, (int)query_length_, code_length_, q->rot.ptr, s);
+    q->rot.alloc(rot_planes_bytes((int)n_rows, code_length_));
+    launch_rot_planes(rows.ptr, (int)n_rows, code_length_, q->rot.ptr, s);
   }
   IRIS_HIP_CHECK(hipStreamSynchronize(s));
   return q;
@@ -296,25 +309,71 @@ void ShareDB::dot_reduce_range(const QuerySet& q, const DbSlice& db, size_t offs
   const int variant = resolve_variant(q);
   GemmArgs a = gemm_args(q, db, variant);
   const GemmTile t = gemm_tile(variant);
+  const size_t qr = (size_t)q.rows;
   a.out = out;
-  a.out_chunk_stride = chunk * query_length_;
+  a.out_chunk_stride = chunk * qr;
   a.n_dtiles = (int)ceil_div(total, (size_t)t.bd);
   a.db_begin = offset;
   a.db_end = offset + total;
   a.chunk = chunk;
   a.ctr_first = counter_;
   // per full chunk: mask keystream + reshare OTP keystream
-  a.ctr_per_chunk = 2 * ceil_div(chunk * query_length_ * 2, 64);
+  a.ctr_per_chunk = 2 * ceil_div(chunk * qr * 2, 64);
   a.nonce = nonce_;
   a.key_own = seed_own_;
   a.key_prev = seed_prev_;
   a.mult = multiplier;
   a.masked = is_remote() ? 1 : 0;
   launch_share_gemm(a, variant, s);
-  if (is_remote()) {
-    for (size_t c0 = 0; c0 < total; c0 += chunk)
-      counter_ += 2 * ceil_div(std::min(chunk, total - c0) * query_length_ * 2, 64);
+  if (is_remote()) counter_ += range_counter_advance(total, chunk, qr);
+}
+
+uint64_t ShareDB::range_counter_advance(size_t total, size_t chunk, size_t rows) {
+  uint64_t n = 0;
+  for (size_t c0 = 0; c0 < total; c0 += chunk) n += 2 * ceil_div(std::min(chunk, total - c0) * rows * 2, 64);
+  return n;
+}
+
+void ShareDB::dot_reduce_packed(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
+                                size_t chunk, uint16_t multiplier, const size_t* rows,
+                                uint16_t* const* outs, int n_batches, const uint64_t* ctr_first,
+                                hipStream_t s) {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(q.code_length == code_length_ && db.code_length == code_length_, "code length mismatch");
+  IRIS_REQUIRE(total > 0 && total % 64 == 0 && chunk % 256 == 0, "bad range geometry");
+  IRIS_REQUIRE(offset % 16 == 0 && offset + total <= db.capacity - kMaxBD, "range beyond DB");
+  IRIS_REQUIRE(n_batches >= 1 && n_batches <= kMaxSegments, "1 .. 64 batches per packed launch");
+  const int variant = resolve_variant(q);
+  GemmArgs a = gemm_args(q, db, variant);
+  const GemmTile t = gemm_tile(variant);
+  size_t row = 0;
+  uint64_t ctr = counter_;
+  for (int i = 0; i < n_batches; i++) {
+    IRIS_REQUIRE(rows[i] > 0 && outs[i] != nullptr, "empty batch in a packed launch");
+    GemmSegment& sg = a.seg[i];
+    sg.out = outs[i];
+    sg.out_chunk_stride = chunk * rows[i];
+    sg.ctr_first = ctr_first ? ctr_first[i] : ctr;
+    sg.ctr_per_chunk = 2 * ceil_div(chunk * rows[i] * 2, 64);
+    sg.row_begin = (int)row;
+    sg.rows = (int)rows[i];
+    row += rows[i];
+    if (is_remote()) ctr += range_counter_advance(total, chunk, rows[i]);
   }
+  IRIS_REQUIRE(row == (size_t)q.rows, "batch rows must add up to the packed query rows");
+  a.n_seg = n_batches;
+  a.out = outs[0];
+  a.n_dtiles = (int)ceil_div(total, (size_t)t.bd);
+  a.db_begin = offset;
+  a.db_end = offset + total;
+  a.chunk = chunk;
+  a.nonce = nonce_;
+  a.key_own = seed_own_;
+  a.key_prev = seed_prev_;
+  a.mult = multiplier;
+  a.masked = is_remote() ? 1 : 0;
+  launch_share_gemm(a, variant, s);
+  if (!ctr_first) counter_ = ctr;
 }
 
 namespace {

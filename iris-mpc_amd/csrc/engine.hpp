@@ -133,6 +133,10 @@ class ShareDB {
   // are applied on the GPU; rows are padded to query_length.
   std::unique_ptr<QuerySet> upload_query_shares(const uint16_t* host_shares, size_t batch,
                                                 hipStream_t s) const;
+  // Same with exactly `rows` query rows (batch * 31 <= rows; 0 = query_length),
+  // e.g. several batches' queries packed back to back for one launch.
+  std::unique_ptr<QuerySet> upload_query_shares_rows(const uint16_t* host_shares, size_t batch,
+                                                     size_t rows, hipStream_t s) const;
   // dot + dot_reduce_and_multiply (fused): DB rows [offset, offset + chunk)
   // against all query rows; chunk is the padded chunk size (multiple of 64).
   void dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
@@ -141,6 +145,15 @@ class ShareDB {
   // `chunk` rows, results written chunk after chunk into `out`.
   void dot_reduce_range(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
                         size_t chunk, uint16_t multiplier, uint16_t* out, hipStream_t s);
+  // Several batches packed in one query set (rows[i] rows each, back to back)
+  // in one launch; batch i's results go to outs[i] with the layout and mask
+  // counters its own dot_reduce_range would have used, in order.
+  // ctr_first (optional) gives each batch's first ChaCha block explicitly and
+  // leaves the engine counter alone (for plans that interleave batches).
+  void dot_reduce_packed(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
+                         size_t chunk, uint16_t multiplier, const size_t* rows,
+                         uint16_t* const* outs, int n_batches, const uint64_t* ctr_first,
+                         hipStream_t s);
   void reshare_results(size_t chunk, hipStream_t s);  // collective
   static void reshare_trio(ShareDB* const d[3], size_t chunk, hipStream_t const s[3]);
   uint16_t* results() const { return results_.ptr; }
@@ -152,6 +165,8 @@ class ShareDB {
   uint64_t counter() const { return counter_; }
   void set_counter(uint64_t c) { counter_ = c; }
   bool is_remote() const { return ex_ != nullptr; }
+  // ChaCha blocks a remote engine's dot_reduce_range consumes (mask + OTP per chunk)
+  static uint64_t range_counter_advance(size_t total, size_t chunk, size_t rows);
   // GEMM tile variant; kAutoVariant picks the rotation-aware kernel for query
   // sets with rotation structure and the best generic one otherwise
   static constexpr int kAutoVariant = -1;

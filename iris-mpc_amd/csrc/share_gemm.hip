@@ -297,12 +297,29 @@ __device__ __forceinline__ void epilogue_store(const GemmArgs& a, const uint16_t
     const size_t local = rl - c * a.chunk;
     const size_t rem = (a.db_end - a.db_begin) - c * a.chunk;
     const size_t w = rem < a.chunk ? rem : a.chunk;
-    const size_t idx = (size_t)q * w + local;
+    uint16_t* out = a.out;
+    size_t stride = a.out_chunk_stride;
+    uint64_t ctr0 = a.ctr_first, cpc = a.ctr_per_chunk;
+    int qq = q;
+    if (a.n_seg) {  // last segment starting at or before q
+      int lo = 0, hi = a.n_seg - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.seg[mid].row_begin <= q) lo = mid;
+        else hi = mid - 1;
+      }
+      out = a.seg[lo].out;
+      stride = a.seg[lo].out_chunk_stride;
+      ctr0 = a.seg[lo].ctr_first;
+      cpc = a.seg[lo].ctr_per_chunk;
+      qq = q - a.seg[lo].row_begin;
+    }
+    const size_t idx = (size_t)qq * w + local;
     uint32_t v[16];
     __builtin_memcpy(v, &tile[ql * BD + seg * 32], 64);
     if (a.masked) {
       uint32_t ko[16], kp[16];
-      const uint64_t ctr = a.ctr_first + c * a.ctr_per_chunk + idx / 32;
+      const uint64_t ctr = ctr0 + c * cpc + idx / 32;
       chacha_block<12>(a.key_own, a.nonce, ctr, ko);
       chacha_block<12>(a.key_prev, a.nonce, ctr, kp);
 #pragma unroll
@@ -318,7 +335,7 @@ __device__ __forceinline__ void epilogue_store(const GemmArgs& a, const uint16_t
         v[k] = (lo & 0xffff) | (hi << 16);
       }
     }
-    uint4* dst = reinterpret_cast<uint4*>(a.out + c * a.out_chunk_stride + idx);
+    uint4* dst = reinterpret_cast<uint4*>(out + c * stride + idx);
 #pragma unroll
     for (int k = 0; k < 4; k++) dst[k] = uint4{v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]};
     if (a.out_peer) {
@@ -1188,6 +1205,8 @@ void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s) {
   IRIS_REQUIRE(a.n_qtiles == (int)ceil_div((size_t)a.qr, (size_t)t.bq), "n_qtiles mismatch");
   IRIS_REQUIRE(!is_rot_variant(variant) || (a.qrot != nullptr && (a.n_kt * kKTile) % 800 == 0),
                "rotation-aware variant needs a query uploaded from raw shares");
+  IRIS_REQUIRE(a.n_seg >= 0 && a.n_seg <= kMaxSegments, "too many batches in one packed launch");
+  IRIS_REQUIRE(a.n_seg == 0 || a.out_peer == nullptr, "packed launches write no peer copy");
   IRIS_REQUIRE(a.n_dtiles == (int)ceil_div(a.db_end - a.db_begin, (size_t)t.bd), "n_dtiles mismatch");
   const int T = a.n_qtiles * a.n_dtiles;
   const unsigned grid = (unsigned)(8 * ((T + 7) / 8));

@@ -13,6 +13,19 @@ namespace irismpc {
 // into chunks of `chunk` rows, chunk c has width
 // w_c = min(chunk, (db_end - db_begin) - c*chunk) and the result element index
 // inside the chunk is idx = q * w_c + (r - db_begin - c*chunk).
+// A packed launch carries several query batches back to back. Each keeps the
+// output array, result layout (idx = q * w_c + local with its own q) and mask
+// counters of its own plain launch, so the shares are those of per-batch runs.
+constexpr int kMaxSegments = 64;
+struct GemmSegment {
+  uint16_t* out;            // chunk c at out + c * out_chunk_stride
+  size_t out_chunk_stride;  // chunk * rows
+  uint64_t ctr_first;       // ChaCha block counter of the batch's chunk 0
+  uint64_t ctr_per_chunk;
+  int row_begin;            // first row of the batch in the packed query set
+  int rows;
+};
+
 struct GemmArgs {
   const int8_t* q;      // tiled query limbs (rows padded to the q tile)
   const int8_t* db;     // tiled DB limbs
@@ -37,6 +50,8 @@ struct GemmArgs {
   ChaChaKey key_own, key_prev;
   uint32_t mult;
   int masked;
+  int n_seg;                // 0: one batch (out, ctr_first, ...); else the table below
+  GemmSegment seg[kMaxSegments];
 };
 
 // Tiles u16 rows [n_rows][K] into the fragment layout at row offset row0 and

@@ -52,6 +52,12 @@ SIGNATURES = {
     "irismpc_db_fill_random": [vp, vp, sz, sz, vp, u64, vp],
     "irismpc_db_load_records_device": [vp, vp, sz, sz, vp, vp],
     "irismpc_db_replicate_records": [vp, vp, sz, sz, sz, vp],
+    "irismpc_query_upload_shares_rows": [vp, vp, sz, sz, vp, vp],
+    "irismpc_share_db_dot_reduce_packed": [vp, vp, vp, sz, sz, sz, u16, vp, vp, i32, vp, vp],
+    "irismpc_graph_capture_begin": [vp],
+    "irismpc_graph_capture_end": [vp, vp],
+    "irismpc_graph_launch": [vp, vp],
+    "irismpc_graph_destroy": [vp],
     "irismpc_synth_iris": [u64, u64, vp, vp],
     "irismpc_synth_flip": [vp, u64, u64, u32],
     "irismpc_synth_plain_device": [u64, u64, sz, vp, vp, vp],

@@ -1,0 +1,235 @@
+"""Batching form: query batches of uneven size packed into few launches.
+
+The plain form runs one batch (1-256 queries x 31 rotations) at a time: for
+every DB chunk one share-GEMM launch per matrix. A batch of a few queries
+reads the whole DB share for little MFMA work (the GEMM is HBM-bound below
+~7 queries), and the 4-query tiles of the rotation-aware kernel are only
+partly used for small batches.
+
+The batching form concatenates consecutive batches' query shares into one
+packed query set (up to `max_queries` queries, at most 64 batches). One launch
+per chunk and matrix then serves all of them. Each batch keeps the output
+layout and ChaCha12 mask counters of its own plain run (segment table of the
+packed launch), so both forms produce identical shares.
+
+The whole packed schedule is captured once into a HIP graph and replayed.
+The Autotuner times the GEMM variants and the pack size on the device and
+caches its choices.
+"""
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from .engine import Graph, gemm_rot_variants
+
+ROTATIONS = 31
+
+
+def log_uniform_batch_sizes(n, lo=1, hi=256, seed=0):
+    """n batch sizes from a seeded log-uniform distribution over [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    return np.clip(np.floor(np.exp(rng.uniform(math.log(lo), math.log(hi + 1), n))), lo, hi).astype(int)
+
+
+def plan_packs(sizes, max_queries, max_batches=64):
+    """Greedy packing of consecutive batches: a pack holds at most max_queries
+    queries (a larger batch gets a pack of its own) and at most max_batches."""
+    packs, cur, q = [], [], 0
+    for i, b in enumerate(sizes):
+        if cur and (q + b > max_queries or len(cur) == max_batches):
+            packs.append(cur)
+            cur, q = [], 0
+        cur.append(i)
+        q += int(b)
+    if cur:
+        packs.append(cur)
+    return packs
+
+
+def counter_advance(total, chunk, rows):
+    """ChaCha blocks a remote engine's dot_reduce_range uses over `total` rows
+    (mask + reshare OTP per chunk)."""
+    n, c0 = 0, 0
+    while c0 < total:
+        n += 2 * -(-(min(chunk, total - c0) * rows * 2) // 64)
+        c0 += chunk
+    return n
+
+
+def range_advance(n_rows, chunk, rows):
+    """ChaCha blocks of a whole scan of n_rows DB rows, launched one chunk at a
+    time with the last chunk padded to 64 rows (as the runners launch it)."""
+    return sum(counter_advance(tot, chunk, rows) for _, tot in _ranges(n_rows, chunk))
+
+
+class Matrix:
+    """One matrix (code or mask) of one party: engine, DB share, multiplier and
+    the raw query shares of every batch ([b][K] u16 on the host)."""
+
+    def __init__(self, engine, db, n_rows, multiplier, shares):
+        self.engine, self.db, self.n_rows, self.mult, self.shares = engine, db, n_rows, multiplier, shares
+
+
+def _ranges(n_rows, chunk):
+    off = 0
+    while off < n_rows:
+        yield off, -(-min(chunk, n_rows - off) // 64) * 64
+        off += chunk
+
+
+class BatchRunner:
+    """Plain and packed runs of the same batches over one or more matrices;
+    each batch's last chunk of results stays in its own output region."""
+
+    def __init__(self, matrices, sizes, chunk, stream, max_queries=256, max_batches=64):
+        import torch
+
+        self.m, self.sizes, self.chunk, self.stream = matrices, [int(b) for b in sizes], chunk, stream
+        self.plan = plan_packs(self.sizes, max_queries, max_batches)
+        self.outs = [[torch.zeros(chunk * b * ROTATIONS + 64, dtype=torch.int16, device="cuda")
+                      for b in self.sizes] for _ in matrices]
+        self.q_plain = [[m.engine.upload_query_shares_rows(m.shares[i], b * ROTATIONS)
+                         for i, b in enumerate(self.sizes)] for m in matrices]
+        self.q_packed = []
+        for m in matrices:
+            sets = []
+            for pack in self.plan:
+                raw = np.concatenate([m.shares[i] for i in pack])
+                sets.append(m.engine.upload_query_shares_rows(raw, raw.shape[0] * ROTATIONS))
+            self.q_packed.append(sets)
+        self.graph = None
+
+    def comparisons(self):
+        return sum(b * ROTATIONS for b in self.sizes) * self.m[0].n_rows
+
+    def run_plain(self, ctr0=0):
+        """One batch at a time, each to completion (stream synchronised)."""
+        from .engine import stream_synchronize
+
+        for k, m in enumerate(self.m):
+            m.engine.counter = ctr0
+        for i in range(len(self.sizes)):
+            for k, m in enumerate(self.m):
+                for off, tot in _ranges(m.n_rows, self.chunk):
+                    m.engine.dot_reduce_range(self.q_plain[k][i], m.db, off, tot, self.chunk, m.mult,
+                                              self.outs[k][i].data_ptr(), self.stream)
+            stream_synchronize(self.stream)
+
+    def _issue_packed(self, ctr0):
+        for k, m in enumerate(self.m):
+            base, c = [], ctr0
+            for b in self.sizes:  # the plain form's counter at the start of each batch
+                base.append(c)
+                if m.engine.is_remote:
+                    c += range_advance(m.n_rows, self.chunk, b * ROTATIONS)
+            for p, pack in enumerate(self.plan):
+                rows = [self.sizes[i] * ROTATIONS for i in pack]
+                outs = [self.outs[k][i].data_ptr() for i in pack]
+                for off, tot in _ranges(m.n_rows, self.chunk):
+                    ctr = [base[i] + (counter_advance(off, self.chunk, r) if m.engine.is_remote else 0)
+                           for i, r in zip(pack, rows)]
+                    m.engine.dot_reduce_packed(self.q_packed[k][p], m.db, off, tot, self.chunk, m.mult,
+                                               rows, outs, ctr, self.stream)
+
+    def capture(self, ctr0=0):
+        """Captures the packed schedule into a HIP graph."""
+        self.graph = Graph(self.stream).capture()
+        try:
+            self._issue_packed(ctr0)
+        finally:
+            self.graph.end()
+        return self.graph
+
+    def run_packed(self, ctr0=0, graph=True):
+        from .engine import stream_synchronize
+
+        if graph:
+            if self.graph is None:
+                self.capture(ctr0)
+            self.graph.launch()
+        else:
+            self._issue_packed(ctr0)
+        stream_synchronize(self.stream)
+
+
+class Autotuner:
+    """Picks the GEMM variant for a matrix and the pack size of a plan by
+    timing them on the device; choices are cached per device, code length and
+    row count in a JSON file (default iris-mpc_amd/_build/autotune.json)."""
+
+    def __init__(self, path=None):
+        here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+        self.path = path or os.environ.get("IRISMPC_AUTOTUNE", os.path.join(here, "_build", "autotune.json"))
+        try:
+            with open(self.path) as f:
+                self.cache = json.load(f)
+        except (OSError, ValueError):
+            self.cache = {}
+        self.log = []
+
+    def _save(self):
+        os.makedirs(os.path.dirname(self.path), exist_ok=True)
+        with open(self.path, "w") as f:
+            json.dump(self.cache, f, indent=1, sort_keys=True)
+
+    @staticmethod
+    def _device():
+        import torch
+
+        return torch.cuda.get_device_name(0).replace(" ", "_")
+
+    def _time(self, fn, stream, reps=2):
+        from .engine import stream_synchronize
+
+        fn()
+        stream_synchronize(stream)
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            stream_synchronize(stream)
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    def gemm_variant(self, m, query, rows, chunk, stream, sample_rows=4 * 32768, candidates=None):
+        """Fastest variant for query sets of `rows` rows on matrix m (raw-share queries)."""
+        import torch
+
+        key = f"{self._device()}/gemm/K={m.engine.code_length}/rows={rows}"
+        if key in self.cache:
+            return self.cache[key]["variant"]
+        cands = candidates or gemm_rot_variants()
+        n = min(sample_rows, m.n_rows)
+        out = torch.empty(n * rows + 64, dtype=torch.int16, device="cuda")
+        times = {}
+        keep = m.engine.counter
+        for v in cands:
+            m.engine.set_variant(v)
+            times[v] = self._time(lambda: m.engine.dot_reduce_range(query, m.db, 0, -(-n // 64) * 64, chunk,
+                                                                     m.mult, out.data_ptr(), stream), stream)
+        m.engine.counter = keep
+        best = min(times, key=times.get)
+        m.engine.set_variant(best)
+        self.cache[key] = {"variant": best, "seconds": {str(k): v for k, v in times.items()}}
+        self.log.append((key, times))
+        self._save()
+        return best
+
+    def pack_size(self, make_runner, sizes, candidates=(64, 128, 256), stream=0):
+        """Pack size with the best measured throughput: each candidate's packed
+        plan runs once (no graph) and the fastest per comparison wins."""
+        key = f"{self._device()}/pack/n={len(sizes)}/q={int(np.sum(sizes))}/c={','.join(map(str, candidates))}"
+        if key in self.cache:
+            return self.cache[key]["max_queries"]
+        times = {}
+        for mq in candidates:
+            r = make_runner(mq)
+            times[mq] = self._time(lambda: r.run_packed(graph=False), stream, reps=1)
+        best = min(times, key=times.get)
+        self.cache[key] = {"max_queries": best, "seconds": {str(k): v for k, v in times.items()}}
+        self.log.append((key, times))
+        self._save()
+        return best

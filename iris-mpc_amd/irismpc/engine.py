@@ -158,6 +158,11 @@ class ShareDB:
         self.code_length = code_length
         self.comm = comm
 
+    @property
+    def is_remote(self):
+        """Remote engines (with an exchange) mask their results with ChaCha12."""
+        return self.comm is not None
+
     def __del__(self):
         if getattr(self, "h", None):
             call("irismpc_share_db_destroy", self.h)
@@ -183,6 +188,25 @@ class ShareDB:
         h = _out()
         call("irismpc_query_upload_shares", self.h, r.ctypes.data, r.shape[0], stream, ctypes.byref(h))
         return DeviceQuery(h.value)
+
+    def upload_query_shares_rows(self, shares, rows, stream=0):
+        """Raw query shares as exactly `rows` query rows (rows >= batch * 31; may
+        exceed query_length): a packed query set of the batching form."""
+        r = np.ascontiguousarray(shares, dtype=np.uint16).reshape(-1, self.code_length)
+        h = _out()
+        call("irismpc_query_upload_shares_rows", self.h, r.ctypes.data, r.shape[0], rows, stream,
+             ctypes.byref(h))
+        return DeviceQuery(h.value)
+
+    def dot_reduce_packed(self, query, db, offset, total, chunk_size, multiplier, rows, out_ptrs,
+                          ctr_first=None, stream=0):
+        """Several batches packed in `query` (rows[i] rows each) in one launch;
+        batch i's results at out_ptrs[i] as its own dot_reduce_range lays them out."""
+        n = len(rows)
+        r = (ctypes.c_size_t * n)(*rows)
+        c = (ctypes.c_uint64 * n)(*ctr_first) if ctr_first is not None else None
+        call("irismpc_share_db_dot_reduce_packed", self.h, query.h, db.h, offset, total, chunk_size,
+             multiplier, r, ptr_array(out_ptrs), n, c, stream)
 
     def dot_reduce(self, query, db, chunk_size, offset, multiplier=1, stream=0):
         """dot + dot_reduce_and_multiply (fused)."""
@@ -222,6 +246,32 @@ class ShareDB:
 
     def set_variant(self, v):
         call("irismpc_share_db_set_variant", self.h, v)
+
+
+class Graph:
+    """HIP graph of the library calls issued on one stream between capture()
+    and end(); launch() replays it."""
+
+    def __init__(self, stream):
+        self.stream, self.h = stream, None
+
+    def capture(self):
+        call("irismpc_graph_capture_begin", self.stream)
+        return self
+
+    def end(self):
+        h = _out()
+        call("irismpc_graph_capture_end", self.stream, ctypes.byref(h))
+        self.h = h.value
+        return self
+
+    def launch(self, stream=None):
+        call("irismpc_graph_launch", self.h, self.stream if stream is None else stream)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            call("irismpc_graph_destroy", self.h)
+            self.h = None
 
 
 def gemm_variants():
