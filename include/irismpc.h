@@ -262,6 +262,21 @@ int irismpc_scan_db_trio(irismpc_share_db* const codes[3], irismpc_share_db* con
                          const irismpc_query* const q_code[3], const irismpc_query* const q_mask[3],
                          size_t db_size, size_t batch_size, uint32_t threshold_a,
                          uint64_t* const bitmaps[3], void* const streams[3]);
+/* One chunk step of those scans (the body of the chunk loop of actor.rs:
+ * 2318-2533) for the three parties: DB rows [offset, offset + min(32768,
+ * db_size - offset)) against the query sets (their own rows: batch_size
+ * queries x 31 or fewer padding rows). slot_of[p] (device u32 per batch
+ * query, or NULL) sends query j's bits to bitmap row slot_of[p][j]; rows are
+ * row_bits bits apart (0: db_size, the layout of irismpc_scan_db). The
+ * serving loop's iteration: requests join and leave the query set between
+ * chunk steps. */
+int irismpc_scan_chunk_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
+                            irismpc_circuits* const circuits[3], irismpc_comparator* const cmp[3],
+                            const irismpc_db* const code_db[3], const irismpc_db* const mask_db[3],
+                            const irismpc_query* const q_code[3], const irismpc_query* const q_mask[3],
+                            size_t offset, size_t db_size, size_t batch_size, uint32_t threshold_a,
+                            uint64_t* const bitmaps[3], const uint32_t* const slot_of[3],
+                            size_t row_bits, void* const streams[3]);
 
 /* ------------------------------------------------ HIP graphs */
 /* Captures the library calls issued on `stream` between begin and end into a

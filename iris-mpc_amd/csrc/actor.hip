@@ -6,36 +6,44 @@
 
 namespace irismpc {
 
+void scan_chunk(const PartyScan& p, size_t offset, size_t db_size, size_t batch_size,
+                uint32_t threshold_a, uint64_t* bitmap, const uint32_t* slot_of, size_t row_bits,
+                hipStream_t s) {
+  const size_t rows = (size_t)p.code_query->rows;
+  IRIS_REQUIRE((size_t)p.mask_query->rows == rows, "code/mask query sets disagree on rows");
+  IRIS_REQUIRE(batch_size * kRotations <= rows, "batch larger than the query rows");
+  // actor.rs:2286-2291, 2318-2533: the chunk covers rows [offset, offset + real)
+  const size_t real = db_size > offset ? std::min<size_t>(kDbChunk, db_size - offset) : 0;
+  // actor.rs:2331-2337 pads to 64; the circuits need a multiple of 8 words
+  const size_t pad = rows % 8 == 0 ? 64 : 512;
+  const size_t dot_chunk = round_up(std::max<size_t>(real, 1), pad);
+  p.codes->dot_reduce(*p.code_query, *p.code_db, dot_chunk, offset, 1, s);
+  p.masks->dot_reduce(*p.mask_query, *p.mask_db, dot_chunk, offset, 2, s);
+  p.codes->reshare_results(dot_chunk, s);
+  p.masks->reshare_results(dot_chunk, s);
+  const size_t len = dot_chunk * rows;
+  p.circuits->set_chunk_size(len / 64);
+  p.circuits->compare_threshold_masked_many(p.codes->results(), p.codes->results_peer(),
+                                            p.masks->results(), p.masks->results_peer(), len,
+                                            threshold_a, s);
+  p.circuits->open_exchange(s);
+  if (real > 0)
+    p.comparator->open_results(p.circuits->result_a(), p.circuits->result_b(),
+                               p.circuits->result_c(), bitmap, dot_chunk, real, offset,
+                               row_bits ? row_bits : db_size, batch_size, s, rows, slot_of);
+}
+
 void scan_db(const PartyScan& p, size_t db_size, size_t batch_size, uint32_t threshold_a,
              uint64_t* bitmap, hipStream_t s) {
-  const size_t qr = p.codes->query_length();
-  IRIS_REQUIRE(p.masks->query_length() == qr, "code/mask engines disagree on query_length");
-  IRIS_REQUIRE(batch_size * kRotations <= qr, "batch larger than the engines' query_length");
-  // actor.rs:2286-2291, 2318-2533: chunk c covers rows [c*C, c*C + chunk_size_c)
   size_t offset = 0;
   do {
-    const size_t real = db_size > offset ? std::min<size_t>(kDbChunk, db_size - offset) : 0;
-    const size_t dot_chunk = round_up(std::max<size_t>(real, 1), 64);  // actor.rs:2331-2337
-    p.codes->dot_reduce(*p.code_query, *p.code_db, dot_chunk, offset, 1, s);
-    p.masks->dot_reduce(*p.mask_query, *p.mask_db, dot_chunk, offset, 2, s);
-    p.codes->reshare_results(dot_chunk, s);
-    p.masks->reshare_results(dot_chunk, s);
-    const size_t len = dot_chunk * qr;
-    p.circuits->set_chunk_size(len / 64);
-    p.circuits->compare_threshold_masked_many(p.codes->results(), p.codes->results_peer(),
-                                              p.masks->results(), p.masks->results_peer(), len,
-                                              threshold_a, s);
-    p.circuits->open_exchange(s);
-    if (real > 0)
-      p.comparator->open_results(p.circuits->result_a(), p.circuits->result_b(),
-                                 p.circuits->result_c(), bitmap, dot_chunk, real, offset, db_size,
-                                 batch_size, s);
+    scan_chunk(p, offset, db_size, batch_size, threshold_a, bitmap, nullptr, 0, s);
     offset += kDbChunk;
   } while (offset < db_size);
 }
 
-void scan_db_trio(const PartyScan p[3], size_t db_size, size_t batch_size, uint32_t threshold_a,
-                  uint64_t* const bitmaps[3], hipStream_t const s[3]) {
+template <class F>
+static void run_trio(F&& body) {
   std::exception_ptr err[3];
   std::thread th[3];
   int dev = 0;
@@ -44,7 +52,7 @@ void scan_db_trio(const PartyScan p[3], size_t db_size, size_t batch_size, uint3
     th[i] = std::thread([&, i] {
       try {
         IRIS_HIP_CHECK(hipSetDevice(dev));
-        scan_db(p[i], db_size, batch_size, threshold_a, bitmaps[i], s[i]);
+        body(i);
       } catch (...) {
         err[i] = std::current_exception();
       }
@@ -52,6 +60,20 @@ void scan_db_trio(const PartyScan p[3], size_t db_size, size_t batch_size, uint3
   for (auto& t : th) t.join();
   for (auto& e : err)
     if (e) std::rethrow_exception(e);
+}
+
+void scan_chunk_trio(const PartyScan p[3], size_t offset, size_t db_size, size_t batch_size,
+                     uint32_t threshold_a, uint64_t* const bitmaps[3],
+                     const uint32_t* const slot_of[3], size_t row_bits, hipStream_t const s[3]) {
+  run_trio([&](int i) {
+    scan_chunk(p[i], offset, db_size, batch_size, threshold_a, bitmaps[i],
+               slot_of ? slot_of[i] : nullptr, row_bits, s[i]);
+  });
+}
+
+void scan_db_trio(const PartyScan p[3], size_t db_size, size_t batch_size, uint32_t threshold_a,
+                  uint64_t* const bitmaps[3], hipStream_t const s[3]) {
+  run_trio([&](int i) { scan_db(p[i], db_size, batch_size, threshold_a, bitmaps[i], s[i]); });
 }
 
 }  // namespace irismpc

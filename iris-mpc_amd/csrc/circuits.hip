@@ -457,7 +457,8 @@ __global__ __launch_bounds__(256) void open_kernel(const OpenArgs a) {
   const size_t valid = a.real_chunk_len - d0;
   if (valid < 64) v &= (1ull << valid) - 1;
   if (!v) return;
-  const size_t bit = a.total_db_len * (q / kRotations) + d0 + a.offset;
+  const size_t slot = a.slot_of ? a.slot_of[q / kRotations] : q / kRotations;
+  const size_t bit = a.total_db_len * slot + d0 + a.offset;
   const size_t wi = bit >> 6, sh = bit & 63;
   atomicOr(reinterpret_cast<unsigned long long*>(&a.bitmap[wi]), (unsigned long long)(v << sh));
   if (sh && (v >> (64 - sh)))

@@ -71,6 +71,7 @@ struct OpenArgs {
   size_t offset;         // global DB offset of the chunk
   size_t real_chunk_len;
   size_t total_db_len;
+  const uint32_t* slot_of;  // optional: bitmap row of batch query j
 };
 void launch_open(const OpenArgs& a, hipStream_t s);
 

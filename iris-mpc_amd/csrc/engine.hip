@@ -99,6 +99,7 @@ ShareDB::ShareDB(int device, int peer_id, size_t max_db_length, size_t query_len
       max_db_length_(max_db_length),
       seed_own_(seed_own),
       seed_prev_(seed_prev),
+      last_rows_(query_length),
       ex_(ex) {
   IRIS_REQUIRE(peer_id >= 0 && peer_id < 3, "peer_id must be 0..2");
   IRIS_REQUIRE(code_length % kKTile == 0 && code_length % 800 == 0,
@@ -273,7 +274,7 @@ void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, siz
                          uint16_t multiplier, hipStream_t s) {
   DeviceGuard g(device_);
   IRIS_REQUIRE(q.code_length == code_length_ && db.code_length == code_length_, "code length mismatch");
-  IRIS_REQUIRE((size_t)q.rows == query_length_, "query batch length mismatch");
+  IRIS_REQUIRE(q.rows >= 1 && (size_t)q.rows <= query_length_, "query rows exceed query_length");
   IRIS_REQUIRE(chunk > 0 && chunk % 64 == 0, "chunk size must be a positive multiple of 64");
   IRIS_REQUIRE(chunk <= max_db_length_, "chunk larger than the engine's result buffer");
   IRIS_REQUIRE(offset % 16 == 0, "offset must be a multiple of 16");
@@ -297,7 +298,8 @@ void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, siz
   a.masked = is_remote() ? 1 : 0;
   launch_share_gemm(a, variant, s);
   // the reference fills the mask keystream (chunk*Qr*2 bytes) only when remote
-  if (is_remote()) counter_ += ceil_div(chunk * query_length_ * 2, 64);
+  last_rows_ = (size_t)q.rows;
+  if (is_remote()) counter_ += ceil_div(chunk * last_rows_ * 2, 64);
 }
 
 void ShareDB::dot_reduce_range(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
@@ -386,10 +388,11 @@ struct ReshareArg {
 
 struct ShareDBTrio {
   static void reshare(ShareDB* const d[3], size_t chunk, hipStream_t const s[3]) {
-    const size_t bytes = chunk * d[0]->query_length_ * 2;
+    // the rows of each party's last dot_reduce (query_length unless a smaller set ran)
+    const size_t bytes = chunk * d[0]->last_rows_ * 2;
     for (int p = 0; p < 3; p++) {
       IRIS_REQUIRE(d[p]->peer_id_ == p, "trio members must be parties 0, 1, 2 in order");
-      IRIS_REQUIRE(d[p]->query_length_ == d[0]->query_length_ && chunk <= d[p]->max_db_length_,
+      IRIS_REQUIRE(d[p]->last_rows_ == d[0]->last_rows_ && chunk <= d[p]->max_db_length_,
                    "reshare geometry mismatch");
     }
     Event sent[3];
@@ -600,13 +603,14 @@ std::unique_ptr<DevBuf<uint64_t>> DistanceComparator::prepare_db_match_list(size
 void DistanceComparator::open_results(const uint64_t* a, const uint64_t* b, const uint64_t* c,
                                       uint64_t* bitmap, size_t chunk_len, size_t real_chunk_len,
                                       size_t offset, size_t total_db_len, size_t batch_size,
-                                      hipStream_t s) const {
+                                      hipStream_t s, size_t rows, const uint32_t* slot_of) const {
   DeviceGuard g(device_);
+  if (rows == 0) rows = query_length_;
   IRIS_REQUIRE(chunk_len % 64 == 0, "chunk length must be a multiple of 64");
-  IRIS_REQUIRE(batch_size * kRotations <= query_length_, "batch larger than query_length");
+  IRIS_REQUIRE(batch_size * kRotations <= rows && rows <= query_length_, "batch larger than the query rows");
   OpenArgs o{};
-  o.a = a; o.b = b; o.c = c; o.bitmap = bitmap;
-  o.n_words = chunk_len * query_length_ / 64;
+  o.a = a; o.b = b; o.c = c; o.bitmap = bitmap; o.slot_of = slot_of;
+  o.n_words = chunk_len * rows / 64;
   o.chunk_len = chunk_len;
   o.query_len = batch_size * kRotations;
   o.offset = offset;

@@ -178,6 +178,7 @@ class ShareDB {
   size_t query_length_, max_db_length_;
   ChaChaKey seed_own_, seed_prev_;
   uint64_t nonce_ = 0, counter_ = 0;
+  size_t last_rows_;  // query rows of the last dot_reduce (the reshare length)
   Exchange* ex_;
   DevBuf<uint16_t> results_, results_peer_;
   Event ev_send_done_;
@@ -230,9 +231,12 @@ class DistanceComparator {
   DistanceComparator(int device, size_t query_length, size_t max_db_size);
   // Bitmap of size max_db * batch bits (prepare_db_match_list).
   std::unique_ptr<DevBuf<uint64_t>> prepare_db_match_list(size_t db_size) const;
+  // rows: query rows of the circuit result (0 = query_length); slot_of (device,
+  // optional): batch query j's bits go to bitmap row slot_of[j] instead of j
   void open_results(const uint64_t* a, const uint64_t* b, const uint64_t* c, uint64_t* bitmap,
                     size_t chunk_len, size_t real_chunk_len, size_t offset, size_t total_db_len,
-                    size_t batch_size, hipStream_t s) const;
+                    size_t batch_size, hipStream_t s, size_t rows = 0,
+                    const uint32_t* slot_of = nullptr) const;
   // join_db_matches (AND policy) and join_db_matches_with_bitmaps (OR policy).
   void join_db_matches(const uint64_t* left, const uint64_t* right, uint32_t* final_results,
                        size_t db_size, const uint64_t* or_policy, size_t max_db_size,

@@ -402,6 +402,20 @@ def scan_db(party, q_code, q_mask, db_size, batch_size, threshold_a, bitmap_ptr,
          bitmap_ptr, stream)
 
 
+def scan_chunk_trio(parties, q_codes, q_masks, offset, db_size, batch_size, threshold_a, bitmap_ptrs,
+                    streams, slot_ptrs=None, row_bits=0):
+    """One chunk step of the three parties' scans (rows [offset, offset + 32768)
+    of a db_size-row DB); slot_ptrs[p] (device u32 per batch query) picks each
+    query's bitmap row, rows row_bits bits apart (0: db_size)."""
+    call("irismpc_scan_chunk_trio", ptr_array([p.codes.h for p in parties]),
+         ptr_array([p.masks.h for p in parties]), ptr_array([p.circuits.h for p in parties]),
+         ptr_array([p.comparator.h for p in parties]), ptr_array([p.code_db.h for p in parties]),
+         ptr_array([p.mask_db.h for p in parties]), ptr_array([q.h for q in q_codes]),
+         ptr_array([q.h for q in q_masks]), offset, db_size, batch_size, threshold_a,
+         ptr_array(list(bitmap_ptrs)), ptr_array(list(slot_ptrs)) if slot_ptrs is not None else None,
+         row_bits, ptr_array(list(streams)))
+
+
 def scan_db_trio(parties, q_codes, q_masks, db_size, batch_size, threshold_a, bitmap_ptrs, streams):
     call("irismpc_scan_db_trio", ptr_array([p.codes.h for p in parties]),
          ptr_array([p.masks.h for p in parties]), ptr_array([p.circuits.h for p in parties]),

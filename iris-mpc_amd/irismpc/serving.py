@@ -1,0 +1,327 @@
+"""Serving form: a request loop with continuous batching over the DB scan.
+
+Requests arrive over time:
+- uniqueness: one query x 31 rotations against the whole DB;
+- reauth: one query x 31 rotations against one target row;
+- insertion: a new iris appended to the DB;
+- deletion: a DB row overwritten with the shares of the default iris, as
+  get_dummy_shares_for_deletion (iris-mpc-common/src/iris_db/mod.rs:10-23)
+  does; code all zeros and mask all ones never reaches the threshold.
+
+The three parties' DB shares and protocol state stay on the device. The scan
+of actor.rs:2246-2589 becomes a ring of chunk steps (32,768 rows each):
+- Uniqueness requests join the active query set between chunk steps, as slots
+  free up and while free HBM stays above a reserve.
+- A request leaves when it has covered every chunk of the DB as it was when
+  it joined, so requests that arrive during a scan start at the next step
+  instead of waiting for the scan to finish.
+- Every step runs one share GEMM per matrix and party for all active queries,
+  then the reshare, the threshold circuits and the open into each request's
+  bitmap row (slot map).
+- Reauth runs on its own small engine set and streams, concurrently with the
+  scan.
+- Insertions and deletions are applied between steps.
+
+The opened bits do not depend on the mix (the masks do), so each result
+equals the result of the same request run alone (ServingEngines.run_alone,
+the plain per-request form).
+"""
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import engine as E
+from . import synth
+
+ROT = E.ROTATIONS
+CHUNK = E.DB_CHUNK_SIZE
+
+
+@dataclass
+class Request:
+    kind: str                 # "uniqueness" | "reauth" | "insert" | "delete"
+    arrival: float            # seconds from the start of the run
+    shares: list = None       # per party (code [12800] u16, mask [6400] u16): query or new iris
+    target: int = -1          # reauth target row / deletion row
+    rid: int = 0
+    result: object = None     # uniqueness: sorted matching rows; reauth: bool; insert: row; delete: row
+    done: float = -1.0
+    db_rows: int = 0          # DB rows the request was checked against
+    covered: set = field(default_factory=set)
+    slot: int = -1
+
+
+def dummy_shares(rand_dataset=0x0de1e7e, index=0):
+    """Per-party shares of the default iris (code all zeros, mask all ones)."""
+    code = np.zeros((1, 200), np.uint64)
+    mask = np.full((1, 200), np.uint64(0xFFFFFFFFFFFFFFFF))
+    sh = synth.shares_of(code, mask, rand_dataset, index)
+    return [(sh[p][0][0], sh[p][1][0]) for p in range(3)]
+
+
+class PartySet:
+    """Engines of the three parties for one query-set size: codes/masks ShareDB,
+    Circuits and DistanceComparator per party, one exchange, three streams."""
+
+    def __init__(self, max_rows, db_capacity, seeds, tag0):
+        self.ex = E.Exchange(timeout_s=300)
+        self.codes, self.masks, self.circ, self.cmp, self.streams = [], [], [], [], []
+        for p in range(3):
+            so, sp = seeds[p]
+            self.codes.append(E.ShareDB(p, CHUNK, max_rows, E.IRIS_CODE_LENGTH,
+                                        (E.derive_seed(so, 0), E.derive_seed(sp, 0)), comm=self.ex, tag=tag0))
+            self.masks.append(E.ShareDB(p, CHUNK, max_rows, E.MASK_CODE_LENGTH,
+                                        (E.derive_seed(so, 1), E.derive_seed(sp, 1)), comm=self.ex, tag=tag0 + 1))
+            self.circ.append(E.Circuits(p, max_rows * CHUNK, max_rows * CHUNK // 64,
+                                        (E.derive_seed(so, 5), E.derive_seed(sp, 5)), self.ex, tag=tag0 + 2))
+            self.cmp.append(E.DistanceComparator(max_rows, db_capacity))
+            self.streams.append(E.stream_create())
+
+    def parties(self, code_dbs, mask_dbs):
+        return [E.PartyEngines(self.codes[p], self.masks[p], self.circ[p], self.cmp[p], code_dbs[p], mask_dbs[p])
+                for p in range(3)]
+
+    def sync(self):
+        for s in self.streams:
+            E.stream_synchronize(s)
+
+
+class ServingEngines:
+    """DB shares of the three parties plus the scan and reauth engine sets."""
+
+    def __init__(self, n_rows, capacity, max_active=64, seed_base=0, threshold_a=None):
+        import torch
+
+        self.torch = torch
+        self.n_rows, self.capacity, self.max_active = n_rows, capacity, max_active
+        self.A = E.THRESHOLD_ANON_STATS_A if threshold_a is None else threshold_a
+        base = [np.full(8, seed_base + p, np.uint32) for p in range(3)]
+        seeds = [(base[p], base[(p + 2) % 3]) for p in range(3)]
+        self.scan = PartySet(max_active * ROT, capacity, seeds, 0)
+        self.small = PartySet(ROT, capacity, seeds, 8)  # reauth: one query, one chunk of 64 rows
+        self.code_dbs = [self.scan.codes[p].alloc_db(capacity) for p in range(3)]
+        self.mask_dbs = [self.scan.masks[p].alloc_db(capacity) for p in range(3)]
+        self.scan_parties = self.scan.parties(self.code_dbs, self.mask_dbs)
+        self.small_parties = self.small.parties(self.code_dbs, self.mask_dbs)
+        words = -(-capacity // 64)
+        self.words = words
+        # one bitmap row of `capacity` bits per slot, per party; the small set's single row
+        self.bitmaps = [torch.zeros(max_active * words + 2, dtype=torch.int64, device="cuda") for _ in range(3)]
+        self.small_bitmaps = [torch.zeros(words + 2, dtype=torch.int64, device="cuda") for _ in range(3)]
+        self.slot_maps = [torch.zeros(max_active, dtype=torch.int32, device="cuda") for _ in range(3)]
+        self.lock = threading.Lock()  # DB writes vs scan steps
+        self._dummy = None
+
+    # ------------------------------------------------------------------ DB
+    def build(self, n_unique, dataset, rand_dataset):
+        synth.build_db(self.code_dbs, self.mask_dbs, self.n_rows, n_unique, dataset, rand_dataset)
+
+    def insert(self, shares):
+        with self.lock:
+            row = self.n_rows
+            assert row < self.capacity - CHUNK, "DB capacity exhausted"
+            for p in range(3):
+                self.code_dbs[p].load_records(row, shares[p][0].reshape(1, -1))
+                self.mask_dbs[p].load_records(row, shares[p][1].reshape(1, -1))
+            self.n_rows += 1
+            return row
+
+    def delete(self, row):
+        if self._dummy is None:
+            self._dummy = dummy_shares()
+        with self.lock:
+            for p in range(3):
+                self.code_dbs[p].load_records(row, self._dummy[p][0].reshape(1, -1))
+                self.mask_dbs[p].load_records(row, self._dummy[p][1].reshape(1, -1))
+        return row
+
+    # ------------------------------------------------------------------ queries
+    def _query_sets(self, ps, shares_list, rows):
+        qc, qm = [], []
+        for p in range(3):
+            code = np.stack([s[p][0] for s in shares_list])
+            mask = np.stack([s[p][1] for s in shares_list])
+            qc.append(ps.codes[p].upload_query_shares_rows(code, rows))
+            qm.append(ps.masks[p].upload_query_shares_rows(mask, rows))
+        return qc, qm
+
+    def n_chunks(self, rows):
+        return max(1, -(-rows // CHUNK))
+
+    def step(self, qc, qm, batch, chunk_index, db_rows, slots):
+        """One chunk step of the scan engines for the active query set."""
+        for p in range(3):
+            self.slot_maps[p][:len(slots)].copy_(self.torch.tensor(slots, dtype=self.torch.int32))
+        self.torch.cuda.synchronize()
+        E.scan_chunk_trio(self.scan_parties, qc, qm, chunk_index * CHUNK, db_rows, batch, self.A,
+                          [b.data_ptr() for b in self.bitmaps], self.scan.streams,
+                          [m.data_ptr() for m in self.slot_maps], row_bits=self.words * 64)
+
+    def clear_slot(self, slot):
+        for b in self.bitmaps:
+            b[slot * self.words:(slot + 1) * self.words].zero_()
+
+    def slot_matches(self, slot, db_rows):
+        bits = self.bitmaps[0][slot * self.words:(slot + 1) * self.words].cpu().numpy().view(np.uint8)
+        rows = np.nonzero(np.unpackbits(bits, bitorder="little")[:db_rows])[0]
+        return [int(r) for r in rows]
+
+    def reauth(self, shares, target):
+        """One query x 31 rotations against DB row `target` on the small engine set."""
+        qc, qm = self._query_sets(self.small, [shares], ROT)
+        off = target // 64 * 64
+        for b in self.small_bitmaps:
+            b.zero_()
+        self.torch.cuda.synchronize()
+        # a 64-row window: the scan chunk of [off, off + 64) of a DB ending at off + 64
+        E.scan_chunk_trio(self.small_parties, qc, qm, off, off + 64, 1, self.A,
+                          [b.data_ptr() for b in self.small_bitmaps], self.small.streams)
+        self.small.sync()
+        bits = self.small_bitmaps[0][: (off + 64) // 64 + 1].cpu().numpy().view(np.uint8)
+        return bool(np.unpackbits(bits, bitorder="little")[target])
+
+    def run_alone(self, shares):
+        """The plain per-request form: a full scan of one uniqueness query."""
+        qc, qm = self._query_sets(self.scan, [shares], ROT)
+        rows = self.n_rows
+        self.clear_slot(0)
+        for c in range(self.n_chunks(rows)):
+            self.step(qc, qm, 1, c, rows, [0])
+        self.scan.sync()
+        return self.slot_matches(0, rows)
+
+
+class Server:
+    """Real-time request loop over ServingEngines (one scheduler thread plus a
+    reauth worker). Admission: a uniqueness request takes a free slot while
+    free HBM stays above `reserve_bytes` plus its device footprint."""
+
+    def __init__(self, eng, reserve_bytes=4 << 30, max_active=None):
+        self.eng = eng
+        self.max_active = max_active or eng.max_active
+        self.reserve = reserve_bytes
+        self.footprint = 3 * ROT * (E.IRIS_CODE_LENGTH + E.MASK_CODE_LENGTH) * 2 * 2  # limbs + staging
+        self.reauth_q = queue.Queue()
+        self.done = []
+        self.steps = 0
+        self.admission_waits = 0
+        self.max_concurrent = 0
+
+    def _free_hbm(self):
+        free, _ = self.eng.torch.cuda.mem_get_info()
+        return free
+
+    def _reauth_worker(self):
+        while True:
+            r = self.reauth_q.get()
+            if r is None:
+                return
+            r.result = self.eng.reauth(r.shares, r.target)
+            r.db_rows = self.eng.n_rows
+            r.done = time.perf_counter() - self.t0
+            self.done.append(r)
+
+    def run(self, requests):
+        """Serves `requests` (sorted by arrival, seconds from now); returns them
+        with results and completion times."""
+        eng = self.eng
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        self.t0 = time.perf_counter()
+        worker = threading.Thread(target=self._reauth_worker, daemon=True)
+        worker.start()
+        waiting, active = [], []
+        free_slots = list(range(self.max_active))
+        qsets, active_ids = None, None
+        i, cursor = 0, 0
+        while i < len(reqs) or waiting or active:
+            now = time.perf_counter() - self.t0
+            while i < len(reqs) and reqs[i].arrival <= now:
+                r = reqs[i]
+                i += 1
+                if r.kind == "uniqueness":
+                    waiting.append(r)
+                elif r.kind == "reauth":
+                    self.reauth_q.put(r)
+                elif r.kind == "insert":
+                    r.result = eng.insert(r.shares)
+                    r.db_rows = eng.n_rows
+                    r.done = time.perf_counter() - self.t0
+                    self.done.append(r)
+                else:
+                    r.result = eng.delete(r.target)
+                    r.db_rows = eng.n_rows
+                    r.done = time.perf_counter() - self.t0
+                    self.done.append(r)
+            # admission between chunk steps
+            while waiting and free_slots:
+                # with nothing active one request always goes in (progress)
+                if active and self._free_hbm() < self.reserve + self.footprint * (len(active) + 1):
+                    self.admission_waits += 1
+                    break
+                r = waiting.pop(0)
+                r.slot = free_slots.pop(0)
+                r.db_rows = eng.n_rows
+                r.covered = set()
+                eng.clear_slot(r.slot)
+                active.append(r)
+            self.max_concurrent = max(self.max_concurrent, len(active))
+            if not active:
+                if i < len(reqs):
+                    time.sleep(max(0.0, min(0.002, reqs[i].arrival - (time.perf_counter() - self.t0))))
+                continue
+            ids = [r.rid for r in active]
+            if ids != active_ids:
+                qsets = eng._query_sets(eng.scan, [r.shares for r in active], len(active) * ROT)
+                active_ids = ids
+            n_chunks = eng.n_chunks(eng.n_rows)
+            cursor %= n_chunks
+            with eng.lock:
+                eng.step(qsets[0], qsets[1], len(active), cursor, eng.n_rows, [r.slot for r in active])
+                eng.scan.sync()
+            self.steps += 1
+            for r in active:
+                r.covered.add(cursor)
+            cursor += 1
+            finished = [r for r in active if len(r.covered) >= eng.n_chunks(r.db_rows)
+                        and all(c in r.covered for c in range(eng.n_chunks(r.db_rows)))]
+            for r in finished:
+                r.result = [m for m in eng.slot_matches(r.slot, r.db_rows)]
+                r.done = time.perf_counter() - self.t0
+                free_slots.append(r.slot)
+                active.remove(r)
+                self.done.append(r)
+        self.reauth_q.put(None)
+        worker.join()
+        return reqs
+
+
+def poisson_mix(rate, n, rng, make_uniqueness, make_reauth, make_insert, make_delete,
+                mix=(0.85, 0.10, 0.05)):
+    """n requests with exponential inter-arrival times at `rate` per second and
+    the given uniqueness / reauth / insert-or-delete mix (insert and delete
+    alternate)."""
+    t, out, toggle = 0.0, [], 0
+    kinds = rng.choice(3, size=n, p=list(mix))
+    for k, kind in enumerate(kinds):
+        t += rng.exponential(1.0 / rate)
+        if kind == 0:
+            r = make_uniqueness(k)
+        elif kind == 1:
+            r = make_reauth(k)
+        else:
+            r = make_insert(k) if toggle == 0 else make_delete(k)
+            toggle ^= 1
+        r.arrival, r.rid = t, k
+        out.append(r)
+    return out
+
+
+def latency_stats(reqs, kind=None):
+    lat = np.array([r.done - r.arrival for r in reqs if (kind is None or r.kind == kind) and r.done >= 0])
+    if lat.size == 0:
+        return {"n": 0}
+    return {"n": int(lat.size), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3)}
