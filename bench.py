@@ -237,12 +237,10 @@ def run_c4(args):
         variants[m.engine.code_length] = tuner.gemm_variant(m, q, int(sizes.max()) * 31, chunk, stream)
         m.engine.set_variant(variants[m.engine.code_length])
 
-    def sample_runner(mq):
-        sm = [Matrix(m.engine, m.db, sample, m.mult, m.shares) for m in mats]
-        return BatchRunner(sm, sizes, chunk, stream, max_queries=mq)
-
-    max_q = tuner.pack_size(sample_runner, sizes, stream=stream)
-    runner = BatchRunner(mats, sizes, chunk, stream, max_queries=max_q)
+    del probe
+    torch.cuda.empty_cache()
+    runner = BatchRunner(mats, sizes, chunk, stream, max_queries=256)
+    max_q = tuner.pack_size(runner, sample)
     runner.capture()
     torch.cuda.synchronize()
 

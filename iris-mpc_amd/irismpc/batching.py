@@ -93,14 +93,22 @@ class BatchRunner:
                       for b in self.sizes] for _ in matrices]
         self.q_plain = [[m.engine.upload_query_shares_rows(m.shares[i], b * ROTATIONS)
                          for i, b in enumerate(self.sizes)] for m in matrices]
+        self.max_batches = max_batches
         self.q_packed = []
-        for m in matrices:
+        self.replan(max_queries)
+
+    def replan(self, max_queries):
+        """New pack size: rebuilds the plan and the packed query sets (the
+        output regions and the plain query sets stay)."""
+        self.plan = plan_packs(self.sizes, max_queries, self.max_batches)
+        self.max_queries = max_queries
+        self.q_packed, self.graph = [], None  # old sets are freed before the new ones exist
+        for m in self.m:
             sets = []
             for pack in self.plan:
                 raw = np.concatenate([m.shares[i] for i in pack])
                 sets.append(m.engine.upload_query_shares_rows(raw, raw.shape[0] * ROTATIONS))
             self.q_packed.append(sets)
-        self.graph = None
 
     def comparisons(self):
         return sum(b * ROTATIONS for b in self.sizes) * self.m[0].n_rows
@@ -118,17 +126,18 @@ class BatchRunner:
                                               self.outs[k][i].data_ptr(), self.stream)
             stream_synchronize(self.stream)
 
-    def _issue_packed(self, ctr0):
+    def _issue_packed(self, ctr0, n_rows=None):
         for k, m in enumerate(self.m):
+            n = m.n_rows if n_rows is None else n_rows
             base, c = [], ctr0
             for b in self.sizes:  # the plain form's counter at the start of each batch
                 base.append(c)
                 if m.engine.is_remote:
-                    c += range_advance(m.n_rows, self.chunk, b * ROTATIONS)
+                    c += range_advance(n, self.chunk, b * ROTATIONS)
             for p, pack in enumerate(self.plan):
                 rows = [self.sizes[i] * ROTATIONS for i in pack]
                 outs = [self.outs[k][i].data_ptr() for i in pack]
-                for off, tot in _ranges(m.n_rows, self.chunk):
+                for off, tot in _ranges(n, self.chunk):
                     ctr = [base[i] + (counter_advance(off, self.chunk, r) if m.engine.is_remote else 0)
                            for i, r in zip(pack, rows)]
                     m.engine.dot_reduce_packed(self.q_packed[k][p], m.db, off, tot, self.chunk, m.mult,
@@ -143,15 +152,17 @@ class BatchRunner:
             self.graph.end()
         return self.graph
 
-    def run_packed(self, ctr0=0, graph=True):
+    def run_packed(self, ctr0=0, graph=True, n_rows=None):
+        """The packed schedule (replayed from its graph); n_rows limits the scan
+        to the first n_rows DB rows (no graph)."""
         from .engine import stream_synchronize
 
-        if graph:
+        if graph and n_rows is None:
             if self.graph is None:
                 self.capture(ctr0)
             self.graph.launch()
         else:
-            self._issue_packed(ctr0)
+            self._issue_packed(ctr0, n_rows)
         stream_synchronize(self.stream)
 
 
@@ -218,17 +229,24 @@ class Autotuner:
         self._save()
         return best
 
-    def pack_size(self, make_runner, sizes, candidates=(64, 128, 256), stream=0):
-        """Pack size with the best measured throughput: each candidate's packed
-        plan runs once (no graph) and the fastest per comparison wins."""
+    def pack_size(self, runner, sample_rows, candidates=(64, 128, 256)):
+        """Pack size with the best measured time for the runner's batches over
+        the first sample_rows DB rows; the runner is left on that pack size."""
+        sizes = runner.sizes
         key = f"{self._device()}/pack/n={len(sizes)}/q={int(np.sum(sizes))}/c={','.join(map(str, candidates))}"
         if key in self.cache:
-            return self.cache[key]["max_queries"]
+            best = self.cache[key]["max_queries"]
+            if runner.max_queries != best:
+                runner.replan(best)
+            return best
         times = {}
         for mq in candidates:
-            r = make_runner(mq)
-            times[mq] = self._time(lambda: r.run_packed(graph=False), stream, reps=1)
+            runner.replan(mq)
+            times[mq] = self._time(lambda: runner.run_packed(graph=False, n_rows=sample_rows), runner.stream,
+                                   reps=1)
         best = min(times, key=times.get)
+        if runner.max_queries != best:
+            runner.replan(best)
         self.cache[key] = {"max_queries": best, "seconds": {str(k): v for k, v in times.items()}}
         self.log.append((key, times))
         self._save()
