@@ -75,8 +75,33 @@ __device__ __forceinline__ void store8(uint64_t* p, const uint64_t v[8]) {
 
 // ------------------------------------------------------------ transpose
 // Plane j, word i, bit k = bit j of element 64 i + k (kernel.cu:63-84).
-// Block per 64 consecutive words: each wave turns 16 words into bit planes by
-// ballots, the planes go through LDS and leave as 512-byte runs per plane.
+// 64 x 32 bit-matrix transpose across a wave: lane l holds x_l; lanes j and
+// 32 + j receive the 64-bit word whose bit l is bit j of x_l. Five butterfly
+// stages inside each 32-lane half swap the off-diagonal s x s blocks with the
+// partner lane l ^ s (ds_swizzle): a per-lane rotate of the partner's value
+// and one bitfield insert. The halves are then joined across lane ^ 32.
+template <int S, uint32_t LOW>
+__device__ __forceinline__ uint32_t transpose_stage(uint32_t x, int lane) {
+  const uint32_t y = (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (S << 10));
+  const bool hi = (lane & S) != 0;
+  const uint32_t keep = hi ? ~LOW : LOW;
+  const uint32_t rot = __builtin_rotateright32(y, hi ? S : 32 - S);
+  return (x & keep) | (rot & ~keep);
+}
+
+__device__ __forceinline__ uint64_t wave_transpose32(uint32_t x, int lane) {
+  x = transpose_stage<16, 0x0000FFFFu>(x, lane);
+  x = transpose_stage<8, 0x00FF00FFu>(x, lane);
+  x = transpose_stage<4, 0x0F0F0F0Fu>(x, lane);
+  x = transpose_stage<2, 0x33333333u>(x, lane);
+  x = transpose_stage<1, 0x55555555u>(x, lane);
+  const uint32_t other = (uint32_t)__shfl_xor((int)x, 32);
+  return lane < 32 ? ((uint64_t)x | ((uint64_t)other << 32)) : ((uint64_t)other | ((uint64_t)x << 32));
+}
+
+// Block per 64 consecutive words: each wave turns 16 words into bit planes
+// (a in bits 0..15 and b in bits 16..31 of one transpose), the planes go
+// through LDS and leave as 512-byte runs per plane.
 __global__ __launch_bounds__(256) void transpose16_kernel(const Trio t) {
   const int p = party_of(t);
   const CircuitBufs& P = t.p[p];
@@ -92,15 +117,8 @@ __global__ __launch_bounds__(256) void transpose16_kernel(const Trio t) {
       va = P.mask_a[w * 64 + lane];
       vb = P.mask_b[w * 64 + lane];
     }
-    uint64_t mine = 0;
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const uint64_t ba = __ballot((va >> j) & 1u);
-      const uint64_t bb = __ballot((vb >> j) & 1u);
-      if (lane == j) mine = ba;
-      if (lane == 16 + j) mine = bb;
-    }
-    if (lane < 32) planes[lane][wl] = mine;
+    const uint64_t plane = wave_transpose32(va | (vb << 16), lane);
+    if (lane < 32) planes[lane][wl] = plane;
   }
   __syncthreads();
 #pragma unroll
@@ -394,7 +412,7 @@ __global__ __launch_bounds__(256) void inject_kernel(const Trio t, int step, uin
 
 // ------------------------------------------------------------ lift_mul_sub + transpose32
 // m = (mask - c1<<16 - c2<<17) * A - code<<16, minus 1 on x_0 (kernel.cu:181-195,
-// 407-426), then the 32x64 bit transpose (kernel.cu:86-107) by ballots.
+// 407-426), then the 32x64 bit transpose (kernel.cu:86-107).
 __global__ __launch_bounds__(256) void lms_transpose32_kernel(const Trio t, uint32_t A) {
   const int p = party_of(t);
   const CircuitBufs& P = t.p[p];
@@ -420,15 +438,9 @@ __global__ __launch_bounds__(256) void lms_transpose32_kernel(const Trio t, uint
       if (p == 0) ma -= 1u;
       if (p == 1) mb -= 1u;
     }
-    uint64_t mine = 0;
-#pragma unroll
-    for (int j = 0; j < 32; j++) {
-      const uint64_t ba = __ballot((ma >> j) & 1u);
-      const uint64_t bb = __ballot((mb >> j) & 1u);
-      if (lane == j) mine = ba;
-      if (lane == 32 + j) mine = bb;
-    }
-    planes[lane][wl] = mine;
+    // plane j of a in lane j, of b in lane 32 + j
+    const uint64_t pa = wave_transpose32(ma, lane), pb = wave_transpose32(mb, lane);
+    planes[lane][wl] = lane < 32 ? pa : pb;
   }
   __syncthreads();
 #pragma unroll
