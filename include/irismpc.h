@@ -92,6 +92,12 @@ int irismpc_db_load_records(irismpc_share_db* e, irismpc_db* db, size_t index, s
  * ("odd") bytes, limb1 = high ("even") bytes, already biased by -128. */
 int irismpc_db_load_records_limbs(irismpc_share_db* e, irismpc_db* db, size_t index, size_t n,
                                   const uint8_t* limb0, const uint8_t* limb1, void* stream);
+/* ProcessedDatabase::prefetch_subset (share_db.rs:344-407): DB rows ids[0..n)
+ * (device u32) of db into rows 0..n of subset (a slice of at least n rows,
+ * e.g. irismpc_share_db_alloc_db(e, 32768, ...)). Ids beyond the DB's
+ * capacity leave their subset row as it was. */
+int irismpc_db_prefetch_subset(irismpc_share_db* e, const irismpc_db* db, const uint32_t* ids,
+                               size_t n, irismpc_db* subset, void* stream);
 /* ProcessedDatabase::write_at_index (share_db.rs:214-262): copies the centre
  * rotation of query src_index into DB row dst_index. */
 int irismpc_db_write_at_index(irismpc_share_db* e, irismpc_db* db, const irismpc_query* q,
@@ -236,6 +242,38 @@ int irismpc_comparator_join_db_matches(irismpc_comparator* d, const uint64_t* le
                                        const uint64_t* right, uint32_t* final_results,
                                        size_t db_size, const uint64_t* or_policy,
                                        size_t max_db_size, void* stream);
+/* DistanceComparator::get_partial_results (distance_comparator.rs:545-648)
+ * with the OR-rule / reauth indices the actor appends (actor.rs:946-970):
+ * the ascending list, without repeats, of DB ids d < db_size set in any of
+ * the batch_size bitmap rows (row_bits bits apart; 0 = db_size) or listed in
+ * extra_ids (host, n_extra; ids >= db_size are skipped). The list stays on the
+ * device (irismpc_comparator_partial_ids); *n_ids receives its length and,
+ * when host_ids is not NULL, its first host_capacity ids are copied there.
+ * The reference keeps at most DB_CHUNK_SIZE set bits (dropping the rest);
+ * this list keeps every id. Synchronises the stream. */
+int irismpc_comparator_get_partial_results(irismpc_comparator* d, const uint64_t* bitmap,
+                                           size_t db_size, size_t batch_size, size_t row_bits,
+                                           const uint32_t* extra_ids, size_t n_extra,
+                                           uint32_t* host_ids, size_t host_capacity,
+                                           size_t* n_ids, void* stream);
+/* Device pointer of the last get_partial_results list. */
+int irismpc_comparator_partial_ids(irismpc_comparator* d, const uint32_t** ids);
+/* open_results_with_index_mapping (distance_comparator.rs:281-342,
+ * dot/kernel.cu:118-151): column j < real_chunk_len of the opened subset
+ * chunk is DB row index_map[j] (device u32); each match sets bitmap bit
+ * total_db_len * (q / 31) + index_map[j] and is captured as (q / 31,
+ * index_map[j], q % 31 - 15). */
+int irismpc_comparator_open_results_with_index_mapping(
+    irismpc_comparator* d, const uint64_t* a, const uint64_t* b, const uint64_t* c,
+    uint64_t* bitmap, size_t chunk_len, size_t real_chunk_len, size_t total_db_len,
+    size_t batch_size, const uint32_t* index_map, void* stream);
+/* get_partial_results_with_rotations (distance_comparator.rs:650-711): copies
+ * up to capacity captured (query, db id, rotation) triples to the host arrays
+ * and resets the capture; *n_matches = matches seen (the capture holds the
+ * first 256 * query_length). */
+int irismpc_comparator_fetch_partial_rotations(irismpc_comparator* d, uint32_t* query,
+                                               uint32_t* db, int8_t* rotation, size_t capacity,
+                                               size_t* n_matches, void* stream);
 /* Resets match counters (start of a batch). */
 int irismpc_comparator_reset(irismpc_comparator* d, void* stream);
 /* Device pointers of match_counters, all_matches (256 per query),
@@ -277,6 +315,22 @@ int irismpc_scan_chunk_trio(irismpc_share_db* const codes[3], irismpc_share_db* 
                             size_t offset, size_t db_size, size_t batch_size, uint32_t threshold_a,
                             uint64_t* const bitmaps[3], const uint32_t* const slot_of[3],
                             size_t row_bits, void* const streams[3]);
+/* The DB half of ServerActor::compare_query_against_db_subset_and_self
+ * (actor.rs:1900-2244) for the three parties: the subset ids[p][0..n_ids)
+ * (device u32 DB row ids, e.g. irismpc_comparator_partial_ids) are gathered
+ * from code_db/mask_db into code_sub/mask_sub (slices of DB_CHUNK_SIZE rows)
+ * DB_CHUNK_SIZE at a time, compared at threshold_a, opened with the index
+ * mapping into bitmaps[p] (rows total_db_len bits apart) and captured with
+ * their rotations. */
+int irismpc_scan_subset_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
+                             irismpc_circuits* const circuits[3], irismpc_comparator* const cmp[3],
+                             const irismpc_db* const code_db[3], const irismpc_db* const mask_db[3],
+                             irismpc_db* const code_sub[3], irismpc_db* const mask_sub[3],
+                             const irismpc_query* const q_code[3],
+                             const irismpc_query* const q_mask[3], const uint32_t* const ids[3],
+                             size_t n_ids, size_t total_db_len, size_t batch_size,
+                             uint32_t threshold_a, uint64_t* const bitmaps[3],
+                             void* const streams[3]);
 
 /* ------------------------------------------------ HIP graphs */
 /* Captures the library calls issued on `stream` between begin and end into a

@@ -33,6 +33,38 @@ void scan_chunk(const PartyScan& p, size_t offset, size_t db_size, size_t batch_
                                row_bits ? row_bits : db_size, batch_size, s, rows, slot_of);
 }
 
+void scan_subset(const PartyScan& p, DbSlice& code_sub, DbSlice& mask_sub, const uint32_t* ids,
+                 size_t n_ids, size_t total_db_len, size_t batch_size, uint32_t threshold_a,
+                 uint64_t* bitmap, hipStream_t s) {
+  const size_t rows = (size_t)p.code_query->rows;
+  IRIS_REQUIRE((size_t)p.mask_query->rows == rows, "code/mask query sets disagree on rows");
+  IRIS_REQUIRE(batch_size * kRotations <= rows, "batch larger than the query rows");
+  const size_t pad = rows % 8 == 0 ? 64 : 512;
+  IRIS_REQUIRE(round_up(std::min<size_t>(n_ids, kDbChunk), pad) <=
+                   std::min(code_sub.capacity, mask_sub.capacity),
+               "subset slices hold fewer than DB_CHUNK_SIZE rows");
+  // actor.rs:1955-1975: subset chunks of DB_CHUNK_SIZE ids
+  for (size_t c0 = 0; c0 < n_ids; c0 += kDbChunk) {
+    const size_t real = std::min<size_t>(kDbChunk, n_ids - c0);
+    const size_t dot_chunk = round_up(real, pad);  // actor.rs:1977-1980
+    p.codes->prefetch_subset(*p.code_db, ids + c0, real, code_sub, s);
+    p.masks->prefetch_subset(*p.mask_db, ids + c0, real, mask_sub, s);
+    p.codes->dot_reduce(*p.code_query, code_sub, dot_chunk, 0, 1, s);
+    p.masks->dot_reduce(*p.mask_query, mask_sub, dot_chunk, 0, 2, s);
+    p.codes->reshare_results(dot_chunk, s);
+    p.masks->reshare_results(dot_chunk, s);
+    const size_t len = dot_chunk * rows;
+    p.circuits->set_chunk_size(len / 64);
+    p.circuits->compare_threshold_masked_many(p.codes->results(), p.codes->results_peer(),
+                                              p.masks->results(), p.masks->results_peer(), len,
+                                              threshold_a, s);
+    p.circuits->open_exchange(s);
+    p.comparator->open_results_mapped(p.circuits->result_a(), p.circuits->result_b(),
+                                      p.circuits->result_c(), bitmap, dot_chunk, real,
+                                      total_db_len, batch_size, ids + c0, s, rows);
+  }
+}
+
 void scan_db(const PartyScan& p, size_t db_size, size_t batch_size, uint32_t threshold_a,
              uint64_t* bitmap, hipStream_t s) {
   size_t offset = 0;
@@ -68,6 +100,16 @@ void scan_chunk_trio(const PartyScan p[3], size_t offset, size_t db_size, size_t
   run_trio([&](int i) {
     scan_chunk(p[i], offset, db_size, batch_size, threshold_a, bitmaps[i],
                slot_of ? slot_of[i] : nullptr, row_bits, s[i]);
+  });
+}
+
+void scan_subset_trio(const PartyScan p[3], DbSlice* const code_sub[3], DbSlice* const mask_sub[3],
+                      const uint32_t* const ids[3], size_t n_ids, size_t total_db_len,
+                      size_t batch_size, uint32_t threshold_a, uint64_t* const bitmaps[3],
+                      hipStream_t const s[3]) {
+  run_trio([&](int i) {
+    scan_subset(p[i], *code_sub[i], *mask_sub[i], ids[i], n_ids, total_db_len, batch_size,
+                threshold_a, bitmaps[i], s[i]);
   });
 }
 

@@ -38,6 +38,21 @@ void scan_chunk_trio(const PartyScan p[3], size_t offset, size_t db_size, size_t
                      uint32_t threshold_a, uint64_t* const bitmaps[3],
                      const uint32_t* const slot_of[3], size_t row_bits, hipStream_t const s[3]);
 
+// The DB half of compare_query_against_db_subset_and_self (actor.rs:1900-2244):
+// the subset rows ids[0..n_ids) (device u32, DB row ids below total_db_len)
+// are gathered chunk by chunk (DB_CHUNK_SIZE rows) from the HBM DB into the
+// subset slices (prefetch_subset), compared against the whole query set at
+// threshold_a, and the opened bits set at the rows' DB ids
+// (openResultsWithIndexMapping), with the rotation capture. n_ids = 0 does
+// nothing. Collective like scan_db.
+void scan_subset(const PartyScan& p, DbSlice& code_sub, DbSlice& mask_sub, const uint32_t* ids,
+                 size_t n_ids, size_t total_db_len, size_t batch_size, uint32_t threshold_a,
+                 uint64_t* bitmap, hipStream_t s);
+void scan_subset_trio(const PartyScan p[3], DbSlice* const code_sub[3], DbSlice* const mask_sub[3],
+                      const uint32_t* const ids[3], size_t n_ids, size_t total_db_len,
+                      size_t batch_size, uint32_t threshold_a, uint64_t* const bitmaps[3],
+                      hipStream_t const s[3]);
+
 // Runs the three parties' scans from one call (one host thread per party).
 void scan_db_trio(const PartyScan p[3], size_t db_size, size_t batch_size, uint32_t threshold_a,
                   uint64_t* const bitmaps[3], hipStream_t const s[3]);

@@ -1,4 +1,5 @@
 // extern "C" boundary of libirismpc (declared in include/irismpc.h).
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -179,6 +180,10 @@ int irismpc_db_write_at_index(irismpc_share_db* e, irismpc_db* db, const irismpc
                               size_t src_index, size_t dst_index, void* stream) {
   return guarded(
       [&] { e->impl->write_at_index(*db->impl, *q->impl, src_index, dst_index, st(stream)); });
+}
+int irismpc_db_prefetch_subset(irismpc_share_db* e, const irismpc_db* db, const uint32_t* ids,
+                               size_t n, irismpc_db* subset, void* stream) {
+  return guarded([&] { e->impl->prefetch_subset(*db->impl, ids, n, *subset->impl, st(stream)); });
 }
 int irismpc_db_read_records(irismpc_share_db* e, const irismpc_db* db, size_t index, size_t n,
                             uint16_t* out, void* stream) {
@@ -476,6 +481,46 @@ int irismpc_comparator_join_db_matches(irismpc_comparator* d, const uint64_t* le
                              st(stream));
   });
 }
+int irismpc_comparator_get_partial_results(irismpc_comparator* d, const uint64_t* bitmap,
+                                           size_t db_size, size_t batch_size, size_t row_bits,
+                                           const uint32_t* extra_ids, size_t n_extra,
+                                           uint32_t* host_ids, size_t host_capacity,
+                                           size_t* n_ids, void* stream) {
+  return guarded([&] {
+    IRIS_REQUIRE(n_ids, "n_ids is NULL");
+    IRIS_REQUIRE(!n_extra || extra_ids, "extra_ids is NULL");
+    const size_t n = d->impl->get_partial_results(bitmap, db_size, batch_size, extra_ids, n_extra,
+                                                  st(stream), row_bits);
+    *n_ids = n;
+    const size_t m = std::min(n, host_capacity);
+    if (host_ids && m) {
+      IRIS_HIP_CHECK(hipMemcpyAsync(host_ids, d->impl->partial_ids(), m * 4,
+                                    hipMemcpyDeviceToHost, st(stream)));
+      IRIS_HIP_CHECK(hipStreamSynchronize(st(stream)));
+    }
+  });
+}
+int irismpc_comparator_partial_ids(irismpc_comparator* d, const uint32_t** ids) {
+  return guarded([&] { *ids = d->impl->partial_ids(); });
+}
+int irismpc_comparator_open_results_with_index_mapping(
+    irismpc_comparator* d, const uint64_t* a, const uint64_t* b, const uint64_t* c,
+    uint64_t* bitmap, size_t chunk_len, size_t real_chunk_len, size_t total_db_len,
+    size_t batch_size, const uint32_t* index_map, void* stream) {
+  return guarded([&] {
+    d->impl->open_results_mapped(a, b, c, bitmap, chunk_len, real_chunk_len, total_db_len,
+                                 batch_size, index_map, st(stream));
+  });
+}
+int irismpc_comparator_fetch_partial_rotations(irismpc_comparator* d, uint32_t* query,
+                                               uint32_t* db, int8_t* rotation, size_t capacity,
+                                               size_t* n_matches, void* stream) {
+  return guarded([&] {
+    IRIS_REQUIRE(n_matches, "n_matches is NULL");
+    IRIS_REQUIRE(!capacity || (query && db && rotation), "capture arrays are NULL");
+    *n_matches = d->impl->fetch_partial_rotations(query, db, rotation, capacity, st(stream));
+  });
+}
 int irismpc_comparator_reset(irismpc_comparator* d, void* stream) {
   return guarded([&] { d->impl->reset(st(stream)); });
 }
@@ -553,6 +598,31 @@ int irismpc_scan_chunk_trio(irismpc_share_db* const codes[3], irismpc_share_db* 
       s[i] = st(streams[i]);
     }
     scan_chunk_trio(p, offset, db_size, batch_size, threshold_a, bitmaps, slot_of, row_bits, s);
+  });
+}
+
+int irismpc_scan_subset_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
+                             irismpc_circuits* const circuits[3], irismpc_comparator* const cmp[3],
+                             const irismpc_db* const code_db[3], const irismpc_db* const mask_db[3],
+                             irismpc_db* const code_sub[3], irismpc_db* const mask_sub[3],
+                             const irismpc_query* const q_code[3],
+                             const irismpc_query* const q_mask[3], const uint32_t* const ids[3],
+                             size_t n_ids, size_t total_db_len, size_t batch_size,
+                             uint32_t threshold_a, uint64_t* const bitmaps[3],
+                             void* const streams[3]) {
+  return guarded([&] {
+    PartyScan p[3];
+    hipStream_t s[3];
+    DbSlice* cs[3];
+    DbSlice* ms[3];
+    for (int i = 0; i < 3; i++) {
+      p[i] = party_scan(codes[i], masks[i], circuits[i], cmp[i], code_db[i], mask_db[i], q_code[i],
+                        q_mask[i]);
+      s[i] = st(streams[i]);
+      cs[i] = code_sub[i]->impl.get();
+      ms[i] = mask_sub[i]->impl.get();
+    }
+    scan_subset_trio(p, cs, ms, ids, n_ids, total_db_len, batch_size, threshold_a, bitmaps, s);
   });
 }
 

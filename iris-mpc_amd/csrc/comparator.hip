@@ -1,4 +1,5 @@
-// DistanceComparator kernels: eye merge of match bitmaps and buffer fills.
+// DistanceComparator kernels: eye merge of match bitmaps, candidate lists of
+// the two-stage flow, the index-mapped open, and buffer fills.
 //
 // mergeDbResults / mergeDbResultsWithOrPolicyBitmap (dot/kernel.cu:241-349):
 // AND (or per-pair OR policy) of the left/right bitmaps, atomicMin of the
@@ -46,6 +47,120 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeArgs a) {
   }
 }
 
+// Exclusive prefix sum over a 256-thread block; *total = the block's sum.
+// lds holds the four wave totals (the caller syncs before reusing it).
+__device__ inline uint32_t block_exclusive_scan256(uint32_t v, uint32_t* lds, uint32_t* total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[wv] = x;
+  __syncthreads();
+  uint32_t base = 0;
+  for (int i = 0; i < wv; i++) base += lds[i];
+  *total = lds[0] + lds[1] + lds[2] + lds[3];
+  return base + x - v;
+}
+
+constexpr int kCandTileWords = 256;  // one 256-thread block per 16,384 ids
+
+// Word w of the candidate set: OR over the batch rows of bitmap bits
+// [row_bits * q + 64 w, +64), clipped to db_size.
+__global__ __launch_bounds__(256) void cand_union_kernel(const CandidateArgs a) {
+  const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w * 64 >= a.db_size) return;
+  uint64_t acc = 0;
+  for (size_t q = 0; q < a.batch; q++) {
+    const size_t o = q * a.row_bits + w * 64;
+    const size_t wi = o >> 6;
+    const unsigned sh = (unsigned)(o & 63);
+    uint64_t v = a.bitmap[wi] >> sh;
+    if (sh) v |= a.bitmap[wi + 1] << (64 - sh);
+    acc |= v;
+  }
+  const size_t valid = a.db_size - w * 64;
+  if (valid < 64) acc &= (1ull << valid) - 1;
+  a.cand[w] = acc;
+}
+
+__global__ __launch_bounds__(256) void cand_extra_kernel(const CandidateArgs a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_extra) return;
+  const uint32_t d = a.extra[i];
+  if (d < a.db_size)
+    atomicOr(reinterpret_cast<unsigned long long*>(&a.cand[d >> 6]), 1ull << (d & 63));
+}
+
+__global__ __launch_bounds__(256) void cand_count_kernel(const CandidateArgs a) {
+  __shared__ uint32_t lds[4];
+  const size_t w = (size_t)blockIdx.x * kCandTileWords + threadIdx.x;
+  const uint32_t pc = w * 64 < a.db_size ? (uint32_t)__popcll(a.cand[w]) : 0u;
+  uint32_t total;
+  block_exclusive_scan256(pc, lds, &total);
+  if (threadIdx.x == 0) a.tiles[blockIdx.x] = total;
+}
+
+// One block: tile counts -> tile start positions, total -> *count.
+__global__ __launch_bounds__(256) void cand_scan_kernel(const CandidateArgs a, size_t n_tiles) {
+  __shared__ uint32_t lds[4];
+  uint32_t carry = 0;
+  for (size_t base = 0; base < n_tiles; base += 256) {
+    const size_t i = base + threadIdx.x;
+    const uint32_t v = i < n_tiles ? a.tiles[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan256(v, lds, &total);
+    if (i < n_tiles) a.tiles[i] = carry + ex;
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *a.count = carry;
+}
+
+__global__ __launch_bounds__(256) void cand_emit_kernel(const CandidateArgs a) {
+  __shared__ uint32_t lds[4];
+  const size_t w = (size_t)blockIdx.x * kCandTileWords + threadIdx.x;
+  uint64_t v = w * 64 < a.db_size ? a.cand[w] : 0ull;
+  uint32_t total;
+  uint32_t pos = a.tiles[blockIdx.x] + block_exclusive_scan256((uint32_t)__popcll(v), lds, &total);
+  while (v) {
+    const int i = __builtin_ctzll(v);
+    v &= v - 1;
+    a.ids[pos++] = (uint32_t)(w * 64 + (size_t)i);
+  }
+}
+
+__global__ __launch_bounds__(256) void open_mapped_kernel(const MappedOpenArgs a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_words) return;
+  uint64_t v = a.a[i] ^ a.b[i] ^ a.c[i];
+  const size_t e0 = i * 64;
+  const size_t q = e0 / a.chunk_len, d0 = e0 % a.chunk_len;
+  if (q >= a.query_len || d0 >= a.real_chunk_len) return;
+  const size_t valid = a.real_chunk_len - d0;
+  if (valid < 64) v &= (1ull << valid) - 1;
+  if (!v) return;
+  const size_t slot = a.slot_of ? a.slot_of[q / kRotations] : q / kRotations;
+  const uint32_t qi = (uint32_t)(q / kRotations);
+  const int8_t rot = (int8_t)((int)(q % kRotations) - kRotations / 2);
+  while (v) {
+    const int b = __builtin_ctzll(v);
+    v &= v - 1;
+    const uint32_t id = a.index_map[d0 + (size_t)b];
+    if (id >= a.total_db_len) continue;
+    const size_t pos = a.total_db_len * slot + id;
+    atomicOr(reinterpret_cast<unsigned long long*>(&a.bitmap[pos >> 6]), 1ull << (pos & 63));
+    const uint32_t c = atomicAdd(a.counter, 1u);
+    if (c < a.capacity) {
+      a.cap_query[c] = qi;
+      a.cap_db[c] = id;
+      a.cap_rot[c] = rot;
+    }
+  }
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -60,6 +175,35 @@ void launch_merge(const MergeArgs& a, hipStream_t s) {
   const size_t words = ceil_div(a.query_len * a.db_len, 64);
   if (!words) return;
   merge_kernel<<<(unsigned)ceil_div(words, 256), 256, 0, s>>>(a);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_candidates(const CandidateArgs& a, hipStream_t s) {
+  const size_t words = ceil_div(a.db_size, 64);
+  const size_t n_tiles = ceil_div(words, kCandTileWords);
+  if (words) {
+    cand_union_kernel<<<(unsigned)ceil_div(words, 256), 256, 0, s>>>(a);
+    IRIS_HIP_CHECK(hipGetLastError());
+  }
+  if (words && a.n_extra) {
+    cand_extra_kernel<<<(unsigned)ceil_div(a.n_extra, 256), 256, 0, s>>>(a);
+    IRIS_HIP_CHECK(hipGetLastError());
+  }
+  if (n_tiles) {
+    cand_count_kernel<<<(unsigned)n_tiles, 256, 0, s>>>(a);
+    IRIS_HIP_CHECK(hipGetLastError());
+  }
+  cand_scan_kernel<<<1, 256, 0, s>>>(a, n_tiles);
+  IRIS_HIP_CHECK(hipGetLastError());
+  if (n_tiles) {
+    cand_emit_kernel<<<(unsigned)n_tiles, 256, 0, s>>>(a);
+    IRIS_HIP_CHECK(hipGetLastError());
+  }
+}
+
+void launch_open_mapped(const MappedOpenArgs& a, hipStream_t s) {
+  if (!a.n_words) return;
+  open_mapped_kernel<<<(unsigned)ceil_div(a.n_words, 256), 256, 0, s>>>(a);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 

@@ -3,6 +3,7 @@
 #include "comparator.hpp"
 #include "query_prep.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -160,6 +161,16 @@ void ShareDB::replicate_records(DbSlice& db, size_t src, size_t n, size_t dst, h
                                 n / 16 * group_bytes, hipMemcpyDeviceToDevice, s));
   IRIS_HIP_CHECK(hipMemcpyAsync(db.sum0.ptr + dst, db.sum0.ptr + src, n * 4, hipMemcpyDeviceToDevice, s));
   IRIS_HIP_CHECK(hipMemcpyAsync(db.sum1.ptr + dst, db.sum1.ptr + src, n * 4, hipMemcpyDeviceToDevice, s));
+}
+
+void ShareDB::prefetch_subset(const DbSlice& src, const uint32_t* ids, size_t n, DbSlice& dst,
+                              hipStream_t s) const {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(src.code_length == code_length_ && dst.code_length == code_length_,
+               "subset and DB slices belong to another code length");
+  IRIS_REQUIRE(n <= dst.capacity - kMaxBD, "subset larger than the subset slice");
+  launch_gather_rows(src.limbs.ptr, src.sum0.ptr, src.sum1.ptr, src.capacity - kMaxBD, ids, n,
+                     dst.limbs.ptr, dst.sum0.ptr, dst.sum1.ptr, code_length_, s);
 }
 
 void ShareDB::load_records_limbs(DbSlice& db, size_t index, size_t n, const uint8_t* l0,
@@ -592,6 +603,14 @@ DistanceComparator::DistanceComparator(int device, size_t query_length, size_t m
   mcr_.alloc(B);
   prl_.alloc(kAllMatchesLen * B);
   prr_.alloc(kAllMatchesLen * B);
+  cand_.alloc(ceil_div(max_db_size, 64) + 1);
+  tiles_.alloc(ceil_div(max_db_size, 64 * 256) + 1);
+  ids_.alloc(max_db_size + 1);
+  count_.alloc(1);
+  cap_counter_.alloc(1);
+  cap_q_.alloc(kAllMatchesLen * query_length);
+  cap_db_.alloc(kAllMatchesLen * query_length);
+  cap_rot_.alloc(kAllMatchesLen * query_length);
 }
 
 std::unique_ptr<DevBuf<uint64_t>> DistanceComparator::prepare_db_match_list(size_t db_size) const {
@@ -617,6 +636,74 @@ void DistanceComparator::open_results(const uint64_t* a, const uint64_t* b, cons
   o.real_chunk_len = real_chunk_len;
   o.total_db_len = total_db_len;
   launch_open(o, s);
+}
+
+size_t DistanceComparator::get_partial_results(const uint64_t* bitmap, size_t db_size,
+                                               size_t batch_size, const uint32_t* extra_host,
+                                               size_t n_extra, hipStream_t s, size_t row_bits) {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(db_size <= max_db_size_, "db_size beyond the comparator's max_db_size");
+  IRIS_REQUIRE(batch_size * kRotations <= query_length_, "batch larger than the query rows");
+  if (row_bits == 0) row_bits = db_size;
+  IRIS_REQUIRE(row_bits >= db_size, "bitmap rows shorter than the DB");
+  if (n_extra) {
+    if (extra_.n < n_extra) extra_.alloc(n_extra, false);
+    IRIS_HIP_CHECK(hipMemcpyAsync(extra_.ptr, extra_host, n_extra * 4, hipMemcpyHostToDevice, s));
+  }
+  CandidateArgs c{};
+  c.bitmap = bitmap; c.row_bits = row_bits; c.batch = batch_size; c.db_size = db_size;
+  c.extra = extra_.ptr; c.n_extra = n_extra;
+  c.cand = cand_.ptr; c.tiles = tiles_.ptr; c.ids = ids_.ptr; c.count = count_.ptr;
+  launch_candidates(c, s);
+  uint32_t n = 0;
+  IRIS_HIP_CHECK(hipMemcpyAsync(&n, count_.ptr, 4, hipMemcpyDeviceToHost, s));
+  IRIS_HIP_CHECK(hipStreamSynchronize(s));
+  return n;
+}
+
+void DistanceComparator::open_results_mapped(const uint64_t* a, const uint64_t* b,
+                                             const uint64_t* c, uint64_t* bitmap,
+                                             size_t chunk_len, size_t real_chunk_len,
+                                             size_t total_db_len, size_t batch_size,
+                                             const uint32_t* index_map, hipStream_t s,
+                                             size_t rows, const uint32_t* slot_of) {
+  DeviceGuard g(device_);
+  if (rows == 0) rows = query_length_;
+  IRIS_REQUIRE(chunk_len % 64 == 0, "chunk length must be a multiple of 64");
+  IRIS_REQUIRE(real_chunk_len <= chunk_len, "real chunk longer than the chunk");
+  IRIS_REQUIRE(batch_size * kRotations <= rows && rows <= query_length_, "batch larger than the query rows");
+  MappedOpenArgs o{};
+  o.a = a; o.b = b; o.c = c; o.bitmap = bitmap;
+  o.n_words = chunk_len * rows / 64;
+  o.chunk_len = chunk_len;
+  o.query_len = batch_size * kRotations;
+  o.real_chunk_len = real_chunk_len;
+  o.total_db_len = total_db_len;
+  o.index_map = index_map;
+  o.slot_of = slot_of;
+  o.counter = cap_counter_.ptr;
+  o.cap_query = cap_q_.ptr;
+  o.cap_db = cap_db_.ptr;
+  o.cap_rot = cap_rot_.ptr;
+  o.capacity = cap_q_.n;
+  launch_open_mapped(o, s);
+}
+
+size_t DistanceComparator::fetch_partial_rotations(uint32_t* query, uint32_t* db, int8_t* rot,
+                                                   size_t capacity, hipStream_t s) {
+  DeviceGuard g(device_);
+  uint32_t n = 0;
+  IRIS_HIP_CHECK(hipMemcpyAsync(&n, cap_counter_.ptr, 4, hipMemcpyDeviceToHost, s));
+  IRIS_HIP_CHECK(hipStreamSynchronize(s));
+  const size_t m = std::min<size_t>({(size_t)n, capacity, cap_q_.n});
+  if (m) {
+    IRIS_HIP_CHECK(hipMemcpyAsync(query, cap_q_.ptr, m * 4, hipMemcpyDeviceToHost, s));
+    IRIS_HIP_CHECK(hipMemcpyAsync(db, cap_db_.ptr, m * 4, hipMemcpyDeviceToHost, s));
+    IRIS_HIP_CHECK(hipMemcpyAsync(rot, cap_rot_.ptr, m, hipMemcpyDeviceToHost, s));
+  }
+  fill_u32(cap_counter_.ptr, 1, 0, s);
+  IRIS_HIP_CHECK(hipStreamSynchronize(s));
+  return n;
 }
 
 void DistanceComparator::join_db_matches(const uint64_t* left, const uint64_t* right,

@@ -124,6 +124,10 @@ class ShareDB {
                            hipStream_t s) const;
   // copies rows [src, src + n) to [dst, dst + n) (multiples of 16, no overlap)
   void replicate_records(DbSlice& db, size_t src, size_t n, size_t dst, hipStream_t s) const;
+  // prefetch_subset (share_db.rs:344-407): rows ids[0..n) (device u32) of
+  // `src` to rows 0..n of `dst` (a subset slice, e.g. alloc_db(DB_CHUNK_SIZE))
+  void prefetch_subset(const DbSlice& src, const uint32_t* ids, size_t n, DbSlice& dst,
+                       hipStream_t s) const;
   // preprocess_query + htod + query_sums: rows x code_length u16 (already rotated
   // and Lagrange-preprocessed, like the reference's CompactQuery).
   std::unique_ptr<QuerySet> upload_query(const uint16_t* host_rows, size_t n_rows, hipStream_t s) const;
@@ -241,6 +245,29 @@ class DistanceComparator {
   void join_db_matches(const uint64_t* left, const uint64_t* right, uint32_t* final_results,
                        size_t db_size, const uint64_t* or_policy, size_t max_db_size,
                        hipStream_t s);
+  // get_partial_results (distance_comparator.rs:545-648) plus the OR-rule and
+  // reauth indices the actor appends (actor.rs:946-970): the ascending list of
+  // DB ids d < db_size set in any batch row of `bitmap` (rows row_bits bits
+  // apart; 0 = db_size) or listed in extra_host, kept on the device
+  // (partial_ids()). Returns its length (synchronises the stream).
+  size_t get_partial_results(const uint64_t* bitmap, size_t db_size, size_t batch_size,
+                             const uint32_t* extra_host, size_t n_extra, hipStream_t s,
+                             size_t row_bits = 0);
+  const uint32_t* partial_ids() const { return ids_.ptr; }
+  // open_results_with_index_mapping (distance_comparator.rs:281-342): subset
+  // chunk column j is DB row index_map[j] (device); matches also go to the
+  // rotation capture (partial_results_*, read by fetch_partial_rotations)
+  void open_results_mapped(const uint64_t* a, const uint64_t* b, const uint64_t* c,
+                           uint64_t* bitmap, size_t chunk_len, size_t real_chunk_len,
+                           size_t total_db_len, size_t batch_size, const uint32_t* index_map,
+                           hipStream_t s, size_t rows = 0, const uint32_t* slot_of = nullptr);
+  // get_partial_results_with_rotations (distance_comparator.rs:650-711):
+  // copies up to `capacity` captured (query, db id, rotation) triples to the
+  // host, resets the capture and returns the number of matches seen (may
+  // exceed the capture's size, 256 * query_length)
+  size_t fetch_partial_rotations(uint32_t* query, uint32_t* db, int8_t* rot, size_t capacity,
+                                 hipStream_t s);
+  size_t rotation_capture_size() const { return cap_q_.n; }
   uint32_t* match_counters() const { return match_counters_.ptr; }
   uint32_t* all_matches() const { return all_matches_.ptr; }
   uint32_t* match_counters_left() const { return mcl_.ptr; }
@@ -255,6 +282,10 @@ class DistanceComparator {
   int device_;
   size_t query_length_, max_db_size_;
   DevBuf<uint32_t> match_counters_, all_matches_, mcl_, mcr_, prl_, prr_;
+  DevBuf<uint64_t> cand_;
+  DevBuf<uint32_t> tiles_, ids_, count_, extra_;
+  DevBuf<uint32_t> cap_counter_, cap_q_, cap_db_;
+  DevBuf<int8_t> cap_rot_;
 };
 
 }  // namespace irismpc

@@ -136,6 +136,29 @@ __global__ void copy_row_kernel(const int8_t* __restrict__ src, const uint32_t* 
       *reinterpret_cast<const uint4*>(src + frag_offset(src_row / 16, kt, limb, n_kt) + ls * 16);
 }
 
+// Thread (j, piece): 16-byte piece `piece` (limb, 16-coefficient group) of
+// subset row j; consecutive threads take consecutive rows, so the writes of 16
+// rows fill one 256-byte stretch of a fragment.
+__global__ __launch_bounds__(256) void gather_rows_kernel(
+    const int8_t* __restrict__ src, const uint32_t* __restrict__ src_s0,
+    const uint32_t* __restrict__ src_s1, size_t src_rows, const uint32_t* __restrict__ ids,
+    size_t n, int8_t* __restrict__ dst, uint32_t* dst_s0, uint32_t* dst_s1, int K) {
+  const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const size_t r = ids[j];
+  if (r >= src_rows) return;
+  const int groups = K / 16, piece = (int)blockIdx.y;
+  if (piece == 0) {
+    dst_s0[j] = src_s0[r];
+    dst_s1[j] = src_s1[r];
+  }
+  const int limb = piece / groups, m = piece % groups;
+  const int n_kt = K / kKTile, kt = m / 4;
+  const int ls = (int)(r % 16) + 16 * (m % 4), ld = (int)(j % 16) + 16 * (m % 4);
+  *reinterpret_cast<uint4*>(dst + frag_offset(j / 16, kt, limb, n_kt) + ld * 16) =
+      *reinterpret_cast<const uint4*>(src + frag_offset(r / 16, kt, limb, n_kt) + ls * 16);
+}
+
 // Uniformly random limbs for rows [row0, row0 + n_rows): byte b of the tiled
 // row-group range comes from ChaCha20(key, nonce, counter = offset / 64).
 // Uniform u16 shares have exactly the distribution of real Shamir shares.
@@ -148,6 +171,16 @@ __global__ void fill_random_kernel(int8_t* __restrict__ dst, size_t byte0, size_
   uint4* d = reinterpret_cast<uint4*>(dst + byte0 + t * 64);
 #pragma unroll
   for (int i = 0; i < 4; i++) d[i] = uint4{ks[4 * i], ks[4 * i + 1], ks[4 * i + 2], ks[4 * i + 3]};
+}
+
+void launch_gather_rows(const int8_t* src, const uint32_t* src_s0, const uint32_t* src_s1,
+                        size_t src_rows, const uint32_t* ids, size_t n, int8_t* dst,
+                        uint32_t* dst_s0, uint32_t* dst_s1, int K, hipStream_t s) {
+  if (!n) return;
+  dim3 grid((unsigned)ceil_div(n, 256), (unsigned)(2 * (K / 16)));
+  gather_rows_kernel<<<grid, 256, 0, s>>>(src, src_s0, src_s1, src_rows, ids, n, dst, dst_s0,
+                                          dst_s1, K);
+  IRIS_HIP_CHECK(hipGetLastError());
 }
 
 void launch_fill_random(int8_t* dst, size_t row0, size_t n_rows, int K, const ChaChaKey& key,

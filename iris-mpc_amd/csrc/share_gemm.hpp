@@ -66,6 +66,12 @@ void launch_tile_limbs(const uint8_t* l0, const uint8_t* l1, size_t n_rows, int 
 void launch_copy_row(const int8_t* src, const uint32_t* src_s0, const uint32_t* src_s1,
                      size_t src_row, int8_t* dst, uint32_t* dst_s0, uint32_t* dst_s1,
                      size_t dst_row, int K, hipStream_t s);
+// prefetch_subset (share_db.rs:344-407): tiled rows ids[0..n) of `src` (ids
+// below src_rows; others leave their row alone) to rows 0..n of `dst`, sums
+// included.
+void launch_gather_rows(const int8_t* src, const uint32_t* src_s0, const uint32_t* src_s1,
+                        size_t src_rows, const uint32_t* ids, size_t n, int8_t* dst,
+                        uint32_t* dst_s0, uint32_t* dst_s1, int K, hipStream_t s);
 // Synthetic uniformly random shares (bench data; rows rounded up to 16).
 void launch_fill_random(int8_t* dst, size_t row0, size_t n_rows, int K, const ChaChaKey& key,
                         uint64_t nonce, uint32_t* sum0, uint32_t* sum1, hipStream_t s);

@@ -96,6 +96,12 @@ SIGNATURES = {
     "irismpc_comparator_open_results": [vp, vp, vp, vp, vp, sz, sz, sz, sz, sz, vp],
     "irismpc_comparator_join_db_matches": [vp, vp, vp, vp, sz, vp, sz, vp],
     "irismpc_comparator_reset": [vp, vp],
+    "irismpc_comparator_get_partial_results": [vp, vp, sz, sz, sz, vp, sz, vp, sz, vp, vp],
+    "irismpc_comparator_partial_ids": [vp, vp],
+    "irismpc_comparator_open_results_with_index_mapping": [vp, vp, vp, vp, vp, sz, sz, sz, sz, vp, vp],
+    "irismpc_comparator_fetch_partial_rotations": [vp, vp, vp, vp, sz, vp, vp],
+    "irismpc_db_prefetch_subset": [vp, vp, vp, sz, vp, vp],
+    "irismpc_scan_subset_trio": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, sz, sz, u32, vp, vp],
     "irismpc_comparator_buffers": [vp, vp, vp, vp, vp, vp, vp],
 }
 

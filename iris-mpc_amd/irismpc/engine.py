@@ -118,6 +118,11 @@ class SlicedProcessedDatabase:
         """Rows [src, src + n) -> [dst, dst + n) on the device (16-row multiples)."""
         call("irismpc_db_replicate_records", self.engine.h, self.h, src, n, dst, stream)
 
+    def prefetch_subset(self, ids_ptr, n, subset, stream=0):
+        """prefetch_subset (share_db.rs:344-407): rows ids[0..n) (device u32)
+        -> rows 0..n of `subset`."""
+        call("irismpc_db_prefetch_subset", self.engine.h, self.h, ids_ptr, n, subset.h, stream)
+
     def read_records(self, index, n, stream=0):
         out = np.zeros((n, self.engine.code_length), np.uint16)
         call("irismpc_db_read_records", self.engine.h, self.h, index, n, out.ctypes.data, stream)
@@ -379,6 +384,50 @@ class DistanceComparator:
     def reset(self, stream=0):
         call("irismpc_comparator_reset", self.h, stream)
 
+    def get_partial_results(self, bitmap_ptr, db_size, batch_size, extra_ids=(), row_bits=0,
+                            stream=0):
+        """get_partial_results (distance_comparator.rs:545-648) with the OR-rule /
+        reauth indices appended as the actor does (actor.rs:946-970): ascending
+        unique candidate DB ids (numpy u32). The list also stays on the device
+        (partial_ids_ptr)."""
+        extra = np.ascontiguousarray(np.asarray(extra_ids, dtype=np.uint32))
+        n = ctypes.c_size_t()
+        call("irismpc_comparator_get_partial_results", self.h, bitmap_ptr, db_size, batch_size,
+             row_bits, extra.ctypes.data if extra.size else None, extra.size, None, 0,
+             ctypes.byref(n), stream)
+        out = np.zeros(n.value, np.uint32)
+        if n.value:
+            call("irismpc_comparator_get_partial_results", self.h, bitmap_ptr, db_size, batch_size,
+                 row_bits, extra.ctypes.data if extra.size else None, extra.size, out.ctypes.data,
+                 out.size, ctypes.byref(n), stream)
+        return out
+
+    def partial_ids_ptr(self):
+        p = _out()
+        call("irismpc_comparator_partial_ids", self.h, ctypes.byref(p))
+        return p.value
+
+    def open_results_with_index_mapping(self, a, b, c, bitmap_ptr, chunk_len, real_chunk_len,
+                                        total_db_len, batch_size, index_map_ptr, stream=0):
+        call("irismpc_comparator_open_results_with_index_mapping", self.h, a, b, c, bitmap_ptr,
+             chunk_len, real_chunk_len, total_db_len, batch_size, index_map_ptr, stream)
+
+    def get_partial_results_with_rotations(self, stream=0):
+        """distance_comparator.rs:650-711: {query: {db id: [rotations]}} of the
+        index-mapped opens since the last call (the capture is reset)."""
+        cap = 256 * self.query_length
+        q = np.zeros(cap, np.uint32)
+        d = np.zeros(cap, np.uint32)
+        r = np.zeros(cap, np.int8)
+        n = ctypes.c_size_t()
+        call("irismpc_comparator_fetch_partial_rotations", self.h, q.ctypes.data, d.ctypes.data,
+             r.ctypes.data, cap, ctypes.byref(n), stream)
+        out = {}
+        for qi, di, ri in zip(q[:min(n.value, cap)].tolist(), d[:min(n.value, cap)].tolist(),
+                              r[:min(n.value, cap)].tolist()):
+            out.setdefault(qi, {}).setdefault(di, []).append(ri)
+        return out
+
     def buffers(self):
         outs = [_out() for _ in range(6)]
         call("irismpc_comparator_buffers", self.h, *[ctypes.byref(o) for o in outs])
@@ -414,6 +463,21 @@ def scan_chunk_trio(parties, q_codes, q_masks, offset, db_size, batch_size, thre
          ptr_array([q.h for q in q_masks]), offset, db_size, batch_size, threshold_a,
          ptr_array(list(bitmap_ptrs)), ptr_array(list(slot_ptrs)) if slot_ptrs is not None else None,
          row_bits, ptr_array(list(streams)))
+
+
+def scan_subset_trio(parties, code_subs, mask_subs, q_codes, q_masks, ids_ptrs, n_ids, total_db_len,
+                     batch_size, threshold_a, bitmap_ptrs, streams):
+    """The DB half of compare_query_against_db_subset_and_self (actor.rs:
+    1900-2244) for the three parties: subset ids (device u32 per party) are
+    gathered into the subset slices, compared at threshold_a and opened at
+    their DB ids."""
+    call("irismpc_scan_subset_trio", ptr_array([p.codes.h for p in parties]),
+         ptr_array([p.masks.h for p in parties]), ptr_array([p.circuits.h for p in parties]),
+         ptr_array([p.comparator.h for p in parties]), ptr_array([p.code_db.h for p in parties]),
+         ptr_array([p.mask_db.h for p in parties]), ptr_array([d.h for d in code_subs]),
+         ptr_array([d.h for d in mask_subs]), ptr_array([q.h for q in q_codes]),
+         ptr_array([q.h for q in q_masks]), ptr_array(list(ids_ptrs)), n_ids, total_db_len,
+         batch_size, threshold_a, ptr_array(list(bitmap_ptrs)), ptr_array(list(streams)))
 
 
 def scan_db_trio(parties, q_codes, q_masks, db_size, batch_size, threshold_a, bitmap_ptrs, streams):

@@ -1,0 +1,212 @@
+"""Two-stage matching flow (iris-mpc-gpu/src/server/actor.rs:860-1131) on one
+GPU: full scan of the left eye at 0.375, candidate subset (get_partial_results
++ OR-rule / reauth ids), both eyes compared on the subset at 0.345
+(prefetch_subset + openResultsWithIndexMapping), AND / OR-rule merge.
+
+test_two_stage_vs_plaintext: every output against a plaintext brute force with
+the integer MPC predicate, per rotation: the candidate list, both eyes'
+opened bitmaps, the (query, DB id, rotation) captures, the merged results,
+counters and match lists; and the merged answer equals that of a full scan of
+both eyes at 0.345. The queries cover: a match in both eyes; a left-eye
+distance between 0.345 and 0.375 (a candidate that fails the second stage);
+left-only and right-only matches under the OR rule and without it; a query
+that is a rotated copy of its DB iris (rotation capture); a reauth target.
+
+test_two_stage_multi_chunk_subset: 40,960 candidates (more than one subset
+chunk of 32,768) on a 65,536-row DB of 64 distinct irises tiled, answer known
+by construction.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+A_0375 = 16384
+
+
+@pytest.fixture(scope="module")
+def irismpc():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import irismpc as m
+
+    m.lib()
+    return m
+
+
+def _rotation_bits(orc, db_c, db_m, q_c, q_m, A):
+    """[B][31][N] plaintext MPC predicate per query rotation."""
+    N, B = len(db_c), len(q_c)
+    bits = np.zeros((B, 31, N), bool)
+    for j in range(B):
+        rc, rm = orc.iris_all_rotations(q_c[j]), orc.iris_all_rotations(q_m[j])
+        for t in range(31):
+            mm = rm[t][None, :] & db_m
+            M = np.bitwise_count(mm).sum(axis=1).astype(np.int64)
+            H = np.bitwise_count((rc[t][None, :] ^ db_c) & mm).sum(axis=1).astype(np.int64)
+            c = (M - 2 * H) & 0xFFFF
+            r = (M * A - (c << 16) - 1) & 0xFFFFFFFF
+            bits[j, t] = (r >> 31).astype(bool)
+    return bits
+
+
+def _engines(irismpc, N, Qr, tag_base=0):
+    base = [np.full(8, p + tag_base, np.uint32) for p in range(3)]
+    ex = irismpc.Exchange(timeout_s=120)
+    chunk = irismpc.DB_CHUNK_SIZE
+    out = []
+    for p in range(3):
+        der = lambda n: (irismpc.derive_seed(base[p], n), irismpc.derive_seed(base[(p + 2) % 3], n))  # noqa: E731
+        codes = irismpc.ShareDB(p, chunk, Qr, 12800, der(0), comm=ex, tag=0)
+        masks = irismpc.ShareDB(p, chunk, Qr, 6400, der(1), comm=ex, tag=1)
+        circ = irismpc.Circuits(p, Qr * chunk, Qr * chunk // 64, der(5), ex, tag=2)
+        cmp = irismpc.DistanceComparator(Qr, N)
+        out.append(irismpc.PartyEngines(codes, masks, circ, cmp, codes.alloc_db(N), masks.alloc_db(N)))
+    return ex, out
+
+
+def test_two_stage_vs_plaintext(irismpc, orc):
+    from irismpc import synth
+    from irismpc.two_stage import TwoStageMatcher
+
+    N, B = 2000, 8
+    Qr = B * 31
+    rng = np.random.default_rng(77)
+    t = rng.choice(N, size=8, replace=False)
+    db = {}
+    for e, ds in (("left", 3), ("right", 4)):
+        c, m = zip(*[orc.synth_iris(ds, i) for i in range(N)])
+        db[e] = (np.array(c), np.array(m))
+    flips = {  # query j: (left target / flips, right target / flips); None = random iris
+        0: ((t[0], 1280), (t[0], 1280)),
+        1: ((t[1], 4608), (t[1], 1280)),   # left FHD ~0.36: candidate, fails 0.345
+        2: ((t[2], 1280), None),           # left only, OR rule
+        3: (None, (t[3], 1280)),           # right only, OR rule (candidate via the rule)
+        4: ((t[4], 1280), None),           # left only, AND
+        5: ("rot", "rot"),                 # rotated copy of DB iris t[5]
+        6: (None, None),
+        7: ((t[7], 2000), (t[7], 1500)),
+    }
+    q = {"left": ([], []), "right": ([], [])}
+    for j in range(B):
+        for side, e in enumerate(("left", "right")):
+            spec = flips[j][side]
+            dc, dm = db[e]
+            if spec is None:
+                c, m = orc.synth_iris(20 + side, j)
+            elif spec == "rot":
+                c = orc.synth_flip(orc.iris_all_rotations(dc[t[5]])[12], 40 + side, j, 640)
+                m = orc.iris_all_rotations(dm[t[5]])[12]
+            else:
+                c, m = orc.synth_flip(dc[spec[0]], 30 + side, j, spec[1]), dm[spec[0]]
+            q[e][0].append(c)
+            q[e][1].append(m)
+    q = {e: (np.array(v[0]), np.array(v[1])) for e, v in q.items()}
+    or_rule = [[] for _ in range(B)]
+    or_rule[2] = [int(t[2])]
+    or_rule[3] = [int(t[3])]
+    reauth = [int(t[6]), N + 5]  # the second is beyond the DB and skipped
+
+    eyes, queries, keep = {}, {}, []
+    for k, e in enumerate(("left", "right")):
+        ex, parties = _engines(irismpc, N, Qr, tag_base=3 * k)
+        keep.append(ex)
+        db_sh = synth.shares_of(db[e][0], db[e][1], 50 + k)
+        q_sh = synth.shares_of(q[e][0], q[e][1], 60 + k)
+        for p in range(3):
+            parties[p].code_db.load_full_db(db_sh[p][0])
+            parties[p].mask_db.load_full_db(db_sh[p][1])
+        eyes[e] = parties
+        queries[e] = ([parties[p].codes.upload_query_shares(q_sh[p][0]) for p in range(3)],
+                      [parties[p].masks.upload_query_shares(q_sh[p][1]) for p in range(3)])
+
+    m = TwoStageMatcher(eyes, N, B, full_scan_side="left")
+    res = m.match(queries, N, B, or_rule_indices=or_rule, reauth_targets=reauth)
+
+    # plaintext answer
+    l375 = _rotation_bits(orc, *db["left"], *q["left"], A_0375).any(axis=1)
+    lr = _rotation_bits(orc, *db["left"], *q["left"], irismpc.THRESHOLD_A)
+    rr = _rotation_bits(orc, *db["right"], *q["right"], irismpc.THRESHOLD_A)
+    extra = {int(d) for ids in or_rule for d in ids} | {d for d in reauth if d < N}
+    cand = np.array(sorted(set(np.nonzero(l375.any(axis=0))[0].tolist()) | extra), np.uint32)
+    assert l375[1, t[1]] and not lr[1, :, t[1]].any(), "query 1 must fall between the thresholds"
+    assert np.array_equal(res.candidates, cand)
+    cmask = np.zeros(N, bool)
+    cmask[cand] = True
+    L, R = lr.any(axis=1) & cmask, rr.any(axis=1) & cmask
+    or_pair = np.zeros((B, N), bool)
+    for j, ids in enumerate(or_rule):
+        or_pair[j, ids] = True
+    merged = np.where(or_pair, L | R, L & R)
+    full = np.where(or_pair, lr.any(axis=1) | rr.any(axis=1), lr.any(axis=1) & rr.any(axis=1))
+    assert np.array_equal(merged, full), "two-stage answer differs from the full scan at 0.345"
+    want = np.array([np.nonzero(r)[0].min() if r.any() else 0xFFFFFFFF for r in merged], np.uint32)
+    assert want[0] == t[0] and want[2] == t[2] and want[3] == t[3] and want[5] == t[5] and want[7] == t[7]
+    assert want[1] == want[4] == want[6] == 0xFFFFFFFF
+
+    for p in range(3):
+        assert np.array_equal(res.final_results[p], want), f"party {p} final results"
+        assert np.array_equal(res.match_counters[p], merged.sum(axis=1)), f"party {p} match counters"
+        for j in range(B):
+            n = int(merged[j].sum())
+            assert set(res.all_matches[p][j][:n].tolist()) == set(np.nonzero(merged[j])[0].tolist())
+        for e, bits in (("left", L), ("right", R)):
+            cnt, lists = res.partial[e][p]
+            assert np.array_equal(cnt, bits.sum(axis=1)), f"party {p} {e} partial counters"
+            for j in range(B):
+                assert set(lists[j][:int(cnt[j])].tolist()) == set(np.nonzero(bits[j])[0].tolist())
+        for e, rb in (("left", lr), ("right", rr)):
+            exp = {(j, int(d), tt - 15) for j, tt, d in zip(*np.nonzero(rb & cmask[None, None, :]))}
+            got = {(qi, di, r) for qi, dd in res.rotations[e][p].items() for di, rs in dd.items() for r in rs}
+            n_got = sum(len(rs) for dd in res.rotations[e][p].values() for rs in dd.values())
+            assert got == exp and n_got == len(exp), f"party {p} {e} rotation capture"
+    assert (5, int(t[5]), 3) in {(qi, di, r) for qi, dd in res.rotations["left"][0].items()
+                                 for di, rs in dd.items() for r in rs}
+
+
+def test_two_stage_multi_chunk_subset(irismpc):
+    import torch
+
+    from irismpc import synth
+    from irismpc.two_stage import TwoStageMatcher
+
+    N, U, B = 65536, 64, 64
+    Qr = B * 31
+    rng = np.random.default_rng(5)
+    targets = rng.choice(U, size=40, replace=False)
+    eyes, queries, keep = {}, {}, []
+    for k, e in enumerate(("left", "right")):
+        ex, parties = _engines(irismpc, N, Qr, tag_base=3 * k)
+        keep.append(ex)
+        ds = 70 + k
+        synth.build_db([pe.code_db for pe in parties], [pe.mask_db for pe in parties], N, U, ds, ds + 30)
+        # left eye: all 40 planted; right eye: the even ones
+        planted = [(j, int(targets[j]), 1000 + 30 * j) for j in range(40) if e == "left" or j % 2 == 0]
+        _, _, sh = synth.make_queries(B, planted, ds, ds + 20, ds + 10, ds + 40)
+        eyes[e] = parties
+        queries[e] = ([parties[p].codes.upload_query_shares(sh[p][0]) for p in range(3)],
+                      [parties[p].masks.upload_query_shares(sh[p][1]) for p in range(3)])
+    torch.cuda.synchronize()
+    print("DBs built, matching", flush=True)
+    res = TwoStageMatcher(eyes, N, B).match(queries, N, B)
+
+    rows = np.arange(N, dtype=np.uint32)
+    cand = rows[np.isin(rows % U, targets)]
+    assert cand.size == 40 * N // U > irismpc.DB_CHUNK_SIZE
+    assert np.array_equal(res.candidates, cand)
+    want = np.full(B, 0xFFFFFFFF, np.uint32)
+    want[0:40:2] = targets[0:40:2]
+    cnt = np.zeros(B, np.uint32)
+    cnt[0:40:2] = N // U
+    for p in range(3):
+        assert np.array_equal(res.final_results[p], want), f"party {p} final results"
+        assert np.array_equal(res.match_counters[p], cnt), f"party {p} match counters"
+        lc, _ = res.partial["left"][p]
+        assert np.array_equal(lc[:40], np.full(40, N // U)) and not lc[40:].any()
+        left = res.rotations["left"][p]
+        assert sorted(left) == list(range(40))
+        for j in range(40):
+            assert sorted(left[j]) == cand[(cand % U) == targets[j]].tolist()
+            assert all(r == [0] for r in left[j].values())
