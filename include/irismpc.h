@@ -256,6 +256,15 @@ int irismpc_comparator_get_partial_results(irismpc_comparator* d, const uint64_t
                                            const uint32_t* extra_ids, size_t n_extra,
                                            uint32_t* host_ids, size_t host_capacity,
                                            size_t* n_ids, void* stream);
+/* DistanceComparator::join_batch_matches (distance_comparator.rs:731-746,
+ * mergeBatchResults dot/kernel.cu:352-386) over the batch self-comparison
+ * bitmaps of irismpc_scan_batch_self_trio (bit row_bits * q + j: query q
+ * matches query j; row_bits 0 = batch_size): for j < q matching in both
+ * eyes, final_results[q] = min(final_results[q], 0xFFFFFFFE) and
+ * 0xFFFFFFFF - j is appended to q's match list (counters as for DB matches). */
+int irismpc_comparator_join_batch_matches(irismpc_comparator* d, const uint64_t* left,
+                                          const uint64_t* right, uint32_t* final_results,
+                                          size_t batch_size, size_t row_bits, void* stream);
 /* Device pointer of the last get_partial_results list. */
 int irismpc_comparator_partial_ids(irismpc_comparator* d, const uint32_t** ids);
 /* open_results_with_index_mapping (distance_comparator.rs:281-342,
@@ -331,6 +340,20 @@ int irismpc_scan_subset_trio(irismpc_share_db* const codes[3], irismpc_share_db*
                              size_t n_ids, size_t total_db_len, size_t batch_size,
                              uint32_t threshold_a, uint64_t* const bitmaps[3],
                              void* const streams[3]);
+/* ServerActor::compare_query_against_self (actor.rs:1789-1897) for the three
+ * parties: the batch's queries (all rotations) against the batch's own
+ * irises in DB form at threshold_a; bit row_bits * q + j of bitmaps[p] = some
+ * rotation of query q matches query j (row_bits 0 = batch_size). Needs query
+ * sets uploaded from raw shares (irismpc_query_upload_shares[_rows]). The
+ * reference also computes the rotated DB-side columns, which its merge never
+ * reads; they are skipped here. */
+int irismpc_scan_batch_self_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
+                                 irismpc_circuits* const circuits[3],
+                                 irismpc_comparator* const cmp[3],
+                                 const irismpc_query* const q_code[3],
+                                 const irismpc_query* const q_mask[3], size_t batch_size,
+                                 uint32_t threshold_a, uint64_t* const bitmaps[3], size_t row_bits,
+                                 void* const streams[3]);
 
 /* ------------------------------------------------ HIP graphs */
 /* Captures the library calls issued on `stream` between begin and end into a

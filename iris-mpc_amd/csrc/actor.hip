@@ -65,6 +65,20 @@ void scan_subset(const PartyScan& p, DbSlice& code_sub, DbSlice& mask_sub, const
   }
 }
 
+void scan_batch_self(const PartyScan& p, size_t batch_size, uint32_t threshold_a,
+                     uint64_t* bitmap, size_t row_bits, hipStream_t s) {
+  IRIS_REQUIRE(p.code_query->insert && p.mask_query->insert,
+               "the batch self-comparison needs query sets uploaded from raw shares");
+  IRIS_REQUIRE(batch_size >= 1 && batch_size <= (size_t)p.code_query->batch &&
+                   batch_size <= (size_t)p.mask_query->batch,
+               "batch larger than the uploaded queries");
+  PartyScan b = p;
+  b.code_db = p.code_query->insert.get();
+  b.mask_db = p.mask_query->insert.get();
+  scan_chunk(b, 0, batch_size, batch_size, threshold_a, bitmap, nullptr,
+             row_bits ? row_bits : batch_size, s);
+}
+
 void scan_db(const PartyScan& p, size_t db_size, size_t batch_size, uint32_t threshold_a,
              uint64_t* bitmap, hipStream_t s) {
   size_t offset = 0;
@@ -111,6 +125,11 @@ void scan_subset_trio(const PartyScan p[3], DbSlice* const code_sub[3], DbSlice*
     scan_subset(p[i], *code_sub[i], *mask_sub[i], ids[i], n_ids, total_db_len, batch_size,
                 threshold_a, bitmaps[i], s[i]);
   });
+}
+
+void scan_batch_self_trio(const PartyScan p[3], size_t batch_size, uint32_t threshold_a,
+                          uint64_t* const bitmaps[3], size_t row_bits, hipStream_t const s[3]) {
+  run_trio([&](int i) { scan_batch_self(p[i], batch_size, threshold_a, bitmaps[i], row_bits, s[i]); });
 }
 
 void scan_db_trio(const PartyScan p[3], size_t db_size, size_t batch_size, uint32_t threshold_a,

@@ -53,6 +53,17 @@ void scan_subset_trio(const PartyScan p[3], DbSlice* const code_sub[3], DbSlice*
                       size_t batch_size, uint32_t threshold_a, uint64_t* const bitmaps[3],
                       hipStream_t const s[3]);
 
+// compare_query_against_self (actor.rs:1789-1897): the batch's query rows
+// (all rotations) against the batch's own irises in DB form (the query sets'
+// insert slices) at threshold_a. Bit row_bits * q + j (row_bits >= batch;
+// 0 = batch) = some rotation of query q matches query j. The reference's batch
+// DB holds all 31 rotations of each query, of which mergeBatchResults reads
+// only the unrotated one; only those columns are computed here.
+void scan_batch_self(const PartyScan& p, size_t batch_size, uint32_t threshold_a,
+                     uint64_t* bitmap, size_t row_bits, hipStream_t s);
+void scan_batch_self_trio(const PartyScan p[3], size_t batch_size, uint32_t threshold_a,
+                          uint64_t* const bitmaps[3], size_t row_bits, hipStream_t const s[3]);
+
 // Runs the three parties' scans from one call (one host thread per party).
 void scan_db_trio(const PartyScan p[3], size_t db_size, size_t batch_size, uint32_t threshold_a,
                   uint64_t* const bitmaps[3], hipStream_t const s[3]);

@@ -500,6 +500,13 @@ int irismpc_comparator_get_partial_results(irismpc_comparator* d, const uint64_t
     }
   });
 }
+int irismpc_comparator_join_batch_matches(irismpc_comparator* d, const uint64_t* left,
+                                          const uint64_t* right, uint32_t* final_results,
+                                          size_t batch_size, size_t row_bits, void* stream) {
+  return guarded([&] {
+    d->impl->join_batch_matches(left, right, final_results, batch_size, row_bits, st(stream));
+  });
+}
 int irismpc_comparator_partial_ids(irismpc_comparator* d, const uint32_t** ids) {
   return guarded([&] { *ids = d->impl->partial_ids(); });
 }
@@ -623,6 +630,30 @@ int irismpc_scan_subset_trio(irismpc_share_db* const codes[3], irismpc_share_db*
       ms[i] = mask_sub[i]->impl.get();
     }
     scan_subset_trio(p, cs, ms, ids, n_ids, total_db_len, batch_size, threshold_a, bitmaps, s);
+  });
+}
+
+int irismpc_scan_batch_self_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
+                                 irismpc_circuits* const circuits[3],
+                                 irismpc_comparator* const cmp[3],
+                                 const irismpc_query* const q_code[3],
+                                 const irismpc_query* const q_mask[3], size_t batch_size,
+                                 uint32_t threshold_a, uint64_t* const bitmaps[3], size_t row_bits,
+                                 void* const streams[3]) {
+  return guarded([&] {
+    PartyScan p[3];
+    hipStream_t s[3];
+    for (int i = 0; i < 3; i++) {
+      p[i] = PartyScan{};
+      p[i].codes = codes[i]->impl.get();
+      p[i].masks = masks[i]->impl.get();
+      p[i].circuits = circuits[i]->impl.get();
+      p[i].comparator = cmp[i]->impl.get();
+      p[i].code_query = q_code[i]->impl.get();
+      p[i].mask_query = q_mask[i]->impl.get();
+      s[i] = st(streams[i]);
+    }
+    scan_batch_self_trio(p, batch_size, threshold_a, bitmaps, row_bits, s);
   });
 }
 

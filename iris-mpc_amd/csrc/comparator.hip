@@ -47,6 +47,27 @@ __global__ __launch_bounds__(256) void merge_kernel(const MergeArgs a) {
   }
 }
 
+// mergeBatchResults (dot/kernel.cu:352-386) on the compact batch bitmaps:
+// one thread per word, visiting the bits set in both eyes.
+__global__ __launch_bounds__(256) void merge_batch_kernel(const BatchMergeArgs a) {
+  const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = a.batch * a.row_bits;
+  if (w * 64 >= total) return;
+  uint64_t v = a.left[w] & a.right[w];
+  const size_t valid = total - w * 64;
+  if (valid < 64) v &= (1ull << valid) - 1;
+  while (v) {
+    const int i = __builtin_ctzll(v);
+    v &= v - 1;
+    const size_t bit = w * 64 + (size_t)i;
+    const uint32_t q = (uint32_t)(bit / a.row_bits), j = (uint32_t)(bit % a.row_bits);
+    if (j >= q) continue;  // only earlier queries of the batch
+    atomicMin(&a.final_results[q], 0xFFFFFFFEu);
+    const uint32_t c = atomicAdd(&a.match_counters[q], 1u);
+    if (c < kAllMatchesLen) a.all_matches[kAllMatchesLen * q + c] = 0xFFFFFFFFu - j;
+  }
+}
+
 // Exclusive prefix sum over a 256-thread block; *total = the block's sum.
 // lds holds the four wave totals (the caller syncs before reusing it).
 __device__ inline uint32_t block_exclusive_scan256(uint32_t v, uint32_t* lds, uint32_t* total) {
@@ -175,6 +196,13 @@ void launch_merge(const MergeArgs& a, hipStream_t s) {
   const size_t words = ceil_div(a.query_len * a.db_len, 64);
   if (!words) return;
   merge_kernel<<<(unsigned)ceil_div(words, 256), 256, 0, s>>>(a);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_merge_batch(const BatchMergeArgs& a, hipStream_t s) {
+  const size_t words = ceil_div(a.batch * a.row_bits, 64);
+  if (!words) return;
+  merge_batch_kernel<<<(unsigned)ceil_div(words, 256), 256, 0, s>>>(a);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 

@@ -22,6 +22,17 @@ struct MergeArgs {
 
 void launch_merge(const MergeArgs& a, hipStream_t s);
 
+struct BatchMergeArgs {
+  const uint64_t* left;
+  const uint64_t* right;
+  uint32_t* final_results;
+  uint32_t* match_counters;
+  uint32_t* all_matches;
+  size_t batch;
+  size_t row_bits;
+};
+void launch_merge_batch(const BatchMergeArgs& a, hipStream_t s);
+
 // Candidate DB ids of a match bitmap (partialDbResults, dot/kernel.cu:212-239,
 // and the host-side unique + sort of get_partial_results,
 // distance_comparator.rs:545-648): every id d < db_size with bit

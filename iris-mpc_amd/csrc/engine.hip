@@ -195,10 +195,18 @@ void ShareDB::write_at_index(DbSlice& db, const QuerySet& q, size_t src_index, s
                              hipStream_t s) const {
   DeviceGuard g(device_);
   IRIS_REQUIRE(q.code_length == code_length_, "query/engine code length mismatch");
-  // reference copies the centre rotation (rotation 15) of query src_index
+  IRIS_REQUIRE(dst_index < db.capacity - kMaxBD, "dst_index beyond DB capacity");
+  if (q.insert) {  // the query's DB form
+    IRIS_REQUIRE(src_index < (size_t)q.batch, "src_index beyond query batch");
+    const DbSlice& in = *q.insert;
+    launch_copy_row(in.limbs.ptr, in.sum0.ptr, in.sum1.ptr, src_index, db.limbs.ptr, db.sum0.ptr,
+                    db.sum1.ptr, dst_index, code_length_, s);
+    return;
+  }
+  // rows given by the caller: the centre rotation (rotation 15) of query
+  // src_index, as the reference copies it from its insert-form rows
   const size_t src_row = src_index * kRotations + 15;
   IRIS_REQUIRE(src_row < (size_t)q.rows, "src_index beyond query batch");
-  IRIS_REQUIRE(dst_index < db.capacity - kMaxBD, "dst_index beyond DB capacity");
   launch_copy_row(q.limbs.ptr, q.sum0.ptr, q.sum1.ptr, src_row, db.limbs.ptr, db.sum0.ptr,
                   db.sum1.ptr, dst_index, code_length_, s);
 }
@@ -254,6 +262,10 @@ std::unique_ptr<QuerySet> ShareDB::upload_query_shares_rows(const uint16_t* host
     q->rot.alloc(rot_planes_bytes((int)n_rows, code_length_));
     launch_rot_planes(rows.ptr, (int)n_rows, code_length_, q->rot.ptr, s);
   }
+  q->batch = (int)batch;
+  q->insert = alloc_db(round_up(batch, 512));  // room for a padded self-comparison chunk
+  launch_tile_rows(raw.ptr, batch, code_length_, q->insert->limbs.ptr, 0, q->insert->sum0.ptr,
+                   q->insert->sum1.ptr, s);
   IRIS_HIP_CHECK(hipStreamSynchronize(s));
   return q;
 }
@@ -719,6 +731,20 @@ void DistanceComparator::join_db_matches(const uint64_t* left, const uint64_t* r
   m.db_len = db_size;
   m.max_db_len = max_db_size;
   launch_merge(m, s);
+}
+
+void DistanceComparator::join_batch_matches(const uint64_t* left, const uint64_t* right,
+                                            uint32_t* final_results, size_t batch_size,
+                                            size_t row_bits, hipStream_t s) {
+  DeviceGuard g(device_);
+  if (row_bits == 0) row_bits = batch_size;
+  IRIS_REQUIRE(batch_size * kRotations <= query_length_, "batch larger than the query rows");
+  IRIS_REQUIRE(row_bits >= batch_size, "batch bitmap rows shorter than the batch");
+  BatchMergeArgs m{};
+  m.left = left; m.right = right; m.final_results = final_results;
+  m.match_counters = match_counters_.ptr; m.all_matches = all_matches_.ptr;
+  m.batch = batch_size; m.row_bits = row_bits;
+  launch_merge_batch(m, s);
 }
 
 void DistanceComparator::reset(hipStream_t s) {

@@ -104,6 +104,12 @@ struct QuerySet {
   // padded unrotated limb planes; present when the rows are the 31 rotations
   // of each query (built from raw shares on the device)
   DevBuf<int8_t> rot;
+  // The batch's queries in DB form (their raw shares, unrotated, not
+  // interpolated: the centre rotation of the reference's code_query_insert),
+  // present when uploaded from raw shares. Source of write_at_index and the
+  // DB side of the batch self-comparison.
+  int batch = 0;
+  std::unique_ptr<DbSlice> insert;
 };
 
 class ShareDB {
@@ -245,6 +251,13 @@ class DistanceComparator {
   void join_db_matches(const uint64_t* left, const uint64_t* right, uint32_t* final_results,
                        size_t db_size, const uint64_t* or_policy, size_t max_db_size,
                        hipStream_t s);
+  // join_batch_matches (distance_comparator.rs:731-746, mergeBatchResults
+  // dot/kernel.cu:352-386) over batch self-comparison bitmaps: bit
+  // row_bits * q + j = query q matches query j's unrotated iris. For j < q
+  // matching in both eyes: final_results[q] = min(., 0xFFFFFFFE) and
+  // 0xFFFFFFFF - j joins q's match list.
+  void join_batch_matches(const uint64_t* left, const uint64_t* right, uint32_t* final_results,
+                          size_t batch_size, size_t row_bits, hipStream_t s);
   // get_partial_results (distance_comparator.rs:545-648) plus the OR-rule and
   // reauth indices the actor appends (actor.rs:946-970): the ascending list of
   // DB ids d < db_size set in any batch row of `bitmap` (rows row_bits bits

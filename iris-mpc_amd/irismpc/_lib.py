@@ -98,6 +98,8 @@ SIGNATURES = {
     "irismpc_comparator_reset": [vp, vp],
     "irismpc_comparator_get_partial_results": [vp, vp, sz, sz, sz, vp, sz, vp, sz, vp, vp],
     "irismpc_comparator_partial_ids": [vp, vp],
+    "irismpc_comparator_join_batch_matches": [vp, vp, vp, vp, sz, sz, vp],
+    "irismpc_scan_batch_self_trio": [vp, vp, vp, vp, vp, vp, sz, u32, vp, sz, vp],
     "irismpc_comparator_open_results_with_index_mapping": [vp, vp, vp, vp, vp, sz, sz, sz, sz, vp, vp],
     "irismpc_comparator_fetch_partial_rotations": [vp, vp, vp, vp, sz, vp, vp],
     "irismpc_db_prefetch_subset": [vp, vp, vp, sz, vp, vp],

@@ -402,6 +402,12 @@ class DistanceComparator:
                  out.size, ctypes.byref(n), stream)
         return out
 
+    def join_batch_matches(self, left, right, final_results, batch_size, row_bits=0, stream=0):
+        """join_batch_matches (distance_comparator.rs:731-746) over batch
+        self-comparison bitmaps (bit row_bits * q + j)."""
+        call("irismpc_comparator_join_batch_matches", self.h, left, right, final_results, batch_size,
+             row_bits, stream)
+
     def partial_ids_ptr(self):
         p = _out()
         call("irismpc_comparator_partial_ids", self.h, ctypes.byref(p))
@@ -478,6 +484,18 @@ def scan_subset_trio(parties, code_subs, mask_subs, q_codes, q_masks, ids_ptrs, 
          ptr_array([d.h for d in mask_subs]), ptr_array([q.h for q in q_codes]),
          ptr_array([q.h for q in q_masks]), ptr_array(list(ids_ptrs)), n_ids, total_db_len,
          batch_size, threshold_a, ptr_array(list(bitmap_ptrs)), ptr_array(list(streams)))
+
+
+def scan_batch_self_trio(parties, q_codes, q_masks, batch_size, threshold_a, bitmap_ptrs, streams,
+                         row_bits=0):
+    """compare_query_against_self (actor.rs:1789-1897) for the three parties:
+    bit row_bits * q + j = query q matches query j (query sets uploaded from
+    raw shares)."""
+    call("irismpc_scan_batch_self_trio", ptr_array([p.codes.h for p in parties]),
+         ptr_array([p.masks.h for p in parties]), ptr_array([p.circuits.h for p in parties]),
+         ptr_array([p.comparator.h for p in parties]), ptr_array([q.h for q in q_codes]),
+         ptr_array([q.h for q in q_masks]), batch_size, threshold_a, ptr_array(list(bitmap_ptrs)),
+         row_bits, ptr_array(list(streams)))
 
 
 def scan_db_trio(parties, q_codes, q_masks, db_size, batch_size, threshold_a, bitmap_ptrs, streams):

@@ -12,14 +12,22 @@ on one GPU (iris-mpc-gpu/src/server/actor.rs:860-1100):
 4. the two eye bitmaps are merged with the AND rule, or per (query, DB id) the
    OR rule (join_db_matches[_with_bitmaps], actor.rs:1097-1131).
 
+With self_compare, each eye's queries are also compared against the batch's
+own irises before its subset scan (compare_query_against_self,
+actor.rs:1789-1897), and the batch bitmaps are merged after the DB ones
+(join_batch_matches, actor.rs:1132-1137): a query matching an earlier query of
+the batch in both eyes gets final result 0xFFFFFFFE and match id
+0xFFFFFFFF - (that query's position).
+
 Matches at 0.345 are a subset of the matches at 0.375, so the answer equals a
 full scan of both eyes at 0.345 merged the same way, at the cost of one full
 scan instead of two.
 """
 import numpy as np
 
-from .engine import (DB_CHUNK_SIZE, THRESHOLD_A, THRESHOLD_ANON_STATS_A, download, scan_db_trio,
-                     scan_subset_trio, stream_create, stream_synchronize)
+from .engine import (DB_CHUNK_SIZE, THRESHOLD_A, THRESHOLD_ANON_STATS_A, download,
+                     scan_batch_self_trio, scan_db_trio, scan_subset_trio, stream_create,
+                     stream_synchronize)
 
 EYES = ("left", "right")
 
@@ -67,22 +75,27 @@ class TwoStageMatcher:
         words = -(-max_db_size * max_batch // 64) + 2
         self.bitmaps = {e: [torch.zeros(words, dtype=torch.int64, device="cuda") for _ in range(3)]
                         for e in EYES}
+        bwords = -(-max_batch * max_batch // 64) + 2  # batch rows max_batch bits apart
+        self.batch_bitmaps = {e: [torch.zeros(bwords, dtype=torch.int64, device="cuda") for _ in range(3)]
+                              for e in EYES}
 
     def _sync(self):
         for s in self.streams:
             stream_synchronize(s)
 
-    def _clear(self, eye):
+    def _clear(self, eye, batch=False):
         import torch
 
-        for b in self.bitmaps[eye]:
+        for b in (self.batch_bitmaps if batch else self.bitmaps)[eye]:
             b.zero_()
         torch.cuda.synchronize()
 
-    def match(self, queries, db_size, batch_size, or_rule_indices=None, reauth_targets=()):
+    def match(self, queries, db_size, batch_size, or_rule_indices=None, reauth_targets=(),
+              self_compare=False):
         """queries[eye] = (code query sets, mask query sets), three each.
         or_rule_indices: per batch query a list of DB ids (or None);
-        reauth_targets: DB ids. Returns a TwoStageResult."""
+        reauth_targets: DB ids; self_compare: also deduplicate within the batch
+        (query sets uploaded from raw shares). Returns a TwoStageResult."""
         import torch
 
         assert batch_size <= self.max_batch and db_size <= self.max_db_size
@@ -91,6 +104,8 @@ class TwoStageMatcher:
         res = TwoStageResult()
         for e in EYES:
             self._clear(e)
+            if self_compare:
+                self._clear(e, batch=True)
 
         # phase 1: full scan of one eye at the prefilter threshold
         qc, qm = queries[side1]
@@ -115,6 +130,10 @@ class TwoStageMatcher:
         n = int(res.candidates.size)
         for eye in (side1, side2):
             qc, qm = queries[eye]
+            if self_compare:  # actor.rs:1929-1935: batch dedup first
+                scan_batch_self_trio(self.eyes[eye], qc, qm, batch_size, THRESHOLD_A,
+                                     [b.data_ptr() for b in self.batch_bitmaps[eye]], self.streams,
+                                     row_bits=self.max_batch)
             scan_subset_trio(self.eyes[eye], self.code_subs, self.mask_subs, qc, qm, ids, n, db_size,
                              batch_size, THRESHOLD_A, [b.data_ptr() for b in self.bitmaps[eye]],
                              self.streams)
@@ -138,6 +157,10 @@ class TwoStageMatcher:
             cmp.join_db_matches(self.bitmaps["left"][p].data_ptr(), self.bitmaps["right"][p].data_ptr(),
                                 final.data_ptr(), db_size, or_dev.data_ptr() if or_dev is not None else None,
                                 self.max_db_size, s)
+            if self_compare:
+                cmp.join_batch_matches(self.batch_bitmaps["left"][p].data_ptr(),
+                                       self.batch_bitmaps["right"][p].data_ptr(), final.data_ptr(),
+                                       batch_size, self.max_batch, s)
             stream_synchronize(s)
             mc, am, mcl, mcr, prl, prr = cmp.buffers()
             B = self.max_batch

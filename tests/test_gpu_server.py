@@ -1,0 +1,137 @@
+"""Batch processing end to end (ServerActor::process_batch_query,
+iris-mpc-gpu/src/server/actor.rs:674-1790): two batches against a 1,000-row
+DB of both eyes, with the outcome of every request known by construction
+(flipped copies at 10% distance match, distinct synthetic irises sit near
+0.5). Modelled on the request kinds of the reference's e2e generator
+(iris-mpc-common/src/test.rs): unique insertions, DB matches, batch
+duplicates, skip_persistence, successful and failed reauths, a left-only
+match under the AND rule and under the OR rule, a deletion, and in the second
+batch queries of irises written by the first (insertions, a reauth) and of the
+deleted row.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N0, MAXB, MAXDB = 1000, 16, 4096
+NM = 0xFFFFFFFF
+
+
+@pytest.fixture(scope="module")
+def irismpc():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import irismpc as m
+
+    m.lib()
+    return m
+
+
+def _iris(ds, i):
+    from irismpc import synth
+
+    return synth.synth_iris(ds, i)
+
+
+def _flip(code, key, n=1280):
+    from irismpc import synth
+
+    return synth.synth_flip(code, 900, key, n)
+
+
+class Batch:
+    def __init__(self):
+        self.ids, self.types, self.irises = [], [], {"left": [], "right": []}
+        self.skip, self.reauth, self.or_rule = [], {}, []
+
+    def add(self, rid, left, right, kind="uniqueness", skip=False, reauth=None, or_rule=()):
+        self.ids.append(rid)
+        self.types.append(kind)
+        self.irises["left"].append(left)
+        self.irises["right"].append(right)
+        self.skip.append(skip)
+        self.or_rule.append(list(or_rule))
+        if reauth is not None:
+            self.reauth[rid] = reauth
+
+    def query(self, deletions=()):
+        from irismpc import synth
+        from irismpc.server import BatchQuery
+
+        sh = {}
+        for k, eye in enumerate(("left", "right")):
+            codes = np.array([c for c, _ in self.irises[eye]])
+            masks = np.array([m for _, m in self.irises[eye]])
+            sh[eye] = synth.shares_of(codes, masks, 500 + k + 2 * len(self.ids))
+        return BatchQuery(self.ids, self.types, sh["left"], sh["right"], self.or_rule, self.skip,
+                          self.reauth, list(deletions))
+
+
+def test_server_two_batches(irismpc):
+    from irismpc import synth
+    from irismpc.server import ServerActor
+
+    db = {"left": [_iris(11, i) for i in range(N0)], "right": [_iris(12, i) for i in range(N0)]}
+    actor = ServerActor(MAXDB, MAXB)
+    sh = {}
+    for k, eye in enumerate(("left", "right")):
+        codes = np.array([c for c, _ in db[eye]])
+        masks = np.array([m for _, m in db[eye]])
+        sh[eye] = synth.shares_of(codes, masks, 300 + k)
+    actor.load_db(sh["left"], sh["right"])
+    assert actor.current_db_size == N0
+
+    def copy(eye, i, key):
+        c, m = db[eye][i]
+        return _flip(c, key), m
+
+    new = {n: (_iris(21, n), _iris(22, n)) for n in range(6)}  # fresh irises (left, right)
+    fresh = lambda n, key: ((_flip(new[n][0][0], key), new[n][0][1]), (_flip(new[n][1][0], key), new[n][1][1]))  # noqa: E731
+
+    b1 = Batch()
+    b1.add("u0", new[0][0], new[0][1])                                     # 0 unique -> row 1000
+    b1.add("m17", copy("left", 17, 1), copy("right", 17, 2))               # 1 DB match 17
+    b1.add("dup0", *fresh(0, 3))                                            # 2 duplicate of 0
+    b1.add("skip1", new[1][0], new[1][1], skip=True)                       # 3 unique, not stored
+    b1.add("re33", copy("left", 33, 4), copy("right", 33, 5), kind="reauth", reauth=33)  # 4 ok
+    b1.add("re40", new[2][0], new[2][1], kind="reauth", reauth=40)         # 5 fails
+    b1.add("l50", copy("left", 50, 6), _iris(23, 0))                       # 6 left only -> unique, row 1001
+    b1.add("or60", copy("left", 60, 7), _iris(23, 1), or_rule=[60])        # 7 OR rule -> 60
+    b1.add("u3", new[3][0], new[3][1])                                     # 8 unique -> row 1002
+    b1.add("dup3", *fresh(3, 8))                                            # 9 duplicate of 8
+    b1.add("d70", copy("left", 70, 9), copy("right", 70, 10))              # 10 deleted row -> unique, 1003
+    b1.add("u4", new[4][0], new[4][1])                                     # 11 unique -> row 1004
+    r1 = actor.process_batch_query(b1.query(deletions=[70]))
+
+    FB = 0xFFFFFFFE
+    assert r1.merged_results == [1000, 17, FB, NM, 33, NM, 1001, 60, 1002, FB, 1003, 1004]
+    assert r1.matches == [False, True, True, True, True, True, False, True, False, True, False, False]
+    assert r1.matches_with_skip_persistence[3] is False and r1.matches_with_skip_persistence[1] is True
+    assert r1.matched_batch_request_ids[2] == ["u0"] and r1.matched_batch_request_ids[9] == ["u3"]
+    assert all(not r1.matched_batch_request_ids[i] for i in (0, 1, 3, 4, 5, 6, 7, 8, 10, 11))
+    assert r1.successful_reauths == [i == 4 for i in range(12)]
+    assert r1.match_ids[1] == [17] and r1.match_ids[4] == [33] and r1.match_ids[7] == [60]
+    assert r1.match_ids[2] == [NM - 0] and r1.match_ids[9] == [NM - 8]
+    assert r1.partial_match_ids_left[6] == [50] and r1.partial_match_ids_right[6] == []
+    assert r1.partial_match_ids_left[7] == [60] and r1.partial_match_counters_left[7] == 1
+    assert r1.partial_match_ids_left[10] == [] and r1.partial_match_ids_right[10] == []  # deleted
+    assert r1.partial_match_rotation_indices_left[1] == [[0]]
+    assert r1.deleted_ids == [70]
+    assert actor.current_db_size == N0 + 5
+
+    b2 = Batch()
+    b2.add("q_u0", *fresh(0, 20))                       # 0 -> row 1000 (inserted in batch 1)
+    b2.add("q_skip1", *fresh(1, 21))                    # 1 -> not stored before: unique, row 1005
+    b2.add("q_re33", copy("left", 33, 22), copy("right", 33, 23))  # 2 -> 33
+    b2.add("q_d70", copy("left", 70, 24), copy("right", 70, 25))   # 3 -> 1003, no longer 70
+    b2.add("q_u3", *fresh(3, 26))                       # 4 -> 1002
+    b2.add("q_l50", copy("left", 50, 27), _iris(23, 0))  # 5 -> 1001 (its left and right)
+    b2.add("q_u4", *fresh(4, 28), kind="reauth", reauth=1004)      # 6 reauth of an inserted row
+    r2 = actor.process_batch_query(b2.query())
+    assert r2.merged_results == [1000, 1005, 33, 1003, 1002, 1001, 1004]
+    assert r2.successful_reauths == [False] * 6 + [True]
+    assert r2.partial_match_ids_left[3] == [1003] and r2.partial_match_ids_right[3] == [1003]
+    assert actor.current_db_size == N0 + 6

@@ -183,3 +183,20 @@ def test_write_at_index(irismpc, orc):
     slices.write_at_index(dq, 3, 17)
     # write_at_index copies the centre rotation (row 3*31 + 15) of the query batch
     assert np.array_equal(slices.read_records(17, 1)[0], q[3 * 31 + 15])
+
+
+def test_write_at_index_insert_form(irismpc, orc):
+    """Query sets uploaded from raw shares carry the DB form of each query
+    (the reference's code_query_insert centre rotation): write_at_index stores
+    the raw share itself, not the Lagrange-interpolated query row."""
+    rng = np.random.default_rng(6)
+    K, B = 12800, 5
+    raw = _rand(rng, (B, K))
+    eng = irismpc.ShareDB(1, 256, B * 31, K, ([0] * 8, [0] * 8))
+    slices = eng.alloc_db(64)
+    slices.load_full_db(_rand(rng, (64, K)))
+    dq = eng.upload_query_shares(raw)
+    slices.write_at_index(dq, 3, 40)
+    assert np.array_equal(slices.read_records(40, 1)[0], raw[3])
+    with pytest.raises(irismpc.IrisMpcError, match="beyond query batch"):
+        slices.write_at_index(dq, 5, 41)
