@@ -92,6 +92,22 @@ int irismpc_db_load_records(irismpc_share_db* e, irismpc_db* db, size_t index, s
  * ("odd") bytes, limb1 = high ("even") bytes, already biased by -128. */
 int irismpc_db_load_records_limbs(irismpc_share_db* e, irismpc_db* db, size_t index, size_t n,
                                   const uint8_t* limb0, const uint8_t* limb1, void* stream);
+/* InMemoryStore::load_single_record_from_s3 (inmemory_store.rs:55-105) for
+ * a run of stored DB-chunk records in host memory, loaded straight into HBM.
+ * Record layout (iris-mpc-store/src/s3_importer.rs:13-87, 76,806 bytes):
+ * big-endian u32 serial id; left code, left mask, right code, right mask,
+ * each as its low limb bytes ("odd") then its high limb bytes ("even"),
+ * biased by -128; big-endian u16 version id. Serial id s goes to row s - 1
+ * of the four slices (loader.rs:196-233); ids above max_serial_id (0 = no
+ * cap) are skipped; id 0, an id repeated within the call or a row beyond a
+ * slice fails before anything is written. *n_loaded = records loaded;
+ * *row_end = largest loaded row + 1 (0 if none). */
+int irismpc_db_load_stored_records(irismpc_share_db* codes, irismpc_share_db* masks,
+                                   irismpc_db* left_code, irismpc_db* left_mask,
+                                   irismpc_db* right_code, irismpc_db* right_mask,
+                                   const uint8_t* records, size_t n_records,
+                                   uint32_t max_serial_id, size_t* n_loaded, size_t* row_end,
+                                   void* stream);
 /* ProcessedDatabase::prefetch_subset (share_db.rs:344-407): DB rows ids[0..n)
  * (device u32) of db into rows 0..n of subset (a slice of at least n rows,
  * e.g. irismpc_share_db_alloc_db(e, 32768, ...)). Ids beyond the DB's

@@ -181,6 +181,23 @@ int irismpc_db_write_at_index(irismpc_share_db* e, irismpc_db* db, const irismpc
   return guarded(
       [&] { e->impl->write_at_index(*db->impl, *q->impl, src_index, dst_index, st(stream)); });
 }
+int irismpc_db_load_stored_records(irismpc_share_db* codes, irismpc_share_db* masks,
+                                   irismpc_db* left_code, irismpc_db* left_mask,
+                                   irismpc_db* right_code, irismpc_db* right_mask,
+                                   const uint8_t* records, size_t n_records,
+                                   uint32_t max_serial_id, size_t* n_loaded, size_t* row_end,
+                                   void* stream) {
+  return guarded([&] {
+    IRIS_REQUIRE(!n_records || records, "records is NULL");
+    DevSet g(codes->device);
+    size_t end = 0;
+    const size_t n = load_stored_records(*codes->impl, *masks->impl, *left_code->impl,
+                                         *left_mask->impl, *right_code->impl, *right_mask->impl,
+                                         records, n_records, max_serial_id, &end, st(stream));
+    if (n_loaded) *n_loaded = n;
+    if (row_end) *row_end = end;
+  });
+}
 int irismpc_db_prefetch_subset(irismpc_share_db* e, const irismpc_db* db, const uint32_t* ids,
                                size_t n, irismpc_db* subset, void* stream) {
   return guarded([&] { e->impl->prefetch_subset(*db->impl, ids, n, *subset->impl, st(stream)); });

@@ -4,6 +4,7 @@
 #include "query_prep.hpp"
 
 #include <algorithm>
+#include <vector>
 #include <chrono>
 #include <cstring>
 
@@ -141,6 +142,61 @@ void ShareDB::load_records_u16(DbSlice& db, size_t index, size_t n, const uint16
                      db.sum1.ptr, s);
     IRIS_HIP_CHECK(hipStreamSynchronize(s));  // staging reuse
   }
+}
+
+void ShareDB::load_records_stored(DbSlice& db, const uint8_t* dev_records, size_t n,
+                                  size_t stride, size_t off_lo, size_t off_hi,
+                                  uint32_t max_serial, hipStream_t s) const {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(db.code_length == code_length_, "DB slice belongs to another code length");
+  IRIS_REQUIRE(off_lo + (size_t)code_length_ <= stride && off_hi + (size_t)code_length_ <= stride,
+               "field beyond the record");
+  launch_tile_records(dev_records, n, stride, off_lo, off_hi, code_length_, max_serial,
+                      db.limbs.ptr, db.sum0.ptr, db.sum1.ptr, s);
+}
+
+size_t load_stored_records(const ShareDB& codes, const ShareDB& masks, DbSlice& left_code,
+                           DbSlice& left_mask, DbSlice& right_code, DbSlice& right_mask,
+                           const uint8_t* records, size_t n, uint32_t max_serial,
+                           size_t* row_end, hipStream_t s) {
+  IRIS_REQUIRE(codes.code_length() == kCodeLen && masks.code_length() == kMaskLen,
+               "stored records hold 12,800-coefficient codes and 6,400-coefficient masks");
+  if (max_serial == 0) max_serial = 0xFFFFFFFFu;
+  size_t cap = std::min({left_code.capacity, left_mask.capacity, right_code.capacity,
+                         right_mask.capacity}) - kMaxBD;
+  std::vector<uint32_t> ids;
+  size_t loaded = 0, end = 0;
+  for (size_t r = 0; r < n; r++) {
+    const uint8_t* b = records + r * kStoredRecordBytes;
+    const uint32_t id = ((uint32_t)b[0] << 24) | ((uint32_t)b[1] << 16) | ((uint32_t)b[2] << 8) | b[3];
+    IRIS_REQUIRE(id != 0, "stored record with serial id 0");
+    if (id > max_serial) continue;
+    IRIS_REQUIRE((size_t)id - 1 < cap, "serial id beyond the DB slices");
+    ids.push_back(id);
+    loaded++;
+    end = std::max(end, (size_t)id);
+  }
+  std::sort(ids.begin(), ids.end());
+  IRIS_REQUIRE(std::adjacent_find(ids.begin(), ids.end()) == ids.end(),
+               "serial id repeated within the records");
+  const size_t window = std::min<size_t>(n, 512);
+  DevBuf<uint8_t> stage(window * kStoredRecordBytes, false);
+  for (size_t r0 = 0; r0 < n; r0 += window) {
+    const size_t m = std::min(window, n - r0);
+    IRIS_HIP_CHECK(hipMemcpyAsync(stage.ptr, records + r0 * kStoredRecordBytes,
+                                  m * kStoredRecordBytes, hipMemcpyHostToDevice, s));
+    codes.load_records_stored(left_code, stage.ptr, m, kStoredRecordBytes, kStoredLeftCode.lo,
+                              kStoredLeftCode.hi, max_serial, s);
+    masks.load_records_stored(left_mask, stage.ptr, m, kStoredRecordBytes, kStoredLeftMask.lo,
+                              kStoredLeftMask.hi, max_serial, s);
+    codes.load_records_stored(right_code, stage.ptr, m, kStoredRecordBytes, kStoredRightCode.lo,
+                              kStoredRightCode.hi, max_serial, s);
+    masks.load_records_stored(right_mask, stage.ptr, m, kStoredRecordBytes, kStoredRightMask.lo,
+                              kStoredRightMask.hi, max_serial, s);
+    IRIS_HIP_CHECK(hipStreamSynchronize(s));  // staging reuse
+  }
+  if (row_end) *row_end = end;
+  return loaded;
 }
 
 void ShareDB::load_records_device(DbSlice& db, size_t index, size_t n, const uint16_t* dev_rows,

@@ -125,6 +125,10 @@ class ShareDB {
                           hipStream_t s) const;
   void write_at_index(DbSlice& db, const QuerySet& q, size_t src_index, size_t dst_index,
                       hipStream_t s) const;
+  // stored records already on the device (layout above): the field at
+  // (off_lo, off_hi) of each record with serial id in [1, max_serial] -> row id - 1
+  void load_records_stored(DbSlice& db, const uint8_t* dev_records, size_t n, size_t stride,
+                           size_t off_lo, size_t off_hi, uint32_t max_serial, hipStream_t s) const;
   // n u16 rows already on the device -> HBM limbs at `index` (stream-ordered)
   void load_records_device(DbSlice& db, size_t index, size_t n, const uint16_t* dev_rows,
                            hipStream_t s) const;
@@ -194,6 +198,32 @@ class ShareDB {
   Event ev_send_done_;
   friend struct ShareDBTrio;
 };
+
+// Stored-record layout of the DB chunks (iris-mpc-store/src/s3_importer.rs:
+// 13-87, written by iris-mpc-db-exporter/src/converter/binary.go:59-152):
+// BE u32 serial id; left code, left mask, right code, right mask, each as its
+// low limb bytes ("odd") then its high limb bytes ("even"), biased by -128;
+// BE u16 version id.
+constexpr size_t kStoredRecordBytes = 4 + 2 * (2 * kCodeLen) + 2 * (2 * kMaskLen) + 2;
+struct StoredField {
+  size_t lo, hi;
+};
+constexpr StoredField kStoredLeftCode{4, 4 + kCodeLen};
+constexpr StoredField kStoredLeftMask{4 + 2 * kCodeLen, 4 + 2 * kCodeLen + kMaskLen};
+constexpr StoredField kStoredRightCode{4 + 2 * kCodeLen + 2 * kMaskLen, 4 + 3 * kCodeLen + 2 * kMaskLen};
+constexpr StoredField kStoredRightMask{4 + 4 * kCodeLen + 2 * kMaskLen,
+                                       4 + 4 * kCodeLen + 3 * kMaskLen};
+
+// InMemoryStore::load_single_record_from_s3 (inmemory_store.rs:55-105) over n
+// stored records in host memory, loader.rs:196-233 placement: serial id s ->
+// row s - 1 of all four slices; ids above max_serial (0 = no cap) skipped; id
+// 0, an id repeated within the call or a row beyond a slice is an error
+// (checked before anything is written). Returns the records loaded; *row_end
+// = largest loaded row + 1 (0 if none).
+size_t load_stored_records(const ShareDB& codes, const ShareDB& masks, DbSlice& left_code,
+                           DbSlice& left_mask, DbSlice& right_code, DbSlice& right_mask,
+                           const uint8_t* records, size_t n, uint32_t max_serial,
+                           size_t* row_end, hipStream_t s);
 
 class Circuits {
  public:

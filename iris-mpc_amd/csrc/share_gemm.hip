@@ -69,6 +69,42 @@ __global__ void tile_limbs_kernel(const uint8_t* __restrict__ l0, const uint8_t*
       *reinterpret_cast<const uint4*>(l1 + so);
 }
 
+// Serial id of a stored record: big-endian u32 at its start. Row = id - 1;
+// records with id 0 or above max_serial are not loaded.
+__device__ inline bool record_row(const uint8_t* r, uint32_t max_serial, size_t* row) {
+  const uint32_t id = ((uint32_t)r[0] << 24) | ((uint32_t)r[1] << 16) | ((uint32_t)r[2] << 8) | r[3];
+  if (id == 0 || id > max_serial) return false;
+  *row = id - 1;
+  return true;
+}
+
+// Stored-record ingest: thread (record, 16-coefficient group) copies the
+// group's low-limb bytes (at off_lo in the record) and high-limb bytes (at
+// off_hi) into the fragments of row id - 1. Stored bytes are the biased limbs
+// already (value - 128 mod 256). Records are only 2-byte aligned: byte loads.
+__global__ void tile_records_kernel(const uint8_t* __restrict__ rec, size_t n, size_t stride,
+                                    size_t off_lo, size_t off_hi, int K, uint32_t max_serial,
+                                    int8_t* __restrict__ dst) {
+  const int groups = K / 16;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * (size_t)groups) return;
+  const size_t r = t / groups;
+  const int m = (int)(t % groups);
+  const uint8_t* base = rec + r * stride;
+  size_t R;
+  if (!record_row(base, max_serial, &R)) return;
+  const int n_kt = K / kKTile;
+  const int kt = m / 4, lane = (int)(R % 16) + 16 * (m % 4);
+  uint8_t lo[16], hi[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    lo[i] = base[off_lo + (size_t)m * 16 + i];
+    hi[i] = base[off_hi + (size_t)m * 16 + i];
+  }
+  memcpy(dst + frag_offset(R / 16, kt, 0, n_kt) + lane * 16, lo, 16);
+  memcpy(dst + frag_offset(R / 16, kt, 1, n_kt) + lane * 16, hi, 16);
+}
+
 // One thread per row: sums of the signed limbs (u32 wrapping).
 __global__ void row_sums_kernel(const int8_t* __restrict__ tiled, size_t row0, size_t n_rows,
                                 int K, uint32_t* __restrict__ s0, uint32_t* __restrict__ s1) {
@@ -86,6 +122,36 @@ __global__ void row_sums_kernel(const int8_t* __restrict__ tiled, size_t row0, s
       int8_t b0[16], b1[16];
       *reinterpret_cast<uint4*>(b0) = *reinterpret_cast<const uint4*>(p0);
       *reinterpret_cast<uint4*>(b1) = *reinterpret_cast<const uint4*>(p1);
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        acc0 += b0[i];
+        acc1 += b1[i];
+      }
+    }
+  }
+  s0[R] = (uint32_t)acc0;
+  s1[R] = (uint32_t)acc1;
+}
+
+// Row sums of the rows written by tile_records_kernel (one thread per record).
+__global__ void record_sums_kernel(const uint8_t* __restrict__ rec, size_t n, size_t stride,
+                                   uint32_t max_serial, const int8_t* __restrict__ tiled, int K,
+                                   uint32_t* __restrict__ s0, uint32_t* __restrict__ s1) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  size_t R;
+  if (!record_row(rec + r * stride, max_serial, &R)) return;
+  const int n_kt = K / kKTile;
+  int acc0 = 0, acc1 = 0;
+  for (int kt = 0; kt < n_kt; kt++) {
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const int lane = (int)(R % 16) + 16 * g;
+      int8_t b0[16], b1[16];
+      *reinterpret_cast<uint4*>(b0) =
+          *reinterpret_cast<const uint4*>(tiled + frag_offset(R / 16, kt, 0, n_kt) + lane * 16);
+      *reinterpret_cast<uint4*>(b1) =
+          *reinterpret_cast<const uint4*>(tiled + frag_offset(R / 16, kt, 1, n_kt) + lane * 16);
 #pragma unroll
       for (int i = 0; i < 16; i++) {
         acc0 += b0[i];
@@ -212,6 +278,19 @@ void launch_tile_limbs(const uint8_t* l0, const uint8_t* l1, size_t n_rows, int 
   tile_limbs_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(l0, l1, n_rows, K, dst, row0);
   IRIS_HIP_CHECK(hipGetLastError());
   row_sums_kernel<<<(unsigned)ceil_div(n_rows, 256), 256, 0, s>>>(dst, row0, n_rows, K, sum0, sum1);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_tile_records(const uint8_t* records, size_t n, size_t stride, size_t off_lo,
+                         size_t off_hi, int K, uint32_t max_serial, int8_t* dst, uint32_t* sum0,
+                         uint32_t* sum1, hipStream_t s) {
+  if (n == 0) return;
+  const size_t t = n * (size_t)(K / 16);
+  tile_records_kernel<<<(unsigned)ceil_div(t, 256), 256, 0, s>>>(records, n, stride, off_lo, off_hi,
+                                                                 K, max_serial, dst);
+  IRIS_HIP_CHECK(hipGetLastError());
+  record_sums_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(records, n, stride, max_serial, dst,
+                                                                K, sum0, sum1);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 

@@ -61,6 +61,13 @@ void launch_tile_rows(const uint16_t* src, size_t n_rows, int K, int8_t* dst, si
 // Same from split limbs (S3 even/odd format, share_db.rs:657-684).
 void launch_tile_limbs(const uint8_t* l0, const uint8_t* l1, size_t n_rows, int K, int8_t* dst,
                        size_t row0, uint32_t* sum0, uint32_t* sum1, hipStream_t s);
+// Stored records (iris-mpc-store/src/s3_importer.rs:13-87): n records of
+// `stride` bytes on the device, each opening with its big-endian u32 serial
+// id; the matrix's low-limb bytes start at off_lo and its high-limb bytes at
+// off_hi within the record. Record with id in [1, max_serial] -> row id - 1.
+void launch_tile_records(const uint8_t* records, size_t n, size_t stride, size_t off_lo,
+                         size_t off_hi, int K, uint32_t max_serial, int8_t* dst, uint32_t* sum0,
+                         uint32_t* sum1, hipStream_t s);
 // Copies tiled row `src_row` of `src` to row `dst_row` of `dst` with its sums
 // (write_at_index, share_db.rs:187-240).
 void launch_copy_row(const int8_t* src, const uint32_t* src_s0, const uint32_t* src_s1,

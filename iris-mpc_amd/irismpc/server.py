@@ -199,6 +199,21 @@ class ServerActor:
                 self.eyes[eye][p].mask_db.load_records(index, sh[p][1])
         self.current_db_size = max(self.current_db_size, index + n)
 
+    def load_stored_records(self, chunks, max_serial_id=0):
+        """Party p's DB-chunk bytes chunks[p] (the store module's record
+        format) straight into HBM; the DB size grows to the largest loaded
+        serial id (loader.rs:196-240)."""
+        from .store import load_stored_records
+
+        ends = []
+        for p in range(3):
+            L, R = self.eyes["left"][p], self.eyes["right"][p]
+            n, end = load_stored_records(L.codes, L.masks, L.code_db, L.mask_db, R.code_db, R.mask_db,
+                                         chunks[p], max_serial_id)
+            ends.append(end)
+        self.current_db_size = max([self.current_db_size] + ends)
+        return ends
+
     def _write_db_at_index(self, queries, src_index, dst_index):
         for eye in EYES:
             qc, qm = queries[eye]
