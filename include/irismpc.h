@@ -152,6 +152,15 @@ int irismpc_query_upload(irismpc_share_db* e, const uint16_t* rows, size_t n_row
  * or a batch without padding. */
 int irismpc_query_upload_shares_rows(irismpc_share_db* e, const uint16_t* shares, size_t batch,
                                      size_t rows, void* stream, irismpc_query** out);
+/* Queries of the Mirror orientation (full-face mirror attack pass,
+ * actor.rs:591-615; server/mod.rs:156-176): the query rows come from the
+ * mirrored shares of `shares` (the other eye's raw shares; mirrored_code for
+ * 12,800-coefficient engines, mirrored_mask otherwise, as the server derives
+ * them at iris-mpc/src/services/processors/mod.rs:39-40), their DB form
+ * (batch self-comparison) from insert_shares (this eye's raw shares). */
+int irismpc_query_upload_shares_mirrored(irismpc_share_db* e, const uint16_t* shares,
+                                         const uint16_t* insert_shares, size_t batch,
+                                         void* stream, irismpc_query** out);
 int irismpc_query_upload_shares(irismpc_share_db* e, const uint16_t* shares, size_t batch,
                                 void* stream, irismpc_query** out);
 int irismpc_query_destroy(irismpc_query* q);

@@ -307,6 +307,15 @@ int irismpc_query_upload_shares_rows(irismpc_share_db* e, const uint16_t* shares
     *out = h;
   });
 }
+int irismpc_query_upload_shares_mirrored(irismpc_share_db* e, const uint16_t* shares,
+                                         const uint16_t* insert_shares, size_t batch,
+                                         void* stream, irismpc_query** out) {
+  return guarded([&] {
+    auto h = std::make_unique<irismpc_query>();
+    h->impl = e->impl->upload_query_shares_mirrored(shares, insert_shares, batch, 0, st(stream));
+    *out = h.release();
+  });
+}
 int irismpc_query_destroy(irismpc_query* q) {
   return guarded([&] { delete q; });
 }

@@ -299,13 +299,29 @@ std::unique_ptr<QuerySet> ShareDB::upload_query_shares(const uint16_t* host_shar
 
 std::unique_ptr<QuerySet> ShareDB::upload_query_shares_rows(const uint16_t* host_shares, size_t batch,
                                                             size_t n_rows, hipStream_t s) const {
+  return upload_shares_impl(host_shares, nullptr, batch, n_rows, 0, s);
+}
+
+std::unique_ptr<QuerySet> ShareDB::upload_query_shares_mirrored(const uint16_t* host_shares,
+                                                                const uint16_t* insert_shares,
+                                                                size_t batch, size_t n_rows,
+                                                                hipStream_t s) const {
+  IRIS_REQUIRE(insert_shares, "mirrored queries need this eye's raw shares for the DB form");
+  return upload_shares_impl(host_shares, insert_shares, batch, n_rows,
+                            code_length_ == kCodeLen ? 2 : 1, s);
+}
+
+std::unique_ptr<QuerySet> ShareDB::upload_shares_impl(const uint16_t* host_shares,
+                                                      const uint16_t* insert_shares, size_t batch,
+                                                      size_t n_rows, int mirror,
+                                                      hipStream_t s) const {
   DeviceGuard g(device_);
   if (n_rows == 0) n_rows = query_length_;
   IRIS_REQUIRE(batch >= 1 && batch * kRotations <= n_rows, "batch exceeds the query rows / 31");
   const size_t K = (size_t)code_length_;
   DevBuf<uint16_t> raw(batch * K, false), rows(n_rows * K, false);
   IRIS_HIP_CHECK(hipMemcpyAsync(raw.ptr, host_shares, raw.n * 2, hipMemcpyHostToDevice, s));
-  launch_query_prep(raw.ptr, (int)batch, code_length_, peer_id_, rows.ptr, (int)n_rows, s);
+  launch_query_prep(raw.ptr, (int)batch, code_length_, peer_id_, rows.ptr, (int)n_rows, s, mirror);
   auto q = std::make_unique<QuerySet>();
   q->code_length = code_length_;
   q->rows = (int)n_rows;
@@ -317,6 +333,10 @@ std::unique_ptr<QuerySet> ShareDB::upload_query_shares_rows(const uint16_t* host
   if (code_length_ % 800 == 0) {
     q->rot.alloc(rot_planes_bytes((int)n_rows, code_length_));
     launch_rot_planes(rows.ptr, (int)n_rows, code_length_, q->rot.ptr, s);
+  }
+  if (insert_shares) {  // the DB form comes from other shares than the query rows
+    IRIS_HIP_CHECK(hipStreamSynchronize(s));  // raw is reused
+    IRIS_HIP_CHECK(hipMemcpyAsync(raw.ptr, insert_shares, raw.n * 2, hipMemcpyHostToDevice, s));
   }
   q->batch = (int)batch;
   q->insert = alloc_db(round_up(batch, 512));  // room for a padded self-comparison chunk

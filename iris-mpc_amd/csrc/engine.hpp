@@ -151,6 +151,14 @@ class ShareDB {
   // e.g. several batches' queries packed back to back for one launch.
   std::unique_ptr<QuerySet> upload_query_shares_rows(const uint16_t* host_shares, size_t batch,
                                                      size_t rows, hipStream_t s) const;
+  // Queries of the Mirror orientation (server/mod.rs:156-176): query rows from
+  // the mirrored host_shares (mirrored_code for 12,800-coefficient engines,
+  // mirrored_mask otherwise; galois_engine.rs:218-224,518-545), the DB form
+  // from insert_shares (batch x code_length, this eye's raw shares).
+  std::unique_ptr<QuerySet> upload_query_shares_mirrored(const uint16_t* host_shares,
+                                                         const uint16_t* insert_shares,
+                                                         size_t batch, size_t rows,
+                                                         hipStream_t s) const;
   // dot + dot_reduce_and_multiply (fused): DB rows [offset, offset + chunk)
   // against all query rows; chunk is the padded chunk size (multiple of 64).
   void dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
@@ -188,6 +196,9 @@ class ShareDB {
   int resolve_variant(const QuerySet& q) const;
 
  private:
+  std::unique_ptr<QuerySet> upload_shares_impl(const uint16_t* host_shares,
+                                               const uint16_t* insert_shares, size_t batch,
+                                               size_t rows, int mirror, hipStream_t s) const;
   int device_, peer_id_, code_length_, tag_;
   size_t query_length_, max_db_length_;
   ChaChaKey seed_own_, seed_prev_;

@@ -194,6 +194,17 @@ class ShareDB:
         call("irismpc_query_upload_shares", self.h, r.ctypes.data, r.shape[0], stream, ctypes.byref(h))
         return DeviceQuery(h.value)
 
+    def upload_query_shares_mirrored(self, shares, insert_shares, stream=0):
+        """Mirror-orientation queries: rows from the mirrored `shares` (the other
+        eye's raw shares), DB form from insert_shares (this eye's)."""
+        r = np.ascontiguousarray(shares, dtype=np.uint16).reshape(-1, self.code_length)
+        i = np.ascontiguousarray(insert_shares, dtype=np.uint16).reshape(-1, self.code_length)
+        assert i.shape == r.shape
+        h = _out()
+        call("irismpc_query_upload_shares_mirrored", self.h, r.ctypes.data, i.ctypes.data, r.shape[0],
+             stream, ctypes.byref(h))
+        return DeviceQuery(h.value)
+
     def upload_query_shares_rows(self, shares, rows, stream=0):
         """Raw query shares as exactly `rows` query rows (rows >= batch * 31; may
         exceed query_length): a packed query set of the batching form."""

@@ -16,6 +16,12 @@ Per batch:
 - DB writes: insertions at the end of the DB, then successful reauths at their
   targets, both eyes (write_db_at_index, actor.rs:1522-1600,3547-3599).
 
+process_job runs the full-face mirror-attack pass first when the batch asks
+for it (actor.rs:585-640): the Mirror orientation queries each eye's DB with
+the other eye's shares mirrored, writes nothing, and its matches both flag
+attacks (no Normal match but a Mirror match) and veto insertions in the
+Normal pass that follows.
+
 Lists that the reference fills in atomic-arrival order on the device (match
 ids, partial ids, rotations) are returned sorted.
 """
@@ -31,6 +37,7 @@ NON_MATCH_ID = 0xFFFFFFFF                 # actor.rs:192
 SUPERMATCH_THRESHOLD = 4000               # actor.rs:117
 UNIQUENESS_MESSAGE_TYPE = "uniqueness"    # iris-mpc-common/src/helpers/smpc_request.rs:109
 REAUTH_MESSAGE_TYPE = "reauth"            # smpc_request.rs:110
+NORMAL, MIRROR = "normal", "mirror"       # Orientation (actor.rs:119-134)
 ALL_MATCHES_LEN = 256
 
 
@@ -97,6 +104,7 @@ class BatchQuery:
     reauth_target_indices: dict = field(default_factory=dict)
     deletion_requests_indices: list = field(default_factory=list)
     valid_entries: list = None
+    full_face_mirror_attacks_detection_enabled: bool = False
 
     def retain(self, keep):
         """BatchQuery::retain: only the entries at positions `keep`."""
@@ -108,7 +116,7 @@ class BatchQuery:
             [self.or_rule_indices[i] for i in k] if self.or_rule_indices else [],
             [self.skip_persistence[i] for i in k] if self.skip_persistence is not None else None,
             {r: t for r, t in self.reauth_target_indices.items() if r in ids},
-            list(self.deletion_requests_indices), None)
+            list(self.deletion_requests_indices), None, self.full_face_mirror_attacks_detection_enabled)
 
 
 @dataclass
@@ -133,6 +141,12 @@ class ServerJobResult:
     successful_reauths: list
     reauth_target_indices: dict
     candidates: np.ndarray = None
+    full_face_mirror_attack_detected: list = field(default_factory=list)
+    full_face_mirror_match_ids: list = field(default_factory=list)
+    full_face_mirror_partial_match_ids_left: list = field(default_factory=list)
+    full_face_mirror_partial_match_ids_right: list = field(default_factory=list)
+    full_face_mirror_partial_match_counters_left: list = field(default_factory=list)
+    full_face_mirror_partial_match_counters_right: list = field(default_factory=list)
 
 
 def _dummy_shares():
@@ -157,8 +171,9 @@ class ServerActor:
     for codes, masks, circuits)."""
 
     def __init__(self, max_db_size, max_batch_size, full_scan_side="left", seeds=None,
-                 return_partial_results=True):
+                 return_partial_results=True, full_scan_side_switching_enabled=False):
         self.max_db_size, self.max_batch_size = max_db_size, max_batch_size
+        self.full_scan_side_switching_enabled = full_scan_side_switching_enabled
         self.return_partial_results = return_partial_results
         Qr = max_batch_size * 31
         if seeds is None:
@@ -235,22 +250,53 @@ class ServerActor:
                     self.eyes[eye][p].code_db.load_records(d, self._dummy[p][0])
                     self.eyes[eye][p].mask_db.load_records(d, self._dummy[p][1])
 
-    def process_batch_query(self, batch):
+    def process_job(self, batch):
+        """The actor's job loop body (actor.rs:585-640): with mirror-attack
+        detection, the Mirror orientation first, then the Normal one with the
+        mirror results; the full-scan side alternates when switching is on."""
+        if batch.full_face_mirror_attacks_detection_enabled:
+            mirror = self.process_batch_query(batch, MIRROR, None)
+            res = self.process_batch_query(batch, NORMAL, mirror)
+        else:
+            res = self.process_batch_query(batch, NORMAL, None)
+        if self.full_scan_side_switching_enabled:
+            self.matcher.full_scan_side = "right" if self.matcher.full_scan_side == "left" else "left"
+        return res
+
+    def _queries(self, batch, orientation):
+        """Device query sets per eye. Mirror (server/mod.rs:156-176): eye E's
+        DB is queried with the other eye's shares mirrored; the batch
+        self-comparison still uses E's own shares as its DB side."""
+        queries = {}
+        for eye, other in (("left", "right"), ("right", "left")):
+            own = getattr(batch, eye)
+            qc, qm = [], []
+            for p in range(3):
+                codes, masks = self.eyes[eye][p].codes, self.eyes[eye][p].masks
+                if orientation == MIRROR:
+                    mir = getattr(batch, other)
+                    qc.append(codes.upload_query_shares_mirrored(mir[p][0], own[p][0]))
+                    qm.append(masks.upload_query_shares_mirrored(mir[p][1], own[p][1]))
+                else:
+                    qc.append(codes.upload_query_shares(own[p][0]))
+                    qm.append(masks.upload_query_shares(own[p][1]))
+            queries[eye] = (qc, qm)
+        return queries
+
+    def process_batch_query(self, batch, orientation=NORMAL, previous_results=None):
+        assert orientation in (NORMAL, MIRROR)
         if batch.valid_entries is not None and not all(batch.valid_entries):
             batch = batch.retain([i for i, v in enumerate(batch.valid_entries) if v])
         B = len(batch.request_ids)
         assert B <= self.max_batch_size
         skip = list(batch.skip_persistence) if batch.skip_persistence is not None else [False] * B
-        self._apply_deletions(list(batch.deletion_requests_indices))
+        if previous_results is None:  # actor.rs:2637-2639: only the first pass of a job
+            self._apply_deletions(list(batch.deletion_requests_indices))
         if B == 0:
             return ServerJobResult([], [], [], [], [], [], [], [], [], [], [], [], [],
                                    list(batch.deletion_requests_indices), [], [], {})
 
-        queries = {}
-        for eye, sh in (("left", batch.left), ("right", batch.right)):
-            qc = [self.eyes[eye][p].codes.upload_query_shares(sh[p][0]) for p in range(3)]
-            qm = [self.eyes[eye][p].masks.upload_query_shares(sh[p][1]) for p in range(3)]
-            queries[eye] = (qc, qm)
+        queries = self._queries(batch, orientation)
         or_rule = batch.or_rule_indices if batch.or_rule_indices else None
         r = self.matcher.match(queries, self.current_db_size, B, or_rule_indices=or_rule,
                                reauth_targets=list(batch.reauth_target_indices.values()),
@@ -273,11 +319,21 @@ class ServerActor:
             rot_idx[eye] = [[sorted(rots.get(j, {}).get(d, [])) for d in partial_ids[eye][j]
                              if d in rots.get(j, {})] for j in range(B)]
 
-        # uniqueness insertions (actor.rs:1383-1417)
+        # mirror-attack detection (actor.rs:1304-1374)
+        prev = previous_results if orientation == NORMAL else None
+        if prev is not None:
+            attack = [merged[i] == NON_MATCH_ID and prev.matches_with_skip_persistence[i]
+                      and batch.request_types[i] == UNIQUENESS_MESSAGE_TYPE for i in range(B)]
+        else:
+            attack = [False] * B
+
+        # uniqueness insertions (actor.rs:1383-1417); with mirror results, only
+        # entries that did not match in the mirror orientation either
         uniq = [i for i in range(B)
                 if batch.request_types[i] == UNIQUENESS_MESSAGE_TYPE and merged[i] == NON_MATCH_ID
                 and partial_cnt["left"][i] <= SUPERMATCH_THRESHOLD
-                and partial_cnt["right"][i] <= SUPERMATCH_THRESHOLD]
+                and partial_cnt["right"][i] <= SUPERMATCH_THRESHOLD
+                and not (prev is not None and prev.matches_with_skip_persistence[i])]
         inserts = [i for i in uniq if not skip[i]]
         skipped = [i for i in uniq if skip[i]]
         insertion_list = distribute_insertions(inserts, [self.current_db_size])
@@ -287,22 +343,26 @@ class ServerActor:
             mwsp[i] = False
         matched_batch = [[batch.request_ids[NON_MATCH_ID - x] for x in ids if x > NON_MATCH_ID - B]
                          for ids in match_ids]
+        if previous_results is not None:  # actor.rs:1431-1450: union with the mirror pass's
+            matched_batch = [ids + [x for x in m if x not in ids]
+                             for ids, m in zip(matched_batch, previous_results.matched_batch_request_ids)]
         filtered = [[x for x in ids if x <= NON_MATCH_ID - self.max_batch_size] for ids in match_ids]
         reauth_ok = [batch.request_types[i] == REAUTH_MESSAGE_TYPE
                      and batch.reauth_target_indices[batch.request_ids[i]] in filtered[i] for i in range(B)]
 
-        # DB writes (actor.rs:1480-1600)
-        n_ins = sum(len(x) for x in insertion_list)
-        if self.current_db_size + n_ins > self.max_db_size:
-            raise RuntimeError("DB size exceeded")
-        for i in insertion_list[0]:
-            self._write_db_at_index(queries, i, self.current_db_size)
-            self.current_db_size += 1
-        for i in range(B):
-            if reauth_ok[i] and not skip[i]:
-                t = batch.reauth_target_indices[batch.request_ids[i]]
-                if t < self.current_db_size:
-                    self._write_db_at_index(queries, i, t)
+        # DB writes, Normal orientation only (actor.rs:1480-1600)
+        if orientation == NORMAL:
+            n_ins = sum(len(x) for x in insertion_list)
+            if self.current_db_size + n_ins > self.max_db_size:
+                raise RuntimeError("DB size exceeded")
+            for i in insertion_list[0]:
+                self._write_db_at_index(queries, i, self.current_db_size)
+                self.current_db_size += 1
+            for i in range(B):
+                if reauth_ok[i] and not skip[i]:
+                    t = batch.reauth_target_indices[batch.request_ids[i]]
+                    if t < self.current_db_size:
+                        self._write_db_at_index(queries, i, t)
 
         def per_eye(d):
             return d["left"], d["right"]
@@ -310,7 +370,15 @@ class ServerActor:
         pil, pir = per_eye(partial_ids)
         ril, rir = per_eye(rot_idx) if self.return_partial_results else ([[]] * B, [[]] * B)
         pcl, pcr = per_eye(partial_cnt)
-        return ServerJobResult(merged, list(batch.request_ids), list(batch.request_types), matches,
-                               mwsp, skip, match_ids, pil, pir, ril, rir, pcl, pcr,
-                               list(batch.deletion_requests_indices), matched_batch, reauth_ok,
-                               dict(batch.reauth_target_indices), r.candidates)
+        res = ServerJobResult(merged, list(batch.request_ids), list(batch.request_types), matches,
+                              mwsp, skip, match_ids, pil, pir, ril, rir, pcl, pcr,
+                              list(batch.deletion_requests_indices), matched_batch, reauth_ok,
+                              dict(batch.reauth_target_indices), r.candidates)
+        res.full_face_mirror_attack_detected = attack
+        if previous_results is not None:
+            res.full_face_mirror_match_ids = previous_results.match_ids
+            res.full_face_mirror_partial_match_ids_left = previous_results.partial_match_ids_left
+            res.full_face_mirror_partial_match_ids_right = previous_results.partial_match_ids_right
+            res.full_face_mirror_partial_match_counters_left = previous_results.partial_match_counters_left
+            res.full_face_mirror_partial_match_counters_right = previous_results.partial_match_counters_right
+        return res

@@ -68,6 +68,9 @@ void orc_gr_monomial_to_B(const uint16_t in[4], uint16_t out[4]);
 void orc_gr_lagrange_at_zero(uint16_t out[12]); /* 3 elements for points 1, x, 1+x */
 size_t orc_remap_new_to_old(size_t i);
 size_t orc_remap_old_to_new(size_t i);
+size_t orc_remap_new_to_mirrored(size_t i);  /* galois_engine.rs:276-289 */
+/* galois_engine.rs:218-224,528-545: mirrored share (flip_imaginary for codes) */
+void orc_share_mirrored(const uint16_t *in, size_t n, int flip_imaginary, uint16_t *out);
 
 /* Encode plaintext (code, mask) into 3 additive-by-Lagrange GR shares.
  * rand supplies one random GR element (4 u16) per 4-coef block.
@@ -90,6 +93,9 @@ void orc_iris_rotate_left(uint64_t code[200], int by);  /* by may be negative */
 void orc_iris_all_rotations(const uint64_t code[200], uint64_t *out /* 31*200 */);
 void orc_iris_distance(const uint64_t *c1, const uint64_t *m1, const uint64_t *c2,
                        const uint64_t *m2, uint32_t *hd, uint32_t *md);
+/* iris.rs:587-601 */
+void orc_iris_mirrored(const uint64_t code[200], const uint64_t mask[200], uint64_t out_code[200],
+                       uint64_t out_mask[200]);
 /* Integer predicate of the MPC circuit: msb_32(M*A - (c<<16) - 1) with
  * c = code dot (signed), M = mask dot (threshold_ring/cuda/kernel.cu:189-195,
  * 407-426; tests/threshold.rs:111-124). */

@@ -55,6 +55,10 @@ def _declare(L):
     L.orc_remap_new_to_old.restype = sz
     L.orc_remap_old_to_new.argtypes = [sz]
     L.orc_remap_old_to_new.restype = sz
+    L.orc_remap_new_to_mirrored.argtypes = [sz]
+    L.orc_remap_new_to_mirrored.restype = sz
+    L.orc_share_mirrored.argtypes = [vp, sz, i32, vp]
+    L.orc_iris_mirrored.argtypes = [vp, vp, vp, vp]
     L.orc_encode_iris_code.argtypes = [vp, vp, vp, vp]
     L.orc_encode_mask_trimmed.argtypes = [vp, vp, vp]
     L.orc_preprocess_query.argtypes = [i32, vp, sz]
@@ -227,6 +231,21 @@ def iris_all_rotations(code):
     c = np.ascontiguousarray(code, np.uint64)
     out = np.zeros((ROTATIONS, 200), np.uint64)
     lib().orc_iris_all_rotations(_p(c), _p(out))
+    return out
+
+
+def iris_mirrored(code, mask):
+    c = np.ascontiguousarray(code, np.uint64)
+    m = np.ascontiguousarray(mask, np.uint64)
+    oc, om = np.zeros(200, np.uint64), np.zeros(200, np.uint64)
+    lib().orc_iris_mirrored(_p(c), _p(m), _p(oc), _p(om))
+    return oc, om
+
+
+def share_mirrored(coefs, flip_imaginary):
+    a = np.ascontiguousarray(coefs, np.uint16)
+    out = np.zeros_like(a)
+    lib().orc_share_mirrored(_p(a), a.size, 1 if flip_imaginary else 0, _p(out))
     return out
 
 

@@ -149,6 +149,29 @@ size_t orc_remap_old_to_new(size_t i) {
   return b * 6400 + w * 3200 + r1 * 800 + c * 4 + r2;
 }
 
+/* galois_engine.rs:276-289: column c of each 800-coefficient row is mirrored
+ * inside its half (0..99 and 100..199), other index parts kept. */
+size_t orc_remap_new_to_mirrored(size_t i) {
+  size_t b = i / 6400, w = i % 6400 / 3200, r1 = i % 3200 / 800, c = i % 800 / 4, r2 = i % 4;
+  size_t fc = c < 100 ? 99 - c : 199 - (c - 100);
+  return b * 6400 + w * 3200 + r1 * 800 + fc * 4 + r2;
+}
+
+/* GaloisRingIrisCodeShare::mirrored (galois_engine.rs:528-545) and the trimmed
+ * mask's mirrored (:218-224): out[mirror(i)] = in[i]; with flip_imaginary the
+ * coefficients of the imaginary half (i >= 6400) are first negated. The
+ * reference negates by subtracting from a fixed sharing of zero
+ * (flipped_imaginary_coeffs, :291-300, randomness StdRng(0)); any sharing of
+ * zero gives a sharing of the same negated secret, and this restatement uses
+ * the all-zero one: the opened dot products are the same. */
+void orc_share_mirrored(const uint16_t *in, size_t n, int flip_imaginary, uint16_t *out) {
+  for (size_t i = 0; i < n; i++) {
+    uint16_t v = in[i];
+    if (flip_imaginary && i >= ORC_CODE_LEN / 2) v = (uint16_t)(0u - v);
+    out[orc_remap_new_to_mirrored(i)] = v;
+  }
+}
+
 static inline int get_bit(const uint64_t *a, size_t i) { return (int)((a[i / 64] >> (i % 64)) & 1); }
 
 /* Degree-1 sharing of a monomial-basis element s with randomness r at the

@@ -214,3 +214,19 @@ void orc_open_bitmap(const uint64_t *r1, const uint64_t *r2, const uint64_t *r3,
     }
   }
 }
+
+/* IrisCode::mirrored (iris_db/iris.rs:587-601): bit i moves to
+ * old(mirror(new(i))); code bits of the imaginary part (i odd) are inverted,
+ * mask bits move unchanged. */
+void orc_iris_mirrored(const uint64_t code[200], const uint64_t mask[200], uint64_t out_code[200],
+                       uint64_t out_mask[200]) {
+  memset(out_code, 0, 1600);
+  memset(out_mask, 0, 1600);
+  for (size_t i = 0; i < 12800; i++) {
+    size_t m = orc_remap_new_to_old(orc_remap_new_to_mirrored(orc_remap_old_to_new(i)));
+    uint64_t cb = (code[i / 64] >> (i % 64)) & 1, mb = (mask[i / 64] >> (i % 64)) & 1;
+    if (i % 2) cb ^= 1;
+    out_code[m / 64] |= cb << (m % 64);
+    out_mask[m / 64] |= mb << (m % 64);
+  }
+}

@@ -135,3 +135,46 @@ def test_server_two_batches(irismpc):
     assert r2.successful_reauths == [False] * 6 + [True]
     assert r2.partial_match_ids_left[3] == [1003] and r2.partial_match_ids_right[3] == [1003]
     assert actor.current_db_size == N0 + 6
+
+
+def test_server_mirror_attack(irismpc, orc):
+    """Full-face mirror attack detection (actor.rs:585-640,1304-1417): a face
+    presented mirrored shows each eye as the mirror image of the other. The
+    Mirror pass (each eye's DB against the other eye's query mirrored) finds
+    it, the Normal pass does not; it is flagged and not inserted."""
+    from irismpc import synth
+    from irismpc.server import ServerActor
+
+    N = 300
+    db = {"left": [_iris(31, i) for i in range(N)], "right": [_iris(32, i) for i in range(N)]}
+    actor = ServerActor(1024, 8)
+    sh = {}
+    for k, eye in enumerate(("left", "right")):
+        sh[eye] = synth.shares_of(np.array([c for c, _ in db[eye]]), np.array([m for _, m in db[eye]]), 310 + k)
+    actor.load_db(sh["left"], sh["right"])
+
+    def mirrored(eye, i, key):
+        c, m = orc.iris_mirrored(*db[eye][i])
+        return _flip(c, key, 800), m
+
+    def build():
+        b = Batch()
+        b.add("attack", mirrored("right", 10, 1), mirrored("left", 10, 2))      # mirror image of face 10
+        b.add("m20", (_flip(db["left"][20][0], 3), db["left"][20][1]),
+              (_flip(db["right"][20][0], 4), db["right"][20][1]))               # ordinary DB match
+        b.add("new", _iris(33, 0), _iris(34, 0))                                # unique
+        return b
+
+    q = build().query()
+    q.full_face_mirror_attacks_detection_enabled = True
+    r = actor.process_job(q)
+    assert r.merged_results == [NM, 20, N]
+    assert r.full_face_mirror_attack_detected == [True, False, False]
+    assert r.full_face_mirror_match_ids[0] == [10] and r.full_face_mirror_match_ids[1] == []
+    assert r.matches == [True, True, False]
+    assert actor.current_db_size == N + 1
+
+    # without detection the mirrored face is taken as a new identity
+    q = build().query()
+    r = actor.process_job(q)
+    assert r.merged_results[0] == N + 1 and r.full_face_mirror_attack_detected == [False] * 3

@@ -200,3 +200,30 @@ def test_write_at_index_insert_form(irismpc, orc):
     assert np.array_equal(slices.read_records(40, 1)[0], raw[3])
     with pytest.raises(irismpc.IrisMpcError, match="beyond query batch"):
         slices.write_at_index(dq, 5, 41)
+
+
+@pytest.mark.parametrize("K,flip", [(12800, True), (6400, False)])
+def test_mirrored_query_upload(irismpc, orc, K, flip):
+    """Mirror-orientation queries: rows = 31 rotations of the preprocessed
+    mirrored share (mirrored_code with the imaginary half negated for codes,
+    mirrored_mask for masks), checked through the GEMM against the oracle; the
+    DB form is the separately given shares."""
+    rng = np.random.default_rng(13)
+    N, chunk, party, batch = 300, 512, 2, 3
+    qr = batch * 31
+    raw, own = _rand(rng, (batch, K)), _rand(rng, (batch, K))
+    db = _rand(rng, (N, K))
+    rows = np.zeros((qr, K), np.uint16)
+    for j in range(batch):
+        rows[j * 31:(j + 1) * 31] = orc.share_rotations(
+            orc.preprocess_query(party, orc.share_mirrored(raw[j], flip)))
+    ref = orc.u16_dot(rows, db)
+    eng = irismpc.ShareDB(party, chunk, qr, K, ([0] * 8, [0] * 8))
+    slices = eng.alloc_db(N)
+    slices.load_full_db(db)
+    dq = eng.upload_query_shares_mirrored(raw, own)
+    eng.dot_reduce(dq, slices, chunk, 0, 1)
+    got = eng.fetch_results(qr * chunk).reshape(qr, chunk)
+    assert np.array_equal(got[:, :N], ref)
+    slices.write_at_index(dq, 1, 7)
+    assert np.array_equal(slices.read_records(7, 1)[0], own[1])

@@ -372,3 +372,52 @@ def test_open_bitmap_or_over_rotations(orc):
     orc.open_bitmap(*shares, chunk, 31, 64, 60, 200, bm)
     bits = np.unpackbits(bm.view(np.uint8), bitorder="little")
     assert np.nonzero(bits)[0].tolist() == [69]  # totalDb*0 + db 5 + offset 64
+
+
+# --------------------------------------------------------------- mirror
+def test_iris_mirrored_golden(orc):
+    """iris.rs:739-768: flipped_codes.txt holds an iris and its mirror image;
+    mirroring the mirror gives the original at distance 0."""
+    lines = _lines("flipped_codes.txt")
+    code, mask, fcode, fmask = (orc.iris_from_base64(s) for s in lines[:4])
+    mc, mm = orc.iris_mirrored(fcode, fmask)
+    h, m = orc.iris_distance(code, mask, mc, mm)
+    assert m > 0 and h == 0
+    assert np.array_equal(mm, mask)
+    # the unmirrored pair sits near 0.5 (galois_engine.rs:940-1000 asserts 0.5 +- 0.1)
+    h2, m2 = orc.iris_distance(code, mask, fcode, fmask)
+    assert abs(h2 / m2 - 0.5) < 0.1
+    # involution
+    back = orc.iris_mirrored(mc, mm)
+    assert np.array_equal(back[0], fcode) and np.array_equal(back[1], fmask)
+
+
+def test_share_mirrored_matches_plaintext_mirror(orc):
+    """galois_engine.rs:940-1000: mirrored shares of the flipped iris match the
+    original's shares at distance 0; and mirroring shares equals encoding the
+    mirrored plaintext with correspondingly moved (and, on the imaginary half,
+    negated) randomness, share for share."""
+    lines = _lines("flipped_codes.txt")
+    code, mask, fcode, fmask = (orc.iris_from_base64(s) for s in lines[:4])
+    db_c, db_m = _shares_of(orc, code, mask, 7)
+    q_c, q_m = _shares_of(orc, fcode, fmask, 8)
+    mq_c = [orc.share_mirrored(q_c[p], True) for p in range(3)]
+    mq_m = [orc.share_mirrored(q_m[p], False) for p in range(3)]
+    cd = sum(orc.trick_dot(db_c[p], orc.preprocess_query(p, mq_c[p])) for p in range(3)) & 0xFFFF
+    md = 2 * sum(orc.trick_dot(db_m[p], orc.preprocess_query(p, mq_m[p])) for p in range(3)) & 0xFFFF
+    mc, mm = orc.iris_mirrored(fcode, fmask)
+    h, m = orc.iris_distance(code, mask, mc, mm)
+    assert h == 0 and md == m and cd == m  # code dot M - 2H with H = 0
+
+    rng = np.random.default_rng(9)
+    cr = rng.integers(0, 1 << 16, 12800, dtype=np.uint16)
+    mirror = np.array([orc.lib().orc_remap_new_to_mirrored(i) for i in range(12800)])
+    cr2 = np.zeros_like(cr)
+    neg = np.arange(12800) >= 6400
+    cr2[mirror] = np.where(neg, (0 - cr.astype(np.int64)) & 0xFFFF, cr).astype(np.uint16)
+    want = orc.encode_iris_code(mc, mm, cr2)
+    got = orc.encode_iris_code(fcode, fmask, cr)
+    for p in range(3):
+        assert np.array_equal(orc.share_mirrored(got[p], True), want[p])
+    # a wrong variant (no imaginary negation) does not reproduce it
+    assert not np.array_equal(orc.share_mirrored(got[0], False), want[0])
