@@ -290,6 +290,21 @@ int irismpc_comparator_get_partial_results(irismpc_comparator* d, const uint64_t
 int irismpc_comparator_join_batch_matches(irismpc_comparator* d, const uint64_t* left,
                                           const uint64_t* right, uint32_t* final_results,
                                           size_t batch_size, size_t row_bits, void* stream);
+/* Match-distance capture for anonymized statistics (openResults,
+ * dot/kernel.cu:54-116; storeAnonStatsWithIndexMapping :153-210; buffers of
+ * server/anon_stats.rs:13-123) in the opens that follow: for every opened
+ * match (and every rotation of each target pair db * (max_query_length / 31)
+ * + query from targets, host u64, matching or not) the four dot-product
+ * shares (code a/b, mask a/b, u16) and the match id batch_id * max_db_length
+ * * max_query_length + db * max_query_length + query row (u64) are appended
+ * to the device buffers (capacity entries; *counter counts every entry).
+ * irismpc_scan_db* capture in their opens; irismpc_scan_subset_trio captures
+ * in its separate anon_threshold pass. capacity 0 switches the capture off. */
+int irismpc_comparator_set_anon_capture(irismpc_comparator* d, uint16_t* code_a, uint16_t* code_b,
+                                        uint16_t* mask_a, uint16_t* mask_b, uint64_t* idx,
+                                        uint32_t* counter, size_t capacity, uint64_t batch_id,
+                                        size_t max_query_length, size_t max_db_length,
+                                        const uint64_t* targets, size_t n_targets);
 /* Device pointer of the last get_partial_results list. */
 int irismpc_comparator_partial_ids(irismpc_comparator* d, const uint32_t** ids);
 /* open_results_with_index_mapping (distance_comparator.rs:281-342,
@@ -355,7 +370,9 @@ int irismpc_scan_chunk_trio(irismpc_share_db* const codes[3], irismpc_share_db* 
  * from code_db/mask_db into code_sub/mask_sub (slices of DB_CHUNK_SIZE rows)
  * DB_CHUNK_SIZE at a time, compared at threshold_a, opened with the index
  * mapping into bitmaps[p] (rows total_db_len bits apart) and captured with
- * their rotations. */
+ * their rotations. anon_threshold != 0: every chunk is first compared at it
+ * for the distance capture alone (the db_threshold_anon_stats pass,
+ * actor.rs:2040-2120), when the comparator has a capture set. */
 int irismpc_scan_subset_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
                              irismpc_circuits* const circuits[3], irismpc_comparator* const cmp[3],
                              const irismpc_db* const code_db[3], const irismpc_db* const mask_db[3],
@@ -363,8 +380,8 @@ int irismpc_scan_subset_trio(irismpc_share_db* const codes[3], irismpc_share_db*
                              const irismpc_query* const q_code[3],
                              const irismpc_query* const q_mask[3], const uint32_t* const ids[3],
                              size_t n_ids, size_t total_db_len, size_t batch_size,
-                             uint32_t threshold_a, uint64_t* const bitmaps[3],
-                             void* const streams[3]);
+                             uint32_t threshold_a, uint32_t anon_threshold,
+                             uint64_t* const bitmaps[3], void* const streams[3]);
 /* ServerActor::compare_query_against_self (actor.rs:1789-1897) for the three
  * parties: the batch's queries (all rotations) against the batch's own
  * irises in DB form at threshold_a; bit row_bits * q + j of bitmaps[p] = some

@@ -6,6 +6,15 @@
 
 namespace irismpc {
 
+static DotViews dot_views(const PartyScan& p) {
+  DotViews d;
+  d.code_a = p.codes->results();
+  d.code_b = p.codes->results_peer();
+  d.mask_a = p.masks->results();
+  d.mask_b = p.masks->results_peer();
+  return d;
+}
+
 void scan_chunk(const PartyScan& p, size_t offset, size_t db_size, size_t batch_size,
                 uint32_t threshold_a, uint64_t* bitmap, const uint32_t* slot_of, size_t row_bits,
                 hipStream_t s) {
@@ -27,15 +36,17 @@ void scan_chunk(const PartyScan& p, size_t offset, size_t db_size, size_t batch_
                                             p.masks->results(), p.masks->results_peer(), len,
                                             threshold_a, s);
   p.circuits->open_exchange(s);
-  if (real > 0)
+  if (real > 0) {
+    const DotViews dots = dot_views(p);
     p.comparator->open_results(p.circuits->result_a(), p.circuits->result_b(),
                                p.circuits->result_c(), bitmap, dot_chunk, real, offset,
-                               row_bits ? row_bits : db_size, batch_size, s, rows, slot_of);
+                               row_bits ? row_bits : db_size, batch_size, s, rows, slot_of, &dots);
+  }
 }
 
 void scan_subset(const PartyScan& p, DbSlice& code_sub, DbSlice& mask_sub, const uint32_t* ids,
                  size_t n_ids, size_t total_db_len, size_t batch_size, uint32_t threshold_a,
-                 uint64_t* bitmap, hipStream_t s) {
+                 uint64_t* bitmap, hipStream_t s, uint32_t anon_threshold) {
   const size_t rows = (size_t)p.code_query->rows;
   IRIS_REQUIRE((size_t)p.mask_query->rows == rows, "code/mask query sets disagree on rows");
   IRIS_REQUIRE(batch_size * kRotations <= rows, "batch larger than the query rows");
@@ -55,6 +66,17 @@ void scan_subset(const PartyScan& p, DbSlice& code_sub, DbSlice& mask_sub, const
     p.masks->reshare_results(dot_chunk, s);
     const size_t len = dot_chunk * rows;
     p.circuits->set_chunk_size(len / 64);
+    if (anon_threshold && p.comparator->anon_capture_enabled()) {  // actor.rs:2040-2120
+      const DotViews dots = dot_views(p);
+      p.circuits->compare_threshold_masked_many(p.codes->results(), p.codes->results_peer(),
+                                                p.masks->results(), p.masks->results_peer(), len,
+                                                anon_threshold, s);
+      p.circuits->open_exchange(s);
+      p.comparator->open_results_mapped(p.circuits->result_a(), p.circuits->result_b(),
+                                        p.circuits->result_c(), nullptr, dot_chunk, real,
+                                        total_db_len, batch_size, ids + c0, s, rows, nullptr,
+                                        &dots);
+    }
     p.circuits->compare_threshold_masked_many(p.codes->results(), p.codes->results_peer(),
                                               p.masks->results(), p.masks->results_peer(), len,
                                               threshold_a, s);
@@ -120,10 +142,10 @@ void scan_chunk_trio(const PartyScan p[3], size_t offset, size_t db_size, size_t
 void scan_subset_trio(const PartyScan p[3], DbSlice* const code_sub[3], DbSlice* const mask_sub[3],
                       const uint32_t* const ids[3], size_t n_ids, size_t total_db_len,
                       size_t batch_size, uint32_t threshold_a, uint64_t* const bitmaps[3],
-                      hipStream_t const s[3]) {
+                      hipStream_t const s[3], uint32_t anon_threshold) {
   run_trio([&](int i) {
     scan_subset(p[i], *code_sub[i], *mask_sub[i], ids[i], n_ids, total_db_len, batch_size,
-                threshold_a, bitmaps[i], s[i]);
+                threshold_a, bitmaps[i], s[i], anon_threshold);
   });
 }
 

@@ -20,7 +20,8 @@ struct PartyScan {
 
 // For every chunk of kDbChunk rows: code/mask dot products with the fused
 // correction and zero-share masking, reshare, compare against threshold_a,
-// open and OR the opened bits into `bitmap` (bit total_db*(q/31) + db).
+// open and OR the opened bits into `bitmap` (bit total_db*(q/31) + db); with
+// a distance capture set on the comparator, matches' distances are captured.
 // Collective: the three parties call it concurrently (one thread each).
 void scan_db(const PartyScan& p, size_t db_size, size_t batch_size, uint32_t threshold_a,
              uint64_t* bitmap, hipStream_t s);
@@ -45,13 +46,16 @@ void scan_chunk_trio(const PartyScan p[3], size_t offset, size_t db_size, size_t
 // threshold_a, and the opened bits set at the rows' DB ids
 // (openResultsWithIndexMapping), with the rotation capture. n_ids = 0 does
 // nothing. Collective like scan_db.
+// anon_threshold != 0 (and a distance capture set on the comparator): each
+// chunk is first compared at anon_threshold and only its distances captured
+// (the db_threshold_anon_stats pass, actor.rs:2040-2120).
 void scan_subset(const PartyScan& p, DbSlice& code_sub, DbSlice& mask_sub, const uint32_t* ids,
                  size_t n_ids, size_t total_db_len, size_t batch_size, uint32_t threshold_a,
-                 uint64_t* bitmap, hipStream_t s);
+                 uint64_t* bitmap, hipStream_t s, uint32_t anon_threshold = 0);
 void scan_subset_trio(const PartyScan p[3], DbSlice* const code_sub[3], DbSlice* const mask_sub[3],
                       const uint32_t* const ids[3], size_t n_ids, size_t total_db_len,
                       size_t batch_size, uint32_t threshold_a, uint64_t* const bitmaps[3],
-                      hipStream_t const s[3]);
+                      hipStream_t const s[3], uint32_t anon_threshold = 0);
 
 // compare_query_against_self (actor.rs:1789-1897): the batch's query rows
 // (all rotations) against the batch's own irises in DB form (the query sets'

@@ -533,6 +533,19 @@ int irismpc_comparator_join_batch_matches(irismpc_comparator* d, const uint64_t*
     d->impl->join_batch_matches(left, right, final_results, batch_size, row_bits, st(stream));
   });
 }
+int irismpc_comparator_set_anon_capture(irismpc_comparator* d, uint16_t* code_a, uint16_t* code_b,
+                                        uint16_t* mask_a, uint16_t* mask_b, uint64_t* idx,
+                                        uint32_t* counter, size_t capacity, uint64_t batch_id,
+                                        size_t max_query_length, size_t max_db_length,
+                                        const uint64_t* targets, size_t n_targets) {
+  return guarded([&] {
+    AnonCapture c;
+    c.code_a = code_a; c.code_b = code_b; c.mask_a = mask_a; c.mask_b = mask_b;
+    c.idx = idx; c.counter = counter; c.capacity = capacity; c.batch_id = batch_id;
+    c.max_query_length = max_query_length; c.max_db_length = max_db_length;
+    d->impl->set_anon_capture(c, targets, n_targets);
+  });
+}
 int irismpc_comparator_partial_ids(irismpc_comparator* d, const uint32_t** ids) {
   return guarded([&] { *ids = d->impl->partial_ids(); });
 }
@@ -641,8 +654,8 @@ int irismpc_scan_subset_trio(irismpc_share_db* const codes[3], irismpc_share_db*
                              const irismpc_query* const q_code[3],
                              const irismpc_query* const q_mask[3], const uint32_t* const ids[3],
                              size_t n_ids, size_t total_db_len, size_t batch_size,
-                             uint32_t threshold_a, uint64_t* const bitmaps[3],
-                             void* const streams[3]) {
+                             uint32_t threshold_a, uint32_t anon_threshold,
+                             uint64_t* const bitmaps[3], void* const streams[3]) {
   return guarded([&] {
     PartyScan p[3];
     hipStream_t s[3];
@@ -655,7 +668,8 @@ int irismpc_scan_subset_trio(irismpc_share_db* const codes[3], irismpc_share_db*
       cs[i] = code_sub[i]->impl.get();
       ms[i] = mask_sub[i]->impl.get();
     }
-    scan_subset_trio(p, cs, ms, ids, n_ids, total_db_len, batch_size, threshold_a, bitmaps, s);
+    scan_subset_trio(p, cs, ms, ids, n_ids, total_db_len, batch_size, threshold_a, bitmaps, s,
+                     anon_threshold);
   });
 }
 

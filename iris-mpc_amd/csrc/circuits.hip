@@ -469,6 +469,14 @@ __global__ __launch_bounds__(256) void open_kernel(const OpenArgs a) {
   const size_t valid = a.real_chunk_len - d0;
   if (valid < 64) v &= (1ull << valid) - 1;
   if (!v) return;
+  if (a.anon.enabled()) {
+    uint64_t m = v;
+    while (m) {
+      const int k = __builtin_ctzll(m);
+      m &= m - 1;
+      anon_store(a.anon, a.dots, e0 + (size_t)k, a.offset + d0 + (size_t)k, q);
+    }
+  }
   const size_t slot = a.slot_of ? a.slot_of[q / kRotations] : q / kRotations;
   const size_t bit = a.total_db_len * slot + d0 + a.offset;
   const size_t wi = bit >> 6, sh = bit & 63;

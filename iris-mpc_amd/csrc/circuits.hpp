@@ -1,5 +1,6 @@
 // Threshold-comparison circuits on replicated shares (3 parties, one GPU).
 #pragma once
+#include "anon.hpp"
 #include "common.hpp"
 
 namespace irismpc {
@@ -72,6 +73,8 @@ struct OpenArgs {
   size_t real_chunk_len;
   size_t total_db_len;
   const uint32_t* slot_of;  // optional: bitmap row of batch query j
+  AnonCapture anon;         // optional match-distance capture
+  DotViews dots;
 };
 void launch_open(const OpenArgs& a, hipStream_t s);
 

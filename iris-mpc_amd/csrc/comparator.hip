@@ -171,6 +171,8 @@ __global__ __launch_bounds__(256) void open_mapped_kernel(const MappedOpenArgs a
     v &= v - 1;
     const uint32_t id = a.index_map[d0 + (size_t)b];
     if (id >= a.total_db_len) continue;
+    if (a.anon.enabled()) anon_store(a.anon, a.dots, e0 + (size_t)b, id, q);
+    if (!a.set_bits) continue;
     const size_t pos = a.total_db_len * slot + id;
     atomicOr(reinterpret_cast<unsigned long long*>(&a.bitmap[pos >> 6]), 1ull << (pos & 63));
     const uint32_t c = atomicAdd(a.counter, 1u);
@@ -180,6 +182,35 @@ __global__ __launch_bounds__(256) void open_mapped_kernel(const MappedOpenArgs a
       a.cap_rot[c] = rot;
     }
   }
+}
+
+__global__ __launch_bounds__(256) void anon_targets_kernel(const AnonTargetArgs a) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.cap.n_targets * kRotations) return;
+  const size_t max_batch = a.cap.max_query_length / kRotations;
+  const uint64_t target = a.cap.targets[t / kRotations];
+  const size_t db = (size_t)(target / max_batch), j = (size_t)(target % max_batch);
+  if (j >= a.batch) return;
+  size_t p = 0;
+  if (a.index_map) {
+    for (p = 0; p < a.real_chunk_len; p++)
+      if (a.index_map[p] == db) break;
+    if (p == a.real_chunk_len) return;
+  } else {
+    if (db < a.offset || db >= a.offset + a.real_chunk_len) return;
+    p = db - a.offset;
+  }
+  const size_t q = j * kRotations + t % kRotations;
+  const size_t e = q * a.chunk_len + p;
+  const uint64_t w = a.a[e >> 6] ^ a.b[e >> 6] ^ a.c[e >> 6];
+  if ((w >> (e & 63)) & 1) return;  // a match: captured by the open itself
+  anon_store(a.cap, a.dots, e, db, q);
+}
+
+void launch_anon_targets(const AnonTargetArgs& a, hipStream_t s) {
+  if (!a.cap.enabled() || a.cap.n_targets == 0) return;
+  anon_targets_kernel<<<(unsigned)ceil_div(a.cap.n_targets * kRotations, 256), 256, 0, s>>>(a);
+  IRIS_HIP_CHECK(hipGetLastError());
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {

@@ -1,5 +1,6 @@
 // DistanceComparator device helpers (merge of eye bitmaps, buffer fills).
 #pragma once
+#include "anon.hpp"
 #include "common.hpp"
 
 namespace irismpc {
@@ -75,6 +76,9 @@ struct MappedOpenArgs {
   uint32_t* cap_db;
   int8_t* cap_rot;
   size_t capacity;
+  int set_bits;      // 0: capture distances only (the anonymized-statistics pass)
+  AnonCapture anon;  // optional match-distance capture
+  DotViews dots;
 };
 void launch_open_mapped(const MappedOpenArgs& a, hipStream_t s);
 void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);

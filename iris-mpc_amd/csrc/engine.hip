@@ -707,10 +707,31 @@ std::unique_ptr<DevBuf<uint64_t>> DistanceComparator::prepare_db_match_list(size
       ceil_div(db_size * (query_length_ / kRotations), 64) + 1);
 }
 
+void DistanceComparator::set_anon_capture(const AnonCapture& c, const uint64_t* host_targets,
+                                          size_t n_targets) {
+  DeviceGuard g(device_);
+  if (!c.enabled()) {
+    anon_ = AnonCapture{};
+    return;
+  }
+  IRIS_REQUIRE(c.code_a && c.code_b && c.mask_a && c.mask_b && c.idx, "capture buffers are NULL");
+  IRIS_REQUIRE(c.max_query_length % kRotations == 0 && c.max_query_length > 0,
+               "max_query_length must be a multiple of 31");
+  IRIS_REQUIRE(!n_targets || host_targets, "targets are NULL");
+  anon_ = c;
+  if (n_targets) {
+    if (anon_targets_.n < n_targets) anon_targets_.alloc(n_targets, false);
+    IRIS_HIP_CHECK(hipMemcpy(anon_targets_.ptr, host_targets, n_targets * 8, hipMemcpyHostToDevice));
+  }
+  anon_.targets = anon_targets_.ptr;
+  anon_.n_targets = n_targets;
+}
+
 void DistanceComparator::open_results(const uint64_t* a, const uint64_t* b, const uint64_t* c,
                                       uint64_t* bitmap, size_t chunk_len, size_t real_chunk_len,
                                       size_t offset, size_t total_db_len, size_t batch_size,
-                                      hipStream_t s, size_t rows, const uint32_t* slot_of) const {
+                                      hipStream_t s, size_t rows, const uint32_t* slot_of,
+                                      const DotViews* dots) const {
   DeviceGuard g(device_);
   if (rows == 0) rows = query_length_;
   IRIS_REQUIRE(chunk_len % 64 == 0, "chunk length must be a multiple of 64");
@@ -723,7 +744,19 @@ void DistanceComparator::open_results(const uint64_t* a, const uint64_t* b, cons
   o.offset = offset;
   o.real_chunk_len = real_chunk_len;
   o.total_db_len = total_db_len;
+  const bool capture = dots && anon_.enabled();
+  if (capture) {
+    o.anon = anon_;
+    o.dots = *dots;
+  }
   launch_open(o, s);
+  if (capture) {
+    AnonTargetArgs t{};
+    t.cap = anon_; t.dots = *dots; t.a = a; t.b = b; t.c = c;
+    t.chunk_len = chunk_len; t.real_chunk_len = real_chunk_len; t.offset = offset;
+    t.batch = batch_size; t.index_map = nullptr;
+    launch_anon_targets(t, s);
+  }
 }
 
 size_t DistanceComparator::get_partial_results(const uint64_t* bitmap, size_t db_size,
@@ -754,7 +787,8 @@ void DistanceComparator::open_results_mapped(const uint64_t* a, const uint64_t* 
                                              size_t chunk_len, size_t real_chunk_len,
                                              size_t total_db_len, size_t batch_size,
                                              const uint32_t* index_map, hipStream_t s,
-                                             size_t rows, const uint32_t* slot_of) {
+                                             size_t rows, const uint32_t* slot_of,
+                                             const DotViews* dots) {
   DeviceGuard g(device_);
   if (rows == 0) rows = query_length_;
   IRIS_REQUIRE(chunk_len % 64 == 0, "chunk length must be a multiple of 64");
@@ -774,7 +808,21 @@ void DistanceComparator::open_results_mapped(const uint64_t* a, const uint64_t* 
   o.cap_db = cap_db_.ptr;
   o.cap_rot = cap_rot_.ptr;
   o.capacity = cap_q_.n;
+  o.set_bits = bitmap != nullptr;
+  const bool capture = dots && anon_.enabled();
+  IRIS_REQUIRE(bitmap || capture, "nothing to open into: no bitmap and no distance capture");
+  if (capture) {
+    o.anon = anon_;
+    o.dots = *dots;
+  }
   launch_open_mapped(o, s);
+  if (capture) {
+    AnonTargetArgs t{};
+    t.cap = anon_; t.dots = *dots; t.a = a; t.b = b; t.c = c;
+    t.chunk_len = chunk_len; t.real_chunk_len = real_chunk_len; t.offset = 0;
+    t.batch = batch_size; t.index_map = index_map;
+    launch_anon_targets(t, s);
+  }
 }
 
 size_t DistanceComparator::fetch_partial_rotations(uint32_t* query, uint32_t* db, int8_t* rot,

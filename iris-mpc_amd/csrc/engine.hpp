@@ -284,10 +284,18 @@ class DistanceComparator {
   std::unique_ptr<DevBuf<uint64_t>> prepare_db_match_list(size_t db_size) const;
   // rows: query rows of the circuit result (0 = query_length); slot_of (device,
   // optional): batch query j's bits go to bitmap row slot_of[j] instead of j
+  // dots (optional): the chunk's dot-product shares, for the match-distance
+  // capture when one is set
   void open_results(const uint64_t* a, const uint64_t* b, const uint64_t* c, uint64_t* bitmap,
                     size_t chunk_len, size_t real_chunk_len, size_t offset, size_t total_db_len,
                     size_t batch_size, hipStream_t s, size_t rows = 0,
-                    const uint32_t* slot_of = nullptr) const;
+                    const uint32_t* slot_of = nullptr, const DotViews* dots = nullptr) const;
+  // Match-distance capture of the following opens (openResults' anonymized-
+  // statistics buffers, dot/kernel.cu:73-106; server/anon_stats.rs): device
+  // buffers of `capacity` entries and a counter; targets (host, db * max_batch
+  // + query) are also captured when not matching. capacity 0 switches it off.
+  void set_anon_capture(const AnonCapture& c, const uint64_t* host_targets, size_t n_targets);
+  bool anon_capture_enabled() const { return anon_.enabled(); }
   // join_db_matches (AND policy) and join_db_matches_with_bitmaps (OR policy).
   void join_db_matches(const uint64_t* left, const uint64_t* right, uint32_t* final_results,
                        size_t db_size, const uint64_t* or_policy, size_t max_db_size,
@@ -311,10 +319,14 @@ class DistanceComparator {
   // open_results_with_index_mapping (distance_comparator.rs:281-342): subset
   // chunk column j is DB row index_map[j] (device); matches also go to the
   // rotation capture (partial_results_*, read by fetch_partial_rotations)
+  // With dots set and bitmap == nullptr: the anonymized-statistics pass of
+  // the subset (storeAnonStatsWithIndexMapping, dot/kernel.cu:153-210), which
+  // only captures distances.
   void open_results_mapped(const uint64_t* a, const uint64_t* b, const uint64_t* c,
                            uint64_t* bitmap, size_t chunk_len, size_t real_chunk_len,
                            size_t total_db_len, size_t batch_size, const uint32_t* index_map,
-                           hipStream_t s, size_t rows = 0, const uint32_t* slot_of = nullptr);
+                           hipStream_t s, size_t rows = 0, const uint32_t* slot_of = nullptr,
+                           const DotViews* dots = nullptr);
   // get_partial_results_with_rotations (distance_comparator.rs:650-711):
   // copies up to `capacity` captured (query, db id, rotation) triples to the
   // host, resets the capture and returns the number of matches seen (may
@@ -340,6 +352,8 @@ class DistanceComparator {
   DevBuf<uint32_t> tiles_, ids_, count_, extra_;
   DevBuf<uint32_t> cap_counter_, cap_q_, cap_db_;
   DevBuf<int8_t> cap_rot_;
+  AnonCapture anon_;
+  DevBuf<uint64_t> anon_targets_;
 };
 
 }  // namespace irismpc

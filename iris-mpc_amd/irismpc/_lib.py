@@ -105,7 +105,8 @@ SIGNATURES = {
     "irismpc_comparator_fetch_partial_rotations": [vp, vp, vp, vp, sz, vp, vp],
     "irismpc_db_prefetch_subset": [vp, vp, vp, sz, vp, vp],
     "irismpc_db_load_stored_records": [vp, vp, vp, vp, vp, vp, vp, sz, u32, vp, vp, vp],
-    "irismpc_scan_subset_trio": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, sz, sz, u32, vp, vp],
+    "irismpc_scan_subset_trio": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, sz, sz, u32, u32, vp, vp],
+    "irismpc_comparator_set_anon_capture": [vp, vp, vp, vp, vp, vp, vp, sz, u64, sz, sz, vp, sz],
     "irismpc_comparator_buffers": [vp, vp, vp, vp, vp, vp, vp],
 }
 

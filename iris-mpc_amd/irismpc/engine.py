@@ -419,6 +419,19 @@ class DistanceComparator:
         call("irismpc_comparator_join_batch_matches", self.h, left, right, final_results, batch_size,
              row_bits, stream)
 
+    def set_anon_capture(self, buffers=None, batch_id=0, max_query_length=0, max_db_length=0,
+                         targets=()):
+        """Match-distance capture of the following opens into `buffers` (an
+        anon_stats.DistanceBuffers); None switches it off."""
+        t = np.ascontiguousarray(np.asarray(targets, dtype=np.uint64))
+        if buffers is None:
+            call("irismpc_comparator_set_anon_capture", self.h, None, None, None, None, None, None, 0,
+                 0, 0, 0, None, 0)
+            return
+        call("irismpc_comparator_set_anon_capture", self.h, *buffers.pointers(), buffers.capacity,
+             batch_id, max_query_length or self.query_length, max_db_length or self.max_db_size,
+             t.ctypes.data if t.size else None, t.size)
+
     def partial_ids_ptr(self):
         p = _out()
         call("irismpc_comparator_partial_ids", self.h, ctypes.byref(p))
@@ -483,7 +496,7 @@ def scan_chunk_trio(parties, q_codes, q_masks, offset, db_size, batch_size, thre
 
 
 def scan_subset_trio(parties, code_subs, mask_subs, q_codes, q_masks, ids_ptrs, n_ids, total_db_len,
-                     batch_size, threshold_a, bitmap_ptrs, streams):
+                     batch_size, threshold_a, bitmap_ptrs, streams, anon_threshold=0):
     """The DB half of compare_query_against_db_subset_and_self (actor.rs:
     1900-2244) for the three parties: subset ids (device u32 per party) are
     gathered into the subset slices, compared at threshold_a and opened at
@@ -494,7 +507,7 @@ def scan_subset_trio(parties, code_subs, mask_subs, q_codes, q_masks, ids_ptrs, 
          ptr_array([p.mask_db.h for p in parties]), ptr_array([d.h for d in code_subs]),
          ptr_array([d.h for d in mask_subs]), ptr_array([q.h for q in q_codes]),
          ptr_array([q.h for q in q_masks]), ptr_array(list(ids_ptrs)), n_ids, total_db_len,
-         batch_size, threshold_a, ptr_array(list(bitmap_ptrs)), ptr_array(list(streams)))
+         batch_size, threshold_a, anon_threshold, ptr_array(list(bitmap_ptrs)), ptr_array(list(streams)))
 
 
 def scan_batch_self_trio(parties, q_codes, q_masks, batch_size, threshold_a, bitmap_ptrs, streams,

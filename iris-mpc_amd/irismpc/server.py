@@ -147,6 +147,8 @@ class ServerJobResult:
     full_face_mirror_partial_match_ids_right: list = field(default_factory=list)
     full_face_mirror_partial_match_counters_left: list = field(default_factory=list)
     full_face_mirror_partial_match_counters_right: list = field(default_factory=list)
+    one_sided_match_distances: dict = None   # eye -> per party OneSidedDistanceCache
+    two_sided_match_distances: list = None   # per party TwoSidedDistanceCache
 
 
 def _dummy_shares():
@@ -171,8 +173,19 @@ class ServerActor:
     for codes, masks, circuits)."""
 
     def __init__(self, max_db_size, max_batch_size, full_scan_side="left", seeds=None,
-                 return_partial_results=True, full_scan_side_switching_enabled=False):
+                 return_partial_results=True, full_scan_side_switching_enabled=False,
+                 match_distances_buffer_size=0):
         self.max_db_size, self.max_batch_size = max_db_size, max_batch_size
+        # anonymized statistics: capture buffers per orientation, eye and party
+        # (DistanceCache, anon_stats.rs:13-57); 0 = no capture
+        self.match_distances_buffer_size = match_distances_buffer_size
+        self.internal_batch_counter = 0
+        self.anon_buffers = None
+        if match_distances_buffer_size:
+            from .anon_stats import DistanceBuffers
+
+            self.anon_buffers = {o: {e: [DistanceBuffers(match_distances_buffer_size) for _ in range(3)]
+                                     for e in EYES} for o in (NORMAL, MIRROR)}
         self.full_scan_side_switching_enabled = full_scan_side_switching_enabled
         self.return_partial_results = return_partial_results
         Qr = max_batch_size * 31
@@ -250,6 +263,21 @@ class ServerActor:
                     self.eyes[eye][p].code_db.load_records(d, self._dummy[p][0])
                     self.eyes[eye][p].mask_db.load_records(d, self._dummy[p][1])
 
+    def _distances(self, batch, orientation, skip, targets):
+        """One-sided caches per eye and party (load_additions_since,
+        actor.rs:2196-2240,2555-2589) and their two-sided merge
+        (actor.rs:1631-1650)."""
+        from .anon_stats import RECOVERY, REAUTH, UNIQUENESS, TwoSidedDistanceCache, load_additions
+
+        ops = [REAUTH if k == REAUTH_MESSAGE_TYPE else RECOVERY if k == "recovery_check" else UNIQUENESS
+               for k in batch.request_types]
+        qmax = self.max_batch_size * 31
+        one = {e: [load_additions(self.anon_buffers[orientation][e][p], 0, self.match_distances_buffer_size,
+                                  ops, skip, targets, qmax, self.max_db_size) for p in range(3)]
+               for e in EYES}
+        two = [TwoSidedDistanceCache.merge(one["left"][p], one["right"][p]) for p in range(3)]
+        return one, two
+
     def process_job(self, batch):
         """The actor's job loop body (actor.rs:585-640): with mirror-attack
         detection, the Mirror orientation first, then the Normal one with the
@@ -298,9 +326,18 @@ class ServerActor:
 
         queries = self._queries(batch, orientation)
         or_rule = batch.or_rule_indices if batch.or_rule_indices else None
+        self.internal_batch_counter += 1  # actor.rs:860
+        anon = None
+        if self.anon_buffers is not None:
+            # reauth pairs are captured even when not matching (actor.rs:838-858)
+            targets = [i + batch.reauth_target_indices[rid] * self.max_batch_size
+                       for i, (kind, rid) in enumerate(zip(batch.request_types, batch.request_ids))
+                       if kind == REAUTH_MESSAGE_TYPE and rid in batch.reauth_target_indices]
+            anon = {"buffers": self.anon_buffers[orientation], "batch_id": self.internal_batch_counter,
+                    "targets": targets}
         r = self.matcher.match(queries, self.current_db_size, B, or_rule_indices=or_rule,
                                reauth_targets=list(batch.reauth_target_indices.values()),
-                               self_compare=True)
+                               self_compare=True, anon=anon)
         for p in (1, 2):
             if not (np.array_equal(r.final_results[p], r.final_results[0])
                     and np.array_equal(r.match_counters[p], r.match_counters[0])):
@@ -370,11 +407,15 @@ class ServerActor:
         pil, pir = per_eye(partial_ids)
         ril, rir = per_eye(rot_idx) if self.return_partial_results else ([[]] * B, [[]] * B)
         pcl, pcr = per_eye(partial_cnt)
+        # the result's match ids leave out the batch matches (actor.rs:1451-1458,1686)
         res = ServerJobResult(merged, list(batch.request_ids), list(batch.request_types), matches,
-                              mwsp, skip, match_ids, pil, pir, ril, rir, pcl, pcr,
+                              mwsp, skip, filtered, pil, pir, ril, rir, pcl, pcr,
                               list(batch.deletion_requests_indices), matched_batch, reauth_ok,
                               dict(batch.reauth_target_indices), r.candidates)
         res.full_face_mirror_attack_detected = attack
+        if anon is not None:
+            res.one_sided_match_distances, res.two_sided_match_distances = self._distances(
+                batch, orientation, skip, anon["targets"])
         if previous_results is not None:
             res.full_face_mirror_match_ids = previous_results.match_ids
             res.full_face_mirror_partial_match_ids_left = previous_results.partial_match_ids_left

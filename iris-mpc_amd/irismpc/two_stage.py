@@ -91,11 +91,17 @@ class TwoStageMatcher:
         torch.cuda.synchronize()
 
     def match(self, queries, db_size, batch_size, or_rule_indices=None, reauth_targets=(),
-              self_compare=False):
+              self_compare=False, anon=None):
         """queries[eye] = (code query sets, mask query sets), three each.
         or_rule_indices: per batch query a list of DB ids (or None);
         reauth_targets: DB ids; self_compare: also deduplicate within the batch
-        (query sets uploaded from raw shares). Returns a TwoStageResult."""
+        (query sets uploaded from raw shares); anon: distance capture for the
+        anonymized statistics, a dict with "buffers" (eye -> three
+        anon_stats.DistanceBuffers), "batch_id" and "targets" (db * max_batch +
+        query pairs captured even when not matching). The full-scan eye is
+        captured in its 0.375 scan (actor.rs:2476-2530), the other eye in a
+        0.375 pass over the subset (actor.rs:2040-2120). Returns a
+        TwoStageResult."""
         import torch
 
         assert batch_size <= self.max_batch and db_size <= self.max_db_size
@@ -107,11 +113,25 @@ class TwoStageMatcher:
             if self_compare:
                 self._clear(e, batch=True)
 
+        def capture(eye, on):
+            for p in range(3):
+                cmp = self.eyes[eye][p].comparator
+                if on:
+                    anon["buffers"][eye][p].reset()
+                    cmp.set_anon_capture(anon["buffers"][eye][p], anon.get("batch_id", 0),
+                                         targets=anon.get("targets", ()))
+                else:
+                    cmp.set_anon_capture(None)
+
         # phase 1: full scan of one eye at the prefilter threshold
         qc, qm = queries[side1]
+        if anon:
+            capture(side1, True)
         scan_db_trio(self.eyes[side1], qc, qm, db_size, batch_size, THRESHOLD_ANON_STATS_A,
                      [b.data_ptr() for b in self.bitmaps[side1]], self.streams)
         self._sync()
+        if anon:
+            capture(side1, False)
 
         # candidate subset (actor.rs:938-970)
         or_ids = [] if or_rule_indices is None else [d for ids in or_rule_indices for d in ids]
@@ -134,10 +154,15 @@ class TwoStageMatcher:
                 scan_batch_self_trio(self.eyes[eye], qc, qm, batch_size, THRESHOLD_A,
                                      [b.data_ptr() for b in self.batch_bitmaps[eye]], self.streams,
                                      row_bits=self.max_batch)
+            stats = bool(anon) and eye == side2
+            if stats:
+                capture(eye, True)
             scan_subset_trio(self.eyes[eye], self.code_subs, self.mask_subs, qc, qm, ids, n, db_size,
                              batch_size, THRESHOLD_A, [b.data_ptr() for b in self.bitmaps[eye]],
-                             self.streams)
+                             self.streams, anon_threshold=THRESHOLD_ANON_STATS_A if stats else 0)
             self._sync()
+            if stats:
+                capture(eye, False)
             res.rotations[eye] = [self.eyes[eye][p].comparator.get_partial_results_with_rotations(
                 self.streams[p]) for p in range(3)]
 

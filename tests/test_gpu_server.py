@@ -114,7 +114,7 @@ def test_server_two_batches(irismpc):
     assert all(not r1.matched_batch_request_ids[i] for i in (0, 1, 3, 4, 5, 6, 7, 8, 10, 11))
     assert r1.successful_reauths == [i == 4 for i in range(12)]
     assert r1.match_ids[1] == [17] and r1.match_ids[4] == [33] and r1.match_ids[7] == [60]
-    assert r1.match_ids[2] == [NM - 0] and r1.match_ids[9] == [NM - 8]
+    assert r1.match_ids[2] == [] and r1.match_ids[9] == []  # batch matches are reported by request id
     assert r1.partial_match_ids_left[6] == [50] and r1.partial_match_ids_right[6] == []
     assert r1.partial_match_ids_left[7] == [60] and r1.partial_match_counters_left[7] == 1
     assert r1.partial_match_ids_left[10] == [] and r1.partial_match_ids_right[10] == []  # deleted
@@ -178,3 +178,35 @@ def test_server_mirror_attack(irismpc, orc):
     q = build().query()
     r = actor.process_job(q)
     assert r.merged_results[0] == N + 1 and r.full_face_mirror_attack_detected == [False] * 3
+
+
+def test_server_anon_stats(irismpc):
+    """Anonymized-statistics distances of a batch (actor.rs:1631-1650): the
+    two-sided cache holds the (batch, DB row, query) pairs captured in both
+    eyes: 0.375 matches and reauth target pairs, matching or not; a left-only
+    match stays one-sided. All parties hold the same keys."""
+    from irismpc import synth
+    from irismpc.server import ServerActor
+
+    N, MAXDB, B = 300, 1024, 8
+    db = {"left": [_iris(51, i) for i in range(N)], "right": [_iris(52, i) for i in range(N)]}
+    actor = ServerActor(MAXDB, B, match_distances_buffer_size=1 << 14)
+    sh = {}
+    for k, eye in enumerate(("left", "right")):
+        sh[eye] = synth.shares_of(np.array([c for c, _ in db[eye]]), np.array([m for _, m in db[eye]]), 320 + k)
+    actor.load_db(sh["left"], sh["right"])
+    copy = lambda eye, i, key: (_flip(db[eye][i][0], key), db[eye][i][1])  # noqa: E731
+    b = Batch()
+    b.add("m10", copy("left", 10, 1), copy("right", 10, 2))
+    b.add("l20", copy("left", 20, 3), _iris(53, 0))
+    b.add("re30", copy("left", 30, 4), copy("right", 30, 5), kind="reauth", reauth=30)
+    b.add("re40", _iris(53, 1), _iris(54, 1), kind="reauth", reauth=40)
+    r = actor.process_batch_query(b.query())
+    # query 1 matches in one eye only: unique, inserted as row N
+    assert r.merged_results[:3] == [10, N, 30] and r.successful_reauths == [False, False, True, False]
+    key = lambda d, j: actor.internal_batch_counter * MAXDB * B + d * B + j  # noqa: E731
+    for p in range(3):
+        assert set(r.two_sided_match_distances[p].map) == {key(10, 0), key(30, 2), key(40, 3)}
+        assert key(20, 1) in r.one_sided_match_distances["left"][p].map
+        left, right = r.two_sided_match_distances[p].map[key(40, 3)]
+        assert len(left) == len(right) == 31  # every rotation of the reauth pair

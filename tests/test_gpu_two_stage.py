@@ -15,6 +15,9 @@ that is a rotated copy of its DB iris (rotation capture); a reauth target.
 test_two_stage_multi_chunk_subset: 40,960 candidates (more than one subset
 chunk of 32,768) on a 65,536-row DB of 64 distinct irises tiled, answer known
 by construction.
+
+test_anon_distance_capture: the anonymized-statistics distance capture of both
+stages against the plaintext dot products.
 """
 import numpy as np
 import pytest
@@ -210,3 +213,105 @@ def test_two_stage_multi_chunk_subset(irismpc):
         for j in range(40):
             assert sorted(left[j]) == cand[(cand % U) == targets[j]].tolist()
             assert all(r == [0] for r in left[j].values())
+
+
+def _rotation_dots(orc, db_c, db_m, q_c, q_m):
+    """[B][31][N] plaintext mask popcount M and masked Hamming distance H."""
+    N, B = len(db_c), len(q_c)
+    M = np.zeros((B, 31, N), np.int64)
+    H = np.zeros((B, 31, N), np.int64)
+    for j in range(B):
+        rc, rm = orc.iris_all_rotations(q_c[j]), orc.iris_all_rotations(q_m[j])
+        for t in range(31):
+            mm = rm[t][None, :] & db_m
+            M[j, t] = np.bitwise_count(mm).sum(axis=1)
+            H[j, t] = np.bitwise_count((rc[t][None, :] ^ db_c) & mm).sum(axis=1)
+    return M, H
+
+
+def test_anon_distance_capture(irismpc, orc):
+    """Match-distance capture of the anonymized statistics (openResults /
+    storeAnonStatsWithIndexMapping, dot/kernel.cu:54-210): the full-scan eye
+    captures every (DB row, query row) opened at 0.375, the other eye those in
+    the candidate subset (its separate 0.375 pass), both plus every rotation of
+    the reauth target pairs; the captured shares of the three parties add up
+    to the plaintext code dot M - 2H and mask dot M, with replicated b shares,
+    and match ids encode (batch, DB row, query row)."""
+    from irismpc import synth
+    from irismpc.anon_stats import DistanceBuffers
+    from irismpc.two_stage import TwoStageMatcher
+
+    N, B = 1500, 8
+    Qr = B * 31
+    rng = np.random.default_rng(78)
+    t = rng.choice(N, size=4, replace=False)
+    db = {}
+    for e, ds in (("left", 5), ("right", 6)):
+        c, m = zip(*[orc.synth_iris(ds, i) for i in range(N)])
+        db[e] = (np.array(c), np.array(m))
+    q = {"left": ([], []), "right": ([], [])}
+    plan = {0: (1280, 1280), 1: (4608, 1280), 2: (1280, None), 3: (None, 1280)}
+    for j in range(B):
+        for side, e in enumerate(("left", "right")):
+            f = plan.get(j, (None, None))[side]
+            if f is None:
+                c, m = orc.synth_iris(25 + side, j)
+            else:
+                c, m = orc.synth_flip(db[e][0][t[j]], 35 + side, j, f), db[e][1][t[j]]
+            q[e][0].append(c)
+            q[e][1].append(m)
+    q = {e: (np.array(v[0]), np.array(v[1])) for e, v in q.items()}
+    reauth_db, reauth_q = int(t[0]), 5  # query 5 (random) asks for DB row t[0]: captured, no match
+
+    eyes, queries, keep = {}, {}, []
+    for k, e in enumerate(("left", "right")):
+        ex, parties = _engines(irismpc, N, Qr, tag_base=3 * k)
+        keep.append(ex)
+        db_sh = synth.shares_of(db[e][0], db[e][1], 51 + k)
+        q_sh = synth.shares_of(q[e][0], q[e][1], 61 + k)
+        for p in range(3):
+            parties[p].code_db.load_full_db(db_sh[p][0])
+            parties[p].mask_db.load_full_db(db_sh[p][1])
+        eyes[e] = parties
+        queries[e] = ([parties[p].codes.upload_query_shares(q_sh[p][0]) for p in range(3)],
+                      [parties[p].masks.upload_query_shares(q_sh[p][1]) for p in range(3)])
+    bufs = {e: [DistanceBuffers(1 << 14) for _ in range(3)] for e in ("left", "right")}
+    batch_id = 7
+    anon = {"buffers": bufs, "batch_id": batch_id, "targets": [reauth_db * B + reauth_q]}
+    res = TwoStageMatcher(eyes, N, B).match(queries, N, B, reauth_targets=[reauth_db], anon=anon)
+
+    cand = set(res.candidates.tolist())
+    assert reauth_db in cand
+    for e in ("left", "right"):
+        M, H = _rotation_dots(orc, *db[e], *q[e])
+        c = (M - 2 * H) & 0xFFFF
+        r = (M * 16384 - (c << 16) - 1) & 0xFFFFFFFF
+        match375 = (r >> 31).astype(bool)  # [B][31][N]
+        want = set()
+        for j, tt, d in zip(*np.nonzero(match375)):
+            if e == "left" or d in cand:
+                want.add((int(d), int(j * 31 + tt)))
+        want |= {(reauth_db, reauth_q * 31 + tt) for tt in range(31)}
+        got, shares = None, []
+        for p in range(3):
+            n, idx, ca, cb, ma, mb = bufs[e][p].fetch()
+            assert n == len(idx) and n < bufs[e][p].capacity
+            order = np.argsort(idx)
+            shares.append((idx[order], ca[order], cb[order], ma[order], mb[order]))
+            pairs = {(int(i % (N * Qr)) // Qr, int(i % Qr)) for i in idx}
+            assert all(int(i) // (N * Qr) == batch_id for i in idx)
+            assert got is None or pairs == got
+            got = pairs
+        assert got == want, f"{e} captured pairs"
+        assert len(shares[0][0]) == len(want)
+        for p in range(3):
+            assert np.array_equal(shares[p][0], shares[0][0])
+            assert np.array_equal(shares[p][2], shares[(p + 2) % 3][1])  # b = previous party's a
+            assert np.array_equal(shares[p][4], shares[(p + 2) % 3][3])
+        code = (shares[0][1].astype(np.int64) + shares[1][1] + shares[2][1]) & 0xFFFF
+        mask = (shares[0][3].astype(np.int64) + shares[1][3] + shares[2][3]) & 0xFFFF
+        for k, i in enumerate(shares[0][0]):
+            d, row = int(i % (N * Qr)) // Qr, int(i % Qr)
+            j, tt = divmod(row, 31)
+            assert code[k] == (M[j, tt, d] - 2 * H[j, tt, d]) & 0xFFFF
+            assert mask[k] == M[j, tt, d] & 0xFFFF
