@@ -197,8 +197,15 @@ int irismpc_share_db_reshare_results(irismpc_share_db* e, size_t chunk_size, voi
 int irismpc_share_db_reshare_results_trio(irismpc_share_db* const e[3], size_t chunk_size,
                                           void* const streams[3]);
 /* ShareDB::result_chunk_shares (share_db.rs:1050-1069): (a, b) = (results,
- * results_peer) device pointers. */
+ * results_peer) device pointers. On one GPU the reshare hands b over as a
+ * view of the previous party's own result buffer (no copy): it stays valid
+ * until that party's next dot_reduce. Engines that need b to outlive it set
+ * irismpc_share_db_set_peer_copy. */
 int irismpc_share_db_result_views(irismpc_share_db* e, const uint16_t** a, const uint16_t** b);
+/* 1: the reshare copies the previous party's share into this engine's own
+ * buffer (when any of the three engines asks for it); 0 (default): view.
+ * No reference counterpart (the reference's NCCL receive always copies). */
+int irismpc_share_db_set_peer_copy(irismpc_share_db* e, int copy);
 /* ShareDB::fetch_results (share_db.rs:1038-1048). */
 int irismpc_share_db_fetch_results(irismpc_share_db* e, uint16_t* out, size_t n, void* stream);
 int irismpc_share_db_get_counter(const irismpc_share_db* e, uint64_t* counter);
@@ -237,7 +244,8 @@ int irismpc_circuits_compare_threshold_masked_many_trio(
  * shares (a, b), chunk_size words each. */
 int irismpc_circuits_result_buffer(irismpc_circuits* c, const uint64_t** a, const uint64_t** b);
 /* The exchange inside actor.rs:2928-2997 `open`: own b share to next, prev's b
- * into this party's c buffer. Collective. */
+ * into this party's c view (the previous party's buffer itself, valid until
+ * the next compare). Collective. */
 int irismpc_circuits_open(irismpc_circuits* c, void* stream);
 int irismpc_circuits_open_trio(irismpc_circuits* const c[3], void* const streams[3]);
 int irismpc_circuits_open_views(irismpc_circuits* c, const uint64_t** a, const uint64_t** b,

@@ -28,6 +28,9 @@ void scan_chunk(const PartyScan& p, size_t offset, size_t db_size, size_t batch_
   const size_t dot_chunk = round_up(std::max<size_t>(real, 1), pad);
   p.codes->dot_reduce(*p.code_query, *p.code_db, dot_chunk, offset, 1, s);
   p.masks->dot_reduce(*p.mask_query, *p.mask_db, dot_chunk, offset, 2, s);
+  // the distance capture reads the peer shares on this party's stream after
+  // the circuits: they need a copy the peer's next GEMM cannot overwrite
+  p.codes->peer_copy = p.masks->peer_copy = p.comparator->anon_capture_enabled();
   p.codes->reshare_results(dot_chunk, s);
   p.masks->reshare_results(dot_chunk, s);
   const size_t len = dot_chunk * rows;
@@ -62,6 +65,7 @@ void scan_subset(const PartyScan& p, DbSlice& code_sub, DbSlice& mask_sub, const
     p.masks->prefetch_subset(*p.mask_db, ids + c0, real, mask_sub, s);
     p.codes->dot_reduce(*p.code_query, code_sub, dot_chunk, 0, 1, s);
     p.masks->dot_reduce(*p.mask_query, mask_sub, dot_chunk, 0, 2, s);
+    p.codes->peer_copy = p.masks->peer_copy = p.comparator->anon_capture_enabled();
     p.codes->reshare_results(dot_chunk, s);
     p.masks->reshare_results(dot_chunk, s);
     const size_t len = dot_chunk * rows;

@@ -356,6 +356,9 @@ int irismpc_share_db_reshare_results_trio(irismpc_share_db* const e[3], size_t c
     ShareDB::reshare_trio(d, chunk_size, s);
   });
 }
+int irismpc_share_db_set_peer_copy(irismpc_share_db* e, int copy) {
+  return guarded([&] { e->impl->peer_copy = copy != 0; });
+}
 int irismpc_share_db_result_views(irismpc_share_db* e, const uint16_t** a, const uint16_t** b) {
   return guarded([&] {
     *a = e->impl->results();

@@ -494,17 +494,25 @@ struct ShareDBTrio {
       IRIS_REQUIRE(d[p]->last_rows_ == d[0]->last_rows_ && chunk <= d[p]->max_db_length_,
                    "reshare geometry mismatch");
     }
+    const bool copy = d[0]->peer_copy || d[1]->peer_copy || d[2]->peer_copy;
     Event sent[3];
     for (int p = 0; p < 3; p++) IRIS_HIP_CHECK(hipEventRecord(sent[p].e, s[p]));
     for (int p = 0; p < 3; p++) {
       const int q = (p + 1) % 3;
+      // the receiver reads after the sender's GEMM finished
       IRIS_HIP_CHECK(hipStreamWaitEvent(s[q], sent[p].e, 0));
-      IRIS_HIP_CHECK(hipMemcpyAsync(d[q]->results_peer_.ptr, d[p]->results_.ptr, bytes,
-                                    hipMemcpyDeviceToDevice, s[q]));
-      IRIS_HIP_CHECK(hipEventRecord(d[p]->ev_send_done_.e, s[q]));
+      if (copy) {
+        IRIS_HIP_CHECK(hipMemcpyAsync(d[q]->results_peer_.ptr, d[p]->results_.ptr, bytes,
+                                      hipMemcpyDeviceToDevice, s[q]));
+        IRIS_HIP_CHECK(hipEventRecord(d[p]->ev_send_done_.e, s[q]));
+        d[q]->peer_view_ = nullptr;
+      } else {
+        d[q]->peer_view_ = d[p]->results_.ptr;  // the message is the sender's buffer
+      }
     }
     // the sender's stream must not overwrite `results` before its message left
-    for (int p = 0; p < 3; p++) IRIS_HIP_CHECK(hipStreamWaitEvent(s[p], d[p]->ev_send_done_.e, 0));
+    if (copy)
+      for (int p = 0; p < 3; p++) IRIS_HIP_CHECK(hipStreamWaitEvent(s[p], d[p]->ev_send_done_.e, 0));
     for (int p = 0; p < 3; p++)
       if (d[p]->is_remote()) d[p]->counter_ += ceil_div(bytes, 64);  // OTP keystream
   }
@@ -652,17 +660,11 @@ void Circuits::compare_threshold_masked_many(const uint16_t* code_a, const uint1
   });
 }
 
-void Circuits::open_exchange_trio(Circuits* const c[3], hipStream_t const s[3]) {
-  Exchange* ex = c[0]->ex_;
-  DeviceGuard g(ex->device());
-  hipStream_t ps = ex->protocol_stream();
-  ex->join_streams(s);
-  for (int p = 0; p < 3; p++) {
-    const int prev = (p + 2) % 3;
-    IRIS_HIP_CHECK(hipMemcpyAsync(c[p]->open_c_.ptr, c[prev]->res_b_.ptr,
-                                  c[p]->chunk_words_ * 8, hipMemcpyDeviceToDevice, ps));
-  }
-  ex->fork_streams(s);
+void Circuits::open_exchange_trio(Circuits* const c[3], hipStream_t const*) {
+  // each party's stream reads the previous party's b after the circuits that
+  // wrote it (the compare forked the streams after its last launch); the next
+  // compare joins the streams before writing it again
+  for (int p = 0; p < 3; p++) c[p]->open_view_ = c[(p + 2) % 3]->res_b_.ptr;
 }
 
 void Circuits::open_exchange(hipStream_t s) {

@@ -179,7 +179,14 @@ class ShareDB {
   void reshare_results(size_t chunk, hipStream_t s);  // collective
   static void reshare_trio(ShareDB* const d[3], size_t chunk, hipStream_t const s[3]);
   uint16_t* results() const { return results_.ptr; }
-  uint16_t* results_peer() const { return results_peer_.ptr; }
+  // The previous party's result share. The reshare hands it over as a view of
+  // that party's own result buffer (no copy; valid until that party's next
+  // dot_reduce, which the three-party circuit launches order after their
+  // reads), or, with peer_copy set on any of the three engines, as a copy.
+  uint16_t* results_peer() const {
+    return (peer_copy || !peer_view_) ? results_peer_.ptr : peer_view_;
+  }
+  bool peer_copy = false;
   size_t results_capacity() const { return results_.n; }
   int peer_id() const { return peer_id_; }
   int code_length() const { return code_length_; }
@@ -206,6 +213,7 @@ class ShareDB {
   size_t last_rows_;  // query rows of the last dot_reduce (the reshare length)
   Exchange* ex_;
   DevBuf<uint16_t> results_, results_peer_;
+  uint16_t* peer_view_ = nullptr;
   Event ev_send_done_;
   friend struct ShareDBTrio;
 };
@@ -256,7 +264,9 @@ class Circuits {
   static void open_exchange_trio(Circuits* const c[3], hipStream_t const s[3]);
   const uint64_t* result_a() const { return res_a_.ptr; }
   const uint64_t* result_b() const { return res_b_.ptr; }
-  const uint64_t* result_c() const { return open_c_.ptr; }
+  // the previous party's result b, handed over as a view of its buffer (valid
+  // until the next compare, whose launches wait for this party's stream)
+  const uint64_t* result_c() const { return open_view_ ? open_view_ : open_c_.ptr; }
   uint64_t counter() const { return counter_; }
   void set_counter(uint64_t c) { counter_ = c; }
   int peer_id() const { return peer_id_; }
@@ -274,6 +284,7 @@ class Circuits {
   DevBuf<uint64_t> tm_a_, tm_b_, cl_a_, cl_b_, c_a_, c_b_, tmp_a_, tmp_b_, tx_a_, tx_b_, cm_a_,
       cm_b_, res_a_, res_b_, open_c_;
   DevBuf<uint16_t> inbox_, corr_a_, corr_b_;
+  const uint64_t* open_view_ = nullptr;
 };
 
 // DistanceComparator analogue (dot/distance_comparator.rs).

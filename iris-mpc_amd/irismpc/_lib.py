@@ -74,6 +74,7 @@ SIGNATURES = {
     "irismpc_share_db_reshare_results": [vp, sz, vp],
     "irismpc_share_db_reshare_results_trio": [vp, sz, vp],
     "irismpc_share_db_result_views": [vp, vp, vp],
+    "irismpc_share_db_set_peer_copy": [vp, i32],
     "irismpc_share_db_fetch_results": [vp, vp, sz, vp],
     "irismpc_share_db_get_counter": [vp, vp],
     "irismpc_share_db_set_counter": [vp, u64],

@@ -240,6 +240,11 @@ class ShareDB:
         call("irismpc_share_db_reshare_results_trio", ptr_array([e.h for e in engines]), chunk_size,
              ptr_array(list(streams)))
 
+    def set_peer_copy(self, copy=True):
+        """Reshare into a private copy instead of a view of the previous party's
+        result buffer."""
+        call("irismpc_share_db_set_peer_copy", self.h, 1 if copy else 0)
+
     def result_chunk_shares(self):
         a, b = _out(), _out()
         call("irismpc_share_db_result_views", self.h, ctypes.byref(a), ctypes.byref(b))
