@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void csa_kernel(const Trio t, int msb, uint64_
   store8(nb + 8 * g, v);
 }
 
-// ------------------------------------------------------------ ripple rounds
+// ------------------------------------------------------------ ripple helpers
 // a_k = s plane k = transposed plane k+1; b_k = carry plane k ^ x3 plane k
 // (x3 = (0, b) for p0, (a, 0) for p2, 0 for p1).
 __device__ __forceinline__ void load_bk(int p, const uint64_t* Ca, const uint64_t* Cb,
@@ -190,96 +190,87 @@ __device__ __forceinline__ uint64_t and_pre(uint64_t xa, uint64_t xb, uint64_t y
   return (xa & ya) ^ (xb & ya) ^ (xa & yb) ^ r;
 }
 
-__global__ __launch_bounds__(256) void ripple_kernel(const Trio t, int msb, int mode, int k,
-                                                     uint64_t ctr) {
-  const int p = party_of(t);
+// ------------------------------------------------------------ fused ripple
+// Wave p of a block is party p; lane l handles words 8 (64 blockIdx.x + l) ..
+// + 7. Per party: x = running carry (a, b); after round k the party's AND
+// output v goes to LDS slot [k & 1][p], and the party reads its "b" message
+// from slot [(p + 2) % 3] after the barrier. The carry update of the per-round
+// kernels (x ^= previous round's exchanged output) is the same sequence.
+__global__ __launch_bounds__(192) void ripple_fused_kernel(const Trio t, int msb, const RippleCtrs rc) {
+  __shared__ uint64_t vbuf[2][3][64][8];
+  const int p = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   const CircuitBufs& P = t.p[p];
-  const CircuitBufs& N = t.p[(p + 1) % 3];
-  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (8 * g >= t.W) return;
-  const size_t W = t.W, o = 8 * g;
+  const size_t W = t.W, g = (size_t)blockIdx.x * 64 + lane, o = 8 * g;
+  const bool live = o < W;
   const uint64_t* Ta = msb ? P.tx_a : P.tm_a;
   const uint64_t* Tb = msb ? P.tx_b : P.tm_b;
   const uint64_t* Ca = msb ? P.cm_a : P.cl_a;
   const uint64_t* Cb = msb ? P.cm_b : P.cl_b;
-  uint64_t* ca = msb ? P.res_a : P.c_a;
-  uint64_t* cb = msb ? P.res_b : P.c_b;
-  uint64_t* ncb = msb ? N.res_b : N.c_b;
-  uint64_t r[8];
+  const int last = msb ? 30 : 15;  // the final step's plane index
+  uint64_t xa[8] = {}, xb[8] = {}, va[8] = {}, vb[8] = {}, r[8];
 
-  if (mode == (int)RippleMode::First) {
-    uint64_t aa[8], ab[8], ba[8], bb[8], v[8];
-    load8(Ta + W + o, aa);
-    load8(Tb + W + o, ab);
-    load_bk(p, Ca, Cb, Ta, Tb, o, ba, bb);
-    corr_u64x8(P, t.nonce, ctr + g, r);
-#pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = and_pre(aa[i], ab[i], ba[i], bb[i], r[i]);
-    store8(ca + o, v);
-    store8(ncb + o, v);
-    return;
-  }
-
-  // running carry c (a, b), plus the previous round's exchanged AND output
-  uint64_t xa[8], xb[8];
-  load8(ca + o, xa);
-  load8(cb + o, xb);
-  const bool has_tmp = (mode != (int)RippleMode::Round) || k >= 2;
-  if (has_tmp) {
-    const int slot = (k - 1) & 1;
-    uint64_t ta[8], tb[8];
-    load8(P.tmp_a + slot * W + o, ta);
-    load8(P.tmp_b + slot * W + o, tb);
+  for (int k = 0; k < last; k++) {
+    // a_k = T plane k + 1, b_k = C plane k (x3 folded in by load_bk)
+    uint64_t aa[8] = {}, ab[8] = {}, ba[8] = {}, bb[8] = {};
+    if (live) {
+      load8(Ta + (size_t)(k + 1) * W + o, aa);
+      load8(Tb + (size_t)(k + 1) * W + o, ab);
+      load_bk(p, Ca, Cb, Ta, Tb, (size_t)k * W + o, ba, bb);
+      corr_u64x8(P, t.nonce, rc.c[k] + g, r);
+    }
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      xa[i] ^= ta[i];
-      xb[i] ^= tb[i];
+      xa[i] ^= va[i];  // carry after round k - 1 (va = vb = 0 before round 1)
+      xb[i] ^= vb[i];
+      va[i] = live ? and_pre(aa[i] ^ xa[i], ab[i] ^ xb[i], ba[i] ^ xa[i], bb[i] ^ xb[i], r[i]) : 0;
     }
+    if (k == 0) {  // the first round has no carry in
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        xa[i] = 0;
+        xb[i] = 0;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) vbuf[k & 1][p][lane][i] = va[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; i++) vb[i] = vbuf[k & 1][(p + 2) % 3][lane][i];
   }
-
-  if (mode == (int)RippleMode::Round) {
-    uint64_t aa[8], ab[8], ba[8], bb[8], v[8];
-    load8(Ta + (size_t)(k + 1) * W + o, aa);
-    load8(Tb + (size_t)(k + 1) * W + o, ab);
-    load_bk(p, Ca, Cb, Ta, Tb, (size_t)k * W + o, ba, bb);
-    corr_u64x8(P, t.nonce, ctr + g, r);
+  // carry after the last round
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    xa[i] ^= va[i];
+    xb[i] ^= vb[i];
+  }
+  if (!msb) {
+    // carry-out (protocol.rs:2229-2239): c2 = b15 & c (shared AND), c1 = c ^ b15
+    uint64_t ba[8] = {}, bb[8] = {};
+    if (live) {
+      load_bk(p, Ca, Cb, Ta, Tb, 15 * W + o, ba, bb);
+      corr_u64x8(P, t.nonce, rc.c[15] + g, r);
+    }
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      aa[i] ^= xa[i];
-      ab[i] ^= xb[i];
-      ba[i] ^= xa[i];
-      bb[i] ^= xb[i];
-      v[i] = and_pre(aa[i], ab[i], ba[i], bb[i], r[i]);
-    }
-    if (has_tmp) {
-      store8(ca + o, xa);
-      store8(cb + o, xb);
-    }
-    const int slot = k & 1;
-    store8(P.tmp_a + slot * W + o, v);
-    store8(N.tmp_b + slot * W + o, v);
-    return;
-  }
-
-  if (mode == (int)RippleMode::LiftFinal) {
-    // c2 = b15 & c (shared AND), c1 = c ^ b15 (protocol.rs:2229-2239)
-    uint64_t ba[8], bb[8], v[8];
-    load_bk(p, Ca, Cb, Ta, Tb, 15 * W + o, ba, bb);
-    corr_u64x8(P, t.nonce, ctr + g, r);
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      v[i] = and_pre(ba[i], bb[i], xa[i], xb[i], r[i]);
+      va[i] = live ? and_pre(ba[i], bb[i], xa[i], xb[i], r[i]) : 0;
       xa[i] ^= ba[i];
       xb[i] ^= bb[i];
     }
-    store8(ca + o, xa);
-    store8(cb + o, xb);
-    store8(ca + W + o, v);
-    store8(ncb + W + o, v);
+#pragma unroll
+    for (int i = 0; i < 8; i++) vbuf[1][p][lane][i] = va[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; i++) vb[i] = vbuf[1][(p + 2) % 3][lane][i];
+    if (live) {
+      store8(P.c_a + o, xa);
+      store8(P.c_b + o, xb);
+      store8(P.c_a + W + o, va);
+      store8(P.c_b + W + o, vb);
+    }
     return;
   }
-
-  // MsbFinal: msb = c ^ a_30 ^ b_30 (protocol.rs:2347-2352)
+  // MSB (protocol.rs:2347-2352): msb = c ^ a_30 ^ b_30
+  if (!live) return;
   uint64_t aa[8], ab[8], ba[8], bb[8];
   load8(Ta + 31 * W + o, aa);
   load8(Tb + 31 * W + o, ab);
@@ -289,8 +280,8 @@ __global__ __launch_bounds__(256) void ripple_kernel(const Trio t, int msb, int 
     xa[i] ^= aa[i] ^ ba[i];
     xb[i] ^= ab[i] ^ bb[i];
   }
-  store8(ca + o, xa);
-  store8(cb + o, xb);
+  store8(P.res_a + o, xa);
+  store8(P.res_b + o, xb);
 }
 
 // ------------------------------------------------------------ bit injection
@@ -500,10 +491,9 @@ void launch_csa(const Trio& t, int np, bool msb, uint64_t ctr, hipStream_t s) {
   IRIS_HIP_CHECK(hipGetLastError());
 }
 
-void launch_ripple(const Trio& t, int np, bool msb, RippleMode mode, int k, uint64_t ctr,
-                   hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(t.W / 8, 256), (unsigned)np);
-  ripple_kernel<<<grid, 256, 0, s>>>(t, msb ? 1 : 0, (int)mode, k, ctr);
+void launch_ripple_fused(const Trio& t, bool msb, const RippleCtrs& ctrs, hipStream_t s) {
+  IRIS_REQUIRE(t.party0 == 0, "the fused ripple runs all three parties");
+  ripple_fused_kernel<<<(unsigned)ceil_div(t.W, 512), 192, 0, s>>>(t, msb ? 1 : 0, ctrs);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 

@@ -16,8 +16,7 @@ struct CircuitBufs {
   const uint16_t* mask_b;
   uint64_t *tm_a, *tm_b;    // 16 W: transposed mask shares
   uint64_t *cl_a, *cl_b;    // 16 W: lift carry-save carries
-  uint64_t *c_a, *c_b;      // 2 W: ripple carry; after the lift: (bit 16, bit 17)
-  uint64_t *tmp_a, *tmp_b;  // 2 W: ping-pong exchange slots of the ripple rounds
+  uint64_t *c_a, *c_b;      // 2 W: lift carries (bit 16, bit 17)
   uint16_t* inbox;          // 128 W: bit-inject message received from prev
   uint16_t *corr_a, *corr_b;  // 128 W: injected carries (arithmetic u16 shares)
   uint64_t *tx_a, *tx_b;    // 32 W: transposed lifted comparison values
@@ -50,12 +49,18 @@ struct CircuitSchedule {
   static CircuitSchedule make(uint64_t start, size_t W, size_t alloc_words);
 };
 
-enum class RippleMode : int { First = 0, Round = 1, LiftFinal = 2, MsbFinal = 3 };
+// All ripple rounds of one adder in one launch (the lift: first + 14 rounds +
+// carry-out; the MSB: first + 29 rounds + final XOR). A block holds the same
+// 512 words of all three parties, one wave per party: each round's AND output
+// reaches the next party through LDS, and the running carry stays in
+// registers, so a round reads only its two input planes per party.
+struct RippleCtrs {
+  uint64_t c[32];  // ChaCha block counter of round k (k = 0: first, last: carry-out)
+};
 
 void launch_transpose16(const Trio& t, int n_parties, hipStream_t s);
 void launch_csa(const Trio& t, int n_parties, bool msb, uint64_t ctr, hipStream_t s);
-void launch_ripple(const Trio& t, int n_parties, bool msb, RippleMode mode, int k, uint64_t ctr,
-                   hipStream_t s);
+void launch_ripple_fused(const Trio& t, bool msb, const RippleCtrs& ctrs, hipStream_t s);
 void launch_inject(const Trio& t, int n_parties, int step, uint64_t ctr, hipStream_t s);
 void launch_lms_transpose32(const Trio& t, int n_parties, uint32_t threshold_a, hipStream_t s);
 

@@ -560,7 +560,6 @@ Circuits::Circuits(int device, int peer_id, size_t input_size, size_t alloc_size
   tm_a_.alloc(16 * A); tm_b_.alloc(16 * A);
   cl_a_.alloc(16 * A); cl_b_.alloc(16 * A);
   c_a_.alloc(2 * A); c_b_.alloc(2 * A);
-  tmp_a_.alloc(2 * A); tmp_b_.alloc(2 * A);
   tx_a_.alloc(32 * A); tx_b_.alloc(32 * A);
   cm_a_.alloc(31 * A); cm_b_.alloc(31 * A);
   res_a_.alloc(A); res_b_.alloc(A); open_c_.alloc(A);
@@ -578,7 +577,7 @@ CircuitBufs Circuits::bufs(const uint16_t* ca, const uint16_t* cb, const uint16_
   CircuitBufs b{};
   b.code_a = ca; b.code_b = cb; b.mask_a = ma; b.mask_b = mb;
   b.tm_a = tm_a_.ptr; b.tm_b = tm_b_.ptr; b.cl_a = cl_a_.ptr; b.cl_b = cl_b_.ptr;
-  b.c_a = c_a_.ptr; b.c_b = c_b_.ptr; b.tmp_a = tmp_a_.ptr; b.tmp_b = tmp_b_.ptr;
+  b.c_a = c_a_.ptr; b.c_b = c_b_.ptr;
   b.inbox = inbox_.ptr; b.corr_a = corr_a_.ptr; b.corr_b = corr_b_.ptr;
   b.tx_a = tx_a_.ptr; b.tx_b = tx_b_.ptr; b.cm_a = cm_a_.ptr; b.cm_b = cm_b_.ptr;
   b.res_a = res_a_.ptr; b.res_b = res_b_.ptr;
@@ -614,15 +613,18 @@ void Circuits::compare_trio(Circuits* const c[3], const uint16_t* const code_a[3
   ex->join_streams(s);
   launch_transpose16(t, 3, ps);
   launch_csa(t, 3, false, sc.lift_csa, ps);
-  launch_ripple(t, 3, false, RippleMode::First, 0, sc.lift_first, ps);
-  for (int k = 1; k <= 14; k++) launch_ripple(t, 3, false, RippleMode::Round, k, sc.lift_round[k], ps);
-  launch_ripple(t, 3, false, RippleMode::LiftFinal, 15, sc.lift_final, ps);
+  RippleCtrs lift{};
+  lift.c[0] = sc.lift_first;
+  for (int k = 1; k <= 14; k++) lift.c[k] = sc.lift_round[k];
+  lift.c[15] = sc.lift_final;
+  launch_ripple_fused(t, false, lift, ps);
   for (int step = 1; step <= 3; step++) launch_inject(t, 3, step, sc.inject, ps);
   launch_lms_transpose32(t, 3, A, ps);
   launch_csa(t, 3, true, sc.msb_csa, ps);
-  launch_ripple(t, 3, true, RippleMode::First, 0, sc.msb_first, ps);
-  for (int k = 1; k <= 29; k++) launch_ripple(t, 3, true, RippleMode::Round, k, sc.msb_round[k], ps);
-  launch_ripple(t, 3, true, RippleMode::MsbFinal, 30, 0, ps);
+  RippleCtrs msb{};
+  msb.c[0] = sc.msb_first;
+  for (int k = 1; k <= 29; k++) msb.c[k] = sc.msb_round[k];
+  launch_ripple_fused(t, true, msb, ps);
   ex->fork_streams(s);
   for (int p = 0; p < 3; p++) c[p]->counter_ = sc.end;
 }
