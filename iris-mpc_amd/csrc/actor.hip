@@ -105,8 +105,64 @@ void scan_batch_self(const PartyScan& p, size_t batch_size, uint32_t threshold_a
              row_bits ? row_bits : batch_size, s);
 }
 
+// The chunk loop with two result buffers, as the reference alternates two
+// streams (actor.rs:2318-2533): chunk c's GEMMs run on the party stream `s`
+// into result slot c % 2 while chunk c - 1's circuits and open run on the
+// exchange's protocol stream. A slot is reused once the open of the chunk
+// that last read it has run (event on the protocol stream); the circuits of a
+// chunk wait for the three parties' GEMMs (the compare joins the streams).
+static void scan_db_pipelined(const PartyScan& p, size_t db_size, size_t batch_size,
+                              uint32_t threshold_a, uint64_t* bitmap, hipStream_t s) {
+  const size_t rows = (size_t)p.code_query->rows;
+  IRIS_REQUIRE((size_t)p.mask_query->rows == rows, "code/mask query sets disagree on rows");
+  IRIS_REQUIRE(batch_size * kRotations <= rows, "batch larger than the query rows");
+  hipStream_t ps = p.circuits->exchange()->protocol_stream();
+  Event freed[2];
+  bool used[2] = {false, false};
+  const size_t pad = rows % 8 == 0 ? 64 : 512;
+  size_t offset = 0;
+  int c = 0;
+  p.codes->peer_copy = p.masks->peer_copy = false;
+  do {
+    const int slot = c & 1;
+    const size_t real = db_size > offset ? std::min<size_t>(kDbChunk, db_size - offset) : 0;
+    const size_t dot_chunk = round_up(std::max<size_t>(real, 1), pad);
+    if (used[slot]) IRIS_HIP_CHECK(hipStreamWaitEvent(s, freed[slot].e, 0));
+    p.codes->set_slot(slot);
+    p.masks->set_slot(slot);
+    p.codes->dot_reduce(*p.code_query, *p.code_db, dot_chunk, offset, 1, s);
+    p.masks->dot_reduce(*p.mask_query, *p.mask_db, dot_chunk, offset, 2, s);
+    p.codes->reshare_results(dot_chunk, s);
+    p.masks->reshare_results(dot_chunk, s);
+    const size_t len = dot_chunk * rows;
+    p.circuits->set_chunk_size(len / 64);
+    p.circuits->compare_threshold_masked_many(p.codes->results(), p.codes->results_peer(),
+                                              p.masks->results(), p.masks->results_peer(), len,
+                                              threshold_a, s, /*fork=*/false);
+    p.circuits->open_exchange(s);
+    if (real > 0)
+      p.comparator->open_results(p.circuits->result_a(), p.circuits->result_b(),
+                                 p.circuits->result_c(), bitmap, dot_chunk, real, offset, db_size,
+                                 batch_size, ps, rows);
+    IRIS_HIP_CHECK(hipEventRecord(freed[slot].e, ps));
+    used[slot] = true;
+    offset += kDbChunk;
+    c++;
+  } while (offset < db_size);
+  // the caller's stream covers the whole scan
+  Event done;
+  IRIS_HIP_CHECK(hipEventRecord(done.e, ps));
+  IRIS_HIP_CHECK(hipStreamWaitEvent(s, done.e, 0));
+  p.codes->set_slot(0);
+  p.masks->set_slot(0);
+}
+
 void scan_db(const PartyScan& p, size_t db_size, size_t batch_size, uint32_t threshold_a,
              uint64_t* bitmap, hipStream_t s) {
+  if (!p.comparator->anon_capture_enabled()) {
+    scan_db_pipelined(p, db_size, batch_size, threshold_a, bitmap, s);
+    return;
+  }
   size_t offset = 0;
   do {
     scan_chunk(p, offset, db_size, batch_size, threshold_a, bitmap, nullptr, 0, s);

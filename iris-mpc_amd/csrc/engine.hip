@@ -115,6 +115,15 @@ ShareDB::ShareDB(int device, int peer_id, size_t max_db_length, size_t query_len
 
 ShareDB::~ShareDB() = default;
 
+void ShareDB::set_slot(int slot) {
+  IRIS_REQUIRE(slot == 0 || slot == 1, "result slot must be 0 or 1");
+  if (slot == 1 && !results_alt_.ptr) {
+    DeviceGuard g(device_);
+    results_alt_.alloc(results_.n);
+  }
+  slot_ = slot;
+}
+
 std::unique_ptr<DbSlice> ShareDB::alloc_db(size_t max_db_length) const {
   DeviceGuard g(device_);
   auto db = std::make_unique<DbSlice>();
@@ -381,7 +390,7 @@ void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, siz
   const int variant = resolve_variant(q);
   GemmArgs a = gemm_args(q, db, variant);
   const GemmTile t = gemm_tile(variant);
-  a.out = results_.ptr;
+  a.out = results();
   a.out_peer = nullptr;
   a.out_chunk_stride = 0;
   a.n_dtiles = (int)ceil_div(chunk, (size_t)t.bd);
@@ -502,12 +511,12 @@ struct ShareDBTrio {
       // the receiver reads after the sender's GEMM finished
       IRIS_HIP_CHECK(hipStreamWaitEvent(s[q], sent[p].e, 0));
       if (copy) {
-        IRIS_HIP_CHECK(hipMemcpyAsync(d[q]->results_peer_.ptr, d[p]->results_.ptr, bytes,
+        IRIS_HIP_CHECK(hipMemcpyAsync(d[q]->results_peer_.ptr, d[p]->results(), bytes,
                                       hipMemcpyDeviceToDevice, s[q]));
         IRIS_HIP_CHECK(hipEventRecord(d[p]->ev_send_done_.e, s[q]));
         d[q]->peer_view_ = nullptr;
       } else {
-        d[q]->peer_view_ = d[p]->results_.ptr;  // the message is the sender's buffer
+        d[q]->peer_view_ = d[p]->results();  // the message is the sender's buffer
       }
     }
     // the sender's stream must not overwrite `results` before its message left
@@ -589,7 +598,7 @@ CircuitBufs Circuits::bufs(const uint16_t* ca, const uint16_t* cb, const uint16_
 void Circuits::compare_trio(Circuits* const c[3], const uint16_t* const code_a[3],
                             const uint16_t* const code_b[3], const uint16_t* const mask_a[3],
                             const uint16_t* const mask_b[3], size_t len, uint32_t A,
-                            hipStream_t const s[3]) {
+                            hipStream_t const s[3], bool fork) {
   IRIS_REQUIRE(len % 64 == 0, "input length must be a multiple of 64");
   const size_t W = len / 64;
   IRIS_REQUIRE(W % 8 == 0, "elements per call must be a multiple of 512 "
@@ -625,7 +634,7 @@ void Circuits::compare_trio(Circuits* const c[3], const uint16_t* const code_a[3
   msb.c[0] = sc.msb_first;
   for (int k = 1; k <= 29; k++) msb.c[k] = sc.msb_round[k];
   launch_ripple_fused(t, true, msb, ps);
-  ex->fork_streams(s);
+  if (fork) ex->fork_streams(s);
   for (int p = 0; p < 3; p++) c[p]->counter_ = sc.end;
 }
 
@@ -636,6 +645,7 @@ struct CompareArg {
   size_t len;
   uint32_t A;
   hipStream_t s;
+  bool fork;
 };
 struct OpenArg {
   Circuits* self;
@@ -645,8 +655,8 @@ struct OpenArg {
 
 void Circuits::compare_threshold_masked_many(const uint16_t* code_a, const uint16_t* code_b,
                                              const uint16_t* mask_a, const uint16_t* mask_b,
-                                             size_t len, uint32_t A, hipStream_t s) {
-  CompareArg arg{this, code_a, code_b, mask_a, mask_b, len, A, s};
+                                             size_t len, uint32_t A, hipStream_t s, bool fork) {
+  CompareArg arg{this, code_a, code_b, mask_a, mask_b, len, A, s, fork};
   ex_->collective(peer_id_, op_id("compare_threshold", tag_), &arg, [](void* const* args) {
     Circuits* c[3];
     const uint16_t *ca[3], *cb[3], *ma[3], *mb[3];
@@ -654,11 +664,12 @@ void Circuits::compare_threshold_masked_many(const uint16_t* code_a, const uint1
     auto* a0 = static_cast<CompareArg*>(args[0]);
     for (int i = 0; i < 3; i++) {
       auto* a = static_cast<CompareArg*>(args[i]);
-      IRIS_REQUIRE(a->len == a0->len && a->A == a0->A, "parties disagree on length/threshold");
+      IRIS_REQUIRE(a->len == a0->len && a->A == a0->A && a->fork == a0->fork,
+                   "parties disagree on length/threshold/fork");
       const int p = a->self->peer_id();
       c[p] = a->self; ca[p] = a->ca; cb[p] = a->cb; ma[p] = a->ma; mb[p] = a->mb; st[p] = a->s;
     }
-    compare_trio(c, ca, cb, ma, mb, a0->len, a0->A, st);
+    compare_trio(c, ca, cb, ma, mb, a0->len, a0->A, st, a0->fork);
   });
 }
 

@@ -178,7 +178,11 @@ class ShareDB {
                          hipStream_t s);
   void reshare_results(size_t chunk, hipStream_t s);  // collective
   static void reshare_trio(ShareDB* const d[3], size_t chunk, hipStream_t const s[3]);
-  uint16_t* results() const { return results_.ptr; }
+  uint16_t* results() const { return slot_ ? results_alt_.ptr : results_.ptr; }
+  // Result buffer of the following dot_reduce calls: 0 or 1 (the second is
+  // allocated on first use), so a chunk's GEMM can run while the previous
+  // chunk's circuits still read the other one.
+  void set_slot(int slot);
   // The previous party's result share. The reshare hands it over as a view of
   // that party's own result buffer (no copy; valid until that party's next
   // dot_reduce, which the three-party circuit launches order after their
@@ -212,8 +216,9 @@ class ShareDB {
   uint64_t nonce_ = 0, counter_ = 0;
   size_t last_rows_;  // query rows of the last dot_reduce (the reshare length)
   Exchange* ex_;
-  DevBuf<uint16_t> results_, results_peer_;
+  DevBuf<uint16_t> results_, results_peer_, results_alt_;
   uint16_t* peer_view_ = nullptr;
+  int slot_ = 0;
   Event ev_send_done_;
   friend struct ShareDBTrio;
 };
@@ -251,14 +256,17 @@ class Circuits {
   void set_chunk_size(size_t words);
   size_t chunk_size() const { return chunk_words_; }
   // compare_threshold_masked_many: collective over the three parties.
+  // fork = false: the party streams are not made to wait for the circuits
+  // (work that reads the results goes on exchange()->protocol_stream())
   void compare_threshold_masked_many(const uint16_t* code_a, const uint16_t* code_b,
                                      const uint16_t* mask_a, const uint16_t* mask_b, size_t len,
-                                     uint32_t threshold_a, hipStream_t s);
+                                     uint32_t threshold_a, hipStream_t s, bool fork = true);
   // Runs the three parties' compare in one call (single host thread).
   static void compare_trio(Circuits* const c[3], const uint16_t* const code_a[3],
                            const uint16_t* const code_b[3], const uint16_t* const mask_a[3],
                            const uint16_t* const mask_b[3], size_t len, uint32_t threshold_a,
-                           hipStream_t const s[3]);
+                           hipStream_t const s[3], bool fork = true);
+  Exchange* exchange() const { return ex_; }
   // open exchange (actor.rs:2928-2997): own result b -> next, prev's b -> c.
   void open_exchange(hipStream_t s);  // collective
   static void open_exchange_trio(Circuits* const c[3], hipStream_t const s[3]);
