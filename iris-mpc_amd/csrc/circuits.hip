@@ -142,87 +142,73 @@ __device__ __forceinline__ uint64_t csa_and(int p, uint64_t a, uint64_t b) {
   return p == 2 ? (a & ~b) : (a & b);
 }
 
-// Thread per 8 consecutive words of the plane-major packed array.
-__global__ __launch_bounds__(256) void csa_kernel(const Trio t, int msb, uint64_t ctr) {
-  const int p = party_of(t);
-  const CircuitBufs& P = t.p[p];
-  const CircuitBufs& N = t.p[(p + 1) % 3];
-  const size_t n = (msb ? 31 : 16) * t.W;
-  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (8 * g >= n) return;
-  const uint64_t* Ta = msb ? P.tx_a : P.tm_a;
-  const uint64_t* Tb = msb ? P.tx_b : P.tm_b;
-  uint64_t* out = msb ? P.cm_a : P.cl_a;
-  uint64_t* nb = msb ? N.cm_b : N.cl_b;
-  uint64_t a[8], b[8], r[8], v[8];
-  load8(Ta + 8 * g, a);
-  load8(Tb + 8 * g, b);
-  corr_u64x8(P, t.nonce, ctr + g, r);
-#pragma unroll
-  for (int i = 0; i < 8; i++) v[i] = csa_and(p, a[i], b[i]) ^ r[i];
-  store8(out + 8 * g, v);
-  store8(nb + 8 * g, v);
-}
-
 // ------------------------------------------------------------ ripple helpers
-// a_k = s plane k = transposed plane k+1; b_k = carry plane k ^ x3 plane k
+// a_k = s plane k = transposed plane k+1; b_k = carry-save plane k ^ x3 plane k
 // (x3 = (0, b) for p0, (a, 0) for p2, 0 for p1).
-__device__ __forceinline__ void load_bk(int p, const uint64_t* Ca, const uint64_t* Cb,
-                                        const uint64_t* Ta, const uint64_t* Tb, size_t off,
-                                        uint64_t ba[8], uint64_t bb[8]) {
-  load8(Ca + off, ba);
-  load8(Cb + off, bb);
-  if (p == 0) {
-    uint64_t x[8];
-    load8(Tb + off, x);
-#pragma unroll
-    for (int i = 0; i < 8; i++) bb[i] ^= x[i];
-  } else if (p == 2) {
-    uint64_t x[8];
-    load8(Ta + off, x);
-#pragma unroll
-    for (int i = 0; i < 8; i++) ba[i] ^= x[i];
-  }
-}
-
 __device__ __forceinline__ uint64_t and_pre(uint64_t xa, uint64_t xb, uint64_t ya, uint64_t yb,
                                             uint64_t r) {
   return (xa & ya) ^ (xb & ya) ^ (xa & yb) ^ r;
 }
 
-// ------------------------------------------------------------ fused ripple
-// Wave p of a block is party p; lane l handles words 8 (64 blockIdx.x + l) ..
-// + 7. Per party: x = running carry (a, b); after round k the party's AND
-// output v goes to LDS slot [k & 1][p], and the party reads its "b" message
-// from slot [(p + 2) % 3] after the barrier. The carry update of the per-round
-// kernels (x ^= previous round's exchanged output) is the same sequence.
-__global__ __launch_bounds__(192) void ripple_fused_kernel(const Trio t, int msb, const RippleCtrs rc) {
-  __shared__ uint64_t vbuf[2][3][64][8];
-  const int p = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+// ------------------------------------------------------------ fused adder
+// The carry-save stage and every ripple round of one adder (the lift: 16
+// planes, first + 14 rounds + carry-out; the MSB: 31 planes, first + 29
+// rounds + final XOR) in one launch. Wave p of a block is party p; lane l
+// handles words 8 (64 blockIdx.x + l) .. + 7. Each round loads one
+// transposed plane pair, T_{k+1}: it is a_k of this round and the input of
+// the carry-save AND of plane k + 1, whose output b_{k+1} the next round uses.
+// Both AND outputs of a round reach the next party through one LDS exchange;
+// carry-save outputs and the running carry never leave the block.
+__device__ __forceinline__ void csa8(int p, const uint64_t a[8], const uint64_t b[8],
+                                     const uint64_t r[8], uint64_t v[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) v[i] = csa_and(p, a[i], b[i]) ^ r[i];
+}
+
+__global__ __launch_bounds__(192) void adder_fused_kernel(const Trio t, int msb, const RippleCtrs rc) {
+  __shared__ uint64_t xbuf[2][3][64][16];  // per round slot: ripple AND (8), carry-save AND (8)
+  const int p = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63), prev = (p + 2) % 3;
   const CircuitBufs& P = t.p[p];
-  const size_t W = t.W, g = (size_t)blockIdx.x * 64 + lane, o = 8 * g;
+  const size_t W = t.W, g = (size_t)blockIdx.x * 64 + lane, o = 8 * g, G = W / 8;
   const bool live = o < W;
   const uint64_t* Ta = msb ? P.tx_a : P.tm_a;
   const uint64_t* Tb = msb ? P.tx_b : P.tm_b;
-  const uint64_t* Ca = msb ? P.cm_a : P.cl_a;
-  const uint64_t* Cb = msb ? P.cm_b : P.cl_b;
-  const int last = msb ? 30 : 15;  // the final step's plane index
-  uint64_t xa[8] = {}, xb[8] = {}, va[8] = {}, vb[8] = {}, r[8];
+  const int last = msb ? 30 : 15;  // ripple rounds 0 .. last - 1, carry planes 0 .. last
+  uint64_t ta[8] = {}, tb[8] = {}, ca[8] = {}, cb[8] = {}, xa[8] = {}, xb[8] = {}, va[8] = {},
+           vb[8] = {}, r[8];
+
+  // carry-save AND of plane 0
+  if (live) {
+    load8(Ta + o, ta);
+    load8(Tb + o, tb);
+    corr_u64x8(P, t.nonce, rc.csa + g, r);
+    csa8(p, ta, tb, r, ca);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) xbuf[1][p][lane][8 + i] = ca[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 8; i++) cb[i] = xbuf[1][prev][lane][8 + i];
 
   for (int k = 0; k < last; k++) {
-    // a_k = T plane k + 1, b_k = C plane k (x3 folded in by load_bk)
-    uint64_t aa[8] = {}, ab[8] = {}, ba[8] = {}, bb[8] = {};
+    uint64_t na[8] = {}, nb[8] = {}, cs[8] = {};
+    // b_k = carry-save plane k ^ x3 (x3 = (0, Tb_k) for p0, (Ta_k, 0) for p2)
+    uint64_t ba[8], bb[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      ba[i] = ca[i] ^ (p == 2 ? ta[i] : 0);
+      bb[i] = cb[i] ^ (p == 0 ? tb[i] : 0);
+    }
     if (live) {
-      load8(Ta + (size_t)(k + 1) * W + o, aa);
-      load8(Tb + (size_t)(k + 1) * W + o, ab);
-      load_bk(p, Ca, Cb, Ta, Tb, (size_t)k * W + o, ba, bb);
+      load8(Ta + (size_t)(k + 1) * W + o, na);  // a_k = T plane k + 1
+      load8(Tb + (size_t)(k + 1) * W + o, nb);
       corr_u64x8(P, t.nonce, rc.c[k] + g, r);
     }
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       xa[i] ^= va[i];  // carry after round k - 1 (va = vb = 0 before round 1)
       xb[i] ^= vb[i];
-      va[i] = live ? and_pre(aa[i] ^ xa[i], ab[i] ^ xb[i], ba[i] ^ xa[i], bb[i] ^ xb[i], r[i]) : 0;
+      va[i] = live ? and_pre(na[i] ^ xa[i], nb[i] ^ xb[i], ba[i] ^ xa[i], bb[i] ^ xb[i], r[i]) : 0;
     }
     if (k == 0) {  // the first round has no carry in
 #pragma unroll
@@ -231,11 +217,24 @@ __global__ __launch_bounds__(192) void ripple_fused_kernel(const Trio t, int msb
         xb[i] = 0;
       }
     }
+    if (live) {  // carry-save AND of plane k + 1 (planes 1 .. last)
+      corr_u64x8(P, t.nonce, rc.csa + (size_t)(k + 1) * G + g, r);
+      csa8(p, na, nb, r, cs);
+    }
 #pragma unroll
-    for (int i = 0; i < 8; i++) vbuf[k & 1][p][lane][i] = va[i];
+    for (int i = 0; i < 8; i++) {
+      xbuf[k & 1][p][lane][i] = va[i];
+      xbuf[k & 1][p][lane][8 + i] = cs[i];
+    }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 8; i++) vb[i] = vbuf[k & 1][(p + 2) % 3][lane][i];
+    for (int i = 0; i < 8; i++) {
+      vb[i] = xbuf[k & 1][prev][lane][i];
+      cb[i] = xbuf[k & 1][prev][lane][8 + i];
+      ca[i] = cs[i];
+      ta[i] = na[i];
+      tb[i] = nb[i];
+    }
   }
   // carry after the last round
 #pragma unroll
@@ -243,13 +242,16 @@ __global__ __launch_bounds__(192) void ripple_fused_kernel(const Trio t, int msb
     xa[i] ^= va[i];
     xb[i] ^= vb[i];
   }
+  // b_last = carry-save plane last ^ x3
+  uint64_t ba[8], bb[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    ba[i] = ca[i] ^ (p == 2 ? ta[i] : 0);
+    bb[i] = cb[i] ^ (p == 0 ? tb[i] : 0);
+  }
   if (!msb) {
     // carry-out (protocol.rs:2229-2239): c2 = b15 & c (shared AND), c1 = c ^ b15
-    uint64_t ba[8] = {}, bb[8] = {};
-    if (live) {
-      load_bk(p, Ca, Cb, Ta, Tb, 15 * W + o, ba, bb);
-      corr_u64x8(P, t.nonce, rc.c[15] + g, r);
-    }
+    if (live) corr_u64x8(P, t.nonce, rc.c[15] + g, r);
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       va[i] = live ? and_pre(ba[i], bb[i], xa[i], xb[i], r[i]) : 0;
@@ -257,10 +259,10 @@ __global__ __launch_bounds__(192) void ripple_fused_kernel(const Trio t, int msb
       xb[i] ^= bb[i];
     }
 #pragma unroll
-    for (int i = 0; i < 8; i++) vbuf[1][p][lane][i] = va[i];
+    for (int i = 0; i < 8; i++) xbuf[(last - 1 + 1) & 1][p][lane][i] = va[i];
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 8; i++) vb[i] = vbuf[1][(p + 2) % 3][lane][i];
+    for (int i = 0; i < 8; i++) vb[i] = xbuf[(last - 1 + 1) & 1][prev][lane][i];
     if (live) {
       store8(P.c_a + o, xa);
       store8(P.c_b + o, xb);
@@ -269,12 +271,11 @@ __global__ __launch_bounds__(192) void ripple_fused_kernel(const Trio t, int msb
     }
     return;
   }
-  // MSB (protocol.rs:2347-2352): msb = c ^ a_30 ^ b_30
+  // MSB (protocol.rs:2347-2352): msb = c ^ a_30 ^ b_30, a_30 = T plane 31
   if (!live) return;
-  uint64_t aa[8], ab[8], ba[8], bb[8];
+  uint64_t aa[8], ab[8];
   load8(Ta + 31 * W + o, aa);
   load8(Tb + 31 * W + o, ab);
-  load_bk(p, Ca, Cb, Ta, Tb, 30 * W + o, ba, bb);
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     xa[i] ^= aa[i] ^ ba[i];
@@ -484,16 +485,10 @@ void launch_transpose16(const Trio& t, int np, hipStream_t s) {
   IRIS_HIP_CHECK(hipGetLastError());
 }
 
-void launch_csa(const Trio& t, int np, bool msb, uint64_t ctr, hipStream_t s) {
-  const size_t groups = (msb ? 31 : 16) * t.W / 8;
-  dim3 grid((unsigned)ceil_div(groups, 256), (unsigned)np);
-  csa_kernel<<<grid, 256, 0, s>>>(t, msb ? 1 : 0, ctr);
-  IRIS_HIP_CHECK(hipGetLastError());
-}
-
-void launch_ripple_fused(const Trio& t, bool msb, const RippleCtrs& ctrs, hipStream_t s) {
-  IRIS_REQUIRE(t.party0 == 0, "the fused ripple runs all three parties");
-  ripple_fused_kernel<<<(unsigned)ceil_div(t.W, 512), 192, 0, s>>>(t, msb ? 1 : 0, ctrs);
+void launch_adder_fused(const Trio& t, bool msb, const RippleCtrs& ctrs, hipStream_t s) {
+  IRIS_REQUIRE(t.party0 == 0, "the fused adder runs all three parties");
+  IRIS_REQUIRE(t.W % 8 == 0, "bit planes must be a multiple of 8 words");
+  adder_fused_kernel<<<(unsigned)ceil_div(t.W, 512), 192, 0, s>>>(t, msb ? 1 : 0, ctrs);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 

@@ -28,7 +28,14 @@ struct DevBuf {
     n = count;
     if (count) {
       IRIS_HIP_CHECK(hipMalloc(&ptr, count * sizeof(T) + 256));
-      if (zero) IRIS_HIP_CHECK(hipMemset(ptr, 0, count * sizeof(T) + 256));
+      if (zero) {
+        // hipMemset runs on the null stream, which does not order with the
+        // engines' non-blocking streams: finish it before any of them can
+        // write the buffer (a lazily allocated result slot is first written
+        // by a GEMM on such a stream right after this call)
+        IRIS_HIP_CHECK(hipMemset(ptr, 0, count * sizeof(T) + 256));
+        IRIS_HIP_CHECK(hipStreamSynchronize(nullptr));
+      }
     }
   }
   void release() {
@@ -289,8 +296,7 @@ class Circuits {
   ChaChaKey seed_own_, seed_prev_;
   uint64_t nonce_ = 0, counter_ = 0;
   Exchange* ex_;
-  DevBuf<uint64_t> tm_a_, tm_b_, cl_a_, cl_b_, c_a_, c_b_, tx_a_, tx_b_, cm_a_,
-      cm_b_, res_a_, res_b_, open_c_;
+  DevBuf<uint64_t> tm_a_, tm_b_, c_a_, c_b_, tx_a_, tx_b_, res_a_, res_b_, open_c_;
   DevBuf<uint16_t> inbox_, corr_a_, corr_b_;
   const uint64_t* open_view_ = nullptr;
 };
