@@ -58,81 +58,6 @@ __device__ __forceinline__ void corr_u64x8(const CircuitBufs& P, uint64_t nonce,
     r[i] = (uint64_t)(k1[2 * i] ^ k2[2 * i]) | ((uint64_t)(k1[2 * i + 1] ^ k2[2 * i + 1]) << 32);
 }
 
-// 8 consecutive words; every call site addresses a multiple of 8 words from a
-// 256-byte aligned allocation (W is a multiple of 8), so 64-byte alignment
-// holds and the accesses vectorise.
-__device__ __forceinline__ void load8(const uint64_t* p, uint64_t v[8]) {
-  const uint64_t* q = static_cast<const uint64_t*>(__builtin_assume_aligned(p, 64));
-#pragma unroll
-  for (int i = 0; i < 8; i++) v[i] = q[i];
-}
-
-__device__ __forceinline__ void store8(uint64_t* p, const uint64_t v[8]) {
-  uint64_t* q = static_cast<uint64_t*>(__builtin_assume_aligned(p, 64));
-#pragma unroll
-  for (int i = 0; i < 8; i++) q[i] = v[i];
-}
-
-// ------------------------------------------------------------ transpose
-// Plane j, word i, bit k = bit j of element 64 i + k (kernel.cu:63-84).
-// 64 x 32 bit-matrix transpose across a wave: lane l holds x_l; lanes j and
-// 32 + j receive the 64-bit word whose bit l is bit j of x_l. Five butterfly
-// stages inside each 32-lane half swap the off-diagonal s x s blocks with the
-// partner lane l ^ s (ds_swizzle): a per-lane rotate of the partner's value
-// and one bitfield insert. The halves are then joined across lane ^ 32.
-template <int S, uint32_t LOW>
-__device__ __forceinline__ uint32_t transpose_stage(uint32_t x, int lane) {
-  const uint32_t y = (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (S << 10));
-  const bool hi = (lane & S) != 0;
-  const uint32_t keep = hi ? ~LOW : LOW;
-  const uint32_t rot = __builtin_rotateright32(y, hi ? S : 32 - S);
-  return (x & keep) | (rot & ~keep);
-}
-
-__device__ __forceinline__ uint64_t wave_transpose32(uint32_t x, int lane) {
-  x = transpose_stage<16, 0x0000FFFFu>(x, lane);
-  x = transpose_stage<8, 0x00FF00FFu>(x, lane);
-  x = transpose_stage<4, 0x0F0F0F0Fu>(x, lane);
-  x = transpose_stage<2, 0x33333333u>(x, lane);
-  x = transpose_stage<1, 0x55555555u>(x, lane);
-  const uint32_t other = (uint32_t)__shfl_xor((int)x, 32);
-  return lane < 32 ? ((uint64_t)x | ((uint64_t)other << 32)) : ((uint64_t)other | ((uint64_t)x << 32));
-}
-
-// Block per 64 consecutive words: each wave turns 16 words into bit planes
-// (a in bits 0..15 and b in bits 16..31 of one transpose), the planes go
-// through LDS and leave as 512-byte runs per plane.
-__global__ __launch_bounds__(256) void transpose16_kernel(const Trio t) {
-  const int p = party_of(t);
-  const CircuitBufs& P = t.p[p];
-  __shared__ uint64_t planes[32][64];  // planes 0..15: a, 16..31: b
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const size_t w0 = (size_t)blockIdx.x * 64;
-#pragma unroll 4
-  for (int k = 0; k < 16; k++) {
-    const int wl = wave * 16 + k;
-    const size_t w = w0 + wl;
-    uint32_t va = 0, vb = 0;
-    if (w < t.W) {
-      va = P.mask_a[w * 64 + lane];
-      vb = P.mask_b[w * 64 + lane];
-    }
-    const uint64_t plane = wave_transpose32(va | (vb << 16), lane);
-    if (lane < 32) planes[lane][wl] = plane;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < 8; m++) {
-    const int idx = (int)threadIdx.x + 256 * m, pl = idx >> 6, wl = idx & 63;
-    const size_t w = w0 + wl;
-    if (w >= t.W) continue;
-    if (pl < 16)
-      P.tm_a[(size_t)pl * t.W + w] = planes[pl][wl];
-    else
-      P.tm_b[(size_t)(pl - 16) * t.W + w] = planes[pl][wl];
-  }
-}
-
 // ------------------------------------------------------------ carry-save AND
 // and_pre(x1^x3, x2^x3) for the split of party p (kernel.cu:39-42,205-235):
 //   p0: x1=(a,0) x2=0 x3=(0,b)  -> a & b
@@ -148,141 +73,6 @@ __device__ __forceinline__ uint64_t csa_and(int p, uint64_t a, uint64_t b) {
 __device__ __forceinline__ uint64_t and_pre(uint64_t xa, uint64_t xb, uint64_t ya, uint64_t yb,
                                             uint64_t r) {
   return (xa & ya) ^ (xb & ya) ^ (xa & yb) ^ r;
-}
-
-// ------------------------------------------------------------ fused adder
-// The carry-save stage and every ripple round of one adder (the lift: 16
-// planes, first + 14 rounds + carry-out; the MSB: 31 planes, first + 29
-// rounds + final XOR) in one launch. Wave p of a block is party p; lane l
-// handles words 8 (64 blockIdx.x + l) .. + 7. Each round loads one
-// transposed plane pair, T_{k+1}: it is a_k of this round and the input of
-// the carry-save AND of plane k + 1, whose output b_{k+1} the next round uses.
-// Both AND outputs of a round reach the next party through one LDS exchange;
-// carry-save outputs and the running carry never leave the block.
-__device__ __forceinline__ void csa8(int p, const uint64_t a[8], const uint64_t b[8],
-                                     const uint64_t r[8], uint64_t v[8]) {
-#pragma unroll
-  for (int i = 0; i < 8; i++) v[i] = csa_and(p, a[i], b[i]) ^ r[i];
-}
-
-__global__ __launch_bounds__(192) void adder_fused_kernel(const Trio t, int msb, const RippleCtrs rc) {
-  __shared__ uint64_t xbuf[2][3][64][16];  // per round slot: ripple AND (8), carry-save AND (8)
-  const int p = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63), prev = (p + 2) % 3;
-  const CircuitBufs& P = t.p[p];
-  const size_t W = t.W, g = (size_t)blockIdx.x * 64 + lane, o = 8 * g, G = W / 8;
-  const bool live = o < W;
-  const uint64_t* Ta = msb ? P.tx_a : P.tm_a;
-  const uint64_t* Tb = msb ? P.tx_b : P.tm_b;
-  const int last = msb ? 30 : 15;  // ripple rounds 0 .. last - 1, carry planes 0 .. last
-  uint64_t ta[8] = {}, tb[8] = {}, ca[8] = {}, cb[8] = {}, xa[8] = {}, xb[8] = {}, va[8] = {},
-           vb[8] = {}, r[8];
-
-  // carry-save AND of plane 0
-  if (live) {
-    load8(Ta + o, ta);
-    load8(Tb + o, tb);
-    corr_u64x8(P, t.nonce, rc.csa + g, r);
-    csa8(p, ta, tb, r, ca);
-  }
-#pragma unroll
-  for (int i = 0; i < 8; i++) xbuf[1][p][lane][8 + i] = ca[i];
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < 8; i++) cb[i] = xbuf[1][prev][lane][8 + i];
-
-  for (int k = 0; k < last; k++) {
-    uint64_t na[8] = {}, nb[8] = {}, cs[8] = {};
-    // b_k = carry-save plane k ^ x3 (x3 = (0, Tb_k) for p0, (Ta_k, 0) for p2)
-    uint64_t ba[8], bb[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      ba[i] = ca[i] ^ (p == 2 ? ta[i] : 0);
-      bb[i] = cb[i] ^ (p == 0 ? tb[i] : 0);
-    }
-    if (live) {
-      load8(Ta + (size_t)(k + 1) * W + o, na);  // a_k = T plane k + 1
-      load8(Tb + (size_t)(k + 1) * W + o, nb);
-      corr_u64x8(P, t.nonce, rc.c[k] + g, r);
-    }
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      xa[i] ^= va[i];  // carry after round k - 1 (va = vb = 0 before round 1)
-      xb[i] ^= vb[i];
-      va[i] = live ? and_pre(na[i] ^ xa[i], nb[i] ^ xb[i], ba[i] ^ xa[i], bb[i] ^ xb[i], r[i]) : 0;
-    }
-    if (k == 0) {  // the first round has no carry in
-#pragma unroll
-      for (int i = 0; i < 8; i++) {
-        xa[i] = 0;
-        xb[i] = 0;
-      }
-    }
-    if (live) {  // carry-save AND of plane k + 1 (planes 1 .. last)
-      corr_u64x8(P, t.nonce, rc.csa + (size_t)(k + 1) * G + g, r);
-      csa8(p, na, nb, r, cs);
-    }
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      xbuf[k & 1][p][lane][i] = va[i];
-      xbuf[k & 1][p][lane][8 + i] = cs[i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      vb[i] = xbuf[k & 1][prev][lane][i];
-      cb[i] = xbuf[k & 1][prev][lane][8 + i];
-      ca[i] = cs[i];
-      ta[i] = na[i];
-      tb[i] = nb[i];
-    }
-  }
-  // carry after the last round
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    xa[i] ^= va[i];
-    xb[i] ^= vb[i];
-  }
-  // b_last = carry-save plane last ^ x3
-  uint64_t ba[8], bb[8];
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    ba[i] = ca[i] ^ (p == 2 ? ta[i] : 0);
-    bb[i] = cb[i] ^ (p == 0 ? tb[i] : 0);
-  }
-  if (!msb) {
-    // carry-out (protocol.rs:2229-2239): c2 = b15 & c (shared AND), c1 = c ^ b15
-    if (live) corr_u64x8(P, t.nonce, rc.c[15] + g, r);
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      va[i] = live ? and_pre(ba[i], bb[i], xa[i], xb[i], r[i]) : 0;
-      xa[i] ^= ba[i];
-      xb[i] ^= bb[i];
-    }
-#pragma unroll
-    for (int i = 0; i < 8; i++) xbuf[(last - 1 + 1) & 1][p][lane][i] = va[i];
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; i++) vb[i] = xbuf[(last - 1 + 1) & 1][prev][lane][i];
-    if (live) {
-      store8(P.c_a + o, xa);
-      store8(P.c_b + o, xb);
-      store8(P.c_a + W + o, va);
-      store8(P.c_b + W + o, vb);
-    }
-    return;
-  }
-  // MSB (protocol.rs:2347-2352): msb = c ^ a_30 ^ b_30, a_30 = T plane 31
-  if (!live) return;
-  uint64_t aa[8], ab[8];
-  load8(Ta + 31 * W + o, aa);
-  load8(Tb + 31 * W + o, ab);
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    xa[i] ^= aa[i] ^ ba[i];
-    xb[i] ^= ab[i] ^ bb[i];
-  }
-  store8(P.res_a + o, xa);
-  store8(P.res_b + o, xb);
 }
 
 // ------------------------------------------------------------ bit injection
@@ -402,48 +192,260 @@ __global__ __launch_bounds__(256) void inject_kernel(const Trio t, int step, uin
   }
 }
 
-// ------------------------------------------------------------ lift_mul_sub + transpose32
-// m = (mask - c1<<16 - c2<<17) * A - code<<16, minus 1 on x_0 (kernel.cu:181-195,
-// 407-426), then the 32x64 bit transpose (kernel.cu:86-107).
-__global__ __launch_bounds__(256) void lms_transpose32_kernel(const Trio t, uint32_t A) {
-  const int p = party_of(t);
-  const CircuitBufs& P = t.p[p];
-  __shared__ uint64_t planes[64][64];  // planes 0..31: a, 32..63: b
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const size_t w0 = (size_t)blockIdx.x * 64;
-  const size_t L = 64 * t.W;
-#pragma unroll 4
-  for (int k = 0; k < 16; k++) {
-    const int wl = wave * 16 + k;
-    const size_t w = w0 + wl;
-    uint32_t ma = 0, mb = 0;
-    if (w < t.W) {
-      const size_t e = w * 64 + lane;
-      ma = (uint32_t)P.mask_a[e] - ((uint32_t)P.corr_a[e] << 16) - ((uint32_t)P.corr_a[e + L] << 17);
-      mb = (uint32_t)P.mask_b[e] - ((uint32_t)P.corr_b[e] << 16) - ((uint32_t)P.corr_b[e + L] << 17);
-      if (P.lifted_a) {
-        P.lifted_a[e] = ma;
-        P.lifted_b[e] = mb;
-      }
-      ma = ma * A - ((uint32_t)P.code_a[e] << 16);
-      mb = mb * A - ((uint32_t)P.code_b[e] << 16);
-      if (p == 0) ma -= 1u;
-      if (p == 1) mb -= 1u;
-    }
-    // plane j of a in lane j, of b in lane 32 + j
-    const uint64_t pa = wave_transpose32(ma, lane), pb = wave_transpose32(mb, lane);
-    planes[lane][wl] = lane < 32 ? pa : pb;
-  }
-  __syncthreads();
+// ------------------------------------------------------------ word adders
+// One launch per adder, all three parties: the lift (16 planes of the masked
+// mask share, carry-out into c planes 0 and 1) and the MSB stage (the 32
+// planes of m = (mask - c1<<16 - c2<<17) * A - code<<16 - [party], result into
+// res). A block holds the same 32 words of the three parties, one wave per
+// party and one lane per half word (32 elements): the lane loads its
+// elements, forms the values and turns them into 32-bit plane slices in
+// registers (one 32 x 32 bit transpose per share), so no plane goes through
+// HBM. Then step 0 is the carry-save AND of plane 0 and step s the
+// carry-save AND of plane s with ripple round s - 1 (protocol.rs:2137-2359,
+// kernel.cu:205-235); each step's AND outputs reach the next party through
+// LDS.
+//
+// The ChaCha12 block pair behind one AND of 8 words (F(own) ^ F(prev),
+// counter as in the schedule) is computed once per 16-lane group: for a batch
+// of 8 steps (16 gates) lane i of a group computes gate i and LDS hands every
+// lane its half word of all 16. The lift's unused slot (step 0 has no ripple
+// round) carries the carry-out AND.
+
+// Transpose a 32 x 32 bit matrix in place: afterwards bit i of x[j] is bit j
+// of the old x[i] (recursive block swap, 5 stages).
+template <int S, uint32_t M>
+__device__ __forceinline__ void bit_transpose_stage(uint32_t x[32]) {
 #pragma unroll
-  for (int m = 0; m < 16; m++) {
-    const int idx = (int)threadIdx.x + 256 * m, pl = idx >> 6, wl = idx & 63;
-    const size_t w = w0 + wl;
-    if (w >= t.W) continue;
-    if (pl < 32)
-      P.tx_a[(size_t)pl * t.W + w] = planes[pl][wl];
-    else
-      P.tx_b[(size_t)(pl - 32) * t.W + w] = planes[pl][wl];
+  for (int k = 0; k < 32; k++)
+    if ((k & S) == 0) {
+      const uint32_t t = ((x[k] >> S) ^ x[k + S]) & M;
+      x[k + S] ^= t;
+      x[k] ^= t << S;
+    }
+}
+
+__device__ __forceinline__ void bit_transpose32(uint32_t x[32]) {
+  bit_transpose_stage<16, 0x0000FFFFu>(x);
+  bit_transpose_stage<8, 0x00FF00FFu>(x);
+  bit_transpose_stage<4, 0x0F0F0F0Fu>(x);
+  bit_transpose_stage<2, 0x33333333u>(x);
+  bit_transpose_stage<1, 0x55555555u>(x);
+}
+
+// A lane's plane slices: lift: pl[j], j < 16 the a share and j >= 16 the b
+// share of plane j mod 16; MSB: pl[j] a, pl[32 + j] b.
+template <bool MSB>
+struct PlaneSlices {
+  uint32_t pl[MSB ? 64 : 32];
+  __device__ __forceinline__ uint32_t a(int j) const { return pl[j]; }
+  __device__ __forceinline__ uint32_t b(int j) const { return pl[(MSB ? 32 : 16) + j]; }
+};
+
+struct AdderState {
+  uint32_t ta, tb;  // plane s - 1
+  uint32_t ca, cb;  // carry-save AND of plane s - 1: own, previous party's
+  uint32_t xa, xb;  // carry
+  uint32_t va, vb;  // last ripple AND output: own, previous party's
+};
+
+struct AdderLds {
+  uint32_t rnd[3][16][16][5];  // [party][gate of the batch][half word of the group][group] (padded)
+  uint32_t xch[2][3][2][64];   // [step parity][party][ripple, carry-save][lane]
+};
+
+__device__ __forceinline__ uint32_t csa_and32(int p, uint32_t a, uint32_t b) {
+  return p == 2 ? (a & ~b) : (a & b);
+}
+
+__device__ __forceinline__ uint32_t and_pre32(uint32_t xa, uint32_t xb, uint32_t ya, uint32_t yb,
+                                              uint32_t r) {
+  return (xa & ya) ^ (xb & ya) ^ (xa & yb) ^ r;
+}
+
+// Step S (last plane LAST). r16[2 (S % 8)] is the carry-save randomness of
+// plane S, r16[2 (S % 8) + 1] the ripple randomness of round S - 1; the lift's
+// carry-out randomness is r_out.
+template <bool MSB, int S>
+__device__ __forceinline__ void adder_step(const PlaneSlices<MSB>& W, AdderState& st,
+                                           const uint32_t r16[16], uint32_t r_out, AdderLds& L,
+                                           int p, int prev, int lane) {
+  constexpr int LAST = MSB ? 30 : 15;
+  constexpr int J = S % 8;
+  constexpr int X = S & 1;
+  if constexpr (S == 0) {
+    const uint32_t na = W.a(0), nb = W.b(0);
+    const uint32_t cs = csa_and32(p, na, nb) ^ r16[0];
+    L.xch[X][p][1][lane] = cs;
+    __syncthreads();
+    st.cb = L.xch[X][prev][1][lane];
+    st.ca = cs;
+    st.ta = na;
+    st.tb = nb;
+    st.xa = st.xb = st.va = st.vb = 0;
+  } else if constexpr (S <= LAST) {
+    const uint32_t ba = st.ca ^ (p == 2 ? st.ta : 0u), bb = st.cb ^ (p == 0 ? st.tb : 0u);
+    const uint32_t na = W.a(S), nb = W.b(S);
+    st.xa ^= st.va;  // carry after round S - 2
+    st.xb ^= st.vb;
+    const uint32_t va = and_pre32(na ^ st.xa, nb ^ st.xb, ba ^ st.xa, bb ^ st.xb, r16[2 * J + 1]);
+    const uint32_t cs = csa_and32(p, na, nb) ^ r16[2 * J];
+    L.xch[X][p][0][lane] = va;
+    L.xch[X][p][1][lane] = cs;
+    __syncthreads();
+    st.vb = L.xch[X][prev][0][lane];
+    st.cb = L.xch[X][prev][1][lane];
+    st.va = va;
+    st.ca = cs;
+    st.ta = na;
+    st.tb = nb;
+    if constexpr (!MSB && S == LAST) {
+      // carry-out (protocol.rs:2229-2239): c2 = b15 & c (shared AND), c1 = c ^ b15
+      st.xa ^= st.va;
+      st.xb ^= st.vb;
+      const uint32_t ba2 = st.ca ^ (p == 2 ? st.ta : 0u), bb2 = st.cb ^ (p == 0 ? st.tb : 0u);
+      const uint32_t v2 = and_pre32(ba2, bb2, st.xa, st.xb, r_out);
+      st.xa ^= ba2;
+      st.xb ^= bb2;
+      L.xch[X ^ 1][p][0][lane] = v2;
+      __syncthreads();
+      st.vb = L.xch[X ^ 1][prev][0][lane];
+      st.va = v2;
+    }
+  }
+}
+
+template <bool MSB, int B>
+__device__ __forceinline__ void adder_batch(const PlaneSlices<MSB>& W, AdderState& st,
+                                            const uint32_t r16[16], uint32_t r_out, AdderLds& L,
+                                            int p, int prev, int lane) {
+#define IRISMPC_STEP(j) adder_step<MSB, 8 * B + j>(W, st, r16, r_out, L, p, prev, lane)
+  IRISMPC_STEP(0); IRISMPC_STEP(1); IRISMPC_STEP(2); IRISMPC_STEP(3);
+  IRISMPC_STEP(4); IRISMPC_STEP(5); IRISMPC_STEP(6); IRISMPC_STEP(7);
+#undef IRISMPC_STEP
+}
+
+template <bool MSB>
+__global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t A, const RippleCtrs rc) {
+  constexpr int LAST = MSB ? 30 : 15;    // carry-save planes 0 .. LAST
+  constexpr int LAST_C = MSB ? 29 : 14;  // ripple rounds 0 .. LAST_C (lift carry-out: c[15])
+  constexpr int N_BATCH = MSB ? 4 : 2;   // steps 0 .. 31 / 0 .. 15, 8 per batch
+  __shared__ AdderLds L;
+  // one party per wave: p is wave-uniform, so its keys and pointers stay scalar
+  const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63), prev = (p + 2) % 3;
+  const int grp = lane >> 4, gl = lane & 15;  // 16-lane group = 8 words = one ChaCha block
+  const CircuitBufs& P = t.p[p];
+  const size_t W = t.W, G = W / 8, n_el = 64 * W;
+  const size_t hw = (size_t)blockIdx.x * 64 + lane;  // half word: elements 32 hw .. 32 hw + 31
+  const size_t g = (size_t)blockIdx.x * 4 + grp;     // ChaCha block of words 8 g .. 8 g + 7
+  const bool live = hw < 2 * W;
+
+  PlaneSlices<MSB> sl;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const size_t e = 32 * hw + 8 * q;
+    if constexpr (MSB) {
+      // m = (mask - c1<<16 - c2<<17) * A - code<<16, minus 1 on x_0
+      // (kernel.cu:181-195,407-426)
+      uint16_t ma[8] = {}, mb[8] = {}, c1a[8] = {}, c2a[8] = {}, c1b[8] = {}, c2b[8] = {},
+               xa[8] = {}, xb[8] = {};
+      if (live) {
+        u16x8_load(P.mask_a + e, ma);
+        u16x8_load(P.mask_b + e, mb);
+        u16x8_load(P.corr_a + e, c1a);
+        u16x8_load(P.corr_a + n_el + e, c2a);
+        u16x8_load(P.corr_b + e, c1b);
+        u16x8_load(P.corr_b + n_el + e, c2b);
+        u16x8_load(P.code_a + e, xa);
+        u16x8_load(P.code_b + e, xb);
+      }
+      uint32_t la[8], lb[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        la[i] = (uint32_t)ma[i] - ((uint32_t)c1a[i] << 16) - ((uint32_t)c2a[i] << 17);
+        lb[i] = (uint32_t)mb[i] - ((uint32_t)c1b[i] << 16) - ((uint32_t)c2b[i] << 17);
+      }
+      if (P.lifted_a && live) {
+        uint4* da = reinterpret_cast<uint4*>(P.lifted_a + e);
+        uint4* db = reinterpret_cast<uint4*>(P.lifted_b + e);
+        da[0] = uint4{la[0], la[1], la[2], la[3]};
+        da[1] = uint4{la[4], la[5], la[6], la[7]};
+        db[0] = uint4{lb[0], lb[1], lb[2], lb[3]};
+        db[1] = uint4{lb[4], lb[5], lb[6], lb[7]};
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        sl.pl[8 * q + i] = la[i] * A - ((uint32_t)xa[i] << 16) - (p == 0 ? 1u : 0u);
+        sl.pl[32 + 8 * q + i] = lb[i] * A - ((uint32_t)xb[i] << 16) - (p == 1 ? 1u : 0u);
+      }
+    } else {
+      uint16_t ma[8] = {}, mb[8] = {};
+      if (live) {
+        u16x8_load(P.mask_a + e, ma);
+        u16x8_load(P.mask_b + e, mb);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) sl.pl[8 * q + i] = (uint32_t)ma[i] | ((uint32_t)mb[i] << 16);
+    }
+  }
+  bit_transpose32(sl.pl);
+  if constexpr (MSB) bit_transpose32(sl.pl + 32);
+
+  AdderState st{};
+  uint32_t r_out = 0;
+#pragma unroll 1
+  for (int b = 0; b < N_BATCH; b++) {
+    {  // this lane's gate of the batch: step 8 b + gl / 2, slot gl & 1
+      const int sj = gl >> 1, s = 8 * b + sj;
+      uint64_t c_round = 0;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int k = 8 * b + j - 1;  // ripple round of step 8 b + j
+        const uint64_t v = (k >= 0 && k <= LAST_C) ? rc.c[k] : (!MSB && k < 0 ? rc.c[15] : 0);
+        if (sj == j) c_round = v;
+      }
+      bool valid;
+      uint64_t ctr;
+      if ((gl & 1) == 0) {  // carry-save AND of plane s
+        valid = s <= LAST;
+        ctr = rc.csa + (uint64_t)s * G + g;
+      } else {  // ripple round s - 1; the lift's step 0 slot: carry-out
+        valid = (s >= 1 && s <= LAST_C + 1) || (!MSB && s == 0);
+        ctr = c_round + g;
+      }
+      uint64_t r[8] = {};
+      if (valid) corr_u64x8(P, t.nonce, ctr, r);
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        L.rnd[p][gl][2 * i][grp] = (uint32_t)r[i];
+        L.rnd[p][gl][2 * i + 1][grp] = (uint32_t)(r[i] >> 32);
+      }
+    }
+    __syncthreads();
+    uint32_t r16[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) r16[c] = L.rnd[p][c][gl][grp];
+    if (!MSB && b == 0) r_out = r16[1];
+    switch (b) {
+      case 0: adder_batch<MSB, 0>(sl, st, r16, r_out, L, p, prev, lane); break;
+      case 1: adder_batch<MSB, 1>(sl, st, r16, r_out, L, p, prev, lane); break;
+      case 2: adder_batch<MSB, 2>(sl, st, r16, r_out, L, p, prev, lane); break;
+      default: adder_batch<MSB, 3>(sl, st, r16, r_out, L, p, prev, lane); break;
+    }
+  }
+  if (!live) return;
+  if constexpr (MSB) {
+    // msb = c ^ a_30 ^ b_30, a_30 = plane 31 (protocol.rs:2347-2352)
+    st.xa ^= st.va;
+    st.xb ^= st.vb;
+    const uint32_t ba = st.ca ^ (p == 2 ? st.ta : 0u), bb = st.cb ^ (p == 0 ? st.tb : 0u);
+    reinterpret_cast<uint32_t*>(P.res_a)[hw] = st.xa ^ sl.a(31) ^ ba;
+    reinterpret_cast<uint32_t*>(P.res_b)[hw] = st.xb ^ sl.b(31) ^ bb;
+  } else {
+    reinterpret_cast<uint32_t*>(P.c_a)[hw] = st.xa;
+    reinterpret_cast<uint32_t*>(P.c_b)[hw] = st.xb;
+    reinterpret_cast<uint32_t*>(P.c_a + W)[hw] = st.va;
+    reinterpret_cast<uint32_t*>(P.c_b + W)[hw] = st.vb;
   }
 }
 
@@ -479,28 +481,21 @@ __global__ __launch_bounds__(256) void open_kernel(const OpenArgs a) {
 }
 
 // ------------------------------------------------------------ launchers
-void launch_transpose16(const Trio& t, int np, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(t.W, 64), (unsigned)np);
-  transpose16_kernel<<<grid, 256, 0, s>>>(t);
-  IRIS_HIP_CHECK(hipGetLastError());
-}
-
-void launch_adder_fused(const Trio& t, bool msb, const RippleCtrs& ctrs, hipStream_t s) {
-  IRIS_REQUIRE(t.party0 == 0, "the fused adder runs all three parties");
+void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const RippleCtrs& ctrs,
+                       hipStream_t s) {
+  IRIS_REQUIRE(t.party0 == 0, "the word adders run all three parties");
   IRIS_REQUIRE(t.W % 8 == 0, "bit planes must be a multiple of 8 words");
-  adder_fused_kernel<<<(unsigned)ceil_div(t.W, 512), 192, 0, s>>>(t, msb ? 1 : 0, ctrs);
+  const unsigned blocks = (unsigned)ceil_div(t.W, 32);
+  if (msb)
+    adder_word_kernel<true><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
+  else
+    adder_word_kernel<false><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 
 void launch_inject(const Trio& t, int np, int step, uint64_t ctr, hipStream_t s) {
   dim3 grid((unsigned)ceil_div(128 * t.W, 8192), (unsigned)np);
   inject_kernel<<<grid, 256, 0, s>>>(t, step, ctr);
-  IRIS_HIP_CHECK(hipGetLastError());
-}
-
-void launch_lms_transpose32(const Trio& t, int np, uint32_t A, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(t.W, 64), (unsigned)np);
-  lms_transpose32_kernel<<<grid, 256, 0, s>>>(t, A);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 

@@ -14,11 +14,9 @@ struct CircuitBufs {
   const uint16_t* code_b;
   const uint16_t* mask_a;
   const uint16_t* mask_b;
-  uint64_t *tm_a, *tm_b;    // 16 W: transposed mask shares
   uint64_t *c_a, *c_b;      // 2 W: lift carries (bit 16, bit 17)
   uint16_t* inbox;          // 128 W: bit-inject message received from prev
   uint16_t *corr_a, *corr_b;  // 128 W: injected carries (arithmetic u16 shares)
-  uint64_t *tx_a, *tx_b;    // 32 W: transposed lifted comparison values
   uint64_t *res_a, *res_b;  // W: result bit shares (plane 0)
   uint32_t *lifted_a, *lifted_b;  // optional (debug): corrected lifted mask
   ChaChaKey key_own, key_prev;    // Circuits ChaCha seeds (chacha1 / chacha2)
@@ -47,20 +45,20 @@ struct CircuitSchedule {
   static CircuitSchedule make(uint64_t start, size_t W, size_t alloc_words);
 };
 
-// The carry-save stage and all ripple rounds of one adder in one launch (the
-// lift: 16 planes, the MSB: 31). A block holds the same 512 words of all
-// three parties, one wave per party: each round's AND outputs reach the next
-// party through LDS, and carries never leave the block, so a round reads
-// only one transposed plane pair per party.
+// ChaCha12 block counters of one adder: the carry-save AND of plane j uses
+// csa + j W / 8 (+ word / 8), ripple round k uses c[k] (lift: c[15] is the
+// carry-out AND).
 struct RippleCtrs {
-  uint64_t csa;    // ChaCha block counter of the carry-save AND of plane 0 (plane j: + j W / 8)
-  uint64_t c[32];  // ripple round k (k = 0: first round; lift: c[15] = carry-out)
+  uint64_t csa;
+  uint64_t c[32];
 };
 
-void launch_transpose16(const Trio& t, int n_parties, hipStream_t s);
-void launch_adder_fused(const Trio& t, bool msb, const RippleCtrs& ctrs, hipStream_t s);
+// One launch per adder, all three parties (party0 must be 0): msb = false is
+// the lift (mask shares -> carries in c planes 0 and 1), msb = true the
+// lift_mul_sub and the MSB adder (-> res). The bit planes stay in registers.
+void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const RippleCtrs& ctrs,
+                       hipStream_t s);
 void launch_inject(const Trio& t, int n_parties, int step, uint64_t ctr, hipStream_t s);
-void launch_lms_transpose32(const Trio& t, int n_parties, uint32_t threshold_a, hipStream_t s);
 
 // Open + bitmap (actor.rs:2928-2997, dot/kernel.cu:30-116). Party views:
 // a = own result.a, b = own result.b, c = prev party's result.b (received).

@@ -566,9 +566,7 @@ Circuits::Circuits(int device, int peer_id, size_t input_size, size_t alloc_size
   IRIS_REQUIRE(ex != nullptr, "Circuits needs an exchange handle");
   DeviceGuard g(device_);
   const size_t A = alloc_words_ + 8;
-  tm_a_.alloc(16 * A); tm_b_.alloc(16 * A);
   c_a_.alloc(2 * A); c_b_.alloc(2 * A);
-  tx_a_.alloc(32 * A); tx_b_.alloc(32 * A);
   res_a_.alloc(A); res_b_.alloc(A); open_c_.alloc(A);
   inbox_.alloc(128 * A);
   corr_a_.alloc(128 * A); corr_b_.alloc(128 * A);
@@ -583,10 +581,8 @@ CircuitBufs Circuits::bufs(const uint16_t* ca, const uint16_t* cb, const uint16_
                            const uint16_t* mb) const {
   CircuitBufs b{};
   b.code_a = ca; b.code_b = cb; b.mask_a = ma; b.mask_b = mb;
-  b.tm_a = tm_a_.ptr; b.tm_b = tm_b_.ptr;
   b.c_a = c_a_.ptr; b.c_b = c_b_.ptr;
   b.inbox = inbox_.ptr; b.corr_a = corr_a_.ptr; b.corr_b = corr_b_.ptr;
-  b.tx_a = tx_a_.ptr; b.tx_b = tx_b_.ptr;
   b.res_a = res_a_.ptr; b.res_b = res_b_.ptr;
   b.lifted_a = lifted_dbg_a; b.lifted_b = lifted_dbg_b;
   b.key_own = seed_own_; b.key_prev = seed_prev_;
@@ -618,20 +614,18 @@ void Circuits::compare_trio(Circuits* const c[3], const uint16_t* const code_a[3
   const CircuitSchedule sc = CircuitSchedule::make(c[0]->counter_, W, c[0]->alloc_words_);
   hipStream_t ps = ex->protocol_stream();
   ex->join_streams(s);
-  launch_transpose16(t, 3, ps);
   RippleCtrs lift{};
   lift.csa = sc.lift_csa;
   lift.c[0] = sc.lift_first;
   for (int k = 1; k <= 14; k++) lift.c[k] = sc.lift_round[k];
   lift.c[15] = sc.lift_final;
-  launch_adder_fused(t, false, lift, ps);
+  launch_adder_word(t, false, A, lift, ps);
   for (int step = 1; step <= 3; step++) launch_inject(t, 3, step, sc.inject, ps);
-  launch_lms_transpose32(t, 3, A, ps);
   RippleCtrs msb{};
   msb.csa = sc.msb_csa;
   msb.c[0] = sc.msb_first;
   for (int k = 1; k <= 29; k++) msb.c[k] = sc.msb_round[k];
-  launch_adder_fused(t, true, msb, ps);
+  launch_adder_word(t, true, A, msb, ps);
   if (fork) ex->fork_streams(s);
   for (int p = 0; p < 3; p++) c[p]->counter_ = sc.end;
 }

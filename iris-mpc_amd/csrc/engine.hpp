@@ -296,7 +296,7 @@ class Circuits {
   ChaChaKey seed_own_, seed_prev_;
   uint64_t nonce_ = 0, counter_ = 0;
   Exchange* ex_;
-  DevBuf<uint64_t> tm_a_, tm_b_, c_a_, c_b_, tx_a_, tx_b_, res_a_, res_b_, open_c_;
+  DevBuf<uint64_t> c_a_, c_b_, res_a_, res_b_, open_c_;
   DevBuf<uint16_t> inbox_, corr_a_, corr_b_;
   const uint64_t* open_view_ = nullptr;
 };
