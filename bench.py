@@ -25,6 +25,15 @@ for _p in (ROOT, os.path.join(ROOT, "iris-mpc_amd")):
 
 INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, 5.0 POPS
 HBM_PEAK_GBS = 8000.0
+# 32-bit integer VALU lane-ops: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_PEAK_TOPS = 39.3
+# ChaCha12 block pairs per (query-rotation, DB) pair per party in the threshold
+# circuits (one bit per AND gate and element, two streams per party): lift 32
+# ANDs + MSB adder 61 ANDs -> 93 x 2 / 512; bit injection: 2 u16 per pair per
+# acting step, 5 acting party-steps over 3 parties, two streams -> 5/3 x 2 x 4/64
+CHACHA_BLOCKS_PER_PAIR = 93 * 2 / 512 + 5 / 3 * 2 * 4 / 64
+CHACHA12_OPS = 600  # 6 double rounds x 8 quarter rounds x 12 ops + feed-forward and set-up
+THRESHOLD_ANON_A = 16384  # 0.375: the prefilter threshold of the full scan
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "share_gemm_c2_traffic.json")
 # kernel behind each GEMM variant id (-1: automatic, rotation-aware for raw-share queries)
 KERNEL_OF_VARIANT = {-1: "share_gemm_rot_kernel<2, 4>", 100: "share_gemm_rot_kernel<2, 4>",
@@ -171,7 +180,7 @@ def run_c3(args):
     pairs = Qr * N
     gemm_ops = 115200 * pairs * 3 * 2
     achieved = gemm_ops / (gemm_ms / 1e3) / 1e12
-    stream_bytes = 47.5 * pairs * 3 * 2
+    stream_ops = CHACHA_BLOCKS_PER_PAIR * CHACHA12_OPS * pairs * 3 * 2
     return {
         "metric": "3-party iris comparisons/sec on 1 MI355X + p50/p99 request latency",
         "value": 2 * pairs / (ms_step / 1e3),
@@ -190,11 +199,13 @@ def run_c3(args):
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
                      "stage": f"share GEMM alone (6 party-eye x code+mask launches, {gemm_ms:.1f} ms)"},
-        "streaming_stage": {"bound": "hbm", "algorithmic_bytes": stream_bytes,
+        "streaming_stage": {"bound": "valu", "algorithmic_ops": stream_ops,
                             "serial_ms_est": ms_step - gemm_ms,
-                            "achieved_GBs_est": stream_bytes / max(ms_step - gemm_ms, 1e-3) / 1e6,
-                            "peak_GBs": HBM_PEAK_GBS,
-                            "note": "47.5 B per pair per party; time = batch minus GEMM-alone time"},
+                            "achieved_Tops_est": stream_ops / max(ms_step - gemm_ms, 1e-3) / 1e9,
+                            "peak_Tops": VALU_PEAK_TOPS,
+                            "note": f"ChaCha12 correlated randomness: {CHACHA_BLOCKS_PER_PAIR:.3f} blocks x "
+                                    f"{CHACHA12_OPS} int32 ops per pair per party; time = batch minus "
+                                    "GEMM-alone time"},
     }
 
 
@@ -386,6 +397,63 @@ def run_c5(args):
     }
 
 
+def run_circuits(args):
+    """Threshold circuits alone at one chunk of the full protocol: the three
+    parties' lift, bit injection, MSB adder and open over 32,768 DB rows x
+    args.queries x 31 rotations of uniformly random code/mask dot shares."""
+    import numpy as np
+    import torch
+
+    import irismpc
+
+    torch.cuda.set_device(0)
+    Qr = args.queries * 31
+    L = Qr * irismpc.DB_CHUNK_SIZE
+    W = L // 64
+    ex = irismpc.Exchange(timeout_s=300)
+    seeds = [(np.full(8, p, np.uint32), np.full(8, (p + 2) % 3, np.uint32)) for p in range(3)]
+    circ = [irismpc.Circuits(p, L, W, seeds[p], ex) for p in range(3)]
+    streams = [irismpc.stream_create() for _ in range(3)]
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    bufs = [[torch.randint(-32768, 32767, (L,), dtype=torch.int16, device="cuda", generator=gen)
+             for _ in range(4)] for _ in range(3)]
+    torch.cuda.synchronize()
+    codes = [(b[0].data_ptr(), b[1].data_ptr()) for b in bufs]
+    masks = [(b[2].data_ptr(), b[3].data_ptr()) for b in bufs]
+
+    def step():
+        irismpc.Circuits.compare_threshold_masked_many_trio(circ, codes, masks, L, THRESHOLD_ANON_A, streams)
+        irismpc.Circuits.open_trio(circ, streams)
+
+    for _ in range(args.warmup):
+        step()
+    for st in streams:
+        irismpc.stream_synchronize(st)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    for st in streams:
+        irismpc.stream_synchronize(st)
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    ops = CHACHA_BLOCKS_PER_PAIR * CHACHA12_OPS * L * 3
+    achieved = ops / (ms_step / 1e3) / 1e12
+    return {
+        "metric": "threshold-circuit comparisons/sec (3 parties, one DB chunk)",
+        "value": L / (ms_step / 1e3), "unit": "comparisons/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "GF(2) bit-sliced circuits on u16/u32 shares, ChaCha12 correlated randomness",
+        "data": "uniformly random u16 code and mask dot shares per party",
+        "config": {"workload": f"lift, bit injection, MSB adder and open of 3 parties over one chunk: "
+                               f"{Qr:,} query-rotation rows x {irismpc.DB_CHUNK_SIZE:,} DB rows "
+                               f"({W:,} result words)", "threshold_a": THRESHOLD_ANON_A},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                     "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                     "stage": f"{CHACHA_BLOCKS_PER_PAIR:.3f} ChaCha12 blocks x {CHACHA12_OPS} int32 ops "
+                              "per pair per party"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,10 +464,11 @@ def main():
     ap.add_argument("--variant", type=int, default=-1,
                     help="GEMM kernel variant (-1: automatic, the rotation-aware kernel)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "circuits"), default="c2",
                     help="c2: per-party share GEMM (the headline line); c3: full 3-party protocol, "
                          "both eyes; c4: batching form (use --db 4194304 for the full-size DB); "
-                         "c5: serving form (use --db 2097152 for the full-size DB)")
+                         "c5: serving form (use --db 2097152 for the full-size DB); circuits: the "
+                         "threshold circuits alone at one chunk")
     ap.add_argument("--rates", default="2,5,10,20,40", help="c5: request rates per second")
     ap.add_argument("--seconds", type=float, default=8.0, help="c5: arrival window per rate")
     ap.add_argument("--plain-requests", type=int, default=8, help="c5: requests timed in the plain form")
@@ -412,6 +481,9 @@ def main():
 
     if args.workload == "c3":
         print(json.dumps(run_c3(args)))
+        return
+    if args.workload == "circuits":
+        print(json.dumps(run_circuits(args)))
         return
     if args.workload == "c4":
         print(json.dumps(run_c4(args)))
