@@ -152,6 +152,16 @@ int irismpc_query_upload(irismpc_share_db* e, const uint16_t* rows, size_t n_row
  * or a batch without padding. */
 int irismpc_query_upload_shares_rows(irismpc_share_db* e, const uint16_t* shares, size_t batch,
                                      size_t rows, void* stream, irismpc_query** out);
+/* Refills a query set made by irismpc_query_upload_shares[_rows] with new raw
+ * shares (batch queries, `rows` rows, 0 = query_length), reusing its device
+ * buffers: no allocation or free, so no implicit device-wide synchronisation.
+ * rows and batch must not exceed what the set was made with. Replaces the
+ * per-batch query buffer allocation of the reference's query processor
+ * (iris-mpc-gpu/src/server/query_processor.rs:30-72) followed by
+ * preprocess_query (dot/share_db.rs:45-59); used by a request loop whose
+ * active query set changes between chunk steps. */
+int irismpc_query_refill_shares(irismpc_share_db* e, irismpc_query* q, const uint16_t* shares,
+                                size_t batch, size_t rows, void* stream);
 /* Queries of the Mirror orientation (full-face mirror attack pass,
  * actor.rs:591-615; server/mod.rs:156-176): the query rows come from the
  * mirrored shares of `shares` (the other eye's raw shares; mirrored_code for

@@ -348,11 +348,35 @@ std::unique_ptr<QuerySet> ShareDB::upload_shares_impl(const uint16_t* host_share
     IRIS_HIP_CHECK(hipMemcpyAsync(raw.ptr, insert_shares, raw.n * 2, hipMemcpyHostToDevice, s));
   }
   q->batch = (int)batch;
-  q->insert = alloc_db(round_up(batch, 512));  // room for a padded self-comparison chunk
+  q->cap_rows = n_rows;
+  q->cap_batch = round_up(batch, 512);
+  q->insert = alloc_db(q->cap_batch);  // room for a padded self-comparison chunk
   launch_tile_rows(raw.ptr, batch, code_length_, q->insert->limbs.ptr, 0, q->insert->sum0.ptr,
                    q->insert->sum1.ptr, s);
   IRIS_HIP_CHECK(hipStreamSynchronize(s));
   return q;
+}
+
+void ShareDB::refill_query_shares(QuerySet& q, const uint16_t* host_shares, size_t batch,
+                                  size_t n_rows, hipStream_t s) const {
+  DeviceGuard g(device_);
+  if (n_rows == 0) n_rows = query_length_;
+  IRIS_REQUIRE(q.code_length == code_length_, "query set of another code length");
+  IRIS_REQUIRE(q.insert != nullptr && q.cap_rows > 0, "refill needs a set uploaded from raw shares");
+  IRIS_REQUIRE(batch >= 1 && batch * kRotations <= n_rows, "batch exceeds the query rows / 31");
+  IRIS_REQUIRE(n_rows <= q.cap_rows && batch <= q.cap_batch, "refill larger than the query set");
+  const size_t K = (size_t)code_length_;
+  if (qraw_.n < batch * K) qraw_.alloc(q.cap_rows / kRotations * K, false);
+  if (qrows_.n < n_rows * K) qrows_.alloc(q.cap_rows * K, false);
+  IRIS_HIP_CHECK(hipMemcpyAsync(qraw_.ptr, host_shares, batch * K * 2, hipMemcpyHostToDevice, s));
+  launch_query_prep(qraw_.ptr, (int)batch, code_length_, peer_id_, qrows_.ptr, (int)n_rows, s, 0);
+  launch_tile_rows(qrows_.ptr, n_rows, code_length_, q.limbs.ptr, 0, q.sum0.ptr, q.sum1.ptr, s);
+  if (q.rot.ptr) launch_rot_planes(qrows_.ptr, (int)n_rows, code_length_, q.rot.ptr, s);
+  launch_tile_rows(qraw_.ptr, batch, code_length_, q.insert->limbs.ptr, 0, q.insert->sum0.ptr,
+                   q.insert->sum1.ptr, s);
+  q.rows = (int)n_rows;
+  q.batch = (int)batch;
+  IRIS_HIP_CHECK(hipStreamSynchronize(s));  // the staging buffers are reused by the next refill
 }
 
 int ShareDB::resolve_variant(const QuerySet& q) const {

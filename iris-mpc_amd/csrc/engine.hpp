@@ -117,6 +117,8 @@ struct QuerySet {
   // DB side of the batch self-comparison.
   int batch = 0;
   std::unique_ptr<DbSlice> insert;
+  // what the buffers hold (a refill with fewer rows or queries reuses them)
+  size_t cap_rows = 0, cap_batch = 0;
 };
 
 class ShareDB {
@@ -166,6 +168,12 @@ class ShareDB {
                                                          const uint16_t* insert_shares,
                                                          size_t batch, size_t rows,
                                                          hipStream_t s) const;
+  // Refills a query set made by upload_query_shares[_rows] with new raw
+  // shares (batch queries, n_rows rows, 0 = query_length), reusing its
+  // buffers: no device allocation or free, so no implicit device-wide
+  // synchronisation. n_rows and batch must fit what the set was made with.
+  void refill_query_shares(QuerySet& q, const uint16_t* host_shares, size_t batch, size_t n_rows,
+                           hipStream_t s) const;
   // dot + dot_reduce_and_multiply (fused): DB rows [offset, offset + chunk)
   // against all query rows; chunk is the padded chunk size (multiple of 64).
   void dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
@@ -224,6 +232,8 @@ class ShareDB {
   size_t last_rows_;  // query rows of the last dot_reduce (the reshare length)
   Exchange* ex_;
   DevBuf<uint16_t> results_, results_peer_, results_alt_;
+  // query staging of refill_query_shares (raw shares, prepared rows)
+  mutable DevBuf<uint16_t> qraw_, qrows_;
   uint16_t* peer_view_ = nullptr;
   int slot_ = 0;
   Event ev_send_done_;

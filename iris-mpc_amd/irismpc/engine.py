@@ -214,6 +214,13 @@ class ShareDB:
              ctypes.byref(h))
         return DeviceQuery(h.value)
 
+    def refill_query_shares(self, query, shares, rows=0, stream=0):
+        """New raw shares into a query set made by upload_query_shares[_rows],
+        reusing its device buffers (rows and batch within what it was made with)."""
+        r = np.ascontiguousarray(shares, dtype=np.uint16).reshape(-1, self.code_length)
+        call("irismpc_query_refill_shares", self.h, query.h, r.ctypes.data, r.shape[0], rows, stream)
+        return query
+
     def dot_reduce_packed(self, query, db, offset, total, chunk_size, multiplier, rows, out_ptrs,
                           ctr_first=None, stream=0):
         """Several batches packed in `query` (rows[i] rows each) in one launch;

@@ -139,13 +139,19 @@ class ServingEngines:
         return row
 
     # ------------------------------------------------------------------ queries
-    def _query_sets(self, ps, shares_list, rows):
+    def _query_sets(self, ps, shares_list, rows, reuse=None):
+        """The query sets of the three parties; `reuse` (earlier sets made with
+        at least `rows` rows) is refilled in place instead of reallocated."""
         qc, qm = [], []
         for p in range(3):
             code = np.stack([s[p][0] for s in shares_list])
             mask = np.stack([s[p][1] for s in shares_list])
-            qc.append(ps.codes[p].upload_query_shares_rows(code, rows))
-            qm.append(ps.masks[p].upload_query_shares_rows(mask, rows))
+            if reuse is None:
+                qc.append(ps.codes[p].upload_query_shares_rows(code, rows))
+                qm.append(ps.masks[p].upload_query_shares_rows(mask, rows))
+            else:
+                qc.append(ps.codes[p].refill_query_shares(reuse[0][p], code, rows))
+                qm.append(ps.masks[p].refill_query_shares(reuse[1][p], mask, rows))
         return qc, qm
 
     def n_chunks(self, rows):
@@ -234,7 +240,7 @@ class Server:
         worker.start()
         waiting, active = [], []
         free_slots = list(range(self.max_active))
-        qsets, active_ids = None, None
+        qsets, qsets_full, active_ids = None, None, None
         i, cursor = 0, 0
         while i < len(reqs) or waiting or active:
             now = time.perf_counter() - self.t0
@@ -274,7 +280,13 @@ class Server:
                 continue
             ids = [r.rid for r in active]
             if ids != active_ids:
-                qsets = eng._query_sets(eng.scan, [r.shares for r in active], len(active) * ROT)
+                # one set per party and matrix, made once at full size and refilled
+                if qsets_full is None:
+                    first = [r.shares for r in active]
+                    pad = first + [first[0]] * (self.max_active - len(first))
+                    qsets_full = eng._query_sets(eng.scan, pad, self.max_active * ROT)
+                qsets = eng._query_sets(eng.scan, [r.shares for r in active], len(active) * ROT,
+                                        reuse=qsets_full)
                 active_ids = ids
             n_chunks = eng.n_chunks(eng.n_rows)
             cursor %= n_chunks

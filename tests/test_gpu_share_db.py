@@ -134,6 +134,39 @@ def test_share_gemm_rotation_variants(irismpc, orc, K, mult, batch, qr):
         assert np.array_equal(got[:, :N], ref), f"variant {v}"
 
 
+@pytest.mark.parametrize("K,mult", [(12800, 1), (6400, 2)])
+def test_query_refill_matches_fresh_upload(irismpc, orc, K, mult):
+    """A query set refilled in place with other shares and fewer queries (as
+    the request loop does when its active set changes) gives the dot shares of
+    a fresh upload of those shares, under the automatic and the generic GEMM
+    variants; its DB form follows the new shares; a refill larger than the set
+    is rejected."""
+    rng = np.random.default_rng(12)
+    N, chunk, party = 1000, 1024, 0
+    first, second = _rand(rng, (8, K)), _rand(rng, (5, K))
+    db = _rand(rng, (N, K))
+    eng = irismpc.ShareDB(party, chunk, 8 * 31, K, ([0] * 8, [0] * 8))
+    slices = eng.alloc_db(N)
+    slices.load_full_db(db)
+    dq = eng.upload_query_shares_rows(first, 8 * 31)
+    for raw in (second, first[:3], second):
+        qr = raw.shape[0] * 31
+        ref = orc.u16_dot(_rotated_rows(orc, party, raw, qr), db)
+        ref = (ref.astype(np.uint32) * mult).astype(np.uint16)
+        eng.refill_query_shares(dq, raw, qr)
+        for v in (-1, 7):
+            eng.set_variant(v)
+            eng.dot_reduce(dq, slices, chunk, 0, mult)
+            got = eng.fetch_results(qr * chunk).reshape(qr, chunk)
+            assert np.array_equal(got[:, :N], ref), f"variant {v}, batch {raw.shape[0]}"
+        eng.set_variant(-1)
+        slices.write_at_index(dq, raw.shape[0] - 1, 11)
+        assert np.array_equal(slices.read_records(11, 1)[0], raw[-1])
+        db[11] = raw[-1]  # the DB now holds that row
+    with pytest.raises(RuntimeError):
+        eng.refill_query_shares(dq, _rand(rng, (9, K)), 9 * 31)
+
+
 def test_share_gemm_rotation_masked_range(irismpc, orc):
     """Masked multi-chunk launch with the rotation-aware kernel: same shares as
     the generic kernel, first chunk checked against the oracle."""
