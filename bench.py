@@ -27,11 +27,11 @@ INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, 5.0 POPS
 HBM_PEAK_GBS = 8000.0
 # 32-bit integer VALU lane-ops: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_PEAK_TOPS = 39.3
-# ChaCha12 block pairs per (query-rotation, DB) pair per party in the threshold
+# ChaCha12 blocks per (query-rotation, DB) pair per party in the threshold
 # circuits (one bit per AND gate and element, two streams per party): lift 32
-# ANDs + MSB adder 61 ANDs -> 93 x 2 / 512; bit injection: 2 u16 per pair per
-# acting step, 5 acting party-steps over 3 parties, two streams -> 5/3 x 2 x 4/64
-CHACHA_BLOCKS_PER_PAIR = 93 * 2 / 512 + 5 / 3 * 2 * 4 / 64
+# ANDs + MSB adder 61 ANDs -> 93 x 2 / 512; bit injection: 2 u16 per pair from
+# each of the party's two streams, drawn once -> 2 x 4 / 64
+CHACHA_BLOCKS_PER_PAIR = 93 * 2 / 512 + 2 * 4 / 64
 CHACHA12_OPS = 600  # 6 double rounds x 8 quarter rounds x 12 ops + feed-forward and set-up
 THRESHOLD_ANON_A = 16384  # 0.375: the prefilter threshold of the full scan
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "share_gemm_c2_traffic.json")

@@ -4,15 +4,16 @@
 // binary_add_3_get_two_carries :2137-2241, bit_inject :1302-1643,
 // lift_mul_sub :1831-1872, extract_msb/binary_add_3_get_msb :2243-2359) and
 // threshold_ring/cuda/kernel.cu. Design for one GPU:
-//  * the 16x64 / 32x64 bit transposes are wave ballots (lane = element);
+//  * a block holds the same elements of all three parties, one wave per
+//    party; a message to the next party goes to that party's wave through
+//    LDS, and a barrier is the exchange;
+//  * the 16 / 32 bit planes of the adders are formed by in-register bit
+//    transposes and never leave registers;
 //  * x1/x2/x3 of `split` are never materialised: for party p they are fixed
-//    selections of the transposed (a, b) planes, so x2^x3, x1^x3 and the
-//    carry-save AND collapse to one expression per party (bit-identical);
+//    selections of the (a, b) planes, so x2^x3, x1^x3 and the carry-save AND
+//    collapse to one expression per party (bit-identical);
 //  * correlated randomness is generated in registers from the block counter
-//    the reference would have used (no keystream buffers);
-//  * each launch covers all three parties (grid.y); a party's threads read
-//    only that party's state and write messages into the next party's "b"
-//    arrays / inbox. The kernel boundary is the exchange barrier.
+//    the reference would have used (no keystream buffers).
 #include "circuits.hpp"
 #include "chacha.hpp"
 
@@ -45,8 +46,6 @@ CircuitSchedule CircuitSchedule::make(uint64_t c, size_t W, size_t alloc) {
   return s;
 }
 
-__device__ __forceinline__ int party_of(const Trio& t) { return t.party0 + (int)blockIdx.y; }
-
 // Correlated randomness F(own) ^ F(prev) as 8 u64 words of block `blk`.
 __device__ __forceinline__ void corr_u64x8(const CircuitBufs& P, uint64_t nonce, uint64_t blk,
                                            uint64_t r[8]) {
@@ -63,15 +62,17 @@ __device__ __forceinline__ void corr_u64x8(const CircuitBufs& P, uint64_t nonce,
 //   p0: x1=(a,0) x2=0 x3=(0,b)  -> a & b
 //   p1: x1=(0,b) x2=(a,0) x3=0  -> a & b
 //   p2: x1=0 x2=(0,b) x3=(a,0)  -> a ^ (a & b)
-__device__ __forceinline__ uint64_t csa_and(int p, uint64_t a, uint64_t b) {
+__device__ __forceinline__ uint32_t csa_and(int p, uint32_t a, uint32_t b) {
   return p == 2 ? (a & ~b) : (a & b);
 }
 
 // ------------------------------------------------------------ ripple helpers
-// a_k = s plane k = transposed plane k+1; b_k = carry-save plane k ^ x3 plane k
-// (x3 = (0, b) for p0, (a, 0) for p2, 0 for p1).
-__device__ __forceinline__ uint64_t and_pre(uint64_t xa, uint64_t xb, uint64_t ya, uint64_t yb,
-                                            uint64_t r) {
+// Local share of the AND of replicated shares (x_a, x_b) and (y_a, y_b) plus
+// the zero-share r (protocol.rs:610-710). In ripple round k, a_k is plane
+// k + 1 and b_k the carry-save plane k ^ x3 plane k (x3 = (0, b) for p0,
+// (a, 0) for p2, 0 for p1).
+__device__ __forceinline__ uint32_t and_pre(uint32_t xa, uint32_t xb, uint32_t ya, uint32_t yb,
+                                            uint32_t r) {
   return (xa & ya) ^ (xb & ya) ^ (xa & yb) ^ r;
 }
 
@@ -80,10 +81,12 @@ __device__ __forceinline__ uint64_t and_pre(uint64_t xa, uint64_t xb, uint64_t y
 // e >= 64 W: bit 17); ChaCha12 block g of each stream covers elements 32 g .. 32 g + 31.
 // kernel.cu:526-629 with the message flow P1 -> P2 -> P0 and P0 -> P1.
 
-// Wave per 2048 elements (64 ChaCha12 blocks, one per lane; block g covers
-// elements 32 g .. 32 g + 31 as the u16 view of its keystream). The two
-// keystreams go through LDS in element order, then each lane handles 8
-// consecutive elements per pass with 16-byte loads and stores.
+// Block per 2048 elements, one wave per party: lane l draws block 64 g + l of
+// its party's two streams once (block g covers elements 32 g .. 32 g + 31 as
+// the u16 view of its keystream). The keystreams go through LDS in element
+// order, then each lane handles 8 consecutive elements per pass. The three
+// steps (P0 -> P1 and P1 -> P2, then P2 -> P0) pass their u16 messages to the
+// receiving party's wave through LDS, one barrier apart.
 __device__ __forceinline__ void u16x8_load(const uint16_t* p, uint16_t v[8]) {
   const uint4 q = *reinterpret_cast<const uint4*>(p);
   const uint32_t w[4] = {q.x, q.y, q.z, q.w};
@@ -98,44 +101,43 @@ __device__ __forceinline__ void u16x8_store(uint16_t* p, const uint16_t v[8]) {
   *reinterpret_cast<uint4*>(p) = uint4{w[0], w[1], w[2], w[3]};
 }
 
-__global__ __launch_bounds__(256) void inject_kernel(const Trio t, int step, uint64_t ctr) {
-  const int p = party_of(t);
-  // which party acts in which step
-  const bool act = (step == 1 && p != 2) || (step == 2 && p != 0) || (step == 3 && p == 0);
-  if (!act) return;
+__global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr) {
+  // one party per wave: p is wave-uniform, so its keys and pointers stay scalar
+  const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63);
   const CircuitBufs& P = t.p[p];
-  const CircuitBufs& N = t.p[(p + 1) % 3];
-  __shared__ uint32_t ks[4][2][1024];  // [wave][own, prev][u16 pairs in element order]
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ uint32_t ks[3][2][1024];  // [party][own, prev][u16 pairs in element order]
+  __shared__ uint32_t msg[3][1024];    // [receiving party][u16 pairs in element order]
   const size_t n = 128 * t.W;
-  const size_t group = (size_t)blockIdx.x * 4 + wave;
-  const size_t e_base = group * 2048;
-  if (e_base < n) {
+  const size_t e_base = (size_t)blockIdx.x * 2048;
+  {
     uint32_t ko[16], kp[16];
-    chacha_block<12>(P.key_own, t.nonce, ctr + group * 64 + lane, ko);   // my_rng (chacha1)
-    chacha_block<12>(P.key_prev, t.nonce, ctr + group * 64 + lane, kp);  // their_rng (chacha2)
+    chacha_block<12>(P.key_own, t.nonce, ctr + (uint64_t)blockIdx.x * 64 + lane, ko);   // my_rng (chacha1)
+    chacha_block<12>(P.key_prev, t.nonce, ctr + (uint64_t)blockIdx.x * 64 + lane, kp);  // their_rng (chacha2)
 #pragma unroll
     for (int i = 0; i < 16; i += 4) {
-      *reinterpret_cast<uint4*>(&ks[wave][0][lane * 16 + i]) = uint4{ko[i], ko[i + 1], ko[i + 2], ko[i + 3]};
-      *reinterpret_cast<uint4*>(&ks[wave][1][lane * 16 + i]) = uint4{kp[i], kp[i + 1], kp[i + 2], kp[i + 3]};
+      *reinterpret_cast<uint4*>(&ks[p][0][lane * 16 + i]) = uint4{ko[i], ko[i + 1], ko[i + 2], ko[i + 3]};
+      *reinterpret_cast<uint4*>(&ks[p][1][lane * 16 + i]) = uint4{kp[i], kp[i + 1], kp[i + 2], kp[i + 3]};
     }
   }
   __syncthreads();
-  if (e_base >= n) return;
-  const uint16_t* own16 = reinterpret_cast<const uint16_t*>(ks[wave][0]);
-  const uint16_t* prv16 = reinterpret_cast<const uint16_t*>(ks[wave][1]);
+  const uint16_t* own16 = reinterpret_cast<const uint16_t*>(ks[p][0]);
+  const uint16_t* prv16 = reinterpret_cast<const uint16_t*>(ks[p][1]);
+  uint16_t* to_next = reinterpret_cast<uint16_t*>(msg[(p + 1) % 3]);
+  const uint16_t* inbox = reinterpret_cast<const uint16_t*>(msg[p]);
+  // step 1: P0 sends y to P1, P1 sends x to P2; both write corr_a
+  if (p != 2) {
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const int el = 8 * (k * 64 + lane);
-    const size_t e = e_base + el;
-    if (e >= n) break;
-    uint16_t own[8], prv[8], in[8], o1[8], o2[8], o3[8];
-    u16x8_load(own16 + el, own);
-    u16x8_load(prv16 + el, prv);
-    if (p == 0) {
-      // rand_01 = my rng, rand_02 = their rng; in_b = party 0's b share
-      const uint32_t bits = (uint32_t)(P.c_b[e >> 6] >> (e & 63)) & 0xffu;
-      if (step == 1) {
+    for (int k = 0; k < 4; k++) {
+      const int el = 8 * (k * 64 + lane);
+      const size_t e = e_base + el;
+      if (e >= n) break;
+      uint16_t own[8], prv[8], o1[8], o2[8];
+      u16x8_load(own16 + el, own);
+      u16x8_load(prv16 + el, prv);
+      if (p == 0) {
+        // rand_01 = my rng, rand_02 = their rng; in_b = party 0's b share
+        const uint32_t bits = (uint32_t)(P.c_b[e >> 6] >> (e & 63)) & 0xffu;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
           const uint16_t b = (uint16_t)((bits >> i) & 1u), r01 = own[i], r02 = prv[i];
@@ -143,20 +145,8 @@ __global__ __launch_bounds__(256) void inject_kernel(const Trio t, int step, uin
           o1[i] = y;
           o2[i] = (uint16_t)(r01 - (y << 1));
         }
-        u16x8_store(N.inbox + e, o1);
-        u16x8_store(P.corr_a + e, o2);
       } else {
-        u16x8_load(P.inbox + e, in);
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-          const uint16_t b = (uint16_t)((bits >> i) & 1u), r02 = prv[i], z = in[i];
-          o1[i] = (uint16_t)(-((r02 + z) << 1) + b);
-        }
-        u16x8_store(P.corr_b + e, o1);
-      }
-    } else if (p == 1) {
-      // rand_01 = their rng, rand_12 = my rng
-      if (step == 1) {
+        // rand_01 = their rng, rand_12 = my rng
         const uint32_t bits = (uint32_t)((P.c_a[e >> 6] ^ P.c_b[e >> 6]) >> (e & 63)) & 0xffu;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -165,29 +155,62 @@ __global__ __launch_bounds__(256) void inject_kernel(const Trio t, int step, uin
           o1[i] = x;
           o2[i] = (uint16_t)(x - (r12 << 1));
         }
-        u16x8_store(N.inbox + e, o1);
-        u16x8_store(P.corr_a + e, o2);
-      } else {
-        u16x8_load(P.inbox + e, in);
+      }
+      u16x8_store(to_next + el, o1);
+      u16x8_store(P.corr_a + e, o2);
+    }
+  }
+  __syncthreads();
+  // step 2: P1 writes corr_b from y; P2 sends z to P0 and writes corr_a, corr_b
+  if (p != 0) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int el = 8 * (k * 64 + lane);
+      const size_t e = e_base + el;
+      if (e >= n) break;
+      uint16_t own[8], prv[8], in[8], o1[8], o2[8], o3[8];
+      u16x8_load(own16 + el, own);
+      u16x8_load(prv16 + el, prv);
+      u16x8_load(inbox + el, in);
+      if (p == 1) {
 #pragma unroll
         for (int i = 0; i < 8; i++) o1[i] = (uint16_t)(prv[i] - (in[i] << 1));
         u16x8_store(P.corr_b + e, o1);
+      } else {
+        // rand_12 = their rng, rand_02 = my rng; in_a = party 2's a share
+        const uint32_t bits = (uint32_t)(P.c_a[e >> 6] >> (e & 63)) & 0xffu;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const uint16_t r12 = prv[i], r02 = own[i], x = in[i], b = (uint16_t)((bits >> i) & 1u);
+          const uint16_t z = (uint16_t)(b * x - r12);
+          o1[i] = z;
+          o2[i] = (uint16_t)(-((r02 + z) << 1) + b);
+          o3[i] = (uint16_t)(x - (r12 << 1));
+        }
+        u16x8_store(to_next + el, o1);
+        u16x8_store(P.corr_a + e, o2);
+        u16x8_store(P.corr_b + e, o3);
       }
-    } else {
-      // rand_12 = their rng, rand_02 = my rng; in_a = party 2's a share
-      const uint32_t bits = (uint32_t)(P.c_a[e >> 6] >> (e & 63)) & 0xffu;
-      u16x8_load(P.inbox + e, in);
+    }
+  }
+  __syncthreads();
+  // step 3: P0 writes corr_b from z
+  if (p == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int el = 8 * (k * 64 + lane);
+      const size_t e = e_base + el;
+      if (e >= n) break;
+      uint16_t prv[8], in[8], o1[8];
+      u16x8_load(prv16 + el, prv);
+      u16x8_load(inbox + el, in);
+      const uint32_t bits = (uint32_t)(P.c_b[e >> 6] >> (e & 63)) & 0xffu;
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        const uint16_t r12 = prv[i], r02 = own[i], x = in[i], b = (uint16_t)((bits >> i) & 1u);
-        const uint16_t z = (uint16_t)(b * x - r12);
-        o1[i] = z;
-        o2[i] = (uint16_t)(-((r02 + z) << 1) + b);
-        o3[i] = (uint16_t)(x - (r12 << 1));
+        const uint16_t b = (uint16_t)((bits >> i) & 1u), r02 = prv[i], z = in[i];
+        o1[i] = (uint16_t)(-((r02 + z) << 1) + b);
       }
-      u16x8_store(N.inbox + e, o1);
-      u16x8_store(P.corr_a + e, o2);
-      u16x8_store(P.corr_b + e, o3);
+      u16x8_store(P.corr_b + e, o1);
     }
   }
 }
@@ -253,15 +276,6 @@ struct AdderLds {
   uint32_t xch[2][3][2][64];   // [step parity][party][ripple, carry-save][lane]
 };
 
-__device__ __forceinline__ uint32_t csa_and32(int p, uint32_t a, uint32_t b) {
-  return p == 2 ? (a & ~b) : (a & b);
-}
-
-__device__ __forceinline__ uint32_t and_pre32(uint32_t xa, uint32_t xb, uint32_t ya, uint32_t yb,
-                                              uint32_t r) {
-  return (xa & ya) ^ (xb & ya) ^ (xa & yb) ^ r;
-}
-
 // Step S (last plane LAST). r16[2 (S % 8)] is the carry-save randomness of
 // plane S, r16[2 (S % 8) + 1] the ripple randomness of round S - 1; the lift's
 // carry-out randomness is r_out.
@@ -274,7 +288,7 @@ __device__ __forceinline__ void adder_step(const PlaneSlices<MSB>& W, AdderState
   constexpr int X = S & 1;
   if constexpr (S == 0) {
     const uint32_t na = W.a(0), nb = W.b(0);
-    const uint32_t cs = csa_and32(p, na, nb) ^ r16[0];
+    const uint32_t cs = csa_and(p, na, nb) ^ r16[0];
     L.xch[X][p][1][lane] = cs;
     __syncthreads();
     st.cb = L.xch[X][prev][1][lane];
@@ -287,8 +301,8 @@ __device__ __forceinline__ void adder_step(const PlaneSlices<MSB>& W, AdderState
     const uint32_t na = W.a(S), nb = W.b(S);
     st.xa ^= st.va;  // carry after round S - 2
     st.xb ^= st.vb;
-    const uint32_t va = and_pre32(na ^ st.xa, nb ^ st.xb, ba ^ st.xa, bb ^ st.xb, r16[2 * J + 1]);
-    const uint32_t cs = csa_and32(p, na, nb) ^ r16[2 * J];
+    const uint32_t va = and_pre(na ^ st.xa, nb ^ st.xb, ba ^ st.xa, bb ^ st.xb, r16[2 * J + 1]);
+    const uint32_t cs = csa_and(p, na, nb) ^ r16[2 * J];
     L.xch[X][p][0][lane] = va;
     L.xch[X][p][1][lane] = cs;
     __syncthreads();
@@ -303,7 +317,7 @@ __device__ __forceinline__ void adder_step(const PlaneSlices<MSB>& W, AdderState
       st.xa ^= st.va;
       st.xb ^= st.vb;
       const uint32_t ba2 = st.ca ^ (p == 2 ? st.ta : 0u), bb2 = st.cb ^ (p == 0 ? st.tb : 0u);
-      const uint32_t v2 = and_pre32(ba2, bb2, st.xa, st.xb, r_out);
+      const uint32_t v2 = and_pre(ba2, bb2, st.xa, st.xb, r_out);
       st.xa ^= ba2;
       st.xb ^= bb2;
       L.xch[X ^ 1][p][0][lane] = v2;
@@ -483,7 +497,6 @@ __global__ __launch_bounds__(256) void open_kernel(const OpenArgs a) {
 // ------------------------------------------------------------ launchers
 void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const RippleCtrs& ctrs,
                        hipStream_t s) {
-  IRIS_REQUIRE(t.party0 == 0, "the word adders run all three parties");
   IRIS_REQUIRE(t.W % 8 == 0, "bit planes must be a multiple of 8 words");
   const unsigned blocks = (unsigned)ceil_div(t.W, 32);
   if (msb)
@@ -493,9 +506,8 @@ void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const Ripp
   IRIS_HIP_CHECK(hipGetLastError());
 }
 
-void launch_inject(const Trio& t, int np, int step, uint64_t ctr, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(128 * t.W, 8192), (unsigned)np);
-  inject_kernel<<<grid, 256, 0, s>>>(t, step, ctr);
+void launch_inject(const Trio& t, uint64_t ctr, hipStream_t s) {
+  inject_kernel<<<(unsigned)ceil_div(128 * t.W, 2048), 192, 0, s>>>(t, ctr);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 

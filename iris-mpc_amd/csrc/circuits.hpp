@@ -6,16 +6,15 @@
 namespace irismpc {
 
 // One party's Circuits state (device pointers). W = words per bit plane
-// (elements / 64). Messages to the next party are written straight into that
-// party's "b" arrays (or its inbox), which is the device-to-device exchange
-// that replaces the reference's NCCL send/recv.
+// (elements / 64). The kernels run the three parties together (one wave per
+// party in a block), and a message to the next party reaches its wave through
+// LDS: that is the exchange which replaces the reference's NCCL send/recv.
 struct CircuitBufs {
   const uint16_t* code_a;
   const uint16_t* code_b;
   const uint16_t* mask_a;
   const uint16_t* mask_b;
   uint64_t *c_a, *c_b;      // 2 W: lift carries (bit 16, bit 17)
-  uint16_t* inbox;          // 128 W: bit-inject message received from prev
   uint16_t *corr_a, *corr_b;  // 128 W: injected carries (arithmetic u16 shares)
   uint64_t *res_a, *res_b;  // W: result bit shares (plane 0)
   uint32_t *lifted_a, *lifted_b;  // optional (debug): corrected lifted mask
@@ -26,7 +25,6 @@ struct Trio {
   CircuitBufs p[3];
   size_t W;
   uint64_t nonce;
-  int party0;  // first party of this launch (grid.y covers party0 .. party0+gridDim.y-1)
 };
 
 // Block counters of every RNG draw of compare_threshold_masked_many, following
@@ -53,12 +51,14 @@ struct RippleCtrs {
   uint64_t c[32];
 };
 
-// One launch per adder, all three parties (party0 must be 0): msb = false is
+// One launch per adder, all three parties: msb = false is
 // the lift (mask shares -> carries in c planes 0 and 1), msb = true the
 // lift_mul_sub and the MSB adder (-> res). The bit planes stay in registers.
 void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const RippleCtrs& ctrs,
                        hipStream_t s);
-void launch_inject(const Trio& t, int n_parties, int step, uint64_t ctr, hipStream_t s);
+// Bit injection of the two lift carries (all three parties): corrections
+// corr_a / corr_b of every element, the three message steps inside one launch.
+void launch_inject(const Trio& t, uint64_t ctr, hipStream_t s);
 
 // Open + bitmap (actor.rs:2928-2997, dot/kernel.cu:30-116). Party views:
 // a = own result.a, b = own result.b, c = prev party's result.b (received).

@@ -592,7 +592,6 @@ Circuits::Circuits(int device, int peer_id, size_t input_size, size_t alloc_size
   const size_t A = alloc_words_ + 8;
   c_a_.alloc(2 * A); c_b_.alloc(2 * A);
   res_a_.alloc(A); res_b_.alloc(A); open_c_.alloc(A);
-  inbox_.alloc(128 * A);
   corr_a_.alloc(128 * A); corr_b_.alloc(128 * A);
 }
 
@@ -606,7 +605,7 @@ CircuitBufs Circuits::bufs(const uint16_t* ca, const uint16_t* cb, const uint16_
   CircuitBufs b{};
   b.code_a = ca; b.code_b = cb; b.mask_a = ma; b.mask_b = mb;
   b.c_a = c_a_.ptr; b.c_b = c_b_.ptr;
-  b.inbox = inbox_.ptr; b.corr_a = corr_a_.ptr; b.corr_b = corr_b_.ptr;
+  b.corr_a = corr_a_.ptr; b.corr_b = corr_b_.ptr;
   b.res_a = res_a_.ptr; b.res_b = res_b_.ptr;
   b.lifted_a = lifted_dbg_a; b.lifted_b = lifted_dbg_b;
   b.key_own = seed_own_; b.key_prev = seed_prev_;
@@ -634,7 +633,6 @@ void Circuits::compare_trio(Circuits* const c[3], const uint16_t* const code_a[3
   }
   t.W = W;
   t.nonce = c[0]->nonce_;
-  t.party0 = 0;
   const CircuitSchedule sc = CircuitSchedule::make(c[0]->counter_, W, c[0]->alloc_words_);
   hipStream_t ps = ex->protocol_stream();
   ex->join_streams(s);
@@ -644,7 +642,7 @@ void Circuits::compare_trio(Circuits* const c[3], const uint16_t* const code_a[3
   for (int k = 1; k <= 14; k++) lift.c[k] = sc.lift_round[k];
   lift.c[15] = sc.lift_final;
   launch_adder_word(t, false, A, lift, ps);
-  for (int step = 1; step <= 3; step++) launch_inject(t, 3, step, sc.inject, ps);
+  launch_inject(t, sc.inject, ps);
   RippleCtrs msb{};
   msb.csa = sc.msb_csa;
   msb.c[0] = sc.msb_first;
