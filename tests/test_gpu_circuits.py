@@ -141,3 +141,13 @@ def test_threshold_three_party_threads(irismpc, orc):
     import torch
 
     _run_case(irismpc, orc, torch, W=256, alloc=256, A=20316, seed=12, threads=True)
+
+
+@pytest.mark.parametrize("W,alloc", [(264, 264), (40, 72)])
+def test_threshold_partial_blocks(irismpc, orc, W, alloc):
+    """Word counts that leave the last block of the word adders (32 words) and
+    of the bit injection (2,048 elements) partly empty, with the allocation
+    larger than the chunk in the second case."""
+    import torch
+
+    _run_case(irismpc, orc, torch, W=W, alloc=alloc, A=16384, seed=W)
