@@ -57,10 +57,10 @@ def _setup(irismpc, torch, W, alloc, counter=0):
     return ex, circ, streams
 
 
-def _run_case(irismpc, orc, torch, W, alloc, A, seed, counter=0, threads=False):
+def _run_case(irismpc, orc, torch, W, alloc, A, seed, counter=0, threads=False, sampler=None):
     rng = np.random.default_rng(seed)
     L = 64 * W
-    code, mask = _sample(rng, L)
+    code, mask = (sampler or _sample)(rng, L)
     ca, cb = _rep(rng, code)
     ma, mb = _rep(rng, mask)
     ex, circ, streams = _setup(irismpc, torch, W, alloc, counter)
@@ -151,3 +151,25 @@ def test_threshold_partial_blocks(irismpc, orc, W, alloc):
     import torch
 
     _run_case(irismpc, orc, torch, W=W, alloc=alloc, A=16384, seed=W)
+
+
+def _boundary_sample(A):
+    """Code/mask dots at the threshold: c = ceil(M A / 2^16) + d, d in
+    {-2, -1, 0, 1}, where msb(M A - (c << 16) - 1) flips between d = -1 and
+    d = 0 (the CloseToThreshold cases of the reference's e2e generator,
+    iris-mpc-common/src/test.rs:1181-1220, at the integer predicate)."""
+    def sample(rng, L):
+        mask = rng.integers(0, 12801, L).astype(np.int64)
+        d = rng.integers(-2, 2, L)
+        c = np.clip(-((-mask * A) // 65536) + d, -12800, 12800)
+        return (c & 0xFFFF).astype(np.uint16), mask.astype(np.uint16)
+    return sample
+
+
+@pytest.mark.parametrize("A", [20316, 16384])
+def test_threshold_boundary(irismpc, orc, A):
+    """Every element sits within two of the threshold; shares and opened bits
+    must equal the oracle's, so a lift or comparison off by one shows."""
+    import torch
+
+    _run_case(irismpc, orc, torch, W=256, alloc=256, A=A, seed=A, sampler=_boundary_sample(A))
