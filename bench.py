@@ -83,11 +83,10 @@ def cpu_baseline(n_target_s=12.0):
     }
 
 
-def run_c3(args):
-    """Full 3-party protocol on one GPU, both eyes: 3 party streams per eye, each
-    party holding a left- and a right-eye DB share of args.db rows in HBM; share
-    GEMM, reshare, lift, threshold circuits, open, OR over rotations, then AND
-    over eyes."""
+def _c3_engines(args):
+    """Both eyes' engines of the full protocol: 3 parties per eye, each party
+    holding the eye's DB share of args.db rows in HBM, synthetic irises with
+    planted queries (copies of DB irises, 10-30 % of code bits flipped)."""
     import numpy as np
     import torch
 
@@ -100,7 +99,7 @@ def run_c3(args):
     Qr, chunk = B * 31, irismpc.DB_CHUNK_SIZE
     U = min(N, 32768)  # distinct synthetic irises, tiled over the DB (row r = iris r mod U)
     rng = np.random.default_rng(3)
-    n_plant = B // 2   # planted queries: copies of DB irises with 10-30% of code bits flipped
+    n_plant = B // 2
     targets = rng.choice(U, size=n_plant, replace=False)
     flips = rng.integers(1280, 3841, size=n_plant)
     planted = [(j, int(t), int(f)) for j, t, f in zip(range(n_plant), targets, flips)]
@@ -125,6 +124,23 @@ def run_c3(args):
         qcs = [parties[p].codes.upload_query_shares(shares[p][0]) for p in range(3)]
         qms = [parties[p].masks.upload_query_shares(shares[p][1]) for p in range(3)]
         eyes.append(dict(ex=ex, parties=parties, qcs=qcs, qms=qms, bitmaps=bitmaps, streams=streams))
+    expect = np.full(B, 0xFFFFFFFF, np.uint32)
+    for j, t, _ in planted:
+        expect[j] = t  # first DB row holding iris t
+    return N, B, Qr, chunk, U, n_plant, eyes, expect
+
+
+def run_c3(args):
+    """Full 3-party protocol on one GPU, both eyes: 3 party streams per eye, each
+    party holding a left- and a right-eye DB share of args.db rows in HBM; share
+    GEMM, reshare, lift, threshold circuits, open, OR over rotations, then AND
+    over eyes."""
+    import numpy as np
+    import torch
+
+    import irismpc
+
+    N, B, Qr, chunk, U, n_plant, eyes, expect = _c3_engines(args)
     finals = [torch.zeros(B, dtype=torch.int32, device="cuda") for _ in range(3)]
     torch.cuda.synchronize()
 
@@ -159,9 +175,6 @@ def run_c3(args):
         batch()
     ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
     same = all(torch.equal(eyes[e]["bitmaps"][0], eyes[e]["bitmaps"][p]) for e in range(2) for p in (1, 2))
-    expect = np.full(B, 0xFFFFFFFF, np.uint32)
-    for j, t, _ in planted:
-        expect[j] = t  # first DB row holding iris t
     found = all(np.array_equal(f.cpu().numpy().view(np.uint32), expect) for f in finals)
 
     # share GEMM stage alone, same engines: code + mask GEMMs of all 3 parties x 2 eyes
@@ -206,6 +219,49 @@ def run_c3(args):
                             "note": f"ChaCha12 correlated randomness: {CHACHA_BLOCKS_PER_PAIR:.3f} blocks x "
                                     f"{CHACHA12_OPS} int32 ops per pair per party; time = batch minus "
                                     "GEMM-alone time"},
+    }
+
+
+def run_c3_two_stage(args):
+    """The reference's batch answer on the full-protocol engines: one eye's full
+    scan at the 0.375 prefilter, the candidate subset, both eyes against the
+    subset at 0.345, AND over eyes (actor.rs:939-1131, 1900-2244)."""
+    import numpy as np
+    import torch
+
+    from irismpc.two_stage import TwoStageMatcher
+
+    N, B, Qr, chunk, U, n_plant, eyes, expect = _c3_engines(args)
+    matcher = TwoStageMatcher({"left": eyes[0]["parties"], "right": eyes[1]["parties"]}, N, B)
+    queries = {"left": (eyes[0]["qcs"], eyes[0]["qms"]), "right": (eyes[1]["qcs"], eyes[1]["qms"])}
+    torch.cuda.synchronize()
+    res = None
+    for _ in range(args.warmup):
+        res = matcher.match(queries, N, B)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = matcher.match(queries, N, B)
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    exact = all(np.array_equal(f, expect) for f in res.final_results)
+    return {
+        "metric": "3-party two-stage batch on 1 MI355X: DB-row decisions per second",
+        "value": 2 * Qr * N / (ms_step / 1e3),
+        "unit": "comparisons/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8 MFMA (i32 acc) -> u16 shares; u32 lift; GF(2) bit-sliced circuits",
+        "data": f"synthetic irises GR-shared for 3 parties on the device; {U:,} distinct irises per eye tiled "
+                f"over the DB; {n_plant} of {B} queries are copies of DB irises with 10-30% of code bits "
+                "flipped, matched in both eyes",
+        "config": {"workload": f"two-stage flow of the reference's batch: left eye full scan at 0.375 over "
+                               f"{N:,} rows, candidate subset, both eyes on the candidates at 0.345, AND over "
+                               f"eyes; 3 parties, {B} queries x 31 rotations; value counts the 2 eyes x "
+                               "query-rotation rows x DB rows pairs the batch decides, the right eye only "
+                               "computing the candidates",
+                   "db_rows": N, "query_rows": Qr, "candidates": int(res.candidates.size),
+                   "final_results_exact": exact},
+        "roofline": None,
     }
 
 
@@ -464,11 +520,12 @@ def main():
     ap.add_argument("--variant", type=int, default=-1,
                     help="GEMM kernel variant (-1: automatic, the rotation-aware kernel)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "circuits"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c3-two-stage", "c4", "c5", "circuits"), default="c2",
                     help="c2: per-party share GEMM (the headline line); c3: full 3-party protocol, "
                          "both eyes; c4: batching form (use --db 4194304 for the full-size DB); "
                          "c5: serving form (use --db 2097152 for the full-size DB); circuits: the "
-                         "threshold circuits alone at one chunk")
+                         "threshold circuits alone at one chunk; c3-two-stage: the reference's batch flow "
+                         "(one-eye prefilter scan, both eyes on the candidates)")
     ap.add_argument("--rates", default="2,5,10,20,40", help="c5: request rates per second")
     ap.add_argument("--seconds", type=float, default=8.0, help="c5: arrival window per rate")
     ap.add_argument("--plain-requests", type=int, default=8, help="c5: requests timed in the plain form")
@@ -484,6 +541,9 @@ def main():
         return
     if args.workload == "circuits":
         print(json.dumps(run_circuits(args)))
+        return
+    if args.workload == "c3-two-stage":
+        print(json.dumps(run_c3_two_stage(args)))
         return
     if args.workload == "c4":
         print(json.dumps(run_c4(args)))

@@ -415,7 +415,6 @@ void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, siz
   GemmArgs a = gemm_args(q, db, variant);
   const GemmTile t = gemm_tile(variant);
   a.out = results();
-  a.out_peer = nullptr;
   a.out_chunk_stride = 0;
   a.n_dtiles = (int)ceil_div(chunk, (size_t)t.bd);
   a.db_begin = offset;

@@ -450,11 +450,6 @@ __device__ __forceinline__ void epilogue_store(const GemmArgs& a, const uint16_t
     uint4* dst = reinterpret_cast<uint4*>(out + c * stride + idx);
 #pragma unroll
     for (int k = 0; k < 4; k++) dst[k] = uint4{v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]};
-    if (a.out_peer) {
-      uint4* pd = reinterpret_cast<uint4*>(a.out_peer + c * a.out_chunk_stride + idx);
-#pragma unroll
-      for (int k = 0; k < 4; k++) pd[k] = uint4{v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]};
-    }
   }
 }
 
@@ -1318,7 +1313,6 @@ void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s) {
   IRIS_REQUIRE(!is_rot_variant(variant) || (a.qrot != nullptr && (a.n_kt * kKTile) % 800 == 0),
                "rotation-aware variant needs a query uploaded from raw shares");
   IRIS_REQUIRE(a.n_seg >= 0 && a.n_seg <= kMaxSegments, "too many batches in one packed launch");
-  IRIS_REQUIRE(a.n_seg == 0 || a.out_peer == nullptr, "packed launches write no peer copy");
   IRIS_REQUIRE(a.n_dtiles == (int)ceil_div(a.db_end - a.db_begin, (size_t)t.bd), "n_dtiles mismatch");
   const int T = a.n_qtiles * a.n_dtiles;
   const unsigned grid = (unsigned)(8 * ((T + 7) / 8));

@@ -35,7 +35,6 @@ struct GemmArgs {
   const uint32_t* dsum0;
   const uint32_t* dsum1;
   uint16_t* out;        // results, chunk c at out + c*out_chunk_stride
-  uint16_t* out_peer;   // optional copy written straight into the next party's inbox
   size_t out_chunk_stride;
   int n_kt;             // K / 64
   int qr;               // query rows produced (query_length)
