@@ -26,6 +26,7 @@ The opened bits do not depend on the mix (the masks do), so each result
 equals the result of the same request run alone (ServingEngines.run_alone,
 the plain per-request form).
 """
+import collections
 import queue
 import threading
 import time
@@ -114,6 +115,14 @@ class ServingEngines:
         self.slot_maps = [torch.zeros(max_active, dtype=torch.int32, device="cuda") for _ in range(3)]
         self.lock = threading.Lock()  # DB writes vs scan steps
         self._dummy = None
+        self._slots = None  # active slots the device slot maps hold
+        self._inflight = collections.deque()  # per queued scan step: one event per party stream
+        self._xstreams = [torch.cuda.ExternalStream(s) for s in self.scan.streams]
+
+    def sync_scan(self):
+        """Waits for every queued scan step."""
+        self.scan.sync()
+        self._inflight.clear()
 
     # ------------------------------------------------------------------ DB
     def build(self, n_unique, dataset, rand_dataset):
@@ -121,6 +130,7 @@ class ServingEngines:
 
     def insert(self, shares):
         with self.lock:
+            self.sync_scan()  # no queued step may read the rows being written
             row = self.n_rows
             assert row < self.capacity - CHUNK, "DB capacity exhausted"
             for p in range(3):
@@ -133,6 +143,7 @@ class ServingEngines:
         if self._dummy is None:
             self._dummy = dummy_shares()
         with self.lock:
+            self.sync_scan()
             for p in range(3):
                 self.code_dbs[p].load_records(row, self._dummy[p][0].reshape(1, -1))
                 self.mask_dbs[p].load_records(row, self._dummy[p][1].reshape(1, -1))
@@ -141,7 +152,10 @@ class ServingEngines:
     # ------------------------------------------------------------------ queries
     def _query_sets(self, ps, shares_list, rows, reuse=None):
         """The query sets of the three parties; `reuse` (earlier sets made with
-        at least `rows` rows) is refilled in place instead of reallocated."""
+        at least `rows` rows) is refilled in place instead of reallocated, once
+        the queued steps that read them have finished."""
+        if reuse is not None:
+            self.sync_scan()
         qc, qm = [], []
         for p in range(3):
             code = np.stack([s[p][0] for s in shares_list])
@@ -157,20 +171,40 @@ class ServingEngines:
     def n_chunks(self, rows):
         return max(1, -(-rows // CHUNK))
 
-    def step(self, qc, qm, batch, chunk_index, db_rows, slots):
-        """One chunk step of the scan engines for the active query set."""
-        for p in range(3):
-            self.slot_maps[p][:len(slots)].copy_(self.torch.tensor(slots, dtype=self.torch.int32))
-        self.torch.cuda.synchronize()
+    def step(self, qc, qm, batch, chunk_index, db_rows, slots, max_queued=2):
+        """Queues one chunk step of the scan engines for the active query set.
+        At most `max_queued` steps wait on the scan streams, so the next step
+        is queued while one runs; the device slot maps are rewritten only when
+        the active slots change, after the steps that read them."""
+        slots = list(slots)
+        if slots != self._slots:
+            self.sync_scan()
+            for p in range(3):
+                self.slot_maps[p][:len(slots)].copy_(self.torch.tensor(slots, dtype=self.torch.int32))
+            self.torch.cuda.synchronize()  # the copies (and any slot clearing) precede the step
+            self._slots = slots
+        while len(self._inflight) >= max_queued:
+            for ev in self._inflight.popleft():
+                ev.synchronize()
         E.scan_chunk_trio(self.scan_parties, qc, qm, chunk_index * CHUNK, db_rows, batch, self.A,
                           [b.data_ptr() for b in self.bitmaps], self.scan.streams,
                           [m.data_ptr() for m in self.slot_maps], row_bits=self.words * 64)
+        evs = []
+        for xs in self._xstreams:
+            ev = self.torch.cuda.Event()
+            ev.record(xs)
+            evs.append(ev)
+        self._inflight.append(evs)
 
     def clear_slot(self, slot):
+        """Zeroes a free slot's bitmap rows (no queued step writes them); the
+        fill runs on torch's stream, so it is waited for before any scan step."""
         for b in self.bitmaps:
             b[slot * self.words:(slot + 1) * self.words].zero_()
+        self.torch.cuda.synchronize()
 
     def slot_matches(self, slot, db_rows):
+        self.sync_scan()
         bits = self.bitmaps[0][slot * self.words:(slot + 1) * self.words].cpu().numpy().view(np.uint8)
         rows = np.nonzero(np.unpackbits(bits, bitorder="little")[:db_rows])[0]
         return [int(r) for r in rows]
@@ -182,10 +216,11 @@ class ServingEngines:
         for b in self.small_bitmaps:
             b.zero_()
         self.torch.cuda.synchronize()
-        # a 64-row window: the scan chunk of [off, off + 64) of a DB ending at off + 64
-        E.scan_chunk_trio(self.small_parties, qc, qm, off, off + 64, 1, self.A,
-                          [b.data_ptr() for b in self.small_bitmaps], self.small.streams)
-        self.small.sync()
+        with self.lock:  # the DB rows are not written meanwhile
+            # a 64-row window: the scan chunk of [off, off + 64) of a DB ending at off + 64
+            E.scan_chunk_trio(self.small_parties, qc, qm, off, off + 64, 1, self.A,
+                              [b.data_ptr() for b in self.small_bitmaps], self.small.streams)
+            self.small.sync()
         bits = self.small_bitmaps[0][: (off + 64) // 64 + 1].cpu().numpy().view(np.uint8)
         return bool(np.unpackbits(bits, bitorder="little")[target])
 
@@ -196,7 +231,6 @@ class ServingEngines:
         self.clear_slot(0)
         for c in range(self.n_chunks(rows)):
             self.step(qc, qm, 1, c, rows, [0])
-        self.scan.sync()
         return self.slot_matches(0, rows)
 
 
@@ -292,7 +326,6 @@ class Server:
             cursor %= n_chunks
             with eng.lock:
                 eng.step(qsets[0], qsets[1], len(active), cursor, eng.n_rows, [r.slot for r in active])
-                eng.scan.sync()
             self.steps += 1
             for r in active:
                 r.covered.add(cursor)
