@@ -315,3 +315,49 @@ def test_anon_distance_capture(irismpc, orc):
             j, tt = divmod(row, 31)
             assert code[k] == (M[j, tt, d] - 2 * H[j, tt, d]) & 0xFFFF
             assert mask[k] == M[j, tt, d] & 0xFFFF
+
+
+def test_pipelined_scan_partial_last_chunk(irismpc):
+    """The chunk loop with two result slots (GEMMs of chunk c + 1 beside the
+    circuits of chunk c) over four chunks, the last one partial, equals a
+    chunk-by-chunk scan and the planted answer, bit for bit."""
+    import torch
+
+    from irismpc import synth
+
+    N, U, B = 3 * 32768 + 1000, 64, 16
+    Qr = B * 31
+    rng = np.random.default_rng(9)
+    targets = rng.choice(U, size=10, replace=False)
+    ex, parties = _engines(irismpc, N, Qr, tag_base=6)
+    synth.build_db([pe.code_db for pe in parties], [pe.mask_db for pe in parties], N, U, 80, 110)
+    planted = [(j, int(targets[j]), 1000 + 50 * j) for j in range(10)]
+    _, _, sh = synth.make_queries(B, planted, 80, 100, 90, 120)
+    qc = [parties[p].codes.upload_query_shares(sh[p][0]) for p in range(3)]
+    qm = [parties[p].masks.upload_query_shares(sh[p][1]) for p in range(3)]
+    streams = [irismpc.stream_create() for _ in range(3)]
+    rows = np.arange(N)
+    want = np.zeros((B, N), bool)
+    for j, t, _ in planted:
+        want[j] = (rows % U) == t
+
+    def bits(bm):
+        return np.unpackbits(bm.cpu().numpy().view(np.uint8), bitorder="little")[: B * N].reshape(B, N)
+
+    maps = []
+    for pipelined in (True, False):
+        bms = [torch.zeros(N * B // 64 + 2, dtype=torch.int64, device="cuda") for _ in range(3)]
+        torch.cuda.synchronize()
+        ptrs = [b.data_ptr() for b in bms]
+        if pipelined:
+            irismpc.scan_db_trio(parties, qc, qm, N, B, irismpc.THRESHOLD_ANON_STATS_A, ptrs, streams)
+        else:
+            for off in range(0, N, irismpc.DB_CHUNK_SIZE):
+                irismpc.scan_chunk_trio(parties, qc, qm, off, N, B, irismpc.THRESHOLD_ANON_STATS_A, ptrs, streams)
+        for s in streams:
+            irismpc.stream_synchronize(s)
+        got = [bits(b) for b in bms]
+        assert all(np.array_equal(g, got[0]) for g in got), "parties opened different bits"
+        maps.append(got[0])
+    assert np.array_equal(maps[0], maps[1])
+    assert np.array_equal(maps[0].astype(bool), want)
