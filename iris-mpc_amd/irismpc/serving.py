@@ -113,11 +113,13 @@ class ServingEngines:
         self.bitmaps = [torch.zeros(max_active * words + 2, dtype=torch.int64, device="cuda") for _ in range(3)]
         self.small_bitmaps = [torch.zeros(words + 2, dtype=torch.int64, device="cuda") for _ in range(3)]
         self.slot_maps = [torch.zeros(max_active, dtype=torch.int32, device="cuda") for _ in range(3)]
-        self.lock = threading.Lock()  # DB writes vs scan steps
+        self.lock = threading.Lock()  # DB writes vs the reauth worker's reads
         self._dummy = None
         self._slots = None  # active slots the device slot maps hold
         self._inflight = collections.deque()  # per queued scan step: one event per party stream
         self._xstreams = [torch.cuda.ExternalStream(s) for s in self.scan.streams]
+        self._small_xstreams = [torch.cuda.ExternalStream(s) for s in self.small.streams]
+        self._small_qsets = None  # the reauth engines' query sets, refilled per request
 
     def sync_scan(self):
         """Waits for every queued scan step."""
@@ -155,7 +157,10 @@ class ServingEngines:
         at least `rows` rows) is refilled in place instead of reallocated, once
         the queued steps that read them have finished."""
         if reuse is not None:
-            self.sync_scan()
+            if ps is self.scan:
+                self.sync_scan()
+            else:
+                ps.sync()
         qc, qm = [], []
         for p in range(3):
             code = np.stack([s[p][0] for s in shares_list])
@@ -211,11 +216,15 @@ class ServingEngines:
 
     def reauth(self, shares, target):
         """One query x 31 rotations against DB row `target` on the small engine set."""
-        qc, qm = self._query_sets(self.small, [shares], ROT)
+        # refilled in place: an allocation or free would synchronise the whole
+        # device and wait for the scan steps queued by the request loop
+        if self._small_qsets is None:
+            self._small_qsets = self._query_sets(self.small, [shares], ROT)
+        qc, qm = self._query_sets(self.small, [shares], ROT, reuse=self._small_qsets)
         off = target // 64 * 64
-        for b in self.small_bitmaps:
-            b.zero_()
-        self.torch.cuda.synchronize()
+        for b, xs in zip(self.small_bitmaps, self._small_xstreams):
+            with self.torch.cuda.stream(xs):  # ordered before this party's scan step
+                b.zero_()
         with self.lock:  # the DB rows are not written meanwhile
             # a 64-row window: the scan chunk of [off, off + 64) of a DB ending at off + 64
             E.scan_chunk_trio(self.small_parties, qc, qm, off, off + 64, 1, self.A,
@@ -324,8 +333,9 @@ class Server:
                 active_ids = ids
             n_chunks = eng.n_chunks(eng.n_rows)
             cursor %= n_chunks
-            with eng.lock:
-                eng.step(qsets[0], qsets[1], len(active), cursor, eng.n_rows, [r.slot for r in active])
+            # no lock: scan steps only read the DB, and the DB writes (insert,
+            # delete) run on this thread after the queued steps finished
+            eng.step(qsets[0], qsets[1], len(active), cursor, eng.n_rows, [r.slot for r in active])
             self.steps += 1
             for r in active:
                 r.covered.add(cursor)
