@@ -83,6 +83,61 @@ def cpu_baseline(n_target_s=12.0):
     }
 
 
+def cpu_baseline_protocol(n_target_s=12.0):
+    """Oracle restatement of the full 3-party step on a bounded sample: the three
+    parties' rotation-aware code and mask dot shares (the reference CPU dot
+    loop) and the threshold circuits of all three parties (lift, bit injection,
+    MSB adder, the same schedule as the GPU), for one query x 31 rotations
+    against 16,384 DB rows per party; the circuits run as independent parts on
+    the threads, which ctypes calls release to."""
+    import threading
+
+    import numpy as np
+
+    from oracle import orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rng = np.random.default_rng(8)
+    n = 16384
+    dc = rng.integers(0, 1 << 16, (n, 12800), dtype=np.uint16)
+    dm = rng.integers(0, 1 << 16, (n, 6400), dtype=np.uint16)
+    qc = rng.integers(0, 1 << 16, 12800, dtype=np.uint16)
+    qm = rng.integers(0, 1 << 16, 6400, dtype=np.uint16)
+    L = 31 * n
+    words = L // 64
+    parts = max(1, min(threads, words // 8))
+    bounds = [(words * i // parts) * 64 for i in range(parts + 1)]
+    shares = [rng.integers(0, 1 << 16, L, dtype=np.uint16) for _ in range(4)]
+    so = np.array([[p] * 8 for p in range(3)], np.uint32)
+    sp = np.array([[(p + 2) % 3] * 8 for p in range(3)], np.uint32)
+
+    def circuits(lo, hi):
+        v = [x[lo:hi] for x in shares]
+        orc.threshold([v[0]] * 3, [v[1]] * 3, [v[2]] * 3, [v[3]] * 3, (hi - lo) // 64, 16384, so, sp)
+
+    steps, elapsed = 0, 0.0
+    while elapsed < n_target_s and steps < 64:
+        t0 = time.perf_counter()
+        for _ in range(3):  # each party's own dot shares
+            orc.cpu_rotation_aware_dot(qc, qm, dc, dm, threads=threads)
+        ts = [threading.Thread(target=circuits, args=(bounds[i], bounds[i + 1])) for i in range(parts)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        elapsed += time.perf_counter() - t0
+        steps += 1
+    return {
+        "value": steps * L / elapsed,
+        "unit": "comparisons/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} x (1 query x 31 rotations vs {n} DB rows, one eye): 3 parties' code + mask dot "
+                  f"shares and the 3-party threshold circuits = {steps * L} comparisons in {elapsed:.1f} s "
+                  f"on {threads} threads",
+    }
+
+
 def _c3_engines(args):
     """Both eyes' engines of the full protocol: 3 parties per eye, each party
     holding the eye's DB share of args.db rows in HBM, synthetic irises with
@@ -219,6 +274,7 @@ def run_c3(args):
                             "note": f"ChaCha12 correlated randomness: {CHACHA_BLOCKS_PER_PAIR:.3f} blocks x "
                                     f"{CHACHA12_OPS} int32 ops per pair per party; time = batch minus "
                                     "GEMM-alone time"},
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_protocol(),
     }
 
 
