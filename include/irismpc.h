@@ -382,6 +382,21 @@ int irismpc_scan_chunk_trio(irismpc_share_db* const codes[3], irismpc_share_db* 
                             size_t offset, size_t db_size, size_t batch_size, uint32_t threshold_a,
                             uint64_t* const bitmaps[3], const uint32_t* const slot_of[3],
                             size_t row_bits, void* const streams[3]);
+/* n_chunks (1 .. 8) consecutive chunk steps from `offset` in one step: each
+ * chunk's dot shares sit side by side in the engines' result buffers
+ * (n_chunks x 32,768 x rows must fit them), one reshare and one threshold
+ * comparison cover them, and each chunk opens into the bitmaps as above. The
+ * opened bits equal those of n_chunks single steps; the circuit randomness
+ * follows one call's schedule. No reference counterpart: the reference's
+ * loop body is one chunk; a request loop with few active queries uses this to
+ * fill the GPU with fewer, larger launches. */
+int irismpc_scan_chunks_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
+                             irismpc_circuits* const circuits[3], irismpc_comparator* const cmp[3],
+                             const irismpc_db* const code_db[3], const irismpc_db* const mask_db[3],
+                             const irismpc_query* const q_code[3], const irismpc_query* const q_mask[3],
+                             size_t offset, size_t n_chunks, size_t db_size, size_t batch_size,
+                             uint32_t threshold_a, uint64_t* const bitmaps[3],
+                             const uint32_t* const slot_of[3], size_t row_bits, void* const streams[3]);
 /* The DB half of ServerActor::compare_query_against_db_subset_and_self
  * (actor.rs:1900-2244) for the three parties: the subset ids[p][0..n_ids)
  * (device u32 DB row ids, e.g. irismpc_comparator_partial_ids) are gathered

@@ -18,36 +18,67 @@ static DotViews dot_views(const PartyScan& p) {
   return d;
 }
 
-void scan_chunk(const PartyScan& p, size_t offset, size_t db_size, size_t batch_size,
-                uint32_t threshold_a, uint64_t* bitmap, const uint32_t* slot_of, size_t row_bits,
-                hipStream_t s) {
+// Several consecutive chunks in one step: each chunk's GEMMs write its own
+// region of the result buffer ([rows][chunk width], side by side), one reshare
+// and one compare cover them all, and each chunk opens from its own result
+// words. The opened bits are those of one step per chunk (the circuit masks
+// differ). With a small query set this fills the GPU with fewer, larger
+// launches.
+void scan_chunks(const PartyScan& p, size_t offset, size_t n_chunks, size_t db_size,
+                 size_t batch_size, uint32_t threshold_a, uint64_t* bitmap,
+                 const uint32_t* slot_of, size_t row_bits, hipStream_t s) {
   const size_t rows = (size_t)p.code_query->rows;
   IRIS_REQUIRE((size_t)p.mask_query->rows == rows, "code/mask query sets disagree on rows");
   IRIS_REQUIRE(batch_size * kRotations <= rows, "batch larger than the query rows");
-  // actor.rs:2286-2291, 2318-2533: the chunk covers rows [offset, offset + real)
-  const size_t real = db_size > offset ? std::min<size_t>(kDbChunk, db_size - offset) : 0;
+  IRIS_REQUIRE(n_chunks >= 1 && n_chunks <= kMaxStepChunks, "1 .. 8 chunks per step");
   // actor.rs:2331-2337 pads to 64; the circuits need a multiple of 8 words
   const size_t pad = rows % 8 == 0 ? 64 : 512;
-  const size_t dot_chunk = round_up(std::max<size_t>(real, 1), pad);
-  p.codes->dot_reduce(*p.code_query, *p.code_db, dot_chunk, offset, 1, s);
-  p.masks->dot_reduce(*p.mask_query, *p.mask_db, dot_chunk, offset, 2, s);
+  size_t real[kMaxStepChunks], width[kMaxStepChunks], start[kMaxStepChunks];
+  size_t total = 0, n = 0;
+  for (size_t c = 0; c < n_chunks; c++) {
+    const size_t off = offset + c * kDbChunk;
+    // actor.rs:2286-2291, 2318-2533: the chunk covers rows [off, off + real)
+    real[c] = db_size > off ? std::min<size_t>(kDbChunk, db_size - off) : 0;
+    if (c > 0 && real[c] == 0) break;  // nothing past the DB's end
+    width[c] = round_up(std::max<size_t>(real[c], 1), pad);
+    start[c] = total;
+    total += width[c];
+    n = c + 1;
+  }
+  for (size_t c = 0; c < n; c++) {
+    p.codes->dot_reduce(*p.code_query, *p.code_db, width[c], offset + c * kDbChunk, 1, s, start[c] * rows);
+    p.masks->dot_reduce(*p.mask_query, *p.mask_db, width[c], offset + c * kDbChunk, 2, s, start[c] * rows);
+  }
   // the distance capture reads the peer shares on this party's stream after
   // the circuits: they need a copy the peer's next GEMM cannot overwrite
   p.codes->peer_copy = p.masks->peer_copy = p.comparator->anon_capture_enabled();
-  p.codes->reshare_results(dot_chunk, s);
-  p.masks->reshare_results(dot_chunk, s);
-  const size_t len = dot_chunk * rows;
+  p.codes->reshare_results(total, s);
+  p.masks->reshare_results(total, s);
+  const size_t len = total * rows;
   p.circuits->set_chunk_size(len / 64);
   p.circuits->compare_threshold_masked_many(p.codes->results(), p.codes->results_peer(),
                                             p.masks->results(), p.masks->results_peer(), len,
                                             threshold_a, s);
   p.circuits->open_exchange(s);
-  if (real > 0) {
-    const DotViews dots = dot_views(p);
-    p.comparator->open_results(p.circuits->result_a(), p.circuits->result_b(),
-                               p.circuits->result_c(), bitmap, dot_chunk, real, offset,
-                               row_bits ? row_bits : db_size, batch_size, s, rows, slot_of, &dots);
+  for (size_t c = 0; c < n; c++) {
+    if (real[c] == 0) continue;
+    const size_t e0 = start[c] * rows, w0 = e0 / 64;
+    DotViews dots = dot_views(p);
+    dots.code_a += e0;
+    dots.code_b += e0;
+    dots.mask_a += e0;
+    dots.mask_b += e0;
+    p.comparator->open_results(p.circuits->result_a() + w0, p.circuits->result_b() + w0,
+                               p.circuits->result_c() + w0, bitmap, width[c], real[c],
+                               offset + c * kDbChunk, row_bits ? row_bits : db_size, batch_size, s,
+                               rows, slot_of, &dots);
   }
+}
+
+void scan_chunk(const PartyScan& p, size_t offset, size_t db_size, size_t batch_size,
+                uint32_t threshold_a, uint64_t* bitmap, const uint32_t* slot_of, size_t row_bits,
+                hipStream_t s) {
+  scan_chunks(p, offset, 1, db_size, batch_size, threshold_a, bitmap, slot_of, row_bits, s);
 }
 
 void scan_subset(const PartyScan& p, DbSlice& code_sub, DbSlice& mask_sub, const uint32_t* ids,
@@ -256,6 +287,15 @@ void scan_chunk_trio(const PartyScan p[3], size_t offset, size_t db_size, size_t
   run_trio([&](int i) {
     scan_chunk(p[i], offset, db_size, batch_size, threshold_a, bitmaps[i],
                slot_of ? slot_of[i] : nullptr, row_bits, s[i]);
+  });
+}
+
+void scan_chunks_trio(const PartyScan p[3], size_t offset, size_t n_chunks, size_t db_size,
+                      size_t batch_size, uint32_t threshold_a, uint64_t* const bitmaps[3],
+                      const uint32_t* const slot_of[3], size_t row_bits, hipStream_t const s[3]) {
+  run_trio([&](int i) {
+    scan_chunks(p[i], offset, n_chunks, db_size, batch_size, threshold_a, bitmaps[i],
+                slot_of ? slot_of[i] : nullptr, row_bits, s[i]);
   });
 }
 

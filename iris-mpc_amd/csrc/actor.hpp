@@ -38,6 +38,15 @@ void scan_chunk(const PartyScan& p, size_t offset, size_t db_size, size_t batch_
 void scan_chunk_trio(const PartyScan p[3], size_t offset, size_t db_size, size_t batch_size,
                      uint32_t threshold_a, uint64_t* const bitmaps[3],
                      const uint32_t* const slot_of[3], size_t row_bits, hipStream_t const s[3]);
+// n_chunks consecutive chunks from `offset` in one step (their results side by
+// side in the result buffer: n_chunks x 32,768 x rows must fit it).
+constexpr size_t kMaxStepChunks = 8;
+void scan_chunks(const PartyScan& p, size_t offset, size_t n_chunks, size_t db_size,
+                 size_t batch_size, uint32_t threshold_a, uint64_t* bitmap,
+                 const uint32_t* slot_of, size_t row_bits, hipStream_t s);
+void scan_chunks_trio(const PartyScan p[3], size_t offset, size_t n_chunks, size_t db_size,
+                      size_t batch_size, uint32_t threshold_a, uint64_t* const bitmaps[3],
+                      const uint32_t* const slot_of[3], size_t row_bits, hipStream_t const s[3]);
 
 // The DB half of compare_query_against_db_subset_and_self (actor.rs:1900-2244):
 // the subset rows ids[0..n_ids) (device u32, DB row ids below total_db_len)

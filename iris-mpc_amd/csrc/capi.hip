@@ -654,6 +654,25 @@ int irismpc_scan_chunk_trio(irismpc_share_db* const codes[3], irismpc_share_db* 
   });
 }
 
+int irismpc_scan_chunks_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
+                             irismpc_circuits* const circuits[3], irismpc_comparator* const cmp[3],
+                             const irismpc_db* const code_db[3], const irismpc_db* const mask_db[3],
+                             const irismpc_query* const q_code[3], const irismpc_query* const q_mask[3],
+                             size_t offset, size_t n_chunks, size_t db_size, size_t batch_size,
+                             uint32_t threshold_a, uint64_t* const bitmaps[3],
+                             const uint32_t* const slot_of[3], size_t row_bits, void* const streams[3]) {
+  return guarded([&] {
+    PartyScan p[3];
+    hipStream_t s[3];
+    for (int i = 0; i < 3; i++) {
+      p[i] = party_scan(codes[i], masks[i], circuits[i], cmp[i], code_db[i], mask_db[i], q_code[i],
+                        q_mask[i]);
+      s[i] = st(streams[i]);
+    }
+    scan_chunks_trio(p, offset, n_chunks, db_size, batch_size, threshold_a, bitmaps, slot_of, row_bits, s);
+  });
+}
+
 int irismpc_scan_subset_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
                              irismpc_circuits* const circuits[3], irismpc_comparator* const cmp[3],
                              const irismpc_db* const code_db[3], const irismpc_db* const mask_db[3],

@@ -403,18 +403,19 @@ static GemmArgs gemm_args(const QuerySet& q, const DbSlice& db, int variant) {
 }
 
 void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
-                         uint16_t multiplier, hipStream_t s) {
+                         uint16_t multiplier, hipStream_t s, size_t out_offset) {
   DeviceGuard g(device_);
   IRIS_REQUIRE(q.code_length == code_length_ && db.code_length == code_length_, "code length mismatch");
   IRIS_REQUIRE(q.rows >= 1 && (size_t)q.rows <= query_length_, "query rows exceed query_length");
   IRIS_REQUIRE(chunk > 0 && chunk % 64 == 0, "chunk size must be a positive multiple of 64");
-  IRIS_REQUIRE(chunk <= max_db_length_, "chunk larger than the engine's result buffer");
+  IRIS_REQUIRE(out_offset + chunk * (size_t)q.rows <= results_.n,
+               "chunk larger than the engine's result buffer");
   IRIS_REQUIRE(offset % 16 == 0, "offset must be a multiple of 16");
   IRIS_REQUIRE(offset + chunk <= db.capacity - kMaxBD, "chunk beyond DB capacity");
   const int variant = resolve_variant(q);
   GemmArgs a = gemm_args(q, db, variant);
   const GemmTile t = gemm_tile(variant);
-  a.out = results();
+  a.out = results() + out_offset;
   a.out_chunk_stride = 0;
   a.n_dtiles = (int)ceil_div(chunk, (size_t)t.bd);
   a.db_begin = offset;
@@ -523,7 +524,7 @@ struct ShareDBTrio {
     const size_t bytes = chunk * d[0]->last_rows_ * 2;
     for (int p = 0; p < 3; p++) {
       IRIS_REQUIRE(d[p]->peer_id_ == p, "trio members must be parties 0, 1, 2 in order");
-      IRIS_REQUIRE(d[p]->last_rows_ == d[0]->last_rows_ && chunk <= d[p]->max_db_length_,
+      IRIS_REQUIRE(d[p]->last_rows_ == d[0]->last_rows_ && chunk * d[p]->last_rows_ <= d[p]->results_.n,
                    "reshare geometry mismatch");
     }
     const bool copy = d[0]->peer_copy || d[1]->peer_copy || d[2]->peer_copy;

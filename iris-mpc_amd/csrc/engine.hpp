@@ -176,8 +176,10 @@ class ShareDB {
                            hipStream_t s) const;
   // dot + dot_reduce_and_multiply (fused): DB rows [offset, offset + chunk)
   // against all query rows; chunk is the padded chunk size (multiple of 64).
+  // results land at results() + out_offset (elements): several chunks of a
+  // small query set can sit side by side in the result buffer
   void dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
-                  uint16_t multiplier, hipStream_t s);
+                  uint16_t multiplier, hipStream_t s, size_t out_offset = 0);
   // Multi-chunk single launch: rows [offset, offset + total) in chunks of
   // `chunk` rows, results written chunk after chunk into `out`.
   void dot_reduce_range(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,

@@ -507,6 +507,20 @@ def scan_chunk_trio(parties, q_codes, q_masks, offset, db_size, batch_size, thre
          row_bits, ptr_array(list(streams)))
 
 
+def scan_chunks_trio(parties, q_codes, q_masks, offset, n_chunks, db_size, batch_size, threshold_a,
+                     bitmap_ptrs, streams, slot_ptrs=None, row_bits=0):
+    """n_chunks consecutive chunk steps from `offset` as one step (the dot
+    shares of all of them fit the engines' result buffers); the opened bits
+    are those of n_chunks calls of scan_chunk_trio."""
+    call("irismpc_scan_chunks_trio", ptr_array([p.codes.h for p in parties]),
+         ptr_array([p.masks.h for p in parties]), ptr_array([p.circuits.h for p in parties]),
+         ptr_array([p.comparator.h for p in parties]), ptr_array([p.code_db.h for p in parties]),
+         ptr_array([p.mask_db.h for p in parties]), ptr_array([q.h for q in q_codes]),
+         ptr_array([q.h for q in q_masks]), offset, n_chunks, db_size, batch_size, threshold_a,
+         ptr_array(list(bitmap_ptrs)), ptr_array(list(slot_ptrs)) if slot_ptrs is not None else None,
+         row_bits, ptr_array(list(streams)))
+
+
 def scan_subset_trio(parties, code_subs, mask_subs, q_codes, q_masks, ids_ptrs, n_ids, total_db_len,
                      batch_size, threshold_a, bitmap_ptrs, streams, anon_threshold=0):
     """The DB half of compare_query_against_db_subset_and_self (actor.rs:

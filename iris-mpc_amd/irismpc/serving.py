@@ -176,11 +176,21 @@ class ServingEngines:
     def n_chunks(self, rows):
         return max(1, -(-rows // CHUNK))
 
-    def step(self, qc, qm, batch, chunk_index, db_rows, slots, max_queued=2):
-        """Queues one chunk step of the scan engines for the active query set.
-        At most `max_queued` steps wait on the scan streams, so the next step
-        is queued while one runs; the device slot maps are rewritten only when
-        the active slots change, after the steps that read them."""
+    def chunks_per_step(self, batch, chunk_index, n_chunks):
+        """Chunks one step covers: several while few queries are active (their
+        launches alone would leave the GPU partly idle), one under load so that
+        arrivals join at short intervals; bounded by the result buffers
+        (chunks x rows within the engines' query rows) and the ring's end."""
+        k = 4 if batch <= 2 else 2 if batch <= 8 else 1
+        k = min(k, max(1, (self.max_active * ROT) // (batch * ROT)), n_chunks - chunk_index)
+        return max(1, k)
+
+    def step(self, qc, qm, batch, chunk_index, db_rows, slots, max_queued=2, n_chunks=1):
+        """Queues one step of the scan engines for the active query set over
+        chunks [chunk_index, chunk_index + n_chunks). At most `max_queued` steps
+        wait on the scan streams, so the next step is queued while one runs; the
+        device slot maps are rewritten only when the active slots change, after
+        the steps that read them."""
         slots = list(slots)
         if slots != self._slots:
             self.sync_scan()
@@ -191,9 +201,9 @@ class ServingEngines:
         while len(self._inflight) >= max_queued:
             for ev in self._inflight.popleft():
                 ev.synchronize()
-        E.scan_chunk_trio(self.scan_parties, qc, qm, chunk_index * CHUNK, db_rows, batch, self.A,
-                          [b.data_ptr() for b in self.bitmaps], self.scan.streams,
-                          [m.data_ptr() for m in self.slot_maps], row_bits=self.words * 64)
+        E.scan_chunks_trio(self.scan_parties, qc, qm, chunk_index * CHUNK, n_chunks, db_rows, batch, self.A,
+                           [b.data_ptr() for b in self.bitmaps], self.scan.streams,
+                           [m.data_ptr() for m in self.slot_maps], row_bits=self.words * 64)
         evs = []
         for xs in self._xstreams:
             ev = self.torch.cuda.Event()
@@ -238,8 +248,11 @@ class ServingEngines:
         qc, qm = self._query_sets(self.scan, [shares], ROT)
         rows = self.n_rows
         self.clear_slot(0)
-        for c in range(self.n_chunks(rows)):
-            self.step(qc, qm, 1, c, rows, [0])
+        c, n = 0, self.n_chunks(rows)
+        while c < n:
+            k = self.chunks_per_step(1, c, n)
+            self.step(qc, qm, 1, c, rows, [0], n_chunks=k)
+            c += k
         return self.slot_matches(0, rows)
 
 
@@ -335,11 +348,13 @@ class Server:
             cursor %= n_chunks
             # no lock: scan steps only read the DB, and the DB writes (insert,
             # delete) run on this thread after the queued steps finished
-            eng.step(qsets[0], qsets[1], len(active), cursor, eng.n_rows, [r.slot for r in active])
+            k = eng.chunks_per_step(len(active), cursor, n_chunks)
+            eng.step(qsets[0], qsets[1], len(active), cursor, eng.n_rows, [r.slot for r in active],
+                     n_chunks=k)
             self.steps += 1
             for r in active:
-                r.covered.add(cursor)
-            cursor += 1
+                r.covered.update(range(cursor, cursor + k))
+            cursor += k
             finished = [r for r in active if len(r.covered) >= eng.n_chunks(r.db_rows)
                         and all(c in r.covered for c in range(eng.n_chunks(r.db_rows)))]
             for r in finished:

@@ -361,3 +361,48 @@ def test_pipelined_scan_partial_last_chunk(irismpc):
         maps.append(got[0])
     assert np.array_equal(maps[0], maps[1])
     assert np.array_equal(maps[0].astype(bool), want)
+
+
+def test_multi_chunk_step_equals_single_steps(irismpc):
+    """Several chunks in one step (dot shares side by side, one reshare and one
+    comparison) open the same bits as one step per chunk: four chunks with a
+    partial last one, a query count whose rows need 512-row padding, and a
+    step that runs past the DB's end."""
+    import torch
+
+    from irismpc import synth
+
+    N, U, B = 3 * 32768 + 1000, 64, 5
+    rows = B * 31
+    rng = np.random.default_rng(10)
+    targets = rng.choice(U, size=3, replace=False)
+    ex, parties = _engines(irismpc, N, 4 * rows, tag_base=9)  # result buffers for four chunks of `rows`
+    synth.build_db([pe.code_db for pe in parties], [pe.mask_db for pe in parties], N, U, 90, 120)
+    planted = [(j, int(targets[j]), 900 + 100 * j) for j in range(3)]
+    _, _, sh = synth.make_queries(B, planted, 90, 110, 100, 130)
+    qc = [parties[p].codes.upload_query_shares_rows(sh[p][0], rows) for p in range(3)]
+    qm = [parties[p].masks.upload_query_shares_rows(sh[p][1], rows) for p in range(3)]
+    streams = [irismpc.stream_create() for _ in range(3)]
+    A = irismpc.THRESHOLD_ANON_STATS_A
+    want = np.zeros((B, N), bool)
+    for j, t, _ in planted:
+        want[j] = (np.arange(N) % U) == t
+
+    def run(steps):
+        bms = [torch.zeros(N * B // 64 + 2, dtype=torch.int64, device="cuda") for _ in range(3)]
+        torch.cuda.synchronize()
+        ptrs = [b.data_ptr() for b in bms]
+        for off, k in steps:
+            irismpc.scan_chunks_trio(parties, qc, qm, off, k, N, B, A, ptrs, streams)
+        for s in streams:
+            irismpc.stream_synchronize(s)
+        got = [np.unpackbits(b.cpu().numpy().view(np.uint8), bitorder="little")[: B * N].reshape(B, N)
+               for b in bms]
+        assert all(np.array_equal(g, got[0]) for g in got), "parties opened different bits"
+        return got[0].astype(bool)
+
+    C = irismpc.DB_CHUNK_SIZE
+    single = run([(c * C, 1) for c in range(4)])
+    assert np.array_equal(single, want)
+    assert np.array_equal(run([(0, 4)]), single)
+    assert np.array_equal(run([(0, 2), (2 * C, 4)]), single)  # the second step stops at the DB's end
