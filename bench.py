@@ -35,9 +35,14 @@ CHACHA_BLOCKS_PER_PAIR = 93 * 2 / 512 + 2 * 4 / 64
 CHACHA12_OPS = 600  # 6 double rounds x 8 quarter rounds x 12 ops + feed-forward and set-up
 THRESHOLD_ANON_A = 16384  # 0.375: the prefilter threshold of the full scan
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "share_gemm_c2_traffic.json")
-# kernel behind each GEMM variant id (-1: automatic, rotation-aware for raw-share queries)
-KERNEL_OF_VARIANT = {-1: "share_gemm_rot_kernel<2, 4>", 100: "share_gemm_rot_kernel<2, 4>",
-                     101: "share_gemm_rot_kernel<2, 2>"}
+# kernel behind each rotation-aware GEMM variant id; the automatic choice (-1)
+# for raw-share queries is 102 up to two queries (62 rows), else 100
+KERNEL_OF_VARIANT = {100: "share_gemm_rot_kernel<4, 2, 4, 1>", 101: "share_gemm_rot_kernel<4, 2, 2, 1>",
+                     102: "share_gemm_rot_kernel<2, 1, 4, 2>"}
+ROT_SMALL_ROWS = 62
+# int8 ops per DB byte at which the share GEMM turns from HBM- to MFMA-bound:
+# 5.0 POPS / 8 TB/s; a launch does 3 Qr ops per DB byte
+RIDGE_OPS_PER_BYTE = INT8_PEAK_TOPS / HBM_PEAK_GBS * 1e3
 
 
 def measured_traffic(kernel):
@@ -671,8 +676,19 @@ def main():
     # the code and mask launches are the same kernel; per launch on average:
     # algorithmic int8 ops / average launch duration (HIP events on its stream)
     achieved = (ops_code + ops_mask) / ((code_ms + mask_ms) / 1e3) / 1e12
-    kernel = KERNEL_OF_VARIANT.get(variant)
+    resolved = variant if variant != -1 else (102 if Qr <= ROT_SMALL_ROWS else 100)
+    kernel = KERNEL_OF_VARIANT.get(resolved)
     traffic = measured_traffic(kernel) if kernel and (N, Qr) == (1 << 20, 1984) else None
+    if 3 * Qr >= RIDGE_OPS_PER_BYTE:
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
+                    "frac": achieved / INT8_PEAK_TOPS, "traffic": traffic}
+    else:
+        # few query rows: the launch streams the DB limbs once (and writes Qr u16
+        # shares per DB row); algorithmic bytes over the average launch time
+        hbm_bytes = N * (12800 + 6400) * 2 + 2 * Qr * N * 2
+        gbs = hbm_bytes / 2 / ((code_ms + mask_ms) / 2 / 1e3) / 1e9
+        roofline = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": gbs / HBM_PEAK_GBS, "traffic": traffic}
     line = {
         "metric": "3-party iris comparisons/sec on 1 MI355X + p50/p99 request latency",
         "value": pairs / (ms_step / 1e3),
@@ -688,22 +704,15 @@ def main():
         "data": "synthetic: uniformly random u16 DB and query shares (the distribution of real "
                 "Shamir shares), ChaCha20-generated on device",
         "config": {
-            "workload": "per-party share GEMM, plain form: DB 1,048,576 x (12,800 code + 6,400 "
-                        "trimmed-mask) u16 shares in HBM, 64 queries x 31 rotations = 1,984 rows, "
+            "workload": f"per-party share GEMM, plain form: DB {N:,} x (12,800 code + 6,400 "
+                        f"trimmed-mask) u16 shares in HBM, {B} queries x 31 rotations = {Qr:,} rows, "
                         "fused correction + mask x2 + ChaCha12 zero-share epilogue, "
                         "reference chunking 32,768 rows",
             "db_rows": N,
             "query_rows": Qr,
-            "gemm_variant": variant,
+            "gemm_variant": resolved,
         },
-        "roofline": {
-            "bound": "mfma",
-            "achieved": achieved,
-            "peak": INT8_PEAK_TOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / INT8_PEAK_TOPS,
-            "traffic": traffic,
-        },
+        "roofline": roofline,
         "kernels": {
             "dominant": kernel,
             "share_gemm_code_ms": code_ms,

@@ -380,7 +380,10 @@ void ShareDB::refill_query_shares(QuerySet& q, const uint16_t* host_shares, size
 }
 
 int ShareDB::resolve_variant(const QuerySet& q) const {
-  if (variant == kAutoVariant) return q.rot.ptr ? kRotVariantBase : 7;
+  if (variant == kAutoVariant) {
+    if (!q.rot.ptr) return 7;
+    return q.rows <= kRotSmallRows ? kRotSmallVariant : kRotVariantBase;
+  }
   IRIS_REQUIRE(!is_rot_variant(variant) || q.rot.ptr,
                "rotation-aware GEMM variant needs a query uploaded from raw shares");
   return variant;

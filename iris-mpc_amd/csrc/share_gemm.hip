@@ -1038,30 +1038,38 @@ __global__ __launch_bounds__(64 * NW, 1) void share_gemm_hyb_kernel(const GemmAr
 // Rotation-aware form. The 31 query rows of one iris are rotations of one
 // preprocessed share: row (b, t) holds, in every 800-coefficient row r of the
 // share, U_b[r][(c + 4 (15 - t)) mod 800] at coefficient c. A block's query
-// tile is 4 whole queries (124 rows, 4 MFMA row tiles with 4 spare rows), and
-// their unrotated shares sit in LDS for the block's whole K loop as padded
+// tile is TQ whole queries (31 TQ rows in TQ MFMA row tiles, TQ spare rows),
+// and their unrotated shares sit in LDS for the block's whole K loop as padded
 // limb planes P_b[limb][r][i] = limb(U_b[r][(i - 60) mod 800]), i < 920, so
 // rotation t of coefficients [c, c + 16) is the 16 bytes at i = 60 + c +
-// 4 (15 - t) (4-byte aligned). The query then costs 115 KiB of global reads
-// per block instead of 3.2 MiB, the K loop needs no barriers, and only DB rows
-// stream from global memory, straight into VGPRs two K-steps ahead.
+// 4 (15 - t) (4-byte aligned). The query then costs 29 KiB per query of global
+// reads per block instead of 0.8 MiB, the K loop needs no barriers, and only
+// DB rows stream from global memory, straight into VGPRs two K-steps ahead.
+// TQ = 4 is the MFMA-bound form for large query sets. TQ = 2 serves one or two
+// queries, where the scan is bound by DB reads: half the MFMA work of a 4-query
+// tile and a register file small enough for two waves per SIMD, so that more
+// DB bytes are in flight (6.0 against 5.5 TB/s for one query; a plain read
+// kernel reaches 6.2). Plane slots are allocated in groups of kRotQueries.
 constexpr int kRotPad = 60, kRotRow = 800 + 2 * kRotPad, kRotQueries = 4;
 
-template <int TD, int NW>
+template <int TQ, int TD, int NW>
 struct RotLds {
   static constexpr int kMaxRows = kCodeLen / 800;  // 16 rows of 800 coefficients
+  // whole 1 KiB DMA fragments: the plane copy may overrun the last plane
+  static constexpr int kPlaneBytes =
+      (TQ * 2 * kMaxRows * kRotRow + kFragBytes - 1) / kFragBytes * kFragBytes;
   union {
-    v4i planes[kRotQueries * 2 * kMaxRows * kRotRow / 16];
-    uint16_t tile[128 * (NW * TD * 32)];
+    v4i planes[kPlaneBytes / 16];
+    uint16_t tile[32 * TQ * (NW * TD * 32)];
   };
 };
 
-template <int TD, int NW>
-__global__ __launch_bounds__(64 * NW, 1) void share_gemm_rot_kernel(const GemmArgs a) {
-  constexpr int TQ = 4;  // 128 rows >= 4 x 31
-  constexpr int BQ = kRotQueries * kRotations, BD = NW * TD * 32;
+template <int TQ, int TD, int NW, int WPE>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+void share_gemm_rot_kernel(const GemmArgs a) {
+  constexpr int BQ = TQ * kRotations, BD = NW * TD * 32;  // 32 TQ rows >= 31 TQ
   constexpr int G = TQ * TD, HALF = 3 * TQ * TD;  // MFMAs per pass, per k-slice
-  using S = RotLds<TD, NW>;
+  using S = RotLds<TQ, TD, NW>;
   __shared__ S sm;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int q0;
@@ -1076,7 +1084,7 @@ __global__ __launch_bounds__(64 * NW, 1) void share_gemm_rot_kernel(const GemmAr
     // 1 KiB per wave-wide LDS DMA, all in flight at once (the plane buffer
     // carries 1 KiB of slack for the last partial chunk)
     const int8_t* src = a.qrot + (size_t)(q0 / kRotations) * qbytes + lane * 16;
-    const int n_chunks = (kRotQueries * qbytes + kFragBytes - 1) / kFragBytes;
+    const int n_chunks = (TQ * qbytes + kFragBytes - 1) / kFragBytes;
     for (int c = wave; c < n_chunks; c += NW) dma_fragment(src + (size_t)c * kFragBytes, &sm.planes[c * 64]);
     wait_vmcnt<0>();
   }
@@ -1162,32 +1170,38 @@ __global__ __launch_bounds__(64 * NW, 1) void share_gemm_rot_kernel(const GemmAr
   // first use of the operands is where the compiler waits for them), then
   // issues the next slice's A reads (and in slice 0 the DB loads of step
   // kt + 2), then runs the other two passes while those land.
-  // The loads are threaded one or two at a time between the MFMAs of the
-  // other two passes (a batch of them would leave this SIMD's only wave
-  // without an MFMA to issue).
+  // With one wave per SIMD (TQ = 4) the loads are threaded one or two at a
+  // time between the MFMAs of the other two passes (a batch of them would
+  // leave this SIMD's only wave without an MFMA to issue). The small query
+  // tiles have too few MFMAs per pass for that; their SIMDs hold WPE waves.
   constexpr int NBL = TD * 4, NAR = TQ * 2 * 2;  // DB loads, ds_read2 per slice
-  static_assert(NBL * 2 <= HALF - G && NAR <= 2 * (HALF - G), "interleave pattern");
+  constexpr bool kThread = NBL <= HALF - G && NAR <= 2 * (HALF - G);
+  static_assert(kThread || WPE > 1, "one wave per SIMD needs the threaded schedule");
   auto step = [&](int kt, const v4i (&Bc)[TD][2][2], v4i (&Bp)[TD][2][2]) {
     mfma(A0, Bc, 0, C0{}, CG{});
     __builtin_amdgcn_sched_barrier(0);
     load_b(Bp, kt + 2 < last ? kt + 2 : last);
     read_a(A1, kt, 1);
     mfma(A0, Bc, 0, CG{}, CH{});
+    if constexpr (kThread) {
 #pragma unroll
-    for (int r = 0; r < HALF - G; r++) {
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
-      if (r < NBL) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);     // VMEM read
-      if (r < NAR / 2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // DS read
+      for (int r = 0; r < HALF - G; r++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
+        if (r < NBL) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);     // VMEM read
+        if (r < NAR / 2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // DS read
+      }
     }
     __builtin_amdgcn_sched_barrier(0);
     mfma(A1, Bc, 1, C0{}, CG{});
     __builtin_amdgcn_sched_barrier(0);
     read_a(A0, kt + 1 < last ? kt + 1 : last, 0);
     mfma(A1, Bc, 1, CG{}, CH{});
+    if constexpr (kThread) {
 #pragma unroll
-    for (int r = 0; r < HALF - G; r++) {
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
-      if (r < NAR / 2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // DS read
+      for (int r = 0; r < HALF - G; r++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
+        if (r < NAR / 2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // DS read
+      }
     }
     __builtin_amdgcn_sched_barrier(0);
   };
@@ -1203,11 +1217,13 @@ __global__ __launch_bounds__(64 * NW, 1) void share_gemm_rot_kernel(const GemmAr
 
   // The limb-sum correction splits into a query term and a DB term:
   // (s0 << 7) + ((s0 + s1) << 15) with s = dsum + qsum is linear in u32.
-  // Query terms of the tile's 128 rows go to LDS past the output tile.
+  // Query terms of the tile's 32 TQ rows go to LDS past the output tile.
+  constexpr int ROWS = 32 * TQ;
   const int q_end = a.qr < q0 + BQ ? a.qr : q0 + BQ;
-  uint32_t* qterm = reinterpret_cast<uint32_t*>(&sm.tile[128 * BD]);
-  static_assert(sizeof(S) >= 128 * BD * 2 + 128 * 4, "query terms fit behind the tile");
-  if (threadIdx.x < 128) {
+  uint32_t* qterm = reinterpret_cast<uint32_t*>(&sm.tile[ROWS * BD]);
+  static_assert(sizeof(S) >= ROWS * BD * 2 + ROWS * 4, "query terms fit behind the tile");
+  static_assert(ROWS <= 64 * NW, "one thread per query term");
+  if (threadIdx.x < ROWS) {
     const int q = q0 + (int)threadIdx.x;
     uint32_t t = 0;
     if (q < q_end) {
@@ -1243,7 +1259,7 @@ __global__ __launch_bounds__(64 * NW, 1) void share_gemm_rot_kernel(const GemmAr
         }
       }
     }
-  epilogue_store<128, BD>(a, sm.tile, q0, db0, q_end);
+  epilogue_store<ROWS, BD>(a, sm.tile, q0, db0, q_end);
 }
 
 // Padded unrotated limb planes of every query slot from the rotated u16 rows
@@ -1298,8 +1314,9 @@ GemmTile gemm_tile(int variant) {
     case 15: return {128, 256};  // hybrid: query via LDS DMA, DB direct, 4 waves of 128x64
     case 16: return {128, 128};  // hybrid, 2 waves of 128x64
     case 17: return {64, 256};   // hybrid, 4 waves of 64x64
-    case kRotVariantBase: return {kRotQueries * kRotations, 256};      // rotation-aware, 4 waves
-    case kRotVariantBase + 1: return {kRotQueries * kRotations, 128};  // rotation-aware, 2 waves
+    case kRotVariantBase: return {4 * kRotations, 256};      // rotation-aware, 4 waves
+    case kRotVariantBase + 1: return {4 * kRotations, 128};  // rotation-aware, 2 waves
+    case kRotVariantBase + 2: return {2 * kRotations, 128};  // two queries, 2 waves per SIMD
     default: throw Error("unknown share GEMM variant " + std::to_string(variant));
   }
 }
@@ -1335,8 +1352,9 @@ void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s) {
     case 15: share_gemm_hyb_kernel<4, 2, 4><<<grid, 256, 0, s>>>(a); break;
     case 16: share_gemm_hyb_kernel<4, 2, 2><<<grid, 128, 0, s>>>(a); break;
     case 17: share_gemm_hyb_kernel<2, 2, 4><<<grid, 256, 0, s>>>(a); break;
-    case kRotVariantBase: share_gemm_rot_kernel<2, 4><<<grid, 256, 0, s>>>(a); break;
-    case kRotVariantBase + 1: share_gemm_rot_kernel<2, 2><<<grid, 128, 0, s>>>(a); break;
+    case kRotVariantBase: share_gemm_rot_kernel<4, 2, 4, 1><<<grid, 256, 0, s>>>(a); break;
+    case kRotVariantBase + 1: share_gemm_rot_kernel<4, 2, 2, 1><<<grid, 128, 0, s>>>(a); break;
+    case kRotVariantBase + 2: share_gemm_rot_kernel<2, 1, 4, 2><<<grid, 256, 0, s>>>(a); break;
   }
   IRIS_HIP_CHECK(hipGetLastError());
 }

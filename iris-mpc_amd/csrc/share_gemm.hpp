@@ -92,7 +92,9 @@ struct GemmTile {
 constexpr int kGemmVariants = 18;
 // Variants from kRotVariantBase on need a query set with rotation structure
 // (uploaded from raw shares, rotations generated on the device).
-constexpr int kRotVariantBase = 100, kRotVariants = 2;
+constexpr int kRotVariantBase = 100, kRotVariants = 3;
+// The rotation-aware variant for small query sets (two-query tile).
+constexpr int kRotSmallVariant = kRotVariantBase + 2, kRotSmallRows = 2 * 31;
 inline bool is_rot_variant(int v) { return v >= kRotVariantBase && v < kRotVariantBase + kRotVariants; }
 constexpr int kMaxBQ = 256;   // query buffers are padded to this many rows
 constexpr int kMaxBD = 256;   // DB buffers are padded to this many rows

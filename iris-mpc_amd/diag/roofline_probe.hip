@@ -7,6 +7,11 @@
 //                         sustained clock, depends on operand switching.
 //   roofline_probe copy   a plain 16-byte-per-lane copy of 2 GiB, the reference
 //                         kernel for scaling FETCH_SIZE / WRITE_SIZE counters.
+//   roofline_probe read   the HBM read ceiling for a DB-sized buffer (2M rows of
+//                         12,800 coefficients x 2 limbs, 53.7 GB), read once
+//                         linearly and once in the share GEMM's order (each
+//                         wave streams 2 KiB per K-step from four 16-row groups
+//                         400 KiB apart), with no MFMA work.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -89,6 +94,43 @@ __global__ __launch_bounds__(256) void copy16(const v4i* __restrict__ src, v4i* 
     dst[i] = src[i];
 }
 
+__global__ __launch_bounds__(256) void read_linear(const v4i* __restrict__ p, size_t n, v4i* out) {
+  v4i x{};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x ^= p[i];
+  out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = x;
+}
+
+// Block b reads 16-row groups [16 b, 16 b + 16) of the tiled DB layout (group
+// stride n_kt x 2 KiB); the lanes of wave w take groups 4 w + 2 j + (lane / 16
+// mod 2), 16 bytes at lane offsets as the GEMM's B-operand loads, two K-steps
+// per iteration.
+__global__ __launch_bounds__(256) void read_groups(const int8_t* __restrict__ db, int n_kt, v4i* out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t gstride = (size_t)n_kt * 2048;
+  const int r32 = lane & 31;
+  const size_t g = (size_t)blockIdx.x * 16 + wave * 4 + (r32 >> 4);
+  const int8_t* base = db + g * gstride + ((r32 & 15) + 16 * (lane >> 5)) * 16;
+  v4i x{};
+  for (int kt = 0; kt < n_kt; kt += 2) {
+    v4i b[2][2][4];
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int o = 0; o < 4; o++)
+          b[u][j][o] = *reinterpret_cast<const v4i*>(base + 2 * j * gstride + (size_t)(kt + u) * 2048 + o * 512);
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int o = 0; o < 4; o++) x ^= b[u][j][o];
+  }
+  out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = x;
+}
+
 template <bool RANDOM>
 static int time_mfma(const char* name, int blocks, int iters, int reps, int* out) {
   hipEvent_t a, b;
@@ -143,6 +185,45 @@ int main(int argc, char** argv) {
     CHECK(hipFree(dst));
     return 0;
   }
-  fprintf(stderr, "usage: roofline_probe mfma|copy\n");
+  if (!strcmp(mode, "read")) {
+    const int n_kt = 200;                             // K = 12,800
+    const size_t groups = ((size_t)1 << 21) / 16;     // 2M rows
+    const size_t bytes = groups * n_kt * 2048;
+    const unsigned blocks_lin = 2048, blocks_grp = (unsigned)(groups / 16);
+    int8_t* db;
+    v4i* out;
+    CHECK(hipMalloc(&db, bytes));
+    CHECK(hipMalloc(&out, (size_t)blocks_grp * 256 * sizeof(v4i)));
+    CHECK(hipMemset(db, 1, bytes));
+    hipEvent_t a, b;
+    CHECK(hipEventCreate(&a));
+    CHECK(hipEventCreate(&b));
+    for (int pattern = 0; pattern < 2; pattern++) {
+      auto launch = [&] {
+        if (pattern == 0)
+          read_linear<<<blocks_lin, 256>>>(reinterpret_cast<const v4i*>(db), bytes / 16, out);
+        else
+          read_groups<<<blocks_grp, 256>>>(db, n_kt, out);
+      };
+      launch();
+      CHECK(hipGetLastError());
+      CHECK(hipDeviceSynchronize());
+      CHECK(hipEventRecord(a));
+      for (int r = 0; r < 3; r++) launch();
+      CHECK(hipEventRecord(b));
+      CHECK(hipEventSynchronize(b));
+      float ms = 0;
+      CHECK(hipEventElapsedTime(&ms, a, b));
+      ms /= 3;
+      printf("read %-28s %zu bytes per dispatch, %.3f ms, %.1f GB/s\n",
+             pattern == 0 ? "linear (2048 x 256 threads):" : "share GEMM group order:", bytes, ms,
+             bytes / ms / 1e6);
+      fflush(stdout);
+    }
+    CHECK(hipFree(db));
+    CHECK(hipFree(out));
+    return 0;
+  }
+  fprintf(stderr, "usage: roofline_probe mfma|copy|read\n");
   return 2;
 }

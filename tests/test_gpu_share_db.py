@@ -168,8 +168,9 @@ def test_query_refill_matches_fresh_upload(irismpc, orc, K, mult):
 
 
 def test_share_gemm_rotation_masked_range(irismpc, orc):
-    """Masked multi-chunk launch with the rotation-aware kernel: same shares as
-    the generic kernel, first chunk checked against the oracle."""
+    """Masked multi-chunk launch with every rotation-aware kernel (4-query
+    tile and the one- and two-query tiles): same shares as the generic kernel,
+    first chunk checked against the oracle."""
     import torch
 
     rng = np.random.default_rng(12)
@@ -183,14 +184,16 @@ def test_share_gemm_rotation_masked_range(irismpc, orc):
     slices.load_full_db(db)
     dq = eng.upload_query_shares(raw)
     outs = []
-    for v in (irismpc.gemm_rot_variants()[0], 7):
+    variants = irismpc.gemm_rot_variants() + [7]
+    for v in variants:
         eng.set_variant(v)
         eng.counter = 40
         out = torch.zeros(N * qr + 64, dtype=torch.int16, device="cuda")
         eng.dot_reduce_range(dq, slices, 0, N, chunk, 1, out.data_ptr())
         torch.cuda.synchronize()
         outs.append(out.cpu().numpy().view(np.uint16))
-    assert np.array_equal(outs[0], outs[1])
+    for v, o in zip(variants[1:], outs[1:]):
+        assert np.array_equal(outs[0], o), f"variant {v}"
     ref = orc.u16_dot(_rotated_rows(orc, 2, raw, qr), db[:chunk]).reshape(-1)
     ref = orc.mask_dot_share(ref, 1, seeds[0], seeds[1], 0, 40)
     assert np.array_equal(outs[0][:chunk * qr], ref)
