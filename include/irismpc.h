@@ -221,8 +221,9 @@ int irismpc_share_db_fetch_results(irismpc_share_db* e, uint16_t* out, size_t n,
 int irismpc_share_db_get_counter(const irismpc_share_db* e, uint64_t* counter);
 int irismpc_share_db_set_counter(irismpc_share_db* e, uint64_t counter);
 /* GEMM kernel variant: -1 (default) = automatic: the rotation-aware kernel for
- * query sets uploaded from raw shares (irismpc_query_upload_shares), the best
- * generic tile otherwise; 0 .. irismpc_gemm_variants()-1 = generic tiles;
+ * query sets uploaded from raw shares (irismpc_query_upload_shares; its
+ * two-query tile for sets of at most 62 rows, its four-query tile above), the
+ * best generic tile otherwise; 0 .. irismpc_gemm_variants()-1 = generic tiles;
  * irismpc_gemm_rot_variants() = rotation-aware tiles (raw-share queries only).
  * No reference counterpart: the reference has one cuBLAS-backed GEMM path. */
 int irismpc_share_db_set_variant(irismpc_share_db* e, int variant);
