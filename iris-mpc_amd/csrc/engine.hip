@@ -98,7 +98,6 @@ ShareDB::ShareDB(int device, int peer_id, size_t max_db_length, size_t query_len
       code_length_(code_length),
       tag_(tag),
       query_length_(query_length),
-      max_db_length_(max_db_length),
       seed_own_(seed_own),
       seed_prev_(seed_prev),
       last_rows_(query_length),

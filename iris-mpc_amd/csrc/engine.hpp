@@ -228,7 +228,7 @@ class ShareDB {
                                                const uint16_t* insert_shares, size_t batch,
                                                size_t rows, int mirror, hipStream_t s) const;
   int device_, peer_id_, code_length_, tag_;
-  size_t query_length_, max_db_length_;
+  size_t query_length_;
   ChaChaKey seed_own_, seed_prev_;
   uint64_t nonce_ = 0, counter_ = 0;
   size_t last_rows_;  // query rows of the last dot_reduce (the reshare length)
