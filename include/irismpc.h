@@ -181,6 +181,17 @@ int irismpc_query_destroy(irismpc_query* q);
 int irismpc_share_db_dot_reduce(irismpc_share_db* e, const irismpc_query* q, const irismpc_db* db,
                                 size_t chunk_size, size_t offset, uint16_t multiplier,
                                 void* stream);
+/* Consecutive irismpc_share_db_dot_reduce calls in one launch: DB rows
+ * [offset, offset + total) as chunks of chunk_size rows (the last one
+ * total mod chunk_size wide when that is not zero; total a multiple of 64),
+ * results side by side in the engine's result buffer from element out_offset,
+ * chunk c at out_offset + c * chunk_size * Qr, each with the layout and masks
+ * of its own call (share_db.rs:800-905) made in order. No reference
+ * counterpart: it serves the multi-chunk scan steps of a small query set. */
+int irismpc_share_db_dot_reduce_chunks(irismpc_share_db* e, const irismpc_query* q,
+                                       const irismpc_db* db, size_t offset, size_t total,
+                                       size_t chunk_size, uint16_t multiplier,
+                                       size_t out_offset, void* stream);
 /* Whole-range form: rows [offset, offset+total) in chunks of chunk_size rows
  * in one launch; chunk c is written at out + c * chunk_size * Qr, Qr = the
  * query set's rows. No reference counterpart: one launch in place of the
@@ -383,9 +394,10 @@ int irismpc_scan_chunk_trio(irismpc_share_db* const codes[3], irismpc_share_db* 
                             size_t offset, size_t db_size, size_t batch_size, uint32_t threshold_a,
                             uint64_t* const bitmaps[3], const uint32_t* const slot_of[3],
                             size_t row_bits, void* const streams[3]);
-/* n_chunks (1 .. 8) consecutive chunk steps from `offset` in one step: each
+/* n_chunks (1 .. 16) consecutive chunk steps from `offset` in one step: each
  * chunk's dot shares sit side by side in the engines' result buffers
- * (n_chunks x 32,768 x rows must fit them), one reshare and one threshold
+ * (n_chunks x 32,768 x rows must fit them; one GEMM launch per matrix covers
+ * them, as irismpc_share_db_dot_reduce_chunks), one reshare and one threshold
  * comparison cover them, and each chunk opens into the bitmaps as above. The
  * opened bits equal those of n_chunks single steps; the circuit randomness
  * follows one call's schedule. No reference counterpart: the reference's

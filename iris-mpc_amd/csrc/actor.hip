@@ -30,7 +30,7 @@ void scan_chunks(const PartyScan& p, size_t offset, size_t n_chunks, size_t db_s
   const size_t rows = (size_t)p.code_query->rows;
   IRIS_REQUIRE((size_t)p.mask_query->rows == rows, "code/mask query sets disagree on rows");
   IRIS_REQUIRE(batch_size * kRotations <= rows, "batch larger than the query rows");
-  IRIS_REQUIRE(n_chunks >= 1 && n_chunks <= kMaxStepChunks, "1 .. 8 chunks per step");
+  IRIS_REQUIRE(n_chunks >= 1 && n_chunks <= kMaxStepChunks, "1 .. 16 chunks per step");
   // actor.rs:2331-2337 pads to 64; the circuits need a multiple of 8 words
   const size_t pad = rows % 8 == 0 ? 64 : 512;
   size_t real[kMaxStepChunks], width[kMaxStepChunks], start[kMaxStepChunks];
@@ -45,10 +45,10 @@ void scan_chunks(const PartyScan& p, size_t offset, size_t n_chunks, size_t db_s
     total += width[c];
     n = c + 1;
   }
-  for (size_t c = 0; c < n; c++) {
-    p.codes->dot_reduce(*p.code_query, *p.code_db, width[c], offset + c * kDbChunk, 1, s, start[c] * rows);
-    p.masks->dot_reduce(*p.mask_query, *p.mask_db, width[c], offset + c * kDbChunk, 2, s, start[c] * rows);
-  }
+  // every chunk but the last is kDbChunk wide, so one launch per matrix covers
+  // them with the per-chunk layout and masks
+  p.codes->dot_reduce_chunks(*p.code_query, *p.code_db, offset, total, kDbChunk, 1, s);
+  p.masks->dot_reduce_chunks(*p.mask_query, *p.mask_db, offset, total, kDbChunk, 2, s);
   // the distance capture reads the peer shares on this party's stream after
   // the circuits: they need a copy the peer's next GEMM cannot overwrite
   p.codes->peer_copy = p.masks->peer_copy = p.comparator->anon_capture_enabled();

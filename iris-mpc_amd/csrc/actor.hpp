@@ -40,7 +40,7 @@ void scan_chunk_trio(const PartyScan p[3], size_t offset, size_t db_size, size_t
                      const uint32_t* const slot_of[3], size_t row_bits, hipStream_t const s[3]);
 // n_chunks consecutive chunks from `offset` in one step (their results side by
 // side in the result buffer: n_chunks x 32,768 x rows must fit it).
-constexpr size_t kMaxStepChunks = 8;
+constexpr size_t kMaxStepChunks = 16;
 void scan_chunks(const PartyScan& p, size_t offset, size_t n_chunks, size_t db_size,
                  size_t batch_size, uint32_t threshold_a, uint64_t* bitmap,
                  const uint32_t* slot_of, size_t row_bits, hipStream_t s);

@@ -330,6 +330,15 @@ int irismpc_share_db_dot_reduce(irismpc_share_db* e, const irismpc_query* q, con
     e->impl->dot_reduce(*q->impl, *db->impl, chunk_size, offset, multiplier, st(stream));
   });
 }
+int irismpc_share_db_dot_reduce_chunks(irismpc_share_db* e, const irismpc_query* q,
+                                       const irismpc_db* db, size_t offset, size_t total,
+                                       size_t chunk_size, uint16_t multiplier,
+                                       size_t out_offset, void* stream) {
+  return guarded([&] {
+    e->impl->dot_reduce_chunks(*q->impl, *db->impl, offset, total, chunk_size, multiplier,
+                               st(stream), out_offset);
+  });
+}
 int irismpc_share_db_dot_reduce_range(irismpc_share_db* e, const irismpc_query* q,
                                       const irismpc_db* db, size_t offset, size_t total,
                                       size_t chunk_size, uint16_t multiplier, uint16_t* out,

@@ -436,6 +436,38 @@ void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, siz
   if (is_remote()) counter_ += ceil_div(chunk * last_rows_ * 2, 64);
 }
 
+void ShareDB::dot_reduce_chunks(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
+                                size_t chunk, uint16_t multiplier, hipStream_t s, size_t out_offset) {
+  DeviceGuard g(device_);
+  IRIS_REQUIRE(q.code_length == code_length_ && db.code_length == code_length_, "code length mismatch");
+  IRIS_REQUIRE(q.rows >= 1 && (size_t)q.rows <= query_length_, "query rows exceed query_length");
+  IRIS_REQUIRE(total > 0 && total % 64 == 0 && chunk % 256 == 0, "bad chunk run geometry");
+  IRIS_REQUIRE(out_offset + total * (size_t)q.rows <= results_.n,
+               "chunks larger than the engine's result buffer");
+  IRIS_REQUIRE(offset % 16 == 0 && offset + total <= db.capacity - kMaxBD, "chunks beyond DB capacity");
+  const int variant = resolve_variant(q);
+  GemmArgs a = gemm_args(q, db, variant);
+  const GemmTile t = gemm_tile(variant);
+  const size_t qr = (size_t)q.rows;
+  a.out = results() + out_offset;
+  a.out_chunk_stride = chunk * qr;
+  a.n_dtiles = (int)ceil_div(total, (size_t)t.bd);
+  a.db_begin = offset;
+  a.db_end = offset + total;
+  a.chunk = chunk;
+  a.ctr_first = counter_;
+  a.ctr_per_chunk = ceil_div(chunk * qr * 2, 64);  // mask keystream of a full chunk
+  a.nonce = nonce_;
+  a.key_own = seed_own_;
+  a.key_prev = seed_prev_;
+  a.mult = multiplier;
+  a.masked = is_remote() ? 1 : 0;
+  launch_share_gemm(a, variant, s);
+  last_rows_ = qr;
+  if (is_remote())
+    for (size_t c0 = 0; c0 < total; c0 += chunk) counter_ += ceil_div(std::min(chunk, total - c0) * qr * 2, 64);
+}
+
 void ShareDB::dot_reduce_range(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
                                size_t chunk, uint16_t multiplier, uint16_t* out, hipStream_t s) {
   DeviceGuard g(device_);

@@ -180,6 +180,13 @@ class ShareDB {
   // small query set can sit side by side in the result buffer
   void dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
                   uint16_t multiplier, hipStream_t s, size_t out_offset = 0);
+  // Consecutive dot_reduce calls in one launch: DB rows [offset, offset +
+  // total) as chunks of `chunk` rows (the last one total mod chunk wide when
+  // that is not zero; a multiple of 64), results side by side from results() +
+  // out_offset, each chunk with the layout and masks its own dot_reduce call
+  // would give (chunk c's mask right after chunk c - 1's: no reshare between).
+  void dot_reduce_chunks(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
+                         size_t chunk, uint16_t multiplier, hipStream_t s, size_t out_offset = 0);
   // Multi-chunk single launch: rows [offset, offset + total) in chunks of
   // `chunk` rows, results written chunk after chunk into `out`.
   void dot_reduce_range(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,

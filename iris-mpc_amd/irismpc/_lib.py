@@ -73,6 +73,7 @@ SIGNATURES = {
     "irismpc_scan_chunks_trio": [vp, vp, vp, vp, vp, vp, vp, vp, sz, sz, sz, sz, u32, vp, vp, sz, vp],
     "irismpc_share_db_dot_reduce": [vp, vp, vp, sz, sz, u16, vp],
     "irismpc_share_db_dot_reduce_range": [vp, vp, vp, sz, sz, sz, u16, vp, vp],
+    "irismpc_share_db_dot_reduce_chunks": [vp, vp, vp, sz, sz, sz, u16, sz, vp],
     "irismpc_share_db_reshare_results": [vp, sz, vp],
     "irismpc_share_db_reshare_results_trio": [vp, sz, vp],
     "irismpc_share_db_result_views": [vp, vp, vp],

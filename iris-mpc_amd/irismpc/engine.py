@@ -235,6 +235,14 @@ class ShareDB:
         """dot + dot_reduce_and_multiply (fused)."""
         call("irismpc_share_db_dot_reduce", self.h, query.h, db.h, chunk_size, offset, multiplier, stream)
 
+    def dot_reduce_chunks(self, query, db, offset, total, chunk_size, multiplier=1, out_offset=0,
+                          stream=0):
+        """Consecutive dot_reduce calls over rows [offset, offset + total) in one
+        launch; chunk c's results at element out_offset + c * chunk_size * rows
+        of the result buffer, with the masks of those calls."""
+        call("irismpc_share_db_dot_reduce_chunks", self.h, query.h, db.h, offset, total, chunk_size,
+             multiplier, out_offset, stream)
+
     def dot_reduce_range(self, query, db, offset, total, chunk_size, multiplier, out_ptr, stream=0):
         call("irismpc_share_db_dot_reduce_range", self.h, query.h, db.h, offset, total, chunk_size,
              multiplier, out_ptr, stream)

@@ -181,7 +181,7 @@ class ServingEngines:
         launches alone would leave the GPU partly idle), one under load so that
         arrivals join at short intervals; bounded by the result buffers
         (chunks x rows within the engines' query rows) and the ring's end."""
-        k = 4 if batch <= 2 else 2 if batch <= 8 else 1
+        k = 8 if batch == 1 else 4 if batch <= 2 else 2 if batch <= 8 else 1
         k = min(k, max(1, (self.max_active * ROT) // (batch * ROT)), n_chunks - chunk_index)
         return max(1, k)
 

@@ -201,6 +201,41 @@ def test_share_gemm_rotation_masked_range(irismpc, orc):
     assert np.array_equal(outs[0][:chunk * qr], ref)
 
 
+@pytest.mark.parametrize("K,mult,batch", [(12800, 1, 1), (6400, 2, 3)])
+def test_dot_reduce_chunks_equals_per_chunk_calls(irismpc, orc, K, mult, batch):
+    """One launch over consecutive chunks (two full, one partial) gives the
+    shares, masks included, of per-chunk dot_reduce calls made in order, and
+    leaves the engine counter where they leave it; the partial chunk checked
+    against the oracle at its counter."""
+    rng = np.random.default_rng(21)
+    chunk, total, N = 256, 640, 768
+    qr = batch * 31
+    raw = _rand(rng, (batch, K))
+    db = _rand(rng, (N, K))
+    seeds = [np.full(8, 7, np.uint32), np.full(8, 9, np.uint32)]
+    ex = irismpc.Exchange()
+    eng = irismpc.ShareDB(2, total, qr, K, seeds, comm=ex, tag=0)
+    slices = eng.alloc_db(N)
+    slices.load_full_db(db)
+    dq = eng.upload_query_shares(raw)
+    eng.counter = 40
+    parts, ctrs = [], []
+    for c0 in range(0, total, chunk):
+        w = min(chunk, total - c0)
+        ctrs.append(eng.counter)
+        eng.dot_reduce(dq, slices, w, c0, mult)
+        parts.append(eng.fetch_results(qr * w))
+    per_chunk, ctr_end = np.concatenate(parts), eng.counter
+    eng.counter = 40
+    eng.dot_reduce_chunks(dq, slices, 0, total, chunk, mult)
+    got = eng.fetch_results(qr * total)
+    assert np.array_equal(got, per_chunk)
+    assert eng.counter == ctr_end
+    last = orc.u16_dot(_rotated_rows(orc, 2, raw, qr), db[512:640]).reshape(-1)
+    last = orc.mask_dot_share(last, mult, seeds[0], seeds[1], 0, ctrs[-1])
+    assert np.array_equal(got[512 * qr:], last)
+
+
 def test_rotation_variant_needs_structured_query(irismpc):
     eng = irismpc.ShareDB(0, 256, 248, 6400, ([0] * 8, [0] * 8))
     slices = eng.alloc_db(256)
