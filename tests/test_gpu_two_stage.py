@@ -406,3 +406,4 @@ def test_multi_chunk_step_equals_single_steps(irismpc):
     assert np.array_equal(single, want)
     assert np.array_equal(run([(0, 4)]), single)
     assert np.array_equal(run([(0, 2), (2 * C, 4)]), single)  # the second step stops at the DB's end
+    assert np.array_equal(run([(0, 8)]), single)  # a lone query's step size, past the DB's end
