@@ -27,7 +27,7 @@ def test_calculate_insertion_indices():
     ins = [[4], [1, 5], [2]]
     m = calculate_insertion_indices(merged, ins, [10, 7, 9], 6)
     # device order starts at the list whose first insertion is earliest (1 on
-    # device 1): 1 -> 26, 2 -> 27, 4 -> 28, then round 2: 5 -> 29
+    # device 1): 1 -> 26, 2 -> 27, 4 -> 28, then the second sweep: 5 -> 29
     assert merged[1] == 26 and merged[2] == 27 and merged[4] == 28 and merged[5] == 29
     assert m == [True, False, False, True, False, False]
     one = [NON_MATCH_ID, 3, NON_MATCH_ID]
