@@ -236,6 +236,22 @@ def test_dot_reduce_chunks_equals_per_chunk_calls(irismpc, orc, K, mult, batch):
     assert np.array_equal(got[512 * qr:], last)
 
 
+def test_dot_reduce_chunks_rejects_bad_geometry(irismpc):
+    """Chunk runs that do not fit the result buffer or the DB, or whose sizes
+    break the tiling, fail with an error before any launch."""
+    K, qr = 6400, 31
+    eng = irismpc.ShareDB(0, 512, qr, K, ([0] * 8, [0] * 8))
+    slices = eng.alloc_db(512)
+    dq = eng.upload_query_shares(_rand(np.random.default_rng(2), (1, K)))
+    for offset, total, chunk in ((0, 100, 256), (0, 512, 100), (0, 1024, 256), (8, 256, 256),
+                                 (1024, 256, 256)):
+        with pytest.raises(irismpc.IrisMpcError):
+            eng.dot_reduce_chunks(dq, slices, offset, total, chunk, 1)
+    eng.dot_reduce_chunks(dq, slices, 0, 512, 256, 1)  # a fitting run still works
+    with pytest.raises(irismpc.IrisMpcError):
+        eng.dot_reduce_chunks(dq, slices, 0, 256, 256, 1, out_offset=512 * qr)
+
+
 def test_rotation_variant_needs_structured_query(irismpc):
     eng = irismpc.ShareDB(0, 256, 248, 6400, ([0] * 8, [0] * 8))
     slices = eng.alloc_db(256)
