@@ -166,10 +166,20 @@ class BatchRunner:
         stream_synchronize(self.stream)
 
 
+def pick_step_chunks(per_chunk, tol=0.03):
+    """Chunks per scan step from measured seconds per chunk {k: s}: the
+    smallest k within `tol` of the fastest. A larger step shortens the scan
+    only by what it saves per chunk, while a request arriving meanwhile waits
+    up to a whole step to join."""
+    best = min(per_chunk.values())
+    return min(k for k, t in per_chunk.items() if t <= best * (1 + tol))
+
+
 class Autotuner:
-    """Picks the GEMM variant for a matrix and the pack size of a plan by
-    timing them on the device; choices are cached per device, code length and
-    row count in a JSON file (default iris-mpc_amd/_build/autotune.json)."""
+    """Picks the GEMM variant for a matrix and the pack size of a plan, and
+    the serving form's step size (chunks per threshold comparison) per number
+    of active queries, by timing them on the device; choices are cached per
+    device and shape in a JSON file (default iris-mpc_amd/_build/autotune.json)."""
 
     def __init__(self, path=None):
         here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -251,3 +261,26 @@ class Autotuner:
         self.log.append((key, times))
         self._save()
         return best
+
+    def step_chunks(self, eng, buckets=(1, 2, 4, 8, 16, 32, 64), ks=(1, 2, 4, 8, 16), tol=0.03):
+        """Chunks per scan step of the serving form (`eng`: ServingEngines) for
+        each bucket of active queries: every k that the result buffers and the
+        DB allow is timed as one step of that many queries, and
+        pick_step_chunks chooses. Returns {bucket: k}."""
+        n_chunks = eng.n_chunks(eng.n_rows)
+        key = (f"{self._device()}/steps/rows={eng.n_rows}/max_active={eng.max_active}"
+               f"/b={','.join(map(str, buckets))}/k={','.join(map(str, ks))}/tol={tol}")
+        if key in self.cache:
+            return {int(b): k for b, k in self.cache[key]["chunks"].items()}
+        table, times = {}, {}
+        for b in buckets:
+            if b > eng.max_active:
+                break
+            cands = [k for k in ks if k * b <= eng.max_active and k <= n_chunks] or [1]
+            per = {k: eng.time_step(b, k) / k for k in cands}
+            table[b] = pick_step_chunks(per, tol)
+            times[str(b)] = {str(k): v for k, v in per.items()}
+        self.cache[key] = {"chunks": {str(b): k for b, k in table.items()}, "seconds_per_chunk": times}
+        self.log.append((key, times))
+        self._save()
+        return table

@@ -120,6 +120,7 @@ class ServingEngines:
         self._xstreams = [torch.cuda.ExternalStream(s) for s in self.scan.streams]
         self._small_xstreams = [torch.cuda.ExternalStream(s) for s in self.small.streams]
         self._small_qsets = None  # the reauth engines' query sets, refilled per request
+        self.step_table = None  # {active-query bucket: chunks per step} from tune_steps
 
     def sync_scan(self):
         """Waits for every queued scan step."""
@@ -181,9 +182,38 @@ class ServingEngines:
         launches alone would leave the GPU partly idle), one under load so that
         arrivals join at short intervals; bounded by the result buffers
         (chunks x rows within the engines' query rows) and the ring's end."""
-        k = 8 if batch == 1 else 4 if batch <= 2 else 2 if batch <= 8 else 1
+        if self.step_table:
+            fits = [b for b in sorted(self.step_table) if b >= batch]
+            k = self.step_table[fits[0] if fits else max(self.step_table)]
+        else:
+            k = 8 if batch == 1 else 4 if batch <= 2 else 2 if batch <= 8 else 1
         k = min(k, max(1, (self.max_active * ROT) // (batch * ROT)), n_chunks - chunk_index)
         return max(1, k)
+
+    def time_step(self, batch, n_chunks, reps=2):
+        """Seconds of one scan step of `batch` queries (default-iris shares)
+        over n_chunks chunks from the ring's start, best of `reps` after one
+        warm-up; the slots it writes are cleared when requests take them."""
+        if self._dummy is None:
+            self._dummy = dummy_shares()
+        qc, qm = self._query_sets(self.scan, [self._dummy] * batch, batch * ROT)
+        slots = list(range(batch))
+        best = float("inf")
+        for r in range(reps + 1):
+            self.sync_scan()
+            t0 = time.perf_counter()
+            self.step(qc, qm, batch, 0, self.n_rows, slots, n_chunks=n_chunks)
+            self.sync_scan()
+            if r:
+                best = min(best, time.perf_counter() - t0)
+        return best
+
+    def tune_steps(self, tuner=None, **kw):
+        """Step sizes per active-query count from the autotuner (batching.Autotuner)."""
+        from .batching import Autotuner
+
+        self.step_table = (tuner or Autotuner()).step_chunks(self, **kw)
+        return self.step_table
 
     def step(self, qc, qm, batch, chunk_index, db_rows, slots, max_queued=2, n_chunks=1):
         """Queues one step of the scan engines for the active query set over

@@ -1,5 +1,5 @@
 """Batching form planning (host logic, no GPU): the C4 batch-size
-distribution and the greedy pack planner."""
+distribution, the greedy pack planner and the serving step-size choice."""
 import numpy as np
 
 
@@ -27,3 +27,16 @@ def test_range_advance_pads_the_last_chunk():
 
     # the scan launches 32768, 32768 and 64-padded 4480 rows
     assert range_advance(70000, 32768, 62) == 2 * counter_advance(32768, 32768, 62) + counter_advance(4480, 32768, 62)
+
+
+def test_pick_step_chunks():
+    from irismpc.batching import pick_step_chunks
+
+    # seconds per chunk: larger steps amortise fixed costs, then flatten
+    per = {1: 1.0, 2: 0.8, 4: 0.7, 8: 0.69, 16: 0.685}
+    assert pick_step_chunks(per, tol=0.03) == 4  # 0.7 <= 0.685 * 1.03
+    assert pick_step_chunks(per, tol=0.0) == 16
+    assert pick_step_chunks(per, tol=0.5) == 1
+    assert pick_step_chunks({1: 2.0}) == 1
+    # a slower large step is never chosen over the fastest smaller one
+    assert pick_step_chunks({1: 1.0, 2: 0.9, 4: 0.95}, tol=0.0) == 2

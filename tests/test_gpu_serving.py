@@ -8,7 +8,8 @@ Checked on a 4-chunk DB of synthetic irises (4,096 distinct irises tiled):
 - reauth against the right / a wrong row; an inserted iris is found by a
   later query; a deleted row no longer matches;
 - requests overlap in the active set; admission by free HBM lets one request
-  run at a time when the reserve leaves no room.
+  run at a time when the reserve leaves no room;
+- step sizes chosen by the autotuner leave every result unchanged.
 """
 import numpy as np
 import pytest
@@ -101,3 +102,22 @@ def test_admission_by_free_hbm(served):
     out = srv.run(reqs)
     assert all(r.result == [] for r in out)
     assert srv.max_concurrent == 1 and srv.admission_waits > 0
+
+
+def test_tuned_step_sizes_keep_results(served, tmp_path):
+    eng, serving, synth = served
+    from irismpc.batching import Autotuner
+
+    sh, _, _ = _query(synth, 71, 5, 90, 2000, 90)
+    plain = eng.run_alone(sh)
+    assert len(plain) >= N // U - 1 and all(r % U == 5 for r in plain)  # one copy may be deleted
+    table = eng.tune_steps(Autotuner(str(tmp_path / "tune.json")))
+    try:
+        assert set(table) == {1, 2, 4, 8}  # buckets up to max_active = 8
+        for b, k in table.items():
+            assert k >= 1 and k * b <= eng.max_active and k <= eng.n_chunks(eng.n_rows)
+        assert eng.run_alone(sh) == plain
+        # cached: a second tuner on the same file returns the same table without timing
+        assert Autotuner(str(tmp_path / "tune.json")).step_chunks(eng) == table
+    finally:
+        eng.step_table = None
