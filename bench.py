@@ -452,6 +452,12 @@ def run_c5(args):
     def delete(k):
         return serving.Request("delete", 0, target=int(rng.integers(N)))
 
+    # chunks per step per active-query count, timed on this device
+    from irismpc.batching import Autotuner
+
+    tuner = Autotuner()
+    step_table = eng.tune_steps(tuner)
+
     # plain form: one uniqueness request at a time, a full scan each
     alone = [uniq(k).shares for k in range(args.plain_requests)]
     eng.run_alone(alone[0])  # warm-up
@@ -510,6 +516,8 @@ def run_c5(args):
         "plain_form": {"comparisons_per_s": plain_cps, "ms_per_uniqueness_request": t_plain * 1e3,
                        "requests": len(alone)},
         "speedup_vs_plain": capacity["comparisons_per_s"] / plain_cps,
+        "autotune": {"chunks_per_step": {str(b): k for b, k in step_table.items()},
+                     "log": {k: v for k, v in tuner.log}},
         "roofline": None,
     }
 

@@ -166,13 +166,17 @@ class BatchRunner:
         stream_synchronize(self.stream)
 
 
-def pick_step_chunks(per_chunk, tol=0.03):
-    """Chunks per scan step from measured seconds per chunk {k: s}: the
-    smallest k within `tol` of the fastest. A larger step shortens the scan
-    only by what it saves per chunk, while a request arriving meanwhile waits
-    up to a whole step to join."""
-    best = min(per_chunk.values())
-    return min(k for k, t in per_chunk.items() if t <= best * (1 + tol))
+def pick_step_chunks(per_chunk, tol=0.03, max_step=None):
+    """Chunks per scan step from measured seconds per chunk {k: s}. Steps
+    longer than max_step seconds are left out (a request arriving meanwhile
+    waits up to a whole step to join the active set; when every step is
+    longer, the smallest k). Of the rest, the smallest k within `tol` of the
+    fastest per chunk: a larger step pays only by what it saves per chunk."""
+    fits = {k: t for k, t in per_chunk.items() if max_step is None or k * t <= max_step}
+    if not fits:
+        return min(per_chunk)
+    best = min(fits.values())
+    return min(k for k, t in fits.items() if t <= best * (1 + tol))
 
 
 class Autotuner:
@@ -262,14 +266,17 @@ class Autotuner:
         self._save()
         return best
 
-    def step_chunks(self, eng, buckets=(1, 2, 4, 8, 16, 32, 64), ks=(1, 2, 4, 8, 16), tol=0.03):
+    def step_chunks(self, eng, buckets=(1, 2, 4, 8, 16, 32, 64), ks=(1, 2, 4, 8, 16), tol=0.03,
+                    max_step_s=0.008):
         """Chunks per scan step of the serving form (`eng`: ServingEngines) for
         each bucket of active queries: every k that the result buffers and the
         DB allow is timed as one step of that many queries, and
-        pick_step_chunks chooses. Returns {bucket: k}."""
+        pick_step_chunks chooses (steps of at most max_step_s seconds). A
+        bucket's choice serves active counts from it up to the next bucket.
+        Returns {bucket: k}."""
         n_chunks = eng.n_chunks(eng.n_rows)
         key = (f"{self._device()}/steps/rows={eng.n_rows}/max_active={eng.max_active}"
-               f"/b={','.join(map(str, buckets))}/k={','.join(map(str, ks))}/tol={tol}")
+               f"/b={','.join(map(str, buckets))}/k={','.join(map(str, ks))}/tol={tol}/max={max_step_s}")
         if key in self.cache:
             return {int(b): k for b, k in self.cache[key]["chunks"].items()}
         table, times = {}, {}
@@ -278,7 +285,7 @@ class Autotuner:
                 break
             cands = [k for k in ks if k * b <= eng.max_active and k <= n_chunks] or [1]
             per = {k: eng.time_step(b, k) / k for k in cands}
-            table[b] = pick_step_chunks(per, tol)
+            table[b] = pick_step_chunks(per, tol, max_step_s)
             times[str(b)] = {str(k): v for k, v in per.items()}
         self.cache[key] = {"chunks": {str(b): k for b, k in table.items()}, "seconds_per_chunk": times}
         self.log.append((key, times))

@@ -181,10 +181,12 @@ class ServingEngines:
         """Chunks one step covers: several while few queries are active (their
         launches alone would leave the GPU partly idle), one under load so that
         arrivals join at short intervals; bounded by the result buffers
-        (chunks x rows within the engines' query rows) and the ring's end."""
+        (chunks x rows within the engines' query rows) and the ring's end.
+        With a table from tune_steps, the entry of the largest bucket at or
+        below `batch`; otherwise a fixed rule."""
         if self.step_table:
-            fits = [b for b in sorted(self.step_table) if b >= batch]
-            k = self.step_table[fits[0] if fits else max(self.step_table)]
+            under = [b for b in sorted(self.step_table) if b <= batch]
+            k = self.step_table[under[-1] if under else min(self.step_table)]
         else:
             k = 8 if batch == 1 else 4 if batch <= 2 else 2 if batch <= 8 else 1
         k = min(k, max(1, (self.max_active * ROT) // (batch * ROT)), n_chunks - chunk_index)

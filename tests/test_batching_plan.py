@@ -40,3 +40,7 @@ def test_pick_step_chunks():
     assert pick_step_chunks({1: 2.0}) == 1
     # a slower large step is never chosen over the fastest smaller one
     assert pick_step_chunks({1: 1.0, 2: 0.9, 4: 0.95}, tol=0.0) == 2
+    # a step-time cap: k = 16 (11 s) and 8 (5.5 s) exceed 5 s, 4 is the best left
+    assert pick_step_chunks(per, tol=0.03, max_step=5.0) == 4
+    assert pick_step_chunks(per, tol=0.0, max_step=6.0) == 8
+    assert pick_step_chunks(per, max_step=0.5) == 1  # nothing fits: the smallest step
