@@ -522,6 +522,77 @@ def run_c5(args):
     }
 
 
+def run_matmul(args):
+    """The reference's own GEMM benchmark (iris-mpc-gpu/benches/matmul.rs:17-60)
+    on one GPU: DB 2,400,000 x 12,800 u16 code shares, 930 arbitrary query rows
+    (no rotation structure), an engine without exchange (no masks). Each
+    iteration uploads and prepares the query (host -> device copy, limb split,
+    sums) and runs the fused dot + correction over the whole DB, as the
+    reference's iteration does (htod_transfer_query, query_sums, dot,
+    dot_reduce, await_streams). Its criterion throughput counts DB x 930 / 31
+    elements per iteration."""
+    import numpy as np
+    import torch
+
+    import irismpc
+
+    torch.cuda.set_device(0)
+    N, Qr, K = 2_400_000, 930, irismpc.IRIS_CODE_LENGTH
+    stream = irismpc.stream_create()
+    xs = torch.cuda.ExternalStream(stream)
+    eng = irismpc.ShareDB(0, N, Qr, K, ([0] * 8, [0] * 8))
+    if args.variant != -1:
+        eng.set_variant(args.variant)
+    db = eng.alloc_db(N)
+    db.fill_random(0, N, np.arange(8) + 7, 0, stream)
+    q = np.random.default_rng(42).integers(0, 1 << 16, (Qr, K), dtype=np.uint16)
+    irismpc.stream_synchronize(stream)
+    kev = []
+
+    def iteration(ev=None):
+        dq = eng.upload_query(q, stream)
+        if ev is not None:
+            ev[0].record(xs)
+        eng.dot_reduce(dq, db, N, 0, 1, stream)
+        if ev is not None:
+            ev[1].record(xs)
+        irismpc.stream_synchronize(stream)
+        return dq
+
+    for _ in range(args.warmup):
+        iteration()
+    times = []
+    for _ in range(args.steps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        t0 = time.perf_counter()
+        dq = iteration(ev)
+        times.append(time.perf_counter() - t0)
+        kev.append(ev)
+        del dq
+    t = sum(times) / len(times)
+    gemm_ms = sum(e[0].elapsed_time(e[1]) for e in kev) / len(kev)
+    ops = 6 * K * Qr * N
+    achieved = ops / (gemm_ms / 1e3) / 1e12
+    variant = args.variant if args.variant != -1 else 7  # automatic: 7 for rows without rotation structure
+    return {
+        "metric": "3-party iris comparisons/sec on 1 MI355X + p50/p99 request latency",
+        "value": Qr * N / t,
+        "unit": "comparisons/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8 MFMA (i32 acc) -> u16 shares",
+        "data": "synthetic: uniformly random u16 DB and query rows",
+        "config": {"workload": "the reference's matmul bench: DB 2,400,000 x 12,800 u16 code shares, "
+                               "930 query rows without rotation structure, no masks; per iteration the "
+                               "query upload + preparation and the fused dot + correction",
+                   "db_rows": N, "query_rows": Qr, "gemm_variant": variant},
+        "reference_throughput_elements_per_s": N * Qr / 31 / t,
+        "gemm_ms": gemm_ms,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
+                     "frac": achieved / INT8_PEAK_TOPS, "traffic": None},
+    }
+
+
 def run_circuits(args):
     """Threshold circuits alone at one chunk of the full protocol: the three
     parties' lift, bit injection, MSB adder and open over 32,768 DB rows x
@@ -589,12 +660,14 @@ def main():
     ap.add_argument("--variant", type=int, default=-1,
                     help="GEMM kernel variant (-1: automatic, the rotation-aware kernel)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("c2", "c3", "c3-two-stage", "c4", "c5", "circuits"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c3-two-stage", "c4", "c5", "circuits", "matmul"),
+                    default="c2",
                     help="c2: per-party share GEMM (the headline line); c3: full 3-party protocol, "
                          "both eyes; c4: batching form (use --db 4194304 for the full-size DB); "
                          "c5: serving form (use --db 2097152 for the full-size DB); circuits: the "
                          "threshold circuits alone at one chunk; c3-two-stage: the reference's batch flow "
-                         "(one-eye prefilter scan, both eyes on the candidates)")
+                         "(one-eye prefilter scan, both eyes on the candidates); matmul: the reference's "
+                         "own GEMM benchmark shape (2.4M x 930 rows, code only, query upload included)")
     ap.add_argument("--rates", default="2,5,10,20,40", help="c5: request rates per second")
     ap.add_argument("--seconds", type=float, default=8.0, help="c5: arrival window per rate")
     ap.add_argument("--plain-requests", type=int, default=8, help="c5: requests timed in the plain form")
@@ -619,6 +692,9 @@ def main():
         return
     if args.workload == "c5":
         print(json.dumps(run_c5(args)))
+        return
+    if args.workload == "matmul":
+        print(json.dumps(run_matmul(args)))
         return
 
     import numpy as np
