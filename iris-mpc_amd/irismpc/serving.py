@@ -63,6 +63,19 @@ def dummy_shares(rand_dataset=0x0de1e7e, index=0):
     return [(sh[p][0][0], sh[p][1][0]) for p in range(3)]
 
 
+def bitmap_rows(words, n):
+    """Set bits below n of a little-endian u64 bitmap, ascending: only the
+    nonzero words are unpacked (a 2M-row bitmap row has a handful)."""
+    w = np.ascontiguousarray(words).view(np.uint64)
+    nz = np.flatnonzero(w)
+    if nz.size == 0:
+        return []
+    bits = np.unpackbits(w[nz].view(np.uint8).reshape(-1, 8), axis=1, bitorder="little")
+    i, j = np.nonzero(bits)
+    rows = nz[i].astype(np.int64) * 64 + j
+    return [int(r) for r in rows[rows < n]]
+
+
 class PartySet:
     """Engines of the three parties for one query-set size: codes/masks ShareDB,
     Circuits and DistanceComparator per party, one exchange, three streams."""
@@ -121,6 +134,7 @@ class ServingEngines:
         self._small_xstreams = [torch.cuda.ExternalStream(s) for s in self.small.streams]
         self._small_qsets = None  # the reauth engines' query sets, refilled per request
         self.step_table = None  # {active-query bucket: chunks per step} from tune_steps
+        self._alone_qsets = None  # the plain form's query sets, refilled per request
 
     def sync_scan(self):
         """Waits for every queued scan step."""
@@ -252,9 +266,7 @@ class ServingEngines:
 
     def slot_matches(self, slot, db_rows):
         self.sync_scan()
-        bits = self.bitmaps[0][slot * self.words:(slot + 1) * self.words].cpu().numpy().view(np.uint8)
-        rows = np.nonzero(np.unpackbits(bits, bitorder="little")[:db_rows])[0]
-        return [int(r) for r in rows]
+        return bitmap_rows(self.bitmaps[0][slot * self.words:(slot + 1) * self.words].cpu().numpy(), db_rows)
 
     def reauth(self, shares, target):
         """One query x 31 rotations against DB row `target` on the small engine set."""
@@ -277,7 +289,10 @@ class ServingEngines:
 
     def run_alone(self, shares):
         """The plain per-request form: a full scan of one uniqueness query."""
-        qc, qm = self._query_sets(self.scan, [shares], ROT)
+        # one persistent one-query set, refilled per request (no allocation)
+        if self._alone_qsets is None:
+            self._alone_qsets = self._query_sets(self.scan, [shares], ROT)
+        qc, qm = self._query_sets(self.scan, [shares], ROT, reuse=self._alone_qsets)
         rows = self.n_rows
         self.clear_slot(0)
         c, n = 0, self.n_chunks(rows)
