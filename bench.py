@@ -406,6 +406,8 @@ def run_c4(args):
         "roofline": {"bound": "mfma", "achieved": ops / t_packed / 1e12, "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": ops / t_packed / 1e12 / INT8_PEAK_TOPS, "traffic": None},
         "autotune": {k: v for k, v in tuner.cache.items()},
+        # same per-pair work as one party's masked code + mask dot shares
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(),
     }
 
 
@@ -519,6 +521,8 @@ def run_c5(args):
         "autotune": {"chunks_per_step": {str(b): k for b, k in step_table.items()},
                      "log": {k: v for k, v in tuner.log}},
         "roofline": None,
+        # same work per comparison as a serving request: 3 parties' dot shares + threshold circuits
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_protocol(),
     }
 
 
