@@ -9,7 +9,9 @@ Checked on a 4-chunk DB of synthetic irises (4,096 distinct irises tiled):
   later query; a deleted row no longer matches;
 - requests overlap in the active set; admission by free HBM lets one request
   run at a time when the reserve leaves no room;
-- step sizes chosen by the autotuner leave every result unchanged.
+- step sizes chosen by the autotuner leave every result unchanged;
+- a server starting from an empty DB (as the reference's does) answers
+  uniqueness requests with no matches, then finds an inserted iris.
 """
 import numpy as np
 import pytest
@@ -121,3 +123,20 @@ def test_tuned_step_sizes_keep_results(served, tmp_path):
         assert Autotuner(str(tmp_path / "tune.json")).step_chunks(eng) == table
     finally:
         eng.step_table = None
+
+
+def test_empty_db_then_insert(served):
+    _, serving, synth = served
+    import irismpc
+
+    eng = serving.ServingEngines(0, 2 * irismpc.DB_CHUNK_SIZE, max_active=2, seed_base=9)
+    sh, _, _ = _query(synth, 78, 3, 0, 0, 3)
+    assert eng.run_alone(sh) == []
+    R = serving.Request
+    out = serving.Server(eng, reserve_bytes=0).run([R("uniqueness", 0.0, shares=sh, rid=0),
+                                                    R("insert", 0.0, shares=sh, rid=1)])
+    res = {r.rid: r for r in out}
+    assert res[1].result == 0 and eng.n_rows == 1
+    assert res[0].result in ([], [0])  # checked before or after the insertion, as it was admitted
+    near, _, _ = _query(synth, 78, 3, 5, 1500, 4)  # the inserted iris with 1,500 code bits flipped
+    assert eng.run_alone(near) == [0]
