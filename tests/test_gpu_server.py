@@ -137,6 +137,32 @@ def test_server_two_batches(irismpc):
     assert actor.current_db_size == N0 + 6
 
 
+def test_server_first_batch_on_empty_db(irismpc):
+    """The reference's server starts from an empty DB: its first batch has no
+    DB matches, inserts its unique irises and flags in-batch duplicates; the
+    next batch finds the inserted rows."""
+    from irismpc.server import ServerActor
+
+    actor = ServerActor(MAXDB, MAXB)
+    assert actor.current_db_size == 0
+    new = {n: (_iris(31, n), _iris(32, n)) for n in range(2)}
+    b1 = Batch()
+    b1.add("u0", new[0][0], new[0][1])
+    b1.add("dup0", (_flip(new[0][0][0], 40), new[0][0][1]), (_flip(new[0][1][0], 41), new[0][1][1]))
+    b1.add("u1", new[1][0], new[1][1])
+    r1 = actor.process_batch_query(b1.query())
+    FB = 0xFFFFFFFE
+    assert r1.merged_results == [0, FB, 1]
+    assert r1.matches == [False, True, False]
+    assert r1.matched_batch_request_ids[1] == ["u0"]
+    assert actor.current_db_size == 2
+    b2 = Batch()
+    b2.add("q1", (_flip(new[1][0][0], 42), new[1][0][1]), (_flip(new[1][1][0], 43), new[1][1][1]))
+    r2 = actor.process_batch_query(b2.query())
+    assert r2.merged_results == [1] and r2.matches == [True]
+    assert actor.current_db_size == 2
+
+
 def test_server_mirror_attack(irismpc, orc):
     """Full-face mirror attack detection (actor.rs:585-640,1304-1417): a face
     presented mirrored shows each eye as the mirror image of the other. The
