@@ -426,7 +426,7 @@ def run_c5(args):
     torch.cuda.set_device(0)
     N, U = args.db, min(args.db, 32768)
     rates = [float(r) for r in args.rates.split(",")]
-    eng = serving.ServingEngines(N, N + 4 * irismpc.DB_CHUNK_SIZE, max_active=64, step_capacity=2)
+    eng = serving.ServingEngines(N, N + 4 * irismpc.DB_CHUNK_SIZE, max_active=64)
     eng.build(U, 81, 82)
     rng = np.random.default_rng(11)
     counter = [0]
@@ -519,7 +519,6 @@ def run_c5(args):
                        "requests": len(alone)},
         "speedup_vs_plain": capacity["comparisons_per_s"] / plain_cps,
         "autotune": {"chunks_per_step": {str(b): k for b, k in step_table.items()},
-                     "chunks_per_step_backlog": {str(b): k for b, k in eng.backlog_table.items()},
                      "log": {k: v for k, v in tuner.log}},
         "roofline": None,
         # same work per comparison as a serving request: 3 parties' dot shares + threshold circuits

@@ -267,30 +267,27 @@ class Autotuner:
         return best
 
     def step_chunks(self, eng, buckets=(1, 2, 4, 8, 16, 32, 64), ks=(1, 2, 4, 8, 16), tol=0.03,
-                    max_step_s=0.008, backlog=False):
+                    max_step_s=0.008):
         """Chunks per scan step of the serving form (`eng`: ServingEngines) for
         each bucket of active queries: every k that the result buffers and the
         DB allow is timed as one step of that many queries, and
-        pick_step_chunks chooses (steps of at most max_step_s seconds, a
-        request's wait to join; with backlog=True, for steps taken while every
-        slot is taken and requests wait, time per chunk alone). A bucket's
-        choice serves active counts from it up to the next bucket. Returns
-        {bucket: k}."""
+        pick_step_chunks chooses (steps of at most max_step_s seconds). A
+        bucket's choice serves active counts from it up to the next bucket.
+        Returns {bucket: k}."""
         n_chunks = eng.n_chunks(eng.n_rows)
-        cap = eng.max_active * getattr(eng, "step_capacity", 1)
-        key = (f"{self._device()}/steps/rows={eng.n_rows}/max_active={eng.max_active}/cap={cap}"
+        key = (f"{self._device()}/steps/rows={eng.n_rows}/max_active={eng.max_active}"
                f"/b={','.join(map(str, buckets))}/k={','.join(map(str, ks))}/tol={tol}/max={max_step_s}")
-        if key not in self.cache:
-            times = {}
-            for b in buckets:
-                if b > eng.max_active:
-                    break
-                cands = [k for k in ks if k * b <= cap and k <= n_chunks] or [1]
-                times[str(b)] = {str(k): eng.time_step(b, k) / k for k in cands}
-            pick = lambda limit: {b: pick_step_chunks({int(k): v for k, v in per.items()}, tol, limit)  # noqa: E731
-                                  for b, per in times.items()}
-            self.cache[key] = {"chunks": pick(max_step_s), "chunks_backlog": pick(None),
-                               "seconds_per_chunk": times}
-            self.log.append((key, times))
-            self._save()
-        return {int(b): k for b, k in self.cache[key]["chunks_backlog" if backlog else "chunks"].items()}
+        if key in self.cache:
+            return {int(b): k for b, k in self.cache[key]["chunks"].items()}
+        table, times = {}, {}
+        for b in buckets:
+            if b > eng.max_active:
+                break
+            cands = [k for k in ks if k * b <= eng.max_active and k <= n_chunks] or [1]
+            per = {k: eng.time_step(b, k) / k for k in cands}
+            table[b] = pick_step_chunks(per, tol, max_step_s)
+            times[str(b)] = {str(k): v for k, v in per.items()}
+        self.cache[key] = {"chunks": {str(b): k for b, k in table.items()}, "seconds_per_chunk": times}
+        self.log.append((key, times))
+        self._save()
+        return table

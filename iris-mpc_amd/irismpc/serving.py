@@ -106,7 +106,7 @@ class PartySet:
 class ServingEngines:
     """DB shares of the three parties plus the scan and reauth engine sets."""
 
-    def __init__(self, n_rows, capacity, max_active=64, seed_base=0, threshold_a=None, step_capacity=1):
+    def __init__(self, n_rows, capacity, max_active=64, seed_base=0, threshold_a=None):
         import torch
 
         self.torch = torch
@@ -114,10 +114,7 @@ class ServingEngines:
         self.A = E.THRESHOLD_ANON_STATS_A if threshold_a is None else threshold_a
         base = [np.full(8, seed_base + p, np.uint32) for p in range(3)]
         seeds = [(base[p], base[(p + 2) % 3]) for p in range(3)]
-        # the scan engines' result buffers hold step_capacity chunks of the full
-        # active set, so a step of a full active set may cover that many chunks
-        self.step_capacity = step_capacity
-        self.scan = PartySet(max_active * ROT * step_capacity, capacity, seeds, 0)
+        self.scan = PartySet(max_active * ROT, capacity, seeds, 0)
         self.small = PartySet(ROT, capacity, seeds, 8)  # reauth: one query, one chunk of 64 rows
         self.code_dbs = [self.scan.codes[p].alloc_db(capacity) for p in range(3)]
         self.mask_dbs = [self.scan.masks[p].alloc_db(capacity) for p in range(3)]
@@ -137,7 +134,6 @@ class ServingEngines:
         self._small_xstreams = [torch.cuda.ExternalStream(s) for s in self.small.streams]
         self._small_qsets = None  # the reauth engines' query sets, refilled per request
         self.step_table = None  # {active-query bucket: chunks per step} from tune_steps
-        self.backlog_table = None  # the same while every slot is taken and requests wait
         self._alone_qsets = None  # the plain form's query sets, refilled per request
 
     def sync_scan(self):
@@ -195,22 +191,19 @@ class ServingEngines:
     def n_chunks(self, rows):
         return max(1, -(-rows // CHUNK))
 
-    def chunks_per_step(self, batch, chunk_index, n_chunks, backlog=False):
+    def chunks_per_step(self, batch, chunk_index, n_chunks):
         """Chunks one step covers: several while few queries are active (their
         launches alone would leave the GPU partly idle), one under load so that
         arrivals join at short intervals; bounded by the result buffers
         (chunks x rows within the engines' query rows) and the ring's end.
         With a table from tune_steps, the entry of the largest bucket at or
-        below `batch`; otherwise a fixed rule. With `backlog` (every slot taken,
-        requests waiting) no arrival can join before a request finishes, so the
-        backlog table, chosen for time per chunk alone, applies."""
-        table = self.backlog_table if backlog and self.backlog_table else self.step_table
-        if table:
-            under = [b for b in sorted(table) if b <= batch]
-            k = table[under[-1] if under else min(table)]
+        below `batch`; otherwise a fixed rule."""
+        if self.step_table:
+            under = [b for b in sorted(self.step_table) if b <= batch]
+            k = self.step_table[under[-1] if under else min(self.step_table)]
         else:
             k = 8 if batch == 1 else 4 if batch <= 2 else 2 if batch <= 8 else 1
-        k = min(k, max(1, (self.max_active * self.step_capacity) // batch), n_chunks - chunk_index)
+        k = min(k, max(1, (self.max_active * ROT) // (batch * ROT)), n_chunks - chunk_index)
         return max(1, k)
 
     def time_step(self, batch, n_chunks, reps=2):
@@ -235,9 +228,7 @@ class ServingEngines:
         """Step sizes per active-query count from the autotuner (batching.Autotuner)."""
         from .batching import Autotuner
 
-        tuner = tuner or Autotuner()
-        self.step_table = tuner.step_chunks(self, **kw)
-        self.backlog_table = tuner.step_chunks(self, backlog=True, **kw)
+        self.step_table = (tuner or Autotuner()).step_chunks(self, **kw)
         return self.step_table
 
     def step(self, qc, qm, batch, chunk_index, db_rows, slots, max_queued=2, n_chunks=1):
@@ -404,7 +395,7 @@ class Server:
             cursor %= n_chunks
             # no lock: scan steps only read the DB, and the DB writes (insert,
             # delete) run on this thread after the queued steps finished
-            k = eng.chunks_per_step(len(active), cursor, n_chunks, backlog=bool(waiting) and not free_slots)
+            k = eng.chunks_per_step(len(active), cursor, n_chunks)
             eng.step(qsets[0], qsets[1], len(active), cursor, eng.n_rows, [r.slot for r in active],
                      n_chunks=k)
             self.steps += 1

@@ -118,13 +118,11 @@ def test_tuned_step_sizes_keep_results(served, tmp_path):
         assert set(table) == {1, 2, 4, 8}  # buckets up to max_active = 8
         for b, k in table.items():
             assert k >= 1 and k * b <= eng.max_active and k <= eng.n_chunks(eng.n_rows)
-        assert set(eng.backlog_table) == set(table)
-        assert all(k * b <= eng.max_active * eng.step_capacity for b, k in eng.backlog_table.items())
         assert eng.run_alone(sh) == plain
         # cached: a second tuner on the same file returns the same table without timing
         assert Autotuner(str(tmp_path / "tune.json")).step_chunks(eng) == table
     finally:
-        eng.step_table = eng.backlog_table = None
+        eng.step_table = None
 
 
 def test_empty_db_then_insert(served):
