@@ -1,16 +1,22 @@
-"""Benchmark of the share GEMM hot path on one MI355X.
+"""Benchmark of the secret-shared iris-matching hot path on one MI355X.
 
-Workload (per-party share GEMM, plain form): a DB share of 1,048,576 irises
-(u16 code shares, 12,800 coefficients, and trimmed-mask shares, 6,400
-coefficients, stored as biased i8 limbs in HBM) against one query batch of
-64 queries x 31 rotations = 1,984 rows. One step = the code and the mask
-dot products with the fused bias correction, mask x2 and ChaCha12 zero-share
-masking, over the whole DB in reference chunks of 32,768 rows.
+Default workload (the headline line): the full 3-party protocol on one GPU,
+both eyes. Three parties each hold a left- and a right-eye DB share of
+1,048,576 irises in HBM (u16 code shares of 12,800 coefficients and trimmed
+mask shares of 6,400, stored as biased i8 limbs); one step is one batch of
+64 queries x 31 rotations through the whole pipeline: per 32,768-row chunk
+the six share GEMMs (3 parties x code, mask) with the fused correction and
+ChaCha12 zero-share masks, the reshare, the 16 -> 32-bit lift, bit injection
+and MSB comparison circuits at the 0.375 threshold, the open into match
+bitmaps (OR over rotations), then the AND over eyes and first-match merge.
+After the timed steps a serving pass on the same resident left-eye DB
+measures request latency (p50 / p99) under Poisson arrivals.
 
 Prints one JSON line (driver contract). Comparisons = query-rotation rows x
-DB rows. The CPU baseline is the oracle's OpenMP restatement of the
-reference CPU loop (iris-mpc-cpu/src/protocol/ops.rs:238-347) timed on a
-bounded sample on this host.
+DB rows (x 2 eyes). The CPU baseline is the oracle's restatement of the same
+3-party step (the reference CPU dot loop iris-mpc-cpu/src/protocol/ops.rs:
+238-347 for the three parties' dot shares plus the threshold circuits) timed
+on a bounded sample on this host.
 """
 import argparse
 import json
@@ -27,14 +33,22 @@ INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, 5.0 POPS
 HBM_PEAK_GBS = 8000.0
 # 32-bit integer VALU lane-ops: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_PEAK_TOPS = 39.3
-# ChaCha12 blocks per (query-rotation, DB) pair per party in the threshold
-# circuits (one bit per AND gate and element, two streams per party): lift 32
-# ANDs + MSB adder 61 ANDs -> 93 x 2 / 512; bit injection: 2 u16 per pair from
-# each of the party's two streams, drawn once -> 2 x 4 / 64
-CHACHA_BLOCKS_PER_PAIR = 93 * 2 / 512 + 2 * 4 / 64
+# Distinct ChaCha12 blocks per (query-rotation, DB) pair per party in the
+# threshold circuits: every AND gate draws one bit per element from each of
+# the three pairwise streams of the trio (F(own) ^ F(prev) per party, each
+# pairwise block shared by two parties), i.e. one block per 512 elements per
+# party and gate: lift 32 ANDs + MSB adder 61 ANDs -> 93 / 512; bit injection
+# draws 2 u16 per pair from each pairwise stream -> 4 / 64
+CHACHA_BLOCKS_PER_PAIR = 93 / 512 + 4 / 64
+# HBM bytes per pair per party of the circuit stage: the lift reads the mask
+# share (a, b: 4 B), the injection writes its corrections (8 B), the MSB adder
+# reads code, mask and corrections (16 B), the opens read ~3 bits
+CIRCUIT_BYTES_PER_PAIR = 4 + 8 + 16 + 3 / 8
 CHACHA12_OPS = 600  # 6 double rounds x 8 quarter rounds x 12 ops + feed-forward and set-up
 THRESHOLD_ANON_A = 16384  # 0.375: the prefilter threshold of the full scan
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "share_gemm_c2_traffic.json")
+C3_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "share_gemm_c3_traffic.json")
+INT8_OPS_PER_PAIR = 115200  # 2 x 3 limb products x (12,800 + 6,400) per pair per party
 # kernel behind each rotation-aware GEMM variant id; the automatic choice (-1)
 # for raw-share queries is 102 up to two queries (62 rows), else 100
 KERNEL_OF_VARIANT = {100: "share_gemm_rot_kernel<4, 2, 4, 1>", 101: "share_gemm_rot_kernel<4, 2, 2, 1>",
@@ -45,17 +59,22 @@ ROT_SMALL_ROWS = 62
 RIDGE_OPS_PER_BYTE = INT8_PEAK_TOPS / HBM_PEAK_GBS * 1e3
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, path=TRAFFIC_FILE):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (None if absent)."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
     if not any(kernel in k for k in t.get("kernel", [])):
         return None
     return t["traffic_bytes_per_launch"]
+
+
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON result)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(n_target_s=12.0):
@@ -143,10 +162,11 @@ def cpu_baseline_protocol(n_target_s=12.0):
     }
 
 
-def _c3_engines(args):
+def _c3_engines(args, extra_left=0):
     """Both eyes' engines of the full protocol: 3 parties per eye, each party
-    holding the eye's DB share of args.db rows in HBM, synthetic irises with
-    planted queries (copies of DB irises, 10-30 % of code bits flipped)."""
+    holding the eye's DB share of args.db rows in HBM (the left eye's slices
+    with room for extra_left more rows), synthetic irises with planted queries
+    (copies of DB irises, 10-30 % of code bits flipped)."""
     import numpy as np
     import torch
 
@@ -176,7 +196,8 @@ def _c3_engines(args):
             masks = irismpc.ShareDB(p, chunk, Qr, irismpc.MASK_CODE_LENGTH, seed(1), comm=ex, tag=1)
             circ = irismpc.Circuits(p, Qr * chunk, Qr * chunk // 64, seed(5), ex, tag=2)
             cmp = irismpc.DistanceComparator(Qr, N)
-            parties.append(irismpc.PartyEngines(codes, masks, circ, cmp, codes.alloc_db(N), masks.alloc_db(N)))
+            cap = N + (extra_left if eye == 0 else 0)
+            parties.append(irismpc.PartyEngines(codes, masks, circ, cmp, codes.alloc_db(cap), masks.alloc_db(cap)))
             bitmaps.append(torch.zeros(N * B // 64 + 2, dtype=torch.int64, device="cuda"))
             streams.append(irismpc.stream_create())
         synth.build_db([pe.code_db for pe in parties], [pe.mask_db for pe in parties], N, U, ds, ds + 30)
@@ -190,25 +211,93 @@ def _c3_engines(args):
     return N, B, Qr, chunk, U, n_plant, eyes, expect
 
 
+def _serving_requests(rng, U, N, ds, rand_ds, counter):
+    """Request builders of the serving pass over a DB whose row r holds
+    synthetic iris (ds, r mod U) shared with randomness (rand_ds, r mod U):
+    a quarter of the uniqueness requests are returning users (a DB iris with
+    10-30 % of its code bits flipped), reauths target a DB row with 1,500
+    flips, insertions add new irises, deletions hit random rows."""
+    from irismpc import serving, synth
+
+    def shares(dataset, t, flips):
+        c, m = synth.synth_iris(dataset, t)
+        if flips:
+            c = synth.synth_flip(c, 85, counter[0], flips)
+        sh = synth.shares_of(c[None], m[None], 86, counter[0])
+        counter[0] += 1
+        return [(sh[p][0][0], sh[p][1][0]) for p in range(3)]
+
+    def uniq(k):
+        if rng.random() < 0.25:
+            return serving.Request("uniqueness", 0, shares=shares(ds, int(rng.integers(U)),
+                                                                  int(rng.integers(1280, 3841))))
+        return serving.Request("uniqueness", 0, shares=shares(83, counter[0], 0))
+
+    def reauth(k):
+        t = int(rng.integers(U))
+        return serving.Request("reauth", 0, shares=shares(ds, t, 1500), target=t)
+
+    def insert(k):
+        return serving.Request("insert", 0, shares=shares(84, counter[0], 0))
+
+    def delete(k):
+        return serving.Request("delete", 0, target=int(rng.integers(N)))
+
+    return uniq, reauth, insert, delete
+
+
+def _latency_pass(args, parties, N, U):
+    """Serving form on the resident left-eye DB shares of the full-protocol
+    engines (3 parties x N rows): Poisson arrivals at args.latency_rate per
+    second for args.latency_seconds, 85 % uniqueness / 10 % reauth / 5 %
+    insertion or deletion, continuous batching of the scan (step sizes from
+    the autotuner). Returns the latency summary."""
+    import numpy as np
+
+    import irismpc
+    from irismpc import serving
+    from irismpc.batching import Autotuner
+
+    eng = serving.ServingEngines(N, N + 2 * irismpc.DB_CHUNK_SIZE, max_active=64, seed_base=100,
+                                 dbs=([pe.code_db for pe in parties], [pe.mask_db for pe in parties]))
+    steps = eng.tune_steps(Autotuner())
+    rng = np.random.default_rng(12)
+    uniq, reauth, insert, delete = _serving_requests(rng, U, N, 1, 31, [0])
+    n = max(16, int(args.latency_rate * args.latency_seconds))
+    reqs = serving.poisson_mix(args.latency_rate, n, rng, uniq, reauth, insert, delete)
+    srv = serving.Server(eng)
+    out = srv.run(reqs)
+    span = max(r.done for r in out) - min(r.arrival for r in out)
+    comps = sum(31 * r.db_rows for r in out if r.kind == "uniqueness") + sum(31 for r in out if r.kind == "reauth")
+    return {"rate_per_s": args.latency_rate, "requests": n, "db_rows": N,
+            "mix": "85 % uniqueness (vs the whole DB) / 10 % reauth (vs one row) / 5 % insert+delete",
+            "all": serving.latency_stats(out), "uniqueness": serving.latency_stats(out, "uniqueness"),
+            "reauth": serving.latency_stats(out, "reauth"), "comparisons_per_s": comps / span,
+            "max_concurrent": srv.max_concurrent, "chunks_per_step": {str(b): k for b, k in steps.items()}}
+
+
 def run_c3(args):
-    """Full 3-party protocol on one GPU, both eyes: 3 party streams per eye, each
-    party holding a left- and a right-eye DB share of args.db rows in HBM; share
+    """Full 3-party protocol on one GPU, both eyes: 3 parties per eye, each
+    holding a left- and a right-eye DB share of args.db rows in HBM; share
     GEMM, reshare, lift, threshold circuits, open, OR over rotations, then AND
-    over eyes."""
+    over eyes. The share GEMM and circuit launches of the timed steps are timed
+    with HIP events on their streams; a serving pass then measures request
+    latency on the left eye's DB."""
+    import gc
+
     import numpy as np
     import torch
 
     import irismpc
 
-    N, B, Qr, chunk, U, n_plant, eyes, expect = _c3_engines(args)
+    latency = args.latency_rate > 0
+    progress("building the 3 parties x 2 eyes DB shares")
+    N, B, Qr, chunk, U, n_plant, eyes, expect = _c3_engines(args, 2 * irismpc.DB_CHUNK_SIZE if latency else 0)
+    progress("DB shares resident; warm-up")
     finals = [torch.zeros(B, dtype=torch.int32, device="cuda") for _ in range(3)]
+    engines = [x for e in eyes for pe in e["parties"] for x in (pe.codes, pe.masks)]
+    circuits = [e["parties"][0].circuits for e in eyes]  # party 0 keeps the trio's intervals
     torch.cuda.synchronize()
-
-    def scan(e):
-        irismpc.scan_db_trio(e["parties"], e["qcs"], e["qms"], N, B, irismpc.THRESHOLD_ANON_STATS_A,
-                             [b.data_ptr() for b in e["bitmaps"]], e["streams"])
-        for st in e["streams"]:
-            irismpc.stream_synchronize(st)
 
     def batch():
         for e in eyes:
@@ -219,42 +308,47 @@ def run_c3(args):
         for pe in eyes[0]["parties"]:
             pe.comparator.reset()
         torch.cuda.synchronize()
-        # eyes one after the other (each eye's 3 parties run concurrently); two
-        # eyes at once oversubscribe the hardware queues and run slower
-        for e in eyes:
-            scan(e)
+        for e in eyes:  # each eye: the three parties' chunk loop
+            irismpc.scan_db_trio(e["parties"], e["qcs"], e["qms"], N, B, irismpc.THRESHOLD_ANON_STATS_A,
+                                 [b.data_ptr() for b in e["bitmaps"]], e["streams"])
         for p in range(3):  # each party merges its own opened bitmaps (AND over eyes)
             eyes[0]["parties"][p].comparator.join_db_matches(
                 eyes[0]["bitmaps"][p].data_ptr(), eyes[1]["bitmaps"][p].data_ptr(), finals[p].data_ptr(), N)
         torch.cuda.synchronize()
 
+    for x in engines + circuits:
+        x.set_timing(True)
     for _ in range(args.warmup):
         batch()
+    for x in engines + circuits:
+        x.launch_times()  # drop the warm-up intervals
+    progress(f"timing {args.steps} batches")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         batch()
     ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    progress(f"{ms_step:.1f} ms per batch")
+    gemm_n, gemm_ms = 0, 0.0
+    for x in engines:
+        n, ms = x.launch_times()
+        gemm_n, gemm_ms = gemm_n + n, gemm_ms + ms
+    circ_n, circ_ms = 0, 0.0
+    for x in circuits:
+        n, ms = x.launch_times()
+        circ_n, circ_ms = circ_n + n, circ_ms + ms
+    for x in engines + circuits:
+        x.set_timing(False)
     same = all(torch.equal(eyes[e]["bitmaps"][0], eyes[e]["bitmaps"][p]) for e in range(2) for p in (1, 2))
     found = all(np.array_equal(f.cpu().numpy().view(np.uint32), expect) for f in finals)
 
-    # share GEMM stage alone, same engines: code + mask GEMMs of all 3 parties x 2 eyes
-    stream = eyes[0]["streams"][0]
-    xs = torch.cuda.ExternalStream(stream)
-    outs = [torch.empty(N * Qr + 64, dtype=torch.int16, device="cuda") for _ in range(2)]
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(xs)
-    for e in eyes:
-        for pe, qc, qm in zip(e["parties"], e["qcs"], e["qms"]):
-            pe.codes.dot_reduce_range(qc, pe.code_db, 0, N, chunk, 1, outs[0].data_ptr(), stream)
-            pe.masks.dot_reduce_range(qm, pe.mask_db, 0, N, chunk, 2, outs[1].data_ptr(), stream)
-    ev1.record(xs)
-    ev1.synchronize()
-    gemm_ms = ev0.elapsed_time(ev1)
-    pairs = Qr * N
-    gemm_ops = 115200 * pairs * 3 * 2
+    pairs = Qr * N  # per eye and party
+    gemm_ops = INT8_OPS_PER_PAIR * pairs * 3 * 2 * args.steps
     achieved = gemm_ops / (gemm_ms / 1e3) / 1e12
-    stream_ops = CHACHA_BLOCKS_PER_PAIR * CHACHA12_OPS * pairs * 3 * 2
-    return {
+    circ_pairs = pairs * 3 * 2 * args.steps  # per party, summed over parties and eyes
+    chacha_ops = CHACHA_BLOCKS_PER_PAIR * CHACHA12_OPS * circ_pairs
+    circ_bytes = CIRCUIT_BYTES_PER_PAIR * circ_pairs
+    kernel = KERNEL_OF_VARIANT[100 if Qr > ROT_SMALL_ROWS else 102]
+    line = {
         "metric": "3-party iris comparisons/sec on 1 MI355X + p50/p99 request latency",
         "value": 2 * pairs / (ms_step / 1e3),
         "unit": "comparisons/s",
@@ -265,22 +359,49 @@ def run_c3(args):
                 f"{U:,} distinct irises per eye tiled over the DB; {n_plant} of {B} queries are copies of DB "
                 "irises with 10-30% of code bits flipped, matched in both eyes",
         "config": {"workload": f"full 3-party protocol on one GPU, both eyes: 3 parties x 2 eyes x {N:,} "
-                               f"DB share rows in HBM, {B} queries x 31 rotations; comparisons = 2 eyes x "
-                               "query-rotation rows x DB rows per batch",
+                               f"DB share rows in HBM, {B} queries x 31 rotations per batch (one step); "
+                               "comparisons = 2 eyes x query-rotation rows x DB rows",
                    "db_rows": N, "query_rows": Qr, "bitmaps_agree_across_parties": same,
                    "planted_matches_exact": found},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
-                     "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
-                     "stage": f"share GEMM alone (6 party-eye x code+mask launches, {gemm_ms:.1f} ms)"},
-        "streaming_stage": {"bound": "valu", "algorithmic_ops": stream_ops,
-                            "serial_ms_est": ms_step - gemm_ms,
-                            "achieved_Tops_est": stream_ops / max(ms_step - gemm_ms, 1e-3) / 1e9,
+                     "frac": achieved / INT8_PEAK_TOPS,
+                     "traffic": measured_traffic(kernel, C3_TRAFFIC_FILE),
+                     "kernel": kernel,
+                     "launches": gemm_n, "avg_launch_ms": gemm_ms / max(gemm_n, 1),
+                     "int8_ops_per_launch_avg": gemm_ops / max(gemm_n, 1),
+                     "basis": "share GEMM launches inside the timed batches (6 per chunk: 3 parties x code, "
+                              "mask, in order on one stream; HIP events on that stream): 115,200 int8 ops per "
+                              "pair per party / launch duration"},
+        "streaming_stage": {"bound": "valu", "kernels": "adder_word_kernel<false/true>, inject_kernel",
+                            "compares": circ_n, "avg_compare_ms": circ_ms / max(circ_n, 1),
+                            "chacha12_ops": chacha_ops,
+                            "achieved_Tops": chacha_ops / (circ_ms / 1e3) / 1e12 if circ_ms else None,
                             "peak_Tops": VALU_PEAK_TOPS,
-                            "note": f"ChaCha12 correlated randomness: {CHACHA_BLOCKS_PER_PAIR:.3f} blocks x "
-                                    f"{CHACHA12_OPS} int32 ops per pair per party; time = batch minus "
-                                    "GEMM-alone time"},
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_protocol(),
+                            "frac": chacha_ops / (circ_ms / 1e3) / 1e12 / VALU_PEAK_TOPS if circ_ms else None,
+                            "hbm_GBps": circ_bytes / (circ_ms / 1e3) / 1e9 if circ_ms else None,
+                            "note": f"{CHACHA_BLOCKS_PER_PAIR:.3f} distinct ChaCha12 blocks x {CHACHA12_OPS} "
+                                    f"int32 ops and {CIRCUIT_BYTES_PER_PAIR:.1f} HBM bytes per pair per party; "
+                                    "one compare = lift + injection + MSB adder of one chunk, three parties, "
+                                    "HIP events on the protocol stream while the next chunk's GEMMs run"},
     }
+    if latency:
+        # the right eye is done with: its DB shares leave HBM before the serving pass
+        left = eyes[0]["parties"]
+        del eyes[1:]
+        eyes[0]["qcs"] = eyes[0]["qms"] = None
+        del engines, circuits, finals
+        gc.collect()
+        torch.cuda.synchronize()
+        progress(f"serving latency pass at {args.latency_rate:g} requests/s")
+        lat = _latency_pass(args, left, N, U)
+        progress(f"p50 {lat['all']['p50_ms']:.1f} ms, p99 {lat['all']['p99_ms']:.1f} ms")
+        line["latency_p50_ms"] = lat["all"]["p50_ms"]
+        line["latency_p99_ms"] = lat["all"]["p99_ms"]
+        line["latency"] = lat
+    if not args.no_cpu_baseline:
+        progress("CPU baseline")
+    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_protocol()
+    return line
 
 
 def run_c3_two_stage(args):
@@ -387,7 +508,7 @@ def run_c4(args):
     comps = runner.comparisons()
     ops = 115200 * comps
     return {
-        "metric": "3-party iris comparisons/sec on 1 MI355X + p50/p99 request latency",
+        "metric": "per-party share GEMM comparisons/sec on 1 MI355X, batching form (one party's code + mask dot shares)",
         "value": comps / t_packed,
         "unit": "comparisons/s",
         "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": t_packed * 1e3,
@@ -577,9 +698,9 @@ def run_matmul(args):
     gemm_ms = sum(e[0].elapsed_time(e[1]) for e in kev) / len(kev)
     ops = 6 * K * Qr * N
     achieved = ops / (gemm_ms / 1e3) / 1e12
-    variant = args.variant if args.variant != -1 else 7  # automatic: 7 for rows without rotation structure
+    variant = args.variant if args.variant != -1 else 0  # automatic: 0 for rows without rotation structure
     return {
-        "metric": "3-party iris comparisons/sec on 1 MI355X + p50/p99 request latency",
+        "metric": "per-party share GEMM comparisons/sec on 1 MI355X, the reference's matmul bench shape (code only)",
         "value": Qr * N / t,
         "unit": "comparisons/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
@@ -665,9 +786,10 @@ def main():
                     help="GEMM kernel variant (-1: automatic, the rotation-aware kernel)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("c2", "c3", "c3-two-stage", "c4", "c5", "circuits", "matmul"),
-                    default="c2",
-                    help="c2: per-party share GEMM (the headline line); c3: full 3-party protocol, "
-                         "both eyes; c4: batching form (use --db 4194304 for the full-size DB); "
+                    default="c3",
+                    help="c3 (the headline line): full 3-party protocol, both eyes, plus a serving latency "
+                         "pass; c2: one party's share GEMM; c4: batching form (use --db 4194304 for the "
+                         "full-size DB); "
                          "c5: serving form (use --db 2097152 for the full-size DB); circuits: the "
                          "threshold circuits alone at one chunk; c3-two-stage: the reference's batch flow "
                          "(one-eye prefilter scan, both eyes on the candidates); matmul: the reference's "
@@ -675,6 +797,9 @@ def main():
     ap.add_argument("--rates", default="2,5,10,20,40", help="c5: request rates per second")
     ap.add_argument("--seconds", type=float, default=8.0, help="c5: arrival window per rate")
     ap.add_argument("--plain-requests", type=int, default=8, help="c5: requests timed in the plain form")
+    ap.add_argument("--latency-rate", type=float, default=40.0,
+                    help="c3: Poisson request rate of the serving latency pass (0: no pass)")
+    ap.add_argument("--latency-seconds", type=float, default=6.0, help="c3: arrival window of that pass")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -778,7 +903,7 @@ def main():
         roofline = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": gbs / HBM_PEAK_GBS, "traffic": traffic}
     line = {
-        "metric": "3-party iris comparisons/sec on 1 MI355X + p50/p99 request latency",
+        "metric": "per-party share GEMM comparisons/sec on 1 MI355X (one party's code + mask dot shares)",
         "value": pairs / (ms_step / 1e3),
         "unit": "comparisons/s",
         "n_gpus": 1,

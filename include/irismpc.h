@@ -238,6 +238,13 @@ int irismpc_share_db_set_counter(irismpc_share_db* e, uint64_t counter);
  * irismpc_gemm_rot_variants() = rotation-aware tiles (raw-share queries only).
  * No reference counterpart: the reference has one cuBLAS-backed GEMM path. */
 int irismpc_share_db_set_variant(irismpc_share_db* e, int variant);
+/* Launch timing, off by default. No reference counterpart: with on != 0,
+ * every share GEMM launch of the engine is bracketed by HIP events recorded on
+ * its stream (nothing synchronises at launch); launch_times waits for the
+ * recorded events and returns the number of timed launches and their summed
+ * durations (ms) since its previous call. */
+int irismpc_share_db_set_timing(irismpc_share_db* e, int on);
+int irismpc_share_db_launch_times(irismpc_share_db* e, size_t* launches, double* total_ms);
 int irismpc_gemm_variants(int* n);
 int irismpc_gemm_rot_variants(int* first, int* n);
 
@@ -274,6 +281,11 @@ int irismpc_circuits_open_views(irismpc_circuits* c, const uint64_t** a, const u
                                 const uint64_t** cc);
 int irismpc_circuits_get_counter(const irismpc_circuits* c, uint64_t* counter);
 int irismpc_circuits_set_counter(irismpc_circuits* c, uint64_t counter);
+/* Launch timing of the circuits, as irismpc_share_db_set_timing: one timed
+ * interval per threshold comparison (lift, bit injection and MSB adder of the
+ * three parties on the protocol stream), kept by party 0's engine. */
+int irismpc_circuits_set_timing(irismpc_circuits* c, int on);
+int irismpc_circuits_launch_times(irismpc_circuits* c, size_t* launches, double* total_ms);
 /* Debug capture of the corrected lifted mask (u32 a, b), NULL to disable. */
 int irismpc_circuits_set_lift_capture(irismpc_circuits* c, uint32_t* a, uint32_t* b);
 
@@ -372,7 +384,12 @@ int irismpc_scan_db(irismpc_share_db* codes, irismpc_share_db* masks, irismpc_ci
                     irismpc_comparator* cmp, const irismpc_db* code_db, const irismpc_db* mask_db,
                     const irismpc_query* q_code, const irismpc_query* q_mask, size_t db_size,
                     size_t batch_size, uint32_t threshold_a, uint64_t* bitmap, void* stream);
-/* The three parties' scans from one call (one host thread per party inside). */
+/* The three parties' scans from one call. Without a distance capture the loop
+ * runs on the calling thread: each chunk's six share GEMMs (three parties x
+ * code, mask) go in order on streams[0] and the circuits and opens on the
+ * exchange's protocol stream, overlapping the next chunk's GEMMs; with a
+ * capture, one host thread per party runs irismpc_scan_db. Same shares,
+ * counters and bitmaps either way. */
 int irismpc_scan_db_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],
                          irismpc_circuits* const circuits[3], irismpc_comparator* const cmp[3],
                          const irismpc_db* const code_db[3], const irismpc_db* const mask_db[3],

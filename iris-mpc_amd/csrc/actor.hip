@@ -314,8 +314,99 @@ void scan_batch_self_trio(const PartyScan p[3], size_t batch_size, uint32_t thre
   run_trio([&](int i) { scan_batch_self(p[i], batch_size, threshold_a, bitmaps[i], row_bits, s[i]); });
 }
 
+// The three parties' chunk loop from one host thread. Per chunk the six share
+// GEMMs (three parties x code, mask) run in order on s[0] into result slot
+// c % 2, so no two GEMMs share the GPU and each launch's duration is its own;
+// the reshares hand over views, and the circuits and opens of chunk c run on
+// the exchange's protocol stream while chunk c + 1's GEMMs run (a slot is
+// reused once the open of the chunk that last read it has run). Shares,
+// counters and bitmaps are those of scan_db called by the three parties.
+static void scan_db_trio_ordered(const PartyScan p[3], size_t db_size, size_t batch_size,
+                                 uint32_t threshold_a, uint64_t* const bitmaps[3],
+                                 hipStream_t const s[3]) {
+  const size_t rows = (size_t)p[0].code_query->rows;
+  Exchange* ex = p[0].circuits->exchange();
+  for (int i = 0; i < 3; i++) {
+    IRIS_REQUIRE(p[i].codes->peer_id() == i && p[i].masks->peer_id() == i && p[i].circuits->peer_id() == i,
+                 "trio members must be parties 0, 1, 2 in order");
+    IRIS_REQUIRE((size_t)p[i].code_query->rows == rows && (size_t)p[i].mask_query->rows == rows,
+                 "the parties' query sets disagree on rows");
+    IRIS_REQUIRE(p[i].circuits->exchange() == ex, "trio members must share one exchange");
+  }
+  IRIS_REQUIRE(batch_size * kRotations <= rows, "batch larger than the query rows");
+  hipStream_t g = s[0], ps = ex->protocol_stream();
+  const hipStream_t gs[3] = {g, g, g};
+  {  // the GEMM stream starts after the callers' earlier work on s[1], s[2]
+    Event e1, e2;
+    IRIS_HIP_CHECK(hipEventRecord(e1.e, s[1]));
+    IRIS_HIP_CHECK(hipEventRecord(e2.e, s[2]));
+    IRIS_HIP_CHECK(hipStreamWaitEvent(g, e1.e, 0));
+    IRIS_HIP_CHECK(hipStreamWaitEvent(g, e2.e, 0));
+  }
+  ShareDB *cd[3], *md[3];
+  Circuits* cc[3];
+  for (int i = 0; i < 3; i++) {
+    cd[i] = p[i].codes;
+    md[i] = p[i].masks;
+    cc[i] = p[i].circuits;
+    cd[i]->peer_copy = md[i]->peer_copy = false;
+  }
+  Event freed[2];
+  bool used[2] = {false, false};
+  const size_t pad = rows % 8 == 0 ? 64 : 512;
+  size_t offset = 0;
+  int c = 0;
+  do {
+    const int slot = c & 1;
+    const size_t real = db_size > offset ? std::min<size_t>(kDbChunk, db_size - offset) : 0;
+    const size_t dot_chunk = round_up(std::max<size_t>(real, 1), pad);
+    if (used[slot]) IRIS_HIP_CHECK(hipStreamWaitEvent(g, freed[slot].e, 0));
+    for (int i = 0; i < 3; i++) {
+      cd[i]->set_slot(slot);
+      md[i]->set_slot(slot);
+      cd[i]->dot_reduce(*p[i].code_query, *p[i].code_db, dot_chunk, offset, 1, g);
+      md[i]->dot_reduce(*p[i].mask_query, *p[i].mask_db, dot_chunk, offset, 2, g);
+    }
+    ShareDB::reshare_trio(cd, dot_chunk, gs);
+    ShareDB::reshare_trio(md, dot_chunk, gs);
+    const size_t len = dot_chunk * rows;
+    const uint16_t *ca[3], *cb[3], *ma[3], *mb[3];
+    for (int i = 0; i < 3; i++) {
+      cc[i]->set_chunk_size(len / 64);
+      ca[i] = cd[i]->results();
+      cb[i] = cd[i]->results_peer();
+      ma[i] = md[i]->results();
+      mb[i] = md[i]->results_peer();
+    }
+    Circuits::compare_trio(cc, ca, cb, ma, mb, len, threshold_a, gs, /*fork=*/false);
+    Circuits::open_exchange_trio(cc, gs);
+    if (real > 0)
+      for (int i = 0; i < 3; i++)
+        p[i].comparator->open_results(cc[i]->result_a(), cc[i]->result_b(), cc[i]->result_c(),
+                                      bitmaps[i], dot_chunk, real, offset, db_size, batch_size, ps,
+                                      rows);
+    IRIS_HIP_CHECK(hipEventRecord(freed[slot].e, ps));
+    used[slot] = true;
+    offset += kDbChunk;
+    c++;
+  } while (offset < db_size);
+  Event done;
+  IRIS_HIP_CHECK(hipEventRecord(done.e, ps));
+  for (int i = 0; i < 3; i++) {
+    IRIS_HIP_CHECK(hipStreamWaitEvent(s[i], done.e, 0));
+    cd[i]->set_slot(0);
+    md[i]->set_slot(0);
+  }
+}
+
 void scan_db_trio(const PartyScan p[3], size_t db_size, size_t batch_size, uint32_t threshold_a,
                   uint64_t* const bitmaps[3], hipStream_t const s[3]) {
+  bool capture = false;
+  for (int i = 0; i < 3; i++) capture |= p[i].comparator->anon_capture_enabled();
+  if (!capture) {
+    scan_db_trio_ordered(p, db_size, batch_size, threshold_a, bitmaps, s);
+    return;
+  }
   run_trio([&](int i) { scan_db(p[i], db_size, batch_size, threshold_a, bitmaps[i], s[i]); });
 }
 

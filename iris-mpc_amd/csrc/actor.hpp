@@ -77,7 +77,9 @@ void scan_batch_self(const PartyScan& p, size_t batch_size, uint32_t threshold_a
 void scan_batch_self_trio(const PartyScan p[3], size_t batch_size, uint32_t threshold_a,
                           uint64_t* const bitmaps[3], size_t row_bits, hipStream_t const s[3]);
 
-// Runs the three parties' scans from one call (one host thread per party).
+// Runs the three parties' scans from one call: from this thread, with the six
+// share GEMMs of a chunk in order on s[0] and the circuits on the exchange's
+// protocol stream (one host thread per party when a distance capture is set).
 void scan_db_trio(const PartyScan p[3], size_t db_size, size_t batch_size, uint32_t threshold_a,
                   uint64_t* const bitmaps[3], hipStream_t const s[3]);
 

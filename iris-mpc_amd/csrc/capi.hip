@@ -397,6 +397,16 @@ int irismpc_share_db_set_variant(irismpc_share_db* e, int variant) {
     e->impl->variant = variant;
   });
 }
+int irismpc_share_db_set_timing(irismpc_share_db* e, int on) {
+  return guarded([&] { e->impl->timer.enable(on != 0); });
+}
+int irismpc_share_db_launch_times(irismpc_share_db* e, size_t* launches, double* total_ms) {
+  return guarded([&] {
+    DevSet d(e->device);
+    const size_t n = e->impl->timer.collect(total_ms);
+    if (launches) *launches = n;
+  });
+}
 int irismpc_gemm_variants(int* n) {
   return guarded([&] { *n = kGemmVariants; });
 }
@@ -480,6 +490,16 @@ int irismpc_circuits_get_counter(const irismpc_circuits* c, uint64_t* counter) {
 }
 int irismpc_circuits_set_counter(irismpc_circuits* c, uint64_t counter) {
   return guarded([&] { c->impl->set_counter(counter); });
+}
+int irismpc_circuits_set_timing(irismpc_circuits* c, int on) {
+  return guarded([&] { c->impl->timer.enable(on != 0); });
+}
+int irismpc_circuits_launch_times(irismpc_circuits* c, size_t* launches, double* total_ms) {
+  return guarded([&] {
+    DevSet d(c->device);
+    const size_t n = c->impl->timer.collect(total_ms);
+    if (launches) *launches = n;
+  });
 }
 int irismpc_circuits_set_lift_capture(irismpc_circuits* c, uint32_t* a, uint32_t* b) {
   return guarded([&] {

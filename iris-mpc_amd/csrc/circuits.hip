@@ -46,17 +46,6 @@ CircuitSchedule CircuitSchedule::make(uint64_t c, size_t W, size_t alloc) {
   return s;
 }
 
-// Correlated randomness F(own) ^ F(prev) as 8 u64 words of block `blk`.
-__device__ __forceinline__ void corr_u64x8(const CircuitBufs& P, uint64_t nonce, uint64_t blk,
-                                           uint64_t r[8]) {
-  uint32_t k1[16], k2[16];
-  chacha_block<12>(P.key_own, nonce, blk, k1);
-  chacha_block<12>(P.key_prev, nonce, blk, k2);
-#pragma unroll
-  for (int i = 0; i < 8; i++)
-    r[i] = (uint64_t)(k1[2 * i] ^ k2[2 * i]) | ((uint64_t)(k1[2 * i + 1] ^ k2[2 * i + 1]) << 32);
-}
-
 // ------------------------------------------------------------ carry-save AND
 // and_pre(x1^x3, x2^x3) for the split of party p (kernel.cu:39-42,205-235):
 //   p0: x1=(a,0) x2=0 x3=(0,b)  -> a & b
@@ -82,11 +71,13 @@ __device__ __forceinline__ uint32_t and_pre(uint32_t xa, uint32_t xb, uint32_t y
 // kernel.cu:526-629 with the message flow P1 -> P2 -> P0 and P0 -> P1.
 
 // Block per 2048 elements, one wave per party: lane l draws block 64 g + l of
-// its party's two streams once (block g covers elements 32 g .. 32 g + 31 as
-// the u16 view of its keystream). The keystreams go through LDS in element
-// order, then each lane handles 8 consecutive elements per pass. The three
-// steps (P0 -> P1 and P1 -> P2, then P2 -> P0) pass their u16 messages to the
-// receiving party's wave through LDS, one barrier apart.
+// its party's own stream once (block g covers elements 32 g .. 32 g + 31 as
+// the u16 view of its keystream); the party's other stream (their_rng) is the
+// previous party's own stream at the same counter, read from LDS. The
+// keystreams go through LDS in element order, then each lane handles 8
+// consecutive elements per pass. The three steps (P0 -> P1 and P1 -> P2, then
+// P2 -> P0) pass their u16 messages to the receiving party's wave through LDS,
+// one barrier apart.
 __device__ __forceinline__ void u16x8_load(const uint16_t* p, uint16_t v[8]) {
   const uint4 q = *reinterpret_cast<const uint4*>(p);
   const uint32_t w[4] = {q.x, q.y, q.z, q.w};
@@ -104,25 +95,22 @@ __device__ __forceinline__ void u16x8_store(uint16_t* p, const uint16_t v[8]) {
 __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr) {
   // one party per wave: p is wave-uniform, so its keys and pointers stay scalar
   const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int lane = (int)(threadIdx.x & 63);
+  const int lane = (int)(threadIdx.x & 63), prev = (p + 2) % 3;
   const CircuitBufs& P = t.p[p];
-  __shared__ uint32_t ks[3][2][1024];  // [party][own, prev][u16 pairs in element order]
-  __shared__ uint32_t msg[3][1024];    // [receiving party][u16 pairs in element order]
+  __shared__ uint32_t ks[3][1024];   // [party][own stream: u16 pairs in element order]
+  __shared__ uint32_t msg[3][1024];  // [receiving party][u16 pairs in element order]
   const size_t n = 128 * t.W;
   const size_t e_base = (size_t)blockIdx.x * 2048;
   {
-    uint32_t ko[16], kp[16];
-    chacha_block<12>(P.key_own, t.nonce, ctr + (uint64_t)blockIdx.x * 64 + lane, ko);   // my_rng (chacha1)
-    chacha_block<12>(P.key_prev, t.nonce, ctr + (uint64_t)blockIdx.x * 64 + lane, kp);  // their_rng (chacha2)
+    uint32_t ko[16];
+    chacha_block<12>(P.key_own, t.nonce, ctr + (uint64_t)blockIdx.x * 64 + lane, ko);  // my_rng (chacha1)
 #pragma unroll
-    for (int i = 0; i < 16; i += 4) {
-      *reinterpret_cast<uint4*>(&ks[p][0][lane * 16 + i]) = uint4{ko[i], ko[i + 1], ko[i + 2], ko[i + 3]};
-      *reinterpret_cast<uint4*>(&ks[p][1][lane * 16 + i]) = uint4{kp[i], kp[i + 1], kp[i + 2], kp[i + 3]};
-    }
+    for (int i = 0; i < 16; i += 4)
+      *reinterpret_cast<uint4*>(&ks[p][lane * 16 + i]) = uint4{ko[i], ko[i + 1], ko[i + 2], ko[i + 3]};
   }
   __syncthreads();
-  const uint16_t* own16 = reinterpret_cast<const uint16_t*>(ks[p][0]);
-  const uint16_t* prv16 = reinterpret_cast<const uint16_t*>(ks[p][1]);
+  const uint16_t* own16 = reinterpret_cast<const uint16_t*>(ks[p]);
+  const uint16_t* prv16 = reinterpret_cast<const uint16_t*>(ks[prev]);  // their_rng (chacha2)
   uint16_t* to_next = reinterpret_cast<uint16_t*>(msg[(p + 1) % 3]);
   const uint16_t* inbox = reinterpret_cast<const uint16_t*>(msg[p]);
   // step 1: P0 sends y to P1, P1 sends x to P2; both write corr_a
@@ -231,8 +219,10 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
 // The ChaCha12 block pair behind one AND of 8 words (F(own) ^ F(prev),
 // counter as in the schedule) is computed once per 16-lane group: for a batch
 // of 8 steps (16 gates) lane i of a group computes gate i and LDS hands every
-// lane its half word of all 16. The lift's unused slot (step 0 has no ripple
-// round) carries the carry-out AND.
+// lane its half word of all 16. A wave computes F(own) only: its F(prev) is
+// the previous party's F(own) (pairwise seeds), which that party's wave has
+// put in LDS, so the trio evaluates 3 distinct blocks per gate, not 6. The
+// lift's unused slot (step 0 has no ripple round) carries the carry-out AND.
 
 // Transpose a 32 x 32 bit matrix in place: afterwards bit i of x[j] is bit j
 // of the old x[i] (recursive block swap, 5 stages).
@@ -427,18 +417,15 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
         valid = (s >= 1 && s <= LAST_C + 1) || (!MSB && s == 0);
         ctr = c_round + g;
       }
-      uint64_t r[8] = {};
-      if (valid) corr_u64x8(P, t.nonce, ctr, r);
+      uint32_t k[16] = {};
+      if (valid) chacha_block<12>(P.key_own, t.nonce, ctr, k);
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
-        L.rnd[p][gl][2 * i][grp] = (uint32_t)r[i];
-        L.rnd[p][gl][2 * i + 1][grp] = (uint32_t)(r[i] >> 32);
-      }
+      for (int i = 0; i < 16; i++) L.rnd[p][gl][i][grp] = k[i];
     }
     __syncthreads();
-    uint32_t r16[16];
+    uint32_t r16[16];  // F(own) ^ F(prev): bits of the 16 gates for this lane's half word
 #pragma unroll
-    for (int c = 0; c < 16; c++) r16[c] = L.rnd[p][c][gl][grp];
+    for (int c = 0; c < 16; c++) r16[c] = L.rnd[p][c][gl][grp] ^ L.rnd[prev][c][gl][grp];
     if (!MSB && b == 0) r_out = r16[1];
     switch (b) {
       case 0: adder_batch<MSB, 0>(sl, st, r16, r_out, L, p, prev, lane); break;

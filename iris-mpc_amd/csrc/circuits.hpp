@@ -26,6 +26,9 @@ struct Trio {
   size_t W;
   uint64_t nonce;
 };
+// The trio's seeds are pairwise: party p's key_prev is party p - 1's key_own.
+// Each pairwise ChaCha12 block is computed once, by the wave of the party that
+// owns its key, and read by the next party's wave from LDS.
 
 // Block counters of every RNG draw of compare_threshold_masked_many, following
 // the reference schedule (protocol.rs:2519-2546): all six ChaCha streams of the

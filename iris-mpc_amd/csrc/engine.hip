@@ -27,6 +27,43 @@ uint64_t op_id(const char* name, int tag) {
 }
 }  // namespace
 
+// ====================================================================== LaunchTimer
+LaunchTimer::~LaunchTimer() {
+  for (auto& p : ev_) {
+    if (p.a) (void)hipEventDestroy(p.a);
+    if (p.b) (void)hipEventDestroy(p.b);
+  }
+}
+
+void LaunchTimer::begin(hipStream_t s) {
+  if (used_ == ev_.size()) {
+    Pair p;
+    IRIS_HIP_CHECK(hipEventCreate(&p.a));
+    IRIS_HIP_CHECK(hipEventCreate(&p.b));
+    ev_.push_back(p);
+  }
+  IRIS_HIP_CHECK(hipEventRecord(ev_[used_].a, s));
+}
+
+void LaunchTimer::end(hipStream_t s) {
+  IRIS_HIP_CHECK(hipEventRecord(ev_[used_].b, s));
+  used_++;
+}
+
+size_t LaunchTimer::collect(double* total_ms) {
+  double t = 0;
+  for (size_t i = 0; i < used_; i++) {
+    IRIS_HIP_CHECK(hipEventSynchronize(ev_[i].b));
+    float ms = 0;
+    IRIS_HIP_CHECK(hipEventElapsedTime(&ms, ev_[i].a, ev_[i].b));
+    t += ms;
+  }
+  const size_t n = used_;
+  used_ = 0;
+  if (total_ms) *total_ms = t;
+  return n;
+}
+
 // ====================================================================== Exchange
 Exchange::Exchange(int device, double timeout_s) : device_(device), timeout_s_(timeout_s) {
   DeviceGuard g(device_);
@@ -378,9 +415,15 @@ void ShareDB::refill_query_shares(QuerySet& q, const uint16_t* host_shares, size
   IRIS_HIP_CHECK(hipStreamSynchronize(s));  // the staging buffers are reused by the next refill
 }
 
+void ShareDB::launch(const GemmArgs& a, int variant, hipStream_t s) {
+  if (timer.on()) timer.begin(s);
+  launch_share_gemm(a, variant, s);
+  if (timer.on()) timer.end(s);
+}
+
 int ShareDB::resolve_variant(const QuerySet& q) const {
   if (variant == kAutoVariant) {
-    if (!q.rot.ptr) return 7;
+    if (!q.rot.ptr) return 0;
     return q.rows <= kRotSmallRows ? kRotSmallVariant : kRotVariantBase;
   }
   IRIS_REQUIRE(!is_rot_variant(variant) || q.rot.ptr,
@@ -430,7 +473,7 @@ void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, siz
   a.key_prev = seed_prev_;
   a.mult = multiplier;
   a.masked = is_remote() ? 1 : 0;
-  launch_share_gemm(a, variant, s);
+  launch(a, variant, s);
   // the reference fills the mask keystream (chunk*Qr*2 bytes) only when remote
   last_rows_ = (size_t)q.rows;
   if (is_remote()) counter_ += ceil_div(chunk * last_rows_ * 2, 64);
@@ -462,7 +505,7 @@ void ShareDB::dot_reduce_chunks(const QuerySet& q, const DbSlice& db, size_t off
   a.key_prev = seed_prev_;
   a.mult = multiplier;
   a.masked = is_remote() ? 1 : 0;
-  launch_share_gemm(a, variant, s);
+  launch(a, variant, s);
   last_rows_ = qr;
   if (is_remote())
     for (size_t c0 = 0; c0 < total; c0 += chunk) counter_ += ceil_div(std::min(chunk, total - c0) * qr * 2, 64);
@@ -492,7 +535,7 @@ void ShareDB::dot_reduce_range(const QuerySet& q, const DbSlice& db, size_t offs
   a.key_prev = seed_prev_;
   a.mult = multiplier;
   a.masked = is_remote() ? 1 : 0;
-  launch_share_gemm(a, variant, s);
+  launch(a, variant, s);
   if (is_remote()) counter_ += range_counter_advance(total, chunk, qr);
 }
 
@@ -540,7 +583,7 @@ void ShareDB::dot_reduce_packed(const QuerySet& q, const DbSlice& db, size_t off
   a.key_prev = seed_prev_;
   a.mult = multiplier;
   a.masked = is_remote() ? 1 : 0;
-  launch_share_gemm(a, variant, s);
+  launch(a, variant, s);
   if (!ctr_first) counter_ = ctr;
 }
 
@@ -667,6 +710,15 @@ void Circuits::compare_trio(Circuits* const c[3], const uint16_t* const code_a[3
   }
   t.W = W;
   t.nonce = c[0]->nonce_;
+  // The messages between the parties' waves skip the reference's OTP
+  // (chacha1 encrypt by the sender, chacha2 decrypt by the receiver,
+  // protocol.rs:784-892), which is the identity only when party p's prev seed
+  // is party p - 1's own seed; the kernels also rely on it to compute each
+  // pairwise ChaCha12 block once.
+  for (int p = 0; p < 3; p++)
+    IRIS_REQUIRE(std::memcmp(&c[p]->seed_prev_, &c[(p + 2) % 3]->seed_own_, sizeof(ChaChaKey)) == 0,
+                 "the three parties' circuit seeds are not pairwise (party p's prev seed must be "
+                 "party p - 1's own seed)");
   const CircuitSchedule sc = CircuitSchedule::make(c[0]->counter_, W, c[0]->alloc_words_);
   hipStream_t ps = ex->protocol_stream();
   ex->join_streams(s);
@@ -675,6 +727,8 @@ void Circuits::compare_trio(Circuits* const c[3], const uint16_t* const code_a[3
   lift.c[0] = sc.lift_first;
   for (int k = 1; k <= 14; k++) lift.c[k] = sc.lift_round[k];
   lift.c[15] = sc.lift_final;
+  LaunchTimer& timer = c[0]->timer;
+  if (timer.on()) timer.begin(ps);
   launch_adder_word(t, false, A, lift, ps);
   launch_inject(t, sc.inject, ps);
   RippleCtrs msb{};
@@ -682,6 +736,7 @@ void Circuits::compare_trio(Circuits* const c[3], const uint16_t* const code_a[3
   msb.c[0] = sc.msb_first;
   for (int k = 1; k <= 29; k++) msb.c[k] = sc.msb_round[k];
   launch_adder_word(t, true, A, msb, ps);
+  if (timer.on()) timer.end(ps);
   if (fork) ex->fork_streams(s);
   for (int p = 0; p < 3; p++) c[p]->counter_ = sc.end;
 }

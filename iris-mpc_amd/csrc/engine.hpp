@@ -58,6 +58,31 @@ struct Event {
   Event& operator=(const Event&) = delete;
 };
 
+// Opt-in timing of an engine's kernel launches: begin/end record HIP events
+// on the launch stream around one launch (nothing waits at record time);
+// collect() waits for the recorded events and returns the number of timed
+// launches and their summed durations since the previous collect.
+class LaunchTimer {
+ public:
+  LaunchTimer() = default;
+  ~LaunchTimer();
+  LaunchTimer(const LaunchTimer&) = delete;
+  LaunchTimer& operator=(const LaunchTimer&) = delete;
+  void enable(bool on) { on_ = on; }
+  bool on() const { return on_; }
+  void begin(hipStream_t s);
+  void end(hipStream_t s);
+  size_t collect(double* total_ms);
+
+ private:
+  struct Pair {
+    hipEvent_t a = nullptr, b = nullptr;
+  };
+  std::vector<Pair> ev_;
+  size_t used_ = 0;
+  bool on_ = false;
+};
+
 // The link between the three parties on one GPU (replaces NcclComm,
 // helpers/comm.rs). Party p sends to next = p+1 and receives from prev = p-1.
 // Operations that the reference performs as matched send/recv pairs are
@@ -229,8 +254,11 @@ class ShareDB {
   static constexpr int kAutoVariant = -1;
   int variant = kAutoVariant;
   int resolve_variant(const QuerySet& q) const;
+  // share GEMM launches (every dot_reduce form), when enabled
+  LaunchTimer timer;
 
  private:
+  void launch(const GemmArgs& a, int variant, hipStream_t s);
   std::unique_ptr<QuerySet> upload_shares_impl(const uint16_t* host_shares,
                                                const uint16_t* insert_shares, size_t batch,
                                                size_t rows, int mirror, hipStream_t s) const;
@@ -306,6 +334,9 @@ class Circuits {
   int peer_id() const { return peer_id_; }
   uint32_t* lifted_dbg_a = nullptr;  // optional debug capture of the lifted mask
   uint32_t* lifted_dbg_b = nullptr;
+  // the circuit launches of a compare (lift, injection, MSB adder: one timed
+  // interval on the protocol stream, kept by party 0's engine), when enabled
+  LaunchTimer timer;
 
  private:
   CircuitBufs bufs(const uint16_t* ca, const uint16_t* cb, const uint16_t* ma,

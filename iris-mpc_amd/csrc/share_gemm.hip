@@ -311,9 +311,8 @@ void launch_untile_rows(const int8_t* src, size_t row0, size_t n_rows, int K, ui
 }
 
 // ---------------------------------------------------------------- GEMM
-// Block = 4 waves (2 along queries x 2 along DB rows). Wave tile = WQ x WD
-// MFMA 16x16 tiles; two i32 accumulators per tile: acc0 = A0.B0 and
-// acc1 = A0.B1 + A1.B0 (only acc1 mod 2^8 matters after the << 8).
+// Two i32 accumulators per MFMA tile: acc0 = A0.B0 and acc1 = A0.B1 + A1.B0
+// (only acc1 mod 2^8 matters after the << 8).
 // v_mfma_i32_16x16x64_i8 operand layout: lane l supplies row (l % 16) and a
 // 16-byte k-slice selected by l / 16 for both operands, so any consistent
 // k-assignment contracts correctly; the result tile has row 4*(l/16)+r and
@@ -453,58 +452,6 @@ __device__ __forceinline__ void epilogue_store(const GemmArgs& a, const uint16_t
   }
 }
 
-// Direct form: each wave loads its fragments straight from global memory into
-// registers (double-buffered), no LDS staging.
-template <int WQ, int WD>
-__global__ __launch_bounds__(256, 1) void share_gemm_kernel(const GemmArgs a) {
-  constexpr int BQ = 2 * WQ * 16, BD = 2 * WD * 16;
-  __shared__ uint16_t tile[BQ * BD];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wq = wave >> 1, wd = wave & 1;
-  int q0;
-  size_t db0;
-  if (!block_tile<BQ, BD>(a, q0, db0)) return;
-
-  const size_t gstride = (size_t)a.n_kt * 2 * kFragBytes;  // bytes per 16-row group
-  const int8_t* qp = a.q + (size_t)(q0 / 16 + wq * WQ) * gstride + lane * 16;
-  const int8_t* dp = a.db + (db0 / 16 + (size_t)wd * WD) * gstride + lane * 16;
-
-  v4i acc0[WQ][WD], acc1[WQ][WD];
-#pragma unroll
-  for (int i = 0; i < WQ; i++)
-#pragma unroll
-    for (int j = 0; j < WD; j++) {
-      acc0[i][j] = v4i{0, 0, 0, 0};
-      acc1[i][j] = v4i{0, 0, 0, 0};
-    }
-
-  v4i A0[WQ][2], B0[WD][2], A1[WQ][2], B1[WD][2];
-  auto load = [&](v4i (&A)[WQ][2], v4i (&B)[WD][2], int kt) {
-    const size_t ko = (size_t)kt * 2 * kFragBytes;
-#pragma unroll
-    for (int i = 0; i < WQ; i++) {
-      A[i][0] = *reinterpret_cast<const v4i*>(qp + i * gstride + ko);
-      A[i][1] = *reinterpret_cast<const v4i*>(qp + i * gstride + ko + kFragBytes);
-    }
-#pragma unroll
-    for (int j = 0; j < WD; j++) {
-      B[j][0] = *reinterpret_cast<const v4i*>(dp + j * gstride + ko);
-      B[j][1] = *reinterpret_cast<const v4i*>(dp + j * gstride + ko + kFragBytes);
-    }
-  };
-
-  load(A0, B0, 0);
-  int kt = 0;
-  for (; kt + 2 <= a.n_kt; kt += 2) {
-    load(A1, B1, kt + 1);
-    mma_step<WQ, WD>(acc0, acc1, A0, B0);
-    if (kt + 2 < a.n_kt) load(A0, B0, kt + 2);
-    mma_step<WQ, WD>(acc0, acc1, A1, B1);
-  }
-  if (kt < a.n_kt) mma_step<WQ, WD>(acc0, acc1, A0, B0);
-  gemm_epilogue<WQ, WD>(a, tile, acc0, acc1, q0, db0, wq, wd, lane);
-}
-
 // LDS-staged form: per K-step the block's query and DB fragments (each 1 KiB,
 // one coalesced dwordx4 per lane) are fetched from global memory once into a
 // double-buffered LDS stage; waves read their operands with ds_read_b128.
@@ -589,127 +536,6 @@ __global__ __launch_bounds__(64 * NWQ * NWD, 1) void share_gemm_lds_kernel(const
 }
 
 
-// 32x32x32 form (v_mfma_i32_32x32x32_i8 sustains about twice the int8 rate of
-// 16x16x64 on gfx950). The HBM/LDS fragments keep their 16-row x 64-k shape;
-// for the 32-row operand, lane l (row r = l % 32, k-half h = l / 32) of k-slice
-// s reads lane (r % 16) + 16 (2 s + h) of the fragment of row group r / 16, so
-// both operands see the same k assignment. Result tile: lane l, register v
-// hold row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.
-template <int TQ, int TD, int NWQ, int NWD>
-__global__ __launch_bounds__(64 * NWQ * NWD, 1) void share_gemm_lds32_kernel(const GemmArgs a) {
-  constexpr int NW = NWQ * NWD;
-  constexpr int BQ = NWQ * TQ * 32, BD = NWD * TD * 32;
-  using S = LdsStages<TQ * 2, TD * 2, NWQ, NWD>;
-  constexpr int NA = S::NA, NF = S::NF;
-  static_assert(NF % NW == 0, "fragments split evenly over the waves");
-  constexpr int PER_WAVE = NF / NW;
-  __shared__ S sm;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wq = wave / NWD, wd = wave % NWD;
-  int q0;
-  size_t db0;
-  if (!block_tile<BQ, BD>(a, q0, db0)) return;
-
-  const size_t gstride = (size_t)a.n_kt * 2 * kFragBytes;
-  const int8_t* qbase = a.q + (size_t)(q0 / 16) * gstride + lane * 16;
-  const int8_t* dbase = a.db + (db0 / 16) * gstride + lane * 16;
-  auto src = [&](int f, int kt) -> const v4i* {
-    const size_t ko = (size_t)kt * 2 * kFragBytes;
-    if (f < NA) return reinterpret_cast<const v4i*>(qbase + (f >> 1) * gstride + ko + (f & 1) * kFragBytes);
-    const int g = f - NA;
-    return reinterpret_cast<const v4i*>(dbase + (g >> 1) * gstride + ko + (g & 1) * kFragBytes);
-  };
-  const int r32 = lane & 31, h = lane >> 5;
-  // fragment-local lane for k-slice s, plus the row-group offset (0 or 1)
-  const int lane_s0 = (r32 & 15) + 16 * h, lane_s1 = (r32 & 15) + 16 * (2 + h);
-  const int gsub = r32 >> 4;
-
-  v16i acc0[TQ][TD], acc1[TQ][TD];
-#pragma unroll
-  for (int i = 0; i < TQ; i++)
-#pragma unroll
-    for (int j = 0; j < TD; j++) {
-      acc0[i][j] = v16i{};
-      acc1[i][j] = v16i{};
-    }
-
-  v4i stg[PER_WAVE];
-#pragma unroll
-  for (int m = 0; m < PER_WAVE; m++) stg[m] = *src(wave + NW * m, 0);
-#pragma unroll
-  for (int m = 0; m < PER_WAVE; m++) sm.stage[0][(wave + NW * m) * 64 + lane] = stg[m];
-  __syncthreads();
-
-  for (int kt = 0; kt < a.n_kt; kt++) {
-    const int st = kt & 1;
-    const bool more = kt + 1 < a.n_kt;
-    if (more) {
-#pragma unroll
-      for (int m = 0; m < PER_WAVE; m++) stg[m] = *src(wave + NW * m, kt + 1);
-    }
-    v4i A[TQ][2][2], B[TD][2][2];  // [tile][k-slice][limb]
-#pragma unroll
-    for (int i = 0; i < TQ; i++) {
-      const int g = (wq * TQ + i) * 2 + gsub;
-#pragma unroll
-      for (int l = 0; l < 2; l++) {
-        A[i][0][l] = sm.stage[st][(g * 2 + l) * 64 + lane_s0];
-        A[i][1][l] = sm.stage[st][(g * 2 + l) * 64 + lane_s1];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < TD; j++) {
-      const int g = (wd * TD + j) * 2 + gsub;
-#pragma unroll
-      for (int l = 0; l < 2; l++) {
-        B[j][0][l] = sm.stage[st][(NA + g * 2 + l) * 64 + lane_s0];
-        B[j][1][l] = sm.stage[st][(NA + g * 2 + l) * 64 + lane_s1];
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-      for (int i = 0; i < TQ; i++)
-#pragma unroll
-        for (int j = 0; j < TD; j++)
-          acc0[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][0], B[j][s][0], acc0[i][j], 0, 0, 0);
-#pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-      for (int i = 0; i < TQ; i++)
-#pragma unroll
-        for (int j = 0; j < TD; j++)
-          acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][0], B[j][s][1], acc1[i][j], 0, 0, 0);
-#pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-      for (int i = 0; i < TQ; i++)
-#pragma unroll
-        for (int j = 0; j < TD; j++)
-          acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][1], B[j][s][0], acc1[i][j], 0, 0, 0);
-    if (more) {
-#pragma unroll
-      for (int m = 0; m < PER_WAVE; m++) sm.stage[st ^ 1][(wave + NW * m) * 64 + lane] = stg[m];
-    }
-    __syncthreads();
-  }
-
-#pragma unroll
-  for (int i = 0; i < TQ; i++)
-#pragma unroll
-    for (int v = 0; v < 16; v++) {
-      const int ql = (wq * TQ + i) * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-#pragma unroll
-      for (int j = 0; j < TD; j++) {
-        const int dl = (wd * TD + j) * 32 + r32;
-        sm.tile[ql * BD + dl] = corrected(a, q0 + ql, db0 + dl, (uint32_t)acc0[i][j][v],
-                                          (uint32_t)acc1[i][j][v]);
-      }
-    }
-  epilogue_store<BQ, BD>(a, sm.tile, q0, db0, a.qr);
-}
-
-
 // Direct-to-LDS form for gfx950: each fragment (1 KiB, lane l's 16 bytes at
 // l * 16) is one wave-wide global_load_lds_dwordx4, which lands in LDS in the
 // same lane order. With no register staging the ring can be STAGES deep, so a
@@ -736,135 +562,6 @@ __device__ __forceinline__ void stage_barrier() {
   asm volatile("" ::: "memory");
   __builtin_amdgcn_s_barrier();
   asm volatile("" ::: "memory");
-}
-
-template <int TQ, int TD, int NWQ, int NWD, int STAGES>
-struct DmaStages {
-  static constexpr int NA = NWQ * TQ * 4, NB = NWD * TD * 4, NF = NA + NB;
-  union {
-    v4i stage[STAGES][NF * 64];
-    uint16_t tile[(NWQ * TQ * 32) * (NWD * TD * 32)];
-  };
-};
-
-template <int TQ, int TD, int NWQ, int NWD, int STAGES>
-__global__ __launch_bounds__(64 * NWQ * NWD, 1) void share_gemm_dma_kernel(const GemmArgs a) {
-  constexpr int NW = NWQ * NWD;
-  constexpr int BQ = NWQ * TQ * 32, BD = NWD * TD * 32;
-  using S = DmaStages<TQ, TD, NWQ, NWD, STAGES>;
-  constexpr int NA = S::NA, NF = S::NF;
-  static_assert(NF % NW == 0, "fragments split evenly over the waves");
-  constexpr int PER_WAVE = NF / NW;
-  static_assert(PER_WAVE * (STAGES - 2) < 64, "outstanding loads fit vmcnt");
-  __shared__ S sm;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wq = wave / NWD, wd = wave % NWD;
-  int q0;
-  size_t db0;
-  if (!block_tile<BQ, BD>(a, q0, db0)) return;
-
-  const size_t gstride = (size_t)a.n_kt * 2 * kFragBytes;
-  const int8_t* qbase = a.q + (size_t)(q0 / 16) * gstride + lane * 16;
-  const int8_t* dbase = a.db + (db0 / 16) * gstride + lane * 16;
-  auto issue = [&](int kt, int st) {
-    const size_t ko = (size_t)kt * 2 * kFragBytes;
-#pragma unroll
-    for (int m = 0; m < PER_WAVE; m++) {
-      const int f = wave + NW * m;
-      const int8_t* src = f < NA ? qbase + (f >> 1) * gstride + ko + (f & 1) * kFragBytes
-                                 : dbase + ((f - NA) >> 1) * gstride + ko + ((f - NA) & 1) * kFragBytes;
-      dma_fragment(src, &sm.stage[st][f * 64]);
-    }
-  };
-  const int r32 = lane & 31, h = lane >> 5;
-  const int lane_s0 = (r32 & 15) + 16 * h, lane_s1 = (r32 & 15) + 16 * (2 + h);
-  const int gsub = r32 >> 4;
-
-  v16i acc0[TQ][TD], acc1[TQ][TD];
-#pragma unroll
-  for (int i = 0; i < TQ; i++)
-#pragma unroll
-    for (int j = 0; j < TD; j++) {
-      acc0[i][j] = v16i{};
-      acc1[i][j] = v16i{};
-    }
-
-  const int n_kt = a.n_kt;
-#pragma unroll
-  for (int p = 0; p < STAGES - 1; p++)
-    if (p < n_kt) issue(p, p);
-
-  int st = 0;
-  for (int kt = 0; kt < n_kt; kt++) {
-    // own loads of step kt done (later steps may stay in flight), then the
-    // barrier makes every wave's fragments of step kt visible and retires all
-    // reads of the stage that the next issue overwrites
-    if (kt + STAGES - 2 < n_kt) wait_vmcnt<PER_WAVE * (STAGES - 2)>();
-    else wait_vmcnt<0>();
-    stage_barrier();
-    if (kt + STAGES - 1 < n_kt) {
-      int sn = st + STAGES - 1;
-      if (sn >= STAGES) sn -= STAGES;
-      issue(kt + STAGES - 1, sn);
-    }
-    const v4i* stg = sm.stage[st];
-    v4i A[TQ][2][2], B[TD][2][2];  // [tile][k-slice][limb]
-#pragma unroll
-    for (int i = 0; i < TQ; i++) {
-      const int g = (wq * TQ + i) * 2 + gsub;
-#pragma unroll
-      for (int l = 0; l < 2; l++) {
-        A[i][0][l] = stg[(g * 2 + l) * 64 + lane_s0];
-        A[i][1][l] = stg[(g * 2 + l) * 64 + lane_s1];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < TD; j++) {
-      const int g = (wd * TD + j) * 2 + gsub;
-#pragma unroll
-      for (int l = 0; l < 2; l++) {
-        B[j][0][l] = stg[(NA + g * 2 + l) * 64 + lane_s0];
-        B[j][1][l] = stg[(NA + g * 2 + l) * 64 + lane_s1];
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-      for (int i = 0; i < TQ; i++)
-#pragma unroll
-        for (int j = 0; j < TD; j++)
-          acc0[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][0], B[j][s][0], acc0[i][j], 0, 0, 0);
-#pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-      for (int i = 0; i < TQ; i++)
-#pragma unroll
-        for (int j = 0; j < TD; j++)
-          acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][0], B[j][s][1], acc1[i][j], 0, 0, 0);
-#pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-      for (int i = 0; i < TQ; i++)
-#pragma unroll
-        for (int j = 0; j < TD; j++)
-          acc1[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[i][s][1], B[j][s][0], acc1[i][j], 0, 0, 0);
-    st = st + 1 == STAGES ? 0 : st + 1;
-  }
-  __syncthreads();
-
-#pragma unroll
-  for (int i = 0; i < TQ; i++)
-#pragma unroll
-    for (int v = 0; v < 16; v++) {
-      const int ql = (wq * TQ + i) * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-#pragma unroll
-      for (int j = 0; j < TD; j++) {
-        const int dl = (wd * TD + j) * 32 + r32;
-        sm.tile[ql * BD + dl] = corrected(a, q0 + ql, db0 + dl, (uint32_t)acc0[i][j][v],
-                                          (uint32_t)acc1[i][j][v]);
-      }
-    }
-  epilogue_store<BQ, BD>(a, sm.tile, q0, db0, a.qr);
 }
 
 
@@ -1296,24 +993,8 @@ void launch_rot_planes(const uint16_t* rows, int qr, int K, int8_t* out, hipStre
 
 GemmTile gemm_tile(int variant) {
   switch (variant) {
-    case 0: return {128, 128};
-    case 1: return {64, 256};
-    case 2: return {128, 256};
-    case 3: return {64, 128};
-    case 4: return {128, 256};   // LDS-staged
-    case 5: return {128, 128};   // LDS-staged
-    case 6: return {64, 256};    // LDS-staged
-    case 7: return {128, 256};   // LDS-staged, 8 waves (2 per SIMD)
-    case 8: return {256, 128};   // LDS-staged, 8 waves (2 per SIMD)
-    case 9: return {128, 256};   // 32x32x32 MFMA, LDS-staged, 8 waves
-    case 10: return {256, 128};  // 32x32x32 MFMA, LDS-staged, 8 waves
-    case 11: return {128, 256};  // 32x32x32 MFMA, LDS-staged, 4 waves
-    case 12: return {128, 256};  // 32x32x32 MFMA, 3-stage LDS DMA ring, 8 waves
-    case 13: return {256, 128};  // 32x32x32 MFMA, 3-stage LDS DMA ring, 8 waves
-    case 14: return {128, 256};  // 32x32x32 MFMA, 3-stage LDS DMA ring, 4 waves
-    case 15: return {128, 256};  // hybrid: query via LDS DMA, DB direct, 4 waves of 128x64
-    case 16: return {128, 128};  // hybrid, 2 waves of 128x64
-    case 17: return {64, 256};   // hybrid, 4 waves of 64x64
+    case 0: return {128, 256};   // 16x16x64 MFMA, LDS-staged, 8 waves (2 per SIMD)
+    case 1: return {128, 256};   // 32x32x32 MFMA, query via LDS DMA, DB direct, 4 waves of 128x64
     case kRotVariantBase: return {4 * kRotations, 256};      // rotation-aware, 4 waves
     case kRotVariantBase + 1: return {4 * kRotations, 128};  // rotation-aware, 2 waves
     case kRotVariantBase + 2: return {2 * kRotations, 128};  // two queries, 2 waves per SIMD
@@ -1334,24 +1015,8 @@ void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s) {
   const int T = a.n_qtiles * a.n_dtiles;
   const unsigned grid = (unsigned)(8 * ((T + 7) / 8));
   switch (variant) {
-    case 0: share_gemm_kernel<4, 4><<<grid, 256, 0, s>>>(a); break;
-    case 1: share_gemm_kernel<2, 8><<<grid, 256, 0, s>>>(a); break;
-    case 2: share_gemm_kernel<4, 8><<<grid, 256, 0, s>>>(a); break;
-    case 3: share_gemm_kernel<2, 4><<<grid, 256, 0, s>>>(a); break;
-    case 4: share_gemm_lds_kernel<4, 8, 2, 2><<<grid, 256, 0, s>>>(a); break;
-    case 5: share_gemm_lds_kernel<4, 4, 2, 2><<<grid, 256, 0, s>>>(a); break;
-    case 6: share_gemm_lds_kernel<2, 8, 2, 2><<<grid, 256, 0, s>>>(a); break;
-    case 7: share_gemm_lds_kernel<4, 4, 2, 4><<<grid, 512, 0, s>>>(a); break;
-    case 8: share_gemm_lds_kernel<4, 4, 4, 2><<<grid, 512, 0, s>>>(a); break;
-    case 9: share_gemm_lds32_kernel<2, 2, 2, 4><<<grid, 512, 0, s>>>(a); break;
-    case 10: share_gemm_lds32_kernel<2, 2, 4, 2><<<grid, 512, 0, s>>>(a); break;
-    case 11: share_gemm_lds32_kernel<2, 4, 2, 2><<<grid, 256, 0, s>>>(a); break;
-    case 12: share_gemm_dma_kernel<2, 2, 2, 4, 3><<<grid, 512, 0, s>>>(a); break;
-    case 13: share_gemm_dma_kernel<2, 2, 4, 2, 3><<<grid, 512, 0, s>>>(a); break;
-    case 14: share_gemm_dma_kernel<2, 4, 2, 2, 3><<<grid, 256, 0, s>>>(a); break;
-    case 15: share_gemm_hyb_kernel<4, 2, 4><<<grid, 256, 0, s>>>(a); break;
-    case 16: share_gemm_hyb_kernel<4, 2, 2><<<grid, 128, 0, s>>>(a); break;
-    case 17: share_gemm_hyb_kernel<2, 2, 4><<<grid, 256, 0, s>>>(a); break;
+    case 0: share_gemm_lds_kernel<4, 4, 2, 4><<<grid, 512, 0, s>>>(a); break;
+    case 1: share_gemm_hyb_kernel<4, 2, 4><<<grid, 256, 0, s>>>(a); break;
     case kRotVariantBase: share_gemm_rot_kernel<4, 2, 4, 1><<<grid, 256, 0, s>>>(a); break;
     case kRotVariantBase + 1: share_gemm_rot_kernel<4, 2, 2, 1><<<grid, 128, 0, s>>>(a); break;
     case kRotVariantBase + 2: share_gemm_rot_kernel<2, 1, 4, 2><<<grid, 256, 0, s>>>(a); break;

@@ -89,7 +89,7 @@ void launch_untile_rows(const int8_t* src, size_t row0, size_t n_rows, int K, ui
 struct GemmTile {
   int bq, bd;
 };
-constexpr int kGemmVariants = 18;
+constexpr int kGemmVariants = 2;
 // Variants from kRotVariantBase on need a query set with rotation structure
 // (uploaded from raw shares, rotations generated on the device).
 constexpr int kRotVariantBase = 100, kRotVariants = 3;

@@ -82,6 +82,10 @@ SIGNATURES = {
     "irismpc_share_db_get_counter": [vp, vp],
     "irismpc_share_db_set_counter": [vp, u64],
     "irismpc_share_db_set_variant": [vp, i32],
+    "irismpc_share_db_set_timing": [vp, i32],
+    "irismpc_share_db_launch_times": [vp, vp, vp],
+    "irismpc_circuits_set_timing": [vp, i32],
+    "irismpc_circuits_launch_times": [vp, vp, vp],
     "irismpc_gemm_variants": [vp],
     "irismpc_gemm_rot_variants": [vp, vp],
     "irismpc_circuits_new": [i32, i32, sz, sz, vp, vp, vp, i32, vp],

@@ -283,6 +283,16 @@ class ShareDB:
     def set_variant(self, v):
         call("irismpc_share_db_set_variant", self.h, v)
 
+    def set_timing(self, on=True):
+        """Brackets every share GEMM launch with HIP events on its stream."""
+        call("irismpc_share_db_set_timing", self.h, 1 if on else 0)
+
+    def launch_times(self):
+        """(timed launches, summed duration in ms) since the previous call."""
+        n, ms = ctypes.c_size_t(), ctypes.c_double()
+        call("irismpc_share_db_launch_times", self.h, ctypes.byref(n), ctypes.byref(ms))
+        return n.value, ms.value
+
 
 class Graph:
     """HIP graph of the library calls issued on one stream between capture()
@@ -340,6 +350,16 @@ class Circuits:
         if getattr(self, "h", None):
             call("irismpc_circuits_destroy", self.h)
             self.h = None
+
+    def set_timing(self, on=True):
+        """Times each threshold comparison's circuit launches (party 0's engine
+        keeps the intervals of the three parties' joint launches)."""
+        call("irismpc_circuits_set_timing", self.h, 1 if on else 0)
+
+    def launch_times(self):
+        n, ms = ctypes.c_size_t(), ctypes.c_double()
+        call("irismpc_circuits_launch_times", self.h, ctypes.byref(n), ctypes.byref(ms))
+        return n.value, ms.value
 
     def set_chunk_size(self, words):
         call("irismpc_circuits_set_chunk_size", self.h, words)

@@ -50,6 +50,7 @@ class Request:
     rid: int = 0
     result: object = None     # uniqueness: sorted matching rows; reauth: bool; insert: row; delete: row
     done: float = -1.0
+    error: str = None         # set when serving the request failed
     db_rows: int = 0          # DB rows the request was checked against
     covered: set = field(default_factory=set)
     slot: int = -1
@@ -78,19 +79,27 @@ def bitmap_rows(words, n):
 
 class PartySet:
     """Engines of the three parties for one query-set size: codes/masks ShareDB,
-    Circuits and DistanceComparator per party, one exchange, three streams."""
+    Circuits and DistanceComparator per party, one exchange, three streams.
+    Engine seeds are derive_seed(party seed, nonce_base + 0 / 1 / 5): every
+    engine set of one process takes its own nonce_base, so no two sets draw the
+    same keystream (the reference takes a fresh kdf nonce per engine,
+    actor.rs:382-388)."""
 
-    def __init__(self, max_rows, db_capacity, seeds, tag0):
+    def __init__(self, max_rows, db_capacity, seeds, tag0, nonce_base=0):
         self.ex = E.Exchange(timeout_s=300)
         self.codes, self.masks, self.circ, self.cmp, self.streams = [], [], [], [], []
+        self.nonce_base = nonce_base
         for p in range(3):
             so, sp = seeds[p]
-            self.codes.append(E.ShareDB(p, CHUNK, max_rows, E.IRIS_CODE_LENGTH,
-                                        (E.derive_seed(so, 0), E.derive_seed(sp, 0)), comm=self.ex, tag=tag0))
-            self.masks.append(E.ShareDB(p, CHUNK, max_rows, E.MASK_CODE_LENGTH,
-                                        (E.derive_seed(so, 1), E.derive_seed(sp, 1)), comm=self.ex, tag=tag0 + 1))
-            self.circ.append(E.Circuits(p, max_rows * CHUNK, max_rows * CHUNK // 64,
-                                        (E.derive_seed(so, 5), E.derive_seed(sp, 5)), self.ex, tag=tag0 + 2))
+
+            def seed(n):
+                return E.derive_seed(so, nonce_base + n), E.derive_seed(sp, nonce_base + n)
+
+            self.codes.append(E.ShareDB(p, CHUNK, max_rows, E.IRIS_CODE_LENGTH, seed(0), comm=self.ex, tag=tag0))
+            self.masks.append(E.ShareDB(p, CHUNK, max_rows, E.MASK_CODE_LENGTH, seed(1), comm=self.ex,
+                                        tag=tag0 + 1))
+            self.circ.append(E.Circuits(p, max_rows * CHUNK, max_rows * CHUNK // 64, seed(5), self.ex,
+                                        tag=tag0 + 2))
             self.cmp.append(E.DistanceComparator(max_rows, db_capacity))
             self.streams.append(E.stream_create())
 
@@ -104,9 +113,12 @@ class PartySet:
 
 
 class ServingEngines:
-    """DB shares of the three parties plus the scan and reauth engine sets."""
+    """DB shares of the three parties plus the scan and reauth engine sets.
+    dbs = (code_dbs, mask_dbs), per party, serves DB slices that already hold
+    n_rows rows and room for `capacity` (they are written by insertions and
+    deletions); otherwise the slices are allocated here."""
 
-    def __init__(self, n_rows, capacity, max_active=64, seed_base=0, threshold_a=None):
+    def __init__(self, n_rows, capacity, max_active=64, seed_base=0, threshold_a=None, dbs=None):
         import torch
 
         self.torch = torch
@@ -114,10 +126,14 @@ class ServingEngines:
         self.A = E.THRESHOLD_ANON_STATS_A if threshold_a is None else threshold_a
         base = [np.full(8, seed_base + p, np.uint32) for p in range(3)]
         seeds = [(base[p], base[(p + 2) % 3]) for p in range(3)]
-        self.scan = PartySet(max_active * ROT, capacity, seeds, 0)
-        self.small = PartySet(ROT, capacity, seeds, 8)  # reauth: one query, one chunk of 64 rows
-        self.code_dbs = [self.scan.codes[p].alloc_db(capacity) for p in range(3)]
-        self.mask_dbs = [self.scan.masks[p].alloc_db(capacity) for p in range(3)]
+        self.scan = PartySet(max_active * ROT, capacity, seeds, 0, nonce_base=0)
+        # reauth: one query, one chunk of 64 rows; its own keystreams
+        self.small = PartySet(ROT, capacity, seeds, 8, nonce_base=16)
+        if dbs is None:
+            self.code_dbs = [self.scan.codes[p].alloc_db(capacity) for p in range(3)]
+            self.mask_dbs = [self.scan.masks[p].alloc_db(capacity) for p in range(3)]
+        else:
+            self.code_dbs, self.mask_dbs = list(dbs[0]), list(dbs[1])
         self.scan_parties = self.scan.parties(self.code_dbs, self.mask_dbs)
         self.small_parties = self.small.parties(self.code_dbs, self.mask_dbs)
         words = -(-capacity // 64)
@@ -149,7 +165,8 @@ class ServingEngines:
         with self.lock:
             self.sync_scan()  # no queued step may read the rows being written
             row = self.n_rows
-            assert row < self.capacity - CHUNK, "DB capacity exhausted"
+            if row >= self.capacity - CHUNK:
+                raise ValueError(f"DB capacity exhausted ({row} rows of {self.capacity})")
             for p in range(3):
                 self.code_dbs[p].load_records(row, shares[p][0].reshape(1, -1))
                 self.mask_dbs[p].load_records(row, shares[p][1].reshape(1, -1))
@@ -157,6 +174,8 @@ class ServingEngines:
             return row
 
     def delete(self, row):
+        if not 0 <= row < self.n_rows:
+            raise ValueError(f"deletion of row {row} outside the DB ({self.n_rows} rows)")
         if self._dummy is None:
             self._dummy = dummy_shares()
         with self.lock:
@@ -270,6 +289,8 @@ class ServingEngines:
 
     def reauth(self, shares, target):
         """One query x 31 rotations against DB row `target` on the small engine set."""
+        if not 0 <= target < self.n_rows:
+            raise ValueError(f"reauth target {target} outside the DB ({self.n_rows} rows)")
         # refilled in place: an allocation or free would synchronise the whole
         # device and wait for the scan steps queued by the request loop
         if self._small_qsets is None:
@@ -318,6 +339,7 @@ class Server:
         self.steps = 0
         self.admission_waits = 0
         self.max_concurrent = 0
+        self.failed = []  # requests whose serving raised (run() raises after the loop)
 
     def _free_hbm(self):
         free, _ = self.eng.torch.cuda.mem_get_info()
@@ -328,7 +350,11 @@ class Server:
             r = self.reauth_q.get()
             if r is None:
                 return
-            r.result = self.eng.reauth(r.shares, r.target)
+            try:
+                r.result = self.eng.reauth(r.shares, r.target)
+            except Exception as e:  # noqa: BLE001 - recorded on the request, raised by run()
+                r.error = f"{type(e).__name__}: {e}"
+                self.failed.append(r)
             r.db_rows = self.eng.n_rows
             r.done = time.perf_counter() - self.t0
             self.done.append(r)
@@ -354,13 +380,12 @@ class Server:
                     waiting.append(r)
                 elif r.kind == "reauth":
                     self.reauth_q.put(r)
-                elif r.kind == "insert":
-                    r.result = eng.insert(r.shares)
-                    r.db_rows = eng.n_rows
-                    r.done = time.perf_counter() - self.t0
-                    self.done.append(r)
                 else:
-                    r.result = eng.delete(r.target)
+                    try:
+                        r.result = eng.insert(r.shares) if r.kind == "insert" else eng.delete(r.target)
+                    except ValueError as e:
+                        r.error = str(e)
+                        self.failed.append(r)
                     r.db_rows = eng.n_rows
                     r.done = time.perf_counter() - self.t0
                     self.done.append(r)
@@ -381,7 +406,8 @@ class Server:
                 if i < len(reqs):
                     time.sleep(max(0.0, min(0.002, reqs[i].arrival - (time.perf_counter() - self.t0))))
                 continue
-            ids = [r.rid for r in active]
+            # the active set's identity: request objects, not their (caller-set) rids
+            ids = [id(r) for r in active]
             if ids != active_ids:
                 # one set per party and matrix, made once at full size and refilled
                 if qsets_full is None:
@@ -412,6 +438,9 @@ class Server:
                 self.done.append(r)
         self.reauth_q.put(None)
         worker.join()
+        if self.failed:
+            r = self.failed[0]
+            raise RuntimeError(f"{len(self.failed)} request(s) failed; first: {r.kind} rid {r.rid}: {r.error}")
         return reqs
 
 
@@ -437,7 +466,11 @@ def poisson_mix(rate, n, rng, make_uniqueness, make_reauth, make_insert, make_de
 
 
 def latency_stats(reqs, kind=None):
-    lat = np.array([r.done - r.arrival for r in reqs if (kind is None or r.kind == kind) and r.done >= 0])
+    sel = [r for r in reqs if kind is None or r.kind == kind]
+    unfinished = [r for r in sel if r.done < 0 or r.error]
+    if unfinished:
+        raise RuntimeError(f"{len(unfinished)} request(s) unfinished or failed")
+    lat = np.array([r.done - r.arrival for r in sel])
     if lat.size == 0:
         return {"n": 0}
     return {"n": int(lat.size), "p50_ms": float(np.percentile(lat, 50) * 1e3),

@@ -48,6 +48,8 @@ void orc_chacha_block(const uint32_t in[16], uint32_t out[16], int rounds);
 /* Keystream with the reference state layout: consts, key[8], 64-bit counter in
  * words 12-13, 64-bit nonce in words 14-15 (rng/chacha.rs:205-250). Writes
  * n_words u32 starting at block `counter`. */
+/* Threads the keystream and the protocol simulator may use (default 1). */
+void orc_set_threads(int n);
 void orc_chacha_keystream(const uint32_t key[8], uint64_t nonce, uint64_t counter,
                           int rounds, uint32_t *out, size_t n_words);
 
@@ -100,6 +102,9 @@ void orc_iris_mirrored(const uint64_t code[200], const uint64_t mask[200], uint6
  * c = code dot (signed), M = mask dot (threshold_ring/cuda/kernel.cu:189-195,
  * 407-426; tests/threshold.rs:111-124). */
 int orc_mpc_predicate(uint16_t code_dot, uint16_t mask_dot, uint32_t threshold_a);
+/* orc_mpc_predicate over n elements (out: 0 / 1). */
+void orc_mpc_predicate_many(const uint16_t *code_dot, const uint16_t *mask_dot, size_t n,
+                            uint32_t threshold_a, uint8_t *out);
 
 /* ---------------- limbs, sums, share GEMM ------------------------------- */
 /* share_db.rs:45-59 */

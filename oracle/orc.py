@@ -41,6 +41,8 @@ def _declare(L):
     vp, sz, u64, u32, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
     L.orc_chacha_block.argtypes = [vp, vp, i32]
     L.orc_chacha_keystream.argtypes = [vp, u64, u64, i32, vp, sz]
+    L.orc_set_threads.argtypes = [i32]
+    L.orc_mpc_predicate_many.argtypes = [vp, vp, sz, u32, vp]
     L.orc_sha256.argtypes = [vp, sz, vp]
     L.orc_hmac_sha256.argtypes = [vp, sz, vp, sz, vp]
     L.orc_hkdf_sha256.argtypes = [vp, sz, vp, sz, vp, sz, vp, sz]
@@ -88,6 +90,12 @@ def _declare(L):
     L.orc_synth_flip.argtypes = [vp, u64, u64, u32, vp]
     L.orc_synth_share_rand.argtypes = [u64, u64, vp, vp]
     L.orc_synth_shares.argtypes = [vp, vp, u64, u64, vp, vp]
+
+
+def set_threads(n):
+    """Threads the keystream and the protocol simulator use (results do not
+    depend on it; default 1)."""
+    lib().orc_set_threads(int(n))
 
 
 # ---------------------------------------------------------------- crypto
@@ -259,6 +267,15 @@ def iris_distance(c1, m1, c2, m2):
 
 def mpc_predicate(code_dot, mask_dot, threshold_a):
     return bool(lib().orc_mpc_predicate(code_dot & 0xFFFF, mask_dot & 0xFFFF, threshold_a))
+
+
+def mpc_predicate_many(code_dot, mask_dot, threshold_a):
+    """mpc_predicate over arrays of u16 code / mask dots -> bool array."""
+    c = np.ascontiguousarray(code_dot, np.uint16)
+    m = np.ascontiguousarray(mask_dot, np.uint16)
+    out = np.zeros(c.size, np.uint8)
+    lib().orc_mpc_predicate_many(_p(c), _p(m), c.size, threshold_a, _p(out))
+    return out.astype(bool)
 
 
 # ---------------------------------------------------------------- dots

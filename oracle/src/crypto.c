@@ -33,22 +33,28 @@ void orc_chacha_block(const uint32_t in[16], uint32_t out[16], int rounds) {
   for (int i = 0; i < 16; i++) out[i] = x[i] + in[i];
 }
 
+/* Threads for the keystream and the protocol's element loops (1 = serial).
+ * Only the order of the work changes with it, never a value. */
+int orc_threads = 1;
+
+void orc_set_threads(int n) { orc_threads = n > 0 ? n : 1; }
+
 void orc_chacha_keystream(const uint32_t key[8], uint64_t nonce, uint64_t counter, int rounds,
                           uint32_t *out, size_t n_words) {
-  uint32_t st[16] = {0x61707865, 0x3320646e, 0x79622d32, 0x6b206574};
-  memcpy(st + 4, key, 32);
-  st[14] = (uint32_t)nonce;
-  st[15] = (uint32_t)(nonce >> 32);
-  uint32_t blk[16];
-  size_t done = 0;
-  while (done < n_words) {
-    st[12] = (uint32_t)counter;
-    st[13] = (uint32_t)(counter >> 32);
+  const size_t n_blocks = (n_words + 15) / 16;
+#pragma omp parallel for schedule(static) num_threads(orc_threads) if (orc_threads > 1 && n_blocks >= 4096)
+  for (size_t b = 0; b < n_blocks; b++) {
+    uint32_t st[16] = {0x61707865, 0x3320646e, 0x79622d32, 0x6b206574};
+    memcpy(st + 4, key, 32);
+    st[12] = (uint32_t)(counter + b);
+    st[13] = (uint32_t)((counter + b) >> 32);
+    st[14] = (uint32_t)nonce;
+    st[15] = (uint32_t)(nonce >> 32);
+    uint32_t blk[16];
     orc_chacha_block(st, blk, rounds);
-    size_t take = n_words - done < 16 ? n_words - done : 16;
+    const size_t done = b * 16;
+    const size_t take = n_words - done < 16 ? n_words - done : 16;
     memcpy(out + done, blk, take * 4);
-    done += take;
-    counter++;
   }
 }
 

@@ -87,6 +87,11 @@ int orc_mpc_predicate(uint16_t code_dot, uint16_t mask_dot, uint32_t threshold_a
   return (int)(r >> 31);
 }
 
+void orc_mpc_predicate_many(const uint16_t *code_dot, const uint16_t *mask_dot, size_t n,
+                            uint32_t threshold_a, uint8_t *out) {
+  for (size_t i = 0; i < n; i++) out[i] = (uint8_t)orc_mpc_predicate(code_dot[i], mask_dot[i], threshold_a);
+}
+
 void orc_limbs(const uint16_t *v, size_t n, int8_t *l0, int8_t *l1) {
   for (size_t i = 0; i < n; i++) {
     l0[i] = (int8_t)((int)(v[i] & 0xff) - 128);

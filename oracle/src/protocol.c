@@ -24,6 +24,10 @@
 #include <string.h>
 #include "../oracle.h"
 
+extern int orc_threads;
+/* element loops split over orc_threads when long (order of work only) */
+#define PAR_FOR(n) _Pragma("omp parallel for schedule(static) num_threads(orc_threads) if (orc_threads > 1)")
+
 typedef struct {
   uint32_t s1[8], s2[8]; /* chacha1 = own seed, chacha2 = prev party's seed */
   uint64_t nonce, ctr1, ctr2;
@@ -46,6 +50,7 @@ static uint64_t *corr_rand_u64(party_t *P, size_t n_u64) {
   uint32_t *k1 = ks_alloc(P->s1, P->nonce, &P->ctr1, 2 * n_u64);
   uint32_t *k2 = ks_alloc(P->s2, P->nonce, &P->ctr2, 2 * n_u64);
   uint64_t *r = malloc(n_u64 * 8 + 8);
+  PAR_FOR(n_u64)
   for (size_t i = 0; i < n_u64; i++)
     r[i] = (uint64_t)(k1[2 * i] ^ k2[2 * i]) | ((uint64_t)(k1[2 * i + 1] ^ k2[2 * i + 1]) << 32);
   free(k1);
@@ -57,6 +62,7 @@ static uint64_t *corr_rand_u64(party_t *P, size_t n_u64) {
 static void and_pre(party_t *P, const uint64_t *xa, const uint64_t *xb, const uint64_t *ya,
                     const uint64_t *yb, uint64_t *res_a, size_t n, size_t alloc_u64) {
   uint64_t *r = corr_rand_u64(P, alloc_u64);
+  PAR_FOR(n)
   for (size_t i = 0; i < n; i++) res_a[i] = (xa[i] & ya[i]) ^ (xb[i] & ya[i]) ^ (xa[i] & yb[i]) ^ r[i];
   free(r);
 }
@@ -68,6 +74,7 @@ static void send_recv_u64(party_t P[3], uint64_t *const a[3], uint64_t *const b[
   for (int p = 0; p < 3; p++) {
     uint32_t *k = ks_alloc(P[p].s1, P[p].nonce, &P[p].ctr1, 16 * ks);
     msg[p] = malloc(n * 8 + 8);
+    PAR_FOR(n)
     for (size_t i = 0; i < n; i++)
       msg[p][i] = a[p][i] ^ ((uint64_t)k[2 * i] | ((uint64_t)k[2 * i + 1] << 32));
     free(k);
@@ -75,6 +82,7 @@ static void send_recv_u64(party_t P[3], uint64_t *const a[3], uint64_t *const b[
   for (int p = 0; p < 3; p++) {
     int q = NEXT(p);
     uint32_t *k = ks_alloc(P[q].s2, P[q].nonce, &P[q].ctr2, 16 * ks);
+    PAR_FOR(n)
     for (size_t i = 0; i < n; i++)
       b[q][i] = msg[p][i] ^ ((uint64_t)k[2 * i] | ((uint64_t)k[2 * i + 1] << 32));
     free(k);
@@ -83,26 +91,31 @@ static void send_recv_u64(party_t P[3], uint64_t *const a[3], uint64_t *const b[
 }
 
 static void xor_into(uint64_t *d, const uint64_t *s, size_t n) {
+  PAR_FOR(n)
   for (size_t i = 0; i < n; i++) d[i] ^= s[i];
 }
 
 /* Plane j, word i, bit k = bit j of element 64 i + k (kernel.cu:63-107). */
 static void transpose_u16(const uint16_t *in, size_t W, int planes, uint64_t *out) {
-  memset(out, 0, sizeof(uint64_t) * W * (size_t)planes);
-  for (size_t i = 0; i < W; i++)
+#pragma omp parallel for schedule(static) num_threads(orc_threads) if (orc_threads > 1 && W >= 4096)
+  for (size_t i = 0; i < W; i++) {
+    for (int j = 0; j < planes; j++) out[(size_t)j * W + i] = 0;
     for (int k = 0; k < 64; k++) {
       uint16_t v = in[i * 64 + (size_t)k];
       for (int j = 0; j < planes; j++) out[(size_t)j * W + i] |= (uint64_t)((v >> j) & 1) << k;
     }
+  }
 }
 
 static void transpose_u32(const uint32_t *in, size_t W, int planes, uint64_t *out) {
-  memset(out, 0, sizeof(uint64_t) * W * (size_t)planes);
-  for (size_t i = 0; i < W; i++)
+#pragma omp parallel for schedule(static) num_threads(orc_threads) if (orc_threads > 1 && W >= 4096)
+  for (size_t i = 0; i < W; i++) {
+    for (int j = 0; j < planes; j++) out[(size_t)j * W + i] = 0;
     for (int k = 0; k < 64; k++) {
       uint32_t v = in[i * 64 + (size_t)k];
       for (int j = 0; j < planes; j++) out[(size_t)j * W + i] |= (uint64_t)((v >> j) & 1) << k;
     }
+  }
 }
 
 /* split_inner (kernel.cu:205-235): x1 := (a, b) -> three replicated sharings
@@ -207,6 +220,7 @@ static uint16_t *otp_u16(const uint32_t seed[8], uint64_t nonce, uint64_t *ctr, 
   uint32_t *k = ks_alloc(seed, nonce, ctr, n / 2);
   const uint16_t *k16 = (const uint16_t *)k;
   uint16_t *o = malloc(n * 2 + 2);
+  PAR_FOR(n)
   for (size_t i = 0; i < n; i++) o[i] = in[i] ^ k16[i];
   free(k);
   return o;
@@ -222,6 +236,7 @@ static void bit_inject(party_t P[3], rs_t c[3], uint16_t *out_a[3], uint16_t *ou
   uint32_t *r02_0 = ks_alloc(P[0].s2, P[0].nonce, &P[0].ctr2, rn);
   const uint16_t *R01_0 = (const uint16_t *)r01_0, *R02_0 = (const uint16_t *)r02_0;
   uint16_t *y = malloc(n * 2);
+  PAR_FOR(n)
   for (size_t i = 0; i < n; i++) y[i] = (uint16_t)(bit_of(c[0].b, i) * R01_0[i] - R02_0[i]);
   uint16_t *y_enc = otp_u16(P[0].s1, P[0].nonce, &P[0].ctr1, y, n);
   /* party 1 */
@@ -229,6 +244,7 @@ static void bit_inject(party_t P[3], rs_t c[3], uint16_t *out_a[3], uint16_t *ou
   uint32_t *r12_1 = ks_alloc(P[1].s1, P[1].nonce, &P[1].ctr1, rn);
   const uint16_t *R01_1 = (const uint16_t *)r01_1, *R12_1 = (const uint16_t *)r12_1;
   uint16_t *x = malloc(n * 2);
+  PAR_FOR(n)
   for (size_t i = 0; i < n; i++)
     x[i] = (uint16_t)((bit_of(c[1].a, i) ^ bit_of(c[1].b, i)) - R01_1[i]);
   uint16_t *x_enc = otp_u16(P[1].s1, P[1].nonce, &P[1].ctr1, x, n);
@@ -238,20 +254,24 @@ static void bit_inject(party_t P[3], rs_t c[3], uint16_t *out_a[3], uint16_t *ou
   const uint16_t *R12_2 = (const uint16_t *)r12_2, *R02_2 = (const uint16_t *)r02_2;
   uint16_t *x2 = otp_u16(P[2].s2, P[2].nonce, &P[2].ctr2, x_enc, n);
   uint16_t *z = malloc(n * 2);
+  PAR_FOR(n)
   for (size_t i = 0; i < n; i++) z[i] = (uint16_t)(bit_of(c[2].a, i) * x2[i] - R12_2[i]);
   uint16_t *z_enc = otp_u16(P[2].s1, P[2].nonce, &P[2].ctr1, z, n);
+  PAR_FOR(n)
   for (size_t i = 0; i < n; i++) {
     out_a[2][i] = (uint16_t)(-(((uint32_t)R02_2[i] + z[i]) << 1) + bit_of(c[2].a, i));
     out_b[2][i] = (uint16_t)(x2[i] - ((uint32_t)R12_2[i] << 1));
   }
   /* party 0 receives z */
   uint16_t *z0 = otp_u16(P[0].s2, P[0].nonce, &P[0].ctr2, z_enc, n);
+  PAR_FOR(n)
   for (size_t i = 0; i < n; i++) {
     out_a[0][i] = (uint16_t)(R01_0[i] - ((uint32_t)y[i] << 1));
     out_b[0][i] = (uint16_t)(-(((uint32_t)R02_0[i] + z0[i]) << 1) + bit_of(c[0].b, i));
   }
   /* party 1 receives y */
   uint16_t *y1 = otp_u16(P[1].s2, P[1].nonce, &P[1].ctr2, y_enc, n);
+  PAR_FOR(n)
   for (size_t i = 0; i < n; i++) {
     out_a[1][i] = (uint16_t)(x[i] - ((uint32_t)R12_1[i] << 1));
     out_b[1][i] = (uint16_t)(R01_1[i] - ((uint32_t)y1[i] << 1));
@@ -337,6 +357,7 @@ uint64_t orc_threshold(const uint16_t *const code_a[3], const uint16_t *const co
   lift_mpc(P, mask_a, mask_b, L, alloc_words, xa, xb, ca, cb);
   /* lift_mul_sub (kernel.cu:181-195, 407-426) */
   for (int p = 0; p < 3; p++) {
+    PAR_FOR(L)
     for (size_t i = 0; i < L; i++) {
       uint32_t ma = xa[p][i] - ((uint32_t)ca[p][i] << 16) - ((uint32_t)ca[p][i + L] << 17);
       uint32_t mb = xb[p][i] - ((uint32_t)cb[p][i] << 16) - ((uint32_t)cb[p][i + L] << 17);

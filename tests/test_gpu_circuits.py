@@ -48,22 +48,23 @@ def _dev(torch, arr):
     return torch.from_numpy(arr.view(np.int16).copy()).cuda()
 
 
-def _setup(irismpc, torch, W, alloc, counter=0):
+def _setup(irismpc, torch, W, alloc, counter=0, seeds=(SEEDS_OWN, SEEDS_PREV)):
     ex = irismpc.Exchange()
-    circ = [irismpc.Circuits(p, 64 * W, alloc, (SEEDS_OWN[p], SEEDS_PREV[p]), ex) for p in range(3)]
+    circ = [irismpc.Circuits(p, 64 * W, alloc, (seeds[0][p], seeds[1][p]), ex) for p in range(3)]
     for c in circ:
         c.counter = counter
     streams = [irismpc.stream_create() for _ in range(3)]
     return ex, circ, streams
 
 
-def _run_case(irismpc, orc, torch, W, alloc, A, seed, counter=0, threads=False, sampler=None):
+def _run_case(irismpc, orc, torch, W, alloc, A, seed, counter=0, threads=False, sampler=None,
+              seeds=(SEEDS_OWN, SEEDS_PREV)):
     rng = np.random.default_rng(seed)
     L = 64 * W
     code, mask = (sampler or _sample)(rng, L)
     ca, cb = _rep(rng, code)
     ma, mb = _rep(rng, mask)
-    ex, circ, streams = _setup(irismpc, torch, W, alloc, counter)
+    ex, circ, streams = _setup(irismpc, torch, W, alloc, counter, seeds)
     dca = [_dev(torch, x) for x in ca]
     dcb = [_dev(torch, x) for x in cb]
     dma = [_dev(torch, x) for x in ma]
@@ -103,7 +104,7 @@ def _run_case(irismpc, orc, torch, W, alloc, A, seed, counter=0, threads=False, 
     got_a = [fetch(v[0], W) for v in views]
     got_b = [fetch(v[1], W) for v in views]
     got_c = [fetch(v[2], W) for v in views]
-    ra, rb, ctr, lift_ref = orc.threshold(ca, cb, ma, mb, alloc, A, SEEDS_OWN, SEEDS_PREV,
+    ra, rb, ctr, lift_ref = orc.threshold(ca, cb, ma, mb, alloc, A, seeds[0], seeds[1],
                                           nonce=0, counter=counter, want_lifted=True)
     for p in range(3):
         assert np.array_equal(lifted[p][0].cpu().numpy().view(np.uint32), lift_ref[p]), f"lift p{p}"
@@ -113,7 +114,7 @@ def _run_case(irismpc, orc, torch, W, alloc, A, seed, counter=0, threads=False, 
         assert circ[p].counter == ctr
     opened = got_a[0] ^ got_b[0] ^ got_c[0]
     bits = np.unpackbits(opened.view(np.uint8), bitorder="little").astype(bool)
-    exp = np.array([orc.mpc_predicate(int(c), int(m), A) for c, m in zip(code, mask)])
+    exp = orc.mpc_predicate_many(code, mask, A)
     assert np.array_equal(bits, exp)
     for p in range(3):
         o = views[p]
@@ -173,3 +174,38 @@ def test_threshold_boundary(irismpc, orc, A):
     import torch
 
     _run_case(irismpc, orc, torch, W=256, alloc=256, A=A, seed=A, sampler=_boundary_sample(A))
+
+
+def test_threshold_unpaired_seeds_rejected(irismpc):
+    """Seeds that are not the trio's pairwise seeds (party p's prev seed is not
+    party p - 1's own seed) are rejected before any launch: the exchange
+    between the parties' waves skips the reference's OTP, which is the identity
+    only for pairwise seeds (the oracle, which applies the OTP, would differ)."""
+    import torch
+
+    own = np.array([[10 + p] * 8 for p in range(3)], np.uint32)
+    prev = np.array([[20 + p] * 8 for p in range(3)], np.uint32)
+    W = 64
+    ex, circ, streams = _setup(irismpc, torch, W, W, 0, (own, prev))
+    buf = torch.zeros(64 * W, dtype=torch.int16, device="cuda")
+    torch.cuda.synchronize()
+    ptrs = [(buf.data_ptr(), buf.data_ptr())] * 3
+    with pytest.raises(RuntimeError, match="pairwise"):
+        irismpc.Circuits.compare_threshold_masked_many_trio(circ, ptrs, ptrs, 64 * W, 16384, streams)
+    assert all(c.counter == 0 for c in circ)
+
+
+def test_threshold_full_chunk(irismpc, orc):
+    """One full chunk of the scan: 1,984 query-rotation rows x 32,768 DB rows =
+    1,015,808 result words, share for share against the oracle (its simulator
+    on the host's threads)."""
+    import os
+
+    import torch
+
+    orc.set_threads(min(16, os.cpu_count() or 1))
+    try:
+        W = 1984 * 32768 // 64
+        _run_case(irismpc, orc, torch, W=W, alloc=W, A=16384, seed=77)
+    finally:
+        orc.set_threads(1)

@@ -72,13 +72,20 @@ def _plain_bitmap(orc, db_c, db_m, q_c, q_m, A):
     return bits
 
 
-def test_pipeline_three_party(irismpc, orc):
+@pytest.mark.parametrize("N,B,planted,flips", [
+    (2000, 8, [0, 2, 5], 1280),
+    # the CPU-runnable configuration as stated: DB 4,096, 32 queries x 31
+    # rotations, 8 of them DB copies with exactly 1,920 (15 %) code bits flipped
+    (4096, 32, [0, 4, 8, 12, 16, 20, 24, 28], 1920),
+], ids=["small", "config1"])
+def test_pipeline_three_party(irismpc, orc, N, B, planted, flips):
+    import os
+
     import torch
 
-    N, B = 2000, 8
+    orc.set_threads(min(16, os.cpu_count() or 1))
     Qr = B * 31
-    planted = [0, 2, 5]
-    db_c, db_m, q_c, q_m = _workload(orc, N, B, planted, flips=1280)
+    db_c, db_m, q_c, q_m = _workload(orc, N, B, planted, flips=flips)
     dbc_sh, dbm_sh = _shares(orc, db_c, db_m, 30)
     qc_sh, qm_sh = _shares(orc, q_c, q_m, 31)
 
@@ -108,6 +115,8 @@ def test_pipeline_three_party(irismpc, orc):
 
     exp = _plain_bitmap(orc, db_c, db_m, q_c, q_m, A_0375)
     assert exp.sum() >= len(planted)
+    for j in planted:  # every planted copy is found
+        assert exp[j * N:(j + 1) * N].any()
     for p in range(3):
         got = np.unpackbits(bitmaps[p].cpu().numpy().view(np.uint8), bitorder="little")[:N * B]
         assert np.array_equal(got.astype(bool), exp), f"party {p} bitmap"
@@ -149,3 +158,4 @@ def test_pipeline_three_party(irismpc, orc):
         W = Qr * dot_chunk // 64
         assert np.array_equal(irismpc.download(a, W, np.uint64), ra[p]), f"party {p} result a"
         assert np.array_equal(irismpc.download(b, W, np.uint64), rb[p]), f"party {p} result b"
+    orc.set_threads(1)

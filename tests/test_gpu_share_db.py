@@ -129,7 +129,7 @@ def test_share_gemm_rotation_variants(irismpc, orc, K, mult, batch, qr):
     slices = eng.alloc_db(N)
     slices.load_full_db(db)
     dq = eng.upload_query_shares(raw)
-    for v in [-1, 7, 15] + irismpc.gemm_rot_variants():
+    for v in [-1] + list(range(irismpc.gemm_variants())) + irismpc.gemm_rot_variants():
         eng.set_variant(v)
         eng.dot_reduce(dq, slices, chunk, 0, mult)
         got = eng.fetch_results(qr * chunk).reshape(qr, chunk)
@@ -156,7 +156,7 @@ def test_query_refill_matches_fresh_upload(irismpc, orc, K, mult):
         ref = orc.u16_dot(_rotated_rows(orc, party, raw, qr), db)
         ref = (ref.astype(np.uint32) * mult).astype(np.uint16)
         eng.refill_query_shares(dq, raw, qr)
-        for v in (-1, 7):
+        for v in (-1, 0):
             eng.set_variant(v)
             eng.dot_reduce(dq, slices, chunk, 0, mult)
             got = eng.fetch_results(qr * chunk).reshape(qr, chunk)
@@ -186,7 +186,7 @@ def test_share_gemm_rotation_masked_range(irismpc, orc):
     slices.load_full_db(db)
     dq = eng.upload_query_shares(raw)
     outs = []
-    variants = irismpc.gemm_rot_variants() + [7]
+    variants = irismpc.gemm_rot_variants() + [0]
     for v in variants:
         eng.set_variant(v)
         eng.counter = 40
