@@ -12,9 +12,11 @@
  *   ChaCha block         : RFC 8439 s2.3.2 (20 rounds) + published 12-round
  *                          all-zero vector; reference has no KAT.
  *   SHA-256/HMAC/HKDF    : FIPS 180-2 "abc", RFC 5869 test cases 1 and 3.
- *   derive_seed          : composition of HKDF (pinned) with the reference's
- *                          salt/ikm/info layout (server/actor.rs:2914-2925);
- *                          output values themselves: parity unpinned.
+ *   derive_seed          : equal to an independent RFC 5869 HKDF on Python's
+ *                          hmac/hashlib built from the reference's literals
+ *                          (KDF_SALT actor.rs:116; LE ikm words, 8-byte BE
+ *                          nonce, LE output words, actor.rs:2914-2925); the
+ *                          reference holds no derive_seed fixture.
  *   GR(2^16,4) arithmetic: Sage spec vectors (gr4 product/inverse), basis
  *                          identity <u_A,v_A> == coord0_B(S u_A * S v_A).
  *   Share encoding       : reconstruction invariant sum_i trick_dot == plaintext

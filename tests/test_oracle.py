@@ -64,13 +64,45 @@ def test_sha256_hkdf_vectors(orc):
                           "9d201395faa4b61a96c8")
 
 
-def test_derive_seed_layout(orc):
-    # derive_seed = HKDF(salt=KDF_SALT, ikm=LE bytes of seed, info=BE nonce)[:32] as LE u32
+def _stdlib_derive_seed(seed, nonce):
+    """derive_seed of iris-mpc-gpu/src/server/actor.rs:2914-2925 written
+    independently on Python's hmac/hashlib (RFC 5869 extract + one expand
+    block): salt = hex-decoded KDF_SALT (actor.rs:116), ikm = the 8 u32 seed
+    words as little-endian bytes (bytemuck::cast_slice), info = the usize nonce
+    as 8 big-endian bytes (to_be_bytes on a 64-bit target), output = the first
+    32 bytes read back as little-endian u32 words."""
+    import hashlib
+    import hmac
+
     salt = bytes.fromhex("111a1a93518f670e9bb0c2c68888e2beb9406d4c4ed571dc77b801e676ae3091")
-    seed = np.array([1, 2, 3, 4, 5, 6, 7, 8], np.uint32)
-    okm = orc.hkdf_sha256(salt, seed.tobytes(), (5).to_bytes(8, "big"), 32)
-    assert np.array_equal(orc.derive_seed(seed, 5), np.frombuffer(okm, np.uint32))
-    assert not np.array_equal(orc.derive_seed(seed, 5), orc.derive_seed(seed, 4))
+    ikm = b"".join(int(w).to_bytes(4, "little") for w in seed)
+    prk = hmac.new(salt, ikm, hashlib.sha256).digest()
+    okm = hmac.new(prk, nonce.to_bytes(8, "big") + b"\x01", hashlib.sha256).digest()
+    return [int.from_bytes(okm[4 * i:4 * i + 4], "little") for i in range(8)]
+
+
+@pytest.mark.parametrize("seed,nonce", [
+    ([0] * 8, 0), ([1] * 8, 1), ([2] * 8, 5),           # the test seeds of tests/threshold.rs, server nonces
+    ([1, 2, 3, 4, 5, 6, 7, 8], 5), ([0xFFFFFFFF] * 8, 2 ** 40 + 3),
+])
+def test_derive_seed_layout(orc, seed, nonce):
+    """The oracle's derive_seed equals an independent HKDF over the stdlib with
+    the reference's literals; a plausible layout mistake (big-endian ikm words,
+    a little-endian or 4-byte nonce, a missing salt) gives other words."""
+    want = _stdlib_derive_seed(seed, nonce)
+    got = orc.derive_seed(np.array(seed, np.uint32), nonce)
+    assert [int(x) for x in got] == want
+    assert not np.array_equal(orc.derive_seed(np.array(seed, np.uint32), nonce + 1), got)
+
+
+def test_product_derive_seed_matches_stdlib():
+    """The library's derive_seed (host_crypto.cpp behind irismpc_derive_seed)
+    equals the stdlib construction; the library loads without a GPU."""
+    import irismpc
+
+    for seed, nonce in (([0] * 8, 0), ([2] * 8, 21), ([7, 1, 0, 9, 3, 3, 8, 2], 123456789)):
+        assert [int(x) for x in irismpc.derive_seed(np.array(seed, np.uint32), nonce)] == \
+            _stdlib_derive_seed(seed, nonce)
 
 
 # --------------------------------------------------------------- GR(2^16,4)
