@@ -104,6 +104,10 @@ void orc_iris_mirrored(const uint64_t code[200], const uint64_t mask[200], uint6
  * c = code dot (signed), M = mask dot (threshold_ring/cuda/kernel.cu:189-195,
  * 407-426; tests/threshold.rs:111-124). */
 int orc_mpc_predicate(uint16_t code_dot, uint16_t mask_dot, uint32_t threshold_a);
+/* Plaintext brute force of one query (all rotations) against n DB irises
+ * ([n][200] codes and masks) at the integer predicate: out[d] = 0 / 1. */
+void orc_plain_match(const uint64_t qc[200], const uint64_t qm[200], const uint64_t *dc,
+                     const uint64_t *dm, size_t n, uint32_t threshold_a, uint8_t *out, int threads);
 /* orc_mpc_predicate over n elements (out: 0 / 1). */
 void orc_mpc_predicate_many(const uint16_t *code_dot, const uint16_t *mask_dot, size_t n,
                             uint32_t threshold_a, uint8_t *out);

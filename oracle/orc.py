@@ -43,6 +43,7 @@ def _declare(L):
     L.orc_chacha_keystream.argtypes = [vp, u64, u64, i32, vp, sz]
     L.orc_set_threads.argtypes = [i32]
     L.orc_mpc_predicate_many.argtypes = [vp, vp, sz, u32, vp]
+    L.orc_plain_match.argtypes = [vp, vp, vp, vp, sz, u32, vp, i32]
     L.orc_sha256.argtypes = [vp, sz, vp]
     L.orc_hmac_sha256.argtypes = [vp, sz, vp, sz, vp]
     L.orc_hkdf_sha256.argtypes = [vp, sz, vp, sz, vp, sz, vp, sz]
@@ -275,6 +276,18 @@ def mpc_predicate_many(code_dot, mask_dot, threshold_a):
     m = np.ascontiguousarray(mask_dot, np.uint16)
     out = np.zeros(c.size, np.uint8)
     lib().orc_mpc_predicate_many(_p(c), _p(m), c.size, threshold_a, _p(out))
+    return out.astype(bool)
+
+
+def plain_match(q_code, q_mask, db_codes, db_masks, threshold_a, threads=16):
+    """Plaintext brute force: bool per DB iris ([n][200] u64 codes / masks) that
+    some rotation of the query matches at the integer MPC predicate."""
+    qc = np.ascontiguousarray(q_code, np.uint64)
+    qm = np.ascontiguousarray(q_mask, np.uint64)
+    dc = np.ascontiguousarray(db_codes, np.uint64).reshape(-1, 200)
+    dm = np.ascontiguousarray(db_masks, np.uint64).reshape(-1, 200)
+    out = np.zeros(dc.shape[0], np.uint8)
+    lib().orc_plain_match(_p(qc), _p(qm), _p(dc), _p(dm), dc.shape[0], threshold_a, _p(out), threads)
     return out.astype(bool)
 
 

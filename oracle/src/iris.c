@@ -92,6 +92,33 @@ void orc_mpc_predicate_many(const uint16_t *code_dot, const uint16_t *mask_dot, 
   for (size_t i = 0; i < n; i++) out[i] = (uint8_t)orc_mpc_predicate(code_dot[i], mask_dot[i], threshold_a);
 }
 
+/* Plaintext brute force of one query against n DB irises at the MPC
+ * comparison's integer predicate: out[d] = OR over the 31 rotations t of
+ * orc_mpc_predicate(M - 2H, M, A) with the query's code and mask rotated by
+ * rotation t (orc_iris_all_rotations), M = |qm_t & dm_d| and
+ * H = |(qc_t ^ dc_d) & qm_t & dm_d| (the masked Hamming distance of
+ * iris.rs:534-560; the opened code / mask dots are M - 2H and M). */
+void orc_plain_match(const uint64_t qc[200], const uint64_t qm[200], const uint64_t *dc,
+                     const uint64_t *dm, size_t n, uint32_t threshold_a, uint8_t *out, int threads) {
+  uint64_t *rc = malloc(sizeof(uint64_t) * ORC_ROTATIONS * 200);
+  uint64_t *rm = malloc(sizeof(uint64_t) * ORC_ROTATIONS * 200);
+  orc_iris_all_rotations(qc, rc);
+  orc_iris_all_rotations(qm, rm);
+#pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : 1)
+  for (long d = 0; d < (long)n; d++) {
+    uint8_t hit = 0;
+    for (int t = 0; t < ORC_ROTATIONS && !hit; t++) {
+      uint32_t h, m;
+      orc_iris_distance(rc + (size_t)t * 200, rm + (size_t)t * 200, dc + (size_t)d * 200,
+                        dm + (size_t)d * 200, &h, &m);
+      hit = (uint8_t)orc_mpc_predicate((uint16_t)(m - 2 * h), (uint16_t)m, threshold_a);
+    }
+    out[d] = hit;
+  }
+  free(rc);
+  free(rm);
+}
+
 void orc_limbs(const uint16_t *v, size_t n, int8_t *l0, int8_t *l1) {
   for (size_t i = 0; i < n; i++) {
     l0[i] = (int8_t)((int)(v[i] & 0xff) - 128);

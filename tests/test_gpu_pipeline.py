@@ -159,3 +159,95 @@ def test_pipeline_three_party(irismpc, orc, N, B, planted, flips):
         assert np.array_equal(irismpc.download(a, W, np.uint64), ra[p]), f"party {p} result a"
         assert np.array_equal(irismpc.download(b, W, np.uint64), rb[p]), f"party {p} result b"
     orc.set_threads(1)
+
+
+def test_pipeline_multi_chunk_share_parity(irismpc, orc):
+    """Three chunks (two full, the last 1,000 rows) in the chunk loop with two
+    result slots: every bitmap bit equals the plaintext brute force, and the
+    last chunk's masked dot shares and circuit result shares equal the
+    oracle's at the counters that the two earlier chunks advanced to (ShareDB:
+    mask + OTP blocks per chunk; Circuits: one schedule per chunk)."""
+    import os
+
+    import torch
+
+    from irismpc import synth
+
+    orc.set_threads(min(16, os.cpu_count() or 1))
+    chunk = irismpc.DB_CHUNK_SIZE
+    N, U, B = 2 * chunk + 1000, 1024, 2
+    Qr = B * 31
+    targets = [517, 999]  # copies in every chunk, the last included
+    dc, dm = zip(*[orc.synth_iris(41, u) for u in range(U)])
+    dc, dm = np.array(dc), np.array(dm)
+    q_c = np.array([orc.synth_flip(dc[t], 42, j, 1500) for j, t in enumerate(targets)])
+    q_m = dm[targets]
+    qc_sh, qm_sh = _shares(orc, q_c, q_m, 43)
+
+    base = [np.full(8, 10 + p, np.uint32) for p in range(3)]
+    der = lambda p, n: (irismpc.derive_seed(base[p], n), irismpc.derive_seed(base[(p + 2) % 3], n))  # noqa: E731
+    ex = irismpc.Exchange(timeout_s=120)
+    parties, qcs, qms, bitmaps, streams = [], [], [], [], []
+    for p in range(3):
+        codes = irismpc.ShareDB(p, chunk, Qr, 12800, der(p, 0), comm=ex, tag=0)
+        masks = irismpc.ShareDB(p, chunk, Qr, 6400, der(p, 1), comm=ex, tag=1)
+        circ = irismpc.Circuits(p, Qr * chunk, Qr * chunk // 64, der(p, 5), ex, tag=2)
+        cmp = irismpc.DistanceComparator(Qr, N)
+        parties.append(irismpc.PartyEngines(codes, masks, circ, cmp, codes.alloc_db(N), masks.alloc_db(N)))
+        bitmaps.append(torch.zeros(N * B // 64 + 2, dtype=torch.int64, device="cuda"))
+        streams.append(irismpc.stream_create())
+    synth.build_db([pe.code_db for pe in parties], [pe.mask_db for pe in parties], N, U, 41, 44)
+    for p in range(3):
+        qcs.append(parties[p].codes.upload_query_shares(qc_sh[p]))
+        qms.append(parties[p].masks.upload_query_shares(qm_sh[p]))
+    torch.cuda.synchronize()
+    irismpc.scan_db_trio(parties, qcs, qms, N, B, A_0375, [b.data_ptr() for b in bitmaps], streams)
+    for s in streams:
+        irismpc.stream_synchronize(s)
+
+    hit = np.array([orc.plain_match(q_c[j], q_m[j], dc, dm, A_0375) for j in range(B)])  # [B][U]
+    exp = np.concatenate([np.tile(hit[j], -(-N // U))[:N] for j in range(B)])
+    assert all(hit[j, t] for j, t in enumerate(targets))
+    for p in range(3):
+        got = np.unpackbits(bitmaps[p].cpu().numpy().view(np.uint8), bitorder="little")[:N * B]
+        assert np.array_equal(got.astype(bool), exp), f"party {p} bitmap"
+
+    # counters after the two full chunks (rows % 8 != 0: chunks are padded to 512)
+    last0, real = 2 * chunk, N - 2 * chunk
+    dot_chunk = -(-real // 512) * 512
+    blocks = lambda width: -(-(width * Qr * 2) // 64)  # noqa: E731
+    share_ctr = 2 * 2 * blocks(chunk)
+    assert parties[0].codes.counter == share_ctr + 2 * blocks(dot_chunk)
+    zc = [np.zeros(Qr * chunk, np.uint16)] * 3
+    so = np.array([der(p, 5)[0] for p in range(3)])
+    sp = np.array([der(p, 5)[1] for p in range(3)])
+    alloc = Qr * chunk // 64
+    circ_ctr = 0
+    for _ in range(2):  # the schedule's advance depends on the geometry only
+        _, _, circ_ctr = orc.threshold(zc, zc, zc, zc, alloc, A_0375, so, sp, counter=circ_ctr)
+
+    rows_c = [np.zeros((Qr, 12800), np.uint16) for _ in range(3)]
+    rows_m = [np.zeros((Qr, 6400), np.uint16) for _ in range(3)]
+    code_sh, mask_sh = [], []
+    for p in range(3):
+        for j in range(B):
+            rows_c[p][j * 31:(j + 1) * 31] = orc.share_rotations(orc.preprocess_query(p, qc_sh[p, j]))
+            rows_m[p][j * 31:(j + 1) * 31] = orc.share_rotations(orc.preprocess_query(p, qm_sh[p, j]))
+        pad_c = np.full((dot_chunk, 12800), 0x8080, np.uint16)  # zero limbs beyond the DB
+        pad_m = np.full((dot_chunk, 6400), 0x8080, np.uint16)
+        pad_c[:real] = parties[p].code_db.read_records(last0, real)
+        pad_m[:real] = parties[p].mask_db.read_records(last0, real)
+        sc, sm = der(p, 0), der(p, 1)
+        code_sh.append(orc.mask_dot_share(orc.u16_dot(rows_c[p], pad_c).reshape(-1), 1, sc[0], sc[1], 0, share_ctr))
+        mask_sh.append(orc.mask_dot_share(orc.u16_dot(rows_m[p], pad_m).reshape(-1), 2, sm[0], sm[1], 0, share_ctr))
+        assert np.array_equal(parties[p].codes.fetch_results(Qr * dot_chunk), code_sh[p]), f"party {p} code shares"
+        assert np.array_equal(parties[p].masks.fetch_results(Qr * dot_chunk), mask_sh[p]), f"party {p} mask shares"
+    ra, rb, end = orc.threshold(code_sh, [code_sh[(p + 2) % 3] for p in range(3)], mask_sh,
+                                [mask_sh[(p + 2) % 3] for p in range(3)], alloc, A_0375, so, sp, counter=circ_ctr)
+    W = Qr * dot_chunk // 64
+    for p in range(3):
+        a, b, _c = parties[p].circuits.open_views()
+        assert np.array_equal(irismpc.download(a, W, np.uint64), ra[p]), f"party {p} result a"
+        assert np.array_equal(irismpc.download(b, W, np.uint64), rb[p]), f"party {p} result b"
+        assert parties[p].circuits.counter == end
+    orc.set_threads(1)

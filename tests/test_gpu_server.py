@@ -1,8 +1,9 @@
 """Batch processing end to end (ServerActor::process_batch_query,
 iris-mpc-gpu/src/server/actor.rs:674-1790): two batches against a 1,000-row
-DB of both eyes, with the outcome of every request known by construction
-(flipped copies at 10% distance match, distinct synthetic irises sit near
-0.5). Modelled on the request kinds of the reference's e2e generator
+DB of both eyes. The expected outcome of every request is written out, and
+for the first batch the DB matches and the batch self-deduplication are also
+derived from the oracle's plaintext brute force (both eyes at 0.345, the
+OR rule, the deleted row) and must agree with both. Modelled on the request kinds of the reference's e2e generator
 (iris-mpc-common/src/test.rs): unique insertions, DB matches, batch
 duplicates, skip_persistence, successful and failed reauths, a left-only
 match under the AND rule and under the OR rule, a deletion, and in the second
@@ -70,7 +71,7 @@ class Batch:
                           self.reauth, list(deletions))
 
 
-def test_server_two_batches(irismpc):
+def test_server_two_batches(irismpc, orc):
     from irismpc import synth
     from irismpc.server import ServerActor
 
@@ -121,6 +122,19 @@ def test_server_two_batches(irismpc):
     assert r1.partial_match_rotation_indices_left[1] == [[0]]
     assert r1.deleted_ids == [70]
     assert actor.current_db_size == N0 + 5
+
+    # the same answers from the plaintext brute force (integer MPC predicate, 0.345)
+    A = irismpc.THRESHOLD_A
+    plain_db = {e: (np.array([c for c, _ in db[e]]), np.array([m for _, m in db[e]])) for e in ("left", "right")}
+    for q in range(len(b1.ids)):
+        hit = {e: orc.plain_match(*b1.irises[e][q], *plain_db[e], A) for e in ("left", "right")}
+        rows = set(np.flatnonzero(hit["left"] & hit["right"]).tolist())
+        rows |= {d for d in b1.or_rule[q] if hit["left"][d] or hit["right"][d]}
+        assert set(r1.match_ids[q]) == rows - {70}, f"request {b1.ids[q]}: DB matches"
+        dup = [b1.ids[j] for j in range(q)
+               if all(orc.plain_match(*b1.irises[e][q], *(np.array([x]) for x in b1.irises[e][j]), A)[0]
+                      for e in ("left", "right"))]
+        assert r1.matched_batch_request_ids[q] == dup, f"request {b1.ids[q]}: batch duplicates"
 
     b2 = Batch()
     b2.add("q_u0", *fresh(0, 20))                       # 0 -> row 1000 (inserted in batch 1)

@@ -3,8 +3,8 @@ and deletion requests in the mix.
 
 Checked on a 4-chunk DB of synthetic irises (4,096 distinct irises tiled):
 - every uniqueness result in the mix equals the result of the same request
-  alone (the plain per-request form) and the answer known by construction
-  (planted copies of DB iris t match exactly rows r with r mod 4096 = t);
+  alone (the plain per-request form) and the oracle's plaintext brute force
+  over the tiled DB with the integer MPC predicate;
 - reauth against the right / a wrong row; an inserted iris is found by a
   later query; a deleted row no longer matches;
 - requests overlap in the active set; admission by free HBM lets one request
@@ -44,7 +44,25 @@ def _query(synth, ds_iris, t, flip_idx, flips, rand_idx):
     return [(sh[p][0][0], sh[p][1][0]) for p in range(3)], c, m
 
 
-def test_serving_mix_matches_alone(served):
+def _brute_force(orc, c, m, deleted=(), extra=None):
+    """Rows of the served DB (iris (71, r mod U) in row r < N, deleted rows
+    excepted, `extra` {row: (code, mask)} after them) that the query matches
+    at the 0.375 prefilter: the oracle's plaintext brute force."""
+    global _DB_PLAIN
+    if _DB_PLAIN is None:
+        _DB_PLAIN = tuple(map(np.array, zip(*[orc.synth_iris(71, u) for u in range(U)])))
+    hit = orc.plain_match(c, m, *_DB_PLAIN, 16384)
+    rows = [r for r in np.flatnonzero(np.tile(hit, N // U)).tolist() if r not in deleted]
+    for row, (ec, em) in sorted((extra or {}).items()):
+        if orc.plain_match(c, m, ec[None], em[None], 16384)[0]:
+            rows.append(row)
+    return rows
+
+
+_DB_PLAIN = None
+
+
+def test_serving_mix_matches_alone(served, orc):
     eng, serving, synth = served
     R = serving.Request
     rng = np.random.default_rng(3)
@@ -54,12 +72,12 @@ def test_serving_mix_matches_alone(served):
     # uniqueness: 6 planted (10-30 % flips), 4 fresh irises
     for k in range(10):
         if k < 6:
-            sh, _, _ = _query(synth, 71, int(targets[k]), k, int(rng.integers(1280, 3841)), k)
-            expect[k] = [r for r in range(N) if r % U == targets[k] and r != deleted]
+            sh, c, m = _query(synth, 71, int(targets[k]), k, int(rng.integers(1280, 3841)), k)
         else:
-            sh, _, _ = _query(synth, 73, k, 0, 0, k)
-            expect[k] = []
+            sh, c, m = _query(synth, 73, k, 0, 0, k)
+        expect[k] = _brute_force(orc, c, m, deleted={deleted})
         reqs.append(R("uniqueness", 0.002 * k, shares=sh, rid=k))
+    assert all(expect[k] for k in range(6)) and not any(expect[k] for k in range(6, 10))
     # reauth: right row, wrong row
     sh, _, _ = _query(synth, 71, int(targets[0]), 50, 1500, 50)
     reqs.append(R("reauth", 0.001, shares=sh, target=int(targets[0]) + U, rid=20))
@@ -70,8 +88,8 @@ def test_serving_mix_matches_alone(served):
     reqs.append(R("insert", 0.0, shares=[(ins[p][0][0], ins[p][1][0]) for p in range(3)], rid=30))
     reqs.append(R("delete", 0.0, target=deleted, rid=31))
     # after both applied: a query for the inserted iris, one for the deleted row's iris
-    sh_new, _, _ = _query(synth, 74, 1, 60, 2000, 60)
-    sh_del, _, _ = _query(synth, 71, int(targets[2]), 61, 2000, 61)
+    sh_new, c_new, m_new = _query(synth, 74, 1, 60, 2000, 60)
+    sh_del, c_del, m_del = _query(synth, 71, int(targets[2]), 61, 2000, 61)
     reqs.append(R("uniqueness", 0.6, shares=sh_new, rid=40))
     reqs.append(R("uniqueness", 0.6, shares=sh_del, rid=41))
 
@@ -82,8 +100,9 @@ def test_serving_mix_matches_alone(served):
         assert out[k].result == expect[k], f"uniqueness {k} in the mix"
     assert out[20].result is True and out[21].result is False
     assert out[30].result == N and eng.n_rows == N + 1
-    assert out[40].result == [N]
-    assert out[41].result == expect[2]
+    ins = {N: (new_c, new_m)}
+    assert out[40].result == _brute_force(orc, c_new, m_new, {deleted}, ins) == [N]
+    assert out[41].result == _brute_force(orc, c_del, m_del, {deleted}, ins)
     # the same requests alone (plain per-request form)
     for k in (0, 3, 7):
         assert eng.run_alone(reqs[k].shares) == out[k].result, f"uniqueness {k} alone"
@@ -140,3 +159,27 @@ def test_empty_db_then_insert(served):
     assert res[0].result in ([], [0])  # checked before or after the insertion, as it was admitted
     near, _, _ = _query(synth, 78, 3, 5, 1500, 4)  # the inserted iris with 1,500 code bits flipped
     assert eng.run_alone(near) == [0]
+
+
+def test_default_rids_do_not_alias(served, orc):
+    """Two uniqueness requests left at the default rid, with different
+    answers: the request loop tells them apart (it keys its active set on the
+    requests themselves), so each is scanned with its own shares, served one
+    after the other and together."""
+    eng, serving, synth = served
+    R = serving.Request
+    sh_a, c_a, m_a = _query(synth, 71, 11, 70, 1500, 70)
+    sh_b, c_b, m_b = _query(synth, 71, 12, 71, 1500, 71)
+    ea, eb = _brute_force(orc, c_a, m_a), _brute_force(orc, c_b, m_b)
+    assert ea and eb and not set(ea) & set(eb)
+    alone_a, alone_b = eng.run_alone(sh_a), eng.run_alone(sh_b)
+    # an earlier test deletes one DB row: at most one brute-force row is gone
+    for got, exp in ((alone_a, ea), (alone_b, eb)):
+        assert set(got) <= set(exp) and len(got) >= len(exp) - 1
+    a, b = R("uniqueness", 0.0, shares=sh_a), R("uniqueness", 0.0, shares=sh_b)
+    assert a.rid == b.rid == 0
+    assert serving.Server(eng, reserve_bytes=0).run([a])[0].result == alone_a
+    assert serving.Server(eng, reserve_bytes=0).run([b])[0].result == alone_b
+    a2, b2 = R("uniqueness", 0.0, shares=sh_a), R("uniqueness", 0.01, shares=sh_b)
+    out = serving.Server(eng, reserve_bytes=0).run([a2, b2])
+    assert out[0].result == alone_a and out[1].result == alone_b

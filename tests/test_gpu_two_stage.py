@@ -407,3 +407,75 @@ def test_multi_chunk_step_equals_single_steps(irismpc):
     assert np.array_equal(run([(0, 4)]), single)
     assert np.array_equal(run([(0, 2), (2 * C, 4)]), single)  # the second step stops at the DB's end
     assert np.array_equal(run([(0, 8)]), single)  # a lone query's step size, past the DB's end
+
+
+def _flip_first(code, n):
+    """IrisCodeArray::flip_bit(i) for i in 0..n (bit i = bit i % 64 of word i / 64)."""
+    out = np.array(code, np.uint64).copy()
+    full, rest = divmod(n, 64)
+    out[:full] ^= np.uint64(0xFFFFFFFFFFFFFFFF)
+    if rest:
+        out[full] ^= np.uint64((1 << rest) - 1)
+    return out
+
+
+def test_close_to_threshold_two_stage(irismpc, orc):
+    """The reference's CloseToThreshold / flipped-code cases
+    (iris-mpc-common/src/test.rs:786-815, 1181-1220): a full-mask DB iris with
+    its first THRESHOLD_ABSOLUTE + v code bits flipped (THRESHOLD_ABSOLUTE =
+    4,416 = 0.345 x 12,800, v in -99..99) in each eye, through the share GEMM,
+    the circuits and the two-stage flow; plus left eyes at the 0.375 boundary
+    (4,800 + v flips), which decide the candidate list. With full masks the
+    integer predicate flips exactly between v = 0 and v = 1; the plaintext
+    brute force must agree with that closed form, and every party's merged
+    results, counters and candidates with both."""
+    from irismpc import synth
+    from irismpc.two_stage import TwoStageMatcher
+
+    N, B = 1024, 32
+    Qr = B * 31
+    T345, T375 = 12800 * 345 // 1000, 12800 * 375 // 1000
+    rng = np.random.default_rng(1181)
+    ones = np.full(200, np.uint64(0xFFFFFFFFFFFFFFFF))
+    db = {e: (np.array([orc.synth_iris(ds, i)[0] for i in range(N)]), np.tile(ones, (N, 1)))
+          for e, ds in (("left", 5), ("right", 6))}
+    edge = [(0, 0), (-1, -1), (1, 0), (0, 1), (1, 1), (-1, 1), (-99, -99), (99, -99), (-99, 99),
+            (-50, -3), (3, -50), (0, -1)]
+    spec = [(T345 + a, T345 + b) for a, b in edge]
+    spec += [(T375 + v, T345 - 20) for v in (0, -1, 1, 2)]  # 0.375 boundary in the full-scan eye
+    spec += [(T345 + int(a), T345 + int(b)) for a, b in rng.integers(-99, 100, (B - len(spec), 2))]
+    t = rng.choice(N, size=B, replace=False)
+    q = {e: (np.array([_flip_first(db[e][0][t[j]], spec[j][k]) for j in range(B)]), np.tile(ones, (B, 1)))
+         for k, e in enumerate(("left", "right"))}
+
+    eyes, queries, keep = {}, {}, []
+    for k, e in enumerate(("left", "right")):
+        ex, parties = _engines(irismpc, N, Qr, tag_base=20 + 3 * k)
+        keep.append(ex)
+        db_sh = synth.shares_of(db[e][0], db[e][1], 70 + k)
+        q_sh = synth.shares_of(q[e][0], q[e][1], 80 + k)
+        for p in range(3):
+            parties[p].code_db.load_full_db(db_sh[p][0])
+            parties[p].mask_db.load_full_db(db_sh[p][1])
+        eyes[e] = parties
+        queries[e] = ([parties[p].codes.upload_query_shares(q_sh[p][0]) for p in range(3)],
+                      [parties[p].masks.upload_query_shares(q_sh[p][1]) for p in range(3)])
+    res = TwoStageMatcher(eyes, N, B, full_scan_side="left").match(queries, N, B)
+
+    # plaintext brute force at both thresholds, and the closed form it must agree with
+    plain = {(e, A): np.array([orc.plain_match(q[e][0][j], q[e][1][j], *db[e], A) for j in range(B)])
+             for e in ("left", "right") for A in (A_0375, irismpc.THRESHOLD_A)}
+    for j, (fl, fr) in enumerate(spec):
+        assert plain["left", A_0375][j, t[j]] == (fl <= T375)
+        assert plain["left", irismpc.THRESHOLD_A][j, t[j]] == (fl <= T345)
+        assert plain["right", irismpc.THRESHOLD_A][j, t[j]] == (fr <= T345)
+    cand = np.flatnonzero(plain["left", A_0375].any(axis=0)).astype(np.uint32)
+    assert np.array_equal(res.candidates, cand)
+    cmask = np.zeros(N, bool)
+    cmask[cand] = True
+    merged = plain["left", irismpc.THRESHOLD_A] & plain["right", irismpc.THRESHOLD_A] & cmask[None, :]
+    want = np.array([np.flatnonzero(r).min() if r.any() else 0xFFFFFFFF for r in merged], np.uint32)
+    assert sum(w != 0xFFFFFFFF for w in want) >= 4 and sum(w == 0xFFFFFFFF for w in want) >= 8
+    for p in range(3):
+        assert np.array_equal(res.final_results[p], want), f"party {p} final results"
+        assert np.array_equal(res.match_counters[p], merged.sum(axis=1)), f"party {p} match counters"

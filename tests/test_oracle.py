@@ -192,6 +192,32 @@ def test_random_codes_fhd_golden(orc):
     assert best is not None
 
 
+def test_plain_match_golden_and_brute_force(orc):
+    """orc.plain_match (one query, 31 rotations, integer MPC predicate) on the
+    reference's random_codes.txt pair: its best rotation is at FHD
+    0.48108602806589384 and rotation 0 at 0.513, so the query matches at a
+    0.49 threshold and not at 0.48 (a rotation left out or the unrotated
+    query alone would miss the first); and against a numpy restatement on
+    synthetic irises with a planted copy."""
+    lines = _lines("random_codes.txt")
+    c1, m1, c2, m2 = (orc.iris_from_base64(s) for s in lines[:4])
+    A = lambda f: int((1.0 - 2.0 * f) * (1 << 16))  # noqa: E731 - threshold constant of dot/mod.rs:10-15
+    assert orc.plain_match(c2, m2, c1[None], m1[None], A(0.49))[0]
+    assert not orc.plain_match(c2, m2, c1[None], m1[None], A(0.48))[0]
+    n = 96
+    dc, dm = map(np.array, zip(*[orc.synth_iris(7, i) for i in range(n)]))
+    q, qm = orc.synth_flip(dc[40], 8, 0, 3000), dm[40]
+    rc, rm = orc.iris_all_rotations(q), orc.iris_all_rotations(qm)
+    exp = np.zeros(n, bool)
+    for t in range(31):
+        mm = rm[t][None, :] & dm
+        M = np.bitwise_count(mm).sum(axis=1).astype(np.int64)
+        H = np.bitwise_count((rc[t][None, :] ^ dc) & mm).sum(axis=1).astype(np.int64)
+        exp |= ((((M * 16384) - (((M - 2 * H) & 0xFFFF) << 16) - 1) & 0xFFFFFFFF) >> 31).astype(bool)
+    got = orc.plain_match(q, qm, dc, dm, 16384)
+    assert np.array_equal(got, exp) and got[40] and got.sum() == 1
+
+
 # --------------------------------------------------------------- share encoding
 def _shares_of(orc, code, mask, seed):
     rng = np.random.default_rng(seed)
