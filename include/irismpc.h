@@ -347,6 +347,20 @@ int irismpc_comparator_set_anon_capture(irismpc_comparator* d, uint16_t* code_a,
                                         uint32_t* counter, size_t capacity, uint64_t batch_id,
                                         size_t max_query_length, size_t max_db_length,
                                         const uint64_t* targets, size_t n_targets);
+/* The candidate list of irismpc_comparator_get_partial_results computed
+ * without allocating device memory and without synchronising: no counterpart
+ * in the reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops). On `stream`: the ascending list, without
+ * repeats, of DB ids d < db_size set in any of the batch_size bitmap rows
+ * (row_bits bits apart; 0 = db_size) or listed in extra_ids (device u32,
+ * n_extra; ids >= db_size skipped) goes to ids (device, room for db_size
+ * ids) and its length to *count (device u32). Its scratch is a device array
+ * of the library, 16 MiB per GPU for db_size up to 2^27 rows (134,217,728);
+ * a larger db_size fails before anything runs. Calls on one GPU share that
+ * array: calls that overlap in time on one GPU race. */
+int irismpc_partial_results_noalloc(const uint64_t* bitmap, size_t db_size, size_t batch_size,
+                                    size_t row_bits, const uint32_t* extra_ids, size_t n_extra,
+                                    uint32_t* ids, uint32_t* count, void* stream);
 /* Device pointer of the last get_partial_results list. */
 int irismpc_comparator_partial_ids(irismpc_comparator* d, const uint32_t** ids);
 /* open_results_with_index_mapping (distance_comparator.rs:281-342,

@@ -582,6 +582,14 @@ int irismpc_comparator_set_anon_capture(irismpc_comparator* d, uint16_t* code_a,
     d->impl->set_anon_capture(c, targets, n_targets);
   });
 }
+int irismpc_partial_results_noalloc(const uint64_t* bitmap, size_t db_size, size_t batch_size,
+                                    size_t row_bits, const uint32_t* extra_ids, size_t n_extra,
+                                    uint32_t* ids, uint32_t* count, void* stream) {
+  return guarded([&] {
+    partial_results_noalloc(bitmap, db_size, batch_size, row_bits, extra_ids, n_extra, ids, count,
+                            st(stream));
+  });
+}
 int irismpc_comparator_partial_ids(irismpc_comparator* d, const uint32_t** ids) {
   return guarded([&] { *ids = d->impl->partial_ids(); });
 }

@@ -8,6 +8,9 @@
 // its set bits.
 #include "comparator.hpp"
 
+#include <mutex>
+#include <string>
+
 namespace irismpc {
 
 
@@ -258,6 +261,58 @@ void launch_candidates(const CandidateArgs& a, hipStream_t s) {
     cand_emit_kernel<<<(unsigned)n_tiles, 256, 0, s>>>(a);
     IRIS_HIP_CHECK(hipGetLastError());
   }
+}
+
+// ------------------------------------------------ candidates without allocation
+// Scratch of partial_results_noalloc: the per-word candidate bitmap (one u64
+// per 64 DB ids) and the per-tile offsets (one u32 per 16,384 ids) of
+// launch_candidates, for DB sizes up to kNoAllocMaxDb. Each GPU holds its
+// own copy until the process ends. Every call rewrites the words it reads
+// (cand_union_kernel writes every candidate word before anything reads it,
+// cand_count_kernel every tile), so no call depends on an earlier one's
+// contents; calls that overlap in time on one GPU share the array and race.
+constexpr size_t kNoAllocCandBytes = kNoAllocMaxDb / 64 * sizeof(uint64_t);
+constexpr size_t kNoAllocTileBytes = kNoAllocMaxDb / (64 * kCandTileWords) * sizeof(uint32_t);
+static __device__ __align__(256) uint8_t scratch[kNoAllocCandBytes + kNoAllocTileBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+}  // namespace
+
+// Device address of this device's `scratch` (looked up once per device: the
+// array's name in host code is only a stand-in, not a device address).
+static uint8_t* scratch_ptr() {
+  int dev = 0;
+  IRIS_HIP_CHECK(hipGetDevice(&dev));
+  IRIS_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    IRIS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    IRIS_REQUIRE(p != nullptr, "no device address for the candidate scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  return static_cast<uint8_t*>(g_scratch_ptr[dev]);
+}
+
+void partial_results_noalloc(const uint64_t* bitmap, size_t db_size, size_t batch, size_t row_bits,
+                             const uint32_t* extra, size_t n_extra, uint32_t* ids, uint32_t* count,
+                             hipStream_t s) {
+  IRIS_REQUIRE(db_size <= kNoAllocMaxDb,
+               "db_size beyond the candidate scratch (" + std::to_string(kNoAllocMaxDb) + " rows)");
+  if (row_bits == 0) row_bits = db_size;
+  IRIS_REQUIRE(row_bits >= db_size, "bitmap rows shorter than the DB");
+  IRIS_REQUIRE(ids && count && (bitmap || batch == 0 || db_size == 0) && (extra || n_extra == 0),
+               "NULL device pointer");
+  uint8_t* base = scratch_ptr();
+  CandidateArgs c{};
+  c.bitmap = bitmap; c.row_bits = row_bits; c.batch = batch; c.db_size = db_size;
+  c.extra = extra; c.n_extra = n_extra;
+  c.cand = reinterpret_cast<uint64_t*>(base);
+  c.tiles = reinterpret_cast<uint32_t*>(base + kNoAllocCandBytes);
+  c.ids = ids; c.count = count;
+  launch_candidates(c, s);
 }
 
 void launch_open_mapped(const MappedOpenArgs& a, hipStream_t s) {

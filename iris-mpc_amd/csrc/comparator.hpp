@@ -54,6 +54,15 @@ struct CandidateArgs {
   uint32_t* count;
 };
 void launch_candidates(const CandidateArgs& a, hipStream_t s);
+// The candidate list of launch_candidates with its scratch in a file-scope
+// device array (no allocation, no synchronisation): extra (device, n_extra)
+// ids join the bitmap's; ids (device) receive the list, *count (device) its
+// length. db_size <= kNoAllocMaxDb.
+constexpr size_t kNoAllocMaxDb = size_t(1) << 27;
+constexpr int kMaxDevices = 64;
+void partial_results_noalloc(const uint64_t* bitmap, size_t db_size, size_t batch, size_t row_bits,
+                             const uint32_t* extra, size_t n_extra, uint32_t* ids, uint32_t* count,
+                             hipStream_t s);
 
 // openResultsWithIndexMapping (dot/kernel.cu:118-151): opened bit (q, j) of a
 // subset chunk sets bitmap bit total_db_len * slot + index_map[j] and records

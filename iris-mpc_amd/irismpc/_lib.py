@@ -82,6 +82,7 @@ SIGNATURES = {
     "irismpc_share_db_get_counter": [vp, vp],
     "irismpc_share_db_set_counter": [vp, u64],
     "irismpc_share_db_set_variant": [vp, i32],
+    "irismpc_partial_results_noalloc": [vp, sz, sz, sz, vp, sz, vp, vp, vp],
     "irismpc_share_db_set_timing": [vp, i32],
     "irismpc_share_db_launch_times": [vp, vp, vp],
     "irismpc_circuits_set_timing": [vp, i32],

@@ -504,6 +504,15 @@ class DistanceComparator:
         return [o.value for o in outs]
 
 
+def partial_results_noalloc(bitmap_ptr, db_size, batch_size, ids_ptr, count_ptr, extra_ptr=None, n_extra=0,
+                            row_bits=0, stream=0):
+    """The candidate list of DistanceComparator.get_partial_results on the
+    device without allocating or synchronising (irismpc_partial_results_noalloc):
+    ids_ptr (device u32, room for db_size ids) and count_ptr (device u32)."""
+    call("irismpc_partial_results_noalloc", bitmap_ptr, db_size, batch_size, row_bits, extra_ptr, n_extra,
+         ids_ptr, count_ptr, stream)
+
+
 class PartyEngines:
     """One party's engines for a DB scan (the ServerActor's codes/masks engines,
     phase-2 Circuits and DistanceComparator, with HBM-resident DB shares)."""

@@ -59,3 +59,14 @@ def test_error_reporting_without_gpu():
     assert lib.irismpc_gemm_variants(ctypes.byref(n)) == 0 and n.value > 0
     first, cnt = ctypes.c_int(), ctypes.c_int()
     assert lib.irismpc_gemm_rot_variants(ctypes.byref(first), ctypes.byref(cnt)) == 0 and cnt.value >= 1
+
+
+def test_noalloc_candidates_reject_oversize_db_before_any_launch():
+    """irismpc_partial_results_noalloc refuses a DB beyond its scratch (2^27
+    rows) with an error status and text, before touching a device."""
+    if not os.path.exists(LIB):
+        pytest.skip("libirismpc.so not built")
+    lib = _load()
+    rc = lib.irismpc_partial_results_noalloc(None, (1 << 27) + 1, 1, 0, None, 0, None, None, None)
+    assert rc != 0
+    assert b"scratch" in lib.irismpc_last_error()
