@@ -533,95 +533,105 @@ def run_c4(args):
 
 
 def run_c5(args):
-    """Serving form: Poisson arrivals at 5 fixed rates, 85 % uniqueness (vs the
-    whole DB), 10 % reauth (vs one row), 5 % insertions / deletions, over the
-    three parties' DB shares of args.db rows (one eye); continuous batching of
-    the scan, admission by free HBM. The plain form serves one request at a
-    time (a full scan per uniqueness request)."""
+    """Serving form against the plain per-batch form: Poisson arrivals at
+    fixed rates, 85 % uniqueness (vs the whole DB), 10 % reauth (vs one row),
+    5 % insertions / deletions, over the three parties' DB shares of args.db
+    rows (one eye). The serving form batches continuously at chunk-step
+    granularity; the plain per-batch form runs the reference's batch after
+    batch, every batch padded to max_batch x 31 query rows (64, the batch of
+    the full-protocol workload, and 8, the reference's deployment setting).
+    Each rate serves the same request sequence in every form; the line reports
+    p50 / p99 per form and rate and, per p99 bound, the largest tested rate
+    each form sustains."""
+    import dataclasses
+
     import numpy as np
     import torch
 
     import irismpc
-    from irismpc import serving, synth
+    from irismpc import serving
+    from irismpc.batching import Autotuner
 
     torch.cuda.set_device(0)
     N, U = args.db, min(args.db, 32768)
     rates = [float(r) for r in args.rates.split(",")]
+    progress(f"building 3 x {N:,} DB shares")
     eng = serving.ServingEngines(N, N + 4 * irismpc.DB_CHUNK_SIZE, max_active=64)
     eng.build(U, 81, 82)
     rng = np.random.default_rng(11)
-    counter = [0]
-
-    def shares(ds, t, flips):
-        c, m = synth.synth_iris(ds, t)
-        if flips:
-            c = synth.synth_flip(c, 85, counter[0], flips)
-        sh = synth.shares_of(c[None], m[None], 86, counter[0])
-        counter[0] += 1
-        return [(sh[p][0][0], sh[p][1][0]) for p in range(3)]
-
-    def uniq(k):
-        if rng.random() < 0.25:  # a returning user: a copy of a DB iris with 10-30 % flips
-            return serving.Request("uniqueness", 0, shares=shares(81, int(rng.integers(U)), int(rng.integers(1280, 3841))))
-        return serving.Request("uniqueness", 0, shares=shares(83, counter[0], 0))
-
-    def reauth(k):
-        t = int(rng.integers(U))
-        return serving.Request("reauth", 0, shares=shares(81, t, 1500), target=t)
-
-    def insert(k):
-        return serving.Request("insert", 0, shares=shares(84, counter[0], 0))
-
-    def delete(k):
-        return serving.Request("delete", 0, target=int(rng.integers(N)))
-
-    # chunks per step per active-query count, timed on this device
-    from irismpc.batching import Autotuner
-
+    uniq, reauth, insert, delete = _serving_requests(rng, U, N, 81, 82, [0])
+    progress("tuning step sizes")
     tuner = Autotuner()
     step_table = eng.tune_steps(tuner)
 
-    # plain form: one uniqueness request at a time, a full scan each
+    # one uniqueness request at a time, a full scan each
     alone = [uniq(k).shares for k in range(args.plain_requests)]
     eng.run_alone(alone[0])  # warm-up
     t0 = time.perf_counter()
-    plain_results = [eng.run_alone(sh) for sh in alone]
-    t_plain = (time.perf_counter() - t0) / len(alone)
-    plain_cps = 31 * eng.n_rows / t_plain
+    for sh in alone:
+        eng.run_alone(sh)
+    t_one = (time.perf_counter() - t0) / len(alone)
 
+    def comparisons(out):
+        return sum(31 * r.db_rows for r in out if r.kind == "uniqueness") + sum(31 for r in out if r.kind == "reauth")
+
+    def serve(form, reqs):
+        reqs = [dataclasses.replace(r, covered=set()) for r in reqs]
+        if form == "serving":
+            srv = serving.Server(eng)
+        else:
+            srv = serving.PerBatchServer(eng, int(form.split("_")[-1]))
+        out = srv.run(reqs)
+        span = max(r.done for r in out) - min(r.arrival for r in out)
+        entry = {"comparisons_per_s": comparisons(out) / span, "requests_per_s": len(out) / span,
+                 "all": serving.latency_stats(out), "uniqueness": serving.latency_stats(out, "uniqueness"),
+                 "reauth": serving.latency_stats(out, "reauth")}
+        if form == "serving":
+            entry.update(max_concurrent=srv.max_concurrent, chunk_steps=srv.steps)
+        else:
+            entry.update(batches=srv.batches)
+        return entry, out
+
+    forms = ["serving", "per_batch_64", "per_batch_8"]
     per_rate = []
     for rate in rates:
         n = max(16, int(rate * args.seconds))
         reqs = serving.poisson_mix(rate, n, rng, uniq, reauth, insert, delete)
-        srv = serving.Server(eng)
-        out = srv.run(reqs)
-        span = max(r.done for r in out) - min(r.arrival for r in out)
-        comps = sum(31 * r.db_rows for r in out if r.kind == "uniqueness") + \
-            sum(31 for r in out if r.kind == "reauth")
-        entry = {"rate_per_s": rate, "requests": n, "comparisons_per_s": comps / span,
-                 "all": serving.latency_stats(out), "uniqueness": serving.latency_stats(out, "uniqueness"),
-                 "reauth": serving.latency_stats(out, "reauth"), "chunk_steps": srv.steps,
-                 "max_concurrent": srv.max_concurrent, "admission_waits": srv.admission_waits}
+        entry = {"rate_per_s": rate, "requests": n}
+        for form in forms:
+            progress(f"{form} at {rate:g} requests/s")
+            entry[form], _ = serve(form, reqs)
         per_rate.append(entry)
     # capacity: a burst of 3 x 64 uniqueness requests at once (offered load above capacity)
     burst = [uniq(k) for k in range(3 * eng.max_active)]
     for k, r in enumerate(burst):
         r.arrival, r.rid = 0.0, k
-    srv = serving.Server(eng)
-    out = srv.run(burst)
-    span = max(r.done for r in out)
-    capacity = {"requests": len(burst), "seconds": span,
-                "comparisons_per_s": sum(31 * r.db_rows for r in out) / span,
-                "requests_per_s": len(burst) / span, "chunk_steps": srv.steps,
-                "max_concurrent": srv.max_concurrent, "latency": serving.latency_stats(out)}
-    # the same requests alone vs in the mix: re-run the first plain queries through the loop
+    capacity = {}
+    for form in forms:
+        progress(f"{form}: capacity burst")
+        e, _ = serve(form, burst)
+        capacity[form] = e
+    # each request's matches are the same in the mix as alone (the plain form), on the DB as it is now
     mixed = [serving.Request("uniqueness", 0.001 * k, shares=sh, rid=k) for k, sh in enumerate(alone)]
-    srv = serving.Server(eng)
-    same = [r.result for r in sorted(srv.run(mixed), key=lambda r: r.rid)] == plain_results
-    top = per_rate[-1]
+    out = sorted(serving.Server(eng).run(mixed), key=lambda r: r.rid)
+    same = [r.result for r in out] == [eng.run_alone(sh) for sh in alone]
+
+    bounds = [100.0, 150.0, 250.0, 500.0, 1000.0, 2000.0]
+    sustained = {}
+    for b in bounds:
+        row = {}
+        for form in forms:
+            ok = [e for e in per_rate if e[form]["all"]["p99_ms"] <= b]
+            best = max(ok, key=lambda e: e["rate_per_s"]) if ok else None
+            row[form] = {"rate_per_s": best["rate_per_s"] if best else 0.0,
+                         "comparisons_per_s": best[form]["comparisons_per_s"] if best else 0.0}
+        sustained[f"p99<={b:g}ms"] = row
+    headline_bound = "p99<=250ms"
+    h = sustained[headline_bound]
+    top = per_rate[-1]["serving"]
     return {
         "metric": "3-party iris comparisons/sec on 1 MI355X + p50/p99 request latency",
-        "value": capacity["comparisons_per_s"],
+        "value": h["serving"]["comparisons_per_s"],
         "unit": "comparisons/s",
         "n_gpus": 1, "steps": len(rates), "warmup": 1, "ms_per_step": None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -630,15 +640,19 @@ def run_c5(args):
                 "requests are planted copies with 10-30 % flips",
         "config": {"workload": f"serving form, full 3-party protocol, one eye: DB {N:,} rows per party "
                                "share; Poisson arrivals, 85 % uniqueness / 10 % reauth / 5 % insert+delete; "
-                               "continuous batching over 32,768-row chunk steps, up to 64 active queries",
+                               "continuous batching over 32,768-row chunk steps, up to 64 active queries; "
+                               "value = comparisons/s at the largest tested rate whose p99 <= 250 ms",
                    "db_rows": N, "rates_per_s": rates, "seconds_per_rate": args.seconds,
                    "results_same_alone_and_in_mix": same},
-        "latency_p50_ms": top["all"].get("p50_ms"), "latency_p99_ms": top["all"].get("p99_ms"),
+        "latency_p50_ms": top["all"]["p50_ms"], "latency_p99_ms": top["all"]["p99_ms"],
         "per_rate": per_rate,
         "capacity": capacity,
-        "plain_form": {"comparisons_per_s": plain_cps, "ms_per_uniqueness_request": t_plain * 1e3,
-                       "requests": len(alone)},
-        "speedup_vs_plain": capacity["comparisons_per_s"] / plain_cps,
+        "sustained_under_p99_bound": sustained,
+        "speedup_vs_per_batch_at_p99_250ms": {
+            f: (h["serving"]["comparisons_per_s"] / h[f]["comparisons_per_s"]) if h[f]["comparisons_per_s"] else None
+            for f in ("per_batch_64", "per_batch_8")},
+        "one_request_at_a_time": {"ms_per_uniqueness_request": t_one * 1e3,
+                                  "comparisons_per_s": 31 * N / t_one, "requests": len(alone)},
         "autotune": {"chunks_per_step": {str(b): k for b, k in step_table.items()},
                      "log": {k: v for k, v in tuner.log}},
         "roofline": None,
@@ -794,8 +808,8 @@ def main():
                          "threshold circuits alone at one chunk; c3-two-stage: the reference's batch flow "
                          "(one-eye prefilter scan, both eyes on the candidates); matmul: the reference's "
                          "own GEMM benchmark shape (2.4M x 930 rows, code only, query upload included)")
-    ap.add_argument("--rates", default="2,5,10,20,40", help="c5: request rates per second")
-    ap.add_argument("--seconds", type=float, default=8.0, help="c5: arrival window per rate")
+    ap.add_argument("--rates", default="10,20,40,60,80", help="c5: request rates per second")
+    ap.add_argument("--seconds", type=float, default=6.0, help="c5: arrival window per rate")
     ap.add_argument("--plain-requests", type=int, default=8, help="c5: requests timed in the plain form")
     ap.add_argument("--latency-rate", type=float, default=40.0,
                     help="c3: Poisson request rate of the serving latency pass (0: no pass)")

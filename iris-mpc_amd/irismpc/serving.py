@@ -444,6 +444,84 @@ class Server:
         return reqs
 
 
+class PerBatchServer:
+    """The plain per-batch form of the reference's server (batch after batch,
+    actor.rs:584-644): the uniqueness and reauth requests that arrive while a
+    batch runs form the next batch (at most max_batch of them); every batch is
+    one scan of the whole DB whose query set is padded to max_batch x 31 rows,
+    as the reference sizes its queries and GEMMs by max_batch_size whatever
+    the batch holds (actor.rs:895-901, share_db.rs:800-845 with query_length
+    = max_batch_size x 31). A reauth is answered by its row of the batch's scan
+    at its target; insertions and deletions that arrived before a batch starts
+    are applied first. Same engines, DB and results as the serving form."""
+
+    def __init__(self, eng, max_batch):
+        assert 1 <= max_batch <= eng.max_active
+        self.eng, self.max_batch = eng, max_batch
+        self.batches = 0
+        self.failed = []
+
+    def run(self, requests):
+        eng, B = self.eng, self.max_batch
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        self.t0 = time.perf_counter()
+        if eng._dummy is None:
+            eng._dummy = dummy_shares()
+        qsets = None
+        i, pending = 0, []
+        while i < len(reqs) or pending:
+            now = time.perf_counter() - self.t0
+            while i < len(reqs) and reqs[i].arrival <= now:
+                pending.append(reqs[i])
+                i += 1
+            if not pending:
+                time.sleep(max(0.0, min(0.002, reqs[i].arrival - now)))
+                continue
+            batch = []
+            while pending and len(batch) < B:
+                r = pending.pop(0)
+                if r.kind in ("insert", "delete"):
+                    try:
+                        r.result = eng.insert(r.shares) if r.kind == "insert" else eng.delete(r.target)
+                    except ValueError as e:
+                        r.error = str(e)
+                        self.failed.append(r)
+                    r.db_rows = eng.n_rows
+                    r.done = time.perf_counter() - self.t0
+                else:
+                    batch.append(r)
+            if not batch:
+                continue
+            shares = [r.shares for r in batch] + [eng._dummy] * (B - len(batch))
+            if qsets is None:
+                qsets = eng._query_sets(eng.scan, shares, B * ROT)
+            qsets = eng._query_sets(eng.scan, shares, B * ROT, reuse=qsets)
+            rows = eng.n_rows
+            slots = list(range(len(batch)))
+            for s in slots:
+                eng.clear_slot(s)
+            n = eng.n_chunks(rows)
+            for c in range(n):  # the reference's chunk loop: one chunk per step
+                eng.step(qsets[0], qsets[1], len(batch), c, rows, slots, n_chunks=1)
+            self.batches += 1
+            for s, r in enumerate(batch):
+                matches = eng.slot_matches(s, rows)
+                if r.kind == "reauth":
+                    if 0 <= r.target < rows:
+                        r.result = r.target in set(matches)
+                    else:
+                        r.error = f"reauth target {r.target} outside the DB ({rows} rows)"
+                        self.failed.append(r)
+                else:
+                    r.result = matches
+                r.db_rows = rows
+                r.done = time.perf_counter() - self.t0
+        if self.failed:
+            r = self.failed[0]
+            raise RuntimeError(f"{len(self.failed)} request(s) failed; first: {r.kind} rid {r.rid}: {r.error}")
+        return reqs
+
+
 def poisson_mix(rate, n, rng, make_uniqueness, make_reauth, make_insert, make_delete,
                 mix=(0.85, 0.10, 0.05)):
     """n requests with exponential inter-arrival times at `rate` per second and
