@@ -162,6 +162,15 @@ int irismpc_query_upload_shares_rows(irismpc_share_db* e, const uint16_t* shares
  * active query set changes between chunk steps. */
 int irismpc_query_refill_shares(irismpc_share_db* e, irismpc_query* q, const uint16_t* shares,
                                 size_t batch, size_t rows, void* stream);
+/* irismpc_query_refill_shares without synchronising: the refill is ordered on
+ * `stream` after the work already queued there (which may still read the
+ * set) and before the work queued later, so a request loop changes its query
+ * set between queued chunk steps without draining them. Every asynchronous
+ * refill of one engine must use one stream (its staging buffers are reused in
+ * stream order), and `shares` (pinned host memory) must stay unchanged until
+ * the stream has run the copy. No reference counterpart. */
+int irismpc_query_refill_shares_async(irismpc_share_db* e, irismpc_query* q, const uint16_t* shares,
+                                      size_t batch, size_t rows, void* stream);
 /* Queries of the Mirror orientation (full-face mirror attack pass,
  * actor.rs:591-615; server/mod.rs:156-176): the query rows come from the
  * mirrored shares of `shares` (the other eye's raw shares; mirrored_code for

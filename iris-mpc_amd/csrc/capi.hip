@@ -311,6 +311,10 @@ int irismpc_query_refill_shares(irismpc_share_db* e, irismpc_query* q, const uin
                                 size_t batch, size_t rows, void* stream) {
   return guarded([&] { e->impl->refill_query_shares(*q->impl, shares, batch, rows, st(stream)); });
 }
+int irismpc_query_refill_shares_async(irismpc_share_db* e, irismpc_query* q, const uint16_t* shares,
+                                      size_t batch, size_t rows, void* stream) {
+  return guarded([&] { e->impl->refill_query_shares_async(*q->impl, shares, batch, rows, st(stream)); });
+}
 int irismpc_query_upload_shares_mirrored(irismpc_share_db* e, const uint16_t* shares,
                                          const uint16_t* insert_shares, size_t batch,
                                          void* stream, irismpc_query** out) {

@@ -395,6 +395,12 @@ std::unique_ptr<QuerySet> ShareDB::upload_shares_impl(const uint16_t* host_share
 
 void ShareDB::refill_query_shares(QuerySet& q, const uint16_t* host_shares, size_t batch,
                                   size_t n_rows, hipStream_t s) const {
+  refill_query_shares_async(q, host_shares, batch, n_rows, s);
+  IRIS_HIP_CHECK(hipStreamSynchronize(s));  // the staging buffers are reused by the next refill
+}
+
+void ShareDB::refill_query_shares_async(QuerySet& q, const uint16_t* host_shares, size_t batch,
+                                        size_t n_rows, hipStream_t s) const {
   DeviceGuard g(device_);
   if (n_rows == 0) n_rows = query_length_;
   IRIS_REQUIRE(q.code_length == code_length_, "query set of another code length");
@@ -412,7 +418,6 @@ void ShareDB::refill_query_shares(QuerySet& q, const uint16_t* host_shares, size
                    q.insert->sum1.ptr, s);
   q.rows = (int)n_rows;
   q.batch = (int)batch;
-  IRIS_HIP_CHECK(hipStreamSynchronize(s));  // the staging buffers are reused by the next refill
 }
 
 void ShareDB::launch(const GemmArgs& a, int variant, hipStream_t s) {

@@ -199,6 +199,14 @@ class ShareDB {
   // synchronisation. n_rows and batch must fit what the set was made with.
   void refill_query_shares(QuerySet& q, const uint16_t* host_shares, size_t batch, size_t n_rows,
                            hipStream_t s) const;
+  // The same without synchronising: stream-ordered after the work already
+  // queued on `s` (which may still read the set) and before work queued
+  // later. Every asynchronous refill of this engine must go on one stream
+  // (the staging buffers are reused in stream order), and host_shares
+  // (pinned host memory) must stay unchanged until the stream has run the
+  // copy.
+  void refill_query_shares_async(QuerySet& q, const uint16_t* host_shares, size_t batch,
+                                 size_t n_rows, hipStream_t s) const;
   // dot + dot_reduce_and_multiply (fused): DB rows [offset, offset + chunk)
   // against all query rows; chunk is the padded chunk size (multiple of 64).
   // results land at results() + out_offset (elements): several chunks of a

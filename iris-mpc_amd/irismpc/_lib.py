@@ -54,6 +54,7 @@ SIGNATURES = {
     "irismpc_db_replicate_records": [vp, vp, sz, sz, sz, vp],
     "irismpc_query_upload_shares_rows": [vp, vp, sz, sz, vp, vp],
     "irismpc_query_refill_shares": [vp, vp, vp, sz, sz, vp],
+    "irismpc_query_refill_shares_async": [vp, vp, vp, sz, sz, vp],
     "irismpc_query_upload_shares_mirrored": [vp, vp, vp, sz, vp, vp],
     "irismpc_share_db_dot_reduce_packed": [vp, vp, vp, sz, sz, sz, u16, vp, vp, i32, vp, vp],
     "irismpc_graph_capture_begin": [vp],

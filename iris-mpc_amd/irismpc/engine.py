@@ -56,6 +56,11 @@ def download(ptr, n, dtype, stream=0):
     return out
 
 
+def fill_u64(ptr, n, value=0, stream=0):
+    """n u64 words at ptr := value, by a kernel on `stream` (graph-capture safe)."""
+    call("irismpc_fill_u64", ptr, n, value, stream)
+
+
 def upload(ptr, arr, stream=0):
     a = np.ascontiguousarray(arr)
     call("irismpc_memcpy_htod", ptr, a.ctypes.data, a.nbytes, stream)
@@ -219,6 +224,13 @@ class ShareDB:
         reusing its device buffers (rows and batch within what it was made with)."""
         r = np.ascontiguousarray(shares, dtype=np.uint16).reshape(-1, self.code_length)
         call("irismpc_query_refill_shares", self.h, query.h, r.ctypes.data, r.shape[0], rows, stream)
+        return query
+
+    def refill_query_shares_async(self, query, shares_ptr, batch, rows, stream):
+        """refill_query_shares from pinned host memory at shares_ptr (batch x
+        code_length u16), ordered on `stream` without synchronising; every
+        asynchronous refill of this engine goes on that one stream."""
+        call("irismpc_query_refill_shares_async", self.h, query.h, shares_ptr, batch, rows, stream)
         return query
 
     def dot_reduce_packed(self, query, db, offset, total, chunk_size, multiplier, rows, out_ptrs,

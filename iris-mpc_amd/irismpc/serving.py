@@ -151,6 +151,8 @@ class ServingEngines:
         self._small_qsets = None  # the reauth engines' query sets, refilled per request
         self.step_table = None  # {active-query bucket: chunks per step} from tune_steps
         self._alone_qsets = None  # the plain form's query sets, refilled per request
+        self._readout = None  # pinned host copy of party 0's bitmap rows, one per slot
+        self._pinned, self._pinned_ev, self._pin_next = None, [None] * 3, 0  # refill staging ring
 
     def sync_scan(self):
         """Waits for every queued scan step."""
@@ -207,6 +209,37 @@ class ServingEngines:
                 qm.append(ps.masks[p].refill_query_shares(reuse[1][p], mask, rows))
         return qc, qm
 
+    def refill_scan(self, qsets, shares_list, rows):
+        """Refills the scan engines' query sets (made at full size) with
+        shares_list on the parties' scan streams: ordered after the queued
+        steps that still read the old sets and before the steps queued later,
+        with no synchronisation. The shares pass through pinned host buffers,
+        a ring of three, each reused once the refill that read it has run."""
+        torch = self.torch
+        if self._pinned is None:
+            self._pinned = [[[torch.empty((self.max_active, K), dtype=torch.int16, pin_memory=True)
+                              for K in (E.IRIS_CODE_LENGTH, E.MASK_CODE_LENGTH)] for _ in range(3)]
+                            for _ in range(3)]
+        k = self._pin_next
+        self._pin_next = (k + 1) % 3
+        for ev in self._pinned_ev[k] or ():
+            ev.synchronize()
+        n, evs = len(shares_list), []
+        for p in range(3):
+            bc = self._pinned[k][p][0].numpy().view(np.uint16)
+            bm = self._pinned[k][p][1].numpy().view(np.uint16)
+            for j, sh in enumerate(shares_list):
+                bc[j] = sh[p][0]
+                bm[j] = sh[p][1]
+            st = self.scan.streams[p]
+            self.scan.codes[p].refill_query_shares_async(qsets[0][p], bc.ctypes.data, n, rows, st)
+            self.scan.masks[p].refill_query_shares_async(qsets[1][p], bm.ctypes.data, n, rows, st)
+            ev = torch.cuda.Event()
+            ev.record(self._xstreams[p])
+            evs.append(ev)
+        self._pinned_ev[k] = evs
+        return qsets
+
     def n_chunks(self, rows):
         return max(1, -(-rows // CHUNK))
 
@@ -258,10 +291,12 @@ class ServingEngines:
         the steps that read them."""
         slots = list(slots)
         if slots != self._slots:
-            self.sync_scan()
+            # each party's slot map is rewritten on its scan stream: after the
+            # queued steps that read the old map, before the step below
+            host = self.torch.tensor(slots, dtype=self.torch.int32).pin_memory()
             for p in range(3):
-                self.slot_maps[p][:len(slots)].copy_(self.torch.tensor(slots, dtype=self.torch.int32))
-            self.torch.cuda.synchronize()  # the copies (and any slot clearing) precede the step
+                with self.torch.cuda.stream(self._xstreams[p]):
+                    self.slot_maps[p][:len(slots)].copy_(host, non_blocking=True)
             self._slots = slots
         while len(self._inflight) >= max_queued:
             for ev in self._inflight.popleft():
@@ -277,11 +312,31 @@ class ServingEngines:
         self._inflight.append(evs)
 
     def clear_slot(self, slot):
-        """Zeroes a free slot's bitmap rows (no queued step writes them); the
-        fill runs on torch's stream, so it is waited for before any scan step."""
-        for b in self.bitmaps:
-            b[slot * self.words:(slot + 1) * self.words].zero_()
-        self.torch.cuda.synchronize()
+        """Zeroes a slot's bitmap rows with fill kernels on the scan streams:
+        after the queued steps (and readouts) of the slot's previous request,
+        before every step queued later; nothing waits."""
+        for b, s in zip(self.bitmaps, self.scan.streams):
+            E.fill_u64(b.data_ptr() + 8 * slot * self.words, self.words, 0, s)
+
+    def begin_readout(self, slot):
+        """Queues the copy of party 0's bitmap row of `slot` into pinned host
+        memory on party 0's scan stream, after the steps queued so far, and
+        returns the event that marks its completion (nothing waits)."""
+        if self._readout is None:
+            self._readout = self.torch.empty((self.max_active, self.words), dtype=self.torch.int64,
+                                             pin_memory=True)
+        xs = self._xstreams[0]
+        with self.torch.cuda.stream(xs):
+            self._readout[slot].copy_(self.bitmaps[0][slot * self.words:(slot + 1) * self.words],
+                                      non_blocking=True)
+            ev = self.torch.cuda.Event()
+            ev.record(xs)
+        return ev
+
+    def readout_rows(self, slot, db_rows):
+        """The rows below db_rows set in the slot's copied bitmap row (after its
+        begin_readout event completed)."""
+        return bitmap_rows(self._readout[slot].numpy(), db_rows)
 
     def slot_matches(self, slot, db_rows):
         self.sync_scan()
@@ -345,6 +400,16 @@ class Server:
         free, _ = self.eng.torch.cuda.mem_get_info()
         return free
 
+    def _collect(self, reading, free_slots):
+        """Finishes the requests whose bitmap readout has completed (in order)
+        and frees their slots."""
+        while reading and reading[0][1].query():
+            r, _ = reading.pop(0)
+            r.result = self.eng.readout_rows(r.slot, r.db_rows)
+            r.done = time.perf_counter() - self.t0
+            free_slots.append(r.slot)
+            self.done.append(r)
+
     def _reauth_worker(self):
         while True:
             r = self.reauth_q.get()
@@ -367,11 +432,12 @@ class Server:
         self.t0 = time.perf_counter()
         worker = threading.Thread(target=self._reauth_worker, daemon=True)
         worker.start()
-        waiting, active = [], []
+        waiting, active, reading = [], [], []
         free_slots = list(range(self.max_active))
         qsets, qsets_full, active_ids = None, None, None
         i, cursor = 0, 0
-        while i < len(reqs) or waiting or active:
+        while i < len(reqs) or waiting or active or reading:
+            self._collect(reading, free_slots)
             now = time.perf_counter() - self.t0
             while i < len(reqs) and reqs[i].arrival <= now:
                 r = reqs[i]
@@ -403,7 +469,9 @@ class Server:
                 active.append(r)
             self.max_concurrent = max(self.max_concurrent, len(active))
             if not active:
-                if i < len(reqs):
+                if reading and not waiting:
+                    reading[0][1].synchronize()  # nothing to scan: wait for the oldest readout
+                elif i < len(reqs):
                     time.sleep(max(0.0, min(0.002, reqs[i].arrival - (time.perf_counter() - self.t0))))
                 continue
             # the active set's identity: request objects, not their (caller-set) rids
@@ -414,8 +482,7 @@ class Server:
                     first = [r.shares for r in active]
                     pad = first + [first[0]] * (self.max_active - len(first))
                     qsets_full = eng._query_sets(eng.scan, pad, self.max_active * ROT)
-                qsets = eng._query_sets(eng.scan, [r.shares for r in active], len(active) * ROT,
-                                        reuse=qsets_full)
+                qsets = eng.refill_scan(qsets_full, [r.shares for r in active], len(active) * ROT)
                 active_ids = ids
             n_chunks = eng.n_chunks(eng.n_rows)
             cursor %= n_chunks
@@ -430,12 +497,10 @@ class Server:
             cursor += k
             finished = [r for r in active if len(r.covered) >= eng.n_chunks(r.db_rows)
                         and all(c in r.covered for c in range(eng.n_chunks(r.db_rows)))]
-            for r in finished:
-                r.result = [m for m in eng.slot_matches(r.slot, r.db_rows)]
-                r.done = time.perf_counter() - self.t0
-                free_slots.append(r.slot)
+            for r in finished:  # its bitmap row is copied out after its last step
+                reading.append((r, eng.begin_readout(r.slot)))
                 active.remove(r)
-                self.done.append(r)
+            self._collect(reading, free_slots)
         self.reauth_q.put(None)
         worker.join()
         if self.failed:

@@ -169,6 +169,36 @@ def test_query_refill_matches_fresh_upload(irismpc, orc, K, mult):
         eng.refill_query_shares(dq, _rand(rng, (9, K)), 9 * 31)
 
 
+def test_query_refill_async_is_stream_ordered(irismpc, orc):
+    """Asynchronous refills queued on one stream between GEMMs that read the
+    set: each GEMM sees the shares of the refill queued before it (from pinned
+    host memory, no synchronisation in between), equal to the oracle's dots."""
+    import torch
+
+    rng = np.random.default_rng(13)
+    K, N, chunk = 6400, 700, 768
+    raws = [_rand(rng, (b, K)) for b in (4, 2, 3)]
+    db = _rand(rng, (N, K))
+    eng = irismpc.ShareDB(1, chunk, 4 * 31, K, ([0] * 8, [0] * 8))
+    slices = eng.alloc_db(N)
+    slices.load_full_db(db)
+    dq = eng.upload_query_shares_rows(raws[0], 4 * 31)
+    stream = irismpc.stream_create()
+    pinned = [torch.empty(r.shape, dtype=torch.int16, pin_memory=True) for r in raws]
+    outs = [torch.zeros(4 * 31 * chunk + 64, dtype=torch.int16, device="cuda") for _ in raws]
+    torch.cuda.synchronize()
+    for raw, pin, out in zip(raws, pinned, outs):
+        pin.numpy().view(np.uint16)[:] = raw
+        eng.refill_query_shares_async(dq, pin.data_ptr(), raw.shape[0], raw.shape[0] * 31, stream)
+        eng.dot_reduce_range(dq, slices, 0, chunk, chunk, 2, out.data_ptr(), stream)
+    irismpc.stream_synchronize(stream)
+    for raw, out in zip(raws, outs):
+        qr = raw.shape[0] * 31
+        ref = (orc.u16_dot(_rotated_rows(orc, 1, raw, qr), db).astype(np.uint32) * 2).astype(np.uint16)
+        got = out[:qr * chunk].cpu().numpy().view(np.uint16).reshape(qr, chunk)
+        assert np.array_equal(got[:, :N], ref), f"batch of {raw.shape[0]}"
+
+
 def test_share_gemm_rotation_masked_range(irismpc, orc):
     """Masked multi-chunk launch with every rotation-aware kernel (4-query
     tile and the one- and two-query tiles): same shares as the generic kernel,
