@@ -4,13 +4,18 @@ in the Normal orientation, over the HBM-resident DB of both eyes.
 
 Per batch:
 - drop invalid entries (actor.rs:829-850);
-- deletions overwrite their rows with the dummy iris (apply_non_mpc_updates,
-  actor.rs:2632-2680);
+- deletions overwrite their rows with the dummy iris, then identity updates
+  write their new shares over their rows (apply_non_mpc_updates,
+  actor.rs:2632-2700); a batch of updates only stops there;
+- LUC look-back: the last luc_lookback_records DB rows join every request's
+  OR-rule ids except reauths and reset / recovery checks (generate_luc_records,
+  actor.rs:770-795,3627-3659);
 - two-stage matching with batch deduplication (the two_stage module);
 - result assembly: merged results, match ids (DB ids, and 0xFFFFFFFF - j for
   an earlier query j of the batch), partial per-eye matches with their
   rotations, uniqueness insertions (no match, under the supermatch threshold,
-  not skip_persistence), successful reauths (actor.rs:1300-1480;
+  not skip_persistence; reset and recovery checks are matched but never
+  inserted), successful reauths (actor.rs:1300-1480;
   get_merged_results, distribute_insertions, calculate_insertion_indices
   :3174-3303);
 - DB writes: insertions at the end of the DB, then successful reauths at their
@@ -37,6 +42,8 @@ NON_MATCH_ID = 0xFFFFFFFF                 # actor.rs:192
 SUPERMATCH_THRESHOLD = 4000               # actor.rs:117
 UNIQUENESS_MESSAGE_TYPE = "uniqueness"    # iris-mpc-common/src/helpers/smpc_request.rs:109
 REAUTH_MESSAGE_TYPE = "reauth"            # smpc_request.rs:110
+RESET_CHECK_MESSAGE_TYPE = "reset_check"  # smpc_request.rs:111
+RECOVERY_CHECK_MESSAGE_TYPE = "recovery_check"  # smpc_request.rs:112
 NORMAL, MIRROR = "normal", "mirror"       # Orientation (actor.rs:119-134)
 ALL_MATCHES_LEN = 256
 
@@ -90,6 +97,28 @@ def calculate_insertion_indices(merged_results, insertion_list, db_sizes, batch_
         c += 1
 
 
+def generate_luc_records(latest_luc_index, or_rule_indices, lookback_records, batch_size,
+                         skip_lookback_requests=()):
+    """actor.rs:3627-3659: the ids latest - lookback .. latest (saturating at 0)
+    join each request's OR-rule ids (then sorted, without repeats), except the
+    requests listed in skip_lookback_requests; with no OR-rule ids at all every
+    request gets the look-back ids alone; lookback_records = 0 changes
+    nothing."""
+    if lookback_records == 0:
+        return [list(x) for x in or_rule_indices]
+    start = max(0, latest_luc_index - lookback_records)
+    ids = list(range(start, latest_luc_index + 1))
+    if not or_rule_indices:
+        return [list(ids) for _ in range(batch_size)]
+    out = []
+    for idx, (or_ids, luc) in enumerate(zip(or_rule_indices, [ids] * batch_size)):
+        if idx in skip_lookback_requests:
+            out.append(list(or_ids))
+        else:
+            out.append(sorted(set(or_ids) | set(luc)))
+    return out
+
+
 @dataclass
 class BatchQuery:
     """The fields of BatchQuery (iris-mpc-common/src/job.rs:34-93) that the
@@ -105,6 +134,12 @@ class BatchQuery:
     deletion_requests_indices: list = field(default_factory=list)
     valid_entries: list = None
     full_face_mirror_attacks_detection_enabled: bool = False
+    luc_lookback_records: int = 0
+    # identity updates (job.rs:103-107,152-170): new shares of whole DB rows,
+    # per update (left, right), each per party (code [12800], mask [6400])
+    identity_update_request_ids: list = field(default_factory=list)
+    identity_update_indices: list = field(default_factory=list)
+    identity_update_shares: list = field(default_factory=list)
 
     def retain(self, keep):
         """BatchQuery::retain: only the entries at positions `keep`."""
@@ -116,7 +151,9 @@ class BatchQuery:
             [self.or_rule_indices[i] for i in k] if self.or_rule_indices else [],
             [self.skip_persistence[i] for i in k] if self.skip_persistence is not None else None,
             {r: t for r, t in self.reauth_target_indices.items() if r in ids},
-            list(self.deletion_requests_indices), None, self.full_face_mirror_attacks_detection_enabled)
+            list(self.deletion_requests_indices), None, self.full_face_mirror_attacks_detection_enabled,
+            self.luc_lookback_records, list(self.identity_update_request_ids),
+            list(self.identity_update_indices), list(self.identity_update_shares))
 
 
 @dataclass
@@ -149,6 +186,8 @@ class ServerJobResult:
     full_face_mirror_partial_match_counters_right: list = field(default_factory=list)
     one_sided_match_distances: dict = None   # eye -> per party OneSidedDistanceCache
     two_sided_match_distances: list = None   # per party TwoSidedDistanceCache
+    identity_update_request_ids: list = field(default_factory=list)
+    identity_update_indices: list = field(default_factory=list)
 
 
 def _dummy_shares():
@@ -263,6 +302,17 @@ class ServerActor:
                     self.eyes[eye][p].code_db.load_records(d, self._dummy[p][0])
                     self.eyes[eye][p].mask_db.load_records(d, self._dummy[p][1])
 
+    def _apply_identity_updates(self, batch):
+        """actor.rs:2681-2720: each update's new shares overwrite its DB row in
+        both eyes; rows beyond the DB are skipped."""
+        for idx, (left, right) in zip(batch.identity_update_indices, batch.identity_update_shares):
+            if idx >= self.current_db_size:
+                continue
+            for eye, sh in (("left", left), ("right", right)):
+                for p in range(3):
+                    self.eyes[eye][p].code_db.load_records(idx, np.asarray(sh[p][0]).reshape(1, -1))
+                    self.eyes[eye][p].mask_db.load_records(idx, np.asarray(sh[p][1]).reshape(1, -1))
+
     def _distances(self, batch, orientation, skip, targets):
         """One-sided caches per eye and party (load_additions_since,
         actor.rs:2196-2240,2555-2589) and their two-sided merge
@@ -320,12 +370,21 @@ class ServerActor:
         skip = list(batch.skip_persistence) if batch.skip_persistence is not None else [False] * B
         if previous_results is None:  # actor.rs:2637-2639: only the first pass of a job
             self._apply_deletions(list(batch.deletion_requests_indices))
-        if B == 0:
+            self._apply_identity_updates(batch)
+        if B == 0:  # non-MPC updates only (actor.rs:709-720)
             return ServerJobResult([], [], [], [], [], [], [], [], [], [], [], [], [],
-                                   list(batch.deletion_requests_indices), [], [], {})
+                                   list(batch.deletion_requests_indices), [], [], {},
+                                   identity_update_request_ids=list(batch.identity_update_request_ids),
+                                   identity_update_indices=list(batch.identity_update_indices))
 
         queries = self._queries(batch, orientation)
-        or_rule = batch.or_rule_indices if batch.or_rule_indices else None
+        or_rule = [list(x) for x in batch.or_rule_indices] if batch.or_rule_indices else []
+        if orientation == NORMAL and (or_rule or batch.luc_lookback_records > 0):
+            skip_luc = {i for i, k in enumerate(batch.request_types)
+                        if k in (REAUTH_MESSAGE_TYPE, RESET_CHECK_MESSAGE_TYPE, RECOVERY_CHECK_MESSAGE_TYPE)}
+            or_rule = generate_luc_records(max(self.current_db_size, 1) - 1, or_rule,
+                                           batch.luc_lookback_records, B, skip_luc)
+        or_rule = or_rule or None
         self.internal_batch_counter += 1  # actor.rs:860
         anon = None
         if self.anon_buffers is not None:
@@ -411,7 +470,9 @@ class ServerActor:
         res = ServerJobResult(merged, list(batch.request_ids), list(batch.request_types), matches,
                               mwsp, skip, filtered, pil, pir, ril, rir, pcl, pcr,
                               list(batch.deletion_requests_indices), matched_batch, reauth_ok,
-                              dict(batch.reauth_target_indices), r.candidates)
+                              dict(batch.reauth_target_indices), r.candidates,
+                              identity_update_request_ids=list(batch.identity_update_request_ids),
+                              identity_update_indices=list(batch.identity_update_indices))
         res.full_face_mirror_attack_detected = attack
         if anon is not None:
             res.one_sided_match_distances, res.two_sided_match_distances = self._distances(

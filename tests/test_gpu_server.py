@@ -250,3 +250,86 @@ def test_server_anon_stats(irismpc):
         assert key(20, 1) in r.one_sided_match_distances["left"][p].map
         left, right = r.two_sided_match_distances[p].map[key(40, 3)]
         assert len(left) == len(right) == 31  # every rotation of the reauth pair
+
+
+def test_server_checks_luc_and_identity_updates(irismpc, orc):
+    """Reset and recovery checks are matched like uniqueness requests but never
+    inserted; LUC look-back (luc_lookback_records = 20) adds the last DB rows
+    to the OR rule of uniqueness requests only, so a left-eye-only copy of row
+    990 matches while the same kind of reset check does not; an identity update
+    replaces row 40 before the batch is matched; a batch of updates only
+    (a deletion and an identity update) changes the DB without matching; every
+    DB match is also checked against the plaintext brute force."""
+    from irismpc import synth
+    from irismpc.server import ServerActor
+
+    db = {"left": [_iris(31, i) for i in range(N0)], "right": [_iris(32, i) for i in range(N0)]}
+    actor = ServerActor(MAXDB, MAXB)
+    sh = {}
+    for k, eye in enumerate(("left", "right")):
+        sh[eye] = synth.shares_of(np.array([c for c, _ in db[eye]]), np.array([m for _, m in db[eye]]), 310 + k)
+    actor.load_db(sh["left"], sh["right"])
+
+    def copy(eye, i, key, src=None):
+        c, m = (src or db[eye])[i]
+        return _flip(c, key), m
+
+    def both_shares(left, right, key):
+        out = []
+        for k, (c, m) in enumerate((left, right)):
+            s = synth.shares_of(c[None], m[None], 700 + 2 * key + k)
+            out.append([(s[p][0][0], s[p][1][0]) for p in range(3)])
+        return tuple(out)
+
+    x = (_iris(33, 0), _iris(34, 0))  # the new iris of row 40 (left, right)
+    b = Batch()
+    b.add("rc17", copy("left", 17, 1), copy("right", 17, 2), kind="reset_check")
+    b.add("rcnew", _iris(35, 0), _iris(36, 0), kind="reset_check")
+    b.add("rec33", copy("left", 33, 3), copy("right", 33, 4), kind="recovery_check")
+    b.add("unew", _iris(35, 1), _iris(36, 1))
+    b.add("l990", copy("left", 990, 5), _iris(37, 0))
+    b.add("rcl995", copy("left", 995, 6), _iris(37, 1), kind="reset_check")
+    b.add("qx", (_flip(x[0][0], 7), x[0][1]), (_flip(x[1][0], 8), x[1][1]))
+    b.add("qold40", copy("left", 40, 9), copy("right", 40, 10))
+    q = b.query()
+    q.luc_lookback_records = 20
+    q.identity_update_request_ids = ["upd40"]
+    q.identity_update_indices = [40]
+    q.identity_update_shares = [both_shares(x[0], x[1], 0)]
+    r = actor.process_batch_query(q)
+
+    FB = 0xFFFFFFFE
+    assert r.merged_results == [17, NM, 33, 1000, 990, NM, 40, 1001]
+    assert r.matches == [True, True, True, False, True, True, True, False]
+    assert actor.current_db_size == N0 + 2
+    assert r.identity_update_indices == [40] and r.identity_update_request_ids == ["upd40"]
+    assert FB not in r.merged_results
+    # brute force over the DB as matched (row 40 replaced), OR rule = LUC rows for uniqueness requests
+    A = irismpc.THRESHOLD_A
+    plain = {e: [list(v) for v in db[e]] for e in ("left", "right")}
+    plain["left"][40], plain["right"][40] = list(x[0]), list(x[1])
+    plain = {e: (np.array([c for c, _ in v]), np.array([m for _, m in v])) for e, v in plain.items()}
+    luc = set(range(N0 - 1 - 20, N0))
+    for j, kind in enumerate(b.types):
+        hit = {e: orc.plain_match(*b.irises[e][j], *plain[e], A) for e in ("left", "right")}
+        rows = set(np.flatnonzero(hit["left"] & hit["right"]).tolist())
+        if kind == "uniqueness":
+            rows |= {d for d in luc if hit["left"][d] or hit["right"][d]}
+        assert set(r.match_ids[j]) == rows, f"{b.ids[j]}: DB matches"
+
+    # a batch of non-MPC updates only
+    y = (_iris(33, 1), _iris(34, 1))
+    from irismpc.server import BatchQuery
+
+    empty = [(np.zeros((0, 12800), np.uint16), np.zeros((0, 6400), np.uint16))] * 3
+    u = BatchQuery([], [], empty, empty, deletion_requests_indices=[17],
+                   identity_update_request_ids=["upd33"], identity_update_indices=[33],
+                   identity_update_shares=[both_shares(y[0], y[1], 1)])
+    ru = actor.process_batch_query(u)
+    assert ru.deleted_ids == [17] and ru.identity_update_indices == [33] and ru.merged_results == []
+    assert actor.current_db_size == N0 + 2
+    c = Batch()
+    c.add("q17", copy("left", 17, 11), copy("right", 17, 12))
+    c.add("qy", (_flip(y[0][0], 13), y[0][1]), (_flip(y[1][0], 14), y[1][1]))
+    rc = actor.process_batch_query(c.query())
+    assert rc.merged_results == [N0 + 2, 33]

@@ -2,6 +2,9 @@
 3174-3303): merged results across devices, spreading insertions over
 devices, new ids of inserted queries. Cases worked out by hand from the
 reference's definitions, multi-device included."""
+import numpy as np
+import pytest
+
 from irismpc.server import (NON_MATCH_ID, BatchQuery, calculate_insertion_indices,
                             distribute_insertions, get_merged_results)
 
@@ -61,3 +64,37 @@ def test_bitmap_rows_sparse_decode():
     assert bitmap_rows(w, n) == want
     assert bitmap_rows(np.zeros(4, np.int64), 256) == []
     assert bitmap_rows(w, 64 * 3 + 5) == list(range(192, 197))
+
+
+def test_or_policy_bitmap_known_answers():
+    """iris-mpc-gpu/tests/or_policy.rs:8-33: rows of ceil(8000 / 64) = 125
+    words; query 0's ids 1, 5, 10 -> word 0 = 1058, query 1 none -> word 125 = 0,
+    query 2's id 3 -> word 250 = 8."""
+    from irismpc.two_stage import prepare_or_policy_bitmap
+
+    bm = prepare_or_policy_bitmap(8000, [[1, 5, 10], [], [3]], 3)
+    assert bm.size * 64 == 8000 * 3
+    assert int(bm[0]) == 1058 and format(int(bm[0]), "064b").endswith("10000100010")
+    assert int(bm[125]) == 0 and int(bm[250]) == 8
+    assert int(np.count_nonzero(bm)) == 2
+
+
+@pytest.mark.parametrize("latest,or_rule,lookback,batch,skip,expected", [
+    # test_only_lookback_records
+    (10, [], 5, 2, set(), [[5, 6, 7, 8, 9, 10], [5, 6, 7, 8, 9, 10]]),
+    # test_empty_zero_lookback_records
+    (10, [[1, 3], [4, 5, 9]], 0, 1, set(), [[1, 3], [4, 5, 9]]),
+    # test_basic_lookback
+    (10, [[1, 3], [4, 5, 9]], 3, 2, set(), [[1, 3, 7, 8, 9, 10], [4, 5, 7, 8, 9, 10]]),
+    # test_duplicate_ids
+    (5, [[1, 1, 2, 3], [3, 3, 4]], 2, 2, set(), [[1, 2, 3, 4, 5], [3, 4, 5]]),
+    # test_skip_lookback_requests
+    (10, [[1, 2], [4, 5]], 3, 2, {0}, [[1, 2], [4, 5, 7, 8, 9, 10]]),
+    # saturating start (latest < lookback)
+    (2, [[7]], 5, 1, set(), [[0, 1, 2, 7]]),
+])
+def test_generate_luc_records_known_answers(latest, or_rule, lookback, batch, skip, expected):
+    """generate_luc_records against iris-mpc-gpu/tests/or_policy.rs:36-147."""
+    from irismpc.server import generate_luc_records
+
+    assert generate_luc_records(latest, or_rule, lookback, batch, skip) == expected
