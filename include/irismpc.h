@@ -424,7 +424,8 @@ int irismpc_scan_db_trio(irismpc_share_db* const codes[3], irismpc_share_db* con
  * db_size - offset)) against the query sets (their own rows: batch_size
  * queries x 31 or fewer padding rows). slot_of[p] (device u32 per batch
  * query, or NULL) sends query j's bits to bitmap row slot_of[p][j]; rows are
- * row_bits bits apart (0: db_size, the layout of irismpc_scan_db). The
+ * row_bits bits apart (0: db_size, the layout of irismpc_scan_db; a nonzero
+ * row_bits below db_size is rejected before any launch). The
  * serving loop's iteration: requests join and leave the query set between
  * chunk steps. */
 int irismpc_scan_chunk_trio(irismpc_share_db* const codes[3], irismpc_share_db* const masks[3],

@@ -31,6 +31,9 @@ void scan_chunks(const PartyScan& p, size_t offset, size_t n_chunks, size_t db_s
   IRIS_REQUIRE((size_t)p.mask_query->rows == rows, "code/mask query sets disagree on rows");
   IRIS_REQUIRE(batch_size * kRotations <= rows, "batch larger than the query rows");
   IRIS_REQUIRE(n_chunks >= 1 && n_chunks <= kMaxStepChunks, "1 .. 16 chunks per step");
+  // every opened bit lands at offset + c * chunk + d < db_size within its row:
+  // rows shorter than the DB would spill into the next query's row
+  IRIS_REQUIRE(row_bits == 0 || row_bits >= db_size, "bitmap rows shorter than the DB");
   // actor.rs:2331-2337 pads to 64; the circuits need a multiple of 8 words
   const size_t pad = rows % 8 == 0 ? 64 : 512;
   size_t real[kMaxStepChunks], width[kMaxStepChunks], start[kMaxStepChunks];

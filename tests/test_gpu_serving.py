@@ -183,3 +183,17 @@ def test_default_rids_do_not_alias(served, orc):
     a2, b2 = R("uniqueness", 0.0, shares=sh_a), R("uniqueness", 0.01, shares=sh_b)
     out = serving.Server(eng, reserve_bytes=0).run([a2, b2])
     assert out[0].result == alone_a and out[1].result == alone_b
+
+
+def test_short_bitmap_rows_rejected(served):
+    """A scan step whose bitmap rows are shorter than the DB is refused with an
+    error before anything is launched."""
+    eng, serving, synth = served
+    from irismpc import engine as E
+
+    sh, _, _ = _query(synth, 71, 3, 80, 1500, 80)
+    qc, qm = eng._query_sets(eng.scan, [sh], 31)
+    with pytest.raises(RuntimeError, match="shorter than the DB"):
+        E.scan_chunks_trio(eng.scan_parties, qc, qm, 0, 1, eng.n_rows, 1, eng.A,
+                           [b.data_ptr() for b in eng.bitmaps], eng.scan.streams,
+                           [m.data_ptr() for m in eng.slot_maps], row_bits=eng.n_rows - 64)
