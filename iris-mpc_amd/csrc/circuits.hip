@@ -97,7 +97,10 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
   const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63), prev = (p + 2) % 3;
   const CircuitBufs& P = t.p[p];
-  __shared__ uint32_t ks[3][1024];   // [party][own stream: u16 pairs in element order]
+  // [party][own stream: u16 pairs in element order, 16-byte granules swizzled]:
+  // granule j of lane w's block sits at granule 4 w + (j ^ (w & 3)), so the
+  // lanes' 16-byte writes of one granule spread over more banks
+  __shared__ uint32_t ks[3][1024];
   __shared__ uint32_t msg[3][1024];  // [receiving party][u16 pairs in element order]
   const size_t n = 128 * t.W;
   const size_t e_base = (size_t)blockIdx.x * 2048;
@@ -106,11 +109,18 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
     chacha_block<12>(P.key_own, t.nonce, ctr + (uint64_t)blockIdx.x * 64 + lane, ko);  // my_rng (chacha1)
 #pragma unroll
     for (int i = 0; i < 16; i += 4)
-      *reinterpret_cast<uint4*>(&ks[p][lane * 16 + i]) = uint4{ko[i], ko[i + 1], ko[i + 2], ko[i + 3]};
+      *reinterpret_cast<uint4*>(&ks[p][lane * 16 + 4 * ((i / 4) ^ (lane & 3))]) =
+          uint4{ko[i], ko[i + 1], ko[i + 2], ko[i + 3]};
   }
   __syncthreads();
   const uint16_t* own16 = reinterpret_cast<const uint16_t*>(ks[p]);
   const uint16_t* prv16 = reinterpret_cast<const uint16_t*>(ks[prev]);  // their_rng (chacha2)
+  // u16 offset of the 8 elements starting at element el (el a multiple of 8:
+  // one granule) under the swizzle above
+  auto kso = [](int el) {
+    const int g = el >> 3, w = g >> 2;
+    return 8 * (4 * w + ((g & 3) ^ (w & 3)));
+  };
   uint16_t* to_next = reinterpret_cast<uint16_t*>(msg[(p + 1) % 3]);
   const uint16_t* inbox = reinterpret_cast<const uint16_t*>(msg[p]);
   // step 1: P0 sends y to P1, P1 sends x to P2; both write corr_a
@@ -121,8 +131,8 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
       const size_t e = e_base + el;
       if (e >= n) break;
       uint16_t own[8], prv[8], o1[8], o2[8];
-      u16x8_load(own16 + el, own);
-      u16x8_load(prv16 + el, prv);
+      u16x8_load(own16 + kso(el), own);
+      u16x8_load(prv16 + kso(el), prv);
       if (p == 0) {
         // rand_01 = my rng, rand_02 = their rng; in_b = party 0's b share
         const uint32_t bits = (uint32_t)(P.c_b[e >> 6] >> (e & 63)) & 0xffu;
@@ -157,8 +167,8 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
       const size_t e = e_base + el;
       if (e >= n) break;
       uint16_t own[8], prv[8], in[8], o1[8], o2[8], o3[8];
-      u16x8_load(own16 + el, own);
-      u16x8_load(prv16 + el, prv);
+      u16x8_load(own16 + kso(el), own);
+      u16x8_load(prv16 + kso(el), prv);
       u16x8_load(inbox + el, in);
       if (p == 1) {
 #pragma unroll
@@ -190,7 +200,7 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
       const size_t e = e_base + el;
       if (e >= n) break;
       uint16_t prv[8], in[8], o1[8];
-      u16x8_load(prv16 + el, prv);
+      u16x8_load(prv16 + kso(el), prv);
       u16x8_load(inbox + el, in);
       const uint32_t bits = (uint32_t)(P.c_b[e >> 6] >> (e & 63)) & 0xffu;
 #pragma unroll
@@ -262,7 +272,10 @@ struct AdderState {
 };
 
 struct AdderLds {
-  uint32_t rnd[3][16][16][5];  // [party][gate of the batch][half word of the group][group] (padded)
+  // [party][group][writing lane = gate of the batch][its 16 words, padded to 17]: a
+  // write of one word by all lanes and a read of one gate by all lanes both
+  // spread over the banks (the 17-word row stride staggers them)
+  uint32_t rnd[3][4][16][17];
   uint32_t xch[2][3][2][64];   // [step parity][party][ripple, carry-save][lane]
 };
 
@@ -420,12 +433,12 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
       uint32_t k[16] = {};
       if (valid) chacha_block<12>(P.key_own, t.nonce, ctr, k);
 #pragma unroll
-      for (int i = 0; i < 16; i++) L.rnd[p][gl][i][grp] = k[i];
+      for (int i = 0; i < 16; i++) L.rnd[p][grp][gl][i] = k[i];
     }
     __syncthreads();
     uint32_t r16[16];  // F(own) ^ F(prev): bits of the 16 gates for this lane's half word
 #pragma unroll
-    for (int c = 0; c < 16; c++) r16[c] = L.rnd[p][c][gl][grp] ^ L.rnd[prev][c][gl][grp];
+    for (int c = 0; c < 16; c++) r16[c] = L.rnd[p][grp][c][gl] ^ L.rnd[prev][grp][c][gl];
     if (!MSB && b == 0) r_out = r16[1];
     switch (b) {
       case 0: adder_batch<MSB, 0>(sl, st, r16, r_out, L, p, prev, lane); break;
