@@ -251,3 +251,54 @@ def test_pipeline_multi_chunk_share_parity(irismpc, orc):
         assert np.array_equal(irismpc.download(b, W, np.uint64), rb[p]), f"party {p} result b"
         assert parties[p].circuits.counter == end
     orc.set_threads(1)
+
+
+def test_pipeline_config1_golden(irismpc, orc):
+    """The CPU-runnable configuration as stated (4,096 DB irises, 32 queries x
+    31 rotations, 8 copies with exactly 1,920 flips, targets from the ChaCha20
+    key-0 stream) through the three-party scan: the bitmap, the first DB
+    columns of every party's masked dot shares and the first result words of
+    its circuit shares equal the committed golden vectors."""
+    import os
+    import sys
+
+    import torch
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_config1 as g
+
+    want = g.load()
+    db_c, db_m, q_c, q_m, targets = g.inputs(orc)
+    assert targets == want["targets"].tolist()
+    dbc_sh, dbm_sh = g.shares(orc, db_c, db_m, 30)
+    qc_sh, qm_sh = g.shares(orc, q_c, q_m, 31)
+    N, B, Qr, chunk = g.N, g.B, g.B * 31, irismpc.DB_CHUNK_SIZE
+    ex = irismpc.Exchange(timeout_s=120)
+    parties, qcs, qms, bitmaps, streams = [], [], [], [], []
+    for p in range(3):
+        codes = irismpc.ShareDB(p, chunk, Qr, 12800, g.seeds(orc, p, 0), comm=ex, tag=0)
+        masks = irismpc.ShareDB(p, chunk, Qr, 6400, g.seeds(orc, p, 1), comm=ex, tag=1)
+        circ = irismpc.Circuits(p, Qr * chunk, Qr * chunk // 64, g.seeds(orc, p, 5), ex, tag=2)
+        cmp = irismpc.DistanceComparator(Qr, N)
+        cdb, mdb = codes.alloc_db(N), masks.alloc_db(N)
+        cdb.load_full_db(dbc_sh[p])
+        mdb.load_full_db(dbm_sh[p])
+        parties.append(irismpc.PartyEngines(codes, masks, circ, cmp, cdb, mdb))
+        qcs.append(codes.upload_query_shares(qc_sh[p]))
+        qms.append(masks.upload_query_shares(qm_sh[p]))
+        bitmaps.append(torch.zeros(N * B // 64 + 2, dtype=torch.int64, device="cuda"))
+        streams.append(irismpc.stream_create())
+    torch.cuda.synchronize()
+    irismpc.scan_db_trio(parties, qcs, qms, N, B, g.A, [b.data_ptr() for b in bitmaps], streams)
+    for s in streams:
+        irismpc.stream_synchronize(s)
+    for p in range(3):
+        got = bitmaps[p].cpu().numpy().view(np.uint64)[:N * B // 64]
+        assert np.array_equal(got, want["bitmap"]), f"party {p} bitmap"
+        cd = parties[p].codes.fetch_results(Qr * N).reshape(Qr, N)[:, :g.COLS]
+        md = parties[p].masks.fetch_results(Qr * N).reshape(Qr, N)[:, :g.COLS]
+        assert np.array_equal(cd, want["code_dot_cols"][p]), f"party {p} code dot shares"
+        assert np.array_equal(md, want["mask_dot_cols"][p]), f"party {p} mask dot shares"
+        a, b, _c = parties[p].circuits.open_views()
+        assert np.array_equal(irismpc.download(a, g.WORDS, np.uint64), want["result_a"][p]), f"party {p} result a"
+        assert np.array_equal(irismpc.download(b, g.WORDS, np.uint64), want["result_b"][p]), f"party {p} result b"

@@ -333,3 +333,37 @@ def test_server_checks_luc_and_identity_updates(irismpc, orc):
     c.add("qy", (_flip(y[0][0], 13), y[0][1]), (_flip(y[1][0], 14), y[1][1]))
     rc = actor.process_batch_query(c.query())
     assert rc.merged_results == [N0 + 2, 33]
+
+
+def test_server_refuses_disagreeing_parties(irismpc):
+    """The three parties' merged results are compared before any result is
+    assembled or written (the role of sync_match_results, actor.rs:2591-2630):
+    a party whose first match differs makes the batch fail and the DB stays
+    as it was."""
+    from irismpc import synth
+    from irismpc.server import ServerActor
+
+    db = {"left": [_iris(41, i) for i in range(64)], "right": [_iris(42, i) for i in range(64)]}
+    actor = ServerActor(1024, 4)
+    sh = {}
+    for k, eye in enumerate(("left", "right")):
+        sh[eye] = synth.shares_of(np.array([c for c, _ in db[eye]]), np.array([m for _, m in db[eye]]), 410 + k)
+    actor.load_db(sh["left"], sh["right"])
+    b = Batch()
+    b.add("u", _iris(43, 0), _iris(44, 0))
+    b.add("m5", (_flip(db["left"][5][0], 1), db["left"][5][1]), (_flip(db["right"][5][0], 2), db["right"][5][1]))
+    match = actor.matcher.match
+
+    def tampered(*a, **k):
+        r = match(*a, **k)
+        r.final_results[1] = r.final_results[1].copy()
+        r.final_results[1][1] ^= 1
+        return r
+
+    actor.matcher.match = tampered
+    with pytest.raises(RuntimeError, match="disagree"):
+        actor.process_batch_query(b.query())
+    assert actor.current_db_size == 64  # nothing inserted
+    actor.matcher.match = match
+    r = actor.process_batch_query(b.query())
+    assert r.merged_results == [64, 5]

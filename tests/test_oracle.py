@@ -479,3 +479,30 @@ def test_share_mirrored_matches_plaintext_mirror(orc):
         assert np.array_equal(orc.share_mirrored(got[p], True), want[p])
     # a wrong variant (no imaginary negation) does not reproduce it
     assert not np.array_equal(orc.share_mirrored(got[0], False), want[0])
+
+
+def test_config1_golden_vectors_reproduce(orc):
+    """The committed golden vectors of the CPU-runnable configuration
+    (tests/golden/config1.npz, written by tests/golden/make_config1.py) are
+    what the oracle computes now; every planted copy (1,920 of 12,800 bits
+    flipped) is in the bitmap."""
+    import os
+
+    sys_path_added = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    import sys
+
+    if sys_path_added not in sys.path:
+        sys.path.insert(0, sys_path_added)
+    import make_config1 as g
+
+    want = g.load()
+    orc.set_threads(min(8, os.cpu_count() or 1))
+    try:
+        got = g.expected(orc, threads=min(8, os.cpu_count() or 1))
+    finally:
+        orc.set_threads(1)
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+    bits = np.unpackbits(want["bitmap"].view(np.uint8), bitorder="little").reshape(g.B, g.N)
+    for j, t in zip(g.PLANTED, want["targets"]):
+        assert bits[j, t]
