@@ -9,7 +9,7 @@ Per batch:
   actor.rs:2632-2700); a batch of updates only stops there;
 - LUC look-back: the last luc_lookback_records DB rows join every request's
   OR-rule ids except reauths and reset / recovery checks (generate_luc_records,
-  actor.rs:770-795,3627-3659);
+  actor.rs:772-796,3623-3659);
 - two-stage matching with batch deduplication (the two_stage module);
 - result assembly: merged results, match ids (DB ids, and 0xFFFFFFFF - j for
   an earlier query j of the batch), partial per-eye matches with their
@@ -99,7 +99,7 @@ def calculate_insertion_indices(merged_results, insertion_list, db_sizes, batch_
 
 def generate_luc_records(latest_luc_index, or_rule_indices, lookback_records, batch_size,
                          skip_lookback_requests=()):
-    """actor.rs:3627-3659: the ids latest - lookback .. latest (saturating at 0)
+    """actor.rs:3623-3659: the ids latest - lookback .. latest (saturating at 0)
     join each request's OR-rule ids (then sorted, without repeats), except the
     requests listed in skip_lookback_requests; with no OR-rule ids at all every
     request gets the look-back ids alone; lookback_records = 0 changes
