@@ -606,7 +606,7 @@ def run_c5(args):
         return sum(31 * r.db_rows for r in out if r.kind == "uniqueness") + sum(31 for r in out if r.kind == "reauth")
 
     def serve(form, reqs):
-        reqs = [dataclasses.replace(r, covered=set()) for r in reqs]
+        reqs = [dataclasses.replace(r) for r in reqs]
         if form == "serving":
             srv = serving.Server(eng)
         else:

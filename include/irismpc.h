@@ -41,6 +41,8 @@ typedef struct irismpc_query irismpc_query;
 typedef struct irismpc_circuits irismpc_circuits;
 typedef struct irismpc_comparator irismpc_comparator;
 typedef struct irismpc_graph irismpc_graph;
+typedef struct irismpc_batch_runner irismpc_batch_runner;
+typedef struct irismpc_serving irismpc_serving;
 
 /* ------------------------------------------------------------- misc */
 /* Thread-local text of the last failure (empty string if none). */
@@ -483,6 +485,127 @@ int irismpc_scan_batch_self_trio(irismpc_share_db* const codes[3], irismpc_share
                                  const irismpc_query* const q_mask[3], size_t batch_size,
                                  uint32_t threshold_a, uint64_t* const bitmaps[3], size_t row_bits,
                                  void* const streams[3]);
+
+/* ------------------------------------------------ batching form */
+/* Query batches of uneven size packed into few launches. The reference runs
+ * batch after batch, every query set padded to max_batch_size x 31 rows
+ * (iris-mpc-gpu/src/server/actor.rs:895-901, share_db.rs:800-845); the
+ * packing, its planner and its schedule have no reference counterpart. The
+ * three host-only functions below need no GPU. */
+/* Greedy packing of consecutive batches (sizes[i] queries each): a pack holds
+ * at most max_queries queries (a larger batch gets one of its own) and at most
+ * max_batches batches. pack_of_batch[i] = pack of batch i; *n_packs = packs. */
+int irismpc_batch_plan_packs(const uint32_t* sizes, size_t n_batches, size_t max_queries,
+                             size_t max_batches, uint32_t* pack_of_batch, size_t* n_packs);
+/* ChaCha blocks one remote dot_reduce_range over `total` rows uses (mask and
+ * reshare OTP per chunk), and a whole scan of n_rows rows launched chunk by
+ * chunk with the last chunk padded to 64 rows. */
+int irismpc_batch_counter_advance(size_t total, size_t chunk, size_t rows, uint64_t* blocks);
+int irismpc_batch_scan_counter_advance(size_t n_rows, size_t chunk, size_t rows, uint64_t* blocks);
+/* One matrix (code or mask) of one party: batches of sizes[i] queries whose
+ * raw shares sit back to back in `shares` (sum(sizes) x code_length u16, host);
+ * outs[i] (device) receives batch i's results as its own dot_reduce_range over
+ * the DB's n_rows rows lays them out (the last chunk stays: chunk x sizes[i] x
+ * 31 u16 + 64). The engine's counter is the plain form's counter. */
+int irismpc_batch_runner_create(irismpc_share_db* e, const irismpc_db* db, size_t n_rows,
+                                uint16_t multiplier, const uint32_t* sizes, size_t n_batches,
+                                const uint16_t* shares, uint16_t* const* outs, size_t chunk,
+                                size_t max_queries, size_t max_batches, void* stream,
+                                irismpc_batch_runner** out);
+int irismpc_batch_runner_destroy(irismpc_batch_runner* r);
+/* New pack size (rebuilds the plan and the packed query sets). */
+int irismpc_batch_runner_replan(irismpc_batch_runner* r, size_t max_queries, void* stream);
+int irismpc_batch_runner_plan(const irismpc_batch_runner* r, uint32_t* pack_of_batch, size_t* n_packs);
+/* The packed schedule over the first n_rows rows (0 = all): one launch per
+ * pack and chunk, each batch at the counters its plain run from ctr0 would
+ * use. Allocates and synchronises nothing: it can be captured into a graph. */
+int irismpc_batch_runner_issue_packed(irismpc_batch_runner* r, uint64_t ctr0, size_t n_rows,
+                                      void* stream);
+/* Batch `batch`'s plain run over the whole DB at the engine's counter. */
+int irismpc_batch_runner_issue_plain(irismpc_batch_runner* r, size_t batch, void* stream);
+
+/* ------------------------------------------------ serving form */
+/* Request loops over the three parties' HBM-resident DB shares, with their own
+ * engines: the continuous policy (uniqueness requests join the active query
+ * set between chunk steps as slots and free HBM allow; no reference
+ * counterpart) and the per-batch policy (ServerActor's batch after batch,
+ * actor.rs:584-644, each batch padded to max_batch x 31 query rows, one full
+ * scan per batch). Reauth requests run on a one-query engine set over the
+ * 64-row window of their target; insertions append a row, deletions overwrite
+ * one with shares of the default iris (get_dummy_shares_for_deletion,
+ * iris-mpc-common/src/iris_db/mod.rs:10-23). */
+#define IRISMPC_REQ_UNIQUENESS 0
+#define IRISMPC_REQ_REAUTH 1
+#define IRISMPC_REQ_INSERT 2
+#define IRISMPC_REQ_DELETE 3
+#define IRISMPC_SERVING_CONTINUOUS 0
+#define IRISMPC_SERVING_PER_BATCH 1
+#define IRISMPC_REQ_PENDING 0
+#define IRISMPC_REQ_DONE 1
+#define IRISMPC_REQ_FAILED (-1)
+/* party_seeds: [3][8] u32, party p's seed (its engines' prev seed is party
+ * p - 1's). The DB slices (any engine's alloc_db of >= capacity rows, matching
+ * code lengths) hold n_rows rows and stay owned by the caller; capacity bounds
+ * insertions (capacity - 32,768 rows). max_active: 1 .. 64 slots. */
+int irismpc_serving_create(int device, const uint32_t* party_seeds, irismpc_db* const code_dbs[3],
+                           irismpc_db* const mask_dbs[3], size_t n_rows, size_t capacity,
+                           size_t max_active, uint32_t threshold_a, irismpc_serving** out);
+int irismpc_serving_destroy(irismpc_serving* s);
+/* Starts the loop thread and the reauth worker. policy: IRISMPC_SERVING_*;
+ * max_batch: the per-batch policy's padding (1 .. max_active); reserve_bytes:
+ * free HBM that admission keeps. */
+int irismpc_serving_start(irismpc_serving* s, int policy, size_t max_batch, size_t reserve_bytes);
+/* Waits until every submitted request has finished (timeout_s <= 0: no
+ * limit), then stops the loop; fails if the loop failed or the wait timed out. */
+int irismpc_serving_finish(irismpc_serving* s, double timeout_s);
+/* code_shares [3][12800] u16 and mask_shares [3][6400] u16: the parties' raw
+ * shares (null for deletions); target: reauth / deletion row; arrival_s: the
+ * request's arrival on irismpc_serving_now's clock (< 0: now); a later arrival
+ * is held until its time, so a whole schedule can be submitted at once. */
+int irismpc_serving_submit(irismpc_serving* s, int kind, const uint16_t* code_shares,
+                           const uint16_t* mask_shares, int64_t target, double arrival_s,
+                           uint64_t* id);
+/* state: IRISMPC_REQ_*; value: uniqueness match count, reauth 0/1, insertion /
+ * deletion row; matches (uniqueness, ascending) copied up to max_matches,
+ * *n_matches = their number; times[0] arrival, times[1] completion (s);
+ * db_rows: the DB rows the request was checked against; error: the failure
+ * text (error_len bytes incl. the terminator). Pointers may be null. */
+int irismpc_serving_result(const irismpc_serving* s, uint64_t id, int* state, int64_t* value,
+                           uint32_t* matches, size_t max_matches, size_t* n_matches, double* times,
+                           size_t* db_rows, char* error, size_t error_len);
+int irismpc_serving_now(const irismpc_serving* s, double* seconds);
+/* out[4]: chunk steps, largest active set, admissions deferred by the reserve,
+ * per-batch batches (since start). */
+int irismpc_serving_stats(const irismpc_serving* s, uint64_t* out);
+int irismpc_serving_n_rows(const irismpc_serving* s, size_t* n);
+/* Loop stopped: one uniqueness query alone over the whole DB (the plain
+ * per-request form); matches as in irismpc_serving_result. */
+int irismpc_serving_run_alone(irismpc_serving* s, const uint16_t* code_shares,
+                              const uint16_t* mask_shares, uint32_t* matches, size_t max_matches,
+                              size_t* n_matches);
+/* Loop stopped: seconds of one scan step of `batch` queries over n_chunks
+ * chunks (best of reps after a warm-up). */
+int irismpc_serving_time_step(irismpc_serving* s, size_t batch, size_t n_chunks, int reps,
+                              double* seconds);
+/* Loop stopped: chunks per step for active counts from buckets[i] up to the
+ * next bucket (n = 0: the fixed rule 8 / 4 / 2 / 1 chunks for 1 / 2 / <= 8 /
+ * more active queries). */
+int irismpc_serving_set_step_table(irismpc_serving* s, const uint32_t* buckets,
+                                   const uint32_t* chunks, size_t n);
+/* Loop stopped: times one step for every bucket (<= max_active) and every k of
+ * ks that the result buffers and the DB allow, picks per bucket with
+ * irismpc_serving_pick_step and sets the table. chunks_out[n_buckets] (0 past
+ * max_active); seconds_per_chunk_out[n_buckets][n_ks] (negative: not timed). */
+int irismpc_serving_tune_steps(irismpc_serving* s, const uint32_t* buckets, size_t n_buckets,
+                               const uint32_t* ks, size_t n_ks, double tol, double max_step_s,
+                               uint32_t* chunks_out, double* seconds_per_chunk_out);
+int irismpc_serving_chunks_per_step(const irismpc_serving* s, size_t batch, size_t chunk_index,
+                                    size_t n_chunks, size_t* k);
+/* Host only: a step size from measured seconds per chunk: steps longer than
+ * max_step_s (<= 0: no limit) are left out (all longer: the smallest k); of
+ * the rest the smallest k within tol of the fastest per chunk. */
+int irismpc_serving_pick_step(const uint32_t* ks, const double* seconds_per_chunk, size_t n,
+                              double tol, double max_step_s, uint32_t* k);
 
 /* ------------------------------------------------ HIP graphs */
 /* Captures the library calls issued on `stream` between begin and end into a

@@ -1,0 +1,141 @@
+// Batching form: pack planner and the packed / plain schedules (host code
+// over ShareDB's share-GEMM launches).
+#include "batching.hpp"
+
+#include <algorithm>
+#include <limits>
+
+namespace irismpc {
+
+namespace {
+// [offset, padded width) of each chunk of a scan over n_rows rows
+std::vector<std::pair<size_t, size_t>> chunk_ranges(size_t n_rows, size_t chunk) {
+  std::vector<std::pair<size_t, size_t>> r;
+  for (size_t off = 0; off < n_rows; off += chunk) r.emplace_back(off, round_up(std::min(chunk, n_rows - off), 64));
+  return r;
+}
+}  // namespace
+
+std::vector<std::vector<size_t>> plan_packs(const uint32_t* sizes, size_t n, size_t max_queries,
+                                            size_t max_batches) {
+  IRIS_REQUIRE(max_queries >= 1 && max_batches >= 1, "empty packs");
+  std::vector<std::vector<size_t>> packs;
+  std::vector<size_t> cur;
+  size_t q = 0;
+  for (size_t i = 0; i < n; i++) {
+    if (!cur.empty() && (q + sizes[i] > max_queries || cur.size() == max_batches)) {
+      packs.push_back(cur);
+      cur.clear();
+      q = 0;
+    }
+    cur.push_back(i);
+    q += sizes[i];
+  }
+  if (!cur.empty()) packs.push_back(cur);
+  return packs;
+}
+
+uint64_t scan_counter_advance(size_t n_rows, size_t chunk, size_t rows) {
+  uint64_t n = 0;
+  for (const auto& r : chunk_ranges(n_rows, chunk)) n += ShareDB::range_counter_advance(r.second, chunk, rows);
+  return n;
+}
+
+uint32_t pick_step_chunks(const uint32_t* k, const double* t, size_t n, double tol, double max_step_s) {
+  IRIS_REQUIRE(n >= 1, "no step sizes timed");
+  double best = std::numeric_limits<double>::infinity();
+  bool any = false;
+  for (size_t i = 0; i < n; i++)
+    if (max_step_s <= 0 || k[i] * t[i] <= max_step_s) {
+      best = std::min(best, t[i]);
+      any = true;
+    }
+  uint32_t pick = std::numeric_limits<uint32_t>::max();
+  for (size_t i = 0; i < n; i++) {
+    const bool fits = max_step_s <= 0 || k[i] * t[i] <= max_step_s;
+    if (!any || (fits && t[i] <= best * (1 + tol))) pick = std::min(pick, k[i]);
+  }
+  return pick;
+}
+
+BatchRunner::BatchRunner(ShareDB* engine, const DbSlice* db, size_t n_rows, uint16_t multiplier,
+                         const uint32_t* sizes, size_t n_batches, const uint16_t* shares,
+                         uint16_t* const* outs, size_t chunk, size_t max_queries, size_t max_batches,
+                         hipStream_t s)
+    : e_(engine),
+      db_(db),
+      n_rows_(n_rows),
+      chunk_(chunk),
+      max_queries_(max_queries),
+      max_batches_(max_batches),
+      mult_(multiplier),
+      sizes_(sizes, sizes + n_batches),
+      outs_(outs, outs + n_batches) {
+  IRIS_REQUIRE(engine && db && db->code_length == engine->code_length(), "DB of another code length");
+  IRIS_REQUIRE(n_batches >= 1 && n_rows >= 1 && chunk >= 256 && chunk % 256 == 0, "bad batching geometry");
+  IRIS_REQUIRE(n_rows <= db->capacity - kMaxBD, "scan beyond the DB");
+  const size_t K = (size_t)engine->code_length();
+  size_t q = 0;
+  for (size_t i = 0; i < n_batches; i++) {
+    IRIS_REQUIRE(sizes[i] >= 1 && outs[i], "empty batch or missing output");
+    first_.push_back(q);
+    q += sizes[i];
+  }
+  shares_.assign(shares, shares + q * K);
+  for (size_t i = 0; i < n_batches; i++)
+    q_plain_.push_back(e_->upload_query_shares_rows(shares_.data() + first_[i] * K, sizes_[i],
+                                                    (size_t)sizes_[i] * kRotations, s));
+  replan(max_queries, s);
+}
+
+void BatchRunner::replan(size_t max_queries, hipStream_t s) {
+  plan_ = plan_packs(sizes_.data(), sizes_.size(), max_queries, max_batches_);
+  max_queries_ = max_queries;
+  q_packed_.clear();  // the old sets are freed before the new ones exist
+  const size_t K = (size_t)e_->code_length();
+  for (const auto& pack : plan_) {
+    // a pack's batches are consecutive: their shares sit back to back
+    size_t nq = 0;
+    for (size_t i : pack) nq += sizes_[i];
+    q_packed_.push_back(
+        e_->upload_query_shares_rows(shares_.data() + first_[pack.front()] * K, nq, nq * kRotations, s));
+  }
+}
+
+void BatchRunner::issue_packed(uint64_t ctr0, size_t n_rows, hipStream_t s) {
+  const size_t n = n_rows ? std::min(n_rows, n_rows_) : n_rows_;
+  const bool remote = e_->is_remote();
+  std::vector<uint64_t> base(sizes_.size());
+  uint64_t c = ctr0;
+  for (size_t i = 0; i < sizes_.size(); i++) {  // the plain form's counter at the start of each batch
+    base[i] = c;
+    if (remote) c += scan_counter_advance(n, chunk_, (size_t)sizes_[i] * kRotations);
+  }
+  const auto ranges = chunk_ranges(n, chunk_);
+  std::vector<size_t> rows;
+  std::vector<uint16_t*> outs;
+  std::vector<uint64_t> ctr;
+  for (size_t p = 0; p < plan_.size(); p++) {
+    rows.clear();
+    outs.clear();
+    for (size_t i : plan_[p]) {
+      rows.push_back((size_t)sizes_[i] * kRotations);
+      outs.push_back(outs_[i]);
+    }
+    for (const auto& r : ranges) {
+      ctr.clear();
+      for (size_t j = 0; j < plan_[p].size(); j++)
+        ctr.push_back(base[plan_[p][j]] + (remote ? ShareDB::range_counter_advance(r.first, chunk_, rows[j]) : 0));
+      e_->dot_reduce_packed(*q_packed_[p], *db_, r.first, r.second, chunk_, mult_, rows.data(), outs.data(),
+                            (int)rows.size(), ctr.data(), s);
+    }
+  }
+}
+
+void BatchRunner::issue_plain(size_t batch, hipStream_t s) {
+  IRIS_REQUIRE(batch < sizes_.size(), "batch index beyond the batches");
+  for (const auto& r : chunk_ranges(n_rows_, chunk_))
+    e_->dot_reduce_range(*q_plain_[batch], *db_, r.first, r.second, chunk_, mult_, outs_[batch], s);
+}
+
+}  // namespace irismpc

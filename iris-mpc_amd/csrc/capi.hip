@@ -6,9 +6,11 @@
 
 #include "../../include/irismpc.h"
 #include "actor.hpp"
+#include "batching.hpp"
 #include "comparator.hpp"
 #include "engine.hpp"
 #include "host_crypto.hpp"
+#include "serving.hpp"
 #include "synth.hpp"
 
 using namespace irismpc;
@@ -33,6 +35,13 @@ struct irismpc_circuits {
 };
 struct irismpc_comparator {
   std::unique_ptr<DistanceComparator> impl;
+};
+struct irismpc_batch_runner {
+  std::unique_ptr<BatchRunner> impl;
+  int device;
+};
+struct irismpc_serving {
+  std::unique_ptr<Serving> impl;
 };
 
 namespace {
@@ -762,6 +771,225 @@ int irismpc_scan_batch_self_trio(irismpc_share_db* const codes[3], irismpc_share
     }
     scan_batch_self_trio(p, batch_size, threshold_a, bitmaps, row_bits, s);
   });
+}
+
+// ------------------------------------------------------------------ batching form
+int irismpc_batch_plan_packs(const uint32_t* sizes, size_t n_batches, size_t max_queries,
+                             size_t max_batches, uint32_t* pack_of_batch, size_t* n_packs) {
+  return guarded([&] {
+    const auto packs = plan_packs(sizes, n_batches, max_queries, max_batches);
+    for (size_t p = 0; p < packs.size(); p++)
+      for (size_t i : packs[p]) pack_of_batch[i] = (uint32_t)p;
+    *n_packs = packs.size();
+  });
+}
+
+int irismpc_batch_counter_advance(size_t total, size_t chunk, size_t rows, uint64_t* blocks) {
+  return guarded([&] {
+    IRIS_REQUIRE(chunk > 0, "chunk of 0 rows");
+    *blocks = ShareDB::range_counter_advance(total, chunk, rows);
+  });
+}
+
+int irismpc_batch_scan_counter_advance(size_t n_rows, size_t chunk, size_t rows, uint64_t* blocks) {
+  return guarded([&] {
+    IRIS_REQUIRE(chunk > 0, "chunk of 0 rows");
+    *blocks = scan_counter_advance(n_rows, chunk, rows);
+  });
+}
+
+int irismpc_batch_runner_create(irismpc_share_db* e, const irismpc_db* db, size_t n_rows,
+                                uint16_t multiplier, const uint32_t* sizes, size_t n_batches,
+                                const uint16_t* shares, uint16_t* const* outs, size_t chunk,
+                                size_t max_queries, size_t max_batches, void* stream,
+                                irismpc_batch_runner** out) {
+  return guarded([&] {
+    DevSet g(e->device);
+    auto r = std::make_unique<irismpc_batch_runner>();
+    r->device = e->device;
+    r->impl = std::make_unique<BatchRunner>(e->impl.get(), db->impl.get(), n_rows, multiplier, sizes, n_batches,
+                                            shares, outs, chunk, max_queries, max_batches, st(stream));
+    *out = r.release();
+  });
+}
+
+int irismpc_batch_runner_destroy(irismpc_batch_runner* r) {
+  return guarded([&] { delete r; });
+}
+
+int irismpc_batch_runner_replan(irismpc_batch_runner* r, size_t max_queries, void* stream) {
+  return guarded([&] {
+    DevSet g(r->device);
+    r->impl->replan(max_queries, st(stream));
+  });
+}
+
+int irismpc_batch_runner_plan(const irismpc_batch_runner* r, uint32_t* pack_of_batch, size_t* n_packs) {
+  return guarded([&] {
+    const auto& packs = r->impl->plan();
+    for (size_t p = 0; p < packs.size(); p++)
+      for (size_t i : packs[p]) pack_of_batch[i] = (uint32_t)p;
+    *n_packs = packs.size();
+  });
+}
+
+int irismpc_batch_runner_issue_packed(irismpc_batch_runner* r, uint64_t ctr0, size_t n_rows, void* stream) {
+  return guarded([&] {
+    DevSet g(r->device);
+    r->impl->issue_packed(ctr0, n_rows, st(stream));
+  });
+}
+
+int irismpc_batch_runner_issue_plain(irismpc_batch_runner* r, size_t batch, void* stream) {
+  return guarded([&] {
+    DevSet g(r->device);
+    r->impl->issue_plain(batch, st(stream));
+  });
+}
+
+// ------------------------------------------------------------------ serving form
+namespace {
+void copy_matches(const std::vector<uint32_t>& m, uint32_t* matches, size_t max_matches, size_t* n_matches) {
+  if (matches) std::copy_n(m.begin(), std::min(max_matches, m.size()), matches);
+  if (n_matches) *n_matches = m.size();
+}
+}  // namespace
+
+int irismpc_serving_create(int device, const uint32_t* party_seeds, irismpc_db* const code_dbs[3],
+                           irismpc_db* const mask_dbs[3], size_t n_rows, size_t capacity,
+                           size_t max_active, uint32_t threshold_a, irismpc_serving** out) {
+  return guarded([&] {
+    DevSet g(device);
+    ChaChaKey seeds[3];
+    DbSlice* cd[3];
+    DbSlice* md[3];
+    for (int p = 0; p < 3; p++) {
+      seeds[p] = key_of(party_seeds + 8 * p);
+      IRIS_REQUIRE(code_dbs[p] && mask_dbs[p], "three code and three mask DB slices");
+      cd[p] = code_dbs[p]->impl.get();
+      md[p] = mask_dbs[p]->impl.get();
+    }
+    auto s = std::make_unique<irismpc_serving>();
+    s->impl = std::make_unique<Serving>(device, seeds, cd, md, n_rows, capacity, max_active, threshold_a);
+    *out = s.release();
+  });
+}
+
+int irismpc_serving_destroy(irismpc_serving* s) {
+  return guarded([&] { delete s; });
+}
+
+int irismpc_serving_start(irismpc_serving* s, int policy, size_t max_batch, size_t reserve_bytes) {
+  return guarded([&] { s->impl->start(policy, max_batch, reserve_bytes); });
+}
+
+int irismpc_serving_finish(irismpc_serving* s, double timeout_s) {
+  return guarded([&] { s->impl->finish(timeout_s); });
+}
+
+int irismpc_serving_submit(irismpc_serving* s, int kind, const uint16_t* code_shares,
+                           const uint16_t* mask_shares, int64_t target, double arrival_s, uint64_t* id) {
+  return guarded([&] { *id = s->impl->submit(kind, code_shares, mask_shares, target, arrival_s); });
+}
+
+int irismpc_serving_result(const irismpc_serving* s, uint64_t id, int* state, int64_t* value,
+                           uint32_t* matches, size_t max_matches, size_t* n_matches, double* times,
+                           size_t* db_rows, char* error, size_t error_len) {
+  return guarded([&] {
+    Serving::Result r;
+    IRIS_REQUIRE(s->impl->result(id, &r), "unknown request id");
+    if (state) *state = r.state;
+    if (value) *value = r.value;
+    copy_matches(r.matches, matches, max_matches, n_matches);
+    if (times) {
+      times[0] = r.arrival;
+      times[1] = r.done;
+    }
+    if (db_rows) *db_rows = r.db_rows;
+    if (error && error_len) {
+      const size_t n = std::min(error_len - 1, r.error.size());
+      std::memcpy(error, r.error.data(), n);
+      error[n] = 0;
+    }
+  });
+}
+
+int irismpc_serving_now(const irismpc_serving* s, double* seconds) {
+  return guarded([&] { *seconds = s->impl->now(); });
+}
+
+int irismpc_serving_stats(const irismpc_serving* s, uint64_t* out) {
+  return guarded([&] {
+    const ServingStats st = s->impl->stats();
+    out[0] = st.steps;
+    out[1] = st.max_concurrent;
+    out[2] = st.admission_waits;
+    out[3] = st.batches;
+  });
+}
+
+int irismpc_serving_n_rows(const irismpc_serving* s, size_t* n) {
+  return guarded([&] { *n = s->impl->n_rows(); });
+}
+
+int irismpc_serving_run_alone(irismpc_serving* s, const uint16_t* code_shares, const uint16_t* mask_shares,
+                              uint32_t* matches, size_t max_matches, size_t* n_matches) {
+  return guarded([&] {
+    IRIS_REQUIRE(code_shares && mask_shares, "request without shares");
+    copy_matches(s->impl->run_alone(code_shares, mask_shares), matches, max_matches, n_matches);
+  });
+}
+
+int irismpc_serving_time_step(irismpc_serving* s, size_t batch, size_t n_chunks, int reps, double* seconds) {
+  return guarded([&] { *seconds = s->impl->time_step(batch, n_chunks, reps); });
+}
+
+int irismpc_serving_set_step_table(irismpc_serving* s, const uint32_t* buckets, const uint32_t* chunks,
+                                   size_t n) {
+  return guarded([&] { s->impl->set_step_table(buckets, chunks, n); });
+}
+
+int irismpc_serving_tune_steps(irismpc_serving* s, const uint32_t* buckets, size_t n_buckets,
+                               const uint32_t* ks, size_t n_ks, double tol, double max_step_s,
+                               uint32_t* chunks_out, double* seconds_per_chunk_out) {
+  return guarded([&] {
+    Serving& sv = *s->impl;
+    const size_t n_chunks = Serving::n_chunks_of(sv.n_rows());
+    std::vector<uint32_t> bk, ch;
+    for (size_t b = 0; b < n_buckets; b++) {
+      chunks_out[b] = 0;
+      for (size_t j = 0; j < n_ks; j++) seconds_per_chunk_out[b * n_ks + j] = -1;
+      if (buckets[b] < 1 || buckets[b] > sv.max_active()) continue;
+      std::vector<uint32_t> kk;
+      std::vector<double> tt;
+      for (size_t j = 0; j < n_ks; j++)
+        if (ks[j] >= 1 && (size_t)ks[j] * buckets[b] <= sv.max_active() && ks[j] <= n_chunks &&
+            ks[j] <= kMaxStepChunks) {
+          const double t = sv.time_step(buckets[b], ks[j], 2) / ks[j];
+          seconds_per_chunk_out[b * n_ks + j] = t;
+          kk.push_back(ks[j]);
+          tt.push_back(t);
+        }
+      if (kk.empty()) {
+        kk.push_back(1);
+        tt.push_back(sv.time_step(buckets[b], 1, 2));
+      }
+      chunks_out[b] = pick_step_chunks(kk.data(), tt.data(), kk.size(), tol, max_step_s);
+      bk.push_back(buckets[b]);
+      ch.push_back(chunks_out[b]);
+    }
+    sv.set_step_table(bk.data(), ch.data(), bk.size());
+  });
+}
+
+int irismpc_serving_chunks_per_step(const irismpc_serving* s, size_t batch, size_t chunk_index,
+                                    size_t n_chunks, size_t* k) {
+  return guarded([&] { *k = s->impl->chunks_per_step(batch, chunk_index, n_chunks); });
+}
+
+int irismpc_serving_pick_step(const uint32_t* ks, const double* seconds_per_chunk, size_t n, double tol,
+                              double max_step_s, uint32_t* k) {
+  return guarded([&] { *k = pick_step_chunks(ks, seconds_per_chunk, n, tol, max_step_s); });
 }
 
 }  // extern "C"

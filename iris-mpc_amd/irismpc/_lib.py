@@ -22,6 +22,7 @@ u16 = ctypes.c_uint16
 u32 = ctypes.c_uint32
 u64 = ctypes.c_uint64
 i32 = ctypes.c_int
+i64 = ctypes.c_int64
 dbl = ctypes.c_double
 PP = ctypes.POINTER(ctypes.c_void_p)
 
@@ -118,6 +119,30 @@ SIGNATURES = {
     "irismpc_scan_subset_trio": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, sz, sz, u32, u32, vp, vp],
     "irismpc_comparator_set_anon_capture": [vp, vp, vp, vp, vp, vp, vp, sz, u64, sz, sz, vp, sz],
     "irismpc_comparator_buffers": [vp, vp, vp, vp, vp, vp, vp],
+    "irismpc_batch_plan_packs": [vp, sz, sz, sz, vp, vp],
+    "irismpc_batch_counter_advance": [sz, sz, sz, vp],
+    "irismpc_batch_scan_counter_advance": [sz, sz, sz, vp],
+    "irismpc_batch_runner_create": [vp, vp, sz, u16, vp, sz, vp, vp, sz, sz, sz, vp, vp],
+    "irismpc_batch_runner_destroy": [vp],
+    "irismpc_batch_runner_replan": [vp, sz, vp],
+    "irismpc_batch_runner_plan": [vp, vp, vp],
+    "irismpc_batch_runner_issue_packed": [vp, u64, sz, vp],
+    "irismpc_batch_runner_issue_plain": [vp, sz, vp],
+    "irismpc_serving_create": [i32, vp, vp, vp, sz, sz, sz, u32, vp],
+    "irismpc_serving_destroy": [vp],
+    "irismpc_serving_start": [vp, i32, sz, sz],
+    "irismpc_serving_finish": [vp, dbl],
+    "irismpc_serving_submit": [vp, i32, vp, vp, i64, dbl, vp],
+    "irismpc_serving_result": [vp, u64, vp, vp, vp, sz, vp, vp, vp, vp, sz],
+    "irismpc_serving_now": [vp, vp],
+    "irismpc_serving_stats": [vp, vp],
+    "irismpc_serving_n_rows": [vp, vp],
+    "irismpc_serving_run_alone": [vp, vp, vp, vp, sz, vp],
+    "irismpc_serving_time_step": [vp, sz, sz, i32, vp],
+    "irismpc_serving_set_step_table": [vp, vp, vp, sz],
+    "irismpc_serving_tune_steps": [vp, vp, sz, vp, sz, dbl, dbl, vp, vp],
+    "irismpc_serving_chunks_per_step": [vp, sz, sz, sz, vp],
+    "irismpc_serving_pick_step": [vp, vp, sz, dbl, dbl, vp],
 }
 
 

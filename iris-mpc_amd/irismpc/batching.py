@@ -16,6 +16,7 @@ The whole packed schedule is captured once into a HIP graph and replayed.
 The Autotuner times the GEMM variants and the pack size on the device and
 caches its choices.
 """
+import ctypes
 import json
 import math
 import os
@@ -23,7 +24,8 @@ import time
 
 import numpy as np
 
-from .engine import Graph, gemm_rot_variants
+from ._lib import call
+from .engine import Graph, gemm_rot_variants, stream_synchronize
 
 ROTATIONS = 31
 
@@ -34,35 +36,52 @@ def log_uniform_batch_sizes(n, lo=1, hi=256, seed=0):
     return np.clip(np.floor(np.exp(rng.uniform(math.log(lo), math.log(hi + 1), n))), lo, hi).astype(int)
 
 
-def plan_packs(sizes, max_queries, max_batches=64):
-    """Greedy packing of consecutive batches: a pack holds at most max_queries
-    queries (a larger batch gets a pack of its own) and at most max_batches."""
-    packs, cur, q = [], [], 0
-    for i, b in enumerate(sizes):
-        if cur and (q + b > max_queries or len(cur) == max_batches):
-            packs.append(cur)
-            cur, q = [], 0
-        cur.append(i)
-        q += int(b)
-    if cur:
-        packs.append(cur)
+def _packs(pack_of, n_packs):
+    packs = [[] for _ in range(n_packs)]
+    for i, p in enumerate(pack_of):
+        packs[int(p)].append(i)
     return packs
+
+
+def plan_packs(sizes, max_queries, max_batches=64):
+    """Greedy packing of consecutive batches (the library's planner): a pack
+    holds at most max_queries queries (a larger batch gets a pack of its own)
+    and at most max_batches. Returns lists of batch indices."""
+    sz = np.ascontiguousarray(sizes, dtype=np.uint32)
+    pack_of = np.zeros(max(1, sz.size), np.uint32)
+    n = ctypes.c_size_t()
+    call("irismpc_batch_plan_packs", sz.ctypes.data, sz.size, max_queries, max_batches, pack_of.ctypes.data,
+         ctypes.byref(n))
+    return _packs(pack_of[:sz.size], n.value)
 
 
 def counter_advance(total, chunk, rows):
     """ChaCha blocks a remote engine's dot_reduce_range uses over `total` rows
     (mask + reshare OTP per chunk)."""
-    n, c0 = 0, 0
-    while c0 < total:
-        n += 2 * -(-(min(chunk, total - c0) * rows * 2) // 64)
-        c0 += chunk
-    return n
+    v = ctypes.c_uint64()
+    call("irismpc_batch_counter_advance", total, chunk, rows, ctypes.byref(v))
+    return v.value
 
 
 def range_advance(n_rows, chunk, rows):
     """ChaCha blocks of a whole scan of n_rows DB rows, launched one chunk at a
     time with the last chunk padded to 64 rows (as the runners launch it)."""
-    return sum(counter_advance(tot, chunk, rows) for _, tot in _ranges(n_rows, chunk))
+    v = ctypes.c_uint64()
+    call("irismpc_batch_scan_counter_advance", n_rows, chunk, rows, ctypes.byref(v))
+    return v.value
+
+
+def pick_step_chunks(per_chunk, tol=0.03, max_step=None):
+    """Chunks per scan step from measured seconds per chunk {k: s} (the
+    library's rule): steps longer than max_step seconds are left out (when
+    every step is longer, the smallest k); of the rest, the smallest k within
+    `tol` of the fastest per chunk."""
+    ks = sorted(per_chunk)
+    k = (ctypes.c_uint32 * len(ks))(*ks)
+    t = (ctypes.c_double * len(ks))(*[per_chunk[x] for x in ks])
+    out = ctypes.c_uint32()
+    call("irismpc_serving_pick_step", k, t, len(ks), float(tol), float(max_step or 0.0), ctypes.byref(out))
+    return out.value
 
 
 class Matrix:
@@ -73,75 +92,63 @@ class Matrix:
         self.engine, self.db, self.n_rows, self.mult, self.shares = engine, db, n_rows, multiplier, shares
 
 
-def _ranges(n_rows, chunk):
-    off = 0
-    while off < n_rows:
-        yield off, -(-min(chunk, n_rows - off) // 64) * 64
-        off += chunk
-
-
 class BatchRunner:
-    """Plain and packed runs of the same batches over one or more matrices;
-    each batch's last chunk of results stays in its own output region."""
+    """Plain and packed runs of the same batches over one or more matrices,
+    one native runner (irismpc_batch_runner_*) per matrix; each batch's last
+    chunk of results stays in its own output region."""
 
     def __init__(self, matrices, sizes, chunk, stream, max_queries=256, max_batches=64):
         import torch
 
         self.m, self.sizes, self.chunk, self.stream = matrices, [int(b) for b in sizes], chunk, stream
-        self.plan = plan_packs(self.sizes, max_queries, max_batches)
         self.outs = [[torch.zeros(chunk * b * ROTATIONS + 64, dtype=torch.int16, device="cuda")
                       for b in self.sizes] for _ in matrices]
-        self.q_plain = [[m.engine.upload_query_shares_rows(m.shares[i], b * ROTATIONS)
-                         for i, b in enumerate(self.sizes)] for m in matrices]
-        self.max_batches = max_batches
-        self.q_packed = []
-        self.replan(max_queries)
+        torch.cuda.synchronize()  # zeroed before the library's stream writes them
+        sz = np.asarray(self.sizes, np.uint32)
+        self.h = []
+        for k, m in enumerate(matrices):
+            shares = np.ascontiguousarray(np.concatenate([np.asarray(x, np.uint16) for x in m.shares]))
+            outs = (ctypes.c_void_p * len(self.sizes))(*[o.data_ptr() for o in self.outs[k]])
+            h = ctypes.c_void_p()
+            call("irismpc_batch_runner_create", m.engine.h, m.db.h, m.n_rows, m.mult, sz.ctypes.data, sz.size,
+                 shares.ctypes.data, outs, chunk, max_queries, max_batches, stream, ctypes.byref(h))
+            self.h.append(h.value)
+        self.max_queries, self.graph = max_queries, None
+        self.plan = self._plan()
+
+    def __del__(self):
+        for h in getattr(self, "h", []):
+            call("irismpc_batch_runner_destroy", h)
+        self.h = []
+
+    def _plan(self):
+        pack_of, n = np.zeros(len(self.sizes), np.uint32), ctypes.c_size_t()
+        call("irismpc_batch_runner_plan", self.h[0], pack_of.ctypes.data, ctypes.byref(n))
+        return _packs(pack_of, n.value)
 
     def replan(self, max_queries):
         """New pack size: rebuilds the plan and the packed query sets (the
         output regions and the plain query sets stay)."""
-        self.plan = plan_packs(self.sizes, max_queries, self.max_batches)
-        self.max_queries = max_queries
-        self.q_packed, self.graph = [], None  # old sets are freed before the new ones exist
-        for m in self.m:
-            sets = []
-            for pack in self.plan:
-                raw = np.concatenate([m.shares[i] for i in pack])
-                sets.append(m.engine.upload_query_shares_rows(raw, raw.shape[0] * ROTATIONS))
-            self.q_packed.append(sets)
+        self.graph = None
+        for h in self.h:
+            call("irismpc_batch_runner_replan", h, max_queries, self.stream)
+        self.max_queries, self.plan = max_queries, self._plan()
 
     def comparisons(self):
         return sum(b * ROTATIONS for b in self.sizes) * self.m[0].n_rows
 
     def run_plain(self, ctr0=0):
         """One batch at a time, each to completion (stream synchronised)."""
-        from .engine import stream_synchronize
-
-        for k, m in enumerate(self.m):
+        for m in self.m:
             m.engine.counter = ctr0
         for i in range(len(self.sizes)):
-            for k, m in enumerate(self.m):
-                for off, tot in _ranges(m.n_rows, self.chunk):
-                    m.engine.dot_reduce_range(self.q_plain[k][i], m.db, off, tot, self.chunk, m.mult,
-                                              self.outs[k][i].data_ptr(), self.stream)
+            for h in self.h:
+                call("irismpc_batch_runner_issue_plain", h, i, self.stream)
             stream_synchronize(self.stream)
 
     def _issue_packed(self, ctr0, n_rows=None):
-        for k, m in enumerate(self.m):
-            n = m.n_rows if n_rows is None else n_rows
-            base, c = [], ctr0
-            for b in self.sizes:  # the plain form's counter at the start of each batch
-                base.append(c)
-                if m.engine.is_remote:
-                    c += range_advance(n, self.chunk, b * ROTATIONS)
-            for p, pack in enumerate(self.plan):
-                rows = [self.sizes[i] * ROTATIONS for i in pack]
-                outs = [self.outs[k][i].data_ptr() for i in pack]
-                for off, tot in _ranges(n, self.chunk):
-                    ctr = [base[i] + (counter_advance(off, self.chunk, r) if m.engine.is_remote else 0)
-                           for i, r in zip(pack, rows)]
-                    m.engine.dot_reduce_packed(self.q_packed[k][p], m.db, off, tot, self.chunk, m.mult,
-                                               rows, outs, ctr, self.stream)
+        for h in self.h:
+            call("irismpc_batch_runner_issue_packed", h, ctr0, n_rows or 0, self.stream)
 
     def capture(self, ctr0=0):
         """Captures the packed schedule into a HIP graph."""
@@ -155,8 +162,6 @@ class BatchRunner:
     def run_packed(self, ctr0=0, graph=True, n_rows=None):
         """The packed schedule (replayed from its graph); n_rows limits the scan
         to the first n_rows DB rows (no graph)."""
-        from .engine import stream_synchronize
-
         if graph and n_rows is None:
             if self.graph is None:
                 self.capture(ctr0)
@@ -164,19 +169,6 @@ class BatchRunner:
         else:
             self._issue_packed(ctr0, n_rows)
         stream_synchronize(self.stream)
-
-
-def pick_step_chunks(per_chunk, tol=0.03, max_step=None):
-    """Chunks per scan step from measured seconds per chunk {k: s}. Steps
-    longer than max_step seconds are left out (a request arriving meanwhile
-    waits up to a whole step to join the active set; when every step is
-    longer, the smallest k). Of the rest, the smallest k within `tol` of the
-    fastest per chunk: a larger step pays only by what it saves per chunk."""
-    fits = {k: t for k, t in per_chunk.items() if max_step is None or k * t <= max_step}
-    if not fits:
-        return min(per_chunk)
-    best = min(fits.values())
-    return min(k for k, t in fits.items() if t <= best * (1 + tol))
 
 
 class Autotuner:
@@ -269,25 +261,24 @@ class Autotuner:
     def step_chunks(self, eng, buckets=(1, 2, 4, 8, 16, 32, 64), ks=(1, 2, 4, 8, 16), tol=0.03,
                     max_step_s=0.008):
         """Chunks per scan step of the serving form (`eng`: ServingEngines) for
-        each bucket of active queries: every k that the result buffers and the
-        DB allow is timed as one step of that many queries, and
-        pick_step_chunks chooses (steps of at most max_step_s seconds). A
-        bucket's choice serves active counts from it up to the next bucket.
-        Returns {bucket: k}."""
-        n_chunks = eng.n_chunks(eng.n_rows)
+        each bucket of active queries: the library times one step of that many
+        queries for every k that the result buffers and the DB allow and picks
+        (pick_step_chunks; steps of at most max_step_s seconds). A bucket's
+        choice serves active counts from it up to the next bucket. Returns
+        {bucket: k}."""
         key = (f"{self._device()}/steps/rows={eng.n_rows}/max_active={eng.max_active}"
                f"/b={','.join(map(str, buckets))}/k={','.join(map(str, ks))}/tol={tol}/max={max_step_s}")
         if key in self.cache:
             return {int(b): k for b, k in self.cache[key]["chunks"].items()}
-        table, times = {}, {}
-        for b in buckets:
-            if b > eng.max_active:
-                break
-            cands = [k for k in ks if k * b <= eng.max_active and k <= n_chunks] or [1]
-            per = {k: eng.time_step(b, k) / k for k in cands}
-            table[b] = pick_step_chunks(per, tol, max_step_s)
-            times[str(b)] = {str(k): v for k, v in per.items()}
-        self.cache[key] = {"chunks": {str(b): k for b, k in table.items()}, "seconds_per_chunk": times}
+        nb, nk = len(buckets), len(ks)
+        b = (ctypes.c_uint32 * nb)(*buckets)
+        k = (ctypes.c_uint32 * nk)(*ks)
+        chunks, secs = (ctypes.c_uint32 * nb)(), (ctypes.c_double * (nb * nk))()
+        call("irismpc_serving_tune_steps", eng.h, b, nb, k, nk, float(tol), float(max_step_s), chunks, secs)
+        table = {int(buckets[i]): int(chunks[i]) for i in range(nb) if chunks[i]}
+        times = {str(buckets[i]): {str(ks[j]): secs[i * nk + j] for j in range(nk) if secs[i * nk + j] >= 0}
+                 for i in range(nb) if chunks[i]}
+        self.cache[key] = {"chunks": {str(x): y for x, y in table.items()}, "seconds_per_chunk": times}
         self.log.append((key, times))
         self._save()
         return table

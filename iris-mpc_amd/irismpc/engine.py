@@ -49,6 +49,10 @@ def stream_synchronize(stream):
     call("irismpc_stream_synchronize", stream)
 
 
+def stream_destroy(stream):
+    call("irismpc_stream_destroy", stream)
+
+
 def download(ptr, n, dtype, stream=0):
     """Copies n elements of dtype from a device pointer to a new numpy array."""
     out = np.zeros(n, dtype)

@@ -99,7 +99,8 @@ def test_c4_batching_fullsize(irismpc, orc):
             # chunk 0 (its own launch at the plain schedule's counter) and the last chunk (the plain run's)
             tmp = torch.zeros(chunk * qr + 64, dtype=torch.int16, device="cuda")
             m.engine.counter = int(base[i])
-            m.engine.dot_reduce_range(runner.q_plain[k][i], m.db, 0, chunk, chunk, m.mult, tmp.data_ptr(), stream)
+            q = m.engine.upload_query_shares_rows(m.shares[i], sizes[i] * 31)
+            m.engine.dot_reduce_range(q, m.db, 0, chunk, chunk, m.mult, tmp.data_ptr(), stream)
             irismpc.stream_synchronize(stream)
             for c0, got_t, ctr in ((0, tmp, int(base[i])),
                                    (last0, plain[k][i], int(base[i]) + counter_advance(last0, chunk, qr))):

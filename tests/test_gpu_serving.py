@@ -187,13 +187,69 @@ def test_default_rids_do_not_alias(served, orc):
 
 def test_short_bitmap_rows_rejected(served):
     """A scan step whose bitmap rows are shorter than the DB is refused with an
-    error before anything is launched."""
+    error before anything is launched (the served DB, engines of its own)."""
+    import torch
+
     eng, serving, synth = served
     from irismpc import engine as E
 
     sh, _, _ = _query(synth, 71, 3, 80, 1500, 80)
-    qc, qm = eng._query_sets(eng.scan, [sh], 31)
+    ex = E.Exchange(timeout_s=60)
+    base = [np.full(8, 40 + p, np.uint32) for p in range(3)]
+    parties, qc, qm, streams = [], [], [], []
+    for p in range(3):
+        seeds = (E.derive_seed(base[p], 0), E.derive_seed(base[(p + 2) % 3], 0))
+        codes = E.ShareDB(p, 32768, 31, E.IRIS_CODE_LENGTH, seeds, comm=ex, tag=0)
+        masks = E.ShareDB(p, 32768, 31, E.MASK_CODE_LENGTH, seeds, comm=ex, tag=1)
+        circ = E.Circuits(p, 31 * 32768, 31 * 32768 // 64, seeds, ex, tag=2)
+        parties.append(E.PartyEngines(codes, masks, circ, E.DistanceComparator(31, eng.capacity),
+                                      eng.code_dbs[p], eng.mask_dbs[p]))
+        qc.append(codes.upload_query_shares(sh[p][0][None]))
+        qm.append(masks.upload_query_shares(sh[p][1][None]))
+        streams.append(E.stream_create())
+    n = eng.n_rows
+    bitmaps = [torch.zeros(n // 64 + 2, dtype=torch.int64, device="cuda") for _ in range(3)]
+    slots = [torch.zeros(1, dtype=torch.int32, device="cuda") for _ in range(3)]
+    torch.cuda.synchronize()
     with pytest.raises(RuntimeError, match="shorter than the DB"):
-        E.scan_chunks_trio(eng.scan_parties, qc, qm, 0, 1, eng.n_rows, 1, eng.A,
-                           [b.data_ptr() for b in eng.bitmaps], eng.scan.streams,
-                           [m.data_ptr() for m in eng.slot_maps], row_bits=eng.n_rows - 64)
+        E.scan_chunks_trio(parties, qc, qm, 0, 1, n, 1, eng.A, [b.data_ptr() for b in bitmaps], streams,
+                           [m.data_ptr() for m in slots], row_bits=n - 64)
+    for s in streams:
+        E.stream_destroy(s)
+
+
+def test_native_loop_rejects_bad_requests(served):
+    """Requests the loop cannot serve fail with their reason (the run raises
+    after every request finished); the loop itself keeps serving."""
+    eng, serving, synth = served
+    R = serving.Request
+    sh, _, _ = _query(synth, 73, 200, 0, 0, 200)
+    n = eng.n_rows
+    bad = [R("reauth", 0.0, shares=sh, target=n + 5, rid=1), R("delete", 0.0, target=-3, rid=2)]
+    with pytest.raises(RuntimeError, match="2 request\\(s\\) failed"):
+        serving.Server(eng, reserve_bytes=0).run(bad)
+    assert "outside the DB" in bad[0].error and "outside the DB" in bad[1].error
+    ok = serving.Server(eng, reserve_bytes=0).run([R("uniqueness", 0.0, shares=sh, rid=3)])
+    assert ok[0].error is None and ok[0].result == eng.run_alone(sh)
+
+
+def test_per_batch_form_matches_serving_form(served):
+    """The reference's per-batch form (batches padded to max_batch queries) on
+    the same engines gives every request the result of the serving form."""
+    eng, serving, synth = served
+    R = serving.Request
+    reqs = []
+    for k in range(5):
+        sh, _, _ = _query(synth, 71, 300 + k, 90 + k, 1500, 90 + k)
+        reqs.append(R("uniqueness", 0.001 * k, shares=sh, rid=k))
+    sh, _, _ = _query(synth, 71, 302, 99, 1500, 99)
+    reqs.append(R("reauth", 0.002, shares=sh, target=302 + U, rid=10))
+    srv = serving.PerBatchServer(eng, 4)
+    out = {r.rid: r for r in srv.run([R(r.kind, r.arrival, shares=r.shares, target=r.target, rid=r.rid)
+                                      for r in reqs])}
+    assert srv.batches >= 2  # six requests, at most four per batch
+    ref = {r.rid: r for r in serving.Server(eng, reserve_bytes=0).run(reqs)}
+    for k in range(5):
+        assert out[k].result == ref[k].result == eng.run_alone(reqs[k].shares), f"uniqueness {k}"
+        assert out[k].result, "planted copies match"
+    assert out[10].result is True and ref[10].result is True

@@ -50,22 +50,6 @@ def test_batch_query_retain():
     assert r.left[0][0].tolist() == [[0, 1], [4, 5]] and r.deletion_requests_indices == [9]
 
 
-def test_bitmap_rows_sparse_decode():
-    import numpy as np
-
-    from irismpc.serving import bitmap_rows
-
-    rng = np.random.default_rng(8)
-    w = np.zeros(1000, np.int64)
-    w[rng.choice(1000, 20, replace=False)] = rng.integers(-(1 << 62), 1 << 62, 20)
-    w[3] = -1  # all 64 bits, sign bit included
-    n = 64 * 1000 - 37
-    want = np.nonzero(np.unpackbits(w.view(np.uint8), bitorder="little")[:n])[0].tolist()
-    assert bitmap_rows(w, n) == want
-    assert bitmap_rows(np.zeros(4, np.int64), 256) == []
-    assert bitmap_rows(w, 64 * 3 + 5) == list(range(192, 197))
-
-
 def test_or_policy_bitmap_known_answers():
     """iris-mpc-gpu/tests/or_policy.rs:8-33: rows of ceil(8000 / 64) = 125
     words; query 0's ids 1, 5, 10 -> word 0 = 1058, query 1 none -> word 125 = 0,
