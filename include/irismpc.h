@@ -574,9 +574,13 @@ int irismpc_serving_result(const irismpc_serving* s, uint64_t id, int* state, in
                            uint32_t* matches, size_t max_matches, size_t* n_matches, double* times,
                            size_t* db_rows, char* error, size_t error_len);
 int irismpc_serving_now(const irismpc_serving* s, double* seconds);
-/* out[4]: chunk steps, largest active set, admissions deferred by the reserve,
- * per-batch batches (since start). */
+/* out[5]: chunk steps, largest active set, admissions deferred by the reserve,
+ * per-batch batches, query-set refills (since start). */
 int irismpc_serving_stats(const irismpc_serving* s, uint64_t* out);
+/* out[5]: host seconds of the loop thread since start in query-set refills,
+ * waits for the queued steps (at most two stay queued), queueing steps,
+ * free-HBM checks and bitmap decoding. */
+int irismpc_serving_loop_times(const irismpc_serving* s, double* out);
 int irismpc_serving_n_rows(const irismpc_serving* s, size_t* n);
 /* Loop stopped: one uniqueness query alone over the whole DB (the plain
  * per-request form); matches as in irismpc_serving_result. */
@@ -587,6 +591,13 @@ int irismpc_serving_run_alone(irismpc_serving* s, const uint16_t* code_shares,
  * chunks (best of reps after a warm-up). */
 int irismpc_serving_time_step(irismpc_serving* s, size_t batch, size_t n_chunks, int reps,
                               double* seconds);
+/* Loop stopped: admission of the continuous policy. Requests join while fewer
+ * than active_cap are active, and every waiting request joins (slots and HBM
+ * allowing) while more than queue_threshold wait. Default: active_cap =
+ * max_active (every request a slot allows). A request's latency is the DB's
+ * chunks times the step time of the active set, so a small cap keeps it short;
+ * the threshold buys throughput back when a queue builds up. */
+int irismpc_serving_set_admission(irismpc_serving* s, size_t active_cap, size_t queue_threshold);
 /* Loop stopped: chunks per step for active counts from buckets[i] up to the
  * next bucket (n = 0: the fixed rule 8 / 4 / 2 / 1 chunks for 1 / 2 / <= 8 /
  * more active queries). */

@@ -925,6 +925,18 @@ int irismpc_serving_stats(const irismpc_serving* s, uint64_t* out) {
     out[1] = st.max_concurrent;
     out[2] = st.admission_waits;
     out[3] = st.batches;
+    out[4] = st.refills;
+  });
+}
+
+int irismpc_serving_loop_times(const irismpc_serving* s, double* out) {
+  return guarded([&] {
+    const ServingStats st = s->impl->stats();
+    out[0] = st.t_refill;
+    out[1] = st.t_wait;
+    out[2] = st.t_enqueue;
+    out[3] = st.t_admit;
+    out[4] = st.t_decode;
   });
 }
 
@@ -942,6 +954,10 @@ int irismpc_serving_run_alone(irismpc_serving* s, const uint16_t* code_shares, c
 
 int irismpc_serving_time_step(irismpc_serving* s, size_t batch, size_t n_chunks, int reps, double* seconds) {
   return guarded([&] { *seconds = s->impl->time_step(batch, n_chunks, reps); });
+}
+
+int irismpc_serving_set_admission(irismpc_serving* s, size_t active_cap, size_t queue_threshold) {
+  return guarded([&] { s->impl->set_admission(active_cap, queue_threshold); });
 }
 
 int irismpc_serving_set_step_table(irismpc_serving* s, const uint32_t* buckets, const uint32_t* chunks,

@@ -400,7 +400,7 @@ void ShareDB::refill_query_shares(QuerySet& q, const uint16_t* host_shares, size
 }
 
 void ShareDB::refill_query_shares_async(QuerySet& q, const uint16_t* host_shares, size_t batch,
-                                        size_t n_rows, hipStream_t s) const {
+                                        size_t n_rows, hipStream_t s, bool with_db_form) const {
   DeviceGuard g(device_);
   if (n_rows == 0) n_rows = query_length_;
   IRIS_REQUIRE(q.code_length == code_length_, "query set of another code length");
@@ -414,8 +414,9 @@ void ShareDB::refill_query_shares_async(QuerySet& q, const uint16_t* host_shares
   launch_query_prep(qraw_.ptr, (int)batch, code_length_, peer_id_, qrows_.ptr, (int)n_rows, s, 0);
   launch_tile_rows(qrows_.ptr, n_rows, code_length_, q.limbs.ptr, 0, q.sum0.ptr, q.sum1.ptr, s);
   if (q.rot.ptr) launch_rot_planes(qrows_.ptr, (int)n_rows, code_length_, q.rot.ptr, s);
-  launch_tile_rows(qraw_.ptr, batch, code_length_, q.insert->limbs.ptr, 0, q.insert->sum0.ptr,
-                   q.insert->sum1.ptr, s);
+  if (with_db_form)
+    launch_tile_rows(qraw_.ptr, batch, code_length_, q.insert->limbs.ptr, 0, q.insert->sum0.ptr,
+                     q.insert->sum1.ptr, s);
   q.rows = (int)n_rows;
   q.batch = (int)batch;
 }

@@ -205,8 +205,11 @@ class ShareDB {
   // (the staging buffers are reused in stream order), and host_shares
   // (pinned host memory) must stay unchanged until the stream has run the
   // copy.
+  // with_db_form = false skips the set's DB form (the batch's irises as DB
+  // rows, read only by write_at_index and the batch self-comparison), which
+  // then goes stale.
   void refill_query_shares_async(QuerySet& q, const uint16_t* host_shares, size_t batch,
-                                 size_t n_rows, hipStream_t s) const;
+                                 size_t n_rows, hipStream_t s, bool with_db_form = true) const;
   // dot + dot_reduce_and_multiply (fused): DB rows [offset, offset + chunk)
   // against all query rows; chunk is the padded chunk size (multiple of 64).
   // results land at results() + out_offset (elements): several chunks of a

@@ -35,6 +35,19 @@ ChaChaKey derive(const ChaChaKey& k, uint64_t nonce) {
   return o;
 }
 
+// Adds the seconds between construction and destruction to *acc (under mu).
+struct PhaseTimer {
+  double* acc;
+  std::mutex* mu;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  PhaseTimer(double* a, std::mutex* m) : acc(a), mu(m) {}
+  ~PhaseTimer() {
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::lock_guard<std::mutex> lk(*mu);
+    *acc += dt;
+  }
+};
+
 bool event_done(hipEvent_t e) {
   const hipError_t r = hipEventQuery(e);
   if (r == hipSuccess) return true;
@@ -105,6 +118,7 @@ Serving::Serving(int device, const ChaChaKey party_seeds[3], DbSlice* const code
       threshold_a_(threshold_a),
       n_rows_(n_rows) {
   IRIS_REQUIRE(max_active >= 1 && max_active <= 64, "1 .. 64 active queries");
+  active_cap_ = max_active;
   IRIS_REQUIRE(capacity > (size_t)kDbChunk, "DB capacity must exceed one chunk");
   IRIS_REQUIRE(n_rows <= capacity - kDbChunk, "DB rows beyond the capacity");
   for (int p = 0; p < 3; p++) {
@@ -499,8 +513,10 @@ void Serving::serve_reauth(Req* r) {
   std::memcpy(small_mask_.ptr, r->mask.data(), 3 * kMaskLen * 2);
   for (int p = 0; p < 3; p++) {
     hipStream_t s = small_->s[p];
-    small_->codes[p]->refill_query_shares_async(*small_->qc[p], small_code_.ptr + p * kCodeLen, 1, kRotations, s);
-    small_->masks[p]->refill_query_shares_async(*small_->qm[p], small_mask_.ptr + p * kMaskLen, 1, kRotations, s);
+    small_->codes[p]->refill_query_shares_async(*small_->qc[p], small_code_.ptr + p * kCodeLen, 1, kRotations, s,
+                                                false);
+    small_->masks[p]->refill_query_shares_async(*small_->qm[p], small_mask_.ptr + p * kMaskLen, 1, kRotations, s,
+                                                false);
     fill_u64(small_bitmaps_[p].ptr, words_ + 2, 0, s);
   }
   uint64_t word = 0;
@@ -520,6 +536,11 @@ void Serving::serve_reauth(Req* r) {
 // ---------------------------------------------------------------- device steps
 void Serving::refill_scan(const std::vector<const Req*>& reqs, size_t batch, size_t rows) {
   IRIS_REQUIRE(batch >= 1 && batch <= max_active_ && reqs.size() <= batch, "query set beyond the slots");
+  PhaseTimer pt(&stats_.t_refill, &mu_);
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stats_.refills++;
+  }
   const int k = refill_next_;
   refill_next_ = (refill_next_ + 1) % kRefillRing;
   for (auto e : refill_ev_[k]) {
@@ -538,8 +559,9 @@ void Serving::refill_scan(const std::vector<const Req*>& reqs, size_t batch, siz
       std::memcpy(bm + j * kMaskLen, m + p * kMaskLen, kMaskLen * 2);
     }
     hipStream_t s = scan_->s[p];
-    scan_->codes[p]->refill_query_shares_async(*scan_->qc[p], bc, batch, rows, s);
-    scan_->masks[p]->refill_query_shares_async(*scan_->qm[p], bm, batch, rows, s);
+    // the scan reads only the query rows: no DB form of the active set
+    scan_->codes[p]->refill_query_shares_async(*scan_->qc[p], bc, batch, rows, s, false);
+    scan_->masks[p]->refill_query_shares_async(*scan_->qm[p], bm, batch, rows, s, false);
     hipEvent_t e = event();
     IRIS_HIP_CHECK(hipEventRecord(e, s));
     refill_ev_[k].push_back(e);
@@ -575,13 +597,17 @@ void Serving::clear_slot(int slot) {
 void Serving::step(size_t batch, size_t chunk_index, size_t n_chunks, size_t db_rows) {
   IRIS_REQUIRE(batch >= 1 && batch * n_chunks <= max_active_ && n_chunks <= kMaxStepChunks,
                "step beyond the result buffers");
-  while (inflight_.size() >= kMaxQueuedSteps) {
-    for (auto e : inflight_.front().e) {
-      IRIS_HIP_CHECK(hipEventSynchronize(e));
-      recycle(e);
+  {
+    PhaseTimer pt(&stats_.t_wait, &mu_);
+    while (inflight_.size() >= kMaxQueuedSteps) {
+      for (auto e : inflight_.front().e) {
+        IRIS_HIP_CHECK(hipEventSynchronize(e));
+        recycle(e);
+      }
+      inflight_.pop_front();
     }
-    inflight_.pop_front();
   }
+  PhaseTimer pt(&stats_.t_enqueue, &mu_);
   uint64_t* const bms[3] = {bitmaps_[0].ptr, bitmaps_[1].ptr, bitmaps_[2].ptr};
   const uint32_t* const so[3] = {slot_maps_[0].ptr, slot_maps_[1].ptr, slot_maps_[2].ptr};
   scan_chunks_trio(scan_p_, chunk_index * kDbChunk, n_chunks, db_rows, batch, threshold_a_, bms, so,
@@ -628,6 +654,13 @@ std::vector<uint32_t> Serving::readout_rows(int slot, size_t db_rows) const {
 }
 
 // ---------------------------------------------------------------- step sizes
+void Serving::set_admission(size_t active_cap, size_t queue_threshold) {
+  IRIS_REQUIRE(!running_, "admission changes only while the request loop is stopped");
+  IRIS_REQUIRE(active_cap >= 1, "an active cap of at least one request");
+  active_cap_ = std::min(active_cap, max_active_);
+  queue_threshold_ = queue_threshold;
+}
+
 void Serving::set_step_table(const uint32_t* buckets, const uint32_t* chunks, size_t n) {
   IRIS_REQUIRE(!running_, "step sizes change only while the request loop is stopped");
   std::vector<std::pair<uint32_t, uint32_t>> t;
@@ -721,7 +754,10 @@ void Serving::loop_continuous(size_t reserve_bytes) {
     while (!reading.empty() && event_done(reading.front()->readout)) {
       Req* r = reading.front();
       reading.pop_front();
-      r->res.matches = readout_rows(r->slot, r->res.db_rows);
+      {
+        PhaseTimer pt(&stats_.t_decode, &mu_);
+        r->res.matches = readout_rows(r->slot, r->res.db_rows);
+      }
       r->res.value = (int64_t)r->res.matches.size();
       recycle(r->readout);
       r->readout = nullptr;
@@ -743,11 +779,16 @@ void Serving::loop_continuous(size_t reserve_bytes) {
       }
     }
     if ((stopping && in.empty() && idle) || abort_) break;
-    // admission between chunk steps
-    while (!waiting.empty() && !free_slots.empty()) {
+    // admission between chunk steps: up to active_cap requests, or every
+    // waiting one while more than queue_threshold wait (slots and HBM allowing)
+    while (!waiting.empty() && !free_slots.empty() &&
+           (active.size() < active_cap_ || waiting.size() > queue_threshold_)) {
       if (!active.empty()) {  // with nothing active one request always joins
         size_t free = 0, total = 0;
-        IRIS_HIP_CHECK(hipMemGetInfo(&free, &total));
+        {
+          PhaseTimer pt(&stats_.t_admit, &mu_);
+          IRIS_HIP_CHECK(hipMemGetInfo(&free, &total));
+        }
         if (free < reserve_bytes + kRequestFootprint * (active.size() + 1)) {
           std::lock_guard<std::mutex> lk(mu_);
           stats_.admission_waits++;

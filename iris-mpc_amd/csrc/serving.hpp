@@ -100,6 +100,11 @@ struct ServingStats {
   uint64_t max_concurrent = 0;   // largest active set (serving form)
   uint64_t admission_waits = 0;  // admissions deferred by the free-HBM reserve
   uint64_t batches = 0;          // batches run (per-batch form)
+  uint64_t refills = 0;          // query-set refills (active-set changes)
+  // host seconds of the loop thread: refills (staging and queueing), waits
+  // for the queued steps (at most two stay queued), queueing the steps,
+  // free-HBM checks, bitmap decoding
+  double t_refill = 0, t_wait = 0, t_enqueue = 0, t_admit = 0, t_decode = 0;
 };
 
 class Serving {
@@ -154,6 +159,14 @@ class Serving {
   // chunks from the ring's start, best of `reps` after one warm-up (loop
   // stopped; the slots it writes are cleared when requests take them).
   double time_step(size_t batch, size_t n_chunks, int reps);
+  // Admission of the serving form: requests join while fewer than
+  // active_cap are active, and all waiting ones join (up to the slots) while
+  // more than queue_threshold wait. The default (active_cap = max_active)
+  // admits every request a slot allows. A small cap keeps each active
+  // request's steps short (its latency is the DB's chunks times the step
+  // time of the active set); the threshold trades that for throughput when a
+  // queue builds up.
+  void set_admission(size_t active_cap, size_t queue_threshold);
   // Step sizes: table[i] chunks per step for active counts from buckets[i]
   // up to the next bucket (n = 0 clears the table: a fixed rule applies).
   void set_step_table(const uint32_t* buckets, const uint32_t* chunks, size_t n);
@@ -229,6 +242,7 @@ class Serving {
   std::deque<StepEvents> inflight_;
   std::vector<hipEvent_t> event_pool_;
   std::vector<uint32_t> step_buckets_, step_chunks_;
+  size_t active_cap_ = 64, queue_threshold_ = 0;
 
   // requests and threads
   std::chrono::steady_clock::time_point t0_;

@@ -105,32 +105,59 @@ __global__ void tile_records_kernel(const uint8_t* __restrict__ rec, size_t n, s
   memcpy(dst + frag_offset(R / 16, kt, 1, n_kt) + lane * 16, hi, 16);
 }
 
-// One thread per row: sums of the signed limbs (u32 wrapping).
-__global__ void row_sums_kernel(const int8_t* __restrict__ tiled, size_t row0, size_t n_rows,
-                                int K, uint32_t* __restrict__ s0, uint32_t* __restrict__ s1) {
-  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_rows) return;
-  const size_t R = row0 + r;
+// Sums of the signed limbs (u32 wrapping) of rows [row0, row0 + n_rows):
+// one block of four waves per 16-row group. Lane l of a wave holds row l % 16
+// in every fragment; wave w reads the fragments of k-tiles w, w + 4, ... (one
+// coalesced 1 KiB load per fragment), then the four lanes of a row and the
+// four waves are summed. Whole groups are read (buffers are padded to 16-row
+// groups); only the rows in range are written.
+__global__ void __launch_bounds__(256) row_sums_kernel(const int8_t* __restrict__ tiled, size_t row0,
+                                                       size_t n_rows, int K, uint32_t* __restrict__ s0,
+                                                       uint32_t* __restrict__ s1) {
+  const size_t g = row0 / 16 + blockIdx.x;
+  const int w = (int)threadIdx.x / 64, l = (int)threadIdx.x % 64;
   const int n_kt = K / kKTile;
-  int acc0 = 0, acc1 = 0;
-  for (int kt = 0; kt < n_kt; kt++) {
+  int a0 = 0, a1 = 0;
+#pragma unroll 4
+  for (int kt = w; kt < n_kt; kt += 4) {
+    const uint4 v0 = *reinterpret_cast<const uint4*>(tiled + frag_offset(g, kt, 0, n_kt) + l * 16);
+    const uint4 v1 = *reinterpret_cast<const uint4*>(tiled + frag_offset(g, kt, 1, n_kt) + l * 16);
+    const uint32_t x0[4] = {v0.x, v0.y, v0.z, v0.w}, x1[4] = {v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
-    for (int g = 0; g < 4; g++) {
-      const int lane = (int)(R % 16) + 16 * g;
-      const int8_t* p0 = tiled + frag_offset(R / 16, kt, 0, n_kt) + lane * 16;
-      const int8_t* p1 = tiled + frag_offset(R / 16, kt, 1, n_kt) + lane * 16;
-      int8_t b0[16], b1[16];
-      *reinterpret_cast<uint4*>(b0) = *reinterpret_cast<const uint4*>(p0);
-      *reinterpret_cast<uint4*>(b1) = *reinterpret_cast<const uint4*>(p1);
+    for (int i = 0; i < 4; i++) {
 #pragma unroll
-      for (int i = 0; i < 16; i++) {
-        acc0 += b0[i];
-        acc1 += b1[i];
+      for (int b = 0; b < 32; b += 8) {
+        a0 += (int)(int8_t)(x0[i] >> b);
+        a1 += (int)(int8_t)(x1[i] >> b);
       }
     }
   }
-  s0[R] = (uint32_t)acc0;
-  s1[R] = (uint32_t)acc1;
+  // lanes l, l + 16, l + 32, l + 48 hold row l % 16
+  a0 += __shfl_xor(a0, 16);
+  a1 += __shfl_xor(a1, 16);
+  a0 += __shfl_xor(a0, 32);
+  a1 += __shfl_xor(a1, 32);
+  __shared__ int sm0[4][16], sm1[4][16];
+  if (l < 16) {
+    sm0[w][l] = a0;
+    sm1[w][l] = a1;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const size_t R = g * 16 + threadIdx.x;
+    if (R >= row0 && R < row0 + n_rows) {
+      s0[R] = (uint32_t)(sm0[0][threadIdx.x] + sm0[1][threadIdx.x] + sm0[2][threadIdx.x] + sm0[3][threadIdx.x]);
+      s1[R] = (uint32_t)(sm1[0][threadIdx.x] + sm1[1][threadIdx.x] + sm1[2][threadIdx.x] + sm1[3][threadIdx.x]);
+    }
+  }
+}
+
+static void launch_row_sums(const int8_t* tiled, size_t row0, size_t n_rows, int K, uint32_t* s0,
+                            uint32_t* s1, hipStream_t s) {
+  if (n_rows == 0) return;
+  const size_t groups = (row0 + n_rows - 1) / 16 - row0 / 16 + 1;
+  row_sums_kernel<<<(unsigned)groups, 256, 0, s>>>(tiled, row0, n_rows, K, s0, s1);
+  IRIS_HIP_CHECK(hipGetLastError());
 }
 
 // Row sums of the rows written by tile_records_kernel (one thread per record).
@@ -257,8 +284,7 @@ void launch_fill_random(int8_t* dst, size_t row0, size_t n_rows, int K, const Ch
   const size_t nb = ceil_div(n_rows, 16) * group_bytes;
   fill_random_kernel<<<(unsigned)ceil_div(nb / 64, 256), 256, 0, s>>>(dst, b0, nb, key, nonce);
   IRIS_HIP_CHECK(hipGetLastError());
-  row_sums_kernel<<<(unsigned)ceil_div(n_rows, 256), 256, 0, s>>>(dst, row0, n_rows, K, sum0, sum1);
-  IRIS_HIP_CHECK(hipGetLastError());
+  launch_row_sums(dst, row0, n_rows, K, sum0, sum1, s);
 }
 
 void launch_tile_rows(const uint16_t* src, size_t n_rows, int K, int8_t* dst, size_t row0,
@@ -267,8 +293,7 @@ void launch_tile_rows(const uint16_t* src, size_t n_rows, int K, int8_t* dst, si
   size_t n = n_rows * (size_t)(K / 16);
   tile_rows_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(src, n_rows, K, dst, row0);
   IRIS_HIP_CHECK(hipGetLastError());
-  row_sums_kernel<<<(unsigned)ceil_div(n_rows, 256), 256, 0, s>>>(dst, row0, n_rows, K, sum0, sum1);
-  IRIS_HIP_CHECK(hipGetLastError());
+  launch_row_sums(dst, row0, n_rows, K, sum0, sum1, s);
 }
 
 void launch_tile_limbs(const uint8_t* l0, const uint8_t* l1, size_t n_rows, int K, int8_t* dst,
@@ -277,8 +302,7 @@ void launch_tile_limbs(const uint8_t* l0, const uint8_t* l1, size_t n_rows, int 
   size_t n = n_rows * (size_t)(K / 16);
   tile_limbs_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, s>>>(l0, l1, n_rows, K, dst, row0);
   IRIS_HIP_CHECK(hipGetLastError());
-  row_sums_kernel<<<(unsigned)ceil_div(n_rows, 256), 256, 0, s>>>(dst, row0, n_rows, K, sum0, sum1);
-  IRIS_HIP_CHECK(hipGetLastError());
+  launch_row_sums(dst, row0, n_rows, K, sum0, sum1, s);
 }
 
 void launch_tile_records(const uint8_t* records, size_t n, size_t stride, size_t off_lo,

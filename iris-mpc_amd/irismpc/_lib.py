@@ -136,9 +136,11 @@ SIGNATURES = {
     "irismpc_serving_result": [vp, u64, vp, vp, vp, sz, vp, vp, vp, vp, sz],
     "irismpc_serving_now": [vp, vp],
     "irismpc_serving_stats": [vp, vp],
+    "irismpc_serving_loop_times": [vp, vp],
     "irismpc_serving_n_rows": [vp, vp],
     "irismpc_serving_run_alone": [vp, vp, vp, vp, sz, vp],
     "irismpc_serving_time_step": [vp, sz, sz, i32, vp],
+    "irismpc_serving_set_admission": [vp, sz, sz],
     "irismpc_serving_set_step_table": [vp, vp, vp, sz],
     "irismpc_serving_tune_steps": [vp, vp, sz, vp, sz, dbl, dbl, vp, vp],
     "irismpc_serving_chunks_per_step": [vp, sz, sz, sz, vp],

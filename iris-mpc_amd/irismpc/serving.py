@@ -134,6 +134,11 @@ class ServingEngines:
         call("irismpc_serving_set_step_table", self.h, b, k, len(items))
         self._table = dict(items) if items else None
 
+    def set_admission(self, active_cap=None, queue_threshold=0):
+        """Serving-form admission: up to active_cap active requests, all waiting
+        ones while more than queue_threshold wait (None: every free slot)."""
+        call("irismpc_serving_set_admission", self.h, active_cap or self.max_active, queue_threshold)
+
     def time_step(self, batch, n_chunks, reps=2):
         """Seconds of one scan step of `batch` default-iris queries over n_chunks chunks."""
         t = ctypes.c_double()
@@ -205,9 +210,14 @@ class ServingEngines:
             r.result = int(value.value)
 
     def stats(self):
-        out = (ctypes.c_uint64 * 4)()
+        out = (ctypes.c_uint64 * 5)()
         call("irismpc_serving_stats", self.h, out)
-        return {"steps": out[0], "max_concurrent": out[1], "admission_waits": out[2], "batches": out[3]}
+        t = (ctypes.c_double * 5)()
+        call("irismpc_serving_loop_times", self.h, t)
+        return {"steps": out[0], "max_concurrent": out[1], "admission_waits": out[2], "batches": out[3],
+                "refills": out[4], "loop_seconds": {"refill": t[0], "wait_queued_steps": t[1],
+                                                    "enqueue_steps": t[2], "free_hbm_checks": t[3],
+                                                    "decode": t[4]}}
 
 
 class _Loop:
@@ -240,7 +250,7 @@ class _Loop:
             eng.fetch(i, r, t0)
             if r.error:
                 self.failed.append(r)
-        st = eng.stats()
+        st = self.stats = eng.stats()
         self.steps, self.max_concurrent = st["steps"], st["max_concurrent"]
         self.admission_waits, self.batches = st["admission_waits"], st["batches"]
         if self.failed:
