@@ -842,7 +842,7 @@ def main():
                          "threshold circuits alone at one chunk; c3-two-stage: the reference's batch flow "
                          "(one-eye prefilter scan, both eyes on the candidates); matmul: the reference's "
                          "own GEMM benchmark shape (2.4M x 930 rows, code only, query upload included)")
-    ap.add_argument("--rates", default="10,20,40,60,80", help="c5: request rates per second")
+    ap.add_argument("--rates", default="10,20,40,50,60,70,80", help="c5: request rates per second")
     ap.add_argument("--seconds", type=float, default=6.0, help="c5: arrival window per rate")
     ap.add_argument("--plain-requests", type=int, default=8, help="c5: requests timed in the plain form")
     ap.add_argument("--latency-rate", type=float, default=40.0,
