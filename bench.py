@@ -381,6 +381,7 @@ def run_c3(args):
                             "peak_Tops": VALU_PEAK_TOPS,
                             "frac": chacha_ops / (circ_ms / 1e3) / 1e12 / VALU_PEAK_TOPS if circ_ms else None,
                             "hbm_GBps": circ_bytes / (circ_ms / 1e3) / 1e9 if circ_ms else None,
+                            "hbm_frac": circ_bytes / (circ_ms / 1e3) / 1e9 / HBM_PEAK_GBS if circ_ms else None,
                             "note": f"{CHACHA_BLOCKS_PER_PAIR:.3f} distinct ChaCha12 blocks x {CHACHA12_OPS} "
                                     f"int32 ops and {CIRCUIT_BYTES_PER_PAIR:.1f} HBM bytes per pair per party; "
                                     "one compare = lift + injection + MSB adder of one chunk, three parties, "
@@ -404,6 +405,130 @@ def run_c3(args):
         progress("CPU baseline")
     line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline_protocol()
     return line
+
+
+def cpu_baseline_c1():
+    """The CPU-runnable configuration at full size on the oracle: the three
+    parties' rotation-aware code and mask dot shares (the reference CPU dot
+    loop) of 32 queries x 31 rotations against 4,096 DB rows (2.34e11 u16 MACs)
+    and the 3-party threshold circuits over all 992 x 4,096 elements (split
+    over the threads, which ctypes calls release to). Shares are random: the
+    work does not depend on them."""
+    import threading
+
+    import numpy as np
+
+    from oracle import orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rng = np.random.default_rng(9)
+    N, B = 4096, 32
+    dc = [rng.integers(0, 1 << 16, (N, 12800), dtype=np.uint16) for _ in range(3)]
+    dm = [rng.integers(0, 1 << 16, (N, 6400), dtype=np.uint16) for _ in range(3)]
+    qc = [rng.integers(0, 1 << 16, (B, 12800), dtype=np.uint16) for _ in range(3)]
+    qm = [rng.integers(0, 1 << 16, (B, 6400), dtype=np.uint16) for _ in range(3)]
+    L = B * 31 * N
+    words = L // 64
+    parts = max(1, min(threads, words // 8))
+    bounds = [(words * i // parts) * 64 for i in range(parts + 1)]
+    shares = [rng.integers(0, 1 << 16, L, dtype=np.uint16) for _ in range(4)]
+    so = np.array([[p] * 8 for p in range(3)], np.uint32)
+    sp = np.array([[(p + 2) % 3] * 8 for p in range(3)], np.uint32)
+
+    def circuits(lo, hi):
+        v = [x[lo:hi] for x in shares]
+        orc.threshold([v[0]] * 3, [v[1]] * 3, [v[2]] * 3, [v[3]] * 3, (hi - lo) // 64, 16384, so, sp)
+
+    t0 = time.perf_counter()
+    for p in range(3):
+        for j in range(B):
+            orc.cpu_rotation_aware_dot(qc[p][j], qm[p][j], dc[p], dm[p], threads=threads)
+    t_dot = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    ts = [threading.Thread(target=circuits, args=(bounds[i], bounds[i + 1])) for i in range(parts)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    t_circ = time.perf_counter() - t1
+    total = t_dot + t_circ
+    return {"value": B * 31 * N / total, "unit": "comparisons/s", "cores": threads, "kind": "port",
+            "sample": f"the whole configuration: 3 parties' code + mask dot shares of {B} queries x 31 "
+                      f"rotations vs {N} DB rows ({3 * B * 31 * N * 19200:.3g} u16 MACs) in {t_dot:.2f} s "
+                      f"and the 3-party circuits over {L} elements in {t_circ:.2f} s on {threads} threads"}
+
+
+def run_c1(args):
+    """The CPU-runnable configuration on the GPU: DB of 4,096 synthetic irises,
+    32 queries x 31 rotations, 8 of them copies of DB irises with exactly 1,920
+    code bits flipped, three parties (engine seeds derive_seed(([p; 8],
+    [p - 1; 8]), nonce 0 / 1 / 5)); one batch = the three parties' scan (one
+    chunk). The planted targets and the expected 0.375 bitmap are the committed
+    golden vectors, which every timed batch must reproduce."""
+    import numpy as np
+    import torch
+
+    import irismpc
+    from irismpc import synth
+
+    torch.cuda.set_device(0)
+    golden = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "config1.npz"))
+    N, B, A, FLIPS = 4096, 32, 16384, 1920
+    Qr, chunk = B * 31, irismpc.DB_CHUNK_SIZE
+    planted = [(j, int(t), FLIPS) for j, t in zip(range(0, B, 4), golden["targets"])]
+    ex = irismpc.Exchange(timeout_s=120)
+    parties, bitmaps, streams = [], [], []
+    for p in range(3):
+        def seed(n):
+            return (irismpc.derive_seed(np.full(8, p, np.uint32), n),
+                    irismpc.derive_seed(np.full(8, (p + 2) % 3, np.uint32), n))
+
+        codes = irismpc.ShareDB(p, chunk, Qr, irismpc.IRIS_CODE_LENGTH, seed(0), comm=ex, tag=0)
+        masks = irismpc.ShareDB(p, chunk, Qr, irismpc.MASK_CODE_LENGTH, seed(1), comm=ex, tag=1)
+        circ = irismpc.Circuits(p, Qr * chunk, Qr * chunk // 64, seed(5), ex, tag=2)
+        parties.append(irismpc.PartyEngines(codes, masks, circ, irismpc.DistanceComparator(Qr, N),
+                                            codes.alloc_db(N), masks.alloc_db(N)))
+        bitmaps.append(torch.zeros(N * B // 64 + 2, dtype=torch.int64, device="cuda"))
+        streams.append(irismpc.stream_create())
+    synth.build_db([pe.code_db for pe in parties], [pe.mask_db for pe in parties], N, N, 1, 30)
+    _, _, shares = synth.make_queries(B, planted, 1, 21, 11, 31)
+    qcs = [parties[p].codes.upload_query_shares(shares[p][0]) for p in range(3)]
+    qms = [parties[p].masks.upload_query_shares(shares[p][1]) for p in range(3)]
+    torch.cuda.synchronize()
+    want = golden["bitmap"]
+
+    def batch():
+        for b, s in zip(bitmaps, streams):
+            irismpc.fill_u64(b.data_ptr(), b.numel(), 0, s)
+        irismpc.scan_db_trio(parties, qcs, qms, N, B, A, [b.data_ptr() for b in bitmaps], streams)
+        for s in streams:
+            irismpc.stream_synchronize(s)
+
+    for _ in range(args.warmup):
+        batch()
+    exact = True
+    t = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        batch()
+        t.append(time.perf_counter() - t0)
+        exact = exact and all(np.array_equal(b.cpu().numpy().view(np.uint64)[:N * B // 64], want) for b in bitmaps)
+    ms = float(np.median(t)) * 1e3
+    return {
+        "metric": "3-party iris comparisons/sec on 1 MI355X + p50/p99 request latency",
+        "value": Qr * N / (ms / 1e3),
+        "unit": "comparisons/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8 MFMA (i32 acc) -> u16 shares; u32 lift; GF(2) bit-sliced circuits",
+        "data": "synthetic irises GR-shared for 3 parties (the golden-vector configuration)",
+        "config": {"workload": "CPU-runnable configuration: 3 parties, DB 4,096 rows, 32 queries x 31 "
+                               "rotations (8 copies with 1,920 flips), one eye; median batch of the three "
+                               "parties' scan, host synchronisation included",
+                   "db_rows": N, "query_rows": Qr, "bitmaps_equal_golden_vectors": bool(exact)},
+        "roofline": None,
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_c1(),
+    }
 
 
 def run_c3_two_stage(args):
@@ -833,10 +958,11 @@ def main():
     ap.add_argument("--variant", type=int, default=-1,
                     help="GEMM kernel variant (-1: automatic, the rotation-aware kernel)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("c2", "c3", "c3-two-stage", "c4", "c5", "circuits", "matmul"),
+    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c3-two-stage", "c4", "c5", "circuits", "matmul"),
                     default="c3",
                     help="c3 (the headline line): full 3-party protocol, both eyes, plus a serving latency "
-                         "pass; c2: one party's share GEMM; c4: batching form (use --db 4194304 for the "
+                         "pass; c1: the CPU-runnable configuration (4,096 rows, 32 queries) against its "
+                         "golden vectors, with the oracle at full size as CPU baseline; c2: one party's share GEMM; c4: batching form (use --db 4194304 for the "
                          "full-size DB); "
                          "c5: serving form (use --db 2097152 for the full-size DB); circuits: the "
                          "threshold circuits alone at one chunk; c3-two-stage: the reference's batch flow "
@@ -857,6 +983,9 @@ def main():
 
     if args.workload == "c3":
         print(json.dumps(run_c3(args)))
+        return
+    if args.workload == "c1":
+        print(json.dumps(run_c1(args)))
         return
     if args.workload == "circuits":
         print(json.dumps(run_circuits(args)))
