@@ -194,6 +194,9 @@ def _c3_engines(args, extra_left=0):
                               irismpc.derive_seed(base[(p + 2) % 3], n + 8 * eye))
             codes = irismpc.ShareDB(p, chunk, Qr, irismpc.IRIS_CODE_LENGTH, seed(0), comm=ex, tag=0)
             masks = irismpc.ShareDB(p, chunk, Qr, irismpc.MASK_CODE_LENGTH, seed(1), comm=ex, tag=1)
+            if args.variant >= 0:
+                codes.set_variant(args.variant)
+                masks.set_variant(args.variant)
             circ = irismpc.Circuits(p, Qr * chunk, Qr * chunk // 64, seed(5), ex, tag=2)
             cmp = irismpc.DistanceComparator(Qr, N)
             cap = N + (extra_left if eye == 0 else 0)
@@ -347,7 +350,7 @@ def run_c3(args):
     circ_pairs = pairs * 3 * 2 * args.steps  # per party, summed over parties and eyes
     chacha_ops = CHACHA_BLOCKS_PER_PAIR * CHACHA12_OPS * circ_pairs
     circ_bytes = CIRCUIT_BYTES_PER_PAIR * circ_pairs
-    kernel = KERNEL_OF_VARIANT[100 if Qr > ROT_SMALL_ROWS else 102]
+    kernel = KERNEL_OF_VARIANT[args.variant if args.variant >= 100 else 100 if Qr > ROT_SMALL_ROWS else 102]
     line = {
         "metric": "3-party iris comparisons/sec on 1 MI355X + p50/p99 request latency",
         "value": 2 * pairs / (ms_step / 1e3),
