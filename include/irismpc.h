@@ -573,6 +573,10 @@ int irismpc_serving_submit(irismpc_serving* s, int kind, const uint16_t* code_sh
 int irismpc_serving_result(const irismpc_serving* s, uint64_t id, int* state, int64_t* value,
                            uint32_t* matches, size_t max_matches, size_t* n_matches, double* times,
                            size_t* db_rows, char* error, size_t error_len);
+/* Forgets a finished request (fails while it is pending). A finished
+ * request's shares are dropped at once; its result stays until released, so a
+ * long-running host releases every request it has read. */
+int irismpc_serving_release(irismpc_serving* s, uint64_t id);
 int irismpc_serving_now(const irismpc_serving* s, double* seconds);
 /* out[5]: chunk steps, largest active set, admissions deferred by the reserve,
  * per-batch batches, query-set refills (since start). */

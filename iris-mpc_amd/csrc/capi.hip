@@ -914,6 +914,10 @@ int irismpc_serving_result(const irismpc_serving* s, uint64_t id, int* state, in
   });
 }
 
+int irismpc_serving_release(irismpc_serving* s, uint64_t id) {
+  return guarded([&] { IRIS_REQUIRE(s->impl->release(id), "request still pending"); });
+}
+
 int irismpc_serving_now(const irismpc_serving* s, double* seconds) {
   return guarded([&] { *seconds = s->impl->now(); });
 }

@@ -204,6 +204,7 @@ Serving::~Serving() {
   if (small_) small_->sync();
   for (auto& st : inflight_)
     for (auto e : st.e) (void)hipEventDestroy(e);
+  for (auto e : recycle_later_) (void)hipEventDestroy(e);
   for (int k = 0; k < kRefillRing; k++)
     for (auto e : refill_ev_[k]) (void)hipEventDestroy(e);
   for (int k = 0; k < kSlotRing; k++)
@@ -282,14 +283,28 @@ bool Serving::result(uint64_t id, Result* out) const {
   return true;
 }
 
+// A finished request's shares are dropped at once (they were copied to the
+// staging buffers before its last step); its result stays until released.
 void Serving::complete(Req* r, int state, const std::string& err) {
   std::lock_guard<std::mutex> lk(mu_);
   if (r->res.state != kReqPending) return;
   r->res.done = now();
   if (!err.empty()) r->res.error = err;
   r->res.state = state;
+  std::vector<uint16_t>().swap(r->code);
+  std::vector<uint16_t>().swap(r->mask);
   n_finished_++;
   done_cv_.notify_all();
+}
+
+bool Serving::release(uint64_t id) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = reqs_.find(id);
+  IRIS_REQUIRE(it != reqs_.end(), "unknown request id");
+  if (it->second->res.state == kReqPending) return false;
+  if (it->second->readout) recycle_later_.push_back(it->second->readout);
+  reqs_.erase(it);
+  return true;
 }
 
 void Serving::fail_all(const std::string& msg) {
@@ -301,7 +316,7 @@ void Serving::fail_all(const std::string& msg) {
     res.state = kReqFailed;
     res.error = msg;
     res.done = t;
-    n_finished_++;
+    n_finished_++;  // the shares stay: a queue may still hold the request until the loop ends
   }
   done_cv_.notify_all();
 }

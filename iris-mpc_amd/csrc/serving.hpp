@@ -147,6 +147,10 @@ class Serving {
   };
   // false when the id is unknown
   bool result(uint64_t id, Result* out) const;
+  // Forgets a finished request (its result can no longer be read); false
+  // while it is pending. A serving process releases every request it has
+  // read, so the records do not grow without bound.
+  bool release(uint64_t id);
   double now() const;
   ServingStats stats() const;
   size_t n_rows() const;
@@ -241,6 +245,7 @@ class Serving {
   PinnedBuf<uint16_t> small_code_, small_mask_;
   std::deque<StepEvents> inflight_;
   std::vector<hipEvent_t> event_pool_;
+  std::vector<hipEvent_t> recycle_later_;  // events of released requests (destroyed at the end)
   std::vector<uint32_t> step_buckets_, step_chunks_;
   size_t active_cap_ = 64, queue_threshold_ = 0;
 

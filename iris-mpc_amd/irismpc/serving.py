@@ -248,6 +248,7 @@ class _Loop:
             eng.finish(self.timeout_s + max(0.0, reqs[-1].arrival if reqs else 0.0))
         for r, i in zip(reqs, ids):
             eng.fetch(i, r, t0)
+            call("irismpc_serving_release", eng.h, i)
             if r.error:
                 self.failed.append(r)
         st = self.stats = eng.stats()

@@ -253,3 +253,27 @@ def test_per_batch_form_matches_serving_form(served):
         assert out[k].result == ref[k].result == eng.run_alone(reqs[k].shares), f"uniqueness {k}"
         assert out[k].result, "planted copies match"
     assert out[10].result is True and ref[10].result is True
+
+
+def test_native_loop_records_and_release(served):
+    """Through the C ABI directly: a request submitted ahead of its arrival is
+    held until then; its result stays readable until released, and a pending
+    request cannot be released."""
+    import irismpc
+
+    eng, serving, synth = served
+    sh, _, _ = _query(synth, 73, 400, 0, 0, 400)
+    eng.start(serving.CONTINUOUS, 0, 0)
+    try:
+        t0 = eng.now()
+        i = eng.submit(serving.Request("uniqueness", 0.0, shares=sh), t0 + 0.3)
+        with pytest.raises(irismpc.IrisMpcError, match="still pending"):
+            serving.call("irismpc_serving_release", eng.h, i)
+    finally:
+        eng.finish(60)
+    r = serving.Request("uniqueness", 0.0, shares=sh)
+    eng.fetch(i, r, t0)
+    assert r.error is None and r.result == [] and r.done >= 0.3  # held until its arrival
+    serving.call("irismpc_serving_release", eng.h, i)
+    with pytest.raises(irismpc.IrisMpcError, match="unknown request id"):
+        eng.fetch(i, r, t0)
