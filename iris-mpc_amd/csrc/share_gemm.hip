@@ -772,13 +772,21 @@ __global__ __launch_bounds__(64 * NW, 1) void share_gemm_hyb_kernel(const GemmAr
 // DB bytes are in flight (6.0 against 5.5 TB/s for one query; a plain read
 // kernel reaches 6.2). Plane slots are allocated in groups of kRotQueries.
 constexpr int kRotPad = 60, kRotRow = 800 + 2 * kRotPad, kRotQueries = 4;
+// Bytes per query slot: the two limb planes of R 800-coefficient rows plus one
+// pad dword. A 32-row MFMA tile spans two queries; the lanes of rows
+// rho = 31 b + t read the dword at b * stride / 4 + c - t, which falls in bank
+// (c - rho) mod 32 only when stride / 4 = 1 (mod 32) -- without the pad
+// (stride / 4 = 0 mod 32) the row of one query and the row of the next with the
+// same rotation meet in one bank (measured: 43 % of the LDS cycles of the code
+// launch were bank conflicts).
+__host__ __device__ constexpr int rot_query_stride(int R) { return 2 * R * kRotRow + 4; }
 
 template <int TQ, int TD, int NW>
 struct RotLds {
   static constexpr int kMaxRows = kCodeLen / 800;  // 16 rows of 800 coefficients
   // whole 1 KiB DMA fragments: the plane copy may overrun the last plane
   static constexpr int kPlaneBytes =
-      (TQ * 2 * kMaxRows * kRotRow + kFragBytes - 1) / kFragBytes * kFragBytes;
+      (TQ * rot_query_stride(kMaxRows) + kFragBytes - 1) / kFragBytes * kFragBytes;
   union {
     v4i planes[kPlaneBytes / 16];
     uint16_t tile[32 * TQ * (NW * TD * 32)];
@@ -800,7 +808,7 @@ void share_gemm_rot_kernel(const GemmArgs a) {
   const int n_kt = a.n_kt, last = n_kt - 1;
   const int R = n_kt * kKTile / 800;          // 800-coefficient rows of the share
   const int plane = R * kRotRow;              // bytes per limb plane
-  const int qbytes = 2 * plane;               // bytes per query
+  const int qbytes = rot_query_stride(R);     // bytes per query slot
   {
     // 1 KiB per wave-wide LDS DMA, all in flight at once (the plane buffer
     // carries 1 KiB of slack for the last partial chunk)
@@ -997,14 +1005,14 @@ __global__ __launch_bounds__(256) void rot_planes_kernel(const uint16_t* rows, i
   const int src_row = b * kRotations + 15;
   uint16_t v = 0;
   if (src_row < qr) v = rows[(size_t)src_row * K + r * 800 + (i - kRotPad + 800) % 800];
-  int8_t* q = out + (size_t)b * 2 * R * kRotRow + (size_t)r * kRotRow + i;
+  int8_t* q = out + (size_t)b * rot_query_stride(R) + (size_t)r * kRotRow + i;
   q[0] = (int8_t)((v & 0xff) ^ 0x80);
   q[(size_t)R * kRotRow] = (int8_t)((v >> 8) ^ 0x80);
 }
 
 size_t rot_planes_bytes(int qr, int K) {
   const size_t slots = ceil_div((size_t)qr, (size_t)(kRotQueries * kRotations)) * kRotQueries;
-  return slots * 2 * (size_t)(K / 800) * kRotRow + kFragBytes;
+  return slots * (size_t)rot_query_stride(K / 800) + kFragBytes;
 }
 
 void launch_rot_planes(const uint16_t* rows, int qr, int K, int8_t* out, hipStream_t s) {
