@@ -606,6 +606,7 @@ def run_c4(args):
     stream = irismpc.stream_create()
     torch.cuda.synchronize()
 
+    progress(f"{N:,} DB rows filled; autotuning GEMM variants and pack size")
     tuner = Autotuner()
     sample = min(N, 4 * chunk)
     probe = mats[0].engine.upload_query_shares_rows(mats[0].shares[int(np.argmax(sizes))], int(sizes.max()) * 31)
@@ -623,6 +624,7 @@ def run_c4(args):
     runner.capture()
     torch.cuda.synchronize()
 
+    progress("plain form: batch after batch")
     t0 = time.perf_counter()
     runner.run_plain()
     t_plain = time.perf_counter() - t0
@@ -631,10 +633,22 @@ def run_c4(args):
         for o in outs:
             o.zero_()
     torch.cuda.synchronize()
+    progress(f"plain form {t_plain:.1f} s; packed form, one launch per pack and chunk")
     t0 = time.perf_counter()
     runner.run_packed(graph=True)
     t_packed = time.perf_counter() - t0
     same = all(torch.equal(a, b) for pa, pb in zip(plain, runner.outs) for a, b in zip(pa, pb))
+    # the persistent form: one launch per chunk and matrix covering every pack
+    for outs in runner.outs:
+        for o in outs:
+            o.zero_()
+    torch.cuda.synchronize()
+    runner.capture(persistent=True)
+    progress(f"packed form {t_packed:.1f} s; persistent form, one launch per chunk")
+    t0 = time.perf_counter()
+    runner.run_packed(graph=True, persistent=True)
+    t_persistent = time.perf_counter() - t0
+    same_persistent = all(torch.equal(a, b) for pa, pb in zip(plain, runner.outs) for a, b in zip(pa, pb))
     comps = runner.comparisons()
     ops = 115200 * comps
     del plain
@@ -642,6 +656,7 @@ def run_c4(args):
     # max_batch_size x 31 (actor.rs:895-901, share_db.rs:800-845): every padded
     # batch does the same work, so one (the smallest batch, padded to 256
     # queries) is timed over the whole DB and the 200 batches take 200 x that
+    progress(f"persistent form {t_persistent:.1f} s; one padded batch")
     small = int(np.argmin(sizes))
     pad_rows = 256 * 31
     t_pad = float("inf")
@@ -674,6 +689,10 @@ def run_c4(args):
                    "db_rows": N, "batches": 200, "queries": int(sizes.sum()),
                    "packs": len(runner.plan), "max_queries_per_pack": max_q,
                    "gemm_variants": variants, "packed_equals_plain": same},
+        "packed_persistent": {"value": comps / t_persistent, "ms": t_persistent * 1e3,
+                              "form": "the same packs, one persistent launch per chunk and matrix (a block per CU "
+                                      "looping over every pack's tiles), one HIP graph",
+                              "equals_plain": same_persistent},
         "plain_form": {"value": comps / t_plain, "ms": t_plain * 1e3,
                        "form": "batch after batch, each batch's own rows (unpadded)"},
         "speedup_vs_plain": t_plain / t_packed,

@@ -523,6 +523,14 @@ int irismpc_batch_runner_issue_packed(irismpc_batch_runner* r, uint64_t ctr0, si
                                       void* stream);
 /* Batch `batch`'s plain run over the whole DB at the engine's counter. */
 int irismpc_batch_runner_issue_plain(irismpc_batch_runner* r, size_t batch, void* stream);
+/* Persistent form of the packed schedule: one launch per chunk covering every
+ * pack, a block per CU looping over the chunk's tiles (XCD-aware), with the
+ * outputs and counters of issue_packed(ctr0) over the whole DB. prepare builds
+ * the launch descriptors in device memory (allocates, synchronises); issue
+ * allocates and synchronises nothing and can be captured into a graph. At most
+ * 64 packs; every pack runs the 4-query rotation-aware tile. */
+int irismpc_batch_runner_prepare_persistent(irismpc_batch_runner* r, uint64_t ctr0, void* stream);
+int irismpc_batch_runner_issue_persistent(irismpc_batch_runner* r, void* stream);
 
 /* ------------------------------------------------ serving form */
 /* Request loops over the three parties' HBM-resident DB shares, with their own

@@ -92,6 +92,7 @@ void BatchRunner::replan(size_t max_queries, hipStream_t s) {
   plan_ = plan_packs(sizes_.data(), sizes_.size(), max_queries, max_batches_);
   max_queries_ = max_queries;
   q_packed_.clear();  // the old sets are freed before the new ones exist
+  n_ranges_ = 0;      // persistent descriptors point at the old sets
   const size_t K = (size_t)e_->code_length();
   for (const auto& pack : plan_) {
     // a pack's batches are consecutive: their shares sit back to back
@@ -130,6 +131,63 @@ void BatchRunner::issue_packed(uint64_t ctr0, size_t n_rows, hipStream_t s) {
                             (int)rows.size(), ctr.data(), s);
     }
   }
+}
+
+void BatchRunner::prepare_persistent(uint64_t ctr0, hipStream_t s) {
+  const bool remote = e_->is_remote();
+  IRIS_REQUIRE(plan_.size() <= (size_t)kMaxSegments, "at most 64 packs per persistent launch");
+  std::vector<uint64_t> base(sizes_.size());
+  uint64_t c = ctr0;
+  for (size_t i = 0; i < sizes_.size(); i++) {
+    base[i] = c;
+    if (remote) c += scan_counter_advance(n_rows_, chunk_, (size_t)sizes_[i] * kRotations);
+  }
+  const auto ranges = chunk_ranges(n_rows_, chunk_);
+  const size_t P = plan_.size();
+  std::vector<GemmArgs> host(ranges.size() * P);
+  std::vector<uint32_t> ends(ranges.size() * P);
+  std::vector<size_t> rows;
+  std::vector<uint16_t*> outs;
+  std::vector<uint64_t> ctr;
+  const GemmTile t = gemm_tile(kRotVariantBase);
+  for (size_t r = 0; r < ranges.size(); r++) {
+    uint64_t tiles = 0;
+    for (size_t p = 0; p < P; p++) {
+      rows.clear();
+      outs.clear();
+      ctr.clear();
+      for (size_t i : plan_[p]) {
+        rows.push_back((size_t)sizes_[i] * kRotations);
+        outs.push_back(outs_[i]);
+        ctr.push_back(base[i] + (remote ? ShareDB::range_counter_advance(ranges[r].first, chunk_, rows.back()) : 0));
+      }
+      IRIS_REQUIRE(q_packed_[p]->rot.ptr && (size_t)e_->code_length() % 800 == 0,
+                   "the persistent form needs rotation-structured query sets");
+      GemmArgs& a = host[r * P + p];
+      a = e_->packed_args(*q_packed_[p], *db_, ranges[r].first, ranges[r].second, chunk_, mult_, rows.data(),
+                          outs.data(), (int)rows.size(), ctr.data(), kRotVariantBase);
+      IRIS_REQUIRE(a.n_qtiles == (int)ceil_div((size_t)a.qr, (size_t)t.bq), "descriptor tile geometry");
+      tiles += (uint64_t)a.n_qtiles * (uint64_t)a.n_dtiles;
+      IRIS_REQUIRE(tiles < (1ull << 31), "too many tiles for one persistent launch");
+      ends[r * P + p] = (uint32_t)tiles;
+    }
+  }
+  int cus = 0;
+  IRIS_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e_->device()));
+  grid_ = (int)std::max<size_t>(8, round_up((size_t)std::max(cus, 8), 8));
+  descs_.alloc(host.size(), false);
+  tile_end_.alloc(ends.size(), false);
+  IRIS_HIP_CHECK(hipMemcpyAsync(descs_.ptr, host.data(), host.size() * sizeof(GemmArgs), hipMemcpyHostToDevice, s));
+  IRIS_HIP_CHECK(hipMemcpyAsync(tile_end_.ptr, ends.data(), ends.size() * 4, hipMemcpyHostToDevice, s));
+  IRIS_HIP_CHECK(hipStreamSynchronize(s));
+  n_ranges_ = ranges.size();
+}
+
+void BatchRunner::issue_persistent(hipStream_t s) {
+  IRIS_REQUIRE(n_ranges_ > 0, "persistent launches need prepare_persistent first");
+  const size_t P = plan_.size();
+  for (size_t r = 0; r < n_ranges_; r++)
+    launch_share_gemm_multi(descs_.ptr + r * P, tile_end_.ptr + r * P, (int)P, grid_, s);
 }
 
 void BatchRunner::issue_plain(size_t batch, hipStream_t s) {

@@ -53,6 +53,14 @@ class BatchRunner {
   // One batch's plain run over the whole DB at the engine's counter (which
   // it advances), chunk by chunk.
   void issue_plain(size_t batch, hipStream_t s);
+  // Persistent form of the packed schedule: one launch per chunk covering all
+  // packs (share_gemm_rot_multi_kernel, a block per CU looping over the
+  // chunk's tiles), with the outputs and counters of issue_packed(ctr0).
+  // prepare_persistent builds the launch descriptors in device memory
+  // (synchronises); issue_persistent allocates and synchronises nothing, so
+  // it can be captured into a graph. All packs run the 4-query tile.
+  void prepare_persistent(uint64_t ctr0, hipStream_t s);
+  void issue_persistent(hipStream_t s);
 
  private:
   ShareDB* e_;
@@ -65,6 +73,11 @@ class BatchRunner {
   std::vector<uint16_t*> outs_;
   std::vector<std::vector<size_t>> plan_;
   std::vector<std::unique_ptr<QuerySet>> q_plain_, q_packed_;
+  // persistent form: descriptors [chunk][pack] and running tile counts
+  DevBuf<GemmArgs> descs_;
+  DevBuf<uint32_t> tile_end_;
+  size_t n_ranges_ = 0;
+  int grid_ = 0;
 };
 
 }  // namespace irismpc

@@ -840,6 +840,20 @@ int irismpc_batch_runner_issue_packed(irismpc_batch_runner* r, uint64_t ctr0, si
   });
 }
 
+int irismpc_batch_runner_prepare_persistent(irismpc_batch_runner* r, uint64_t ctr0, void* stream) {
+  return guarded([&] {
+    DevSet g(r->device);
+    r->impl->prepare_persistent(ctr0, st(stream));
+  });
+}
+
+int irismpc_batch_runner_issue_persistent(irismpc_batch_runner* r, void* stream) {
+  return guarded([&] {
+    DevSet g(r->device);
+    r->impl->issue_persistent(st(stream));
+  });
+}
+
 int irismpc_batch_runner_issue_plain(irismpc_batch_runner* r, size_t batch, void* stream) {
   return guarded([&] {
     DevSet g(r->device);

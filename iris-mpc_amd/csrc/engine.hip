@@ -551,16 +551,15 @@ uint64_t ShareDB::range_counter_advance(size_t total, size_t chunk, size_t rows)
   return n;
 }
 
-void ShareDB::dot_reduce_packed(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
-                                size_t chunk, uint16_t multiplier, const size_t* rows,
-                                uint16_t* const* outs, int n_batches, const uint64_t* ctr_first,
-                                hipStream_t s) {
-  DeviceGuard g(device_);
+GemmArgs ShareDB::packed_args(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
+                              size_t chunk, uint16_t multiplier, const size_t* rows,
+                              uint16_t* const* outs, int n_batches, const uint64_t* ctr_first,
+                              int variant, uint64_t* ctr_end) const {
   IRIS_REQUIRE(q.code_length == code_length_ && db.code_length == code_length_, "code length mismatch");
   IRIS_REQUIRE(total > 0 && total % 64 == 0 && chunk % 256 == 0, "bad range geometry");
   IRIS_REQUIRE(offset % 16 == 0 && offset + total <= db.capacity - kMaxBD, "range beyond DB");
   IRIS_REQUIRE(n_batches >= 1 && n_batches <= kMaxSegments, "1 .. 64 batches per packed launch");
-  const int variant = resolve_variant(q);
+  IRIS_REQUIRE(!is_rot_variant(variant) || q.rot.ptr, "rotation-aware variant needs a query uploaded from raw shares");
   GemmArgs a = gemm_args(q, db, variant);
   const GemmTile t = gemm_tile(variant);
   size_t row = 0;
@@ -589,6 +588,19 @@ void ShareDB::dot_reduce_packed(const QuerySet& q, const DbSlice& db, size_t off
   a.key_prev = seed_prev_;
   a.mult = multiplier;
   a.masked = is_remote() ? 1 : 0;
+  if (ctr_end) *ctr_end = ctr;
+  return a;
+}
+
+void ShareDB::dot_reduce_packed(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
+                                size_t chunk, uint16_t multiplier, const size_t* rows,
+                                uint16_t* const* outs, int n_batches, const uint64_t* ctr_first,
+                                hipStream_t s) {
+  DeviceGuard g(device_);
+  const int variant = resolve_variant(q);
+  uint64_t ctr = 0;
+  GemmArgs a = packed_args(q, db, offset, total, chunk, multiplier, rows, outs, n_batches, ctr_first,
+                           variant, &ctr);
   launch(a, variant, s);
   if (!ctr_first) counter_ = ctr;
 }

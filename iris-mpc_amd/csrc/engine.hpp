@@ -236,6 +236,13 @@ class ShareDB {
                          size_t chunk, uint16_t multiplier, const size_t* rows,
                          uint16_t* const* outs, int n_batches, const uint64_t* ctr_first,
                          hipStream_t s);
+  // The launch arguments dot_reduce_packed would use with GEMM variant
+  // `variant` (no launch; the counter stays): for persistent launches that
+  // cover several packs. *ctr_end (optional) = the counter after the range.
+  GemmArgs packed_args(const QuerySet& q, const DbSlice& db, size_t offset, size_t total, size_t chunk,
+                       uint16_t multiplier, const size_t* rows, uint16_t* const* outs, int n_batches,
+                       const uint64_t* ctr_first, int variant, uint64_t* ctr_end = nullptr) const;
+  int device() const { return device_; }
   void reshare_results(size_t chunk, hipStream_t s);  // collective
   static void reshare_trio(ShareDB* const d[3], size_t chunk, hipStream_t const s[3]);
   uint16_t* results() const { return slot_ ? results_alt_.ptr : results_.ptr; }

@@ -391,7 +391,7 @@ __device__ __forceinline__ uint16_t corrected(const GemmArgs& a, int q, size_t d
 
 template <int BQ, int BD>
 __device__ __forceinline__ void epilogue_store(const GemmArgs& a, const uint16_t* tile, int q0,
-                                               size_t db0, int q_end);
+                                               size_t db0, int q_end, int tid, int nthreads);
 
 template <int WQ, int WD, int NWQ = 2, int NWD = 2>
 __device__ __forceinline__ void gemm_epilogue(const GemmArgs& a, uint16_t* tile,
@@ -412,17 +412,17 @@ __device__ __forceinline__ void gemm_epilogue(const GemmArgs& a, uint16_t* tile,
       }
     }
   }
-  epilogue_store<BQ, BD>(a, tile, q0, db0, a.qr);
+  epilogue_store<BQ, BD>(a, tile, q0, db0, a.qr, (int)threadIdx.x, (int)blockDim.x);
 }
 
 // Multiplier, ChaCha12 zero-sharing and 64-byte stores of the u16 tile; query
 // rows at or beyond q_end are not stored.
 template <int BQ, int BD>
 __device__ __forceinline__ void epilogue_store(const GemmArgs& a, const uint16_t* tile, int q0,
-                                               size_t db0, int q_end) {
+                                               size_t db0, int q_end, int tid, int nthreads) {
   __syncthreads();
   constexpr int SEGS = BD / 32;
-  for (int s = threadIdx.x; s < BQ * SEGS; s += blockDim.x) {
+  for (int s = tid; s < BQ * SEGS; s += nthreads) {
     const int ql = s / SEGS, seg = s % SEGS;
     const int q = q0 + ql;
     const size_t r = db0 + (size_t)seg * 32;
@@ -752,7 +752,7 @@ __global__ __launch_bounds__(64 * NW, 1) void share_gemm_hyb_kernel(const GemmAr
                                           (uint32_t)acc1[i][j][v]);
       }
     }
-  epilogue_store<BQ, BD>(a, sm.tile, q0, db0, a.qr);
+  epilogue_store<BQ, BD>(a, sm.tile, q0, db0, a.qr, (int)threadIdx.x, (int)blockDim.x);
 }
 
 
@@ -793,17 +793,17 @@ struct RotLds {
   };
 };
 
-template <int TQ, int TD, int NW, int WPE>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-void share_gemm_rot_kernel(const GemmArgs a) {
+// One block tile (query rows q0 .., DB rows db0 ..) of the rotation-aware
+// form: plane DMA, K loop, epilogue. The caller orders it after every earlier
+// use of `sm` by the block.
+template <int TQ, int TD, int NW, bool kWpe1>
+__device__ __forceinline__ void rot_tile(const GemmArgs& a, int q0, size_t db0, RotLds<TQ, TD, NW>& sm,
+                                         int tid) {
   constexpr int BQ = TQ * kRotations, BD = NW * TD * 32;  // 32 TQ rows >= 31 TQ
   constexpr int G = TQ * TD, HALF = 3 * TQ * TD;  // MFMAs per pass, per k-slice
   using S = RotLds<TQ, TD, NW>;
-  __shared__ S sm;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int q0;
-  size_t db0;
-  if (!block_tile<BQ, BD>(a, q0, db0)) return;
+  constexpr int WPE = kWpe1 ? 1 : 2;
+  const int lane = tid & 63, wave = tid >> 6;
 
   const int n_kt = a.n_kt, last = n_kt - 1;
   const int R = n_kt * kKTile / 800;          // 800-coefficient rows of the share
@@ -952,14 +952,14 @@ void share_gemm_rot_kernel(const GemmArgs a) {
   uint32_t* qterm = reinterpret_cast<uint32_t*>(&sm.tile[ROWS * BD]);
   static_assert(sizeof(S) >= ROWS * BD * 2 + ROWS * 4, "query terms fit behind the tile");
   static_assert(ROWS <= 64 * NW, "one thread per query term");
-  if (threadIdx.x < ROWS) {
-    const int q = q0 + (int)threadIdx.x;
+  if (tid < ROWS) {
+    const int q = q0 + tid;
     uint32_t t = 0;
     if (q < q_end) {
       const uint32_t s0 = a.qsum0[q], s1 = a.qsum1[q];
       t = (s0 << 7) + ((s0 + s1) << 15);
     }
-    qterm[threadIdx.x] = t;
+    qterm[tid] = t;
   }
   uint32_t dterm[TD];
 #pragma unroll
@@ -988,7 +988,51 @@ void share_gemm_rot_kernel(const GemmArgs a) {
         }
       }
     }
-  epilogue_store<ROWS, BD>(a, sm.tile, q0, db0, q_end);
+  epilogue_store<ROWS, BD>(a, sm.tile, q0, db0, q_end, tid, 64 * NW);
+}
+
+template <int TQ, int TD, int NW, int WPE>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+void share_gemm_rot_kernel(const GemmArgs a) {
+  constexpr int BQ = TQ * kRotations, BD = NW * TD * 32;
+  __shared__ RotLds<TQ, TD, NW> sm;
+  int q0;
+  size_t db0;
+  if (!block_tile<BQ, BD>(a, q0, db0)) return;
+  rot_tile<TQ, TD, NW, WPE == 1>(a, q0, db0, sm, (int)threadIdx.x);
+}
+
+// Persistent form over several launches' worth of tiles: descs[0 .. n_desc)
+// are the arguments of consecutive packed launches (the packs of one chunk),
+// tile_end[i] the running tile count through desc i. Each block loops over
+// the tiles t = x T8 + j, j + G/8, ... of its XCD's contiguous run (x = block
+// mod 8, G blocks), so the blocks sharing a DB tile run together on one XCD as
+// in the one-launch-per-pack form; every block leaves once past the run.
+template <int TQ, int TD, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void share_gemm_rot_multi_kernel(const GemmArgs* __restrict__ descs, const uint32_t* __restrict__ tile_end,
+                                 int n_desc) {
+  constexpr int BQ = TQ * kRotations, BD = NW * TD * 32;
+  __shared__ RotLds<TQ, TD, NW> sm;
+  const uint32_t T = tile_end[n_desc - 1];
+  const uint32_t T8 = (T + 7) / 8, x = blockIdx.x & 7, per_xcd = gridDim.x >> 3;
+  const uint32_t run_end = (x + 1) * T8 < T ? (x + 1) * T8 : T;
+  for (uint32_t t = x * T8 + (blockIdx.x >> 3); t < run_end; t += per_xcd) {
+    int d = 0;
+    while (d + 1 < n_desc && tile_end[d] <= t) d++;  // at most 64 descriptors
+    d = __builtin_amdgcn_readfirstlane(d);           // block-uniform: scalar loads of the descriptor
+    const GemmArgs& a = descs[d];
+    const uint32_t local = (uint32_t)__builtin_amdgcn_readfirstlane((int)(t - (d ? tile_end[d - 1] : 0u)));
+    const int q0 = (int)(local % (uint32_t)a.n_qtiles) * BQ;
+    const size_t db0 = a.db_begin + (size_t)(local / (uint32_t)a.n_qtiles) * BD;
+    __syncthreads();  // the previous tile's epilogue is done with the LDS
+    // the thread index, opaque per tile: keeps the compiler from hoisting the
+    // tile's per-lane address arithmetic out of the loop (it would stay live
+    // through the K loop and spill)
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    rot_tile<TQ, TD, NW, true>(a, q0, db0, sm, tid);
+  }
 }
 
 // Padded unrotated limb planes of every query slot from the rotated u16 rows
@@ -1032,6 +1076,14 @@ GemmTile gemm_tile(int variant) {
     case kRotVariantBase + 2: return {2 * kRotations, 128};  // two queries, 2 waves per SIMD
     default: throw Error("unknown share GEMM variant " + std::to_string(variant));
   }
+}
+
+void launch_share_gemm_multi(const GemmArgs* descs, const uint32_t* tile_end, int n_desc, int grid,
+                             hipStream_t s) {
+  IRIS_REQUIRE(n_desc >= 1 && n_desc <= kMaxSegments, "1 .. 64 launches per persistent launch");
+  IRIS_REQUIRE(grid >= 8 && grid % 8 == 0, "persistent grid: a multiple of 8 blocks");
+  share_gemm_rot_multi_kernel<4, 2, 4><<<(unsigned)grid, 256, 0, s>>>(descs, tile_end, n_desc);
+  IRIS_HIP_CHECK(hipGetLastError());
 }
 
 void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s) {

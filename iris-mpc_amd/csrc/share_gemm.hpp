@@ -100,6 +100,12 @@ constexpr int kMaxBQ = 256;   // query buffers are padded to this many rows
 constexpr int kMaxBD = 256;   // DB buffers are padded to this many rows
 GemmTile gemm_tile(int variant);
 void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s);
+// Persistent form of the rotation-aware 4-query kernel (variant
+// kRotVariantBase) over the tiles of n_desc launches: descs and tile_end
+// (running tile counts) in device memory, `grid` blocks looping until every
+// tile is done. No allocation or synchronisation: capturable.
+void launch_share_gemm_multi(const GemmArgs* descs, const uint32_t* tile_end, int n_desc, int grid,
+                             hipStream_t s);
 // Padded unrotated limb planes for the rotation-aware variants, built from the
 // rotated u16 query rows [qr][K].
 size_t rot_planes_bytes(int qr, int K);

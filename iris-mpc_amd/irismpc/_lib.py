@@ -128,6 +128,8 @@ SIGNATURES = {
     "irismpc_batch_runner_plan": [vp, vp, vp],
     "irismpc_batch_runner_issue_packed": [vp, u64, sz, vp],
     "irismpc_batch_runner_issue_plain": [vp, sz, vp],
+    "irismpc_batch_runner_prepare_persistent": [vp, u64, vp],
+    "irismpc_batch_runner_issue_persistent": [vp, vp],
     "irismpc_serving_create": [i32, vp, vp, vp, sz, sz, sz, u32, vp],
     "irismpc_serving_destroy": [vp],
     "irismpc_serving_start": [vp, i32, sz, sz],

@@ -113,7 +113,7 @@ class BatchRunner:
             call("irismpc_batch_runner_create", m.engine.h, m.db.h, m.n_rows, m.mult, sz.ctypes.data, sz.size,
                  shares.ctypes.data, outs, chunk, max_queries, max_batches, stream, ctypes.byref(h))
             self.h.append(h.value)
-        self.max_queries, self.graph = max_queries, None
+        self.max_queries, self.graph, self.graph_persistent, self.prepared = max_queries, None, False, None
         self.plan = self._plan()
 
     def __del__(self):
@@ -129,7 +129,7 @@ class BatchRunner:
     def replan(self, max_queries):
         """New pack size: rebuilds the plan and the packed query sets (the
         output regions and the plain query sets stay)."""
-        self.graph = None
+        self.graph, self.prepared = None, None
         for h in self.h:
             call("irismpc_batch_runner_replan", h, max_queries, self.stream)
         self.max_queries, self.plan = max_queries, self._plan()
@@ -146,28 +146,41 @@ class BatchRunner:
                 call("irismpc_batch_runner_issue_plain", h, i, self.stream)
             stream_synchronize(self.stream)
 
-    def _issue_packed(self, ctr0, n_rows=None):
+    def _issue_packed(self, ctr0, n_rows=None, persistent=False):
         for h in self.h:
-            call("irismpc_batch_runner_issue_packed", h, ctr0, n_rows or 0, self.stream)
+            if persistent:
+                call("irismpc_batch_runner_issue_persistent", h, self.stream)
+            else:
+                call("irismpc_batch_runner_issue_packed", h, ctr0, n_rows or 0, self.stream)
 
-    def capture(self, ctr0=0):
-        """Captures the packed schedule into a HIP graph."""
+    def _prepare(self, ctr0, persistent):
+        if persistent and self.prepared != ctr0:
+            for h in self.h:
+                call("irismpc_batch_runner_prepare_persistent", h, ctr0, self.stream)
+            self.prepared = ctr0
+
+    def capture(self, ctr0=0, persistent=False):
+        """Captures the packed schedule into a HIP graph (persistent: one
+        launch per chunk and matrix covering every pack)."""
+        self._prepare(ctr0, persistent)
         self.graph = Graph(self.stream).capture()
         try:
-            self._issue_packed(ctr0)
+            self._issue_packed(ctr0, persistent=persistent)
         finally:
             self.graph.end()
+        self.graph_persistent = persistent
         return self.graph
 
-    def run_packed(self, ctr0=0, graph=True, n_rows=None):
+    def run_packed(self, ctr0=0, graph=True, n_rows=None, persistent=False):
         """The packed schedule (replayed from its graph); n_rows limits the scan
-        to the first n_rows DB rows (no graph)."""
+        to the first n_rows DB rows (no graph, one launch per pack)."""
         if graph and n_rows is None:
-            if self.graph is None:
-                self.capture(ctr0)
+            if self.graph is None or self.graph_persistent != persistent:
+                self.capture(ctr0, persistent)
             self.graph.launch()
         else:
-            self._issue_packed(ctr0, n_rows)
+            self._prepare(ctr0, persistent and n_rows is None)
+            self._issue_packed(ctr0, n_rows, persistent=persistent and n_rows is None)
         stream_synchronize(self.stream)
 
 

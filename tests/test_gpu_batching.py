@@ -24,8 +24,8 @@ def irismpc():
     return m
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_packed_equals_plain(irismpc, orc, graph):
+@pytest.mark.parametrize("graph,persistent", [(False, False), (True, False), (False, True), (True, True)])
+def test_packed_equals_plain(irismpc, orc, graph, persistent):
     import torch
 
     from irismpc.batching import BatchRunner, Matrix
@@ -52,13 +52,13 @@ def test_packed_equals_plain(irismpc, orc, graph):
         for o in outs:
             o.zero_()
     torch.cuda.synchronize()  # zeroed on torch's stream before the library's stream writes
-    r.run_packed(ctr0=500, graph=graph)
+    r.run_packed(ctr0=500, graph=graph, persistent=persistent)
     if graph:  # a second replay gives the same shares again
         for outs in r.outs:
             for o in outs:
                 o.zero_()
         torch.cuda.synchronize()
-        r.run_packed(ctr0=500, graph=True)
+        r.run_packed(ctr0=500, graph=True, persistent=persistent)
     torch.cuda.synchronize()
     for k in range(2):
         for i in range(len(sizes)):
