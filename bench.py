@@ -72,6 +72,27 @@ def measured_traffic(kernel, path=TRAFFIC_FILE):
     return t["traffic_bytes_per_launch"]
 
 
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo; None where absent)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def with_ratio(line):
+    """Adds the host CPU model and the GPU / CPU ratio of the line's value to its cpu_baseline."""
+    cb = line.get("cpu_baseline")
+    if cb:
+        cb["cpu_model"] = cpu_model()
+        cb["gpu_over_cpu"] = line["value"] / cb["value"] if cb.get("value") else None
+    return line
+
+
 def progress(msg):
     """A progress line on stderr (stdout carries only the JSON result)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -103,7 +124,7 @@ def cpu_baseline(n_target_s=12.0):
         "kind": "port",
         "sample": f"{queries} queries x 31 rotations vs {n} DB share rows (code 12800 + mask 6400 "
                   f"u16) = {queries * 31 * n} comparisons in {elapsed:.1f} s on {threads} threads; "
-                  "same per-pair work as the GPU step",
+                  "same per-pair work as the GPU step; comparisons/s extrapolate linearly in DB rows",
     }
 
 
@@ -158,7 +179,8 @@ def cpu_baseline_protocol(n_target_s=12.0):
         "kind": "port",
         "sample": f"{steps} x (1 query x 31 rotations vs {n} DB rows, one eye): 3 parties' code + mask dot "
                   f"shares and the 3-party threshold circuits = {steps * L} comparisons in {elapsed:.1f} s "
-                  f"on {threads} threads",
+                  f"on {threads} threads; comparisons/s extrapolate linearly in DB rows (the work per pair "
+                  "does not depend on the DB size)",
     }
 
 
@@ -1004,10 +1026,10 @@ def main():
         sys.exit(2)
 
     if args.workload == "c3":
-        print(json.dumps(run_c3(args)))
+        print(json.dumps(with_ratio(run_c3(args))))
         return
     if args.workload == "c1":
-        print(json.dumps(run_c1(args)))
+        print(json.dumps(with_ratio(run_c1(args))))
         return
     if args.workload == "circuits":
         print(json.dumps(run_circuits(args)))
@@ -1016,10 +1038,10 @@ def main():
         print(json.dumps(run_c3_two_stage(args)))
         return
     if args.workload == "c4":
-        print(json.dumps(run_c4(args)))
+        print(json.dumps(with_ratio(run_c4(args))))
         return
     if args.workload == "c5":
-        print(json.dumps(run_c5(args)))
+        print(json.dumps(with_ratio(run_c5(args))))
         return
     if args.workload == "matmul":
         print(json.dumps(run_matmul(args)))
@@ -1135,7 +1157,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
-    print(json.dumps(line))
+    print(json.dumps(with_ratio(line)))
 
 
 if __name__ == "__main__":
