@@ -75,9 +75,9 @@ __device__ __forceinline__ uint32_t and_pre(uint32_t xa, uint32_t xb, uint32_t y
 // the u16 view of its keystream); the party's other stream (their_rng) is the
 // previous party's own stream at the same counter, read from LDS. The
 // keystreams go through LDS in element order, then each lane handles 8
-// consecutive elements per pass. The three steps (P0 -> P1 and P1 -> P2, then
-// P2 -> P0) pass their u16 messages to the receiving party's wave through LDS,
-// one barrier apart.
+// consecutive elements per pass. P0 and P1 write their a shares, P1 passes
+// its message x to P2's wave through LDS, and after one barrier P2 writes its
+// a share; every b share is the previous party's a share and is not stored.
 __device__ __forceinline__ void u16x8_load(const uint16_t* p, uint16_t v[8]) {
   const uint4 q = *reinterpret_cast<const uint4*>(p);
   const uint32_t w[4] = {q.x, q.y, q.z, q.w};
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
   // granule j of lane w's block sits at granule 4 w + (j ^ (w & 3)), so the
   // lanes' 16-byte writes of one granule spread over more banks
   __shared__ uint32_t ks[3][1024];
-  __shared__ uint32_t msg[3][1024];  // [receiving party][u16 pairs in element order]
+  __shared__ uint32_t msg[1024];  // P1's message x to P2, u16 pairs in element order
   const size_t n = 128 * t.W;
   const size_t e_base = (size_t)blockIdx.x * 2048;
   {
@@ -121,9 +121,14 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
     const int g = el >> 3, w = g >> 2;
     return 8 * (4 * w + ((g & 3) ^ (w & 3)));
   };
-  uint16_t* to_next = reinterpret_cast<uint16_t*>(msg[(p + 1) % 3]);
-  const uint16_t* inbox = reinterpret_cast<const uint16_t*>(msg[p]);
-  // step 1: P0 sends y to P1, P1 sends x to P2; both write corr_a
+  // Party p's b share of a correction is party p - 1's a share (replicated
+  // sharing: the reference's P1 writes its b from P0's message y, P2 its b
+  // from P1's x, P0 its b from P2's z, each equal to the sender's a), so only
+  // the a shares are written; consumers read b from the previous party's a.
+  // Of the three messages only P1 -> P2 (x) feeds an a share.
+  uint16_t* to_p2 = reinterpret_cast<uint16_t*>(msg);
+  const uint16_t* inbox = reinterpret_cast<const uint16_t*>(msg);
+  // step 1: P0 and P1 write corr_a; P1 sends x to P2
   if (p != 2) {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -140,7 +145,6 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
         for (int i = 0; i < 8; i++) {
           const uint16_t b = (uint16_t)((bits >> i) & 1u), r01 = own[i], r02 = prv[i];
           const uint16_t y = (uint16_t)(b * r01 - r02);
-          o1[i] = y;
           o2[i] = (uint16_t)(r01 - (y << 1));
         }
       } else {
@@ -153,62 +157,32 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
           o1[i] = x;
           o2[i] = (uint16_t)(x - (r12 << 1));
         }
+        u16x8_store(to_p2 + el, o1);
       }
-      u16x8_store(to_next + el, o1);
       u16x8_store(P.corr_a + e, o2);
     }
   }
   __syncthreads();
-  // step 2: P1 writes corr_b from y; P2 sends z to P0 and writes corr_a, corr_b
-  if (p != 0) {
+  // step 2: P2 writes corr_a from P1's x
+  if (p == 2) {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int el = 8 * (k * 64 + lane);
       const size_t e = e_base + el;
       if (e >= n) break;
-      uint16_t own[8], prv[8], in[8], o1[8], o2[8], o3[8];
+      uint16_t own[8], prv[8], in[8], o2[8];
       u16x8_load(own16 + kso(el), own);
       u16x8_load(prv16 + kso(el), prv);
       u16x8_load(inbox + el, in);
-      if (p == 1) {
-#pragma unroll
-        for (int i = 0; i < 8; i++) o1[i] = (uint16_t)(prv[i] - (in[i] << 1));
-        u16x8_store(P.corr_b + e, o1);
-      } else {
-        // rand_12 = their rng, rand_02 = my rng; in_a = party 2's a share
-        const uint32_t bits = (uint32_t)(P.c_a[e >> 6] >> (e & 63)) & 0xffu;
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-          const uint16_t r12 = prv[i], r02 = own[i], x = in[i], b = (uint16_t)((bits >> i) & 1u);
-          const uint16_t z = (uint16_t)(b * x - r12);
-          o1[i] = z;
-          o2[i] = (uint16_t)(-((r02 + z) << 1) + b);
-          o3[i] = (uint16_t)(x - (r12 << 1));
-        }
-        u16x8_store(to_next + el, o1);
-        u16x8_store(P.corr_a + e, o2);
-        u16x8_store(P.corr_b + e, o3);
-      }
-    }
-  }
-  __syncthreads();
-  // step 3: P0 writes corr_b from z
-  if (p == 0) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int el = 8 * (k * 64 + lane);
-      const size_t e = e_base + el;
-      if (e >= n) break;
-      uint16_t prv[8], in[8], o1[8];
-      u16x8_load(prv16 + kso(el), prv);
-      u16x8_load(inbox + el, in);
-      const uint32_t bits = (uint32_t)(P.c_b[e >> 6] >> (e & 63)) & 0xffu;
+      // rand_12 = their rng, rand_02 = my rng; in_a = party 2's a share
+      const uint32_t bits = (uint32_t)(P.c_a[e >> 6] >> (e & 63)) & 0xffu;
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        const uint16_t b = (uint16_t)((bits >> i) & 1u), r02 = prv[i], z = in[i];
-        o1[i] = (uint16_t)(-((r02 + z) << 1) + b);
+        const uint16_t r12 = prv[i], r02 = own[i], x = in[i], b = (uint16_t)((bits >> i) & 1u);
+        const uint16_t z = (uint16_t)(b * x - r12);
+        o2[i] = (uint16_t)(-((r02 + z) << 1) + b);
       }
-      u16x8_store(P.corr_b + e, o1);
+      u16x8_store(P.corr_a + e, o2);
     }
   }
 }
@@ -371,8 +345,8 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
         u16x8_load(P.mask_b + e, mb);
         u16x8_load(P.corr_a + e, c1a);
         u16x8_load(P.corr_a + n_el + e, c2a);
-        u16x8_load(P.corr_b + e, c1b);
-        u16x8_load(P.corr_b + n_el + e, c2b);
+        u16x8_load(t.p[prev].corr_a + e, c1b);  // b share = the previous party's a share
+        u16x8_load(t.p[prev].corr_a + n_el + e, c2b);
         u16x8_load(P.code_a + e, xa);
         u16x8_load(P.code_b + e, xb);
       }

@@ -15,7 +15,7 @@ struct CircuitBufs {
   const uint16_t* mask_a;
   const uint16_t* mask_b;
   uint64_t *c_a, *c_b;      // 2 W: lift carries (bit 16, bit 17)
-  uint16_t *corr_a, *corr_b;  // 128 W: injected carries (arithmetic u16 shares)
+  uint16_t* corr_a;  // 128 W: injected carries, a share (the b share is the previous party's a)
   uint64_t *res_a, *res_b;  // W: result bit shares (plane 0)
   uint32_t *lifted_a, *lifted_b;  // optional (debug): corrected lifted mask
   ChaChaKey key_own, key_prev;    // Circuits ChaCha seeds (chacha1 / chacha2)
@@ -59,8 +59,9 @@ struct RippleCtrs {
 // lift_mul_sub and the MSB adder (-> res). The bit planes stay in registers.
 void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const RippleCtrs& ctrs,
                        hipStream_t s);
-// Bit injection of the two lift carries (all three parties): corrections
-// corr_a / corr_b of every element, the three message steps inside one launch.
+// Bit injection of the two lift carries (all three parties): the a share
+// corr_a of every element's corrections (the b share is the previous party's
+// a share), the message steps inside one launch.
 void launch_inject(const Trio& t, uint64_t ctr, hipStream_t s);
 
 // Open + bitmap (actor.rs:2928-2997, dot/kernel.cu:30-116). Party views:

@@ -687,7 +687,7 @@ Circuits::Circuits(int device, int peer_id, size_t input_size, size_t alloc_size
   const size_t A = alloc_words_ + 8;
   c_a_.alloc(2 * A); c_b_.alloc(2 * A);
   res_a_.alloc(A); res_b_.alloc(A); open_c_.alloc(A);
-  corr_a_.alloc(128 * A); corr_b_.alloc(128 * A);
+  corr_a_.alloc(128 * A);
 }
 
 void Circuits::set_chunk_size(size_t words) {
@@ -700,7 +700,7 @@ CircuitBufs Circuits::bufs(const uint16_t* ca, const uint16_t* cb, const uint16_
   CircuitBufs b{};
   b.code_a = ca; b.code_b = cb; b.mask_a = ma; b.mask_b = mb;
   b.c_a = c_a_.ptr; b.c_b = c_b_.ptr;
-  b.corr_a = corr_a_.ptr; b.corr_b = corr_b_.ptr;
+  b.corr_a = corr_a_.ptr;
   b.res_a = res_a_.ptr; b.res_b = res_b_.ptr;
   b.lifted_a = lifted_dbg_a; b.lifted_b = lifted_dbg_b;
   b.key_own = seed_own_; b.key_prev = seed_prev_;

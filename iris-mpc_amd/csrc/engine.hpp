@@ -365,7 +365,7 @@ class Circuits {
   uint64_t nonce_ = 0, counter_ = 0;
   Exchange* ex_;
   DevBuf<uint64_t> c_a_, c_b_, res_a_, res_b_, open_c_;
-  DevBuf<uint16_t> corr_a_, corr_b_;
+  DevBuf<uint16_t> corr_a_;
   const uint64_t* open_view_ = nullptr;
 };
 
