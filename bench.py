@@ -72,6 +72,110 @@ def measured_traffic(kernel, path=TRAFFIC_FILE):
     return t["traffic_bytes_per_launch"]
 
 
+PROBE_CHUNKS = 4           # DB chunks of the profiled probe run
+PROBE_COPY_BYTES = 1 << 31  # calibration copy: 2 GiB device to device
+
+
+def read_counters(files, counter):
+    """{kernel name: [value per dispatch]} of `counter` in rocprofv3's
+    counter-collection CSV files."""
+    import csv
+
+    vals = {}
+    for fn in files:
+        with open(fn, newline="") as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") == counter:
+                    vals.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    return vals
+
+
+def _pmc_pass(counter, kernel_regex, child_args):
+    """One rocprofv3 --pmc pass of `counter` over a child process running the
+    traffic probe. Returns ({kernel name: [per-dispatch values]}, None) or
+    (None, reason)."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    d = tempfile.mkdtemp(prefix="irismpc_pmc_")
+    try:
+        cmd = ["timeout", "-s", "KILL", "240", prof, "--pmc", counter, "--kernel-include-regex",
+               kernel_regex + "|copyBuffer", "-f", "csv", "-d", d, "-o", "pmc", "--",
+               sys.executable, os.path.abspath(__file__), "--workload", "traffic-probe"] + child_args
+        r = subprocess.run(cmd, cwd=d, env=dict(os.environ, TMPDIR=d), stdout=subprocess.DEVNULL,
+                           stderr=subprocess.PIPE, text=True)
+        vals = read_counters(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True),
+                             counter)
+        # rocprofv3 may end with a signal after the program has finished: the
+        # counters are used whenever the file holds them
+        if not vals:
+            return None, f"rc {r.returncode}, no counters ({r.stderr.strip()[-200:]})"
+        return vals, None
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def live_traffic(kernel, child_args):
+    """HBM bytes per launch of `kernel` measured in this run: two rocprofv3
+    --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) over the
+    traffic probe, each counter scaled by what it reads for the probe's 2 GiB
+    device copy in the same pass. Returns (bytes, detail) or (None, reason)."""
+    name = kernel.split("<")[0]
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        progress(f"rocprofv3 --pmc {counter} pass over the traffic probe")
+        vals, err = _pmc_pass(counter, name, child_args)
+        if err:
+            return None, f"{counter}: {err}"
+        gemm = [v for k, vs in vals.items() if name in k for v in vs]
+        copies = [v for k, vs in vals.items() if "copyBuffer" in k for v in vs]
+        if not gemm or not copies:
+            return None, f"{counter}: kernels missing from the counter file"
+        scale = PROBE_COPY_BYTES / (max(copies) * 1024.0)
+        out[counter] = {"dispatches": len(gemm), "KiB_avg": sum(gemm) / len(gemm),
+                        "copy_KiB": max(copies), "scale": scale}
+    fetch, write = out["FETCH_SIZE"], out["WRITE_SIZE"]
+    traffic = fetch["KiB_avg"] * 1024 * fetch["scale"] + write["KiB_avg"] * 1024 * write["scale"]
+    return traffic, out
+
+
+def _gemm_algorithmic_bytes(Qr):
+    """HBM bytes one share GEMM launch of the scan must move: its chunk's DB
+    limbs read once (the average of the code and mask launches) and the u16
+    results written once."""
+    db = 32768 * (12800 + 6400) * 2 / 2
+    return db + Qr * 32768 * 2
+
+
+def run_traffic_probe(args):
+    """The child that the traffic passes profile: one batch of the 3-party scan
+    of one eye over a DB of PROBE_CHUNKS chunks (every share GEMM launch is one
+    party's code or mask GEMM over one 32,768-row chunk, the launch shape of
+    the headline workload), then one 2 GiB device-to-device copy."""
+    import torch
+
+    import irismpc
+
+    args.db = PROBE_CHUNKS * irismpc.DB_CHUNK_SIZE
+    N, B, Qr, chunk, U, n_plant, eyes, expect = _c3_engines(args, n_eyes=1)
+    e = eyes[0]
+    torch.cuda.synchronize()
+    irismpc.scan_db_trio(e["parties"], e["qcs"], e["qms"], N, B, irismpc.THRESHOLD_ANON_STATS_A,
+                         [b.data_ptr() for b in e["bitmaps"]], e["streams"])
+    torch.cuda.synchronize()
+    x = torch.ones(PROBE_COPY_BYTES // 4, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print("traffic probe done", flush=True)
+
+
 def cpu_model():
     """The host CPU's model name (from /proc/cpuinfo; None where absent)."""
     try:
@@ -184,11 +288,12 @@ def cpu_baseline_protocol(n_target_s=12.0):
     }
 
 
-def _c3_engines(args, extra_left=0):
-    """Both eyes' engines of the full protocol: 3 parties per eye, each party
-    holding the eye's DB share of args.db rows in HBM (the left eye's slices
-    with room for extra_left more rows), synthetic irises with planted queries
-    (copies of DB irises, 10-30 % of code bits flipped)."""
+def _c3_engines(args, extra_left=0, n_eyes=2):
+    """Both eyes' engines of the full protocol (the left eye's alone with
+    n_eyes=1): 3 parties per eye, each party holding the eye's DB share of
+    args.db rows in HBM (the left eye's slices with room for extra_left more
+    rows), synthetic irises with planted queries (copies of DB irises, 10-30 %
+    of code bits flipped)."""
     import numpy as np
     import torch
 
@@ -207,7 +312,7 @@ def _c3_engines(args, extra_left=0):
     planted = [(j, int(t), int(f)) for j, t, f in zip(range(n_plant), targets, flips)]
     base = [np.full(8, p, np.uint32) for p in range(3)]
     eyes = []
-    for eye in range(2):
+    for eye in range(n_eyes):
         ds = 1 + eye  # left / right eye datasets
         ex = irismpc.Exchange(timeout_s=300)
         parties, bitmaps, streams = [], [], []
@@ -316,6 +421,15 @@ def run_c3(args):
     import irismpc
 
     latency = args.latency_rate > 0
+    Qr = args.queries * 31
+    kernel = KERNEL_OF_VARIANT[args.variant if args.variant >= 100 else 100 if Qr > ROT_SMALL_ROWS else 102]
+    traffic, traffic_detail = None, "not measured (--no-live-traffic)"
+    if args.live_traffic:
+        # before this process takes the GPU: the passes profile a child process
+        traffic, traffic_detail = live_traffic(kernel, ["--queries", str(args.queries),
+                                                        "--variant", str(args.variant)])
+        progress(f"share GEMM traffic per launch: {traffic / 1e9:.3f} GB" if traffic else
+                 f"live traffic not measured: {traffic_detail}")
     progress("building the 3 parties x 2 eyes DB shares")
     N, B, Qr, chunk, U, n_plant, eyes, expect = _c3_engines(args, 2 * irismpc.DB_CHUNK_SIZE if latency else 0)
     progress("DB shares resident; warm-up")
@@ -372,7 +486,15 @@ def run_c3(args):
     circ_pairs = pairs * 3 * 2 * args.steps  # per party, summed over parties and eyes
     chacha_ops = CHACHA_BLOCKS_PER_PAIR * CHACHA12_OPS * circ_pairs
     circ_bytes = CIRCUIT_BYTES_PER_PAIR * circ_pairs
-    kernel = KERNEL_OF_VARIANT[args.variant if args.variant >= 100 else 100 if Qr > ROT_SMALL_ROWS else 102]
+    if traffic:
+        traffic_source = ("measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over a child "
+                          f"process running one batch of the same scan on a {PROBE_CHUNKS}-chunk DB (same "
+                          "launch shape: one party's code or mask GEMM over one 32,768-row chunk), each "
+                          "counter scaled by its reading for a 2 GiB device copy in the same pass")
+    else:
+        traffic = measured_traffic(kernel, C3_TRAFFIC_FILE)
+        traffic_source = (f"live passes unavailable ({traffic_detail}); rocprofv3 --pmc runs of this workload "
+                          "committed as profiles/share_gemm_c3_traffic.json")
     line = {
         "metric": "3-party iris comparisons/sec on 1 MI355X + p50/p99 request latency",
         "value": 2 * pairs / (ms_step / 1e3),
@@ -390,9 +512,10 @@ def run_c3(args):
                    "planted_matches_exact": found},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": achieved / INT8_PEAK_TOPS,
-                     "traffic": measured_traffic(kernel, C3_TRAFFIC_FILE),
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this workload, "
-                                       "copy-calibrated, committed as profiles/share_gemm_c3_traffic.json",
+                     "traffic": traffic,
+                     "traffic_source": traffic_source,
+                     "traffic_passes": traffic_detail if isinstance(traffic_detail, dict) else None,
+                     "algorithmic_bytes_per_launch": _gemm_algorithmic_bytes(Qr),
                      "kernel": kernel,
                      "launches": gemm_n, "avg_launch_ms": gemm_ms / max(gemm_n, 1),
                      "int8_ops_per_launch_avg": gemm_ops / max(gemm_n, 1),
@@ -1002,7 +1125,8 @@ def main():
     ap.add_argument("--variant", type=int, default=-1,
                     help="GEMM kernel variant (-1: automatic, the rotation-aware kernel)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c3-two-stage", "c4", "c5", "circuits", "matmul"),
+    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c3-two-stage", "c4", "c5", "circuits", "matmul",
+                                           "traffic-probe"),
                     default="c3",
                     help="c3 (the headline line): full 3-party protocol, both eyes, plus a serving latency "
                          "pass; c1: the CPU-runnable configuration (4,096 rows, 32 queries) against its "
@@ -1018,6 +1142,9 @@ def main():
     ap.add_argument("--latency-rate", type=float, default=40.0,
                     help="c3: Poisson request rate of the serving latency pass (0: no pass)")
     ap.add_argument("--latency-seconds", type=float, default=6.0, help="c3: arrival window of that pass")
+    ap.add_argument("--no-live-traffic", dest="live_traffic", action="store_false",
+                    help="c3: skip the two rocprofv3 --pmc passes that measure the share GEMM's HBM traffic "
+                         "(the committed profile's figure is reported instead)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -1027,6 +1154,9 @@ def main():
 
     if args.workload == "c3":
         print(json.dumps(with_ratio(run_c3(args))))
+        return
+    if args.workload == "traffic-probe":
+        run_traffic_probe(args)
         return
     if args.workload == "c1":
         print(json.dumps(with_ratio(run_c1(args))))
