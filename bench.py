@@ -1076,11 +1076,13 @@ def run_circuits(args):
     circ = [irismpc.Circuits(p, L, W, seeds[p], ex) for p in range(3)]
     streams = [irismpc.stream_create() for _ in range(3)]
     gen = torch.Generator(device="cuda").manual_seed(7)
+    # each party's a shares; its b shares are the previous party's a buffers,
+    # as the scan's reshare hands them over (zero-copy views)
     bufs = [[torch.randint(-32768, 32767, (L,), dtype=torch.int16, device="cuda", generator=gen)
-             for _ in range(4)] for _ in range(3)]
+             for _ in range(2)] for _ in range(3)]
     torch.cuda.synchronize()
-    codes = [(b[0].data_ptr(), b[1].data_ptr()) for b in bufs]
-    masks = [(b[2].data_ptr(), b[3].data_ptr()) for b in bufs]
+    codes = [(bufs[p][0].data_ptr(), bufs[(p + 2) % 3][0].data_ptr()) for p in range(3)]
+    masks = [(bufs[p][1].data_ptr(), bufs[(p + 2) % 3][1].data_ptr()) for p in range(3)]
 
     def step():
         irismpc.Circuits.compare_threshold_masked_many_trio(circ, codes, masks, L, THRESHOLD_ANON_A, streams)
@@ -1104,7 +1106,8 @@ def run_circuits(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "GF(2) bit-sliced circuits on u16/u32 shares, ChaCha12 correlated randomness",
-        "data": "uniformly random u16 code and mask dot shares per party",
+        "data": "uniformly random u16 code and mask dot shares per party (a shares; each b share is the "
+                "previous party's a buffer, as after the reshare)",
         "config": {"workload": f"lift, bit injection, MSB adder and open of 3 parties over one chunk: "
                                f"{Qr:,} query-rotation rows x {irismpc.DB_CHUNK_SIZE:,} DB rows "
                                f"({W:,} result words)", "threshold_a": THRESHOLD_ANON_A},

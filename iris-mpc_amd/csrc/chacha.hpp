@@ -49,6 +49,38 @@ __host__ __device__ __forceinline__ void chacha_block(const ChaChaKey& key, uint
   out[12] = x12 + i12; out[13] = x13 + i13; out[14] = x14 + i14; out[15] = x15 + i15;
 }
 
+// The same block in pieces, for consumers that spread one block's rounds over
+// other latency-bound work: chacha_start, ROUNDS / 2 chacha_double_round
+// calls, chacha_finish (which adds the input words back in place).
+__device__ __forceinline__ void chacha_start(const ChaChaKey& key, uint64_t nonce, uint64_t counter,
+                                             uint32_t x[16]) {
+  x[0] = 0x61707865u; x[1] = 0x3320646eu; x[2] = 0x79622d32u; x[3] = 0x6b206574u;
+#pragma unroll
+  for (int i = 0; i < 8; i++) x[4 + i] = key.k[i];
+  x[12] = (uint32_t)counter; x[13] = (uint32_t)(counter >> 32);
+  x[14] = (uint32_t)nonce; x[15] = (uint32_t)(nonce >> 32);
+}
+
+__device__ __forceinline__ void chacha_double_round(uint32_t x[16]) {
+  IRIS_QR(x[0], x[4], x[8], x[12]);
+  IRIS_QR(x[1], x[5], x[9], x[13]);
+  IRIS_QR(x[2], x[6], x[10], x[14]);
+  IRIS_QR(x[3], x[7], x[11], x[15]);
+  IRIS_QR(x[0], x[5], x[10], x[15]);
+  IRIS_QR(x[1], x[6], x[11], x[12]);
+  IRIS_QR(x[2], x[7], x[8], x[13]);
+  IRIS_QR(x[3], x[4], x[9], x[14]);
+}
+
+__device__ __forceinline__ void chacha_finish(const ChaChaKey& key, uint64_t nonce, uint64_t counter,
+                                              uint32_t x[16]) {
+  x[0] += 0x61707865u; x[1] += 0x3320646eu; x[2] += 0x79622d32u; x[3] += 0x6b206574u;
+#pragma unroll
+  for (int i = 0; i < 8; i++) x[4 + i] += key.k[i];
+  x[12] += (uint32_t)counter; x[13] += (uint32_t)(counter >> 32);
+  x[14] += (uint32_t)nonce; x[15] += (uint32_t)(nonce >> 32);
+}
+
 #undef IRIS_QR
 
 }  // namespace irismpc

@@ -15,6 +15,8 @@
 //  * correlated randomness is generated in registers from the block counter
 //    the reference would have used (no keystream buffers).
 #include "circuits.hpp"
+
+#include <type_traits>
 #include "chacha.hpp"
 
 namespace irismpc {
@@ -238,6 +240,16 @@ struct PlaneSlices {
   __device__ __forceinline__ uint32_t b(int j) const { return pl[(MSB ? 32 : 16) + j]; }
 };
 
+// MSB planes when every party's b share is the previous party's a share (the
+// reshare's zero-copy views): a wave forms its own a planes only and reads
+// plane j of its b share from the previous party's wave through LDS.
+struct PlanesPrevB {
+  uint32_t pl[32];
+  const uint32_t* prev;  // LDS [32][64] planes of the previous party, at this lane
+  __device__ __forceinline__ uint32_t a(int j) const { return pl[j]; }
+  __device__ __forceinline__ uint32_t b(int j) const { return prev[64 * j]; }
+};
+
 struct AdderState {
   uint32_t ta, tb;  // plane s - 1
   uint32_t ca, cb;  // carry-save AND of plane s - 1: own, previous party's
@@ -253,13 +265,77 @@ struct AdderLds {
   uint32_t xch[2][3][2][64];   // [step parity][party][ripple, carry-save][lane]
 };
 
+// The ChaCha12 block of this lane's gate in batch b (8 steps, 16 gates): gate
+// gl of its 16-lane group is step 8 b + gl / 2, the carry-save AND of that
+// step's plane (gl even) or its ripple round s - 1 (gl odd; the lift's step 0
+// slot is the carry-out). Gates past the circuit draw nothing (zero block).
+template <bool MSB>
+__device__ __forceinline__ void batch_gate(int b, int gl, const RippleCtrs& rc, size_t G, size_t g,
+                                           bool& valid, uint64_t& ctr) {
+  constexpr int LAST = MSB ? 30 : 15;
+  constexpr int LAST_C = MSB ? 29 : 14;
+  const int sj = gl >> 1, s = 8 * b + sj;
+  uint64_t c_round = 0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int k = 8 * b + j - 1;  // ripple round of step 8 b + j
+    const uint64_t v = (k >= 0 && k <= LAST_C) ? rc.c[k] : (!MSB && k < 0 ? rc.c[15] : 0);
+    if (sj == j) c_round = v;
+  }
+  if ((gl & 1) == 0) {  // carry-save AND of plane s
+    valid = s <= LAST;
+    ctr = rc.csa + (uint64_t)s * G + g;
+  } else {  // ripple round s - 1; the lift's step 0 slot: carry-out
+    valid = (s >= 1 && s <= LAST_C + 1) || (!MSB && s == 0);
+    ctr = c_round + g;
+  }
+}
+
+// The next batch's block, computed a piece per exchange step of the current
+// batch so that its VALU work fills the steps' LDS round trips and barriers:
+// steps 0-5 run one double round each (column half before the step's barrier,
+// diagonal half after it), step 6 adds the input words and hands the block to
+// the group through LDS. Step 0's barrier has retired every wave's reads of
+// the current batch's blocks before any wave writes the next one, and step
+// 7's barrier publishes it.
+struct NextDraw {
+  uint32_t x[16];
+  uint64_t ctr;
+  bool valid;
+};
+
+template <int J, bool kBeforeBarrier>
+__device__ __forceinline__ void draw_piece(NextDraw& d, const ChaChaKey& key, uint64_t nonce,
+                                           uint32_t* rnd_row) {
+  uint32_t* x = d.x;
+  if constexpr (J < 6) {
+#define IRISMPC_QR(a, b, c, e)                            \
+  x[a] += x[b]; x[e] = rotl32(x[e] ^ x[a], 16);           \
+  x[c] += x[e]; x[b] = rotl32(x[b] ^ x[c], 12);           \
+  x[a] += x[b]; x[e] = rotl32(x[e] ^ x[a], 8);            \
+  x[c] += x[e]; x[b] = rotl32(x[b] ^ x[c], 7);
+    if constexpr (kBeforeBarrier) {
+      IRISMPC_QR(0, 4, 8, 12); IRISMPC_QR(1, 5, 9, 13); IRISMPC_QR(2, 6, 10, 14); IRISMPC_QR(3, 7, 11, 15);
+    } else {
+      IRISMPC_QR(0, 5, 10, 15); IRISMPC_QR(1, 6, 11, 12); IRISMPC_QR(2, 7, 8, 13); IRISMPC_QR(3, 4, 9, 14);
+    }
+#undef IRISMPC_QR
+  } else if constexpr (J == 6 && kBeforeBarrier) {
+    chacha_finish(key, nonce, d.ctr, x);
+#pragma unroll
+    for (int i = 0; i < 16; i++) rnd_row[i] = d.valid ? x[i] : 0u;
+  }
+}
+
 // Step S (last plane LAST). r16[2 (S % 8)] is the carry-save randomness of
 // plane S, r16[2 (S % 8) + 1] the ripple randomness of round S - 1; the lift's
-// carry-out randomness is r_out.
-template <bool MSB, int S>
-__device__ __forceinline__ void adder_step(const PlaneSlices<MSB>& W, AdderState& st,
+// carry-out randomness is r_out. kDraw: this batch computes the next one's
+// block (d) between its exchanges.
+template <bool MSB, int S, bool kDraw, typename SL>
+__device__ __forceinline__ void adder_step(const SL& W, AdderState& st,
                                            const uint32_t r16[16], uint32_t r_out, AdderLds& L,
-                                           int p, int prev, int lane) {
+                                           int p, int prev, int lane, NextDraw& d,
+                                           const ChaChaKey& key, uint64_t nonce, uint32_t* rnd_row) {
   constexpr int LAST = MSB ? 30 : 15;
   constexpr int J = S % 8;
   constexpr int X = S & 1;
@@ -267,8 +343,15 @@ __device__ __forceinline__ void adder_step(const PlaneSlices<MSB>& W, AdderState
     const uint32_t na = W.a(0), nb = W.b(0);
     const uint32_t cs = csa_and(p, na, nb) ^ r16[0];
     L.xch[X][p][1][lane] = cs;
+    if constexpr (kDraw) {
+      // the writes go out first, the rounds fill their latency before the barrier
+      __builtin_amdgcn_sched_barrier(0);
+      draw_piece<J, true>(d, key, nonce, rnd_row);
+      __builtin_amdgcn_sched_barrier(0);
+    }
     __syncthreads();
     st.cb = L.xch[X][prev][1][lane];
+    if constexpr (kDraw) draw_piece<J, false>(d, key, nonce, rnd_row);
     st.ca = cs;
     st.ta = na;
     st.tb = nb;
@@ -282,9 +365,16 @@ __device__ __forceinline__ void adder_step(const PlaneSlices<MSB>& W, AdderState
     const uint32_t cs = csa_and(p, na, nb) ^ r16[2 * J];
     L.xch[X][p][0][lane] = va;
     L.xch[X][p][1][lane] = cs;
+    if constexpr (kDraw) {
+      // the writes go out first, the rounds fill their latency before the barrier
+      __builtin_amdgcn_sched_barrier(0);
+      draw_piece<J, true>(d, key, nonce, rnd_row);
+      __builtin_amdgcn_sched_barrier(0);
+    }
     __syncthreads();
     st.vb = L.xch[X][prev][0][lane];
     st.cb = L.xch[X][prev][1][lane];
+    if constexpr (kDraw) draw_piece<J, false>(d, key, nonce, rnd_row);
     st.va = va;
     st.ca = cs;
     st.ta = na;
@@ -302,25 +392,33 @@ __device__ __forceinline__ void adder_step(const PlaneSlices<MSB>& W, AdderState
       st.vb = L.xch[X ^ 1][prev][0][lane];
       st.va = v2;
     }
+  } else {
+    static_assert(!kDraw, "only the circuit's last batch has steps past its last plane");
   }
 }
 
-template <bool MSB, int B>
-__device__ __forceinline__ void adder_batch(const PlaneSlices<MSB>& W, AdderState& st,
+template <bool MSB, int B, typename SL>
+__device__ __forceinline__ void adder_batch(const SL& W, AdderState& st,
                                             const uint32_t r16[16], uint32_t r_out, AdderLds& L,
-                                            int p, int prev, int lane) {
-#define IRISMPC_STEP(j) adder_step<MSB, 8 * B + j>(W, st, r16, r_out, L, p, prev, lane)
+                                            int p, int prev, int lane, NextDraw& d,
+                                            const ChaChaKey& key, uint64_t nonce, uint32_t* rnd_row) {
+  constexpr bool kDraw = B + 1 < (MSB ? 4 : 2);
+#define IRISMPC_STEP(j) \
+  adder_step<MSB, 8 * B + j, kDraw>(W, st, r16, r_out, L, p, prev, lane, d, key, nonce, rnd_row)
   IRISMPC_STEP(0); IRISMPC_STEP(1); IRISMPC_STEP(2); IRISMPC_STEP(3);
   IRISMPC_STEP(4); IRISMPC_STEP(5); IRISMPC_STEP(6); IRISMPC_STEP(7);
 #undef IRISMPC_STEP
 }
 
-template <bool MSB>
+// kPrevB (MSB stage only): every party's b share is the previous party's a
+// share, element for element (the caller passes the previous party's a
+// buffers as b), so b planes come from the previous party's wave (PlanesPrevB).
+template <bool MSB, bool kPrevB>
 __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t A, const RippleCtrs rc) {
-  constexpr int LAST = MSB ? 30 : 15;    // carry-save planes 0 .. LAST
-  constexpr int LAST_C = MSB ? 29 : 14;  // ripple rounds 0 .. LAST_C (lift carry-out: c[15])
+  static_assert(MSB || !kPrevB, "the b planes of the lift come with its a planes in one transpose");
   constexpr int N_BATCH = MSB ? 4 : 2;   // steps 0 .. 31 / 0 .. 15, 8 per batch
   __shared__ AdderLds L;
+  __shared__ uint32_t a_planes[kPrevB ? 3 * 32 * 64 : 1];  // [party][plane][lane]
   // one party per wave: p is wave-uniform, so its keys and pointers stay scalar
   const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63), prev = (p + 2) % 3;
@@ -330,12 +428,44 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
   const size_t hw = (size_t)blockIdx.x * 64 + lane;  // half word: elements 32 hw .. 32 hw + 31
   const size_t g = (size_t)blockIdx.x * 4 + grp;     // ChaCha block of words 8 g .. 8 g + 7
   const bool live = hw < 2 * W;
+  uint32_t* rnd_row = L.rnd[p][grp][gl];
 
-  PlaneSlices<MSB> sl;
+  // batch 0's block first: its rounds overlap the loads below
+  NextDraw d;
+  batch_gate<MSB>(0, gl, rc, G, g, d.valid, d.ctr);
+  {
+    uint32_t k[16] = {};
+    if (d.valid) chacha_block<12>(P.key_own, t.nonce, d.ctr, k);
+#pragma unroll
+    for (int i = 0; i < 16; i++) rnd_row[i] = k[i];
+  }
+
+  std::conditional_t<kPrevB, PlanesPrevB, PlaneSlices<MSB>> sl;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const size_t e = 32 * hw + 8 * q;
-    if constexpr (MSB) {
+    if constexpr (kPrevB) {
+      // the a side of the general form below; the b side is the previous wave's
+      uint16_t ma[8] = {}, c1a[8] = {}, c2a[8] = {}, xa[8] = {};
+      if (live) {
+        u16x8_load(P.mask_a + e, ma);
+        u16x8_load(P.corr_a + e, c1a);
+        u16x8_load(P.corr_a + n_el + e, c2a);
+        u16x8_load(P.code_a + e, xa);
+      }
+      uint32_t la[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++)
+        la[i] = (uint32_t)ma[i] - ((uint32_t)c1a[i] << 16) - ((uint32_t)c2a[i] << 17);
+      if (P.lifted_a && live) {  // own a, and the next party's b
+        uint4* da = reinterpret_cast<uint4*>(P.lifted_a + e);
+        uint4* db = reinterpret_cast<uint4*>(t.p[(p + 1) % 3].lifted_b + e);
+        da[0] = db[0] = uint4{la[0], la[1], la[2], la[3]};
+        da[1] = db[1] = uint4{la[4], la[5], la[6], la[7]};
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) sl.pl[8 * q + i] = la[i] * A - ((uint32_t)xa[i] << 16) - (p == 0 ? 1u : 0u);
+    } else if constexpr (MSB) {
       // m = (mask - c1<<16 - c2<<17) * A - code<<16, minus 1 on x_0
       // (kernel.cu:181-195,407-426)
       uint16_t ma[8] = {}, mb[8] = {}, c1a[8] = {}, c2a[8] = {}, c1b[8] = {}, c2b[8] = {},
@@ -380,45 +510,36 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
     }
   }
   bit_transpose32(sl.pl);
-  if constexpr (MSB) bit_transpose32(sl.pl + 32);
+  if constexpr (kPrevB) {
+#pragma unroll
+    for (int j = 0; j < 32; j++) a_planes[(p * 32 + j) * 64 + lane] = sl.pl[j];
+    sl.prev = &a_planes[prev * 32 * 64 + lane];
+  } else if constexpr (MSB) {
+    bit_transpose32(sl.pl + 32);
+  }
+  __syncthreads();
 
   AdderState st{};
   uint32_t r_out = 0;
 #pragma unroll 1
   for (int b = 0; b < N_BATCH; b++) {
-    {  // this lane's gate of the batch: step 8 b + gl / 2, slot gl & 1
-      const int sj = gl >> 1, s = 8 * b + sj;
-      uint64_t c_round = 0;
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int k = 8 * b + j - 1;  // ripple round of step 8 b + j
-        const uint64_t v = (k >= 0 && k <= LAST_C) ? rc.c[k] : (!MSB && k < 0 ? rc.c[15] : 0);
-        if (sj == j) c_round = v;
-      }
-      bool valid;
-      uint64_t ctr;
-      if ((gl & 1) == 0) {  // carry-save AND of plane s
-        valid = s <= LAST;
-        ctr = rc.csa + (uint64_t)s * G + g;
-      } else {  // ripple round s - 1; the lift's step 0 slot: carry-out
-        valid = (s >= 1 && s <= LAST_C + 1) || (!MSB && s == 0);
-        ctr = c_round + g;
-      }
-      uint32_t k[16] = {};
-      if (valid) chacha_block<12>(P.key_own, t.nonce, ctr, k);
-#pragma unroll
-      for (int i = 0; i < 16; i++) L.rnd[p][grp][gl][i] = k[i];
-    }
-    __syncthreads();
     uint32_t r16[16];  // F(own) ^ F(prev): bits of the 16 gates for this lane's half word
 #pragma unroll
     for (int c = 0; c < 16; c++) r16[c] = L.rnd[p][grp][c][gl] ^ L.rnd[prev][grp][c][gl];
     if (!MSB && b == 0) r_out = r16[1];
+    if (b + 1 < N_BATCH) {
+      batch_gate<MSB>(b + 1, gl, rc, G, g, d.valid, d.ctr);
+      chacha_start(P.key_own, t.nonce, d.ctr, d.x);
+    }
     switch (b) {
-      case 0: adder_batch<MSB, 0>(sl, st, r16, r_out, L, p, prev, lane); break;
-      case 1: adder_batch<MSB, 1>(sl, st, r16, r_out, L, p, prev, lane); break;
-      case 2: adder_batch<MSB, 2>(sl, st, r16, r_out, L, p, prev, lane); break;
-      default: adder_batch<MSB, 3>(sl, st, r16, r_out, L, p, prev, lane); break;
+      case 0: adder_batch<MSB, 0>(sl, st, r16, r_out, L, p, prev, lane, d, P.key_own, t.nonce, rnd_row); break;
+      case 1: adder_batch<MSB, 1>(sl, st, r16, r_out, L, p, prev, lane, d, P.key_own, t.nonce, rnd_row); break;
+      case 2:
+        if constexpr (MSB) adder_batch<MSB, 2>(sl, st, r16, r_out, L, p, prev, lane, d, P.key_own, t.nonce, rnd_row);
+        break;
+      default:
+        if constexpr (MSB) adder_batch<MSB, 3>(sl, st, r16, r_out, L, p, prev, lane, d, P.key_own, t.nonce, rnd_row);
+        break;
     }
   }
   if (!live) return;
@@ -473,10 +594,19 @@ void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const Ripp
                        hipStream_t s) {
   IRIS_REQUIRE(t.W % 8 == 0, "bit planes must be a multiple of 8 words");
   const unsigned blocks = (unsigned)ceil_div(t.W, 32);
-  if (msb)
-    adder_word_kernel<true><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
+  // b shares that are the previous party's a buffers: planes from the previous wave
+  bool prev_b = true;
+  for (int p = 0; p < 3; p++) {
+    const int q = (p + 2) % 3;
+    prev_b = prev_b && t.p[p].code_b == t.p[q].code_a && t.p[p].mask_b == t.p[q].mask_a &&
+             (t.p[p].lifted_a == nullptr) == (t.p[p].lifted_b == nullptr);
+  }
+  if (msb && prev_b)
+    adder_word_kernel<true, true><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
+  else if (msb)
+    adder_word_kernel<true, false><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
   else
-    adder_word_kernel<false><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
+    adder_word_kernel<false, false><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 

@@ -58,7 +58,11 @@ def _setup(irismpc, torch, W, alloc, counter=0, seeds=(SEEDS_OWN, SEEDS_PREV)):
 
 
 def _run_case(irismpc, orc, torch, W, alloc, A, seed, counter=0, threads=False, sampler=None,
-              seeds=(SEEDS_OWN, SEEDS_PREV)):
+              seeds=(SEEDS_OWN, SEEDS_PREV), alias_b=False):
+    """alias_b: every party's b views are the previous party's a buffers (as the
+    scan's zero-copy reshare hands them over), which selects the MSB adder form
+    that reads the b planes from the previous party's wave; otherwise the b
+    shares are separate buffers of the same values."""
     rng = np.random.default_rng(seed)
     L = 64 * W
     code, mask = (sampler or _sample)(rng, L)
@@ -69,6 +73,9 @@ def _run_case(irismpc, orc, torch, W, alloc, A, seed, counter=0, threads=False, 
     dcb = [_dev(torch, x) for x in cb]
     dma = [_dev(torch, x) for x in ma]
     dmb = [_dev(torch, x) for x in mb]
+    if alias_b:
+        dcb = [dca[(p + 2) % 3] for p in range(3)]
+        dmb = [dma[(p + 2) % 3] for p in range(3)]
     lifted = [(torch.zeros(L, dtype=torch.int32, device="cuda"),
                torch.zeros(L, dtype=torch.int32, device="cuda")) for _ in range(3)]
     for p in range(3):
@@ -108,6 +115,8 @@ def _run_case(irismpc, orc, torch, W, alloc, A, seed, counter=0, threads=False, 
                                           nonce=0, counter=counter, want_lifted=True)
     for p in range(3):
         assert np.array_equal(lifted[p][0].cpu().numpy().view(np.uint32), lift_ref[p]), f"lift p{p}"
+        assert np.array_equal(lifted[p][1].cpu().numpy().view(np.uint32), lift_ref[(p + 2) % 3]), \
+            f"lift b p{p}"
         assert np.array_equal(got_a[p], ra[p]), f"party {p} share a"
         assert np.array_equal(got_b[p], rb[p]), f"party {p} share b"
         assert np.array_equal(got_c[p], rb[(p + 2) % 3]), f"party {p} opened c"
@@ -122,11 +131,12 @@ def _run_case(irismpc, orc, torch, W, alloc, A, seed, counter=0, threads=False, 
         assert np.array_equal(mine, opened), "all parties open the same bits"
 
 
+@pytest.mark.parametrize("alias_b", [False, True])
 @pytest.mark.parametrize("A", [20316, 16384])
-def test_threshold_share_parity(irismpc, orc, A):
+def test_threshold_share_parity(irismpc, orc, A, alias_b):
     import torch
 
-    _run_case(irismpc, orc, torch, W=512, alloc=512, A=A, seed=0xDEADBEEF)
+    _run_case(irismpc, orc, torch, W=512, alloc=512, A=A, seed=0xDEADBEEF, alias_b=alias_b)
 
 
 def test_threshold_alloc_and_counter(irismpc, orc):
@@ -144,14 +154,15 @@ def test_threshold_three_party_threads(irismpc, orc):
     _run_case(irismpc, orc, torch, W=256, alloc=256, A=20316, seed=12, threads=True)
 
 
+@pytest.mark.parametrize("alias_b", [False, True])
 @pytest.mark.parametrize("W,alloc", [(264, 264), (40, 72)])
-def test_threshold_partial_blocks(irismpc, orc, W, alloc):
+def test_threshold_partial_blocks(irismpc, orc, W, alloc, alias_b):
     """Word counts that leave the last block of the word adders (32 words) and
     of the bit injection (2,048 elements) partly empty, with the allocation
     larger than the chunk in the second case."""
     import torch
 
-    _run_case(irismpc, orc, torch, W=W, alloc=alloc, A=16384, seed=W)
+    _run_case(irismpc, orc, torch, W=W, alloc=alloc, A=16384, seed=W, alias_b=alias_b)
 
 
 def _boundary_sample(A):
@@ -167,13 +178,15 @@ def _boundary_sample(A):
     return sample
 
 
+@pytest.mark.parametrize("alias_b", [False, True])
 @pytest.mark.parametrize("A", [20316, 16384])
-def test_threshold_boundary(irismpc, orc, A):
+def test_threshold_boundary(irismpc, orc, A, alias_b):
     """Every element sits within two of the threshold; shares and opened bits
     must equal the oracle's, so a lift or comparison off by one shows."""
     import torch
 
-    _run_case(irismpc, orc, torch, W=256, alloc=256, A=A, seed=A, sampler=_boundary_sample(A))
+    _run_case(irismpc, orc, torch, W=256, alloc=256, A=A, seed=A, sampler=_boundary_sample(A),
+              alias_b=alias_b)
 
 
 def test_threshold_unpaired_seeds_rejected(irismpc):
@@ -195,7 +208,8 @@ def test_threshold_unpaired_seeds_rejected(irismpc):
     assert all(c.counter == 0 for c in circ)
 
 
-def test_threshold_full_chunk(irismpc, orc):
+@pytest.mark.parametrize("alias_b", [False, True])
+def test_threshold_full_chunk(irismpc, orc, alias_b):
     """One full chunk of the scan: 1,984 query-rotation rows x 32,768 DB rows =
     1,015,808 result words, share for share against the oracle (its simulator
     on the host's threads)."""
@@ -206,6 +220,6 @@ def test_threshold_full_chunk(irismpc, orc):
     orc.set_threads(min(16, os.cpu_count() or 1))
     try:
         W = 1984 * 32768 // 64
-        _run_case(irismpc, orc, torch, W=W, alloc=W, A=16384, seed=77)
+        _run_case(irismpc, orc, torch, W=W, alloc=W, A=16384, seed=77, alias_b=alias_b)
     finally:
         orc.set_threads(1)
