@@ -240,12 +240,13 @@ struct PlaneSlices {
   __device__ __forceinline__ uint32_t b(int j) const { return pl[(MSB ? 32 : 16) + j]; }
 };
 
-// MSB planes when every party's b share is the previous party's a share (the
-// reshare's zero-copy views): a wave forms its own a planes only and reads
-// plane j of its b share from the previous party's wave through LDS.
+// Planes when every party's b share is the previous party's a share (the
+// reshare's zero-copy views): a wave forms its own a planes only (pl[j], of
+// which the lift uses 16) and reads plane j of its b share from the previous
+// party's wave through LDS.
 struct PlanesPrevB {
   uint32_t pl[32];
-  const uint32_t* prev;  // LDS [32][64] planes of the previous party, at this lane
+  const uint32_t* prev;  // LDS [planes][64] of the previous party, at this lane
   __device__ __forceinline__ uint32_t a(int j) const { return pl[j]; }
   __device__ __forceinline__ uint32_t b(int j) const { return prev[64 * j]; }
 };
@@ -410,15 +411,17 @@ __device__ __forceinline__ void adder_batch(const SL& W, AdderState& st,
 #undef IRISMPC_STEP
 }
 
-// kPrevB (MSB stage only): every party's b share is the previous party's a
-// share, element for element (the caller passes the previous party's a
-// buffers as b), so b planes come from the previous party's wave (PlanesPrevB).
+// kPrevB: every party's b share is the previous party's a share, element for
+// element (the caller passes the previous party's a buffers as b), so a wave
+// forms its own a planes only and takes the b planes from the previous
+// party's wave (PlanesPrevB): half the loads and transposes, and the register
+// file of the b planes freed.
 template <bool MSB, bool kPrevB>
 __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t A, const RippleCtrs rc) {
-  static_assert(MSB || !kPrevB, "the b planes of the lift come with its a planes in one transpose");
   constexpr int N_BATCH = MSB ? 4 : 2;   // steps 0 .. 31 / 0 .. 15, 8 per batch
+  constexpr int NPL = MSB ? 32 : 16;     // planes per share
   __shared__ AdderLds L;
-  __shared__ uint32_t a_planes[kPrevB ? 3 * 32 * 64 : 1];  // [party][plane][lane]
+  __shared__ uint32_t a_planes[kPrevB ? 3 * NPL * 64 : 1];  // [party][plane][lane]
   // one party per wave: p is wave-uniform, so its keys and pointers stay scalar
   const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63), prev = (p + 2) % 3;
@@ -444,7 +447,12 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const size_t e = 32 * hw + 8 * q;
-    if constexpr (kPrevB) {
+    if constexpr (kPrevB && !MSB) {
+      uint16_t ma[8] = {};
+      if (live) u16x8_load(P.mask_a + e, ma);
+#pragma unroll
+      for (int i = 0; i < 8; i++) sl.pl[8 * q + i] = ma[i];
+    } else if constexpr (kPrevB) {
       // the a side of the general form below; the b side is the previous wave's
       uint16_t ma[8] = {}, c1a[8] = {}, c2a[8] = {}, xa[8] = {};
       if (live) {
@@ -512,8 +520,8 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
   bit_transpose32(sl.pl);
   if constexpr (kPrevB) {
 #pragma unroll
-    for (int j = 0; j < 32; j++) a_planes[(p * 32 + j) * 64 + lane] = sl.pl[j];
-    sl.prev = &a_planes[prev * 32 * 64 + lane];
+    for (int j = 0; j < NPL; j++) a_planes[(p * NPL + j) * 64 + lane] = sl.pl[j];
+    sl.prev = &a_planes[prev * NPL * 64 + lane];
   } else if constexpr (MSB) {
     bit_transpose32(sl.pl + 32);
   }
@@ -605,6 +613,8 @@ void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const Ripp
     adder_word_kernel<true, true><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
   else if (msb)
     adder_word_kernel<true, false><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
+  else if (prev_b)
+    adder_word_kernel<false, true><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
   else
     adder_word_kernel<false, false><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
   IRIS_HIP_CHECK(hipGetLastError());
