@@ -411,6 +411,53 @@ __device__ __forceinline__ void adder_batch(const SL& W, AdderState& st,
 #undef IRISMPC_STEP
 }
 
+// The adder's batches of 8 steps, the first batch's blocks already in L.rnd
+// (published by a barrier) and d free: each batch reads its gates' bits, then
+// runs its steps while computing the next batch's block.
+template <bool MSB, typename SL>
+__device__ __forceinline__ void adder_batches(const SL& sl, AdderState& st, AdderLds& L, const RippleCtrs& rc,
+                                              size_t G, size_t g, int p, int prev, int lane, NextDraw& d,
+                                              const ChaChaKey& key, uint64_t nonce) {
+  constexpr int N_BATCH = MSB ? 4 : 2;
+  const int grp = lane >> 4, gl = lane & 15;
+  uint32_t* rnd_row = L.rnd[p][grp][gl];
+  uint32_t r_out = 0;
+#pragma unroll 1
+  for (int b = 0; b < N_BATCH; b++) {
+    uint32_t r16[16];  // F(own) ^ F(prev): bits of the 16 gates for this lane's half word
+#pragma unroll
+    for (int c = 0; c < 16; c++) r16[c] = L.rnd[p][grp][c][gl] ^ L.rnd[prev][grp][c][gl];
+    if (!MSB && b == 0) r_out = r16[1];
+    if (b + 1 < N_BATCH) {
+      batch_gate<MSB>(b + 1, gl, rc, G, g, d.valid, d.ctr);
+      chacha_start(key, nonce, d.ctr, d.x);
+    }
+    switch (b) {
+      case 0: adder_batch<MSB, 0>(sl, st, r16, r_out, L, p, prev, lane, d, key, nonce, rnd_row); break;
+      case 1: adder_batch<MSB, 1>(sl, st, r16, r_out, L, p, prev, lane, d, key, nonce, rnd_row); break;
+      case 2:
+        if constexpr (MSB) adder_batch<MSB, 2>(sl, st, r16, r_out, L, p, prev, lane, d, key, nonce, rnd_row);
+        break;
+      default:
+        if constexpr (MSB) adder_batch<MSB, 3>(sl, st, r16, r_out, L, p, prev, lane, d, key, nonce, rnd_row);
+        break;
+    }
+  }
+}
+
+// Writes the first batch's block of this lane's gate to L.rnd (published by
+// the caller's next barrier).
+template <bool MSB>
+__device__ __forceinline__ void adder_first_draw(AdderLds& L, const RippleCtrs& rc, size_t G, size_t g, int p,
+                                                 int lane, NextDraw& d, const ChaChaKey& key, uint64_t nonce) {
+  const int grp = lane >> 4, gl = lane & 15;
+  batch_gate<MSB>(0, gl, rc, G, g, d.valid, d.ctr);
+  uint32_t k[16] = {};
+  if (d.valid) chacha_block<12>(key, nonce, d.ctr, k);
+#pragma unroll
+  for (int i = 0; i < 16; i++) L.rnd[p][grp][gl][i] = k[i];
+}
+
 // kPrevB: every party's b share is the previous party's a share, element for
 // element (the caller passes the previous party's a buffers as b), so a wave
 // forms its own a planes only and takes the b planes from the previous
@@ -418,30 +465,22 @@ __device__ __forceinline__ void adder_batch(const SL& W, AdderState& st,
 // file of the b planes freed.
 template <bool MSB, bool kPrevB>
 __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t A, const RippleCtrs rc) {
-  constexpr int N_BATCH = MSB ? 4 : 2;   // steps 0 .. 31 / 0 .. 15, 8 per batch
   constexpr int NPL = MSB ? 32 : 16;     // planes per share
   __shared__ AdderLds L;
   __shared__ uint32_t a_planes[kPrevB ? 3 * NPL * 64 : 1];  // [party][plane][lane]
   // one party per wave: p is wave-uniform, so its keys and pointers stay scalar
   const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63), prev = (p + 2) % 3;
-  const int grp = lane >> 4, gl = lane & 15;  // 16-lane group = 8 words = one ChaCha block
+  const int grp = lane >> 4;  // 16-lane group = 8 words = one ChaCha block
   const CircuitBufs& P = t.p[p];
   const size_t W = t.W, G = W / 8, n_el = 64 * W;
   const size_t hw = (size_t)blockIdx.x * 64 + lane;  // half word: elements 32 hw .. 32 hw + 31
   const size_t g = (size_t)blockIdx.x * 4 + grp;     // ChaCha block of words 8 g .. 8 g + 7
   const bool live = hw < 2 * W;
-  uint32_t* rnd_row = L.rnd[p][grp][gl];
 
   // batch 0's block first: its rounds overlap the loads below
   NextDraw d;
-  batch_gate<MSB>(0, gl, rc, G, g, d.valid, d.ctr);
-  {
-    uint32_t k[16] = {};
-    if (d.valid) chacha_block<12>(P.key_own, t.nonce, d.ctr, k);
-#pragma unroll
-    for (int i = 0; i < 16; i++) rnd_row[i] = k[i];
-  }
+  adder_first_draw<MSB>(L, rc, G, g, p, lane, d, P.key_own, t.nonce);
 
   std::conditional_t<kPrevB, PlanesPrevB, PlaneSlices<MSB>> sl;
 #pragma unroll
@@ -528,28 +567,7 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
   __syncthreads();
 
   AdderState st{};
-  uint32_t r_out = 0;
-#pragma unroll 1
-  for (int b = 0; b < N_BATCH; b++) {
-    uint32_t r16[16];  // F(own) ^ F(prev): bits of the 16 gates for this lane's half word
-#pragma unroll
-    for (int c = 0; c < 16; c++) r16[c] = L.rnd[p][grp][c][gl] ^ L.rnd[prev][grp][c][gl];
-    if (!MSB && b == 0) r_out = r16[1];
-    if (b + 1 < N_BATCH) {
-      batch_gate<MSB>(b + 1, gl, rc, G, g, d.valid, d.ctr);
-      chacha_start(P.key_own, t.nonce, d.ctr, d.x);
-    }
-    switch (b) {
-      case 0: adder_batch<MSB, 0>(sl, st, r16, r_out, L, p, prev, lane, d, P.key_own, t.nonce, rnd_row); break;
-      case 1: adder_batch<MSB, 1>(sl, st, r16, r_out, L, p, prev, lane, d, P.key_own, t.nonce, rnd_row); break;
-      case 2:
-        if constexpr (MSB) adder_batch<MSB, 2>(sl, st, r16, r_out, L, p, prev, lane, d, P.key_own, t.nonce, rnd_row);
-        break;
-      default:
-        if constexpr (MSB) adder_batch<MSB, 3>(sl, st, r16, r_out, L, p, prev, lane, d, P.key_own, t.nonce, rnd_row);
-        break;
-    }
-  }
+  adder_batches<MSB>(sl, st, L, rc, G, g, p, prev, lane, d, P.key_own, t.nonce);
   if (!live) return;
   if constexpr (MSB) {
     // msb = c ^ a_30 ^ b_30, a_30 = plane 31 (protocol.rs:2347-2352)
@@ -564,6 +582,172 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
     reinterpret_cast<uint32_t*>(P.c_a + W)[hw] = st.va;
     reinterpret_cast<uint32_t*>(P.c_b + W)[hw] = st.vb;
   }
+}
+
+// ------------------------------------------------------------ fused compare
+// lift -> bit injection -> lift_mul_sub -> MSB adder in one launch, for b
+// views that are the previous party's a buffers (the scan's zero-copy
+// reshare). A block holds the same 2,048 elements (32 words) of the three
+// parties, one wave each, and a lane keeps the 32 elements of its half word
+// through all the stages:
+//  * the lift's carries stay in the lane's registers as bit slices (bit 16:
+//    xa, xb; bit 17: va, vb -- the lift kernel's c planes 0 and 1);
+//  * the injection of element e draws u16 e of each stream, so a lane's 32
+//    elements are exactly ChaCha12 block ctr + hw (bit 16) and ctr + 2 W + hw
+//    (bit 17): the lane computes its own party's two blocks and reads the
+//    previous party's from LDS, and party 1's messages reach party 2 through
+//    LDS;
+//  * the corrections enter the lifted mask in registers.
+// No carry plane or correction goes through HBM. Shares, counters and opened
+// bits are those of the three launches (adder_word_kernel, inject_kernel,
+// adder_word_kernel).
+__device__ __forceinline__ uint32_t lo16(uint32_t w, int h) { return (w >> (16 * h)) & 0xffffu; }
+
+__global__ __launch_bounds__(192) void compare_fused_kernel(const Trio t, uint32_t A, const RippleCtrs lift_rc,
+                                                            uint64_t inject_ctr, const RippleCtrs msb_rc) {
+  __shared__ AdderLds L;
+  // a planes of the three parties ([party][plane][lane]: lift 16 planes, MSB
+  // 32), in between the injection's keystreams ([party][bit][word][lane])
+  __shared__ uint32_t xs[3 * 32 * 64];
+  const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63), prev = (p + 2) % 3;
+  const int grp = lane >> 4;
+  const CircuitBufs& P = t.p[p];
+  const size_t W = t.W, G = W / 8;
+  const size_t hw = (size_t)blockIdx.x * 64 + lane;  // half word: elements 32 hw .. 32 hw + 31
+  const size_t g = (size_t)blockIdx.x * 4 + grp;     // adders' ChaCha block of words 8 g .. 8 g + 7
+  const bool live = hw < 2 * W;
+
+  // ---- lift (protocol.rs:2041-2074,2137-2241)
+  NextDraw d;
+  adder_first_draw<false>(L, lift_rc, G, g, p, lane, d, P.key_own, t.nonce);
+  uint32_t mk[16];  // the lane's mask a shares, elements 2 i (low half) and 2 i + 1
+  PlanesPrevB sl;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    uint16_t ma[8] = {};
+    if (live) u16x8_load(P.mask_a + 32 * hw + 8 * q, ma);
+#pragma unroll
+    for (int i = 0; i < 8; i++) sl.pl[8 * q + i] = ma[i];
+#pragma unroll
+    for (int i = 0; i < 4; i++) mk[4 * q + i] = (uint32_t)ma[2 * i] | ((uint32_t)ma[2 * i + 1] << 16);
+  }
+  bit_transpose32(sl.pl);
+#pragma unroll
+  for (int j = 0; j < 16; j++) xs[(p * 16 + j) * 64 + lane] = sl.pl[j];
+  sl.prev = &xs[prev * 16 * 64 + lane];
+  __syncthreads();
+  AdderState st{};
+  adder_batches<false>(sl, st, L, lift_rc, G, g, p, prev, lane, d, P.key_own, t.nonce);
+  const uint32_t bits_a[2] = {st.xa, st.va}, bits_b[2] = {st.xb, st.vb};
+  __syncthreads();  // every wave is done with the lift planes (xs) and L.rnd
+
+  // ---- bit injection (protocol.rs:1302-1643, kernel.cu:526-629); messages
+  // P1 -> P2 through L.rnd, free between the adders
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    uint32_t k[16];
+    chacha_block<12>(P.key_own, t.nonce, inject_ctr + (uint64_t)c * 2 * W + hw, k);
+#pragma unroll
+    for (int w = 0; w < 16; w++) xs[((p * 2 + c) * 16 + w) * 64 + lane] = k[w];
+  }
+  __syncthreads();
+  uint32_t* msg = &L.rnd[0][0][0][0];  // [bit][word][lane], 2,048 of its 3,264 words
+  uint32_t cr[2][16];  // corrections (a share) of bits 16 and 17, two elements per word
+  if (p != 2) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      // P0 holds its b share of the carry, P1 the XOR of its two shares
+      const uint32_t bits = p == 0 ? bits_b[c] : bits_a[c] ^ bits_b[c];
+#pragma unroll
+      for (int w = 0; w < 16; w++) {
+        const uint32_t own = xs[((p * 2 + c) * 16 + w) * 64 + lane];
+        const uint32_t prv = xs[((prev * 2 + c) * 16 + w) * 64 + lane];
+        uint32_t out = 0, m = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const uint32_t b = (bits >> (2 * w + h)) & 1u;
+          uint32_t o;
+          if (p == 0) {  // rand_01 = my rng, rand_02 = their rng
+            const uint32_t r01 = lo16(own, h), r02 = lo16(prv, h);
+            const uint32_t y = b * r01 - r02;
+            o = r01 - (y << 1);
+          } else {  // rand_01 = their rng, rand_12 = my rng
+            const uint32_t r01 = lo16(prv, h), r12 = lo16(own, h);
+            const uint32_t x = b - r01;
+            m |= (x & 0xffffu) << (16 * h);
+            o = x - (r12 << 1);
+          }
+          out |= (o & 0xffffu) << (16 * h);
+        }
+        cr[c][w] = out;
+        if (p == 1) msg[(c * 16 + w) * 64 + lane] = m;
+      }
+    }
+  }
+  __syncthreads();
+  if (p == 2) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const uint32_t bits = bits_a[c];
+#pragma unroll
+      for (int w = 0; w < 16; w++) {
+        const uint32_t own = xs[((p * 2 + c) * 16 + w) * 64 + lane];
+        const uint32_t prv = xs[((prev * 2 + c) * 16 + w) * 64 + lane];
+        const uint32_t in = msg[(c * 16 + w) * 64 + lane];
+        uint32_t out = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          // rand_12 = their rng, rand_02 = my rng, x = P1's message
+          const uint32_t b = (bits >> (2 * w + h)) & 1u;
+          const uint32_t r12 = lo16(prv, h), r02 = lo16(own, h), x = lo16(in, h);
+          const uint32_t z = b * x - r12;
+          const uint32_t o = b - ((r02 + z) << 1);
+          out |= (o & 0xffffu) << (16 * h);
+        }
+        cr[c][w] = out;
+      }
+    }
+  }
+  __syncthreads();  // keystreams and messages are read before the MSB stage reuses xs and L.rnd
+
+  // ---- lift_mul_sub (protocol.rs:1831-1872, kernel.cu:181-195,407-426) and MSB adder
+  adder_first_draw<true>(L, msb_rc, G, g, p, lane, d, P.key_own, t.nonce);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const size_t e = 32 * hw + 8 * q;
+    uint16_t xa[8] = {};
+    if (live) u16x8_load(P.code_a + e, xa);
+    uint32_t la[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int el = 8 * q + i;
+      const uint32_t m = lo16(mk[el >> 1], el & 1);
+      const uint32_t c1 = lo16(cr[0][el >> 1], el & 1), c2 = lo16(cr[1][el >> 1], el & 1);
+      la[i] = m - (c1 << 16) - (c2 << 17);
+      sl.pl[el] = la[i] * A - ((uint32_t)xa[i] << 16) - (p == 0 ? 1u : 0u);
+    }
+    if (P.lifted_a && live) {  // own a, and the next party's b
+      uint4* da = reinterpret_cast<uint4*>(P.lifted_a + e);
+      uint4* db = reinterpret_cast<uint4*>(t.p[(p + 1) % 3].lifted_b + e);
+      da[0] = db[0] = uint4{la[0], la[1], la[2], la[3]};
+      da[1] = db[1] = uint4{la[4], la[5], la[6], la[7]};
+    }
+  }
+  bit_transpose32(sl.pl);
+#pragma unroll
+  for (int j = 0; j < 32; j++) xs[(p * 32 + j) * 64 + lane] = sl.pl[j];
+  sl.prev = &xs[prev * 32 * 64 + lane];
+  __syncthreads();
+  st = AdderState{};
+  adder_batches<true>(sl, st, L, msb_rc, G, g, p, prev, lane, d, P.key_own, t.nonce);
+  if (!live) return;
+  // msb = c ^ a_30 ^ b_30, a_30 = plane 31 (protocol.rs:2347-2352)
+  st.xa ^= st.va;
+  st.xb ^= st.vb;
+  const uint32_t ba = st.ca ^ (p == 2 ? st.ta : 0u), bb = st.cb ^ (p == 0 ? st.tb : 0u);
+  reinterpret_cast<uint32_t*>(P.res_a)[hw] = st.xa ^ sl.a(31) ^ ba;
+  reinterpret_cast<uint32_t*>(P.res_b)[hw] = st.xb ^ sl.b(31) ^ bb;
 }
 
 // ------------------------------------------------------------ open -> bitmap
@@ -603,12 +787,7 @@ void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const Ripp
   IRIS_REQUIRE(t.W % 8 == 0, "bit planes must be a multiple of 8 words");
   const unsigned blocks = (unsigned)ceil_div(t.W, 32);
   // b shares that are the previous party's a buffers: planes from the previous wave
-  bool prev_b = true;
-  for (int p = 0; p < 3; p++) {
-    const int q = (p + 2) % 3;
-    prev_b = prev_b && t.p[p].code_b == t.p[q].code_a && t.p[p].mask_b == t.p[q].mask_a &&
-             (t.p[p].lifted_a == nullptr) == (t.p[p].lifted_b == nullptr);
-  }
+  const bool prev_b = trio_b_is_prev(t);
   if (msb && prev_b)
     adder_word_kernel<true, true><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
   else if (msb)
@@ -617,6 +796,24 @@ void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const Ripp
     adder_word_kernel<false, true><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
   else
     adder_word_kernel<false, false><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
+bool trio_b_is_prev(const Trio& t) {
+  for (int p = 0; p < 3; p++) {
+    const int q = (p + 2) % 3;
+    if (t.p[p].code_b != t.p[q].code_a || t.p[p].mask_b != t.p[q].mask_a ||
+        (t.p[p].lifted_a == nullptr) != (t.p[p].lifted_b == nullptr))
+      return false;
+  }
+  return true;
+}
+
+void launch_compare_fused(const Trio& t, uint32_t threshold_a, const RippleCtrs& lift, uint64_t inject_ctr,
+                          const RippleCtrs& msb, hipStream_t s) {
+  IRIS_REQUIRE(t.W % 8 == 0, "bit planes must be a multiple of 8 words");
+  IRIS_REQUIRE(trio_b_is_prev(t), "the fused compare needs b views that are the previous party's a buffers");
+  compare_fused_kernel<<<(unsigned)ceil_div(t.W, 32), 192, 0, s>>>(t, threshold_a, lift, inject_ctr, msb);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 

@@ -59,6 +59,15 @@ struct RippleCtrs {
 // lift_mul_sub and the MSB adder (-> res). The bit planes stay in registers.
 void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const RippleCtrs& ctrs,
                        hipStream_t s);
+// True when every party's b views (code and mask) are the previous party's a
+// buffers, as the zero-copy reshare hands them over (and the optional lift
+// capture is set for all or none).
+bool trio_b_is_prev(const Trio& t);
+// lift, bit injection, lift_mul_sub and MSB adder in one launch (needs
+// trio_b_is_prev): carries and corrections stay in registers and LDS; the
+// result shares equal those of the three separate launches.
+void launch_compare_fused(const Trio& t, uint32_t threshold_a, const RippleCtrs& lift, uint64_t inject_ctr,
+                          const RippleCtrs& msb, hipStream_t s);
 // Bit injection of the two lift carries (all three parties): the a share
 // corr_a of every element's corrections (the b share is the previous party's
 // a share), the message steps inside one launch.

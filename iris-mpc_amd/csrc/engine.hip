@@ -747,13 +747,17 @@ void Circuits::compare_trio(Circuits* const c[3], const uint16_t* const code_a[3
   lift.c[15] = sc.lift_final;
   LaunchTimer& timer = c[0]->timer;
   if (timer.on()) timer.begin(ps);
-  launch_adder_word(t, false, A, lift, ps);
-  launch_inject(t, sc.inject, ps);
   RippleCtrs msb{};
   msb.csa = sc.msb_csa;
   msb.c[0] = sc.msb_first;
   for (int k = 1; k <= 29; k++) msb.c[k] = sc.msb_round[k];
-  launch_adder_word(t, true, A, msb, ps);
+  if (trio_b_is_prev(t)) {
+    launch_compare_fused(t, A, lift, sc.inject, msb, ps);
+  } else {
+    launch_adder_word(t, false, A, lift, ps);
+    launch_inject(t, sc.inject, ps);
+    launch_adder_word(t, true, A, msb, ps);
+  }
   if (timer.on()) timer.end(ps);
   if (fork) ex->fork_streams(s);
   for (int p = 0; p < 3; p++) c[p]->counter_ = sc.end;
