@@ -581,6 +581,15 @@ int irismpc_serving_submit(irismpc_serving* s, int kind, const uint16_t* code_sh
 int irismpc_serving_result(const irismpc_serving* s, uint64_t id, int* state, int64_t* value,
                            uint32_t* matches, size_t max_matches, size_t* n_matches, double* times,
                            size_t* db_rows, char* error, size_t error_len);
+/* The DB version of a finished request's answer: the number of the loop's
+ * DB writes (insertions and deletions that succeeded, counted from 1 in the
+ * order the loop applied them) that the answer reflects. A uniqueness answer
+ * is that of the DB after exactly these writes, whichever of them happened
+ * while the request was scanning (the serving form scans a written chunk
+ * again for every active request); a write reports its own position. No
+ * counterpart in the reference, whose batches see the writes applied before
+ * them (actor.rs:584-644); it states the same order for the serving form. */
+int irismpc_serving_result_version(const irismpc_serving* s, uint64_t id, uint64_t* db_version);
 /* Forgets a finished request (fails while it is pending). A finished
  * request's shares are dropped at once; its result stays until released, so a
  * long-running host releases every request it has read. */

@@ -928,6 +928,15 @@ int irismpc_serving_result(const irismpc_serving* s, uint64_t id, int* state, in
   });
 }
 
+int irismpc_serving_result_version(const irismpc_serving* s, uint64_t id, uint64_t* db_version) {
+  return guarded([&] {
+    Serving::Result r;
+    IRIS_REQUIRE(s->impl->result(id, &r), "unknown request id");
+    IRIS_REQUIRE(r.state != kReqPending, "request still pending");
+    *db_version = r.db_version;
+  });
+}
+
 int irismpc_serving_release(irismpc_serving* s, uint64_t id) {
   return guarded([&] { IRIS_REQUIRE(s->impl->release(id), "request still pending"); });
 }

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <limits>
+#include <set>
 
 #include "comparator.hpp"
 #include "host_crypto.hpp"
@@ -481,9 +482,36 @@ void Serving::apply_write(Req* r) {
     std::lock_guard<std::mutex> lk(mu_);
     if (r->kind == kReqInsert) n_rows_++;
     r->res.db_rows = n_rows_;
+    r->res.db_version = ++n_writes_;
   }
   r->res.value = (int64_t)row;
   complete(r, kReqDone);
+}
+
+void Serving::clear_chunk(int slot, size_t chunk) {
+  const size_t w0 = (size_t)slot * words_ + chunk * (kDbChunk / 64);
+  const size_t n = std::min<size_t>(kDbChunk / 64, words_ - chunk * (kDbChunk / 64));
+  for (int p = 0; p < 3; p++) fill_u64(bitmaps_[p].ptr + w0, n, 0, scan_->s[p]);
+}
+
+void Serving::patch_step(const std::vector<Req*>& active, size_t chunk, size_t rows) {
+  for (Req* a : active) {
+    if (chunk >= a->need) {  // an insertion opened a chunk past the request's
+      a->covered.resize(chunk + 1, 0);
+      a->need = chunk + 1;
+    } else if (a->covered[chunk]) {
+      // ordered after the queued steps that set these bits
+      clear_chunk(a->slot, chunk);
+      a->covered[chunk] = 0;
+      a->n_covered--;
+    }
+  }
+  step(active.size(), chunk, 1, rows);
+  for (Req* a : active)
+    if (!a->covered[chunk]) {
+      a->covered[chunk] = 1;
+      a->n_covered++;
+    }
 }
 
 // ---------------------------------------------------------------- reauth
@@ -537,6 +565,10 @@ void Serving::serve_reauth(Req* r) {
   uint64_t word = 0;
   {
     std::lock_guard<std::mutex> lk(db_mu_);  // the window's rows are not written meanwhile
+    {
+      std::lock_guard<std::mutex> lk2(mu_);
+      r->res.db_version = n_writes_;
+    }
     for (int p = 0; p < 3; p++) IRIS_HIP_CHECK(hipStreamWaitEvent(small_->s[p], last_write_[p], 0));
     uint64_t* const bms[3] = {small_bitmaps_[0].ptr, small_bitmaps_[1].ptr, small_bitmaps_[2].ptr};
     scan_chunk_trio(small_p_, off, off + 64, 1, threshold_a_, bms, nullptr, 0, small_->s);
@@ -763,6 +795,7 @@ void Serving::loop_continuous(size_t reserve_bytes) {
   std::vector<int> free_slots;
   for (size_t i = 0; i < max_active_; i++) free_slots.push_back((int)i);
   std::vector<uint64_t> active_ids;
+  std::set<size_t> written;  // chunks written since the last step
   size_t cursor = 0;
   for (;;) {
     // finished readouts, oldest first
@@ -791,6 +824,7 @@ void Serving::loop_continuous(size_t reserve_bytes) {
         reauth_cv_.notify_one();
       } else {
         apply_write(r);
+        if (r->res.state == kReqDone) written.insert((size_t)r->res.value / kDbChunk);
       }
     }
     if ((stopping && in.empty() && idle) || abort_) break;
@@ -826,6 +860,7 @@ void Serving::loop_continuous(size_t reserve_bytes) {
       stats_.max_concurrent = std::max<uint64_t>(stats_.max_concurrent, active.size());
     }
     if (active.empty()) {
+      written.clear();  // later requests scan the DB as it is
       if (!reading.empty() && waiting.empty()) IRIS_HIP_CHECK(hipEventSynchronize(reading.front()->readout));
       continue;
     }
@@ -842,6 +877,10 @@ void Serving::loop_continuous(size_t reserve_bytes) {
     }
     set_slots(slots);
     const size_t rows = n_rows(), n = n_chunks_of(rows);
+    // every active request answers for the DB as it stands at its last step:
+    // a written chunk is scanned again for the whole active set right away
+    for (size_t c : written) patch_step(active, c, rows);
+    written.clear();
     cursor %= n;
     const size_t k = chunks_per_step(active.size(), cursor, n);
     step(active.size(), cursor, k, rows);
@@ -855,6 +894,11 @@ void Serving::loop_continuous(size_t reserve_bytes) {
     for (size_t i = 0; i < active.size();) {
       Req* r = active[i];
       if (r->n_covered == r->need) {  // its bitmap row is copied out after its last step
+        r->res.db_rows = rows;
+        {
+          std::lock_guard<std::mutex> lk(mu_);
+          r->res.db_version = n_writes_;
+        }
         r->readout = begin_readout(r->slot);
         reading.push_back(r);
         active.erase(active.begin() + i);
@@ -898,6 +942,10 @@ void Serving::loop_per_batch(size_t max_batch) {
     std::vector<const Req*> q(batch.begin(), batch.end());
     refill_scan(q, max_batch, max_batch * kRotations);
     const size_t rows = n_rows(), n = n_chunks_of(rows);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (Req* r : batch) r->res.db_version = n_writes_;
+    }
     std::vector<int> slots(batch.size());
     for (size_t j = 0; j < batch.size(); j++) slots[j] = (int)j;
     set_slots(slots);

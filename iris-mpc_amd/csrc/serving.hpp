@@ -19,7 +19,13 @@
 // Uniqueness requests join the active query set between steps while a slot is
 // free and free HBM stays above a reserve plus the request's footprint (with
 // nothing active one request always joins); a request leaves once it has
-// covered every chunk of the DB as it stood when it joined. Every step runs,
+// covered every chunk of the DB. After a write (insertion or deletion)
+// between steps, the written chunk is scanned again at once for the whole
+// active set (its stale bits cleared first; an insertion into a new chunk
+// adds that chunk), so each answer is that of the DB as it stands at the
+// request's last step: the DB after exactly Result::db_version of the loop's
+// writes, in the order the loop applied them (as a batch of the reference
+// sees the writes before it). Every step runs,
 // for all active queries, one share GEMM per matrix and party over 1-16
 // chunks, the reshare, the threshold circuits and the open into each
 // request's bitmap row (slot map). Nothing between steps drains the queued
@@ -143,6 +149,9 @@ class Serving {
     std::vector<uint32_t> matches;
     double arrival = 0, done = -1;
     size_t db_rows = 0;  // DB rows the request was checked against
+    // DB writes applied before the answer (a write: its own position, from 1):
+    // the answer is that of the DB after the first db_version writes
+    uint64_t db_version = 0;
     std::string error;
   };
   // false when the id is unknown
@@ -200,6 +209,10 @@ class Serving {
   void reauth_main();
   void serve_reauth(Req* r);
   void apply_write(Req* r);
+  // one step over a chunk written since the last step, for every active
+  // request (the chunk's stale bits cleared first), which then has it covered
+  void patch_step(const std::vector<Req*>& active, size_t chunk, size_t rows);
+  void clear_chunk(int slot, size_t chunk);
   void complete(Req* r, int state, const std::string& err = std::string());
   bool take_incoming(std::vector<Req*>* out, bool wait);
   void fail_all(const std::string& msg);
@@ -260,6 +273,7 @@ class Serving {
   std::vector<Req*> incoming_;
   std::deque<Req*> reauth_q_;
   uint64_t next_id_ = 1, n_submitted_ = 0, n_finished_ = 0;
+  uint64_t n_writes_ = 0;  // DB writes applied (the DB's version)
   size_t n_rows_;
   bool running_ = false, draining_ = false, reauth_stop_ = false;
   std::atomic<bool> abort_{false};  // a timed-out finish: leave at once

@@ -17,9 +17,12 @@ The three parties' DB shares and protocol state stay on the device. The scan
 of actor.rs:2246-2589 becomes a ring of chunk steps (32,768 rows each):
 - Uniqueness requests join the active query set between chunk steps, as slots
   free up and while free HBM stays above a reserve.
-- A request leaves when it has covered every chunk of the DB as it was when
-  it joined, so requests that arrive during a scan start at the next step
-  instead of waiting for the scan to finish.
+- A request leaves when it has covered every chunk of the DB, so requests
+  that arrive during a scan start at the next step instead of waiting for
+  the scan to finish. A chunk written (insertion, deletion) while requests
+  are active is scanned again for all of them at once, so every answer is
+  that of the DB at the request's last step: after exactly
+  Request.db_version of the loop's writes, in the order it applied them.
 - Every step runs one share GEMM per matrix and party for all active queries,
   then the reshare, the threshold circuits and the open into each request's
   bitmap row (slot map).
@@ -33,8 +36,8 @@ The per-batch form (PerBatchServer) runs the reference's batch after batch
 (actor.rs:895-901), one full scan per batch.
 
 The opened bits do not depend on the mix (the masks do), so each result
-equals the result of the same request run alone (ServingEngines.run_alone,
-the plain per-request form).
+equals the result of the same request run alone on the same DB version
+(ServingEngines.run_alone, the plain per-request form).
 """
 import ctypes
 from dataclasses import dataclass
@@ -63,6 +66,7 @@ class Request:
     done: float = -1.0
     error: str = None         # set when serving the request failed
     db_rows: int = 0          # DB rows the request was checked against
+    db_version: int = 0       # DB writes the answer reflects (a write: its own position, from 1)
 
 
 def _pack_shares(shares):
@@ -198,6 +202,9 @@ class ServingEngines:
             r.error = err.value.decode(errors="replace")
             return
         assert state.value == DONE, f"request {r.kind} rid {r.rid} unfinished"
+        version = ctypes.c_uint64()
+        call("irismpc_serving_result_version", self.h, i, ctypes.byref(version))
+        r.db_version = version.value
         if r.kind == "uniqueness":
             out = np.zeros(max(1, n.value), np.uint32)
             if n.value:

@@ -277,3 +277,92 @@ def test_native_loop_records_and_release(served):
     serving.call("irismpc_serving_release", eng.h, i)
     with pytest.raises(irismpc.IrisMpcError, match="unknown request id"):
         eng.fetch(i, r, t0)
+
+
+def _versioned_db_matches(orc, c, m, base, writes, version):
+    """Brute force over the DB after the first `version` writes: base = (codes,
+    masks, n_rows, U) of the tiled DB (row r holds distinct iris r mod U);
+    writes = [(kind, row, code, mask)] in the loop's order. A deleted row holds
+    the default iris (code 0, mask all ones) that deletions write."""
+    codes, masks, n_rows, u = base
+    hit = np.tile(orc.plain_match(c, m, codes, masks, 16384), -(-n_rows // u))[:n_rows].tolist()
+    dummy = orc.plain_match(c, m, np.zeros((1, 200), np.uint64), np.full((1, 200), ~np.uint64(0)), 16384)[0]
+    for kind, row, wc, wm in writes[:version]:
+        if kind == "insert":
+            assert row == len(hit)
+            hit.append(bool(orc.plain_match(c, m, wc[None], wm[None], 16384)[0]))
+        else:
+            hit[row] = bool(dummy)
+    return [r for r, h in enumerate(hit) if h]
+
+
+@pytest.mark.parametrize("form", ["serving", "per_batch"])
+def test_answers_follow_the_writes_in_order(orc, form):
+    """Insertions and deletions land while uniqueness requests scan (one chunk
+    per step, so every request spans several steps): a chunk written during a
+    request's scan is scanned again for the whole active set, so each answer
+    equals the oracle's brute force over the DB after exactly the request's
+    db_version writes, replayed in the loop's order -- whichever writes
+    happened mid-scan. The per-batch form answers for the DB before each
+    batch. Rows: 3 chunks less one, so the second insertion opens a new
+    chunk."""
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from irismpc import serving, synth
+
+    n0, u = 3 * 32768 - 1, 1024
+    eng = serving.ServingEngines(n0, n0 + 3 * 32768, max_active=8, seed_base=41)
+    eng.build(u, 91, 92)
+    eng.step_table = {1: 1}
+    plain = tuple(map(np.array, zip(*[orc.synth_iris(91, i) for i in range(u)])))
+    R = serving.Request
+    rng = np.random.default_rng(17)
+    # irises the writes change: X and Z inserted, two copies of Y's row class deleted
+    xc, xm = synth.synth_iris(93, 0)
+    zc, zm = synth.synth_iris(93, 1)
+    y = int(rng.integers(u))
+
+    def shares(c, m, k):
+        sh = synth.shares_of(c[None], m[None], 94, k)
+        return [(sh[p][0][0], sh[p][1][0]) for p in range(3)]
+
+    reqs, queries, k = [], {}, 0
+    for j in range(8):  # uniqueness every millisecond: X, Y, Z (10-25 % flips) in turn, two fresh irises
+        which = j % 4
+        if which == 3:
+            qc, qm = synth.synth_iris(95, j)
+        else:
+            bc, bm = ((xc, xm), (plain[0][y], plain[1][y]), (zc, zm))[which]
+            qc, qm = synth.synth_flip(bc, 96, j, int(rng.integers(1280, 3201))), bm
+        queries[k] = (qc, qm)
+        reqs.append(R("uniqueness", 0.001 * j, shares=shares(qc, qm, k), rid=k))
+        k += 1
+    writes_in = [("insert", 0.0005, xc, xm), ("delete", 0.0015, y + u, None), ("insert", 0.0025, zc, zm),
+                 ("delete", 0.0035, y + 5 * u, None), ("insert", 0.0045, *synth.synth_iris(93, 2))]
+    for kind, t, a, b in writes_in:
+        if kind == "insert":
+            reqs.append(R("insert", t, shares=shares(a, b, k), rid=k))
+        else:
+            reqs.append(R("delete", t, target=a, rid=k))
+        k += 1
+    srv = serving.Server(eng, reserve_bytes=0) if form == "serving" else serving.PerBatchServer(eng, 4)
+    out = {r.rid: r for r in srv.run(reqs)}
+    writes = sorted((r for r in out.values() if r.kind in ("insert", "delete")), key=lambda r: r.db_version)
+    assert [w.db_version for w in writes] == [1, 2, 3, 4, 5]
+    replay = []
+    for w in writes:
+        kind, _, a, b = writes_in[w.rid - 8]
+        replay.append((kind, w.result, a, b) if kind == "insert" else (kind, w.result, None, None))
+    assert [w.result for w in writes if w.kind == "insert"] == [n0, n0 + 1, n0 + 2]  # the second opens chunk 3
+    base = (plain[0], plain[1], n0, u)
+    mid_scan = 0
+    for j in range(8):
+        r = out[j]
+        assert r.error is None
+        assert r.result == _versioned_db_matches(orc, *queries[j], base, replay, r.db_version), \
+            f"uniqueness {j} at DB version {r.db_version}"
+        mid_scan += sum(1 for w in writes if w.arrival > r.arrival and w.db_version <= r.db_version)
+    # writes did land while requests were being answered (not only before or after them)
+    assert form == "per_batch" or mid_scan > 0
