@@ -40,10 +40,11 @@ VALU_PEAK_TOPS = 39.3
 # party and gate: lift 32 ANDs + MSB adder 61 ANDs -> 93 / 512; bit injection
 # draws 2 u16 per pair from each pairwise stream -> 4 / 64
 CHACHA_BLOCKS_PER_PAIR = 93 / 512 + 4 / 64
-# HBM bytes per pair per party of the circuit stage: the lift reads the mask
-# share (a, b: 4 B), the injection writes its corrections (8 B), the MSB adder
-# reads code, mask and corrections (16 B), the opens read ~3 bits
-CIRCUIT_BYTES_PER_PAIR = 4 + 8 + 16 + 3 / 8
+# HBM bytes per pair per party of the circuit stage in the scan: the fused
+# compare reads the code and mask a shares (4 B; every b share is the previous
+# party's a, read by that party's wave), carries and corrections stay on chip;
+# the opens read ~3 bits
+CIRCUIT_BYTES_PER_PAIR = 4 + 3 / 8
 CHACHA12_OPS = 600  # 6 double rounds x 8 quarter rounds x 12 ops + feed-forward and set-up
 THRESHOLD_ANON_A = 16384  # 0.375: the prefilter threshold of the full scan
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "share_gemm_c2_traffic.json")
@@ -522,7 +523,7 @@ def run_c3(args):
                      "basis": "share GEMM launches inside the timed batches (6 per chunk: 3 parties x code, "
                               "mask, in order on one stream; HIP events on that stream): 115,200 int8 ops per "
                               "pair per party / launch duration"},
-        "streaming_stage": {"bound": "valu", "kernels": "adder_word_kernel<false/true>, inject_kernel",
+        "streaming_stage": {"bound": "valu", "kernels": "compare_fused_kernel (lift, injection, MSB adder)",
                             "compares": circ_n, "avg_compare_ms": circ_ms / max(circ_n, 1),
                             "chacha12_ops": chacha_ops,
                             "achieved_Tops": chacha_ops / (circ_ms / 1e3) / 1e12 if circ_ms else None,
@@ -532,7 +533,7 @@ def run_c3(args):
                             "hbm_frac": circ_bytes / (circ_ms / 1e3) / 1e9 / HBM_PEAK_GBS if circ_ms else None,
                             "note": f"{CHACHA_BLOCKS_PER_PAIR:.3f} distinct ChaCha12 blocks x {CHACHA12_OPS} "
                                     f"int32 ops and {CIRCUIT_BYTES_PER_PAIR:.1f} HBM bytes per pair per party; "
-                                    "one compare = lift + injection + MSB adder of one chunk, three parties, "
+                                    "one compare = the fused lift + injection + MSB adder of one chunk, three parties, "
                                     "HIP events on the protocol stream while the next chunk's GEMMs run"},
     }
     if latency:
@@ -938,7 +939,7 @@ def run_c5(args):
     out = sorted(serving.Server(eng).run(mixed), key=lambda r: r.rid)
     same = [r.result for r in out] == [eng.run_alone(sh) for sh in alone]
 
-    bounds = [100.0, 150.0, 250.0, 500.0, 1000.0, 2000.0]
+    bounds = [100.0, 150.0, 200.0, 250.0, 500.0, 1000.0, 2000.0]
     sustained = {}
     for b in bounds:
         row = {}
