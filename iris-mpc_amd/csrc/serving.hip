@@ -477,9 +477,8 @@ void Serving::apply_write(Req* r) {
         IRIS_HIP_CHECK(hipEventRecord(last_write_[p], s));
       }
     }
-  }
-  {
-    std::lock_guard<std::mutex> lk(mu_);
+    // under db_mu_ as well: a reauth reads the version with the write it sees
+    std::lock_guard<std::mutex> lk2(mu_);
     if (r->kind == kReqInsert) n_rows_++;
     r->res.db_rows = n_rows_;
     r->res.db_version = ++n_writes_;
