@@ -3,13 +3,13 @@
 
 The DB holds 32,768 distinct synthetic irises per eye tiled over the rows
 (row r = iris r mod 32,768) and 32 of the 64 queries are copies of DB irises
-with 10-30% of their code bits flipped, so the plaintext answer is known by
-construction (planted query j matches exactly the rows r with r mod 32,768 =
-target_j, in both eyes; other queries match nothing: random irises sit near
-FHD 0.5). Checked, size-independently: every party's opened bitmap of each eye
-equals that answer bit for bit, and every party's AND-over-eyes merge returns
-the first matching row. Smaller multi-chunk scans are checked against the
-oracle share for share in test_gpu_pipeline.py.
+with 10-30% of their code bits flipped. The expected answer is the oracle's
+plaintext brute force of every query rotation against the 32,768 distinct
+irises with the integer MPC predicate, tiled over the rows (the planted
+copies are among its matches). Checked: every party's opened bitmap of each
+eye equals it bit for bit, and every party's AND-over-eyes merge returns the
+first row matching in both eyes. Smaller multi-chunk scans are checked
+against the oracle share for share in test_gpu_pipeline.py.
 """
 import numpy as np
 import pytest
@@ -32,7 +32,7 @@ def irismpc():
     return m
 
 
-def test_c3_fullsize_planted(irismpc):
+def test_c3_fullsize_planted(irismpc, orc):
     import torch
 
     from irismpc import synth
@@ -44,7 +44,7 @@ def test_c3_fullsize_planted(irismpc):
     flips = rng.integers(1280, 3841, size=B // 2)
     planted = [(j, int(t), int(f)) for j, t, f in zip(range(B // 2), targets, flips)]
     base = [np.full(8, p, np.uint32) for p in range(3)]
-    eyes = []
+    eyes, expects = [], []
     for eye in range(2):
         ds = 61 + eye
         ex = irismpc.Exchange(timeout_s=300)
@@ -59,7 +59,10 @@ def test_c3_fullsize_planted(irismpc):
             bitmaps.append(torch.zeros(N * B // 64 + 2, dtype=torch.int64, device="cuda"))
             streams.append(irismpc.stream_create())
         synth.build_db([pe.code_db for pe in parties], [pe.mask_db for pe in parties], N, U, ds, ds + 30)
-        _, _, shares = synth.make_queries(B, planted, ds, ds + 20, ds + 10, ds + 40)
+        q_codes, q_masks, shares = synth.make_queries(B, planted, ds, ds + 20, ds + 10, ds + 40)
+        db_c, db_m = map(np.array, zip(*[orc.synth_iris(ds, u) for u in range(U)]))
+        hit = np.stack([orc.plain_match(q_codes[j], q_masks[j], db_c, db_m, A_0375) for j in range(B)])
+        expects.append(np.tile(hit, (1, N // U)))
         qcs = [parties[p].codes.upload_query_shares(shares[p][0]) for p in range(3)]
         qms = [parties[p].masks.upload_query_shares(shares[p][1]) for p in range(3)]
         print(f"eye {eye}: DB built, scanning", flush=True)
@@ -68,18 +71,16 @@ def test_c3_fullsize_planted(irismpc):
             irismpc.stream_synchronize(s)
         eyes.append((parties, bitmaps))
 
-    expect = np.zeros((B, N), bool)
     rows = np.arange(N)
-    for j, t, _ in planted:
-        expect[j] = (rows % U) == t
-    expect_words = np.packbits(expect.reshape(-1), bitorder="little").view(np.uint64)
+    for j, t, _ in planted:  # the planted copies are among the brute-force matches, in both eyes
+        assert all(e[j][rows % U == t].all() for e in expects)
     for eye, (parties, bitmaps) in enumerate(eyes):
+        expect_words = np.packbits(expects[eye].reshape(-1), bitorder="little").view(np.uint64)
         for p in range(3):
             got = bitmaps[p].cpu().numpy().view(np.uint64)[: N * B // 64]
             assert np.array_equal(got, expect_words), f"eye {eye} party {p} bitmap"
-    want = np.full(B, 0xFFFFFFFF, np.uint32)
-    for j, t, _ in planted:
-        want[j] = t
+    both = expects[0] & expects[1]
+    want = np.array([np.flatnonzero(both[j])[0] if both[j].any() else 0xFFFFFFFF for j in range(B)], np.uint32)
     for p in range(3):
         final = torch.full((B,), -1, dtype=torch.int32, device="cuda")
         eyes[0][0][p].comparator.reset()

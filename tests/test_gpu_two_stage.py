@@ -13,8 +13,8 @@ left-only and right-only matches under the OR rule and without it; a query
 that is a rotated copy of its DB iris (rotation capture); a reauth target.
 
 test_two_stage_multi_chunk_subset: 40,960 candidates (more than one subset
-chunk of 32,768) on a 65,536-row DB of 64 distinct irises tiled, answer known
-by construction.
+chunk of 32,768) on a 65,536-row DB of 64 distinct irises tiled; candidates,
+final results and counters against the oracle's plaintext brute force.
 
 test_anon_distance_capture: the anonymized-statistics distance capture of both
 stages against the plaintext dot products.
@@ -53,6 +53,16 @@ def _rotation_bits(orc, db_c, db_m, q_c, q_m, A):
             r = (M * A - (c << 16) - 1) & 0xFFFFFFFF
             bits[j, t] = (r >> 31).astype(bool)
     return bits
+
+
+def _tiled_matches(orc, ds, U, N, q_codes, q_masks, A):
+    """[B][N] plaintext brute force of each query against a DB whose row r
+    holds synthetic iris (ds, r mod U): the oracle's integer MPC predicate
+    over all rotations against the U distinct irises, tiled."""
+    db_c, db_m = map(np.array, zip(*[orc.synth_iris(ds, u) for u in range(U)]))
+    reps = -(-N // U)
+    return np.stack([np.tile(orc.plain_match(q_codes[j], q_masks[j], db_c, db_m, A), reps)[:N]
+                     for j in range(len(q_codes))])
 
 
 def _engines(irismpc, N, Qr, tag_base=0):
@@ -169,7 +179,7 @@ def test_two_stage_vs_plaintext(irismpc, orc):
                                  for di, rs in dd.items() for r in rs}
 
 
-def test_two_stage_multi_chunk_subset(irismpc):
+def test_two_stage_multi_chunk_subset(irismpc, orc):
     import torch
 
     from irismpc import synth
@@ -179,7 +189,7 @@ def test_two_stage_multi_chunk_subset(irismpc):
     Qr = B * 31
     rng = np.random.default_rng(5)
     targets = rng.choice(U, size=40, replace=False)
-    eyes, queries, keep = {}, {}, []
+    eyes, queries, keep, plain = {}, {}, [], {}
     for k, e in enumerate(("left", "right")):
         ex, parties = _engines(irismpc, N, Qr, tag_base=3 * k)
         keep.append(ex)
@@ -187,7 +197,8 @@ def test_two_stage_multi_chunk_subset(irismpc):
         synth.build_db([pe.code_db for pe in parties], [pe.mask_db for pe in parties], N, U, ds, ds + 30)
         # left eye: all 40 planted; right eye: the even ones
         planted = [(j, int(targets[j]), 1000 + 30 * j) for j in range(40) if e == "left" or j % 2 == 0]
-        _, _, sh = synth.make_queries(B, planted, ds, ds + 20, ds + 10, ds + 40)
+        q_codes, q_masks, sh = synth.make_queries(B, planted, ds, ds + 20, ds + 10, ds + 40)
+        plain[e] = (ds, q_codes, q_masks)
         eyes[e] = parties
         queries[e] = ([parties[p].codes.upload_query_shares(sh[p][0]) for p in range(3)],
                       [parties[p].masks.upload_query_shares(sh[p][1]) for p in range(3)])
@@ -199,10 +210,15 @@ def test_two_stage_multi_chunk_subset(irismpc):
     cand = rows[np.isin(rows % U, targets)]
     assert cand.size == 40 * N // U > irismpc.DB_CHUNK_SIZE
     assert np.array_equal(res.candidates, cand)
-    want = np.full(B, 0xFFFFFFFF, np.uint32)
-    want[0:40:2] = targets[0:40:2]
-    cnt = np.zeros(B, np.uint32)
-    cnt[0:40:2] = N // U
+    # the oracle's brute force: candidates = rows some query matches in the
+    # left eye at 0.375; final = first row matching in both eyes at 0.345
+    pre = _tiled_matches(orc, plain["left"][0], U, N, *plain["left"][1:], A_0375)
+    assert np.array_equal(rows[pre.any(axis=0)], cand)
+    both = (_tiled_matches(orc, plain["left"][0], U, N, *plain["left"][1:], 20316) &
+            _tiled_matches(orc, plain["right"][0], U, N, *plain["right"][1:], 20316))
+    want = np.array([np.flatnonzero(both[j])[0] if both[j].any() else 0xFFFFFFFF for j in range(B)], np.uint32)
+    cnt = both.sum(axis=1).astype(np.uint32)
+    assert np.array_equal(want[0:40:2], targets[0:40:2]) and (cnt[0:40:2] == N // U).all()
     for p in range(3):
         assert np.array_equal(res.final_results[p], want), f"party {p} final results"
         assert np.array_equal(res.match_counters[p], cnt), f"party {p} match counters"
@@ -317,10 +333,10 @@ def test_anon_distance_capture(irismpc, orc):
             assert mask[k] == M[j, tt, d] & 0xFFFF
 
 
-def test_pipelined_scan_partial_last_chunk(irismpc):
+def test_pipelined_scan_partial_last_chunk(irismpc, orc):
     """The chunk loop with two result slots (GEMMs of chunk c + 1 beside the
     circuits of chunk c) over four chunks, the last one partial, equals a
-    chunk-by-chunk scan and the planted answer, bit for bit."""
+    chunk-by-chunk scan and the oracle's plaintext brute force, bit for bit."""
     import torch
 
     from irismpc import synth
@@ -332,14 +348,14 @@ def test_pipelined_scan_partial_last_chunk(irismpc):
     ex, parties = _engines(irismpc, N, Qr, tag_base=6)
     synth.build_db([pe.code_db for pe in parties], [pe.mask_db for pe in parties], N, U, 80, 110)
     planted = [(j, int(targets[j]), 1000 + 50 * j) for j in range(10)]
-    _, _, sh = synth.make_queries(B, planted, 80, 100, 90, 120)
+    q_codes, q_masks, sh = synth.make_queries(B, planted, 80, 100, 90, 120)
     qc = [parties[p].codes.upload_query_shares(sh[p][0]) for p in range(3)]
     qm = [parties[p].masks.upload_query_shares(sh[p][1]) for p in range(3)]
     streams = [irismpc.stream_create() for _ in range(3)]
     rows = np.arange(N)
-    want = np.zeros((B, N), bool)
-    for j, t, _ in planted:
-        want[j] = (rows % U) == t
+    want = _tiled_matches(orc, 80, U, N, q_codes, q_masks, A_0375)
+    for j, t, _ in planted:  # the planted copies are among the matches
+        assert want[j][rows % U == t].all()
 
     def bits(bm):
         return np.unpackbits(bm.cpu().numpy().view(np.uint8), bitorder="little")[: B * N].reshape(B, N)
@@ -363,7 +379,7 @@ def test_pipelined_scan_partial_last_chunk(irismpc):
     assert np.array_equal(maps[0].astype(bool), want)
 
 
-def test_multi_chunk_step_equals_single_steps(irismpc):
+def test_multi_chunk_step_equals_single_steps(irismpc, orc):
     """Several chunks in one step (dot shares side by side, one reshare and one
     comparison) open the same bits as one step per chunk: four chunks with a
     partial last one, a query count whose rows need 512-row padding, and a
@@ -379,14 +395,14 @@ def test_multi_chunk_step_equals_single_steps(irismpc):
     ex, parties = _engines(irismpc, N, 4 * rows, tag_base=9)  # result buffers for four chunks of `rows`
     synth.build_db([pe.code_db for pe in parties], [pe.mask_db for pe in parties], N, U, 90, 120)
     planted = [(j, int(targets[j]), 900 + 100 * j) for j in range(3)]
-    _, _, sh = synth.make_queries(B, planted, 90, 110, 100, 130)
+    q_codes, q_masks, sh = synth.make_queries(B, planted, 90, 110, 100, 130)
     qc = [parties[p].codes.upload_query_shares_rows(sh[p][0], rows) for p in range(3)]
     qm = [parties[p].masks.upload_query_shares_rows(sh[p][1], rows) for p in range(3)]
     streams = [irismpc.stream_create() for _ in range(3)]
     A = irismpc.THRESHOLD_ANON_STATS_A
-    want = np.zeros((B, N), bool)
-    for j, t, _ in planted:
-        want[j] = (np.arange(N) % U) == t
+    want = _tiled_matches(orc, 90, U, N, q_codes, q_masks, A)
+    for j, t, _ in planted:  # the planted copies are among the matches
+        assert want[j][np.arange(N) % U == t].all()
 
     def run(steps):
         bms = [torch.zeros(N * B // 64 + 2, dtype=torch.int64, device="cuda") for _ in range(3)]
