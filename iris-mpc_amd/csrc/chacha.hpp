@@ -50,8 +50,9 @@ __host__ __device__ __forceinline__ void chacha_block(const ChaChaKey& key, uint
 }
 
 // The same block in pieces, for consumers that spread one block's rounds over
-// other latency-bound work: chacha_start, ROUNDS / 2 chacha_double_round
-// calls, chacha_finish (which adds the input words back in place).
+// other latency-bound work: chacha_start, ROUNDS / 2 double rounds (each a
+// column and a diagonal round), chacha_finish (which adds the input words
+// back in place).
 __device__ __forceinline__ void chacha_start(const ChaChaKey& key, uint64_t nonce, uint64_t counter,
                                              uint32_t x[16]) {
   x[0] = 0x61707865u; x[1] = 0x3320646eu; x[2] = 0x79622d32u; x[3] = 0x6b206574u;
@@ -61,11 +62,15 @@ __device__ __forceinline__ void chacha_start(const ChaChaKey& key, uint64_t nonc
   x[14] = (uint32_t)nonce; x[15] = (uint32_t)(nonce >> 32);
 }
 
-__device__ __forceinline__ void chacha_double_round(uint32_t x[16]) {
+// A double round is a column round, then a diagonal round.
+__device__ __forceinline__ void chacha_column_round(uint32_t x[16]) {
   IRIS_QR(x[0], x[4], x[8], x[12]);
   IRIS_QR(x[1], x[5], x[9], x[13]);
   IRIS_QR(x[2], x[6], x[10], x[14]);
   IRIS_QR(x[3], x[7], x[11], x[15]);
+}
+
+__device__ __forceinline__ void chacha_diagonal_round(uint32_t x[16]) {
   IRIS_QR(x[0], x[5], x[10], x[15]);
   IRIS_QR(x[1], x[6], x[11], x[12]);
   IRIS_QR(x[2], x[7], x[8], x[13]);

@@ -17,6 +17,7 @@
 #include "circuits.hpp"
 
 #include <type_traits>
+
 #include "chacha.hpp"
 
 namespace irismpc {
@@ -308,23 +309,15 @@ struct NextDraw {
 template <int J, bool kBeforeBarrier>
 __device__ __forceinline__ void draw_piece(NextDraw& d, const ChaChaKey& key, uint64_t nonce,
                                            uint32_t* rnd_row) {
-  uint32_t* x = d.x;
   if constexpr (J < 6) {
-#define IRISMPC_QR(a, b, c, e)                            \
-  x[a] += x[b]; x[e] = rotl32(x[e] ^ x[a], 16);           \
-  x[c] += x[e]; x[b] = rotl32(x[b] ^ x[c], 12);           \
-  x[a] += x[b]; x[e] = rotl32(x[e] ^ x[a], 8);            \
-  x[c] += x[e]; x[b] = rotl32(x[b] ^ x[c], 7);
-    if constexpr (kBeforeBarrier) {
-      IRISMPC_QR(0, 4, 8, 12); IRISMPC_QR(1, 5, 9, 13); IRISMPC_QR(2, 6, 10, 14); IRISMPC_QR(3, 7, 11, 15);
-    } else {
-      IRISMPC_QR(0, 5, 10, 15); IRISMPC_QR(1, 6, 11, 12); IRISMPC_QR(2, 7, 8, 13); IRISMPC_QR(3, 4, 9, 14);
-    }
-#undef IRISMPC_QR
+    if constexpr (kBeforeBarrier)
+      chacha_column_round(d.x);
+    else
+      chacha_diagonal_round(d.x);
   } else if constexpr (J == 6 && kBeforeBarrier) {
-    chacha_finish(key, nonce, d.ctr, x);
+    chacha_finish(key, nonce, d.ctr, d.x);
 #pragma unroll
-    for (int i = 0; i < 16; i++) rnd_row[i] = d.valid ? x[i] : 0u;
+    for (int i = 0; i < 16; i++) rnd_row[i] = d.valid ? d.x[i] : 0u;
   }
 }
 
