@@ -213,10 +213,10 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
 
 // Transpose a 32 x 32 bit matrix in place: afterwards bit i of x[j] is bit j
 // of the old x[i] (recursive block swap, 5 stages).
-template <int S, uint32_t M>
-__device__ __forceinline__ void bit_transpose_stage(uint32_t x[32]) {
+template <int S, uint32_t M, int N = 32>
+__device__ __forceinline__ void bit_transpose_stage(uint32_t x[]) {
 #pragma unroll
-  for (int k = 0; k < 32; k++)
+  for (int k = 0; k < N; k++)
     if ((k & S) == 0) {
       const uint32_t t = ((x[k] >> S) ^ x[k + S]) & M;
       x[k + S] ^= t;
@@ -230,6 +230,19 @@ __device__ __forceinline__ void bit_transpose32(uint32_t x[32]) {
   bit_transpose_stage<4, 0x0F0F0F0Fu>(x);
   bit_transpose_stage<2, 0x33333333u>(x);
   bit_transpose_stage<1, 0x55555555u>(x);
+}
+
+// The same for 16-bit values (x[i] < 2^16): afterwards bit i of x[j], j < 16,
+// is bit j of the old x[i] (x[16 ..] are left undefined). Values i and i + 16
+// share a register from the start, so the first stage of bit_transpose32 is
+// the packing itself and the other four stages run on 16 registers.
+__device__ __forceinline__ void bit_transpose16(uint32_t x[32]) {
+#pragma unroll
+  for (int k = 0; k < 16; k++) x[k] |= x[k + 16] << 16;
+  bit_transpose_stage<8, 0x00FF00FFu, 16>(x);
+  bit_transpose_stage<4, 0x0F0F0F0Fu, 16>(x);
+  bit_transpose_stage<2, 0x33333333u, 16>(x);
+  bit_transpose_stage<1, 0x55555555u, 16>(x);
 }
 
 // A lane's plane slices: lift: pl[j], j < 16 the a share and j >= 16 the b
@@ -549,7 +562,10 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
       for (int i = 0; i < 8; i++) sl.pl[8 * q + i] = (uint32_t)ma[i] | ((uint32_t)mb[i] << 16);
     }
   }
-  bit_transpose32(sl.pl);
+  if constexpr (kPrevB && !MSB)
+    bit_transpose16(sl.pl);  // the mask a shares alone: 16-bit values
+  else
+    bit_transpose32(sl.pl);
   if constexpr (kPrevB) {
 #pragma unroll
     for (int j = 0; j < NPL; j++) a_planes[(p * NPL + j) * 64 + lane] = sl.pl[j];
@@ -596,6 +612,15 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
 // adder_word_kernel).
 __device__ __forceinline__ uint32_t lo16(uint32_t w, int h) { return (w >> (16 * h)) & 0xffffu; }
 
+// Two u16 elements of a word, for packed 16-bit arithmetic.
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t w) { return __builtin_bit_cast(u16x2, w); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+// bits 2 w and 2 w + 1 of `bits` as the u16 pair (b0, b1)
+__device__ __forceinline__ uint32_t pair_bits(uint32_t bits, int w) {
+  return ((bits >> (2 * w)) & 1u) | (((bits >> (2 * w + 1)) & 1u) << 16);
+}
+
 __global__ __launch_bounds__(192) void compare_fused_kernel(const Trio t, uint32_t A, const RippleCtrs lift_rc,
                                                             uint64_t inject_ctr, const RippleCtrs msb_rc) {
   __shared__ AdderLds L;
@@ -625,7 +650,7 @@ __global__ __launch_bounds__(192) void compare_fused_kernel(const Trio t, uint32
 #pragma unroll
     for (int i = 0; i < 4; i++) mk[4 * q + i] = (uint32_t)ma[2 * i] | ((uint32_t)ma[2 * i + 1] << 16);
   }
-  bit_transpose32(sl.pl);
+  bit_transpose16(sl.pl);
 #pragma unroll
   for (int j = 0; j < 16; j++) xs[(p * 16 + j) * 64 + lane] = sl.pl[j];
   sl.prev = &xs[prev * 16 * 64 + lane];
@@ -647,6 +672,8 @@ __global__ __launch_bounds__(192) void compare_fused_kernel(const Trio t, uint32
   __syncthreads();
   uint32_t* msg = &L.rnd[0][0][0][0];  // [bit][word][lane], 2,048 of its 3,264 words
   uint32_t cr[2][16];  // corrections (a share) of bits 16 and 17, two elements per word
+  // two elements per word throughout: packed u16 arithmetic; bit b of an
+  // element as a u16 (bv) and as an all-ones or zero u16 (bm = b * 0xffff)
   if (p != 2) {
 #pragma unroll
     for (int c = 0; c < 2; c++) {
@@ -654,27 +681,17 @@ __global__ __launch_bounds__(192) void compare_fused_kernel(const Trio t, uint32
       const uint32_t bits = p == 0 ? bits_b[c] : bits_a[c] ^ bits_b[c];
 #pragma unroll
       for (int w = 0; w < 16; w++) {
-        const uint32_t own = xs[((p * 2 + c) * 16 + w) * 64 + lane];
-        const uint32_t prv = xs[((prev * 2 + c) * 16 + w) * 64 + lane];
-        uint32_t out = 0, m = 0;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-          const uint32_t b = (bits >> (2 * w + h)) & 1u;
-          uint32_t o;
-          if (p == 0) {  // rand_01 = my rng, rand_02 = their rng
-            const uint32_t r01 = lo16(own, h), r02 = lo16(prv, h);
-            const uint32_t y = b * r01 - r02;
-            o = r01 - (y << 1);
-          } else {  // rand_01 = their rng, rand_12 = my rng
-            const uint32_t r01 = lo16(prv, h), r12 = lo16(own, h);
-            const uint32_t x = b - r01;
-            m |= (x & 0xffffu) << (16 * h);
-            o = x - (r12 << 1);
-          }
-          out |= (o & 0xffffu) << (16 * h);
+        const u16x2 own = as_u16x2(xs[((p * 2 + c) * 16 + w) * 64 + lane]);
+        const u16x2 prv = as_u16x2(xs[((prev * 2 + c) * 16 + w) * 64 + lane]);
+        const uint32_t bv = pair_bits(bits, w), bm = bv * 0xffffu;
+        if (p == 0) {  // rand_01 = my rng, rand_02 = their rng: y = b r01 - r02, o = r01 - 2 y
+          const u16x2 y = (as_u16x2(bm) & own) - prv;
+          cr[c][w] = as_u32(own - (y << 1));
+        } else {  // rand_01 = their rng, rand_12 = my rng: x = b - r01 (to P2), o = x - 2 r12
+          const u16x2 x = as_u16x2(bv) - prv;
+          cr[c][w] = as_u32(x - (own << 1));
+          msg[(c * 16 + w) * 64 + lane] = as_u32(x);
         }
-        cr[c][w] = out;
-        if (p == 1) msg[(c * 16 + w) * 64 + lane] = m;
       }
     }
   }
@@ -685,20 +702,13 @@ __global__ __launch_bounds__(192) void compare_fused_kernel(const Trio t, uint32
       const uint32_t bits = bits_a[c];
 #pragma unroll
       for (int w = 0; w < 16; w++) {
-        const uint32_t own = xs[((p * 2 + c) * 16 + w) * 64 + lane];
-        const uint32_t prv = xs[((prev * 2 + c) * 16 + w) * 64 + lane];
-        const uint32_t in = msg[(c * 16 + w) * 64 + lane];
-        uint32_t out = 0;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-          // rand_12 = their rng, rand_02 = my rng, x = P1's message
-          const uint32_t b = (bits >> (2 * w + h)) & 1u;
-          const uint32_t r12 = lo16(prv, h), r02 = lo16(own, h), x = lo16(in, h);
-          const uint32_t z = b * x - r12;
-          const uint32_t o = b - ((r02 + z) << 1);
-          out |= (o & 0xffffu) << (16 * h);
-        }
-        cr[c][w] = out;
+        // rand_12 = their rng, rand_02 = my rng, x = P1's message: z = b x - r12, o = b - 2 (r02 + z)
+        const u16x2 own = as_u16x2(xs[((p * 2 + c) * 16 + w) * 64 + lane]);
+        const u16x2 prv = as_u16x2(xs[((prev * 2 + c) * 16 + w) * 64 + lane]);
+        const u16x2 x = as_u16x2(msg[(c * 16 + w) * 64 + lane]);
+        const uint32_t bv = pair_bits(bits, w), bm = bv * 0xffffu;
+        const u16x2 z = (as_u16x2(bm) & x) - prv;
+        cr[c][w] = as_u32(as_u16x2(bv) - ((own + z) << 1));
       }
     }
   }
