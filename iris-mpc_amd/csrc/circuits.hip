@@ -16,8 +16,6 @@
 //    the reference would have used (no keystream buffers).
 #include "circuits.hpp"
 
-#include <type_traits>
-
 #include "chacha.hpp"
 
 namespace irismpc {
@@ -464,16 +462,13 @@ __device__ __forceinline__ void adder_first_draw(AdderLds& L, const RippleCtrs& 
   for (int i = 0; i < 16; i++) L.rnd[p][grp][gl][i] = k[i];
 }
 
-// kPrevB: every party's b share is the previous party's a share, element for
-// element (the caller passes the previous party's a buffers as b), so a wave
-// forms its own a planes only and takes the b planes from the previous
-// party's wave (PlanesPrevB): half the loads and transposes, and the register
-// file of the b planes freed.
-template <bool MSB, bool kPrevB>
+// The three-launch form's adders (lift: MSB = false; lift_mul_sub and MSB
+// adder: MSB = true), for b views that are separate buffers: each wave forms
+// the planes of both its shares. (When every b view is the previous party's a
+// buffer, compare_fused_kernel runs all stages at once.)
+template <bool MSB>
 __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t A, const RippleCtrs rc) {
-  constexpr int NPL = MSB ? 32 : 16;     // planes per share
   __shared__ AdderLds L;
-  __shared__ uint32_t a_planes[kPrevB ? 3 * NPL * 64 : 1];  // [party][plane][lane]
   // one party per wave: p is wave-uniform, so its keys and pointers stay scalar
   const int p = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63), prev = (p + 2) % 3;
@@ -488,37 +483,11 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
   NextDraw d;
   adder_first_draw<MSB>(L, rc, G, g, p, lane, d, P.key_own, t.nonce);
 
-  std::conditional_t<kPrevB, PlanesPrevB, PlaneSlices<MSB>> sl;
+  PlaneSlices<MSB> sl;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const size_t e = 32 * hw + 8 * q;
-    if constexpr (kPrevB && !MSB) {
-      uint16_t ma[8] = {};
-      if (live) u16x8_load(P.mask_a + e, ma);
-#pragma unroll
-      for (int i = 0; i < 8; i++) sl.pl[8 * q + i] = ma[i];
-    } else if constexpr (kPrevB) {
-      // the a side of the general form below; the b side is the previous wave's
-      uint16_t ma[8] = {}, c1a[8] = {}, c2a[8] = {}, xa[8] = {};
-      if (live) {
-        u16x8_load(P.mask_a + e, ma);
-        u16x8_load(P.corr_a + e, c1a);
-        u16x8_load(P.corr_a + n_el + e, c2a);
-        u16x8_load(P.code_a + e, xa);
-      }
-      uint32_t la[8];
-#pragma unroll
-      for (int i = 0; i < 8; i++)
-        la[i] = (uint32_t)ma[i] - ((uint32_t)c1a[i] << 16) - ((uint32_t)c2a[i] << 17);
-      if (P.lifted_a && live) {  // own a, and the next party's b
-        uint4* da = reinterpret_cast<uint4*>(P.lifted_a + e);
-        uint4* db = reinterpret_cast<uint4*>(t.p[(p + 1) % 3].lifted_b + e);
-        da[0] = db[0] = uint4{la[0], la[1], la[2], la[3]};
-        da[1] = db[1] = uint4{la[4], la[5], la[6], la[7]};
-      }
-#pragma unroll
-      for (int i = 0; i < 8; i++) sl.pl[8 * q + i] = la[i] * A - ((uint32_t)xa[i] << 16) - (p == 0 ? 1u : 0u);
-    } else if constexpr (MSB) {
+    if constexpr (MSB) {
       // m = (mask - c1<<16 - c2<<17) * A - code<<16, minus 1 on x_0
       // (kernel.cu:181-195,407-426)
       uint16_t ma[8] = {}, mb[8] = {}, c1a[8] = {}, c2a[8] = {}, c1b[8] = {}, c2b[8] = {},
@@ -562,17 +531,8 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
       for (int i = 0; i < 8; i++) sl.pl[8 * q + i] = (uint32_t)ma[i] | ((uint32_t)mb[i] << 16);
     }
   }
-  if constexpr (kPrevB && !MSB)
-    bit_transpose16(sl.pl);  // the mask a shares alone: 16-bit values
-  else
-    bit_transpose32(sl.pl);
-  if constexpr (kPrevB) {
-#pragma unroll
-    for (int j = 0; j < NPL; j++) a_planes[(p * NPL + j) * 64 + lane] = sl.pl[j];
-    sl.prev = &a_planes[prev * NPL * 64 + lane];
-  } else if constexpr (MSB) {
-    bit_transpose32(sl.pl + 32);
-  }
+  bit_transpose32(sl.pl);
+  if constexpr (MSB) bit_transpose32(sl.pl + 32);
   __syncthreads();
 
   AdderState st{};
@@ -789,16 +749,10 @@ void launch_adder_word(const Trio& t, bool msb, uint32_t threshold_a, const Ripp
                        hipStream_t s) {
   IRIS_REQUIRE(t.W % 8 == 0, "bit planes must be a multiple of 8 words");
   const unsigned blocks = (unsigned)ceil_div(t.W, 32);
-  // b shares that are the previous party's a buffers: planes from the previous wave
-  const bool prev_b = trio_b_is_prev(t);
-  if (msb && prev_b)
-    adder_word_kernel<true, true><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
-  else if (msb)
-    adder_word_kernel<true, false><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
-  else if (prev_b)
-    adder_word_kernel<false, true><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
+  if (msb)
+    adder_word_kernel<true><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
   else
-    adder_word_kernel<false, false><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
+    adder_word_kernel<false><<<blocks, 192, 0, s>>>(t, threshold_a, ctrs);
   IRIS_HIP_CHECK(hipGetLastError());
 }
 
