@@ -833,17 +833,21 @@ __device__ __forceinline__ void rot_tile(const GemmArgs& a, int q0, size_t db0, 
   }
   const uint8_t* lds = reinterpret_cast<const uint8_t*>(sm.planes);
 
+  constexpr int NBL = TD * 4, NAR = TQ * 2 * 2;  // DB loads, ds_read2 per slice
+  constexpr bool kEarly = NBL <= HALF - G && NAR == 2 * G;
+  constexpr bool kThread = NBL <= HALF - G && NAR <= 2 * (HALF - G);
+  // In the early-read schedule limb 0 of every query tile comes first: the
+  // first two passes of a slice use only limb 0 of A, the third limb 1.
   auto read_a = [&](v4i (&A)[TQ][2], int kt, int sl) {
     const int k0 = kt * kKTile + 32 * sl + 16 * h;
     const int off = k0 + (kRotRow - 800) * (k0 / 800);
 #pragma unroll
-    for (int i = 0; i < TQ; i++)
-#pragma unroll
-      for (int l = 0; l < 2; l++) {
-        const uint32_t* p =
-            reinterpret_cast<const uint32_t*>(__builtin_assume_aligned(lds + abase[i] + l * plane + off, 4));
-        A[i][l] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
-      }
+    for (int x = 0; x < 2 * TQ; x++) {
+      const int i = kEarly ? x % TQ : x / 2, l = kEarly ? x / TQ : x % 2;
+      const uint32_t* p =
+          reinterpret_cast<const uint32_t*>(__builtin_assume_aligned(lds + abase[i] + l * plane + off, 4));
+      A[i][l] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
+    }
   };
   auto load_b = [&](v4i (&B)[TD][2][2], int kt) {
     const size_t ko = (size_t)kt * 2 * kFragBytes;
@@ -895,44 +899,74 @@ __device__ __forceinline__ void rot_tile(const GemmArgs& a, int q0, size_t db0, 
   load_b(B1, last < 1 ? last : 1);
   read_a(A0, 0, 0);
 
-  // One K-step on B(kt): each k-slice opens with one pass of MFMAs (whose
-  // first use of the operands is where the compiler waits for them), then
-  // issues the next slice's A reads (and in slice 0 the DB loads of step
-  // kt + 2), then runs the other two passes while those land.
-  // With one wave per SIMD (TQ = 4) the loads are threaded one or two at a
-  // time between the MFMAs of the other two passes (a batch of them would
-  // leave this SIMD's only wave without an MFMA to issue). The small query
-  // tiles have too few MFMAs per pass for that; their SIMDs hold WPE waves.
-  constexpr int NBL = TD * 4, NAR = TQ * 2 * 2;  // DB loads, ds_read2 per slice
-  constexpr bool kThread = NBL <= HALF - G && NAR <= 2 * (HALF - G);
-  static_assert(kThread || WPE > 1, "one wave per SIMD needs the threaded schedule");
+  // One K-step on B(kt), two k-slices of three passes (limb products) each.
+  // With one wave per SIMD (TQ = 4) the next slice's A reads go out threaded
+  // two at a time between the MFMAs of a slice's first pass (their registers
+  // were last read by the previous slice), so they have two passes to land;
+  // slice 0's other two passes carry the DB loads of step kt + 2, one per
+  // MFMA (a batch of loads would leave this SIMD's only wave without an MFMA
+  // to issue). Measured against issuing the A reads after the first pass:
+  // 2 % less time per launch.
+  // The two-query tile (two waves per SIMD) has too few MFMAs in a first pass
+  // for its reads: each of its slices opens with the first pass (where the
+  // compiler waits for the operands), then issues the next loads threaded
+  // between the MFMAs of the other two passes.
+  static_assert(kEarly || WPE > 1, "one wave per SIMD needs the early-read schedule");
+  auto thread_reads = [&]() {
+#pragma unroll
+    for (int r = 0; r < G; r++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // DS read
+    }
+  };
   auto step = [&](int kt, const v4i (&Bc)[TD][2][2], v4i (&Bp)[TD][2][2]) {
-    mfma(A0, Bc, 0, C0{}, CG{});
-    __builtin_amdgcn_sched_barrier(0);
-    load_b(Bp, kt + 2 < last ? kt + 2 : last);
-    read_a(A1, kt, 1);
-    mfma(A0, Bc, 0, CG{}, CH{});
-    if constexpr (kThread) {
+    if constexpr (kEarly) {
+      read_a(A1, kt, 1);
+      mfma(A0, Bc, 0, C0{}, CG{});
+      thread_reads();
+      __builtin_amdgcn_sched_barrier(0);
+      load_b(Bp, kt + 2 < last ? kt + 2 : last);
+      mfma(A0, Bc, 0, CG{}, CH{});
 #pragma unroll
       for (int r = 0; r < HALF - G; r++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
-        if (r < NBL) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);     // VMEM read
-        if (r < NAR / 2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // DS read
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);               // MFMA
+        if (r < NBL) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
       }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    mfma(A1, Bc, 1, C0{}, CG{});
-    __builtin_amdgcn_sched_barrier(0);
-    read_a(A0, kt + 1 < last ? kt + 1 : last, 0);
-    mfma(A1, Bc, 1, CG{}, CH{});
-    if constexpr (kThread) {
+      __builtin_amdgcn_sched_barrier(0);
+      read_a(A0, kt + 1 < last ? kt + 1 : last, 0);
+      mfma(A1, Bc, 1, C0{}, CG{});
+      thread_reads();
+      __builtin_amdgcn_sched_barrier(0);
+      mfma(A1, Bc, 1, CG{}, CH{});
+      __builtin_amdgcn_sched_barrier(0);
+    } else {
+      mfma(A0, Bc, 0, C0{}, CG{});
+      __builtin_amdgcn_sched_barrier(0);
+      load_b(Bp, kt + 2 < last ? kt + 2 : last);
+      read_a(A1, kt, 1);
+      mfma(A0, Bc, 0, CG{}, CH{});
+      if constexpr (kThread) {
 #pragma unroll
-      for (int r = 0; r < HALF - G; r++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
-        if (r < NAR / 2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // DS read
+        for (int r = 0; r < HALF - G; r++) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
+          if (r < NBL) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);     // VMEM read
+          if (r < NAR / 2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // DS read
+        }
       }
+      __builtin_amdgcn_sched_barrier(0);
+      mfma(A1, Bc, 1, C0{}, CG{});
+      __builtin_amdgcn_sched_barrier(0);
+      read_a(A0, kt + 1 < last ? kt + 1 : last, 0);
+      mfma(A1, Bc, 1, CG{}, CH{});
+      if constexpr (kThread) {
+#pragma unroll
+        for (int r = 0; r < HALF - G; r++) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
+          if (r < NAR / 2) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0); // DS read
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
-    __builtin_amdgcn_sched_barrier(0);
   };
   int kt = 0;
   for (; kt + 2 < n_kt; kt += 3) {
