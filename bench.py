@@ -54,6 +54,9 @@ INT8_OPS_PER_PAIR = 115200  # 2 x 3 limb products x (12,800 + 6,400) per pair pe
 # for raw-share queries is 102 up to two queries (62 rows), else 100
 KERNEL_OF_VARIANT = {100: "share_gemm_rot_kernel<4, 2, 4, 1>", 101: "share_gemm_rot_kernel<4, 2, 2, 1>",
                      102: "share_gemm_rot_kernel<2, 1, 4, 2>"}
+# the scan's six share GEMMs of a chunk (3 parties x code, mask) as one launch
+# of the 4-query kernel (when the scan resolves to variant 100)
+JOINT_KERNEL = "share_gemm_rot_heads_kernel<4, 2, 4>"
 ROT_SMALL_ROWS = 62
 # int8 ops per DB byte at which the share GEMM turns from HBM- to MFMA-bound:
 # 5.0 POPS / 8 TB/s; a launch does 3 Qr ops per DB byte
@@ -146,18 +149,19 @@ def live_traffic(kernel, child_args):
 
 
 def _gemm_algorithmic_bytes(Qr):
-    """HBM bytes one share GEMM launch of the scan must move: its chunk's DB
-    limbs read once (the average of the code and mask launches) and the u16
-    results written once."""
+    """HBM bytes one share GEMM of the scan must move: its chunk's DB limbs
+    read once (the average of a code and a mask GEMM) and the u16 results
+    written once."""
     db = 32768 * (12800 + 6400) * 2 / 2
     return db + Qr * 32768 * 2
 
 
 def run_traffic_probe(args):
     """The child that the traffic passes profile: one batch of the 3-party scan
-    of one eye over a DB of PROBE_CHUNKS chunks (every share GEMM launch is one
-    party's code or mask GEMM over one 32,768-row chunk, the launch shape of
-    the headline workload), then one 2 GiB device-to-device copy."""
+    of one eye over a DB of PROBE_CHUNKS chunks (every share GEMM launch is
+    the launch of the headline workload: one 32,768-row chunk's six GEMMs, or
+    with a forced variant one party's code or mask GEMM), then one 2 GiB
+    device-to-device copy."""
     import torch
 
     import irismpc
@@ -423,7 +427,10 @@ def run_c3(args):
 
     latency = args.latency_rate > 0
     Qr = args.queries * 31
-    kernel = KERNEL_OF_VARIANT[args.variant if args.variant >= 100 else 100 if Qr > ROT_SMALL_ROWS else 102]
+    variant = args.variant if args.variant >= 100 else 100 if Qr > ROT_SMALL_ROWS else 102
+    joint = variant == 100  # one launch per chunk for the six GEMMs
+    kernel = JOINT_KERNEL if joint else KERNEL_OF_VARIANT[variant]
+    gemms_per_launch = 6 if joint else 1
     traffic, traffic_detail = None, "not measured (--no-live-traffic)"
     if args.live_traffic:
         # before this process takes the GPU: the passes profile a child process
@@ -516,13 +523,14 @@ def run_c3(args):
                      "traffic": traffic,
                      "traffic_source": traffic_source,
                      "traffic_passes": traffic_detail if isinstance(traffic_detail, dict) else None,
-                     "algorithmic_bytes_per_launch": _gemm_algorithmic_bytes(Qr),
+                     "algorithmic_bytes_per_launch": gemms_per_launch * _gemm_algorithmic_bytes(Qr),
                      "kernel": kernel,
                      "launches": gemm_n, "avg_launch_ms": gemm_ms / max(gemm_n, 1),
                      "int8_ops_per_launch_avg": gemm_ops / max(gemm_n, 1),
-                     "basis": "share GEMM launches inside the timed batches (6 per chunk: 3 parties x code, "
-                              "mask, in order on one stream; HIP events on that stream): 115,200 int8 ops per "
-                              "pair per party / launch duration"},
+                     "basis": ("share GEMM launches inside the timed batches, HIP events on their stream: "
+                               + ("one per chunk, the chunk's six GEMMs (3 parties x code, mask) as one grid"
+                                  if joint else "6 per chunk (3 parties x code, mask), in order on one stream")
+                               + "; 115,200 int8 ops per pair per party / launch duration")},
         "streaming_stage": {"bound": "valu", "kernels": "compare_fused_kernel (lift, injection, MSB adder)",
                             "compares": circ_n, "avg_compare_ms": circ_ms / max(circ_n, 1),
                             "chacha12_ops": chacha_ops,

@@ -318,8 +318,8 @@ void scan_batch_self_trio(const PartyScan p[3], size_t batch_size, uint32_t thre
 }
 
 // The three parties' chunk loop from one host thread. Per chunk the six share
-// GEMMs (three parties x code, mask) run in order on s[0] into result slot
-// c % 2, so no two GEMMs share the GPU and each launch's duration is its own;
+// GEMMs (three parties x code, mask) run as one launch on s[0] into result
+// slot c % 2 (one kernel boundary per chunk instead of six);
 // the reshares hand over views, and the circuits and opens of chunk c run on
 // the exchange's protocol stream while chunk c + 1's GEMMs run (a slot is
 // reused once the open of the chunk that last read it has run). Shares,
@@ -364,12 +364,21 @@ static void scan_db_trio_ordered(const PartyScan p[3], size_t db_size, size_t ba
     const size_t real = db_size > offset ? std::min<size_t>(kDbChunk, db_size - offset) : 0;
     const size_t dot_chunk = round_up(std::max<size_t>(real, 1), pad);
     if (used[slot]) IRIS_HIP_CHECK(hipStreamWaitEvent(g, freed[slot].e, 0));
+    ShareDB* eng[6];
+    const QuerySet* qs[6];
+    const DbSlice* dbs[6];
+    const uint16_t mult[6] = {1, 2, 1, 2, 1, 2};
     for (int i = 0; i < 3; i++) {
       cd[i]->set_slot(slot);
       md[i]->set_slot(slot);
-      cd[i]->dot_reduce(*p[i].code_query, *p[i].code_db, dot_chunk, offset, 1, g);
-      md[i]->dot_reduce(*p[i].mask_query, *p[i].mask_db, dot_chunk, offset, 2, g);
+      eng[2 * i] = cd[i];
+      qs[2 * i] = p[i].code_query;
+      dbs[2 * i] = p[i].code_db;
+      eng[2 * i + 1] = md[i];
+      qs[2 * i + 1] = p[i].mask_query;
+      dbs[2 * i + 1] = p[i].mask_db;
     }
+    ShareDB::dot_reduce_joint(eng, qs, dbs, mult, 6, dot_chunk, offset, g);
     ShareDB::reshare_trio(cd, dot_chunk, gs);
     ShareDB::reshare_trio(md, dot_chunk, gs);
     const size_t len = dot_chunk * rows;

@@ -112,9 +112,19 @@ void Exchange::collective(int party, uint64_t op, void* arg,
   if (error_gen_ == gen && error_) std::rethrow_exception(error_);
 }
 
+// True for the parties whose stream is not the protocol stream and not an
+// earlier party's: one event wait per distinct stream (a wait on a stream's
+// own work, or a second wait on the same work, would only cost a marker).
+static bool own_party_stream(hipStream_t const s[3], int p, hipStream_t protocol) {
+  if (s[p] == protocol) return false;
+  for (int q = 0; q < p; q++)
+    if (s[q] == s[p]) return false;
+  return true;
+}
+
 void Exchange::join_streams(hipStream_t const s[3]) {
   for (int p = 0; p < 3; p++) {
-    if (s[p] == protocol_) continue;
+    if (!own_party_stream(s, p, protocol_)) continue;
     IRIS_HIP_CHECK(hipEventRecord(ev_party_[p].e, s[p]));
     IRIS_HIP_CHECK(hipStreamWaitEvent(protocol_, ev_party_[p].e, 0));
   }
@@ -123,7 +133,7 @@ void Exchange::join_streams(hipStream_t const s[3]) {
 void Exchange::fork_streams(hipStream_t const s[3]) {
   IRIS_HIP_CHECK(hipEventRecord(ev_proto_.e, protocol_));
   for (int p = 0; p < 3; p++)
-    if (s[p] != protocol_) IRIS_HIP_CHECK(hipStreamWaitEvent(s[p], ev_proto_.e, 0));
+    if (own_party_stream(s, p, protocol_)) IRIS_HIP_CHECK(hipStreamWaitEvent(s[p], ev_proto_.e, 0));
 }
 
 // ====================================================================== ShareDB
@@ -453,9 +463,8 @@ static GemmArgs gemm_args(const QuerySet& q, const DbSlice& db, int variant) {
   return a;
 }
 
-void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
-                         uint16_t multiplier, hipStream_t s, size_t out_offset) {
-  DeviceGuard g(device_);
+GemmArgs ShareDB::dot_reduce_args(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
+                                  uint16_t multiplier, size_t out_offset, int* variant_out) {
   IRIS_REQUIRE(q.code_length == code_length_ && db.code_length == code_length_, "code length mismatch");
   IRIS_REQUIRE(q.rows >= 1 && (size_t)q.rows <= query_length_, "query rows exceed query_length");
   IRIS_REQUIRE(chunk > 0 && chunk % 64 == 0, "chunk size must be a positive multiple of 64");
@@ -479,10 +488,43 @@ void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, siz
   a.key_prev = seed_prev_;
   a.mult = multiplier;
   a.masked = is_remote() ? 1 : 0;
-  launch(a, variant, s);
   // the reference fills the mask keystream (chunk*Qr*2 bytes) only when remote
   last_rows_ = (size_t)q.rows;
   if (is_remote()) counter_ += ceil_div(chunk * last_rows_ * 2, 64);
+  *variant_out = variant;
+  return a;
+}
+
+void ShareDB::dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
+                         uint16_t multiplier, hipStream_t s, size_t out_offset) {
+  DeviceGuard g(device_);
+  int variant;
+  const GemmArgs a = dot_reduce_args(q, db, chunk, offset, multiplier, out_offset, &variant);
+  launch(a, variant, s);
+}
+
+void ShareDB::dot_reduce_joint(ShareDB* const eng[], const QuerySet* const q[], const DbSlice* const db[],
+                               const uint16_t mult[], int n, size_t chunk, size_t offset, hipStream_t s) {
+  IRIS_REQUIRE(n >= 1 && n <= kMaxHeads, "1 .. 6 engines per joint dot_reduce");
+  DeviceGuard g(eng[0]->device_);
+  GemmArgs args[kMaxHeads];
+  int variant[kMaxHeads];
+  bool joint = true;
+  for (int i = 0; i < n; i++) {
+    IRIS_REQUIRE(eng[i]->device_ == eng[0]->device_, "joint dot_reduce engines share one device");
+    args[i] = eng[i]->dot_reduce_args(*q[i], *db[i], chunk, offset, mult[i], 0, &variant[i]);
+    joint = joint && variant[i] == kRotVariantBase;
+  }
+  if (!joint) {
+    for (int i = 0; i < n; i++) eng[i]->launch(args[i], variant[i], s);
+    return;
+  }
+  GemmHead heads[kMaxHeads];
+  for (int i = 0; i < n; i++) heads[i] = args[i];
+  LaunchTimer& timer = eng[0]->timer;
+  if (timer.on()) timer.begin(s);
+  launch_share_gemm_heads(heads, n, s);
+  if (timer.on()) timer.end(s);
 }
 
 void ShareDB::dot_reduce_chunks(const QuerySet& q, const DbSlice& db, size_t offset, size_t total,
@@ -624,11 +666,13 @@ struct ShareDBTrio {
     }
     const bool copy = d[0]->peer_copy || d[1]->peer_copy || d[2]->peer_copy;
     Event sent[3];
-    for (int p = 0; p < 3; p++) IRIS_HIP_CHECK(hipEventRecord(sent[p].e, s[p]));
+    for (int p = 0; p < 3; p++)
+      if (s[(p + 1) % 3] != s[p]) IRIS_HIP_CHECK(hipEventRecord(sent[p].e, s[p]));
     for (int p = 0; p < 3; p++) {
       const int q = (p + 1) % 3;
-      // the receiver reads after the sender's GEMM finished
-      IRIS_HIP_CHECK(hipStreamWaitEvent(s[q], sent[p].e, 0));
+      // the receiver reads after the sender's GEMM finished (stream order
+      // when both run on one stream: a wait there would only cost a marker)
+      if (s[q] != s[p]) IRIS_HIP_CHECK(hipStreamWaitEvent(s[q], sent[p].e, 0));
       if (copy) {
         IRIS_HIP_CHECK(hipMemcpyAsync(d[q]->results_peer_.ptr, d[p]->results(), bytes,
                                       hipMemcpyDeviceToDevice, s[q]));

@@ -216,6 +216,13 @@ class ShareDB {
   // small query set can sit side by side in the result buffer
   void dot_reduce(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
                   uint16_t multiplier, hipStream_t s, size_t out_offset = 0);
+  // dot_reduce for n engines of one device at once (engine i: query q[i], DB
+  // db[i], multiplier mult[i], the same chunk and offset), results and masks
+  // those of the n calls in order. When every call takes the rotation-aware
+  // 4-query kernel they run as one launch (one kernel boundary instead of n);
+  // otherwise as n launches. Timed on eng[0]'s timer when it is enabled.
+  static void dot_reduce_joint(ShareDB* const eng[], const QuerySet* const q[], const DbSlice* const db[],
+                               const uint16_t mult[], int n, size_t chunk, size_t offset, hipStream_t s);
   // Consecutive dot_reduce calls in one launch: DB rows [offset, offset +
   // total) as chunks of `chunk` rows (the last one total mod chunk wide when
   // that is not zero; a multiple of 64), results side by side from results() +
@@ -277,6 +284,9 @@ class ShareDB {
 
  private:
   void launch(const GemmArgs& a, int variant, hipStream_t s);
+  // dot_reduce's checks, arguments and counter bookkeeping, without the launch
+  GemmArgs dot_reduce_args(const QuerySet& q, const DbSlice& db, size_t chunk, size_t offset,
+                           uint16_t multiplier, size_t out_offset, int* variant);
   std::unique_ptr<QuerySet> upload_shares_impl(const uint16_t* host_shares,
                                                const uint16_t* insert_shares, size_t batch,
                                                size_t rows, int mirror, hipStream_t s) const;

@@ -389,8 +389,8 @@ __device__ __forceinline__ uint16_t corrected(const GemmArgs& a, int q, size_t d
   return (uint16_t)(acc0 + (acc1 << 8) + (s0 << 7) + ((s0 + s1) << 15));
 }
 
-template <int BQ, int BD>
-__device__ __forceinline__ void epilogue_store(const GemmArgs& a, const uint16_t* tile, int q0,
+template <int BQ, int BD, class Args>
+__device__ __forceinline__ void epilogue_store(const Args& a, const uint16_t* tile, int q0,
                                                size_t db0, int q_end, int tid, int nthreads);
 
 template <int WQ, int WD, int NWQ = 2, int NWD = 2>
@@ -417,8 +417,8 @@ __device__ __forceinline__ void gemm_epilogue(const GemmArgs& a, uint16_t* tile,
 
 // Multiplier, ChaCha12 zero-sharing and 64-byte stores of the u16 tile; query
 // rows at or beyond q_end are not stored.
-template <int BQ, int BD>
-__device__ __forceinline__ void epilogue_store(const GemmArgs& a, const uint16_t* tile, int q0,
+template <int BQ, int BD, class Args>
+__device__ __forceinline__ void epilogue_store(const Args& a, const uint16_t* tile, int q0,
                                                size_t db0, int q_end, int tid, int nthreads) {
   __syncthreads();
   constexpr int SEGS = BD / 32;
@@ -436,18 +436,20 @@ __device__ __forceinline__ void epilogue_store(const GemmArgs& a, const uint16_t
     size_t stride = a.out_chunk_stride;
     uint64_t ctr0 = a.ctr_first, cpc = a.ctr_per_chunk;
     int qq = q;
-    if (a.n_seg) {  // last segment starting at or before q
-      int lo = 0, hi = a.n_seg - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (a.seg[mid].row_begin <= q) lo = mid;
-        else hi = mid - 1;
+    if constexpr (Args::kPacked) {
+      if (a.n_seg) {  // last segment starting at or before q
+        int lo = 0, hi = a.n_seg - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (a.seg[mid].row_begin <= q) lo = mid;
+          else hi = mid - 1;
+        }
+        out = a.seg[lo].out;
+        stride = a.seg[lo].out_chunk_stride;
+        ctr0 = a.seg[lo].ctr_first;
+        cpc = a.seg[lo].ctr_per_chunk;
+        qq = q - a.seg[lo].row_begin;
       }
-      out = a.seg[lo].out;
-      stride = a.seg[lo].out_chunk_stride;
-      ctr0 = a.seg[lo].ctr_first;
-      cpc = a.seg[lo].ctr_per_chunk;
-      qq = q - a.seg[lo].row_begin;
     }
     const size_t idx = (size_t)qq * w + local;
     uint32_t v[16];
@@ -796,8 +798,8 @@ struct RotLds {
 // One block tile (query rows q0 .., DB rows db0 ..) of the rotation-aware
 // form: plane DMA, K loop, epilogue. The caller orders it after every earlier
 // use of `sm` by the block.
-template <int TQ, int TD, int NW, bool kWpe1>
-__device__ __forceinline__ void rot_tile(const GemmArgs& a, int q0, size_t db0, RotLds<TQ, TD, NW>& sm,
+template <int TQ, int TD, int NW, bool kWpe1, class Args>
+__device__ __forceinline__ void rot_tile(const Args& a, int q0, size_t db0, RotLds<TQ, TD, NW>& sm,
                                          int tid) {
   constexpr int BQ = TQ * kRotations, BD = NW * TD * 32;  // 32 TQ rows >= 31 TQ
   constexpr int G = TQ * TD, HALF = 3 * TQ * TD;  // MFMAs per pass, per k-slice
@@ -1069,6 +1071,33 @@ void share_gemm_rot_multi_kernel(const GemmArgs* __restrict__ descs, const uint3
   }
 }
 
+// Several plain launches of the 4-query kernel as one grid: blocks
+// [block_end[i - 1], block_end[i]) are launch i's, each segment a multiple of
+// 8 blocks, so a block's XCD (block mod 8) and its tile are those of the plain
+// launch (block_tile on the block's index within its segment).
+struct GemmHeads {
+  GemmHead h[kMaxHeads];
+  uint32_t block_end[kMaxHeads];
+  int n;
+};
+
+template <int TQ, int TD, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void share_gemm_rot_heads_kernel(const GemmHeads hs) {
+  constexpr int BQ = TQ * kRotations, BD = NW * TD * 32;
+  __shared__ RotLds<TQ, TD, NW> sm;
+  const uint32_t b = blockIdx.x;
+  int d = 0;
+  while (d + 1 < hs.n && hs.block_end[d] <= b) d++;
+  const GemmHead& a = hs.h[d];
+  const uint32_t local = b - (d ? hs.block_end[d - 1] : 0u);
+  const int T = a.n_qtiles * a.n_dtiles, T8 = (T + 7) / 8;
+  const int t = (int)(local & 7) * T8 + (int)(local >> 3);
+  if (t >= T) return;
+  const int dtile = t / a.n_qtiles, qtile = t % a.n_qtiles;
+  rot_tile<TQ, TD, NW, true>(a, qtile * BQ, a.db_begin + (size_t)dtile * BD, sm, (int)threadIdx.x);
+}
+
 // Padded unrotated limb planes of every query slot from the rotated u16 rows
 // (rotation 15 is the unrotated share; slots past the rows are zero shares).
 __global__ __launch_bounds__(256) void rot_planes_kernel(const uint16_t* rows, int qr, int K,
@@ -1120,7 +1149,8 @@ void launch_share_gemm_multi(const GemmArgs* descs, const uint32_t* tile_end, in
   IRIS_HIP_CHECK(hipGetLastError());
 }
 
-void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s) {
+// The geometry checks of a plain launch.
+static void check_gemm_args(const GemmHead& a, int variant, int n_seg) {
   const GemmTile t = gemm_tile(variant);
   IRIS_REQUIRE(a.db_begin % 16 == 0, "db_begin must be a multiple of 16");
   IRIS_REQUIRE((a.db_end - a.db_begin) % 64 == 0 && a.db_end > a.db_begin, "bad DB range");
@@ -1128,8 +1158,28 @@ void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s) {
   IRIS_REQUIRE(a.n_qtiles == (int)ceil_div((size_t)a.qr, (size_t)t.bq), "n_qtiles mismatch");
   IRIS_REQUIRE(!is_rot_variant(variant) || (a.qrot != nullptr && (a.n_kt * kKTile) % 800 == 0),
                "rotation-aware variant needs a query uploaded from raw shares");
-  IRIS_REQUIRE(a.n_seg >= 0 && a.n_seg <= kMaxSegments, "too many batches in one packed launch");
+  IRIS_REQUIRE(n_seg >= 0 && n_seg <= kMaxSegments, "too many batches in one packed launch");
   IRIS_REQUIRE(a.n_dtiles == (int)ceil_div(a.db_end - a.db_begin, (size_t)t.bd), "n_dtiles mismatch");
+}
+
+void launch_share_gemm_heads(const GemmHead* heads, int n, hipStream_t s) {
+  IRIS_REQUIRE(n >= 1 && n <= kMaxHeads, "1 .. 6 launches per joint launch");
+  GemmHeads hs{};
+  uint32_t blocks = 0;
+  for (int i = 0; i < n; i++) {
+    check_gemm_args(heads[i], kRotVariantBase, 0);
+    const uint32_t T = (uint32_t)(heads[i].n_qtiles * heads[i].n_dtiles);
+    hs.h[i] = heads[i];
+    blocks += 8 * ((T + 7) / 8);
+    hs.block_end[i] = blocks;
+  }
+  hs.n = n;
+  share_gemm_rot_heads_kernel<4, 2, 4><<<blocks, 256, 0, s>>>(hs);
+  IRIS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s) {
+  check_gemm_args(a, variant, a.n_seg);
   const int T = a.n_qtiles * a.n_dtiles;
   const unsigned grid = (unsigned)(8 * ((T + 7) / 8));
   switch (variant) {

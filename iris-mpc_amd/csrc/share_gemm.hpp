@@ -26,7 +26,9 @@ struct GemmSegment {
   int rows;
 };
 
-struct GemmArgs {
+// One launch's arguments without the packed-batch table.
+struct GemmHead {
+  static constexpr bool kPacked = false;
   const int8_t* q;      // tiled query limbs (rows padded to the q tile)
   const int8_t* db;     // tiled DB limbs
   const int8_t* qrot;   // padded unrotated query limb planes (rotation-aware variants)
@@ -49,6 +51,10 @@ struct GemmArgs {
   ChaChaKey key_own, key_prev;
   uint32_t mult;
   int masked;
+};
+
+struct GemmArgs : GemmHead {
+  static constexpr bool kPacked = true;
   int n_seg;                // 0: one batch (out, ctr_first, ...); else the table below
   GemmSegment seg[kMaxSegments];
 };
@@ -106,6 +112,14 @@ void launch_share_gemm(const GemmArgs& a, int variant, hipStream_t s);
 // tile is done. No allocation or synchronisation: capturable.
 void launch_share_gemm_multi(const GemmArgs* descs, const uint32_t* tile_end, int n_desc, int grid,
                              hipStream_t s);
+// Up to kMaxHeads plain launches of the rotation-aware 4-query kernel
+// (variant kRotVariantBase; each head as launch_share_gemm would take it,
+// n_seg = 0) as one grid: head i's blocks follow head i - 1's, so the
+// dispatcher starts a launch's blocks as the previous launch drains and the
+// launches share one kernel boundary. The heads go by value (no device copy):
+// capturable, nothing allocated.
+constexpr int kMaxHeads = 6;
+void launch_share_gemm_heads(const GemmHead* heads, int n, hipStream_t s);
 // Padded unrotated limb planes for the rotation-aware variants, built from the
 // rotated u16 query rows [qr][K].
 size_t rot_planes_bytes(int qr, int K);
