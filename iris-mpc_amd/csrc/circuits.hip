@@ -53,8 +53,16 @@ CircuitSchedule CircuitSchedule::make(uint64_t c, size_t W, size_t alloc) {
 //   p1: x1=(0,b) x2=(a,0) x3=0  -> a & b
 //   p2: x1=0 x2=(0,b) x3=(a,0)  -> a ^ (a & b)
 __device__ __forceinline__ uint32_t csa_and(int p, uint32_t a, uint32_t b) {
-  return p == 2 ? (a & ~b) : (a & b);
+  const uint32_t n = p == 2 ? ~0u : 0u;  // p is wave-uniform: a mask, not a select
+  return a & (b ^ n);
 }
+
+// All ones for party q, zero otherwise. p is wave-uniform, so this is one
+// scalar load from a constant table; a mask the compiler cannot see to be all
+// or nothing stays an AND that folds into a three-input bit op (computed as
+// p == q ? ~0 : 0 it becomes a per-lane select plus the xor).
+__constant__ uint32_t kPartyEq[3] = {~0u, 0u, 0u};
+__device__ __forceinline__ uint32_t party_mask(int p, int q) { return kPartyEq[(p - q + 3) % 3]; }
 
 // ------------------------------------------------------------ ripple helpers
 // Local share of the AND of replicated shares (x_a, x_b) and (y_a, y_b) plus
@@ -211,14 +219,28 @@ __global__ __launch_bounds__(192) void inject_kernel(const Trio t, uint64_t ctr)
 
 // Transpose a 32 x 32 bit matrix in place: afterwards bit i of x[j] is bit j
 // of the old x[i] (recursive block swap, 5 stages).
+// Stage S swaps, between x[k] and x[k + S], the S-bit fields that the mask M
+// marks in x[k + S] with their neighbours above them in x[k]. The 16- and 8-bit
+// stages move whole bytes: one v_perm_b32 per output, whose selector bytes
+// 0-3 pick the bytes of the second operand and 4-7 those of the first. The
+// others are a shift and a bit-field insert per output.
 template <int S, uint32_t M, int N = 32>
 __device__ __forceinline__ void bit_transpose_stage(uint32_t x[]) {
 #pragma unroll
   for (int k = 0; k < N; k++)
     if ((k & S) == 0) {
-      const uint32_t t = ((x[k] >> S) ^ x[k + S]) & M;
-      x[k + S] ^= t;
-      x[k] ^= t << S;
+      const uint32_t a = x[k], b = x[k + S];
+      if constexpr (S == 16) {
+        x[k] = __builtin_amdgcn_perm(b, a, 0x05040100u);      // a.lo | b.lo << 16
+        x[k + S] = __builtin_amdgcn_perm(b, a, 0x07060302u);  // a.hi | b.hi << 16
+      } else if constexpr (S == 8) {
+        x[k] = __builtin_amdgcn_perm(b, a, 0x06020400u);      // bytes a0 b0 a2 b2
+        x[k + S] = __builtin_amdgcn_perm(b, a, 0x07030501u);  // bytes a1 b1 a3 b3
+      } else {
+        static_assert(M << S == ~M, "the fields and their neighbours tile the word");
+        x[k] = (a & M) | ((b << S) & ~M);
+        x[k + S] = ((a >> S) & M) | (b & ~M);
+      }
     }
 }
 
@@ -281,20 +303,19 @@ struct AdderLds {
 // The ChaCha12 block of this lane's gate in batch b (8 steps, 16 gates): gate
 // gl of its 16-lane group is step 8 b + gl / 2, the carry-save AND of that
 // step's plane (gl even) or its ripple round s - 1 (gl odd; the lift's step 0
-// slot is the carry-out). Gates past the circuit draw nothing (zero block).
+// slot is the carry-out). Gates past the circuit (the MSB adder's step 31 and
+// its step 0 ripple slot) are not valid: their blocks are never read.
 template <bool MSB>
 __device__ __forceinline__ void batch_gate(int b, int gl, const RippleCtrs& rc, size_t G, size_t g,
                                            bool& valid, uint64_t& ctr) {
   constexpr int LAST = MSB ? 30 : 15;
   constexpr int LAST_C = MSB ? 29 : 14;
   const int sj = gl >> 1, s = 8 * b + sj;
-  uint64_t c_round = 0;
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const int k = 8 * b + j - 1;  // ripple round of step 8 b + j
-    const uint64_t v = (k >= 0 && k <= LAST_C) ? rc.c[k] : (!MSB && k < 0 ? rc.c[15] : 0);
-    if (sj == j) c_round = v;
-  }
+  // ripple round k = s - 1 of this lane's step: a per-lane index into the
+  // counter table (one load from the kernel arguments, not a chain of selects)
+  const int k = s - 1;
+  const int kc = k < 0 ? (MSB ? 0 : 15) : (k > LAST_C ? LAST_C : k);
+  const uint64_t c_round = rc.c[kc];
   if ((gl & 1) == 0) {  // carry-save AND of plane s
     valid = s <= LAST;
     ctr = rc.csa + (uint64_t)s * G + g;
@@ -328,7 +349,7 @@ __device__ __forceinline__ void draw_piece(NextDraw& d, const ChaChaKey& key, ui
   } else if constexpr (J == 6 && kBeforeBarrier) {
     chacha_finish(key, nonce, d.ctr, d.x);
 #pragma unroll
-    for (int i = 0; i < 16; i++) rnd_row[i] = d.valid ? d.x[i] : 0u;
+    for (int i = 0; i < 16; i++) rnd_row[i] = d.x[i];  // an invalid gate's block is never read
   }
 }
 
@@ -362,7 +383,7 @@ __device__ __forceinline__ void adder_step(const SL& W, AdderState& st,
     st.tb = nb;
     st.xa = st.xb = st.va = st.vb = 0;
   } else if constexpr (S <= LAST) {
-    const uint32_t ba = st.ca ^ (p == 2 ? st.ta : 0u), bb = st.cb ^ (p == 0 ? st.tb : 0u);
+    const uint32_t ba = st.ca ^ (st.ta & party_mask(p, 2)), bb = st.cb ^ (st.tb & party_mask(p, 0));
     const uint32_t na = W.a(S), nb = W.b(S);
     st.xa ^= st.va;  // carry after round S - 2
     st.xb ^= st.vb;
@@ -388,7 +409,7 @@ __device__ __forceinline__ void adder_step(const SL& W, AdderState& st,
       // carry-out (protocol.rs:2229-2239): c2 = b15 & c (shared AND), c1 = c ^ b15
       st.xa ^= st.va;
       st.xb ^= st.vb;
-      const uint32_t ba2 = st.ca ^ (p == 2 ? st.ta : 0u), bb2 = st.cb ^ (p == 0 ? st.tb : 0u);
+      const uint32_t ba2 = st.ca ^ (st.ta & party_mask(p, 2)), bb2 = st.cb ^ (st.tb & party_mask(p, 0));
       const uint32_t v2 = and_pre(ba2, bb2, st.xa, st.xb, r_out);
       st.xa ^= ba2;
       st.xb ^= bb2;
@@ -542,7 +563,7 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
     // msb = c ^ a_30 ^ b_30, a_30 = plane 31 (protocol.rs:2347-2352)
     st.xa ^= st.va;
     st.xb ^= st.vb;
-    const uint32_t ba = st.ca ^ (p == 2 ? st.ta : 0u), bb = st.cb ^ (p == 0 ? st.tb : 0u);
+    const uint32_t ba = st.ca ^ (st.ta & party_mask(p, 2)), bb = st.cb ^ (st.tb & party_mask(p, 0));
     reinterpret_cast<uint32_t*>(P.res_a)[hw] = st.xa ^ sl.a(31) ^ ba;
     reinterpret_cast<uint32_t*>(P.res_b)[hw] = st.xb ^ sl.b(31) ^ bb;
   } else {
@@ -708,7 +729,7 @@ __global__ __launch_bounds__(192) void compare_fused_kernel(const Trio t, uint32
   // msb = c ^ a_30 ^ b_30, a_30 = plane 31 (protocol.rs:2347-2352)
   st.xa ^= st.va;
   st.xb ^= st.vb;
-  const uint32_t ba = st.ca ^ (p == 2 ? st.ta : 0u), bb = st.cb ^ (p == 0 ? st.tb : 0u);
+  const uint32_t ba = st.ca ^ (st.ta & party_mask(p, 2)), bb = st.cb ^ (st.tb & party_mask(p, 0));
   reinterpret_cast<uint32_t*>(P.res_a)[hw] = st.xa ^ sl.a(31) ^ ba;
   reinterpret_cast<uint32_t*>(P.res_b)[hw] = st.xb ^ sl.b(31) ^ bb;
 }
