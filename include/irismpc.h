@@ -590,6 +590,16 @@ int irismpc_serving_result(const irismpc_serving* s, uint64_t id, int* state, in
  * counterpart in the reference, whose batches see the writes applied before
  * them (actor.rs:584-644); it states the same order for the serving form. */
 int irismpc_serving_result_version(const irismpc_serving* s, uint64_t id, uint64_t* db_version);
+/* The in-flight matches of a finished uniqueness request: the ids of the
+ * uniqueness requests admitted before it and still scanning at its admission
+ * (the same batch, before it, in the per-batch form) whose iris it matches at
+ * the loop's threshold, over its 31 rotations, as the reference marks a
+ * batch's later query that matches an earlier one (compare_query_against_self
+ * and mergeBatchResults, actor.rs:1789-1897, dot/kernel.cu:352-386, whose
+ * final result UINT_MAX - 1 flags such a query). Ids copied up to max_ids in
+ * admission order, *n_ids = their number. */
+int irismpc_serving_result_batch_matches(const irismpc_serving* s, uint64_t id, uint64_t* ids, size_t max_ids,
+                                         size_t* n_ids);
 /* Forgets a finished request (fails while it is pending). A finished
  * request's shares are dropped at once; its result stays until released, so a
  * long-running host releases every request it has read. */

@@ -937,6 +937,17 @@ int irismpc_serving_result_version(const irismpc_serving* s, uint64_t id, uint64
   });
 }
 
+int irismpc_serving_result_batch_matches(const irismpc_serving* s, uint64_t id, uint64_t* ids, size_t max_ids,
+                                         size_t* n_ids) {
+  return guarded([&] {
+    Serving::Result r;
+    IRIS_REQUIRE(s->impl->result(id, &r), "unknown request id");
+    IRIS_REQUIRE(r.state != kReqPending, "request still pending");
+    if (ids) std::copy_n(r.batch_matches.begin(), std::min(max_ids, r.batch_matches.size()), ids);
+    if (n_ids) *n_ids = r.batch_matches.size();
+  });
+}
+
 int irismpc_serving_release(irismpc_serving* s, uint64_t id) {
   return guarded([&] { IRIS_REQUIRE(s->impl->release(id), "request still pending"); });
 }

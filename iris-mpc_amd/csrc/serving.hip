@@ -60,7 +60,7 @@ bool event_done(hipEvent_t e) {
 
 // ====================================================================== EngineTrio
 EngineTrio::EngineTrio(int device, size_t maxq, size_t db_capacity, const ChaChaKey own[3],
-                       const ChaChaKey prev[3], int tag0, uint64_t nonce_base)
+                       const ChaChaKey prev[3], int tag0, uint64_t nonce_base, size_t chunk_rows)
     : max_queries(maxq) {
   DeviceGuard g(device);
   ex = std::make_unique<Exchange>(device, 300.0);
@@ -68,11 +68,11 @@ EngineTrio::EngineTrio(int device, size_t maxq, size_t db_capacity, const ChaCha
   const std::vector<uint16_t> zc(maxq * kCodeLen, 0), zm(maxq * kMaskLen, 0);
   for (int p = 0; p < 3; p++) {
     IRIS_HIP_CHECK(hipStreamCreateWithFlags(&s[p], hipStreamNonBlocking));
-    codes[p] = std::make_unique<ShareDB>(device, p, kDbChunk, rows, kCodeLen, derive(own[p], nonce_base),
+    codes[p] = std::make_unique<ShareDB>(device, p, chunk_rows, rows, kCodeLen, derive(own[p], nonce_base),
                                          derive(prev[p], nonce_base), ex.get(), tag0);
-    masks[p] = std::make_unique<ShareDB>(device, p, kDbChunk, rows, kMaskLen, derive(own[p], nonce_base + 1),
+    masks[p] = std::make_unique<ShareDB>(device, p, chunk_rows, rows, kMaskLen, derive(own[p], nonce_base + 1),
                                          derive(prev[p], nonce_base + 1), ex.get(), tag0 + 1);
-    circ[p] = std::make_unique<Circuits>(device, p, rows * kDbChunk, rows * kDbChunk / 64,
+    circ[p] = std::make_unique<Circuits>(device, p, rows * chunk_rows, rows * chunk_rows / 64,
                                          derive(own[p], nonce_base + 5), derive(prev[p], nonce_base + 5),
                                          ex.get(), tag0 + 2);
     cmp[p] = std::make_unique<DistanceComparator>(device, rows, db_capacity);
@@ -140,11 +140,27 @@ Serving::Serving(int device, const ChaChaKey party_seeds[3], DbSlice* const code
   scan_ = std::make_unique<EngineTrio>(device, max_active, capacity, own, prev, 0, 0);
   // reauth: one query over a 64-row window, its own keystreams
   small_ = std::make_unique<EngineTrio>(device, 1, capacity, own, prev, 8, 16);
+  // the self-comparison of requests in flight: up to max_active queries
+  // against the set's irises (one step of a 512-row DB), its own keystreams
+  self_ = std::make_unique<EngineTrio>(device, max_active, kSelfChunk, own, prev, 24, 32, kSelfChunk);
   scan_->parties(code_dbs_, mask_dbs_, scan_p_);
   small_->parties(code_dbs_, mask_dbs_, small_p_);
+  {
+    DbSlice *sc[3], *sm[3];
+    for (int p = 0; p < 3; p++) {
+      self_code_db_[p] = self_->codes[p]->alloc_db(kSelfChunk);
+      self_mask_db_[p] = self_->masks[p]->alloc_db(kSelfChunk);
+      sc[p] = self_code_db_[p].get();
+      sm[p] = self_mask_db_[p].get();
+      self_raw_code_[p].alloc(max_active * kCodeLen, false);
+      self_raw_mask_[p].alloc(max_active * kMaskLen, false);
+    }
+    self_->parties(sc, sm, self_p_);
+  }
   for (int p = 0; p < 3; p++) {
     bitmaps_[p].alloc(max_active * words_ + 2);
     small_bitmaps_[p].alloc(words_ + 2);
+    self_bitmaps_[p].alloc(max_active + 2);
     slot_maps_[p].alloc(max_active);
     dummy_code_[p].alloc(kCodeLen);
     dummy_mask_[p].alloc(kMaskLen);
@@ -178,6 +194,7 @@ Serving::Serving(int device, const ChaChaKey party_seeds[3], DbSlice* const code
     write_dev_mask_[k].alloc(3 * kMaskLen, false);
   }
   readout_.alloc(max_active * words_);
+  for (int k = 0; k < kSelfRing; k++) self_host_[k].alloc(max_active);
   small_code_.alloc(3 * kCodeLen);
   small_mask_.alloc(3 * kMaskLen);
   for (int p = 0; p < 3; p++) {
@@ -405,6 +422,8 @@ void Serving::loop_main(int policy, size_t max_batch, size_t reserve_bytes) {
     else
       loop_per_batch(max_batch);
     drain_steps();
+    for (auto& r : self_rounds_) recycle(r.done);  // a loop left early: rounds nobody reads
+    self_rounds_.clear();
   } catch (const std::exception& e) {
     {
       std::lock_guard<std::mutex> lk(mu_);
@@ -580,7 +599,7 @@ void Serving::serve_reauth(Req* r) {
 }
 
 // ---------------------------------------------------------------- device steps
-void Serving::refill_scan(const std::vector<const Req*>& reqs, size_t batch, size_t rows) {
+int Serving::refill_scan(const std::vector<const Req*>& reqs, size_t batch, size_t rows) {
   IRIS_REQUIRE(batch >= 1 && batch <= max_active_ && reqs.size() <= batch, "query set beyond the slots");
   PhaseTimer pt(&stats_.t_refill, &mu_);
   {
@@ -611,6 +630,66 @@ void Serving::refill_scan(const std::vector<const Req*>& reqs, size_t batch, siz
     hipEvent_t e = event();
     IRIS_HIP_CHECK(hipEventRecord(e, s));
     refill_ev_[k].push_back(e);
+  }
+  return k;
+}
+
+void Serving::self_compare(int k, const std::vector<uint64_t>& ids, size_t q0,
+                           const std::vector<std::pair<Req*, size_t>>& fresh) {
+  const size_t n = ids.size(), nq = n - q0;
+  IRIS_REQUIRE(q0 >= 1 && q0 < n && n <= max_active_, "self-comparison: queries after the first of 2 .. max_active");
+  const int buf = (int)(self_seq_ % kSelfRing);
+  // the round that last used this readout buffer is decoded first
+  if (self_rounds_.size() >= (size_t)kSelfRing) collect_self(self_rounds_.front().seq);
+  for (int p = 0; p < 3; p++) {
+    hipStream_t s = self_->s[p];
+    const uint16_t* bc = refill_code_[k].ptr + (size_t)p * max_active_ * kCodeLen;
+    const uint16_t* bm = refill_mask_[k].ptr + (size_t)p * max_active_ * kMaskLen;
+    // the set's irises (positions 0 .. n) as the DB, the queries at q0 ..
+    IRIS_HIP_CHECK(hipMemcpyAsync(self_raw_code_[p].ptr, bc, n * kCodeLen * 2, hipMemcpyHostToDevice, s));
+    IRIS_HIP_CHECK(hipMemcpyAsync(self_raw_mask_[p].ptr, bm, n * kMaskLen * 2, hipMemcpyHostToDevice, s));
+    self_->codes[p]->load_records_device(*self_code_db_[p], 0, n, self_raw_code_[p].ptr, s);
+    self_->masks[p]->load_records_device(*self_mask_db_[p], 0, n, self_raw_mask_[p].ptr, s);
+    self_->codes[p]->refill_query_shares_async(*self_->qc[p], bc + q0 * kCodeLen, nq, nq * kRotations, s, false);
+    self_->masks[p]->refill_query_shares_async(*self_->qm[p], bm + q0 * kMaskLen, nq, nq * kRotations, s, false);
+    hipEvent_t e = event();  // the staging ring entry is reused after these reads
+    IRIS_HIP_CHECK(hipEventRecord(e, s));
+    refill_ev_[k].push_back(e);
+    fill_u64(self_bitmaps_[p].ptr, nq + 2, 0, s);
+  }
+  uint64_t* const bms[3] = {self_bitmaps_[0].ptr, self_bitmaps_[1].ptr, self_bitmaps_[2].ptr};
+  // bit j of row i: query q0 + i (31 rotations) against iris j
+  scan_chunk_trio(self_p_, 0, n, nq, threshold_a_, bms, nullptr, 64, self_->s);
+  hipStream_t s0 = self_->s[0];
+  IRIS_HIP_CHECK(hipMemcpyAsync(self_host_[buf].ptr, self_bitmaps_[0].ptr, nq * 8, hipMemcpyDeviceToHost, s0));
+  SelfRound r;
+  r.seq = ++self_seq_;
+  r.buf = buf;
+  r.done = event();
+  IRIS_HIP_CHECK(hipEventRecord(r.done, s0));
+  r.ids = ids;
+  r.q0 = q0;
+  r.fresh = fresh;
+  for (auto& f : fresh) f.first->self_round = r.seq;
+  self_rounds_.push_back(std::move(r));
+}
+
+void Serving::collect_self(uint64_t upto) {
+  while (!self_rounds_.empty()) {
+    SelfRound& r = self_rounds_.front();
+    if (r.seq > upto && !event_done(r.done)) break;
+    IRIS_HIP_CHECK(hipEventSynchronize(r.done));
+    const uint64_t* rows = self_host_[r.buf].ptr;
+    for (const auto& f : r.fresh) {
+      Req* q = f.first;
+      const size_t b = f.second;
+      // above the diagonal only: the requests admitted before it
+      for (size_t j = 0; j < b; j++)
+        if (r.ids[j] && ((rows[b - r.q0] >> j) & 1)) q->res.batch_matches.push_back(r.ids[j]);
+      q->self_round = 0;
+    }
+    recycle(r.done);
+    self_rounds_.pop_front();
   }
 }
 
@@ -809,8 +888,10 @@ void Serving::loop_continuous(size_t reserve_bytes) {
       recycle(r->readout);
       r->readout = nullptr;
       free_slots.push_back(r->slot);
+      if (r->self_round) collect_self(r->self_round);
       complete(r, kReqDone);
     }
+    collect_self(0);  // the finished ones
     const bool idle = active.empty() && waiting.empty() && reading.empty();
     in.clear();
     const bool stopping = take_incoming(&in, idle);
@@ -829,6 +910,7 @@ void Serving::loop_continuous(size_t reserve_bytes) {
     if ((stopping && in.empty() && idle) || abort_) break;
     // admission between chunk steps: up to active_cap requests, or every
     // waiting one while more than queue_threshold wait (slots and HBM allowing)
+    std::vector<std::pair<Req*, size_t>> fresh;  // admitted now, with their positions
     while (!waiting.empty() && !free_slots.empty() &&
            (active.size() < active_cap_ || waiting.size() > queue_threshold_)) {
       if (!active.empty()) {  // with nothing active one request always joins
@@ -852,6 +934,7 @@ void Serving::loop_continuous(size_t reserve_bytes) {
       r->covered.assign(r->need, 0);
       r->n_covered = 0;
       clear_slot(r->slot);
+      if (!active.empty()) fresh.emplace_back(r, active.size());
       active.push_back(r);
     }
     {
@@ -871,8 +954,11 @@ void Serving::loop_continuous(size_t reserve_bytes) {
     }
     if (ids != active_ids) {
       std::vector<const Req*> q(active.begin(), active.end());
-      refill_scan(q, active.size(), active.size() * kRotations);
+      const int k = refill_scan(q, active.size(), active.size() * kRotations);
       active_ids = ids;
+      // the newly admitted requests (the set's last positions, in admission
+      // order) against every request admitted before them and still scanning
+      if (!fresh.empty()) self_compare(k, ids, fresh.front().second, fresh);
     }
     set_slots(slots);
     const size_t rows = n_rows(), n = n_chunks_of(rows);
@@ -939,7 +1025,16 @@ void Serving::loop_per_batch(size_t max_batch) {
     }
     if (batch.empty()) continue;
     std::vector<const Req*> q(batch.begin(), batch.end());
-    refill_scan(q, max_batch, max_batch * kRotations);
+    // the batch's uniqueness requests against the earlier ones of the batch
+    std::vector<uint64_t> uids(batch.size(), 0);
+    std::vector<std::pair<Req*, size_t>> fresh;
+    for (size_t j = 0; j < batch.size(); j++)
+      if (batch[j]->kind == kReqUniqueness) {
+        uids[j] = batch[j]->id;
+        if (j > 0) fresh.emplace_back(batch[j], j);
+      }
+    const int k = refill_scan(q, max_batch, max_batch * kRotations);
+    if (!fresh.empty()) self_compare(k, uids, fresh.front().second, fresh);
     const size_t rows = n_rows(), n = n_chunks_of(rows);
     {
       std::lock_guard<std::mutex> lk(mu_);
@@ -973,6 +1068,7 @@ void Serving::loop_per_batch(size_t max_batch) {
       } else {
         r->res.value = (int64_t)m.size();
         r->res.matches = std::move(m);
+        if (r->self_round) collect_self(r->self_round);
       }
       complete(r, kReqDone);
     }

@@ -25,7 +25,14 @@
 // adds that chunk), so each answer is that of the DB as it stands at the
 // request's last step: the DB after exactly Result::db_version of the loop's
 // writes, in the order the loop applied them (as a batch of the reference
-// sees the writes before it). Every step runs,
+// sees the writes before it). Requests in flight together are compared with
+// each other as the reference compares a batch with itself
+// (compare_query_against_self and mergeBatchResults, actor.rs:1789-1897,
+// kernel.cu:352-386): after a refill that admitted uniqueness requests, every
+// query of the active set against the set's own irises, and a request reports
+// the earlier-admitted requests still scanning at its admission that it
+// matches (Result::batch_matches). The per-batch form does the same for each
+// batch. Every step runs,
 // for all active queries, one share GEMM per matrix and party over 1-16
 // chunks, the reshare, the threshold circuits and the open into each
 // request's bitmap row (slot map). Nothing between steps drains the queued
@@ -83,8 +90,9 @@ struct PinnedBuf {
 // its own nonce_base, so no two draw the same keystream (the reference takes
 // a fresh kdf nonce per engine, actor.rs:382-388).
 struct EngineTrio {
+  // chunk_rows: DB rows per scan step (the result and circuit buffers' size)
   EngineTrio(int device, size_t max_queries, size_t db_capacity, const ChaChaKey own[3],
-             const ChaChaKey prev[3], int tag0, uint64_t nonce_base);
+             const ChaChaKey prev[3], int tag0, uint64_t nonce_base, size_t chunk_rows = kDbChunk);
   ~EngineTrio();
   EngineTrio(const EngineTrio&) = delete;
   EngineTrio& operator=(const EngineTrio&) = delete;
@@ -152,6 +160,9 @@ class Serving {
     // DB writes applied before the answer (a write: its own position, from 1):
     // the answer is that of the DB after the first db_version writes
     uint64_t db_version = 0;
+    // uniqueness: the requests admitted before it and scanning with it (the
+    // same batch in the per-batch form) whose iris it matches, by id
+    std::vector<uint64_t> batch_matches;
     std::string error;
   };
   // false when the id is unknown
@@ -198,6 +209,7 @@ class Serving {
     size_t need = 0, n_covered = 0;
     std::vector<uint8_t> covered;
     hipEvent_t readout = nullptr;
+    uint64_t self_round = 0;  // the self-comparison holding its row (0: none)
   };
   struct StepEvents {
     hipEvent_t e[3];
@@ -218,7 +230,18 @@ class Serving {
   void fail_all(const std::string& msg);
 
   // device steps (loop thread, or the caller while stopped)
-  void refill_scan(const std::vector<const Req*>& reqs, size_t batch, size_t rows);
+  // returns the staging ring entry that holds the set's raw shares
+  int refill_scan(const std::vector<const Req*>& reqs, size_t batch, size_t rows);
+  // The self-comparison after a refill from ring entry k: ids[j] is the
+  // uniqueness request at position j of the set (0 elsewhere); the queries
+  // at positions [q0, ids.size()) (31 rotations each) against the set's
+  // irises, on the comparison engines and streams (beside the scan); each
+  // fresh request at position b gets the ids[j], j < b, whose iris it
+  // matches.
+  void self_compare(int k, const std::vector<uint64_t>& ids, size_t q0,
+                    const std::vector<std::pair<Req*, size_t>>& fresh);
+  // decodes the finished self-comparisons (all up to round `upto`, waiting)
+  void collect_self(uint64_t upto);
   void set_slots(const std::vector<int>& slots);
   void clear_slot(int slot);
   void step(size_t batch, size_t chunk_index, size_t n_chunks, size_t db_rows);
@@ -256,6 +279,24 @@ class Serving {
   hipEvent_t last_write_[3] = {nullptr, nullptr, nullptr};  // DB writes, per scan stream
   PinnedBuf<uint64_t> readout_;  // party 0's bitmap row per slot
   PinnedBuf<uint16_t> small_code_, small_mask_;
+  struct SelfRound {
+    uint64_t seq = 0;
+    int buf = 0;
+    hipEvent_t done = nullptr;
+    std::vector<uint64_t> ids;
+    size_t q0 = 0;
+    std::vector<std::pair<Req*, size_t>> fresh;
+  };
+  static constexpr int kSelfRing = 4;
+  static constexpr size_t kSelfChunk = 512;  // DB rows of the comparison engines' step
+  std::unique_ptr<EngineTrio> self_;         // the comparison engines (own keystreams and streams)
+  std::unique_ptr<DbSlice> self_code_db_[3], self_mask_db_[3];  // the set's irises, one row each
+  PartyScan self_p_[3];
+  DevBuf<uint16_t> self_raw_code_[3], self_raw_mask_[3];  // staging of the set's raw shares
+  DevBuf<uint64_t> self_bitmaps_[3];     // one 64-bit row per compared query
+  PinnedBuf<uint64_t> self_host_[kSelfRing];
+  std::deque<SelfRound> self_rounds_;
+  uint64_t self_seq_ = 0;
   std::deque<StepEvents> inflight_;
   std::vector<hipEvent_t> event_pool_;
   std::vector<hipEvent_t> recycle_later_;  // events of released requests (destroyed at the end)

@@ -137,6 +137,7 @@ SIGNATURES = {
     "irismpc_serving_submit": [vp, i32, vp, vp, i64, dbl, vp],
     "irismpc_serving_result": [vp, u64, vp, vp, vp, sz, vp, vp, vp, vp, sz],
     "irismpc_serving_result_version": [vp, u64, vp],
+    "irismpc_serving_result_batch_matches": [vp, u64, vp, sz, vp],
     "irismpc_serving_release": [vp, u64],
     "irismpc_serving_now": [vp, vp],
     "irismpc_serving_stats": [vp, vp],

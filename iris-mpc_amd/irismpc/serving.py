@@ -67,6 +67,10 @@ class Request:
     error: str = None         # set when serving the request failed
     db_rows: int = 0          # DB rows the request was checked against
     db_version: int = 0       # DB writes the answer reflects (a write: its own position, from 1)
+    # uniqueness: the requests admitted before it and scanning with it (its
+    # batch, in the per-batch form) whose iris it matches (native ids from
+    # fetch; the request loops map them to the Request objects' rids)
+    batch_matches: list = None
 
 
 def _pack_shares(shares):
@@ -211,6 +215,12 @@ class ServingEngines:
                 call("irismpc_serving_result", self.h, i, None, None, out.ctypes.data, n.value, ctypes.byref(n),
                      None, None, None, 0)
             r.result = out[:n.value].tolist()
+            k = ctypes.c_size_t()
+            call("irismpc_serving_result_batch_matches", self.h, i, None, 0, ctypes.byref(k))
+            ids = (ctypes.c_uint64 * max(1, k.value))()
+            if k.value:
+                call("irismpc_serving_result_batch_matches", self.h, i, ids, k.value, ctypes.byref(k))
+            r.batch_matches = list(ids[:k.value])
         elif r.kind == "reauth":
             r.result = bool(value.value)
         else:
@@ -253,9 +263,12 @@ class _Loop:
                 raise RuntimeError(f"the schedule was handed over {late * 1e3:.1f} ms after its start")
         finally:
             eng.finish(self.timeout_s + max(0.0, reqs[-1].arrival if reqs else 0.0))
+        rid_of = {i: r.rid for r, i in zip(reqs, ids)}
         for r, i in zip(reqs, ids):
             eng.fetch(i, r, t0)
             call("irismpc_serving_release", eng.h, i)
+            if r.batch_matches is not None:
+                r.batch_matches = [rid_of[j] for j in r.batch_matches]
             if r.error:
                 self.failed.append(r)
         st = self.stats = eng.stats()

@@ -11,7 +11,10 @@ Checked on a 4-chunk DB of synthetic irises (4,096 distinct irises tiled):
   run at a time when the reserve leaves no room;
 - step sizes chosen by the autotuner leave every result unchanged;
 - a server starting from an empty DB (as the reference's does) answers
-  uniqueness requests with no matches, then finds an inserted iris.
+  uniqueness requests with no matches, then finds an inserted iris;
+- requests in flight together are compared with each other (the reference's
+  batch self-comparison): each reports the earlier-admitted ones it matches,
+  against the oracle's plaintext predicate between the request irises.
 """
 import numpy as np
 import pytest
@@ -366,3 +369,33 @@ def test_answers_follow_the_writes_in_order(orc, form):
         mid_scan += sum(1 for w in writes if w.arrival > r.arrival and w.db_version <= r.db_version)
     # writes did land while requests were being answered (not only before or after them)
     assert form == "per_batch" or mid_scan > 0
+
+
+@pytest.mark.parametrize("form", ["serving", "per_batch"])
+def test_requests_in_flight_compared_with_each_other(served, orc, form):
+    """Uniqueness requests scanning together are compared with each other, as
+    the reference compares a batch with itself (compare_query_against_self,
+    mergeBatchResults): a request reports the earlier-admitted requests in
+    flight with it (its batch, in the per-batch form) whose iris it matches.
+    A burst of fresh irises (none in the DB) with noisy copies among them is
+    admitted together; expected: the oracle's plaintext predicate between the
+    request irises (31 rotations, threshold 0.375). A copy arriving after the
+    burst has finished reports none."""
+    eng, serving, synth = served
+    R = serving.Request
+    # (iris of dataset 78, code bits flipped): rids 0-5 one burst, 6 later
+    plan = [(0, 0), (1, 0), (0, 1500), (1, 2500), (0, 3000), (2, 0), (0, 1000)]
+    reqs, plain = [], []
+    for k, (t, f) in enumerate(plan):
+        sh, c, m = _query(synth, 78, t, 80 + k, f, 200 + k)
+        plain.append((c, m))
+        reqs.append(R("uniqueness", 0.0 if k < 6 else 0.5, shares=sh, rid=k))
+    srv = serving.Server(eng, reserve_bytes=0) if form == "serving" else serving.PerBatchServer(eng, max_batch=8)
+    out = {r.rid: r for r in srv.run(reqs)}
+    for k in range(6):
+        c, m = plain[k]
+        want = [j for j in range(k) if orc.plain_match(c, m, plain[j][0][None], plain[j][1][None], 16384)[0]]
+        assert out[k].batch_matches == want, f"{form}: request {k}"
+        assert out[k].result == [], f"{form}: request {k} matched the DB"
+    assert out[6].batch_matches == [] and out[6].result == []
+    assert sum(len(out[k].batch_matches) for k in range(6)) >= 3, "the copies did not match their originals"
