@@ -26,6 +26,11 @@ of actor.rs:2246-2589 becomes a ring of chunk steps (32,768 rows each):
 - Every step runs one share GEMM per matrix and party for all active queries,
   then the reshare, the threshold circuits and the open into each request's
   bitmap row (slot map).
+- Requests in flight together are compared with each other, as the
+  reference compares a batch with itself (compare_query_against_self,
+  mergeBatchResults): the newly admitted requests against the active set's
+  irises, on a third engine set beside the scan; Request.batch_matches lists
+  the earlier-admitted requests (rids) whose iris a request matches.
 - Reauth runs on its own small engine set and streams, concurrently with the
   scan.
 - Insertions and deletions are written between steps, ordered on the scan
@@ -33,7 +38,7 @@ of actor.rs:2246-2589 becomes a ring of chunk steps (32,768 rows each):
 
 The per-batch form (PerBatchServer) runs the reference's batch after batch
 (actor.rs:584-644): every batch padded to max_batch x 31 query rows
-(actor.rs:895-901), one full scan per batch.
+(actor.rs:895-901), one full scan per batch, each batch compared with itself.
 
 The opened bits do not depend on the mix (the masks do), so each result
 equals the result of the same request run alone on the same DB version
