@@ -497,8 +497,9 @@ def run_c3(args):
     if traffic:
         traffic_source = ("measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over a child "
                           f"process running one batch of the same scan on a {PROBE_CHUNKS}-chunk DB (same "
-                          "launch shape: one party's code or mask GEMM over one 32,768-row chunk), each "
-                          "counter scaled by its reading for a 2 GiB device copy in the same pass")
+                          "launch shape: " + ("one 32,768-row chunk's six GEMMs as one launch" if joint else
+                                              "one party's code or mask GEMM over one 32,768-row chunk")
+                          + "), each counter scaled by its reading for a 2 GiB device copy in the same pass")
     else:
         traffic = measured_traffic(kernel, C3_TRAFFIC_FILE)
         traffic_source = (f"live passes unavailable ({traffic_detail}); rocprofv3 --pmc runs of this workload "
