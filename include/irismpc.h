@@ -244,7 +244,9 @@ int irismpc_share_db_get_counter(const irismpc_share_db* e, uint64_t* counter);
 int irismpc_share_db_set_counter(irismpc_share_db* e, uint64_t counter);
 /* GEMM kernel variant: -1 (default) = automatic: the rotation-aware kernel for
  * query sets uploaded from raw shares (irismpc_query_upload_shares; its
- * two-query tile for sets of at most 62 rows, its four-query tile above), the
+ * two-query tile for sets of 1, 2, 5, 6, 9 or 10 queries, where the scan is
+ * bound by DB reads and the four-query tile's last tile would be at most half
+ * full, its four-query tile otherwise), the
  * best generic tile otherwise; 0 .. irismpc_gemm_variants()-1 = generic tiles;
  * irismpc_gemm_rot_variants() = rotation-aware tiles (raw-share queries only).
  * No reference counterpart: the reference has one cuBLAS-backed GEMM path. */

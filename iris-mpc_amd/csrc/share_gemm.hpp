@@ -101,6 +101,8 @@ constexpr int kGemmVariants = 2;
 constexpr int kRotVariantBase = 100, kRotVariants = 3;
 // The rotation-aware variant for small query sets (two-query tile).
 constexpr int kRotSmallVariant = kRotVariantBase + 2, kRotSmallRows = 2 * 31;
+// ... and for sets of up to this many queries whose count is 1 or 2 mod 4
+constexpr int kRotSmallAlsoMax = 10;
 inline bool is_rot_variant(int v) { return v >= kRotVariantBase && v < kRotVariantBase + kRotVariants; }
 constexpr int kMaxBQ = 256;   // query buffers are padded to this many rows
 constexpr int kMaxBD = 256;   // DB buffers are padded to this many rows
