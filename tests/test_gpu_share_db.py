@@ -112,13 +112,15 @@ def _rotated_rows(orc, party, raw, qr):
 
 
 @pytest.mark.parametrize("K,mult,batch,qr", [(12800, 1, 6, 250), (6400, 2, 9, 279), (12800, 1, 1, 31),
-                                              (6400, 2, 2, 62), (12800, 1, 1, 40)])
+                                              (6400, 2, 2, 62), (12800, 1, 1, 40), (12800, 1, 5, 155),
+                                              (6400, 2, 10, 310)])
 def test_share_gemm_rotation_variants(irismpc, orc, K, mult, batch, qr):
     """Rotation-aware kernels on queries uploaded from raw shares (device
     preprocessing + rotations) against the oracle's rotated rows, next to the
     generic variants and the automatic choice; query_length is not a multiple
     of the 4-query tile, so the last tile is partial; one and two queries
-    (where the automatic choice is the two-query tile), also with spare rows."""
+    (where the automatic choice is the two-query tile), also with spare rows;
+    5, 9 and 10 queries (where it takes the two-query tile too)."""
     rng = np.random.default_rng(11)
     N, chunk, party = 1000, 1024, 1
     raw = _rand(rng, (batch, K))
