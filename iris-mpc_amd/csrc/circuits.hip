@@ -593,10 +593,6 @@ __global__ __launch_bounds__(192) void adder_word_kernel(const Trio t, uint32_t 
 // adder_word_kernel).
 __device__ __forceinline__ uint32_t lo16(uint32_t w, int h) { return (w >> (16 * h)) & 0xffffu; }
 
-// Two u16 elements of a word, for packed 16-bit arithmetic.
-typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ u16x2 as_u16x2(uint32_t w) { return __builtin_bit_cast(u16x2, w); }
-__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 // bits 2 w and 2 w + 1 of `bits` as the u16 pair (b0, b1)
 __device__ __forceinline__ uint32_t pair_bits(uint32_t bits, int w) {
   return ((bits >> (2 * w)) & 1u) | (((bits >> (2 * w + 1)) & 1u) << 16);

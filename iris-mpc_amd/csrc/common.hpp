@@ -48,4 +48,9 @@ struct ChaChaKey {
   uint32_t k[8];
 };
 
+// Two u16 elements of a word, for packed 16-bit arithmetic (v_pk_*_u16).
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t w) { return __builtin_bit_cast(u16x2, w); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
 }  // namespace irismpc

@@ -459,18 +459,14 @@ __device__ __forceinline__ void epilogue_store(const Args& a, const uint16_t* ti
       const uint64_t ctr = ctr0 + c * cpc + idx / 32;
       chacha_block<12>(a.key_own, a.nonce, ctr, ko);
       chacha_block<12>(a.key_prev, a.nonce, ctr, kp);
+      const uint16_t m = (uint16_t)a.mult;
 #pragma unroll
-      for (int k = 0; k < 16; k++) {
-        const uint32_t lo = (v[k] & 0xffff) * a.mult + (ko[k] & 0xffff) - (kp[k] & 0xffff);
-        const uint32_t hi = (v[k] >> 16) * a.mult + (ko[k] >> 16) - (kp[k] >> 16);
-        v[k] = (lo & 0xffff) | (hi << 16);
-      }
+      for (int k = 0; k < 16; k++)  // two u16 results per word, mod 2^16 each
+        v[k] = as_u32(as_u16x2(v[k]) * m + as_u16x2(ko[k]) - as_u16x2(kp[k]));
     } else if (a.mult != 1) {
+      const uint16_t m = (uint16_t)a.mult;
 #pragma unroll
-      for (int k = 0; k < 16; k++) {
-        const uint32_t lo = (v[k] & 0xffff) * a.mult, hi = (v[k] >> 16) * a.mult;
-        v[k] = (lo & 0xffff) | (hi << 16);
-      }
+      for (int k = 0; k < 16; k++) v[k] = as_u32(as_u16x2(v[k]) * m);
     }
     uint4* dst = reinterpret_cast<uint4*>(out + c * stride + idx);
 #pragma unroll
