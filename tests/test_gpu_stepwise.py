@@ -234,3 +234,29 @@ def test_dot_reduce_packed_equals_per_batch_ranges(irismpc, orc, K, mult):
     ref = orc.u16_dot(_rotated_rows(orc, 1, raws[0], rows[0]), db[:chunk]).reshape(-1)
     ref = orc.mask_dot_share(ref, mult, seeds[0], seeds[1], 0, 0)
     assert np.array_equal(outs_p[0].cpu().numpy().view(np.uint16)[:rows[0] * chunk], ref)
+
+
+def test_device_memory_helpers(irismpc):
+    """The device-memory entry points a host without torch uses (irismpc_malloc,
+    irismpc_fill_u32, irismpc_fill_u64, irismpc_memcpy_dtoh, irismpc_free),
+    with irismpc_version and irismpc_device_count."""
+    import ctypes
+
+    from irismpc._lib import call, lib
+
+    assert lib().irismpc_version() > 0
+    n = ctypes.c_int()
+    call("irismpc_device_count", ctypes.byref(n))
+    assert n.value >= 1
+    p = ctypes.c_void_p()
+    call("irismpc_malloc", 0, 4096, ctypes.byref(p))
+    assert p.value
+    try:
+        call("irismpc_fill_u32", p.value, 1024, 0xDEADBEEF, None)
+        call("irismpc_fill_u64", p.value + 1024, 64, 0x0123456789ABCDEF, None)
+        got = irismpc.download(p.value, 1024, np.uint32)
+        want = np.full(1024, 0xDEADBEEF, np.uint32)
+        want[256:384] = np.full(64, 0x0123456789ABCDEF, np.uint64).view(np.uint32)
+        assert np.array_equal(got, want)
+    finally:
+        call("irismpc_free", p.value)
