@@ -137,12 +137,19 @@ Serving::Serving(int device, const ChaChaKey party_seeds[3], DbSlice* const code
     own[p] = party_seeds[p];
     prev[p] = party_seeds[(p + 2) % 3];
   }
-  scan_ = std::make_unique<EngineTrio>(device, max_active, capacity, own, prev, 0, 0);
+  // Each engine set derives its code, mask and circuit keys at nonces base,
+  // base + 1 and base + 5: bases more than 5 apart give the sets disjoint
+  // ChaCha12 keystreams. A zero-share mask repeated across two computations
+  // would let the receiver of both masked values learn their difference.
+  constexpr uint64_t kScanNonce = 0, kReauthNonce = 16, kSelfNonce = 32;
+  static_assert(kReauthNonce > kScanNonce + 5 && kSelfNonce > kReauthNonce + 5,
+                "two engine sets would share a keystream");
+  scan_ = std::make_unique<EngineTrio>(device, max_active, capacity, own, prev, 0, kScanNonce);
   // reauth: one query over a 64-row window, its own keystreams
-  small_ = std::make_unique<EngineTrio>(device, 1, capacity, own, prev, 8, 16);
+  small_ = std::make_unique<EngineTrio>(device, 1, capacity, own, prev, 8, kReauthNonce);
   // the self-comparison of requests in flight: up to max_active queries
   // against the set's irises (one step of a 512-row DB), its own keystreams
-  self_ = std::make_unique<EngineTrio>(device, max_active, kSelfChunk, own, prev, 24, 32, kSelfChunk);
+  self_ = std::make_unique<EngineTrio>(device, max_active, kSelfChunk, own, prev, 24, kSelfNonce, kSelfChunk);
   scan_->parties(code_dbs_, mask_dbs_, scan_p_);
   small_->parties(code_dbs_, mask_dbs_, small_p_);
   {
